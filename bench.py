@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""bench.py — real-time TDDFT Hamming-PC steps/s of one 4He droplet on one MI355X.
+
+Workload (the headline configuration): N=1000 droplet on a 128^3 grid, h = 0.4 A
+(box +-25.6 A), Orsay-Trento functional with the alpha_s term (core 'OTC',
+lsolid off), 13-point stencils with icon=13, one classical impurity of 108 amu
+with a Lennard-Jones He-X potential (eps 20 K, sigma 4 A) starting 20 A from the
+centre at 300 m/s toward it, dt = 0.5 fs, 4 RK-Gill start steps, then timed
+Hamming PC iterations (each = STEPPC + potenimp/poten/forceimp/aimp of the
+driver, i.e. two mean-field evaluations).  FP64 throughout (the drop-in path).
+
+A "step" is one PC iteration of one droplet; inputs are resident in HBM when
+the timed region starts; time is measured with HIP events on the library's
+stream around K graph-replayed iterations.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=24)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--n4", type=float, default=1000.0)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="PC iterations in the CPU sample")
+    return ap.parse_args()
+
+
+def algorithmic_bytes_per_pc_step(n: int, w: int = 8) -> float:
+    """Algorithmic bytes of one Hamming PC iteration of one droplet (OTC):
+    2 mean-field evaluations of 14 3-D FFTs each + predictor + corrector.
+    C_fft = w N + 3 (2w) Nh; B_poten = 14 C_fft + 8 w Nh + 20 w N;
+    B_pred = B_corr = 8 (2w) N + 2 w N."""
+    N = n ** 3
+    Nh = (n // 2 + 1) * n * n
+    cfft = w * N + 3 * (2 * w) * Nh
+    b_poten = 14 * cfft + 8 * w * Nh + 20 * w * N
+    b_pc = 8 * (2 * w) * N + 2 * w * N
+    return 2 * b_poten + 2 * b_pc
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs one droplet per GPU on a single GPU; "
+              f"--gpus {a.gpus} is not supported (the multi-GPU design is documented, not built).",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import he_dft_2024_amd as H
+    from he_dft_2024_amd import inputs as I
+
+    n = a.grid
+    xmax = 0.2 * n          # h = 0.4 A
+    dt = 5e-4 / I.PSTOK     # 0.5 fs
+    p = H.default_params(core_otc=1, lsolid=0)
+    drop = H.Droplet(n, n, n, xmax, xmax, xmax, p, mode=H.MODE_TDDFT)
+    I.tddft_setup(drop, n, xmax, a.n4, seed=1, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 3.0))
+    drop.tddft_init()
+    nrk = 4
+    for it in range(1, nrk + 1):
+        drop.tddft_iteration(dt, it, nrk)
+    use_graph = not a.no_graph
+    if a.warmup:
+        drop.run_pc(dt, a.warmup, use_graph)
+    ms = drop.run_pc(dt, a.steps, use_graph)
+    steps_per_s = a.steps / (ms / 1000.0)
+    e = drop.energy()
+    if not np.isfinite(e["etot"]):
+        print("bench.py: non-finite energy after the timed steps", file=sys.stderr)
+        sys.exit(3)
+
+    bytes_step = algorithmic_bytes_per_pc_step(n)
+    step_gbs = bytes_step * steps_per_s / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
+
+    out = {
+        "metric": "TDDFT steps/s at 128^3 (FP64 plain, one droplet)",
+        "value": round(steps_per_s, 3),
+        "unit": "PC steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Woods-Saxon droplet (not relaxed) + LJ impurity table",
+        "config": {"workload": f"TDDFT Hamming-PC iteration, N={int(a.n4)} droplet, {n}^3 grid, h=0.4 A, "
+                               "OTC, 1 classical LJ impurity 108 amu at 300 m/s",
+                   "grid": n, "n4": a.n4, "dt_fs": 0.5, "graph": use_graph},
+        "step_algorithmic_GBps": round(step_gbs, 1),
+        "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "energy_per_atom_K": e["etot4"] / a.n4,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(n, xmax, n4, dt, nsteps):
+    """The FP64 oracle (C restatement, OpenMP) timed on the host on a bounded
+    sample: nsteps PC iterations of the same workload after its RK start."""
+    from oracle import oracle as O
+    from he_dft_2024_amd import inputs as I
+    o = O.Oracle(n, n, n, xmax, xmax, xmax, O.default_params(core_otc=1), O.MODE_TDDFT)
+    I.tddft_setup(o, n, xmax, n4, seed=1, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 3.0))
+    o.tddft_init()
+    for it in range(1, 5):
+        o.tddft_iteration(dt, it, 4)
+    t0 = time.perf_counter()
+    for it in range(5, 5 + nsteps):
+        o.tddft_iteration(dt, it, 4)
+    el = time.perf_counter() - t0
+    return {"value": round(nsteps / el, 5), "unit": "PC steps/s", "cores": O.num_threads(),
+            "kind": "port", "sample": f"{nsteps} PC iterations of the same {n}^3 workload after a 4-step "
+                                      f"RK start (oracle, OpenMP, own mixed-radix FFT)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,6 @@
+"""MI355X-native BCN-4He-DFT time-step hot path (Orsay-Trento mean field,
+13-point stencils, RK-Gill / Hamming predictor-corrector) behind the C-ABI in
+include/hedft.h.  The HIP library lives in he_dft_2024_amd/_build/libhedft.so;
+the Fortran drop-in module is he_dft_2024_amd/fortran/hedft_gpu.f90."""
+from .hedft import (Droplet, HedftError, MODE_STATICS, MODE_TDDFT, default_params,  # noqa: F401
+                    grid_supported, lib)

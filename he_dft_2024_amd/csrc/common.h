@@ -1,0 +1,215 @@
+// common.h — shared device helpers and the context layout of the HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/hedft.h"
+
+#define HEDFT_CHECK(expr)                                                          \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      fprintf(stderr, "hedft: HIP error %s at %s:%d\n", hipGetErrorString(e_),     \
+              __FILE__, __LINE__);                                                  \
+      return HEDFT_EHIP;                                                            \
+    }                                                                               \
+  } while (0)
+
+#define HEDFT_TRY(expr)      \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != HEDFT_OK) return rc_; \
+  } while (0)
+
+namespace hedft {
+
+// ---- complex helpers (interleaved re,im = complex(8)) -------------------------
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by -i (fwd) or +i (inv)
+template <bool INV>
+__host__ __device__ inline double2 mul_mi(double2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ---- launch geometry -----------------------------------------------------------
+struct Dims {
+  int nx, ny, nz, nxh;
+  long long n, nh;
+};
+
+// Stencil coefficients of derivnD for one (id, axis): central row (13) and the
+// 12 boundary rows (6 leading + 6 trailing) of the Bickley matrix after the
+// boundary-mode rewrite, already divided by h^id.
+struct StencilCoef {
+  double c[13];
+  double rows[12][13];
+  int mode;  // 0: boundary rows, 1: periodic, 2: antiperiodic
+};
+
+// ---- impurity evaluation on the fly (updatepoten, potenimp.f90:83-123) -------
+struct ImpDev {
+  int nimp, npot;
+  double delinter, rmax;
+  const double* potion;  // nimp x npot
+  const double* rimp;    // device (nimp,3), k fastest
+  int* lstop;            // device flag
+  double x0, y0, z0, hx, hy, hz;
+};
+
+__device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, double zz, bool flag) {
+  const double rx = im.rimp[k], ry = im.rimp[k + im.nimp], rz = im.rimp[k + 2 * im.nimp];
+  const double zt = (zz - rz) * (zz - rz);
+  const double yt = (yy - ry) * (yy - ry) + zt;
+  double r = sqrt((xx - rx) * (xx - rx) + yt);
+  if (r > im.rmax) {
+    if (flag) *im.lstop = 1;
+    r = im.rmax;
+  }
+  int ir = (int)(r / im.delinter);
+  if (ir > im.npot - 2) ir = im.npot - 2;
+  const double q = r / im.delinter;
+  const double rmod = (r - (double)(long long)q * im.delinter) / im.delinter;
+  const double* tab = im.potion + (size_t)k * im.npot;
+  return tab[ir] * (1.0 - rmod) + tab[ir + 1] * rmod;
+}
+
+__device__ inline double imp_u(const ImpDev& im, int ix, int iy, int iz) {
+  double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
+  double u = 0.0;
+  for (int k = 0; k < im.nimp; ++k) u += imp_u_k(im, k, xx, yy, zz, false);
+  return u;
+}
+
+// ---- block reduction into per-block partials ----------------------------------
+template <int NR>
+__device__ inline void block_reduce_store(double (&v)[NR], double* partials) {
+  __shared__ double red[NR][32];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double s = v[r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[r][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
+    const size_t bid = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+    partials[bid * NR + threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+}  // namespace hedft
+
+// ---- the context -----------------------------------------------------------------
+struct hedft_ctx {
+  hedft_grid g;
+  hedft_params p;
+  int mode;
+  int device;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  hedft::Dims d;
+  double hx, hy, hz, dxyz, renor;
+  // real N fields
+  double *den, *pot4, *dencg, *delj4, *denalf, *intx, *inty, *intz, *ualphas;
+  double *dxden, *dyden, *dzden, *uext, *sto1, *ualph_conv, *work1, *work2;
+  double* timec_r;  // statics local time step
+  double* hpsi_r;   // statics H psi
+  // psi and PC history: double (statics) or double2 (TDDFT)
+  void *psi, *psiold[3], *hpsiold[2], *pc, *q;
+  void* lap;        // Laplacian accumulator (same type as psi)
+  void* workc1;     // complex N scratch
+  // half spectra
+  double2 *fden, *spec[4], *workk1;
+  double *fvlj4, *wcgk, *kalfs;
+  double* tw;       // twiddle tables (complex, interleaved)
+  double* lamtab;   // TDDFT absorbing layer: tanh((|x_a|-mean_a)/surf_a) per axis (nx+ny+nz)
+  size_t tw_x, tw_xpost, tw_y, tw_z;  // offsets (in complex elements)
+  // reductions
+  double* partials;
+  size_t partials_cap;  // doubles
+  double* scal;         // device scalars (see physics.hip)
+  double* host_scal;    // pinned mirror
+  // stencil coefficient sets: [id-1][axis]
+  hedft::StencilCoef coef[2][3];
+  // PC slot bookkeeping (host-side; rkpc: ioldp, ioldh; classicimp: ioldr, ioldv, iolda)
+  int ioldp[3], ioldh[2], ioldr[3], ioldv[3], iolda[2];
+  int rk_done;
+  // classical impurities
+  int nimp, npot;
+  double delinter, rmaxinterpol;
+  double* potion;   // device table
+  double* imp;      // device impurity state block (see physics.hip layout)
+  int* lstop;       // device flag
+  double mass[HEDFT_MAX_IMP];
+  double last_ms;
+  // graph of one PC iteration
+  hipGraph_t graph;
+  hipGraphExec_t graph_exec;
+  double graph_dt;
+  int graph_valid;
+};
+
+namespace hedft {
+// fft.hip
+int fft_setup(hedft_ctx* c);
+int fftfw(hedft_ctx* c, const double* in, double2* out);
+int fftbk(hedft_ctx* c, double2* in, double* out);
+
+// x-pass prologue (R2C input) and epilogue (C2R output) operations
+enum XInOp { XIN_COPY = 0, XIN_CORR = 1, XIN_FGRAD = 2, XIN_DOT3 = 3 };
+enum XOutOp { XOUT_STORE = 0, XOUT_POT4 = 1 };
+struct XIn {
+  int op;
+  const double* a[6];
+  double c0, c1;
+};
+struct XOut {
+  int op;
+  double* out;
+  const double *delj4, *dencg, *den, *uext;
+  double a0, a1, csolid, beta, den_m;
+  int lsolid, lext;
+};
+// forward x R2C of up to 3 real inputs into spectra, then y forward
+int fwd_xy(hedft_ctx* c, int nb, const XIn* in, double2* const* spec);
+// z pass: optional forward, then for each output o: multiply by kern[o], inverse z.
+// nin inputs share the kernels; out[i*nout+o].  store_fwd[i] (may be null)
+// receives the full forward spectrum.
+int zconv(hedft_ctx* c, int nin, double2* const* in, int nout, const double* const* kern,
+          double2* const* out, int do_fwd, double2* const* store_fwd);
+// y inverse then x C2R with epilogue for nb spectra
+int inv_yx(hedft_ctx* c, int nb, double2* const* spec, const XOut* out);
+
+// stencil.hip
+void stencil_setup(hedft_ctx* c);
+enum SInOp { SIN_PLAIN = 0, SIN_FINT = 1 };
+// plain derivnD of a field (real or complex) along axis (0..2)
+int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int is_complex);
+// grad(den) -> dxden,dyden,dzden with impurity force partials (forceimp)
+int derden_force(hedft_ctx* c, int want_force);
+// divergence of (1-denalf/den0s)*int_a, then ualphas and pot4 assembly
+int alpha_div_pot4(hedft_ctx* c);
+// Laplacian of psi (x then y into c->lap); the z pass is the fused step kernel
+int lap_xy(hedft_ctx* c, const void* psi, int is_complex);
+}  // namespace hedft

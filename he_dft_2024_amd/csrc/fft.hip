@@ -1,0 +1,579 @@
+// fft.hip — 3-D R2C / C2R FFTs for gfx950 as LDS-staged Stockham passes.
+//
+// Replaces the FFTW plans of fft.f90 (He_statics/fft.f90:5-89,
+// Dynamics_TDDFT/fft.f90:5-168): forward R2C unnormalised with e^{-i}; C2R
+// scaled by renor = 1/(nx ny nz) after the inverse.  Arrays keep the
+// reference layout: real (nx,ny,nz) x fastest, half spectrum (nx/2+1,ny,nz).
+//
+// Passes:
+//   x  : rows of nx reals packed as nx/2 complex, Stockham FFT of length nx/2
+//        in LDS, then the real-to-half-complex split (forward) or its inverse
+//        before the FFT (backward).  Prologue / epilogue ops fuse the pointwise
+//        work that feeds or consumes a transform (correlation source, alpha_s
+//        products, pot4 assembly).
+//   y  : tiles of TX consecutive kx columns x ny, FFT along y in LDS.
+//   z  : same along z; the convolution variant runs forward z, multiplies by
+//        one to three spectral kernels and runs inverse z while the column
+//        tile stays in LDS (the spectrum of a convolution never round-trips
+//        HBM between the forward and inverse z transforms).
+#include <cmath>
+
+#include "common.h"
+
+namespace hedft {
+
+// ---------------------------------------------------------------------------
+// radix-R DFT in registers; INV selects e^{+i}
+// ---------------------------------------------------------------------------
+template <int R, bool INV>
+__device__ __forceinline__ void dft(double2* u);
+
+template <>
+__device__ __forceinline__ void dft<2, false>(double2* u) {
+  double2 a = u[0];
+  u[0] = cadd(a, u[1]);
+  u[1] = csub(a, u[1]);
+}
+template <>
+__device__ __forceinline__ void dft<2, true>(double2* u) {
+  dft<2, false>(u);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2& u0, double2& u1, double2& u2, double2& u3) {
+  double2 t0 = cadd(u0, u2), t1 = csub(u0, u2), t2 = cadd(u1, u3), t3 = mul_mi<INV>(csub(u1, u3));
+  u0 = cadd(t0, t2);
+  u2 = csub(t0, t2);
+  u1 = cadd(t1, t3);
+  u3 = csub(t1, t3);
+}
+template <>
+__device__ __forceinline__ void dft<4, false>(double2* u) { dft4<false>(u[0], u[1], u[2], u[3]); }
+template <>
+__device__ __forceinline__ void dft<4, true>(double2* u) { dft4<true>(u[0], u[1], u[2], u[3]); }
+
+template <bool INV>
+__device__ __forceinline__ void dft8(double2* u) {
+  double2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+  double2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  const double r = 0.70710678118654752440;
+  if (!INV) {
+    o1 = make_double2((o1.x + o1.y) * r, (o1.y - o1.x) * r);
+    o3 = make_double2((o3.y - o3.x) * r, (-o3.x - o3.y) * r);
+  } else {
+    o1 = make_double2((o1.x - o1.y) * r, (o1.x + o1.y) * r);
+    o3 = make_double2((-o3.x - o3.y) * r, (o3.x - o3.y) * r);
+  }
+  o2 = mul_mi<INV>(o2);
+  u[0] = cadd(e0, o0); u[4] = csub(e0, o0);
+  u[1] = cadd(e1, o1); u[5] = csub(e1, o1);
+  u[2] = cadd(e2, o2); u[6] = csub(e2, o2);
+  u[3] = cadd(e3, o3); u[7] = csub(e3, o3);
+}
+template <>
+__device__ __forceinline__ void dft<8, false>(double2* u) { dft8<false>(u); }
+template <>
+__device__ __forceinline__ void dft<8, true>(double2* u) { dft8<true>(u); }
+
+template <bool INV>
+__device__ __forceinline__ void dft3(double2* u) {
+  const double s3 = 0.86602540378443864676;
+  double2 s = cadd(u[1], u[2]);
+  double2 t = make_double2(u[0].x - 0.5 * s.x, u[0].y - 0.5 * s.y);
+  double2 d = mul_mi<INV>(csub(u[1], u[2]));
+  d = cscale(d, s3);
+  u[0] = cadd(u[0], s);
+  u[1] = cadd(t, d);
+  u[2] = csub(t, d);
+}
+template <>
+__device__ __forceinline__ void dft<3, false>(double2* u) { dft3<false>(u); }
+template <>
+__device__ __forceinline__ void dft<3, true>(double2* u) { dft3<true>(u); }
+
+template <bool INV>
+__device__ __forceinline__ void dft5(double2* u) {
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+  double2 a1 = cadd(u[1], u[4]), d1 = csub(u[1], u[4]);
+  double2 a2 = cadd(u[2], u[3]), d2 = csub(u[2], u[3]);
+  double2 y0 = cadd(u[0], cadd(a1, a2));
+  double2 p1 = make_double2(u[0].x + c1 * a1.x + c2 * a2.x, u[0].y + c1 * a1.y + c2 * a2.y);
+  double2 p2 = make_double2(u[0].x + c2 * a1.x + c1 * a2.x, u[0].y + c2 * a1.y + c1 * a2.y);
+  double2 b1 = mul_mi<INV>(make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+  double2 b2 = mul_mi<INV>(make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+  u[0] = y0;
+  u[1] = cadd(p1, b1);
+  u[4] = csub(p1, b1);
+  u[2] = cadd(p2, b2);
+  u[3] = csub(p2, b2);
+}
+template <>
+__device__ __forceinline__ void dft<5, false>(double2* u) { dft5<false>(u); }
+template <>
+__device__ __forceinline__ void dft<5, true>(double2* u) { dft5<true>(u); }
+
+// ---------------------------------------------------------------------------
+// One Stockham autosort pass (radix R, P = product of earlier radices) over a
+// column of length L held in LDS with element stride `stride`, executed by T
+// threads (t = 0..T-1).  tw[m] = exp(-2 pi i m / L).
+// ---------------------------------------------------------------------------
+template <int L, int R, int P, bool INV, int T>
+__device__ __forceinline__ void spass(double2* col, int stride, int t, const double2* __restrict__ tw) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + T - 1) / T;
+  double2 u[PER][R];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int i = t + b * T;
+    if (PER * T == NB || i < NB) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) u[b][j] = col[(i + j * NB) * stride];
+      if constexpr (P > 1) {
+        const int k = i % P;
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+          double2 w = tw[(k * j) * (L / (P * R))];
+          if (INV) w.y = -w.y;
+          u[b][j] = cmul(u[b][j], w);
+        }
+      }
+      dft<R, INV>(u[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int i = t + b * T;
+    if (PER * T == NB || i < NB) {
+      const int k = i % P;
+      const int base = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) col[(base + j * P) * stride] = u[b][j];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, bool INV, int T, int P, int R, int... Rest>
+__device__ __forceinline__ void run_passes(double2* col, int stride, int t, const double2* tw) {
+  spass<L, R, P, INV, T>(col, stride, t, tw);
+  if constexpr (sizeof...(Rest) > 0) run_passes<L, INV, T, P * R, Rest...>(col, stride, t, tw);
+}
+
+// Factorisations (first radix largest keeps the thread count T = L / 8).
+template <int L>
+struct Rad;
+#define HEDFT_RAD(LEN, TT, ...)                                                          \
+  template <>                                                                            \
+  struct Rad<LEN> {                                                                      \
+    static constexpr int T = TT;                                                         \
+    template <bool INV>                                                                  \
+    __device__ static __forceinline__ void run(double2* c, int s, int t, const double2* tw) { \
+      run_passes<LEN, INV, TT, 1, __VA_ARGS__>(c, s, t, tw);                             \
+    }                                                                                    \
+  };
+HEDFT_RAD(16, 4, 4, 4)
+HEDFT_RAD(24, 3, 8, 3)
+HEDFT_RAD(32, 4, 8, 4)
+HEDFT_RAD(48, 6, 8, 3, 2)
+HEDFT_RAD(64, 8, 8, 8)
+HEDFT_RAD(96, 12, 8, 4, 3)
+HEDFT_RAD(128, 16, 8, 4, 4)
+HEDFT_RAD(192, 24, 8, 8, 3)
+HEDFT_RAD(256, 32, 8, 8, 4)
+#undef HEDFT_RAD
+
+// ---------------------------------------------------------------------------
+// x-pass prologue / epilogue
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double xin_eval(const XIn& in, size_t i) {
+  switch (in.op) {
+    case XIN_CORR: {  // den*dencg*(cp4+cpp4*dencg)   (poten.f90:166-168)
+      const double cg = in.a[1][i];
+      return in.a[0][i] * cg * (in.c0 + in.c1 * cg);
+    }
+    case XIN_FGRAD:  // falfs*d_a(den), falfs = 1 - denalf/den0s  (term_alfa.f90:26-34)
+      return (1.0 - in.a[0][i] / in.c0) * in.a[1][i];
+    case XIN_DOT3:  // grad(den).int_alf  (term_alfa.f90:61-65)
+      return in.a[0][i] * in.a[3][i] + in.a[1][i] * in.a[4][i] + in.a[2][i] * in.a[5][i];
+    default:
+      return in.a[0][i];
+  }
+}
+
+__device__ __forceinline__ void xout_apply(const XOut& o, size_t i, double v) {
+  if (o.op == XOUT_POT4) {
+    // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + conv [+ uext] [+ penalty]
+    // (He_statics/poten.f90:277-289,315-319; Dynamics_TDDFT/poten.f90:194-218)
+    const double cg = o.dencg[i];
+    double p = o.delj4[i] + cg * cg * (o.a0 + o.a1 * cg) + v;
+    if (o.lext) p += o.uext[i];
+    if (o.lsolid) {
+      const double dn = o.den[i];
+      const double th = tanh(o.beta * (dn - o.den_m));
+      p += o.csolid * (1.0 + th + o.beta * dn * (1.0 - th * th));
+    }
+    o.out[i] = p;
+  } else {
+    o.out[i] = v;
+  }
+}
+
+struct XInBatch {
+  XIn in[3];
+  double2* out[3];
+};
+struct XOutBatch {
+  const double2* in[3];
+  XOut out[3];
+};
+struct SpecBatch {
+  double2* p[3];
+};
+struct ZConvArgs {
+  const double2* in[3];
+  double2* out[9];
+  double2* store[3];
+  const double* kern[3];
+  int do_fwd;
+};
+
+// ---------------------------------------------------------------------------
+// x R2C: rows of nx = 2M reals -> M+1 complex
+// ---------------------------------------------------------------------------
+template <int M, int ROWS>
+__global__ void __launch_bounds__(Rad<M>::T * ROWS)
+    k_xr2c(XInBatch a, Dims d, const double2* __restrict__ tw, const double2* __restrict__ twp) {
+  constexpr int T = Rad<M>::T;
+  __shared__ double2 lds[ROWS][M + 1];
+  const XIn in = a.in[blockIdx.y];
+  double2* out = a.out[blockIdx.y];
+  const int nrows = d.ny * d.nz;
+  const int row0 = blockIdx.x * ROWS;
+  for (int e = threadIdx.x; e < ROWS * M; e += T * ROWS) {
+    const int r = e / M, j = e - r * M;
+    const int row = row0 + r;
+    double2 v = make_double2(0.0, 0.0);
+    if (row < nrows) {
+      const size_t i = (size_t)row * (2 * M) + 2 * j;
+      if (in.op == XIN_COPY) {
+        v = *reinterpret_cast<const double2*>(in.a[0] + i);
+      } else {
+        v.x = xin_eval(in, i);
+        v.y = xin_eval(in, i + 1);
+      }
+    }
+    lds[r][j] = v;
+  }
+  __syncthreads();
+  const int t = threadIdx.x % T, r = threadIdx.x / T;
+  Rad<M>::template run<false>(&lds[r][0], 1, t, tw);
+  const int row = row0 + r;
+  if (row < nrows) {
+    for (int k = t; k <= M; k += T) {
+      const double2 zk = lds[r][k == M ? 0 : k];
+      const double2 zm = cconj(lds[r][k == 0 ? 0 : M - k]);
+      const double2 e = cscale(cadd(zk, zm), 0.5);
+      const double2 o = cscale(csub(zk, zm), 0.5);
+      const double2 wo = cmul(twp[k], o);
+      out[(size_t)row * (M + 1) + k] = make_double2(e.x + wo.y, e.y - wo.x);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x C2R: M+1 complex -> nx = 2M reals (imaginary parts of X_0, X_M ignored)
+// ---------------------------------------------------------------------------
+template <int M, int ROWS>
+__global__ void __launch_bounds__(Rad<M>::T * ROWS)
+    k_xc2r(XOutBatch a, Dims d, const double2* __restrict__ tw, const double2* __restrict__ twp,
+           double renor) {
+  constexpr int T = Rad<M>::T;
+  __shared__ double2 lds[ROWS][M + 1];
+  const double2* in = a.in[blockIdx.y];
+  const XOut out = a.out[blockIdx.y];
+  const int nrows = d.ny * d.nz;
+  const int row0 = blockIdx.x * ROWS;
+  for (int e = threadIdx.x; e < ROWS * M; e += T * ROWS) {
+    const int r = e / M, k = e - r * M;
+    const int row = row0 + r;
+    double2 z = make_double2(0.0, 0.0);
+    if (row < nrows) {
+      const double2* h = in + (size_t)row * (M + 1);
+      double2 xk = h[k];
+      double2 xm = h[M - k];
+      if (k == 0) { xk.y = 0.0; xm.y = 0.0; }
+      const double2 xmc = cconj(xm);
+      const double2 s = cadd(xk, xmc);
+      const double2 df = csub(xk, xmc);
+      const double2 w = cconj(twp[k]);
+      const double2 wd = cmul(w, df);
+      z = make_double2(s.x - wd.y, s.y + wd.x);
+    }
+    lds[r][k] = z;
+  }
+  __syncthreads();
+  const int t = threadIdx.x % T, r = threadIdx.x / T;
+  Rad<M>::template run<true>(&lds[r][0], 1, t, tw);
+  const int row = row0 + r;
+  if (row < nrows) {
+    for (int j = t; j < M; j += T) {
+      const double2 v = lds[r][j];
+      const size_t i = (size_t)row * (2 * M) + 2 * j;
+      xout_apply(out, i, v.x * renor);
+      xout_apply(out, i + 1, v.y * renor);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column passes over a half spectrum.  AX = 1 (y) or 2 (z).
+// ---------------------------------------------------------------------------
+template <int L>
+struct ColTile {
+  static constexpr int T = Rad<L>::T;
+  static constexpr int TX = (T <= 16) ? 16 : 8;
+  static constexpr int NT = T * TX;
+};
+
+template <int L, int AX>
+__device__ __forceinline__ void col_geom(const Dims& d, size_t& base, size_t& stride) {
+  const int x0 = blockIdx.x * ColTile<L>::TX;
+  if (AX == 1) {
+    base = (size_t)blockIdx.y * d.nxh * d.ny + x0;
+    stride = d.nxh;
+  } else {
+    base = (size_t)blockIdx.y * d.nxh + x0;
+    stride = (size_t)d.nxh * d.ny;
+  }
+}
+
+template <int L, int AX, bool INV>
+__global__ void __launch_bounds__(ColTile<L>::NT)
+    k_colfft(SpecBatch a, Dims d, const double2* __restrict__ tw) {
+  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
+  __shared__ double2 lds[L * TX];
+  double2* s = a.p[blockIdx.z];
+  size_t base, stride;
+  col_geom<L, AX>(d, base, stride);
+  const int x0 = blockIdx.x * TX;
+  for (int e = threadIdx.x; e < TX * L; e += NT) {
+    const int xl = e % TX, j = e / TX;
+    lds[e] = (x0 + xl < d.nxh) ? s[base + j * stride + xl] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  Rad<L>::template run<INV>(lds + threadIdx.x % TX, TX, threadIdx.x / TX, tw);
+  for (int e = threadIdx.x; e < TX * L; e += NT) {
+    const int xl = e % TX, j = e / TX;
+    if (x0 + xl < d.nxh) s[base + j * stride + xl] = lds[e];
+  }
+}
+
+// z convolution: [forward z] -> (x kern_o -> inverse z -> out_o) for o < NOUT
+template <int L, int NOUT>
+__global__ void __launch_bounds__(ColTile<L>::NT)
+    k_zconv(ZConvArgs a, Dims d, const double2* __restrict__ tw) {
+  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
+  __shared__ double2 src[L * TX];
+  __shared__ double2 wk[(NOUT > 1) ? L * TX : 1];
+  const int b = blockIdx.z;
+  const double2* in = a.in[b];
+  size_t base, stride;
+  col_geom<L, 2>(d, base, stride);
+  const int x0 = blockIdx.x * TX;
+  const int xl0 = threadIdx.x % TX, t0 = threadIdx.x / TX;
+  for (int e = threadIdx.x; e < TX * L; e += NT) {
+    const int xl = e % TX, j = e / TX;
+    src[e] = (x0 + xl < d.nxh) ? in[base + j * stride + xl] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (a.do_fwd) Rad<L>::template run<false>(src + xl0, TX, t0, tw);
+  if (a.store[b]) {
+    for (int e = threadIdx.x; e < TX * L; e += NT) {
+      const int xl = e % TX, j = e / TX;
+      if (x0 + xl < d.nxh) a.store[b][base + j * stride + xl] = src[e];
+    }
+  }
+#pragma unroll 1
+  for (int o = 0; o < NOUT; ++o) {
+    double2* buf = (NOUT > 1) ? wk : src;
+    const double* kern = a.kern[o];
+    for (int e = threadIdx.x; e < TX * L; e += NT) {
+      const int xl = e % TX, j = e / TX;
+      const double kv = (x0 + xl < d.nxh) ? kern[base + j * stride + xl] : 0.0;
+      buf[e] = cscale(src[e], kv);
+    }
+    __syncthreads();
+    Rad<L>::template run<true>(buf + xl0, TX, t0, tw);
+    double2* out = a.out[b * NOUT + o];
+    for (int e = threadIdx.x; e < TX * L; e += NT) {
+      const int xl = e % TX, j = e / TX;
+      if (x0 + xl < d.nxh) out[base + j * stride + xl] = buf[e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <int M>
+static constexpr int xrows() {
+  return 256 / Rad<M>::T > 64 ? 64 : 256 / Rad<M>::T;
+}
+
+static int dispatch_len(int L) {
+  switch (L) {
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: return 1;
+    default: return 0;
+  }
+}
+
+#define HEDFT_LEN_SWITCH(LEN, ...)           \
+  switch (LEN) {                              \
+    case 16: { constexpr int L_ = 16; __VA_ARGS__; } break;   \
+    case 24: { constexpr int L_ = 24; __VA_ARGS__; } break;   \
+    case 32: { constexpr int L_ = 32; __VA_ARGS__; } break;   \
+    case 48: { constexpr int L_ = 48; __VA_ARGS__; } break;   \
+    case 64: { constexpr int L_ = 64; __VA_ARGS__; } break;   \
+    case 96: { constexpr int L_ = 96; __VA_ARGS__; } break;   \
+    case 128: { constexpr int L_ = 128; __VA_ARGS__; } break; \
+    case 192: { constexpr int L_ = 192; __VA_ARGS__; } break; \
+    case 256: { constexpr int L_ = 256; __VA_ARGS__; } break; \
+    default: return HEDFT_EGRID;              \
+  }
+
+static inline const double2* twp(const hedft_ctx* c, size_t off) {
+  return reinterpret_cast<const double2*>(c->tw) + off;
+}
+
+int fft_setup(hedft_ctx* c) {
+  const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, M = nx / 2;
+  if (!dispatch_len(M) || !dispatch_len(ny) || !dispatch_len(nz) || (nx & 1)) return HEDFT_EGRID;
+  std::vector<double2> h;
+  auto add = [&](int n, int count) {
+    size_t off = h.size();
+    for (int m = 0; m < count; ++m) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * m / n;
+      h.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+    }
+    return off;
+  };
+  c->tw_x = add(M, M);
+  c->tw_xpost = add(nx, M + 1);
+  c->tw_y = add(ny, ny);
+  c->tw_z = add(nz, nz);
+  HEDFT_CHECK(hipMalloc(&c->tw, h.size() * sizeof(double2)));
+  HEDFT_CHECK(hipMemcpy(c->tw, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return HEDFT_OK;
+}
+
+static int launch_xr2c(hedft_ctx* c, const XInBatch& a, int nb) {
+  const int M = c->d.nx / 2;
+  const int nrows = c->d.ny * c->d.nz;
+  HEDFT_LEN_SWITCH(M, {
+    constexpr int ROWS = xrows<L_>();
+    dim3 grid((nrows + ROWS - 1) / ROWS, nb);
+    hipLaunchKernelGGL((k_xr2c<L_, ROWS>), grid, dim3(Rad<L_>::T * ROWS), 0, c->stream, a, c->d,
+                       twp(c, c->tw_x), twp(c, c->tw_xpost));
+  });
+  return HEDFT_OK;
+}
+
+static int launch_xc2r(hedft_ctx* c, const XOutBatch& a, int nb) {
+  const int M = c->d.nx / 2;
+  const int nrows = c->d.ny * c->d.nz;
+  HEDFT_LEN_SWITCH(M, {
+    constexpr int ROWS = xrows<L_>();
+    dim3 grid((nrows + ROWS - 1) / ROWS, nb);
+    hipLaunchKernelGGL((k_xc2r<L_, ROWS>), grid, dim3(Rad<L_>::T * ROWS), 0, c->stream, a, c->d,
+                       twp(c, c->tw_x), twp(c, c->tw_xpost), c->renor);
+  });
+  return HEDFT_OK;
+}
+
+template <int AX, bool INV>
+static int launch_col(hedft_ctx* c, const SpecBatch& a, int nb) {
+  const int L = (AX == 1) ? c->d.ny : c->d.nz;
+  const size_t off = (AX == 1) ? c->tw_y : c->tw_z;
+  HEDFT_LEN_SWITCH(L, {
+    constexpr int TX = ColTile<L_>::TX;
+    dim3 grid((c->d.nxh + TX - 1) / TX, (AX == 1) ? c->d.nz : c->d.ny, nb);
+    hipLaunchKernelGGL((k_colfft<L_, AX, INV>), grid, dim3(ColTile<L_>::NT), 0, c->stream, a, c->d,
+                       twp(c, off));
+  });
+  return HEDFT_OK;
+}
+
+int fwd_xy(hedft_ctx* c, int nb, const XIn* in, double2* const* spec) {
+  XInBatch a{};
+  SpecBatch s{};
+  for (int b = 0; b < nb; ++b) {
+    a.in[b] = in[b];
+    a.out[b] = spec[b];
+    s.p[b] = spec[b];
+  }
+  HEDFT_TRY(launch_xr2c(c, a, nb));
+  return launch_col<1, false>(c, s, nb);
+}
+
+int zconv(hedft_ctx* c, int nin, double2* const* in, int nout, const double* const* kern,
+          double2* const* out, int do_fwd, double2* const* store_fwd) {
+  ZConvArgs a{};
+  for (int i = 0; i < nin; ++i) {
+    a.in[i] = in[i];
+    a.store[i] = store_fwd ? store_fwd[i] : nullptr;
+    for (int o = 0; o < nout; ++o) a.out[i * nout + o] = out[i * nout + o];
+  }
+  for (int o = 0; o < nout; ++o) a.kern[o] = kern[o];
+  a.do_fwd = do_fwd;
+  const int L = c->d.nz;
+  HEDFT_LEN_SWITCH(L, {
+    constexpr int TX = ColTile<L_>::TX;
+    dim3 grid((c->d.nxh + TX - 1) / TX, c->d.ny, nin);
+    dim3 blk(ColTile<L_>::NT);
+    const double2* tw = twp(c, c->tw_z);
+    if (nout == 1) hipLaunchKernelGGL((k_zconv<L_, 1>), grid, blk, 0, c->stream, a, c->d, tw);
+    else if (nout == 2) hipLaunchKernelGGL((k_zconv<L_, 2>), grid, blk, 0, c->stream, a, c->d, tw);
+    else if (nout == 3) hipLaunchKernelGGL((k_zconv<L_, 3>), grid, blk, 0, c->stream, a, c->d, tw);
+    else return HEDFT_EINVAL;
+  });
+  return HEDFT_OK;
+}
+
+int inv_yx(hedft_ctx* c, int nb, double2* const* spec, const XOut* out) {
+  SpecBatch s{};
+  XOutBatch a{};
+  for (int b = 0; b < nb; ++b) {
+    s.p[b] = spec[b];
+    a.in[b] = spec[b];
+    a.out[b] = out[b];
+  }
+  HEDFT_TRY((launch_col<1, true>(c, s, nb)));
+  return launch_xc2r(c, a, nb);
+}
+
+int fftfw(hedft_ctx* c, const double* in, double2* out) {
+  XIn xi{};
+  xi.op = XIN_COPY;
+  xi.a[0] = in;
+  HEDFT_TRY(fwd_xy(c, 1, &xi, &out));
+  SpecBatch s{};
+  s.p[0] = out;
+  return launch_col<2, false>(c, s, 1);
+}
+
+int fftbk(hedft_ctx* c, double2* in, double* out) {
+  SpecBatch s{};
+  s.p[0] = in;
+  HEDFT_TRY((launch_col<2, true>(c, s, 1)));
+  XOut xo{};
+  xo.op = XOUT_STORE;
+  xo.out = out;
+  return inv_yx(c, 1, &in, &xo);
+}
+
+}  // namespace hedft

@@ -1,0 +1,1666 @@
+// hedft.hip — mean field, integrators, impurities, energies and the C-ABI.
+//
+// Every entry point replaces one reference subroutine (file:line cited at the
+// definition) and keeps the reference's module-state semantics: fields live in
+// the context on the device; uimp and the TDDFT timec are evaluated on the fly
+// from the impurity tables / absorbing-layer parameters instead of being
+// stored.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <utility>
+
+#include "stencil_kernel.h"
+
+using namespace hedft;
+
+namespace hedft {
+
+// ---- device scalar slots ------------------------------------------------------
+enum {
+  S_MU = 0,      // sum(psi*hpsi)*dxyz/n4 of the current stage
+  S_SUMSQ = 1,   // sum(psi^2)
+  S_ERRPC = 2,   // sum|pc|
+  S_HMEAN = 3,
+  S_MUOLD = 4,
+  S_ERRIMP = 5,
+  S_ERRVIMP = 6,
+  S_ENERGY = 8,  // 8 partial energy sums
+  NSCAL = 32
+};
+// ---- impurity block layout (doubles) -------------------------------------------
+enum {
+  IMP_R = 0, IMP_V = 12, IMP_A = 24, IMP_F = 36, IMP_QR = 48, IMP_QV = 60, IMP_PCR = 72,
+  IMP_PCV = 84, IMP_ROLD = 96, IMP_VOLD = 132, IMP_AOLD = 168, IMP_MASS = 192, IMP_SIZE = 200
+};
+
+static const double RK_A[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 1.0 / 6.0};
+static const double RK_B[4] = {2.0, 1.0, 1.0, 2.0};
+static const double RK_C[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 0.5};
+static const double C112 = 112.0 / 121.0, C9 = 9.0 / 121.0, C4O3 = 4.0 / 3.0;
+
+ImpDev make_impdev(hedft_ctx* c) {
+  ImpDev im;
+  im.nimp = c->nimp;
+  im.npot = c->npot;
+  im.delinter = c->delinter;
+  im.rmax = c->rmaxinterpol;
+  im.potion = c->potion;
+  im.rimp = c->imp + IMP_R;
+  im.lstop = c->lstop;
+  im.x0 = -c->g.xmax;
+  im.y0 = -c->g.ymax;
+  im.z0 = -c->g.zmax;
+  im.hx = c->hx;
+  im.hy = c->hy;
+  im.hz = c->hz;
+  return im;
+}
+
+static size_t partials_region(hedft_ctx* c) { return c->partials_cap / 8; }
+double* force_partials(hedft_ctx* c, int axis) { return c->partials + (size_t)(1 + axis) * partials_region(c); }
+static double* gen_partials(hedft_ctx* c) { return c->partials; }
+
+// ---- reductions -------------------------------------------------------------------
+template <int NR>
+__global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ part, int nblocks, double* out,
+                                                  double scale) {
+  __shared__ double sh[NR][256];
+  for (int r = 0; r < NR; ++r) {
+    double s = 0.0, comp = 0.0;  // Neumaier compensated sum over this thread's blocks
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+      const double v = part[(size_t)b * NR + r];
+      const double t = s + v;
+      comp += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    sh[r][threadIdx.x] = s + comp;
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < NR) out[threadIdx.x] = scale * sh[threadIdx.x][0];
+}
+
+template <int NR>
+static void finalize(hedft_ctx* c, const double* part, size_t nblocks, int slot, double scale) {
+  hipLaunchKernelGGL((k_finalize<NR>), dim3(1), dim3(256), 0, c->stream, part, (int)nblocks,
+                     c->scal + slot, scale);
+}
+
+// F_k,a = -sum_blocks(partials_a) * dxyz; optionally aimp = F/m (DFT4He3d.f90:692-696)
+__global__ void k_force_finalize(const double* p0, int n0, const double* p1, int n1, const double* p2,
+                                 int n2, double* imp, int nimp, double dxyz, int update_aimp) {
+  __shared__ double sh[3][HEDFT_MAX_IMP][256];
+  const double* ps[3] = {p0, p1, p2};
+  const int ns[3] = {n0, n1, n2};
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < HEDFT_MAX_IMP; ++k) {
+      double s = 0.0;
+      for (int b = threadIdx.x; b < ns[a]; b += 256) s += ps[a][(size_t)b * HEDFT_MAX_IMP + k];
+      sh[a][k][threadIdx.x] = s;
+    }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < HEDFT_MAX_IMP; ++k) sh[a][k][threadIdx.x] += sh[a][k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < nimp; ++k) {
+        const double f = -sh[a][k][0] * dxyz;
+        imp[IMP_F + k + nimp * a] = f;
+        if (update_aimp) imp[IMP_A + k + nimp * a] = f / imp[IMP_MASS + k];
+      }
+  }
+}
+
+static int g_update_aimp = 0;
+int force_finalize(hedft_ctx* c) {
+  const size_t n0 = pencil_geom<double>(c->d, 0).nblocks, n1 = pencil_geom<double>(c->d, 1).nblocks,
+               n2 = pencil_geom<double>(c->d, 2).nblocks;
+  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, c->stream, force_partials(c, 0), (int)n0,
+                     force_partials(c, 1), (int)n1, force_partials(c, 2), (int)n2, c->imp, c->nimp, c->dxyz,
+                     g_update_aimp);
+  return HEDFT_OK;
+}
+
+// ---- absorbing layer: timec = Lambdah*(1+tanh(..x)+1+tanh(..y)+1+tanh(..z)) + i
+// (DFT4He3d.f90:592-601); per-axis tanh tables lam[ax][i]
+struct Lam {
+  const double *tx, *ty, *tz;
+  double lambdah;
+  __device__ __forceinline__ double at(int ix, int iy, int iz) const {
+    return lambdah * (1.0 + tx[ix] + 1.0 + ty[iy] + 1.0 + tz[iz]);
+  }
+};
+
+// ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
+// k = timec*((lap)*h2o2m4 - pot4*psi) - i*uimp*psi   (steprk.f90:67, steppc.f90:432,499)
+struct HpsiC {
+  const double2* lapxy;
+  const double2* psi;
+  const double* pot4;
+  Lam lam;
+  ImpDev im;
+  double h2o2m4;
+  __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2& ps) const {
+    const double2 l0 = lapxy[i];
+    const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
+    ps = psi[i];
+    const double v = pot4[i];
+    const double2 t = make_double2(lap.x * h2o2m4 - v * ps.x, lap.y * h2o2m4 - v * ps.y);
+    const double lr = lam.at(ix, iy, iz);
+    // (lr + i) * t
+    const double2 tt = make_double2(lr * t.x - t.y, lr * t.y + t.x);
+    const double u = (im.nimp > 0) ? imp_u(im, ix, iy, iz) : 0.0;
+    // - i u psi = (u psi.y, -u psi.x)
+    return make_double2(tt.x + u * ps.y, tt.y - u * ps.x);
+  }
+};
+
+struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
+  static constexpr int NR = 0;
+  double* partials;
+  HpsiC h;
+  double2 *psi, *q, *po0, *po1, *po2, *ho0, *ho1;
+  double* den;
+  double a, b, cc, dt;
+  int first;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double*) const {
+    double2 ps;
+    const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
+    const double2 qq = q[i];
+    const double2 s1 = make_double2(a * (k4.x - b * qq.x), a * (k4.y - b * qq.y));
+    q[i] = make_double2(qq.x + 3.0 * s1.x - cc * k4.x, qq.y + 3.0 * s1.y - cc * k4.y);
+    if (first) {
+      ho1[i] = ho0[i];
+      ho0[i] = k4;
+      po2[i] = po1[i];
+      po1[i] = po0[i];
+      po0[i] = ps;
+    }
+    const double2 np = make_double2(ps.x + dt * s1.x, ps.y + dt * s1.y);
+    psi[i] = np;
+    den[i] = np.x * np.x + np.y * np.y;
+  }
+};
+
+struct OutPred {  // predictor + modifier (steppc.f90:432-442)
+  static constexpr int NR = 0;
+  double* partials;
+  HpsiC h;
+  double2 *psi, *pc, *po3, *ho1, *ho2;
+  double* den;
+  double dt;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double*) const {
+    double2 ps;
+    const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
+    const double2 o3 = po3[i], h1 = ho1[i], h2 = ho2[i];
+    const double f = C4O3 * dt;
+    const double2 s1 = make_double2(o3.x + f * (2.0 * k4.x - h1.x + 2.0 * h2.x),
+                                    o3.y + f * (2.0 * k4.y - h1.y + 2.0 * h2.y));
+    po3[i] = ps;
+    const double2 p = pc[i];
+    const double2 np = make_double2(s1.x - C112 * p.x, s1.y - C112 * p.y);
+    psi[i] = np;
+    pc[i] = s1;
+    ho2[i] = k4;
+    den[i] = np.x * np.x + np.y * np.y;
+  }
+};
+
+struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
+  static constexpr int NR = 1;
+  double* partials;
+  HpsiC h;
+  double2 *psi, *pc, *po3, *po2, *ho1, *ho2;
+  double* den;
+  double dt;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double* red) const {
+    double2 ps;
+    const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
+    const double2 o3 = po3[i], o2 = po2[i], h1 = ho1[i], h2 = ho2[i];
+    const double f = 3.0 * dt;
+    const double2 s5 = make_double2(0.125 * (9.0 * o3.x - o2.x + f * (k4.x + 2.0 * h1.x - h2.x)),
+                                    0.125 * (9.0 * o3.y - o2.y + f * (k4.y + 2.0 * h1.y - h2.y)));
+    const double2 p0 = pc[i];
+    const double2 p = make_double2(p0.x - s5.x, p0.y - s5.y);
+    pc[i] = p;
+    const double2 np = make_double2(s5.x + C9 * p.x, s5.y + C9 * p.y);
+    psi[i] = np;
+    den[i] = np.x * np.x + np.y * np.y;
+    red[0] += sqrt(p.x * p.x + p.y * p.y);
+  }
+};
+
+// ---- statics: hpsi = -(lap)*h2o2m4 + pot4*psi and sum(psi*hpsi) (steprkr.f90:52-53)
+struct OutHpsiR {
+  static constexpr int NR = 1;
+  double* partials;
+  const double *lapxy, *psi, *pot4;
+  double* hpsi;
+  double h2o2m4;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double dz, double* red) const {
+    const double p = psi[i];
+    const double hp = -(lapxy[i] + dz) * h2o2m4 + pot4[i] * p;
+    hpsi[i] = hp;
+    red[0] += p * hp;
+  }
+};
+
+// ---- energy density (energy.f90): |d psi|^2 accumulated over axes -----------------
+template <typename V>
+__device__ __forceinline__ double vsq(V v);
+template <>
+__device__ __forceinline__ double vsq<double>(double v) { return v * v; }
+template <>
+__device__ __forceinline__ double vsq<double2>(double2 v) { return v.x * v.x + v.y * v.y; }
+
+template <typename V>
+struct OutSq {
+  static constexpr int NR = 0;
+  double* partials;
+  double* out;
+  const double* acc;
+  __device__ __forceinline__ void store(size_t i, int, int, int, V v, double*) const {
+    out[i] = (acc ? acc[i] : 0.0) + vsq<V>(v);
+  }
+};
+
+template <typename V>
+struct OutEnergy {
+  static constexpr int NR = 8;
+  double* partials;
+  const double *acc, *den, *dencg, *delj4, *denalf, *dxden, *dyden, *dzden, *intx, *inty, *intz, *uext;
+  ImpDev im;
+  double a1, a2, den0s, beta, den_m;
+  int otc;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, V v, double* red) const {
+    const double d = den[i], cg = dencg[i];
+    red[0] += acc[i] + vsq<V>(v);
+    red[1] += delj4[i] * d;
+    red[2] += d * cg * cg * (a1 + a2 * cg);
+    if (otc)
+      red[3] += (1.0 - denalf[i] / den0s) * (dxden[i] * intx[i] + dyden[i] * inty[i] + dzden[i] * intz[i]);
+    red[4] += d * (1.0 + tanh(beta * (d - den_m)));
+    red[5] += uext[i] * d;
+    if (im.nimp > 0) red[6] += imp_u(im, ix, iy, iz) * d;
+    red[7] += d;
+  }
+};
+
+// ---- statics elementwise kernels (grid-stride, fixed grid for deterministic sums)
+static constexpr int EW_BLOCKS = 1024;
+
+// one RK-Gill stage of STEPRKR after the mean (steprkr.f90:54-73), sum(psi^2)
+__global__ void __launch_bounds__(256) k_rkr_stage(long long n, double* psi, const double* hpsi, double* q,
+                                                   double* timec, double* po0, double* po1, double* po2,
+                                                   double* ho0, double* ho1, const double* scal, double a,
+                                                   double b, double cc, double deltat, double dtmin,
+                                                   double sdmu, double dmumax, double psimin, int first,
+                                                   double* partials) {
+  double red[1] = {0.0};
+  const double mu = scal[S_MU];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double p = psi[i], hp = hpsi[i];
+    if (first) timec[i] = deltat / (1.0 + exp((fabs(hp / p) - dmumax) / sdmu)) + dtmin;
+    const double s4 = -(hp - mu * p);
+    const double s1 = a * (s4 - b * q[i]);
+    q[i] = q[i] + 3.0 * s1 - cc * s4;
+    if (first) {
+      ho1[i] = ho0[i];
+      ho0[i] = s4;
+      po2[i] = po1[i];
+      po1[i] = po0[i];
+      po0[i] = p;
+    }
+    double np = p + timec[i] * s1;
+    if (psimin > 0.0 && np < psimin) np = psimin;
+    psi[i] = np;
+    red[0] += np * np;
+  }
+  block_reduce_store<1>(red, partials);
+}
+
+// psi = |psi| * sqrt(n4/(sum psi^2 dxyz)) [, max(psimin,.)], den = psi^2
+__global__ void __launch_bounds__(256) k_renorm(long long n, double* psi, double* den, const double* scal,
+                                                double n4, double dxyz, double psimin) {
+  const double cnorm = sqrt(n4 / (scal[S_SUMSQ] * dxyz));
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = fabs(psi[i]) * cnorm;
+    if (psimin > 0.0 && v < psimin) v = psimin;
+    psi[i] = v;
+    den[i] = v * v;
+  }
+}
+
+// STEPPCR predictor + modifier (steppcr.f90:140-153), sum(psi^2)
+__global__ void __launch_bounds__(256) k_pcr_pred(long long n, double* psi, const double* hpsi, double* pc,
+                                                  double* timec, double* po3, const double* ho1, double* ho2,
+                                                  const double* scal, double deltat, double dtmin, double sdmu,
+                                                  double dmumax, double* partials) {
+  double red[1] = {0.0};
+  const double hmean = scal[S_MU];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double p = psi[i], hp = hpsi[i];
+    const double tc = deltat / (1.0 + exp((fabs(hp / p) - dmumax) / sdmu)) + dtmin;
+    timec[i] = tc;
+    const double s4 = -(hp - hmean * p);
+    const double s1 = po3[i] + C4O3 * tc * (2.0 * s4 - ho1[i] + 2.0 * ho2[i]);
+    po3[i] = p;
+    const double np = s1 - C112 * pc[i];
+    psi[i] = np;
+    pc[i] = s1;
+    ho2[i] = s4;
+    red[0] += np * np;
+  }
+  block_reduce_store<1>(red, partials);
+}
+
+// STEPPCR corrector + final (steppcr.f90:174-186): sum(psi^2), sum|pc|
+__global__ void __launch_bounds__(256) k_pcr_corr(long long n, double* psi, const double* hpsi, double* pc,
+                                                  const double* timec, const double* po3, const double* po2,
+                                                  const double* ho1, const double* ho2, const double* scal,
+                                                  double* partials) {
+  double red[2] = {0.0, 0.0};
+  const double mu = scal[S_MU];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double p = psi[i];
+    const double s4 = -(hpsi[i] - mu * p);
+    const double s5 = 0.125 * (9.0 * po3[i] - po2[i] + 3.0 * timec[i] * (s4 + 2.0 * ho1[i] - ho2[i]));
+    const double pcv = pc[i] - s5;
+    pc[i] = pcv;
+    const double np = s5 + C9 * pcv;
+    psi[i] = np;
+    red[0] += np * np;
+    red[1] += fabs(pcv);
+  }
+  block_reduce_store<2>(red, partials);
+}
+
+__global__ void k_copy_scal(double* s, int from, int to) { s[to] = s[from]; }
+
+// ---- impurity Newton updates (single thread; N_imp*3 <= 12 numbers) ------------------
+__global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, double cc, double dt) {
+  const int m = 3 * nimp;
+  double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *QR = imp + IMP_QR, *QV = imp + IMP_QV;
+  double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
+  for (int i = 0; i < m; ++i) {  // steprk.f90:330-340
+    const double st = a * (V[i] - b * QR[i]);
+    QR[i] = QR[i] + 3.0 * st - cc * V[i];
+    if (first) {
+      RO[24 + i] = RO[12 + i];
+      RO[12 + i] = RO[i];
+      RO[i] = R[i];
+    }
+    R[i] = R[i] + dt * st;
+  }
+  for (int i = 0; i < m; ++i) {  // steprk.f90:345-354
+    const double st = a * (A[i] - b * QV[i]);
+    QV[i] = QV[i] + 3.0 * st - cc * A[i];
+    if (first) {
+      AO[12 + i] = AO[i];
+      AO[i] = A[i];
+      VO[24 + i] = VO[12 + i];
+      VO[12 + i] = VO[i];
+      VO[i] = V[i];
+    }
+    V[i] = V[i] + dt * st;
+  }
+}
+
+// steppc.f90:461-480 (slot indices 0-based)
+__global__ void k_imp_pred(double* imp, int nimp, double dt, int r3, int v1, int v2, int v3, int a1, int a2) {
+  const int m = 3 * nimp;
+  double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *PCR = imp + IMP_PCR, *PCV = imp + IMP_PCV;
+  double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
+  double st[12];
+  for (int i = 0; i < m; ++i) st[i] = RO[12 * r3 + i] + C4O3 * dt * (2.0 * V[i] - VO[12 * v1 + i] + 2.0 * VO[12 * v2 + i]);
+  for (int i = 0; i < m; ++i) {
+    RO[12 * r3 + i] = R[i];
+    R[i] = st[i] - C112 * PCR[i];
+    PCR[i] = st[i];
+  }
+  for (int i = 0; i < m; ++i) st[i] = VO[12 * v3 + i] + C4O3 * dt * (2.0 * A[i] - AO[12 * a1 + i] + 2.0 * AO[12 * a2 + i]);
+  for (int i = 0; i < m; ++i) {
+    VO[12 * v3 + i] = st[i] - C112 * PCV[i];
+    PCV[i] = st[i];
+  }
+  for (int i = 0; i < m; ++i) AO[12 * a2 + i] = A[i];
+}
+
+// steppc.f90:530-555; errimp, errvimp into scal
+__global__ void k_imp_corr(double* imp, int nimp, double dt, int r2, int r3, int v1, int v2, int v3, int a1,
+                           int a2, double* scal) {
+  const int m = 3 * nimp;
+  double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *PCR = imp + IMP_PCR, *PCV = imp + IMP_PCV;
+  double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
+  double st[12];
+  double e1 = 0.0, e2 = 0.0;
+  for (int i = 0; i < m; ++i)
+    st[i] = 0.125 * (9.0 * RO[12 * r3 + i] - RO[12 * r2 + i] +
+                     3.0 * dt * (VO[12 * v3 + i] + 2.0 * V[i] - VO[12 * v1 + i]));
+  for (int i = 0; i < m; ++i) {
+    PCR[i] = PCR[i] - st[i];
+    R[i] = st[i] + C9 * PCR[i];
+    e1 += fabs(C9 * PCR[i]);
+  }
+  for (int i = 0; i < m; ++i)
+    st[i] = 0.125 * (9.0 * V[i] - VO[12 * v2 + i] + 3.0 * dt * (A[i] + 2.0 * AO[12 * a1 + i] - AO[12 * a2 + i]));
+  for (int i = 0; i < m; ++i) {
+    PCV[i] = PCV[i] - st[i];
+    VO[12 * v3 + i] = V[i];
+    V[i] = st[i] + C9 * PCV[i];
+    e2 += fabs(C9 * PCV[i]);
+  }
+  scal[S_ERRIMP] = e1 * 0.3333333333 / nimp;
+  scal[S_ERRVIMP] = e2 * 0.3333333333 / nimp;
+}
+
+__global__ void k_imp_aimp(double* imp, int nimp) {
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < nimp; ++k) imp[IMP_A + k + nimp * a] = imp[IMP_F + k + nimp * a] / imp[IMP_MASS + k];
+}
+
+// ---- field evaluation kernels for downloads -------------------------------------------
+__global__ void k_eval_uimp(Dims d, ImpDev im, double* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % d.nx), iy = (int)((i / d.nx) % d.ny), iz = (int)(i / ((long long)d.nx * d.ny));
+    out[i] = (im.nimp > 0) ? imp_u(im, ix, iy, iz) : 0.0;
+  }
+}
+__global__ void k_eval_timec(Dims d, Lam lam, double2* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % d.nx), iy = (int)((i / d.nx) % d.ny), iz = (int)(i / ((long long)d.nx * d.ny));
+    out[i] = make_double2(lam.at(ix, iy, iz), 1.0);
+  }
+}
+__global__ void k_eval_falfs(long long n, const double* denalf, double den0s, double* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = 1.0 - denalf[i] / den0s;
+}
+// TDDFT start: psi = sqrt(den) (real), den = |psi|^2 (DFT4He3d.f90:541-556)
+__global__ void k_tddft_psi0(long long n, double* den, double2* psi) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double s = sqrt(den[i]);
+    psi[i] = make_double2(s, 0.0);
+    den[i] = s * s;
+  }
+}
+// statics start: den *= n4/(sum den dxyz), psi = sqrt(den) (BCN4HeDFT.f90:730-734)
+__global__ void k_sum1(long long n, const double* a, double* partials) {
+  double red[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    red[0] += a[i];
+  block_reduce_store<1>(red, partials);
+}
+__global__ void k_statics_psi0(long long n, double* den, double* psi, const double* scal, double n4, double dxyz) {
+  const double cnorm = n4 / (scal[S_SUMSQ] * dxyz);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = cnorm * den[i];
+    den[i] = v;
+    psi[i] = sqrt(v);
+  }
+}
+
+// ---- kernel tables on the host: fforma / initcg / kalfs ---------------------------------
+// The LJ form factor is a difference of two nearly equal terms at large |p|
+// (fforma.f90:96-100), so it is evaluated in long double (x87, 64-bit mantissa)
+// and rounded once.
+// Sine integral Si(x): power series (Abramowitz & Stegun 5.2.14) for |x| <= 4, else
+// Si = pi/2 + Im E1(i x) with E1 from its continued fraction (A&S 5.1.22, modified
+// Lentz).  comp returns pi/2 - Si(x) for x > 0 without cancellation.
+typedef long double ldbl;
+static void sine_integral(ldbl x, ldbl* si, ldbl* comp) {
+  const ldbl halfpi = 1.570796326794896619231321691639751442L;
+  const ldbl ax = fabsl(x);
+  if (ax <= 4.0L) {
+    const ldbl x2 = ax * ax;
+    ldbl term = ax, sum = ax;
+    for (int k = 1; k < 80; ++k) {
+      term *= -x2 / ((2.0L * k) * (2.0L * k + 1.0L));
+      const ldbl t = term / (2.0L * k + 1.0L);
+      sum += t;
+      if (fabsl(t) < 1e-22L * fabsl(sum)) break;
+    }
+    const ldbl s = (x < 0) ? -sum : sum;
+    *si = s;
+    *comp = halfpi - s;
+    return;
+  }
+  // continued fraction of E1(z), z = i ax, on (re, im) pairs
+  ldbl br = 1.0L, bi = ax, cr = 1e300L, ci = 0.0L;
+  ldbl den = br * br + bi * bi;
+  ldbl dr = br / den, di = -bi / den, hr = dr, hi = di;
+  for (int i = 1; i < 400; ++i) {
+    const ldbl a = -(ldbl)i * i;
+    br += 2.0L;
+    const ldbl tr = a * dr + br, ti = a * di + bi;
+    den = tr * tr + ti * ti;
+    dr = tr / den;
+    di = -ti / den;
+    den = cr * cr + ci * ci;
+    cr = br + a * cr / den;
+    ci = bi - a * ci / den;
+    const ldbl delr = cr * dr - ci * di, deli = cr * di + ci * dr;
+    const ldbl nhr = hr * delr - hi * deli, nhi = hr * deli + hi * delr;
+    hr = nhr;
+    hi = nhi;
+    if (fabsl(delr - 1.0L) + fabsl(deli) < 1e-21L) break;
+  }
+  const ldbl fi = hi * cosl(ax) - hr * sinl(ax);  // Im(h e^{-i ax}) = Si - pi/2
+  const ldbl s = halfpi + fi;
+  *si = (x < 0) ? -s : s;
+  *comp = (x < 0) ? halfpi + s : -fi;
+}
+
+static ldbl sinx11(ldbl a, ldbl x) {  // fforma.f90:73-102: int_x^inf sin(a t)/t^11 dt
+  const ldbl y = a * x, y2 = y * y;
+  const ldbl b[5] = {40320.0L, -720.0L, 24.0L, -2.0L, 1.0L};
+  const ldbl cc[5] = {362880.0L, -5040.0L, 120.0L, -6.0L, 1.0L};
+  const ldbl d[5] = {1.0L, y2, y2 * y2, y2 * y2 * y2, y2 * y2 * y2 * y2};
+  ldbl bd = 0.0L, cd = 0.0L;
+  for (int i = 0; i < 5; ++i) {
+    bd += b[i] * d[i];
+    cd += cc[i] * d[i];
+  }
+  const ldbl res1 = (a * bd * cosl(y) + cd * sinl(y) / x) / (3628800.0L * powl(x, 9));
+  ldbl si, comp;
+  sine_integral(y, &si, &comp);
+  const ldbl sinint = (a > 0.0L) ? comp : 1.570796326794896619231321691639751442L + si;
+  return res1 - powl(a, 10) * sinint / 3628800.0L;
+}
+
+static ldbl sinx5(ldbl a, ldbl x) {  // fforma.f90:108-137: int_x^inf sin(a t)/t^5 dt
+  const ldbl y = a * x, y2 = y * y;
+  const ldbl res1 = -(a * (-2.0L + y2) * cosl(y) + (-6.0L + y2) * sinl(y) / x) / (24.0L * x * x * x);
+  ldbl si, comp;
+  sine_integral(y, &si, &comp);
+  const ldbl sinint = (a > 0.0L) ? comp : 1.570796326794896619231321691639751442L + si;
+  return res1 + powl(a, 4) * sinint / 24.0L;
+}
+
+static double fvlj_radial(double p, double eps, double sigma, double hc) {  // fforma.f90:36-60, 'OT'
+  const ldbl s6 = powl((ldbl)sigma, 6);
+  if (std::fabs(p) < 1e-10) {  // bforce (fforma.f90:256-280)
+    const ldbl pi4 = 4.0L * 3.141592653589793238462643383279502884L, eps4 = 4.0L * eps;
+    return (double)(pi4 * eps4 * s6 * s6 / (9.0L * powl((ldbl)hc, 9)) - pi4 * eps4 * s6 / (3.0L * powl((ldbl)hc, 3)));
+  }
+  const ldbl a = 2.0L * 3.141592653589793238462643383279502884L * (ldbl)p;
+  return (double)(8.0L * (ldbl)eps * s6 * (s6 * sinx11(a, (ldbl)hc) - sinx5(a, (ldbl)hc)) / (ldbl)p);
+}
+
+static void kernel_tables(const hedft_ctx* c, std::vector<double>& fv, std::vector<double>& wc,
+                          std::vector<double>& ka) {
+  const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, nxh = c->d.nxh;
+  const double hpx = 1.0 / (nx * c->hx), hpy = 1.0 / (ny * c->hy), hpz = 1.0 / (nz * c->hz);
+  const double pmx = 1.0 / (2.0 * c->hx), pmy = 1.0 / (2.0 * c->hy), pmz = 1.0 / (2.0 * c->hz);
+  auto pgrid = [](int n, double hp, double pmax, std::vector<double>& p) {  // BCN4HeDFT.f90:337-356
+    p.resize(n);
+    for (int i = 1; i <= n / 2 + 1; ++i) p[i - 1] = hp * (i - 1);
+    for (int i = n / 2 + 2; i <= n; ++i) p[i - 1] = -pmax + hp * (i - (n / 2 + 1));
+  };
+  std::vector<double> px, py, pz;
+  pgrid(nx, hpx, pmx, px);
+  pgrid(ny, hpy, pmy, py);
+  pgrid(nz, hpz, pmz, pz);
+  fv.resize(c->d.nh);
+  wc.resize(c->d.nh);
+  ka.resize(c->d.nh);
+  const double h4 = c->p.h4, twopi = 2.0 * M_PI;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nxh; ++ix) {
+        const size_t k = (size_t)ix + (size_t)nxh * (iy + (size_t)ny * iz);
+        const double pm = std::sqrt(px[ix] * px[ix] + py[iy] * py[iy] + pz[iz] * pz[iz]);
+        fv[k] = fvlj_radial(pm, c->p.eps4, c->p.sigma4, h4);
+        double w = 1.0;  // initcg.f90:314-331
+        if (pm > 0.0) {
+          const double arg = twopi * pm * h4;
+          const double aux = std::sin(arg) - arg * std::cos(arg);
+          w = (std::fabs(aux) < 1e-10) ? 1.0 : 3.0 * aux / (arg * arg * arg);
+        }
+        wc[k] = w;
+        const double e = M_PI * c->p.l * pm;  // BCN4HeDFT.f90:849
+        ka[k] = std::exp(-e * e);
+      }
+}
+
+// ---- mean field pipelines --------------------------------------------------------------------
+static XOut xo_store(double* out) {
+  XOut o{};
+  o.op = XOUT_STORE;
+  o.out = out;
+  return o;
+}
+
+static XOut xo_pot4(hedft_ctx* c) {
+  XOut o{};
+  o.op = XOUT_POT4;
+  o.out = c->pot4;
+  o.delj4 = c->delj4;
+  o.dencg = c->dencg;
+  o.den = c->den;
+  o.uext = c->uext;
+  o.a0 = c->p.cp4 / 2.0;
+  o.a1 = c->p.cpp4 / 3.0;
+  o.csolid = c->p.csolid;
+  o.beta = c->p.beta_solid;
+  o.den_m = c->p.den_m;
+  o.lsolid = c->p.lsolid;
+  o.lext = (c->mode == HEDFT_MODE_STATICS) && c->p.lexternal;
+  return o;
+}
+
+// correlation convolution F^-1[wcgk F(den dencg (cp4+cpp4 dencg))]; for the OT core the
+// C2R epilogue assembles pot4 directly
+static int correlation(hedft_ctx* c) {
+  XIn ci{};
+  ci.op = XIN_CORR;
+  ci.a[0] = c->den;
+  ci.a[1] = c->dencg;
+  ci.c0 = c->p.cp4;
+  ci.c1 = c->p.cpp4;
+  double2* s0 = c->spec[0];
+  HEDFT_TRY(fwd_xy(c, 1, &ci, &s0));
+  const double* kw = c->wcgk;
+  HEDFT_TRY(zconv(c, 1, &s0, 1, &kw, &s0, 1, nullptr));
+  XOut xo = c->p.core_otc ? xo_store(c->sto1) : xo_pot4(c);
+  return inv_yx(c, 1, &s0, &xo);
+}
+
+// term_alfa (term_alfa.f90:5-102) + pot4 assembly
+static int alpha_term(hedft_ctx* c) {
+  XIn fg[3];
+  const double* grads[3] = {c->dxden, c->dyden, c->dzden};
+  for (int a = 0; a < 3; ++a) {
+    fg[a] = XIn{};
+    fg[a].op = XIN_FGRAD;
+    fg[a].a[0] = c->denalf;
+    fg[a].a[1] = grads[a];
+    fg[a].c0 = c->p.den0s;
+  }
+  double2* s3[3] = {c->spec[1], c->spec[2], c->spec[3]};
+  HEDFT_TRY(fwd_xy(c, 3, fg, s3));
+  const double* ka = c->kalfs;
+  HEDFT_TRY(zconv(c, 3, s3, 1, &ka, s3, 1, nullptr));
+  XOut xo3[3] = {xo_store(c->intx), xo_store(c->inty), xo_store(c->intz)};
+  HEDFT_TRY(inv_yx(c, 3, s3, xo3));
+  XIn dot{};
+  dot.op = XIN_DOT3;
+  dot.a[0] = c->dxden;
+  dot.a[1] = c->dyden;
+  dot.a[2] = c->dzden;
+  dot.a[3] = c->intx;
+  dot.a[4] = c->inty;
+  dot.a[5] = c->intz;
+  double2* s0 = c->spec[0];
+  HEDFT_TRY(fwd_xy(c, 1, &dot, &s0));
+  HEDFT_TRY(zconv(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
+  XOut xu = xo_store(c->ualph_conv);
+  HEDFT_TRY(inv_yx(c, 1, &s0, &xu));
+  return alpha_div_pot4(c);
+}
+
+// Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden)
+static int poten_tddft(hedft_ctx* c, int want_force) {
+  XIn xi{};
+  xi.op = XIN_COPY;
+  xi.a[0] = c->den;
+  double2* s0 = c->spec[0];
+  HEDFT_TRY(fwd_xy(c, 1, &xi, &s0));
+  if (c->p.core_otc) {
+    const double* kk[3] = {c->wcgk, c->fvlj4, c->kalfs};
+    double2* so[3] = {c->spec[1], c->spec[2], c->spec[3]};
+    HEDFT_TRY(zconv(c, 1, &s0, 3, kk, so, 1, nullptr));
+    XOut xo[3] = {xo_store(c->dencg), xo_store(c->delj4), xo_store(c->denalf)};
+    HEDFT_TRY(inv_yx(c, 3, so, xo));
+  } else {
+    const double* kk[2] = {c->wcgk, c->fvlj4};
+    double2* so[2] = {c->spec[1], c->spec[2]};
+    HEDFT_TRY(zconv(c, 1, &s0, 2, kk, so, 1, nullptr));
+    XOut xo[2] = {xo_store(c->dencg), xo_store(c->delj4)};
+    HEDFT_TRY(inv_yx(c, 2, so, xo));
+  }
+  g_update_aimp = 0;
+  HEDFT_TRY(derden_force(c, want_force));
+  HEDFT_TRY(correlation(c));
+  if (c->p.core_otc) HEDFT_TRY(alpha_term(c));
+  return HEDFT_OK;
+}
+
+// He_statics/poten.f90:5-221: fden, dencg, denalf and grad(den) as left by the caller
+static int poten_statics(hedft_ctx* c) {
+  const double* kf = c->fvlj4;
+  double2* s2 = c->spec[2];
+  double2* fd = c->fden;
+  HEDFT_TRY(zconv(c, 1, &fd, 1, &kf, &s2, 0, nullptr));
+  XOut xo = xo_store(c->delj4);
+  HEDFT_TRY(inv_yx(c, 1, &s2, &xo));
+  HEDFT_TRY(correlation(c));
+  if (c->p.core_otc) HEDFT_TRY(alpha_term(c));
+  return HEDFT_OK;
+}
+
+// driver refresh after a statics step (BCN4HeDFT.f90:1131-1149): derden, fden, dencg, denalf
+static int statics_refresh(hedft_ctx* c) {
+  HEDFT_TRY(derden_force(c, 0));
+  XIn xi{};
+  xi.op = XIN_COPY;
+  xi.a[0] = c->den;
+  double2* s0 = c->spec[0];
+  HEDFT_TRY(fwd_xy(c, 1, &xi, &s0));
+  double2* fd = c->fden;
+  if (c->p.core_otc) {
+    const double* kk[2] = {c->wcgk, c->kalfs};
+    double2* so[2] = {c->spec[1], c->spec[3]};
+    HEDFT_TRY(zconv(c, 1, &s0, 2, kk, so, 1, &fd));
+    XOut xo[2] = {xo_store(c->dencg), xo_store(c->denalf)};
+    return inv_yx(c, 2, so, xo);
+  }
+  const double* kw = c->wcgk;
+  double2* s1 = c->spec[1];
+  HEDFT_TRY(zconv(c, 1, &s0, 1, &kw, &s1, 1, &fd));
+  XOut xo = xo_store(c->dencg);
+  return inv_yx(c, 1, &s1, &xo);
+}
+
+int poten(hedft_ctx* c, int want_force) {
+  return (c->mode == HEDFT_MODE_TDDFT) ? poten_tddft(c, want_force) : poten_statics(c);
+}
+
+
+// ---- integrator drivers ------------------------------------------------------------------------
+static Lam make_lam(hedft_ctx* c) {
+  Lam l;
+  l.tx = c->lamtab;
+  l.ty = c->lamtab + c->d.nx;
+  l.tz = c->lamtab + c->d.nx + c->d.ny;
+  l.lambdah = c->p.lambdah;
+  return l;
+}
+
+static HpsiC make_hpsic(hedft_ctx* c) {
+  HpsiC h;
+  h.lapxy = (const double2*)c->lap;
+  h.psi = (const double2*)c->psi;
+  h.pot4 = c->pot4;
+  h.lam = make_lam(c);
+  h.im = make_impdev(c);
+  h.h2o2m4 = c->p.h2o2m4;
+  return h;
+}
+
+static inline double2* cz(void* p) { return (double2*)p; }
+static inline double* rz(void* p) { return (double*)p; }
+
+// STEPRK (Dynamics_TDDFT/steprk.f90:1-144)
+static int steprk_impl(hedft_ctx* c, double dt) {
+  for (int jrun = 0; jrun < 4; ++jrun) {
+    HEDFT_TRY(lap_xy(c, c->psi, 1));
+    OutRK o;
+    o.partials = nullptr;
+    o.h = make_hpsic(c);
+    o.psi = cz(c->psi);
+    o.q = cz(c->q);
+    o.po0 = cz(c->psiold[0]);
+    o.po1 = cz(c->psiold[1]);
+    o.po2 = cz(c->psiold[2]);
+    o.ho0 = cz(c->hpsiold[0]);
+    o.ho1 = cz(c->hpsiold[1]);
+    o.den = c->den;
+    o.a = RK_A[jrun];
+    o.b = RK_B[jrun];
+    o.cc = RK_C[jrun];
+    o.dt = dt;
+    o.first = (jrun == 0);
+    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    if (c->nimp > 0)
+      hipLaunchKernelGGL(k_imp_rk, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, (int)(jrun == 0), RK_A[jrun],
+                         RK_B[jrun], RK_C[jrun], dt);
+    if (jrun <= 2) HEDFT_TRY(poten_tddft(c, 0));
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = c->ioldr[k] = c->ioldv[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = c->iolda[k] = k;
+  c->rk_done++;
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+// STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC]
+static int steppc_impl(hedft_ctx* c, double dt) {
+  HEDFT_TRY(lap_xy(c, c->psi, 1));
+  {
+    OutPred o;
+    o.partials = nullptr;
+    o.h = make_hpsic(c);
+    o.psi = cz(c->psi);
+    o.pc = cz(c->pc);
+    o.po3 = cz(c->psiold[c->ioldp[2]]);
+    o.ho1 = cz(c->hpsiold[c->ioldh[0]]);
+    o.ho2 = cz(c->hpsiold[c->ioldh[1]]);
+    o.den = c->den;
+    o.dt = dt;
+    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    std::swap(c->ioldh[0], c->ioldh[1]);
+  }
+  if (c->nimp > 0) {
+    hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, dt, c->ioldr[2], c->ioldv[0],
+                       c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1]);
+    std::swap(c->iolda[0], c->iolda[1]);
+  }
+  HEDFT_TRY(poten_tddft(c, 0));
+  HEDFT_TRY(lap_xy(c, c->psi, 1));
+  {
+    OutCorr o;
+    o.partials = gen_partials(c);
+    o.h = make_hpsic(c);
+    o.psi = cz(c->psi);
+    o.pc = cz(c->pc);
+    o.po3 = cz(c->psiold[c->ioldp[2]]);
+    o.po2 = cz(c->psiold[c->ioldp[1]]);
+    o.ho1 = cz(c->hpsiold[c->ioldh[0]]);
+    o.ho2 = cz(c->hpsiold[c->ioldh[1]]);
+    o.den = c->den;
+    o.dt = dt;
+    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    finalize<1>(c, gen_partials(c), pencil_geom<double2>(c->d, 2).nblocks, S_ERRPC, 1.0);
+    const int t = c->ioldp[2];
+    c->ioldp[2] = c->ioldp[1];
+    c->ioldp[1] = c->ioldp[0];
+    c->ioldp[0] = t;
+  }
+  if (c->nimp > 0) {
+    hipLaunchKernelGGL(k_imp_corr, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, dt, c->ioldr[1], c->ioldr[2],
+                       c->ioldv[0], c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1], c->scal);
+    int t = c->ioldr[2];
+    c->ioldr[2] = c->ioldr[1];
+    c->ioldr[1] = c->ioldr[0];
+    c->ioldr[0] = t;
+    t = c->ioldv[2];
+    c->ioldv[2] = c->ioldv[1];
+    c->ioldv[1] = c->ioldv[0];
+    c->ioldv[0] = t;
+  }
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+// driver part after each step (DFT4He3d.f90:686-696): potenimp, poten, forceimp, aimp = F/m
+static int tddft_after_step(hedft_ctx* c) {
+  HEDFT_TRY(poten_tddft(c, 1));
+  if (c->nimp > 0) {
+    // derden_force issued the force reduction without aimp; redo the finalize with aimp
+    g_update_aimp = 1;
+    HEDFT_TRY(force_finalize(c));
+    g_update_aimp = 0;
+  }
+  return HEDFT_OK;
+}
+
+// ---- statics -------------------------------------------------------------------------------------
+static int hpsi_statics(hedft_ctx* c, double n4) {
+  HEDFT_TRY(lap_xy(c, c->psi, 0));
+  OutHpsiR o;
+  o.partials = gen_partials(c);
+  o.lapxy = rz(c->lap);
+  o.psi = rz(c->psi);
+  o.pot4 = c->pot4;
+  o.hpsi = c->hpsi_r;
+  o.h2o2m4 = c->p.h2o2m4;
+  launch_pencil<double, 2>(c->stream, c->d, c->coef[1][2], InPlain<double>{rz(c->psi)}, o);
+  finalize<1>(c, gen_partials(c), pencil_geom<double>(c->d, 2).nblocks, S_MU, c->dxyz / n4);
+  return HEDFT_OK;
+}
+
+static void renorm(hedft_ctx* c, double n4) {
+  finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+  hipLaunchKernelGGL(k_renorm, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->den, c->scal, n4,
+                     c->dxyz, c->p.psimin);
+}
+
+static int read_scal(hedft_ctx* c, int first, int count, double* out) {
+  HEDFT_CHECK(hipMemcpyAsync(c->host_scal, c->scal + first, count * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < count; ++i) out[i] = c->host_scal[i];
+  return HEDFT_OK;
+}
+
+// STEPRKR (He_statics/steprkr.f90:1-96)
+static int steprkr_impl(hedft_ctx* c, double dt, double n4, double* hmean) {
+  for (int jrun = 0; jrun < 4; ++jrun) {
+    HEDFT_TRY(hpsi_statics(c, n4));
+    if (jrun == 0)
+      hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, c->stream, c->scal, S_MU, S_HMEAN);
+    hipLaunchKernelGGL(k_rkr_stage, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->hpsi_r,
+                       rz(c->q), c->timec_r, rz(c->psiold[0]), rz(c->psiold[1]), rz(c->psiold[2]),
+                       rz(c->hpsiold[0]), rz(c->hpsiold[1]), c->scal, RK_A[jrun], RK_B[jrun], RK_C[jrun], dt,
+                       c->p.dtmin, c->p.sdmu, c->p.dmumax, c->p.psimin, (int)(jrun == 0), gen_partials(c));
+    // psimin clamp is applied before the norm in steprkr (steprkr.f90:73-78)
+    finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+    hipLaunchKernelGGL(k_renorm, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->den, c->scal,
+                       n4, c->dxyz, -1.0);
+    if (jrun <= 2) HEDFT_TRY(poten_statics(c));
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = k;
+  c->rk_done++;
+  (void)hmean;
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+// STEPPCR (He_statics/steppcr.f90:1-108)
+static int steppcr_impl(hedft_ctx* c, double dt, double n4) {
+  HEDFT_TRY(hpsi_statics(c, n4));
+  hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, c->stream, c->scal, S_MU, S_HMEAN);
+  hipLaunchKernelGGL(k_pcr_pred, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->hpsi_r,
+                     rz(c->pc), c->timec_r, rz(c->psiold[c->ioldp[2]]), rz(c->hpsiold[c->ioldh[0]]),
+                     rz(c->hpsiold[c->ioldh[1]]), c->scal, dt, c->p.dtmin, c->p.sdmu, c->p.dmumax, gen_partials(c));
+  renorm(c, n4);
+  std::swap(c->ioldh[0], c->ioldh[1]);
+  HEDFT_TRY(poten_statics(c));
+  HEDFT_TRY(hpsi_statics(c, n4));
+  hipLaunchKernelGGL(k_pcr_corr, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->hpsi_r,
+                     rz(c->pc), c->timec_r, rz(c->psiold[c->ioldp[2]]), rz(c->psiold[c->ioldp[1]]),
+                     rz(c->hpsiold[c->ioldh[0]]), rz(c->hpsiold[c->ioldh[1]]), c->scal, gen_partials(c));
+  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c), EW_BLOCKS,
+                     c->scal + S_SUMSQ, 1.0);  // S_SUMSQ = sum psi^2, S_ERRPC = sum|pc|
+  hipLaunchKernelGGL(k_renorm, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->den, c->scal, n4,
+                     c->dxyz, c->p.psimin);
+  const int t = c->ioldp[2];
+  c->ioldp[2] = c->ioldp[1];
+  c->ioldp[1] = c->ioldp[0];
+  c->ioldp[0] = t;
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+// ---- energy -----------------------------------------------------------------------------------
+template <typename V>
+static int energy_sums(hedft_ctx* c, double* s) {
+  const V* psi = (const V*)c->psi;
+  launch_pencil<V, 0>(c->stream, c->d, c->coef[0][0], InPlain<V>{psi}, OutSq<V>{nullptr, c->work1, nullptr});
+  launch_pencil<V, 1>(c->stream, c->d, c->coef[0][1], InPlain<V>{psi}, OutSq<V>{nullptr, c->work1, c->work1});
+  OutEnergy<V> o;
+  o.partials = gen_partials(c);
+  o.acc = c->work1;
+  o.den = c->den;
+  o.dencg = c->dencg;
+  o.delj4 = c->delj4;
+  o.denalf = c->denalf;
+  o.dxden = c->dxden;
+  o.dyden = c->dyden;
+  o.dzden = c->dzden;
+  o.intx = c->intx;
+  o.inty = c->inty;
+  o.intz = c->intz;
+  o.uext = c->uext;
+  o.im = make_impdev(c);
+  o.a1 = 0.5 * c->p.cp4;
+  o.a2 = c->p.cpp4 / 3.0;
+  o.den0s = c->p.den0s;
+  o.beta = c->p.beta_solid;
+  o.den_m = c->p.den_m;
+  o.otc = c->p.core_otc;
+  launch_pencil<V, 2>(c->stream, c->d, c->coef[0][2], InPlain<V>{psi}, o);
+  hipLaunchKernelGGL(k_finalize<8>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
+                     (int)pencil_geom<V>(c->d, 2).nblocks, c->scal + S_ENERGY, 1.0);
+  return read_scal(c, S_ENERGY, 8, s);
+}
+
+}  // namespace hedft
+
+// ---------------------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------------------
+extern "C" {
+
+const char* hedft_strerror(int rc) {
+  switch (rc) {
+    case HEDFT_OK: return "ok";
+    case HEDFT_EINVAL: return "invalid argument";
+    case HEDFT_EGRID: return "grid edge not supported";
+    case HEDFT_EHIP: return "HIP runtime error";
+    case HEDFT_ENOMEM: return "device allocation failed";
+    case HEDFT_ESTATE: return "call out of order";
+    case HEDFT_EBOUNDS: return "impurity outside the interpolation table";
+    default: return "unknown error";
+  }
+}
+
+void hedft_default_params(hedft_params* p) {
+  std::memset(p, 0, sizeof *p);
+  p->h2o2m4 = 6.05969638298;
+  p->cp4 = -2.41186e4;
+  p->cpp4 = 1.85850e6;
+  p->alphas = 54.31;
+  p->l = 1.0;
+  p->den0s = 0.04;
+  p->csolid = 3.1577504e4;
+  p->beta_solid = 40.0;
+  p->den_m = 0.37;
+  p->h4 = 2.190323;
+  p->eps4 = 10.22;
+  p->sigma4 = 2.556;
+  p->core_otc = 0;
+  p->lsolid = 0;
+  p->npd = 13;
+  p->icon = 13;
+  p->dtmin = 1.0e-10;
+  p->sdmu = 0.1;
+  p->dmumax = 100.0;
+  p->psimin = -5.0;
+  p->lambdah = 2.0;
+  for (int a = 0; a < 3; ++a) {
+    p->tmean[a] = 1000.0;
+    p->tsurf[a] = 2.0;
+  }
+  p->precision = HEDFT_PREC_FP64;
+  p->lexternal = 0;
+}
+
+int hedft_grid_supported(int32_t n) {
+  switch (n) {
+    case 32: case 48: case 64: case 96: case 128: case 192: case 256: return 1;
+    default: return 0;
+  }
+}
+
+static int alloc(hedft_ctx* c, void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) return HEDFT_ENOMEM;
+  if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess) return HEDFT_EHIP;
+  return HEDFT_OK;
+}
+
+int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32_t device, hedft_ctx** out) {
+  if (!g || !p || !out) return HEDFT_EINVAL;
+  if (mode != HEDFT_MODE_STATICS && mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  if (!hedft_grid_supported(g->nx) || !hedft_grid_supported(g->ny) || !hedft_grid_supported(g->nz))
+    return HEDFT_EGRID;
+  if (p->npd != 13) return HEDFT_EINVAL;
+  const int icon = p->icon;
+  if (!(icon == 0 || icon == 8 || icon == 13 || icon == 18 || icon == 20 || icon == 24)) return HEDFT_EINVAL;
+  if (p->precision != HEDFT_PREC_FP64) return HEDFT_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return HEDFT_EHIP;
+  hedft_ctx* c = new hedft_ctx();
+  std::memset((void*)c, 0, sizeof(hedft_ctx));
+  c->g = *g;
+  c->p = *p;
+  if (c->p.lsolid) c->p.core_otc = 0;  // If(Lsolid) core4='OT ' (BCN4HeDFT.f90:654, DFT4He3d.f90:456)
+  c->mode = mode;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return HEDFT_EHIP;
+  }
+  hipEventCreate(&c->ev0);
+  hipEventCreate(&c->ev1);
+  Dims& d = c->d;
+  d.nx = g->nx;
+  d.ny = g->ny;
+  d.nz = g->nz;
+  d.nxh = g->nx / 2 + 1;
+  d.n = (long long)d.nx * d.ny * d.nz;
+  d.nh = (long long)d.nxh * d.ny * d.nz;
+  c->hx = 2.0 * std::fabs(g->xmax) / d.nx;
+  c->hy = 2.0 * std::fabs(g->ymax) / d.ny;
+  c->hz = 2.0 * std::fabs(g->zmax) / d.nz;
+  c->dxyz = c->hx * c->hy * c->hz;
+  c->renor = 1.0 / ((double)d.nx * d.ny * d.nz);
+  const size_t rN = sizeof(double) * d.n, cN = 2 * rN, hN = sizeof(double2) * d.nh, kN = sizeof(double) * d.nh;
+  const size_t pN = (mode == HEDFT_MODE_TDDFT) ? cN : rN;
+  int rc = HEDFT_OK;
+  double** reals[] = {&c->den, &c->pot4, &c->dencg, &c->delj4, &c->denalf, &c->intx, &c->inty, &c->intz,
+                      &c->ualphas, &c->dxden, &c->dyden, &c->dzden, &c->uext, &c->sto1, &c->ualph_conv,
+                      &c->work1, &c->work2};
+  for (double** f : reals) rc = rc ? rc : alloc(c, (void**)f, rN);
+  if (mode == HEDFT_MODE_STATICS) {
+    rc = rc ? rc : alloc(c, (void**)&c->timec_r, rN);
+    rc = rc ? rc : alloc(c, (void**)&c->hpsi_r, rN);
+  }
+  void** psis[] = {&c->psi, &c->psiold[0], &c->psiold[1], &c->psiold[2], &c->hpsiold[0], &c->hpsiold[1],
+                   &c->pc, &c->q, &c->lap};
+  for (void** f : psis) rc = rc ? rc : alloc(c, f, pN);
+  rc = rc ? rc : alloc(c, &c->workc1, cN);
+  double2** specs[] = {&c->fden, &c->spec[0], &c->spec[1], &c->spec[2], &c->spec[3], &c->workk1};
+  for (double2** f : specs) rc = rc ? rc : alloc(c, (void**)f, hN);
+  rc = rc ? rc : alloc(c, (void**)&c->fvlj4, kN);
+  rc = rc ? rc : alloc(c, (void**)&c->wcgk, kN);
+  rc = rc ? rc : alloc(c, (void**)&c->kalfs, kN);
+  // partial sums: 8 regions, each large enough for the largest pencil grid x 8 values
+  size_t maxb = 0;
+  for (int a = 0; a < 3; ++a) {
+    maxb = std::max(maxb, pencil_geom<double2>(d, a).nblocks);
+    maxb = std::max(maxb, pencil_geom<double>(d, a).nblocks);
+  }
+  maxb = std::max(maxb, (size_t)EW_BLOCKS);
+  c->partials_cap = 8 * maxb * 8;
+  rc = rc ? rc : alloc(c, (void**)&c->partials, c->partials_cap * sizeof(double));
+  rc = rc ? rc : alloc(c, (void**)&c->scal, NSCAL * sizeof(double));
+  rc = rc ? rc : alloc(c, (void**)&c->imp, IMP_SIZE * sizeof(double));
+  rc = rc ? rc : alloc(c, (void**)&c->lstop, sizeof(int));
+  rc = rc ? rc : alloc(c, (void**)&c->lamtab, sizeof(double) * (d.nx + d.ny + d.nz));
+  if (rc == HEDFT_OK && hipHostMalloc((void**)&c->host_scal, NSCAL * sizeof(double)) != hipSuccess) rc = HEDFT_ENOMEM;
+  if (rc == HEDFT_OK) rc = fft_setup(c);
+  if (rc != HEDFT_OK) {
+    hedft_destroy(c);
+    return rc;
+  }
+  stencil_setup(c);
+  // spectral kernels (fforma, initcg, kalfs) and absorbing-layer tables
+  std::vector<double> fv, wc, ka;
+  kernel_tables(c, fv, wc, ka);
+  std::vector<double> lam(d.nx + d.ny + d.nz);
+  const double mx[3] = {g->xmax, g->ymax, g->zmax};
+  const double hh[3] = {c->hx, c->hy, c->hz};
+  const int nn[3] = {d.nx, d.ny, d.nz};
+  for (int a = 0, o = 0; a < 3; o += nn[a], ++a)
+    for (int i = 0; i < nn[a]; ++i)
+      lam[o + i] = std::tanh((std::fabs(-mx[a] + hh[a] * i) - c->p.tmean[a]) / c->p.tsurf[a]);
+  if (hipMemcpyAsync(c->fvlj4, fv.data(), kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->wcgk, wc.data(), kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->kalfs, ka.data(), kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->lamtab, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    hedft_destroy(c);
+    return HEDFT_EHIP;
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = c->ioldr[k] = c->ioldv[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = c->iolda[k] = k;
+  *out = c;
+  return HEDFT_OK;
+}
+
+int hedft_destroy(hedft_ctx* c) {
+  if (!c) return HEDFT_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  void* ptrs[] = {c->den, c->pot4, c->dencg, c->delj4, c->denalf, c->intx, c->inty, c->intz, c->ualphas,
+                  c->dxden, c->dyden, c->dzden, c->uext, c->sto1, c->ualph_conv, c->work1, c->work2,
+                  c->timec_r, c->hpsi_r, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
+                  c->hpsiold[1], c->pc, c->q, c->lap, c->workc1, c->fden, c->spec[0], c->spec[1], c->spec[2],
+                  c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->lamtab, c->partials, c->scal,
+                  c->imp, c->lstop, c->potion};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  if (c->host_scal) hipHostFree(c->host_scal);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return HEDFT_OK;
+}
+
+// device pointer, byte size and whether the field is computed on download
+static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* computed) {
+  const size_t rN = sizeof(double) * c->d.n, cN = 2 * rN, hN = sizeof(double2) * c->d.nh,
+               kN = sizeof(double) * c->d.nh;
+  const bool td = (c->mode == HEDFT_MODE_TDDFT);
+  const size_t pN = td ? cN : rN;
+  *computed = 0;
+  switch (f) {
+    case HEDFT_DEN: *ptr = c->den; *bytes = rN; break;
+    case HEDFT_PSI: *ptr = c->psi; *bytes = pN; break;
+    case HEDFT_POT4: *ptr = c->pot4; *bytes = rN; break;
+    case HEDFT_DENCG: *ptr = c->dencg; *bytes = rN; break;
+    case HEDFT_DELJ4: *ptr = c->delj4; *bytes = rN; break;
+    case HEDFT_DENALF: *ptr = c->denalf; *bytes = rN; break;
+    case HEDFT_FALFS: *ptr = c->work2; *bytes = rN; *computed = 1; break;
+    case HEDFT_INTXALF: *ptr = c->intx; *bytes = rN; break;
+    case HEDFT_INTYALF: *ptr = c->inty; *bytes = rN; break;
+    case HEDFT_INTZALF: *ptr = c->intz; *bytes = rN; break;
+    case HEDFT_UALPHAS: *ptr = c->ualphas; *bytes = rN; break;
+    case HEDFT_DXDEN: *ptr = c->dxden; *bytes = rN; break;
+    case HEDFT_DYDEN: *ptr = c->dyden; *bytes = rN; break;
+    case HEDFT_DZDEN: *ptr = c->dzden; *bytes = rN; break;
+    case HEDFT_UIMP: *ptr = c->work2; *bytes = rN; *computed = 1; break;
+    case HEDFT_UEXT: *ptr = c->uext; *bytes = rN; break;
+    case HEDFT_TIMEC:
+      if (td) { *ptr = c->workc1; *bytes = cN; *computed = 1; }
+      else { *ptr = c->timec_r; *bytes = rN; }
+      break;
+    case HEDFT_PSIOLD1: case HEDFT_PSIOLD2: case HEDFT_PSIOLD3:
+      *ptr = c->psiold[f - HEDFT_PSIOLD1]; *bytes = pN; break;
+    case HEDFT_HPSIOLD1: case HEDFT_HPSIOLD2:
+      *ptr = c->hpsiold[f - HEDFT_HPSIOLD1]; *bytes = pN; break;
+    case HEDFT_PC: *ptr = c->pc; *bytes = pN; break;
+    case HEDFT_Q: *ptr = c->q; *bytes = pN; break;
+    case HEDFT_FDEN: *ptr = c->fden; *bytes = hN; break;
+    case HEDFT_FVLJ4: *ptr = c->fvlj4; *bytes = kN; break;
+    case HEDFT_WCGK: *ptr = c->wcgk; *bytes = kN; break;
+    case HEDFT_KALFS: *ptr = c->kalfs; *bytes = kN; break;
+    case HEDFT_WORK1: *ptr = c->work1; *bytes = rN; break;
+    case HEDFT_WORK2: *ptr = c->work2; *bytes = rN; break;
+    case HEDFT_WORKC1: *ptr = c->workc1; *bytes = cN; break;
+    case HEDFT_WORKK1: *ptr = c->workk1; *bytes = hN; break;
+    default: return HEDFT_EINVAL;
+  }
+  return HEDFT_OK;
+}
+
+size_t hedft_field_bytes(const hedft_ctx* c, int32_t field) {
+  void* p;
+  size_t b = 0;
+  int comp;
+  if (field_ptr(const_cast<hedft_ctx*>(c), field, &p, &b, &comp) != HEDFT_OK) return 0;
+  return b;
+}
+
+int hedft_upload(hedft_ctx* c, int32_t field, const void* host, size_t bytes) {
+  if (!c || !host) return HEDFT_EINVAL;
+  void* p;
+  size_t b;
+  int comp;
+  HEDFT_TRY(field_ptr(c, field, &p, &b, &comp));
+  if (comp || bytes != b) return HEDFT_EINVAL;
+  hipSetDevice(c->device);
+  HEDFT_CHECK(hipMemcpyAsync(p, host, b, hipMemcpyHostToDevice, c->stream));
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  c->graph_valid = 0;
+  return HEDFT_OK;
+}
+
+int hedft_download(hedft_ctx* c, int32_t field, void* host, size_t bytes) {
+  if (!c || !host) return HEDFT_EINVAL;
+  void* p;
+  size_t b;
+  int comp;
+  HEDFT_TRY(field_ptr(c, field, &p, &b, &comp));
+  if (bytes != b) return HEDFT_EINVAL;
+  hipSetDevice(c->device);
+  if (comp) {
+    const dim3 gr(1024), bl(256);
+    if (field == HEDFT_UIMP) hipLaunchKernelGGL(k_eval_uimp, gr, bl, 0, c->stream, c->d, make_impdev(c), c->work2);
+    else if (field == HEDFT_FALFS)
+      hipLaunchKernelGGL(k_eval_falfs, gr, bl, 0, c->stream, c->d.n, c->denalf, c->p.den0s, c->work2);
+    else hipLaunchKernelGGL(k_eval_timec, gr, bl, 0, c->stream, c->d, make_lam(c), (double2*)c->workc1);
+  }
+  HEDFT_CHECK(hipMemcpyAsync(host, p, b, hipMemcpyDeviceToHost, c->stream));
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  return HEDFT_OK;
+}
+
+int hedft_synchronize(hedft_ctx* c) {
+  if (!c) return HEDFT_EINVAL;
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  return HEDFT_OK;
+}
+
+static int done(hedft_ctx* c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "hedft: %s\n", hipGetErrorString(e));
+    return HEDFT_EHIP;
+  }
+  (void)c;
+  return HEDFT_OK;
+}
+
+int hedft_derivnD(hedft_ctx* c, int32_t id, int32_t iv, int32_t in, int32_t out) {
+  if (!c || id < 1 || id > 2 || iv < 1 || iv > 3) return HEDFT_EINVAL;
+  void *pi, *po;
+  size_t bi, bo;
+  int ci, co;
+  HEDFT_TRY(field_ptr(c, in, &pi, &bi, &ci));
+  HEDFT_TRY(field_ptr(c, out, &po, &bo, &co));
+  if (ci || co || bi != bo || pi == po) return HEDFT_EINVAL;
+  const size_t rN = sizeof(double) * c->d.n;
+  if (bi != rN && bi != 2 * rN) return HEDFT_EINVAL;
+  HEDFT_TRY(deriv_plain(c, id, iv - 1, pi, po, bi == 2 * rN));
+  return done(c);
+}
+
+int hedft_fftfw(hedft_ctx* c, int32_t in, int32_t out) {
+  void *pi, *po;
+  size_t bi, bo;
+  int ci, co;
+  HEDFT_TRY(field_ptr(c, in, &pi, &bi, &ci));
+  HEDFT_TRY(field_ptr(c, out, &po, &bo, &co));
+  if (ci || co || bi != sizeof(double) * c->d.n || bo != sizeof(double2) * c->d.nh) return HEDFT_EINVAL;
+  HEDFT_TRY(fftfw(c, (const double*)pi, (double2*)po));
+  return done(c);
+}
+
+int hedft_fftbk(hedft_ctx* c, int32_t in, int32_t out) {
+  void *pi, *po;
+  size_t bi, bo;
+  int ci, co;
+  HEDFT_TRY(field_ptr(c, in, &pi, &bi, &ci));
+  HEDFT_TRY(field_ptr(c, out, &po, &bo, &co));
+  if (ci || co || bo != sizeof(double) * c->d.n || bi != sizeof(double2) * c->d.nh) return HEDFT_EINVAL;
+  HEDFT_TRY(fftbk(c, (double2*)pi, (double*)po));
+  return done(c);
+}
+
+int hedft_derden(hedft_ctx* c) {
+  HEDFT_TRY(derden_force(c, 0));
+  return done(c);
+}
+
+int hedft_poten(hedft_ctx* c) {
+  HEDFT_TRY(poten(c, 0));
+  return done(c);
+}
+
+int hedft_term_alfa(hedft_ctx* c) {
+  if (!c->p.core_otc) return HEDFT_OK;
+  // pot4 is re-assembled by alpha_div_pot4 from the stored delj4/dencg/sto1; keep it
+  // untouched for a standalone call by saving and restoring it
+  HEDFT_CHECK(hipMemcpyAsync(c->work2, c->pot4, sizeof(double) * c->d.n, hipMemcpyDeviceToDevice, c->stream));
+  HEDFT_TRY(alpha_term(c));
+  HEDFT_CHECK(hipMemcpyAsync(c->pot4, c->work2, sizeof(double) * c->d.n, hipMemcpyDeviceToDevice, c->stream));
+  return done(c);
+}
+
+int hedft_energy(hedft_ctx* c, hedft_energies* e) {
+  if (!c || !e) return HEDFT_EINVAL;
+  double s[8];
+  if (c->mode == HEDFT_MODE_TDDFT) HEDFT_TRY(energy_sums<double2>(c, s));
+  else HEDFT_TRY(energy_sums<double>(c, s));
+  const hedft_params& p = c->p;
+  const double dv = c->dxyz;
+  std::memset(e, 0, sizeof *e);
+  e->ekin4 = p.h2o2m4 * s[0] * dv;
+  e->elj4 = 0.5 * s[1] * dv;
+  e->ecor4 = s[2] * dv;
+  e->ealphas = p.core_otc ? -p.h2o2m4 * 0.5 * p.alphas * s[3] * dv : 0.0;
+  e->esolid = p.lsolid ? p.csolid * s[4] * dv : 0.0;
+  e->etot4 = e->ekin4 + e->elj4 + e->ecor4 + e->esolid + e->ealphas;
+  if (c->mode == HEDFT_MODE_TDDFT) {
+    e->eext = s[5] * dv;
+    e->etot4 += e->eext;
+    double rv[12];
+    HEDFT_CHECK(hipMemcpy(rv, c->imp + IMP_V, sizeof rv, hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->nimp; ++k) {
+      double v2 = 0.0;
+      for (int a = 0; a < 3; ++a) v2 += rv[k + c->nimp * a] * rv[k + c->nimp * a];
+      e->ekinx += 0.5 * c->mass[k] * v2;
+    }
+    e->eHeX = s[6] * dv;
+    e->eimpu = e->ekinx + e->eHeX;
+  } else {
+    e->eext = p.lexternal ? s[5] * dv : 0.0;
+    e->eimpu = e->eext;
+  }
+  e->etot = e->etot4 + e->eimpu;
+  e->norm = s[7] * dv;
+  return HEDFT_OK;
+}
+
+int hedft_steprkr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  HEDFT_TRY(steprkr_impl(c, deltat, n4, nullptr));
+  double h;
+  HEDFT_TRY(read_scal(c, S_HMEAN, 1, &h));
+  if (mu4err) *mu4err = std::fabs(1.0 - (mu4 ? *mu4 : 0.0) / h);
+  if (mu4) *mu4 = h;
+  return done(c);
+}
+
+int hedft_steppcr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4, double* errpc) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  HEDFT_TRY(steppcr_impl(c, deltat, n4));
+  double s[4];
+  HEDFT_TRY(read_scal(c, 0, 4, s));
+  if (mu4err) *mu4err = std::fabs(1.0 - (mu4 ? *mu4 : 0.0) / s[S_HMEAN]);
+  if (mu4) *mu4 = s[S_HMEAN];
+  if (errpc) *errpc = (9.0 / 121.0) * s[S_ERRPC] / (double)c->d.n;
+  return done(c);
+}
+
+int hedft_steprk(hedft_ctx* c, double deltat) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  HEDFT_TRY(steprk_impl(c, deltat));
+  return done(c);
+}
+
+static int read_pc_errors(hedft_ctx* c, double* errHe, double* errimp, double* errvimp) {
+  if (!errHe && !errimp && !errvimp) return HEDFT_OK;
+  double s[8];
+  HEDFT_TRY(read_scal(c, 0, 8, s));
+  if (errHe) *errHe = (9.0 / 121.0) * s[S_ERRPC] / (double)c->d.n;
+  if (errimp) *errimp = c->nimp ? s[S_ERRIMP] : 0.0;
+  if (errvimp) *errvimp = c->nimp ? s[S_ERRVIMP] : 0.0;
+  return HEDFT_OK;
+}
+
+int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, double* errvimp) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  HEDFT_TRY(steppc_impl(c, deltat));
+  HEDFT_TRY(read_pc_errors(c, errHe, errimp, errvimp));
+  return done(c);
+}
+
+int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delinter, double rmaxinterpol,
+                         const double* potion, const double* mass) {
+  if (!c || nimp < 0 || nimp > HEDFT_MAX_IMP || (nimp > 0 && (!potion || !mass || npot < 2))) return HEDFT_EINVAL;
+  if (c->potion) hipFree(c->potion);
+  c->potion = nullptr;
+  c->nimp = nimp;
+  c->npot = npot;
+  c->delinter = delinter;
+  c->rmaxinterpol = rmaxinterpol;
+  if (nimp == 0) return HEDFT_OK;
+  HEDFT_CHECK(hipMalloc(&c->potion, sizeof(double) * (size_t)nimp * npot));
+  HEDFT_CHECK(hipMemcpy(c->potion, potion, sizeof(double) * (size_t)nimp * npot, hipMemcpyHostToDevice));
+  for (int k = 0; k < nimp; ++k) c->mass[k] = mass[k];
+  double m[HEDFT_MAX_IMP] = {0};
+  for (int k = 0; k < nimp; ++k) m[k] = mass[k];
+  HEDFT_CHECK(hipMemcpy(c->imp + IMP_MASS, m, sizeof m, hipMemcpyHostToDevice));
+  c->graph_valid = 0;
+  return HEDFT_OK;
+}
+
+int hedft_set_impurity_state(hedft_ctx* c, const double* rimp, const double* vimp, const double* aimp) {
+  if (!c) return HEDFT_EINVAL;
+  const size_t b = sizeof(double) * 3 * c->nimp;
+  if (rimp) HEDFT_CHECK(hipMemcpy(c->imp + IMP_R, rimp, b, hipMemcpyHostToDevice));
+  if (vimp) HEDFT_CHECK(hipMemcpy(c->imp + IMP_V, vimp, b, hipMemcpyHostToDevice));
+  if (aimp) HEDFT_CHECK(hipMemcpy(c->imp + IMP_A, aimp, b, hipMemcpyHostToDevice));
+  return HEDFT_OK;
+}
+
+int hedft_get_impurity_state(hedft_ctx* c, double* rimp, double* vimp, double* aimp, double* force,
+                             int32_t* lstopimp) {
+  if (!c) return HEDFT_EINVAL;
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  const size_t b = sizeof(double) * 3 * c->nimp;
+  if (rimp) HEDFT_CHECK(hipMemcpy(rimp, c->imp + IMP_R, b, hipMemcpyDeviceToHost));
+  if (vimp) HEDFT_CHECK(hipMemcpy(vimp, c->imp + IMP_V, b, hipMemcpyDeviceToHost));
+  if (aimp) HEDFT_CHECK(hipMemcpy(aimp, c->imp + IMP_A, b, hipMemcpyDeviceToHost));
+  if (force) HEDFT_CHECK(hipMemcpy(force, c->imp + IMP_F, b, hipMemcpyDeviceToHost));
+  if (lstopimp) {
+    int v = 0;
+    HEDFT_CHECK(hipMemcpy(&v, c->lstop, sizeof v, hipMemcpyDeviceToHost));
+    *lstopimp = v;
+  }
+  return HEDFT_OK;
+}
+
+__global__ void k_check_bounds(ImpDev im, Dims d) {
+  // the farthest grid point from each impurity is a box corner
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < im.nimp; ++k)
+    for (int cz_ = 0; cz_ < 2; ++cz_)
+      for (int cy = 0; cy < 2; ++cy)
+        for (int cx = 0; cx < 2; ++cx) {
+          const double xx = im.x0 + im.hx * (cx ? d.nx - 1 : 0), yy = im.y0 + im.hy * (cy ? d.ny - 1 : 0),
+                       zz = im.z0 + im.hz * (cz_ ? d.nz - 1 : 0);
+          imp_u_k(im, k, xx, yy, zz, true);
+        }
+}
+
+int hedft_potenimp(hedft_ctx* c) {
+  if (!c) return HEDFT_EINVAL;
+  if (c->nimp > 0) hipLaunchKernelGGL(k_check_bounds, dim3(1), dim3(64), 0, c->stream, make_impdev(c), c->d);
+  return done(c);
+}
+
+struct OutForce {  // standalone forceimp from the stored gradient (potenimp.f90:49-52)
+  static constexpr int NR = HEDFT_MAX_IMP;
+  double* partials;
+  const double* g;
+  ImpDev im;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double, double* red) const {
+    const double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
+    const double v = g[i];
+    for (int k = 0; k < im.nimp; ++k) red[k] += v * imp_u_k(im, k, xx, yy, zz, false);
+  }
+};
+
+int hedft_forceimp(hedft_ctx* c) {
+  if (!c) return HEDFT_EINVAL;
+  if (c->nimp == 0) return HEDFT_OK;
+  // reuse the pencil geometry so that force_finalize sees the same partial counts
+  const double* gs[3] = {c->dxden, c->dyden, c->dzden};
+  ImpDev im = make_impdev(c);
+  for (int a = 0; a < 3; ++a) {
+    OutForce o{force_partials(c, a), gs[a], im};
+    if (a == 0) launch_pencil<double, 0>(c->stream, c->d, c->coef[0][0], InPlain<double>{gs[a]}, o);
+    else if (a == 1) launch_pencil<double, 1>(c->stream, c->d, c->coef[0][1], InPlain<double>{gs[a]}, o);
+    else launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], InPlain<double>{gs[a]}, o);
+  }
+  g_update_aimp = 0;
+  HEDFT_TRY(force_finalize(c));
+  return done(c);
+}
+
+int hedft_update_aimp(hedft_ctx* c) {
+  if (!c) return HEDFT_EINVAL;
+  if (c->nimp > 0) hipLaunchKernelGGL(k_imp_aimp, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp);
+  return done(c);
+}
+
+int hedft_statics_init(hedft_ctx* c, double n4, double* mu4) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  hipLaunchKernelGGL(k_sum1, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den, gen_partials(c));
+  finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+  hipLaunchKernelGGL(k_statics_psi0, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den, rz(c->psi), c->scal,
+                     n4, c->dxyz);
+  HEDFT_TRY(statics_refresh(c));
+  HEDFT_TRY(poten_statics(c));
+  HEDFT_TRY(hpsi_statics(c, n4));
+  double m;
+  HEDFT_TRY(read_scal(c, S_MU, 1, &m));
+  if (mu4) *mu4 = m;
+  c->rk_done = 0;
+  return done(c);
+}
+
+int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double n4, double* mu4, double* mu4err) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  double errpc;
+  if (iterrk <= 4) HEDFT_TRY(hedft_steprkr(c, deltat, mu4, mu4err, n4));
+  else HEDFT_TRY(hedft_steppcr(c, deltat, mu4, mu4err, n4, &errpc));
+  HEDFT_TRY(statics_refresh(c));
+  HEDFT_TRY(poten_statics(c));
+  return done(c);
+}
+
+int hedft_tddft_init(hedft_ctx* c) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  hipLaunchKernelGGL(k_tddft_psi0, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den, (double2*)c->psi);
+  // qr = qv = rimpold = vimpold = aimpold = 0 (DFT4He3d.f90:625-629)
+  HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_QR, 0, sizeof(double) * 24, c->stream));
+  HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_ROLD, 0, sizeof(double) * (IMP_MASS - IMP_ROLD), c->stream));
+  HEDFT_TRY(hedft_potenimp(c));
+  HEDFT_TRY(tddft_after_step(c));
+  c->rk_done = 0;
+  return done(c);
+}
+
+int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk, double* errHe, double* errimp,
+                          double* errvimp) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  if (iter <= nrk) {
+    HEDFT_TRY(steprk_impl(c, deltat));
+    if (errHe) *errHe = 0.0;
+    if (errimp) *errimp = 0.0;
+    if (errvimp) *errvimp = 0.0;
+    HEDFT_TRY(tddft_after_step(c));
+  } else {
+    if (!c->rk_done) return HEDFT_ESTATE;
+    HEDFT_TRY(steppc_impl(c, deltat));
+    HEDFT_TRY(tddft_after_step(c));
+    HEDFT_TRY(read_pc_errors(c, errHe, errimp, errvimp));
+  }
+  return done(c);
+}
+
+// One PC iteration per slot phase; the slot rotation has period 6, so six
+// consecutive iterations are captured into one graph and replayed.
+int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph, double* elapsed_ms) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT || nsteps < 0) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  hipSetDevice(c->device);
+  const int period = 6;
+  if (use_graph && (!c->graph_valid || c->graph_dt != deltat)) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    int saved[13];
+    std::memcpy(saved, c->ioldp, sizeof(int) * 3);
+    HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = HEDFT_OK;
+    for (int s = 0; s < period && rc == HEDFT_OK; ++s) {
+      rc = steppc_impl(c, deltat);
+      if (rc == HEDFT_OK) rc = tddft_after_step(c);
+    }
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (rc != HEDFT_OK) return rc;
+    if (e != hipSuccess) return HEDFT_EHIP;
+    c->graph = gr;
+    HEDFT_CHECK(hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0));
+    c->graph_valid = 1;
+    c->graph_dt = deltat;
+    (void)saved;
+    // capture advanced the host-side slot indices by one full period: back to the start
+  }
+  HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
+  int done_steps = 0;
+  if (use_graph) {
+    for (; done_steps + period <= nsteps; done_steps += period) HEDFT_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  }
+  for (; done_steps < nsteps; ++done_steps) {
+    HEDFT_TRY(steppc_impl(c, deltat));
+    HEDFT_TRY(tddft_after_step(c));
+  }
+  HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
+  HEDFT_CHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if (elapsed_ms) *elapsed_ms = ms;
+  return done(c);
+}
+
+int hedft_last_elapsed(hedft_ctx* c, double* ms) {
+  if (!c || !ms) return HEDFT_EINVAL;
+  *ms = c->last_ms;
+  return HEDFT_OK;
+}
+
+}  // extern "C"

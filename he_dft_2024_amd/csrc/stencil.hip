@@ -1,0 +1,235 @@
+// stencil.hip — derivnD coefficient set-up and the non-step stencil passes.
+//
+// Coefficients: the Bickley 13-point matrices that newder.f returns with
+// ICON=1 (rows = differentiation node, table mates.f:172), generated here with
+// Fornberg's recurrence (Math. Comp. 51 (1988) 699) in long double; the
+// boundary rows are then rewritten for the boundary mode icon as redef.f:1-324
+// does (13: zero outside the box; 8/18: (anti)periodic wrap; 24 = 20 + 4:
+// even reflection about both end points; 0: one-sided rows) and divided by
+// h^id (init_deriv_parallel.f90:399-414).
+#include <cmath>
+#include <cstring>
+
+#include "stencil_kernel.h"
+
+namespace hedft {
+
+static void bickley_matrix(int id, double cd[13][13]) {
+  for (int i = 0; i < 13; ++i) {
+    long double w[13][3];
+    std::memset(w, 0, sizeof w);
+    const long double z = (long double)i;
+    long double c1 = 1.0L, c4 = 0.0L - z;
+    w[0][0] = 1.0L;
+    for (int a = 1; a < 13; ++a) {
+      const int mn = a < id ? a : id;
+      long double c2 = 1.0L, c5 = c4;
+      c4 = (long double)a - z;
+      for (int b = 0; b < a; ++b) {
+        const long double c3 = (long double)(a - b);
+        c2 *= c3;
+        if (b == a - 1) {
+          for (int k = mn; k >= 1; --k) w[a][k] = c1 * (k * w[a - 1][k - 1] - c5 * w[a - 1][k]) / c2;
+          w[a][0] = -c1 * c5 * w[a - 1][0] / c2;
+        }
+        for (int k = mn; k >= 1; --k) w[b][k] = (c4 * w[b][k] - k * w[b][k - 1]) / c3;
+        w[b][0] = c4 * w[b][0] / c3;
+      }
+      c1 = c2;
+    }
+    for (int j = 0; j < 13; ++j) cd[i][j] = (double)w[j][id];
+  }
+}
+
+// Boundary-row rewrite for one mode step (the cases of redef.f on this path).
+static void redef_rows(double cd[13][13], int icon) {
+  const int K = 13, ko2 = 6;
+  while (icon != 0) {
+    int next = 0;
+    if (icon == 4 || icon == 5) {  // even / odd reflection about the first point
+      const int ip = (icon == 4) ? 1 : -1;
+      double c0[13];
+      for (int j = 0; j < K; ++j) c0[j] = cd[ko2][j];
+      for (int i = 1; i <= ko2; ++i) {
+        int j0 = ko2 + 2 - i, j1 = j0;
+        cd[i - 1][0] = c0[j0 - 1];
+        for (int j = 2; j <= K; ++j) {
+          ++j0;
+          --j1;
+          cd[i - 1][j - 1] = (j0 <= K) ? ((j1 >= 1) ? c0[j0 - 1] + ip * c0[j1 - 1] : c0[j0 - 1]) : 0.0;
+        }
+      }
+    } else if (icon == 12) {  // zero beyond the last point
+      for (int i = ko2 + 2; i <= K; ++i) {
+        for (int j = K; j >= 2; --j) cd[i - 1][j - 1] = cd[i - 2][j - 2];
+        cd[i - 1][0] = 0.0;
+      }
+    } else if (icon == 13) {  // zero before the first point, then 12
+      for (int i = ko2; i >= 1; --i) {
+        for (int j = 1; j <= K - 1; ++j) cd[i - 1][j - 1] = cd[i][j];
+        cd[i - 1][K - 1] = 0.0;
+      }
+      next = 12;
+    } else if (icon == 20 || icon == 24 || icon == 25) {  // even reflection about the last point
+      double c0[13];
+      for (int j = 0; j < K; ++j) c0[j] = cd[ko2][j];
+      for (int i = K; i >= ko2 + 2; --i) {
+        int j0 = ko2 + 1 - i + K, j1 = j0;
+        cd[i - 1][K - 1] = c0[j0 - 1];
+        for (int j = K - 1; j >= 1; --j) {
+          --j0;
+          ++j1;
+          cd[i - 1][j - 1] = (j0 >= 1) ? ((j1 <= K) ? c0[j0 - 1] + c0[j1 - 1] : c0[j0 - 1]) : 0.0;
+        }
+      }
+      next = (icon == 24) ? 4 : (icon == 25) ? 5 : 0;
+    }
+    icon = next;
+  }
+}
+
+static StencilCoef make_coef(int id, int icon, double h) {
+  StencilCoef cf;
+  double cd[13][13];
+  bickley_matrix(id, cd);
+  if (icon != 0 && icon != 8 && icon != 18) redef_rows(cd, icon);
+  const double sto = 1.0 / std::pow(h, id);
+  for (int j = 0; j < 13; ++j) cf.c[j] = cd[6][j] * sto;
+  for (int r = 0; r < 6; ++r)
+    for (int j = 0; j < 13; ++j) {
+      cf.rows[r][j] = cd[r][j] * sto;
+      cf.rows[6 + r][j] = cd[7 + r][j] * sto;
+    }
+  cf.mode = (icon == 8) ? 1 : (icon == 18) ? 2 : 0;
+  return cf;
+}
+
+void stencil_setup(hedft_ctx* c) {
+  const double h[3] = {c->hx, c->hy, c->hz};
+  for (int id = 1; id <= 2; ++id)
+    for (int a = 0; a < 3; ++a) c->coef[id - 1][a] = make_coef(id, c->p.icon, h[a]);
+}
+
+template <typename V>
+static int deriv_plain_t(hedft_ctx* c, int id, int axis, const V* in, V* out) {
+  const StencilCoef& cf = c->coef[id - 1][axis];
+  InPlain<V> ip{in};
+  OutStore<V> op{out, nullptr};
+  switch (axis) {
+    case 0: launch_pencil<V, 0>(c->stream, c->d, cf, ip, op); break;
+    case 1: launch_pencil<V, 1>(c->stream, c->d, cf, ip, op); break;
+    default: launch_pencil<V, 2>(c->stream, c->d, cf, ip, op); break;
+  }
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int is_complex) {
+  if (c->d.nx < 13 || c->d.ny < 13 || c->d.nz < 13) return HEDFT_EGRID;
+  if (is_complex)
+    return deriv_plain_t<double2>(c, id, axis, (const double2*)in, (double2*)out);
+  return deriv_plain_t<double>(c, id, axis, (const double*)in, (double*)out);
+}
+
+// grad(den) with the He part of the impurity force F_k = -sum(d_a den * U_k) dxyz
+// (potenimp.f90:49-52) accumulated per workgroup in the same pass.
+struct OutGradForce {
+  static constexpr int NR = HEDFT_MAX_IMP;
+  double* out;
+  double* partials;
+  ImpDev im;
+  int want;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double v, double* red) const {
+    out[i] = v;
+    if (want) {
+      const double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
+      for (int k = 0; k < im.nimp; ++k) red[k] += v * imp_u_k(im, k, xx, yy, zz, false);
+    }
+  }
+};
+
+ImpDev make_impdev(hedft_ctx* c);
+double* force_partials(hedft_ctx* c, int axis);
+int force_finalize(hedft_ctx* c);
+
+int derden_force(hedft_ctx* c, int want_force) {
+  want_force = want_force && c->nimp > 0;
+  ImpDev im = make_impdev(c);
+  InPlain<double> ip{c->den};
+  double* outs[3] = {c->dxden, c->dyden, c->dzden};
+  for (int a = 0; a < 3; ++a) {
+    OutGradForce op{outs[a], force_partials(c, a), im, want_force};
+    const StencilCoef& cf = c->coef[0][a];
+    if (a == 0) launch_pencil<double, 0>(c->stream, c->d, cf, ip, op);
+    else if (a == 1) launch_pencil<double, 1>(c->stream, c->d, cf, ip, op);
+    else launch_pencil<double, 2>(c->stream, c->d, cf, ip, op);
+  }
+  if (hipGetLastError() != hipSuccess) return HEDFT_EHIP;
+  return want_force ? force_finalize(c) : HEDFT_OK;
+}
+
+// z pass of the alpha_s divergence with the mean-field assembly:
+// ualphas = (conv/den0s + sum_a d_a(f int_a)) h2o2m4 alphas   (term_alfa.f90:80-98)
+// pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ uext] + ualphas
+// (He_statics/poten.f90:277-296, Dynamics_TDDFT/poten.f90:194-211)
+struct OutDivPot4 {
+  static constexpr int NR = 0;
+  double* partials;
+  const double *acc, *conv, *delj4, *dencg, *corr, *uext;
+  double *ualphas, *pot4;
+  double den0s, halpha, a0, a1;
+  int lext;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double v, double*) const {
+    const double ua = (conv[i] / den0s + (acc[i] + v)) * halpha;
+    ualphas[i] = ua;
+    const double cg = dencg[i];
+    double p = delj4[i] + cg * cg * (a0 + a1 * cg) + corr[i];
+    if (lext) p += uext[i];
+    pot4[i] = p + ua;
+  }
+};
+
+int alpha_div_pot4(hedft_ctx* c) {
+  const double den0s = c->p.den0s;
+  InFInt ix{c->denalf, c->intx, den0s}, iy{c->denalf, c->inty, den0s}, iz{c->denalf, c->intz, den0s};
+  launch_pencil<double, 0>(c->stream, c->d, c->coef[0][0], ix, OutStore<double>{c->work1, nullptr});
+  launch_pencil<double, 1>(c->stream, c->d, c->coef[0][1], iy,
+                           OutAcc<double>{c->work1, c->work1, nullptr});
+  OutDivPot4 op;
+  op.partials = nullptr;
+  op.acc = c->work1;
+  op.conv = c->ualph_conv;
+  op.delj4 = c->delj4;
+  op.dencg = c->dencg;
+  op.corr = c->sto1;
+  op.uext = c->uext;
+  op.ualphas = c->ualphas;
+  op.pot4 = c->pot4;
+  op.den0s = den0s;
+  op.halpha = c->p.h2o2m4 * c->p.alphas;
+  op.a0 = c->p.cp4 / 2.0;
+  op.a1 = c->p.cpp4 / 3.0;
+  op.lext = (c->mode == HEDFT_MODE_STATICS) && c->p.lexternal;
+  launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], iz, op);
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+int lap_xy(hedft_ctx* c, const void* psi, int is_complex) {
+  if (is_complex) {
+    const double2* p = (const double2*)psi;
+    double2* l = (double2*)c->lap;
+    launch_pencil<double2, 0>(c->stream, c->d, c->coef[1][0], InPlain<double2>{p},
+                              OutStore<double2>{l, nullptr});
+    launch_pencil<double2, 1>(c->stream, c->d, c->coef[1][1], InPlain<double2>{p},
+                              OutAcc<double2>{l, l, nullptr});
+  } else {
+    const double* p = (const double*)psi;
+    double* l = (double*)c->lap;
+    launch_pencil<double, 0>(c->stream, c->d, c->coef[1][0], InPlain<double>{p},
+                             OutStore<double>{l, nullptr});
+    launch_pencil<double, 1>(c->stream, c->d, c->coef[1][1], InPlain<double>{p},
+                             OutAcc<double>{l, l, nullptr});
+  }
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+}  // namespace hedft

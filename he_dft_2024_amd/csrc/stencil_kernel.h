@@ -1,0 +1,190 @@
+// stencil_kernel.h — 13-point finite-difference pencil kernel (derivnD on gfx950).
+//
+// One workgroup stages whole pencils of the derivative axis in LDS (x rows, or
+// tiles of TX consecutive x columns along y or z, loaded with coalesced
+// x-contiguous reads), applies the Bickley stencil of derivnD
+// (init_deriv_parallel.f90:373-648 real, :933-1209 complex) with the boundary
+// rows that redef.f builds for the chosen icon, and hands each derivative value
+// to an epilogue functor that stores it or fuses the consumer's arithmetic.
+// Input functors fuse the producer's arithmetic into the LDS fill.
+#pragma once
+#include "common.h"
+
+namespace hedft {
+
+template <typename V>
+__device__ __forceinline__ V vzero();
+template <>
+__device__ __forceinline__ double vzero<double>() { return 0.0; }
+template <>
+__device__ __forceinline__ double2 vzero<double2>() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double vfma(double c, double v, double acc) { return fma(c, v, acc); }
+__device__ __forceinline__ double2 vfma(double c, double2 v, double2 acc) {
+  return make_double2(fma(c, v.x, acc.x), fma(c, v.y, acc.y));
+}
+__device__ __forceinline__ double vmul(double c, double v) { return c * v; }
+__device__ __forceinline__ double2 vmul(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
+
+// Derivative at position i of a pencil P (element stride s) of length n.
+template <typename V>
+__device__ __forceinline__ V stencil_at(const StencilCoef& cf, const V* P, int s, int i, int n) {
+  V acc = vzero<V>();
+  if (cf.mode != 0) {
+    const double sg = (cf.mode == 2) ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < 13; ++j) {
+      int ii = i + j - 6;
+      double c = cf.c[j];
+      if (ii < 0) { ii += n; c *= sg; }
+      else if (ii >= n) { ii -= n; c *= sg; }
+      acc = vfma(c, P[ii * s], acc);
+    }
+  } else if (i < 6) {
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.rows[i][j], P[j * s], acc);
+  } else if (i >= n - 6) {
+    const int r = i - n + 12;
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.rows[r][j], P[(n - 13 + j) * s], acc);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.c[j], P[(i - 6 + j) * s], acc);
+  }
+  return acc;
+}
+
+// Generic pencil kernel.  AX: 0 = x, 1 = y, 2 = z.  In::load(idx) -> V,
+// Out::store(idx, ix, iy, iz, d, red) with Out::NR per-thread partial sums
+// reduced per workgroup into Out::partials.
+template <typename V, int AX, class In, class Out>
+__global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, Dims d, int tx) {
+  extern __shared__ unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
+  constexpr int NR = Out::NR;
+  double red[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int r = 0; r < (NR > 0 ? NR : 1); ++r) red[r] = 0.0;
+  const int n = (AX == 0) ? d.nx : (AX == 1) ? d.ny : d.nz;
+  if (AX == 0) {
+    const int rows = tx;
+    const int row0 = blockIdx.x * rows;
+    const int nrows = d.ny * d.nz;
+    for (int e = threadIdx.x; e < rows * n; e += blockDim.x) {
+      const int r = e / n, i = e - r * n;
+      const int row = row0 + r;
+      lds[e] = (row < nrows) ? in.load((size_t)row * n + i) : vzero<V>();
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * n; e += blockDim.x) {
+      const int r = e / n, i = e - r * n;
+      const int row = row0 + r;
+      if (row < nrows) {
+        const V v = stencil_at<V>(cf, lds + r * n, 1, i, n);
+        out.store((size_t)row * n + i, i, row % d.ny, row / d.ny, v, red);
+      }
+    }
+  } else {
+    const int x0 = blockIdx.x * tx;
+    size_t base, stride;
+    if (AX == 1) {
+      base = (size_t)blockIdx.y * d.nx * d.ny + x0;
+      stride = d.nx;
+    } else {
+      base = (size_t)blockIdx.y * d.nx + x0;
+      stride = (size_t)d.nx * d.ny;
+    }
+    for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
+      const int xl = e % tx, j = e / tx;
+      lds[e] = (x0 + xl < d.nx) ? in.load(base + j * stride + xl) : vzero<V>();
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
+      const int xl = e % tx, i = e / tx;
+      if (x0 + xl < d.nx) {
+        const V v = stencil_at<V>(cf, lds + xl, tx, i, n);
+        const int ix = x0 + xl;
+        const int iy = (AX == 1) ? i : (int)blockIdx.y;
+        const int iz = (AX == 1) ? (int)blockIdx.y : i;
+        out.store(base + i * stride + xl, ix, iy, iz, v, red);
+      }
+    }
+  }
+  if constexpr (NR > 0) block_reduce_store<NR>(red, out.partials);
+}
+
+struct PencilGeom {
+  dim3 grid, block;
+  size_t lds;
+  int tx;
+  size_t nblocks;
+};
+
+template <typename V>
+inline PencilGeom pencil_geom(const Dims& d, int ax) {
+  PencilGeom g;
+  g.block = dim3(256);
+  if (ax == 0) {
+    int rows = 256 / d.nx;
+    if (rows < 1) rows = 1;
+    g.tx = rows;
+    const int nrows = d.ny * d.nz;
+    g.grid = dim3((nrows + rows - 1) / rows);
+    g.lds = sizeof(V) * (size_t)rows * d.nx;
+  } else {
+    const int n = (ax == 1) ? d.ny : d.nz;
+    int tx = (sizeof(V) == 8) ? 32 : 16;
+    while (tx > 4 && sizeof(V) * (size_t)tx * n > 48 * 1024) tx /= 2;  // LDS per workgroup
+    if (tx > d.nx) tx = d.nx;
+    g.tx = tx;
+    g.grid = dim3((d.nx + tx - 1) / tx, (ax == 1) ? d.nz : d.ny);
+    g.lds = sizeof(V) * (size_t)tx * n;
+  }
+  g.nblocks = (size_t)g.grid.x * g.grid.y * g.grid.z;
+  return g;
+}
+
+template <typename V, int AX, class In, class Out>
+inline void launch_pencil(hipStream_t s, const Dims& d, const StencilCoef& cf, const In& in,
+                          const Out& out) {
+  PencilGeom g = pencil_geom<V>(d, AX);
+  hipLaunchKernelGGL((k_pencil<V, AX, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
+}
+
+// ---- common functors ---------------------------------------------------------
+template <typename V>
+struct InPlain {
+  const V* a;
+  __device__ __forceinline__ V load(size_t i) const { return a[i]; }
+};
+// (1 - denalf/den0s) * int_a   (term_alfa.f90:80-85)
+struct InFInt {
+  const double *denalf, *ia;
+  double den0s;
+  __device__ __forceinline__ double load(size_t i) const { return (1.0 - denalf[i] / den0s) * ia[i]; }
+};
+template <typename V>
+struct OutStore {
+  static constexpr int NR = 0;
+  V* out;
+  double* partials;
+  __device__ __forceinline__ void store(size_t i, int, int, int, V v, double*) const { out[i] = v; }
+};
+template <typename V>
+struct OutAcc {  // out = acc + d
+  static constexpr int NR = 0;
+  V* out;
+  const V* acc;
+  double* partials;
+  __device__ __forceinline__ void store(size_t i, int, int, int, V v, double*) const;
+};
+template <>
+__device__ __forceinline__ void OutAcc<double>::store(size_t i, int, int, int, double v, double*) const {
+  out[i] = acc[i] + v;
+}
+template <>
+__device__ __forceinline__ void OutAcc<double2>::store(size_t i, int, int, int, double2 v, double*) const {
+  const double2 a = acc[i];
+  out[i] = make_double2(a.x + v.x, a.y + v.y);
+}
+
+}  // namespace hedft

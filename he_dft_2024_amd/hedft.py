@@ -1,0 +1,279 @@
+"""ctypes binding of libhedft.so (the C-ABI declared in include/hedft.h).
+
+Thin plumbing for tests, the smoke check and bench.py: every call goes to the
+HIP library.  There is no CPU path: if the library is missing the import of
+`lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libhedft.so")
+
+MODE_STATICS, MODE_TDDFT = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+MAX_IMP = 4
+
+FIELDS = [
+    "DEN", "PSI", "POT4", "DENCG", "DELJ4", "DENALF", "FALFS", "INTXALF", "INTYALF", "INTZALF",
+    "UALPHAS", "DXDEN", "DYDEN", "DZDEN", "UIMP", "UEXT", "TIMEC", "PSIOLD1", "PSIOLD2",
+    "PSIOLD3", "HPSIOLD1", "HPSIOLD2", "PC", "Q", "FDEN", "FVLJ4", "WCGK", "KALFS", "WORK1",
+    "WORK2", "WORKC1", "WORKK1",
+]
+FIELD = {name: i for i, name in enumerate(FIELDS)}
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad_", C.c_int32),
+                ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("h2o2m4", C.c_double), ("cp4", C.c_double), ("cpp4", C.c_double),
+        ("alphas", C.c_double), ("l", C.c_double), ("den0s", C.c_double),
+        ("csolid", C.c_double), ("beta_solid", C.c_double), ("den_m", C.c_double),
+        ("h4", C.c_double), ("eps4", C.c_double), ("sigma4", C.c_double),
+        ("core_otc", C.c_int32), ("lsolid", C.c_int32), ("npd", C.c_int32), ("icon", C.c_int32),
+        ("dtmin", C.c_double), ("sdmu", C.c_double), ("dmumax", C.c_double), ("psimin", C.c_double),
+        ("lambdah", C.c_double), ("tmean", C.c_double * 3), ("tsurf", C.c_double * 3),
+        ("precision", C.c_int32), ("lexternal", C.c_int32),
+    ]
+
+
+class Energies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "ekin4", "elj4", "ecor4", "ealphas", "esolid", "eext", "etot4", "ekinx", "eHeX",
+        "eimpu_impu", "eimpu", "etot", "norm")]
+
+
+class HedftError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libhedft.so; raises if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HedftError(f"HIP library not built: {LIB_PATH} (python -m he_dft_2024_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, d, dp = C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_double)
+        L.hedft_strerror.restype = C.c_char_p
+        L.hedft_strerror.argtypes = [C.c_int]
+        L.hedft_default_params.argtypes = [C.POINTER(Params)]
+        L.hedft_grid_supported.argtypes = [ip]
+        L.hedft_create.argtypes = [C.POINTER(Grid), C.POINTER(Params), ip, ip, C.POINTER(vp)]
+        L.hedft_destroy.argtypes = [vp]
+        L.hedft_upload.argtypes = [vp, ip, vp, C.c_size_t]
+        L.hedft_download.argtypes = [vp, ip, vp, C.c_size_t]
+        L.hedft_field_bytes.restype = C.c_size_t
+        L.hedft_field_bytes.argtypes = [vp, ip]
+        L.hedft_synchronize.argtypes = [vp]
+        L.hedft_derivnD.argtypes = [vp, ip, ip, ip, ip]
+        L.hedft_fftfw.argtypes = [vp, ip, ip]
+        L.hedft_fftbk.argtypes = [vp, ip, ip]
+        for n in ("hedft_derden", "hedft_poten", "hedft_term_alfa", "hedft_potenimp",
+                  "hedft_forceimp", "hedft_update_aimp", "hedft_tddft_init"):
+            getattr(L, n).argtypes = [vp]
+        L.hedft_energy.argtypes = [vp, C.POINTER(Energies)]
+        L.hedft_steprkr.argtypes = [vp, d, dp, dp, d]
+        L.hedft_steppcr.argtypes = [vp, d, dp, dp, d, dp]
+        L.hedft_steprk.argtypes = [vp, d]
+        L.hedft_steppc.argtypes = [vp, d, dp, dp, dp]
+        L.hedft_set_impurities.argtypes = [vp, ip, ip, d, d, vp, vp]
+        L.hedft_set_impurity_state.argtypes = [vp, vp, vp, vp]
+        L.hedft_get_impurity_state.argtypes = [vp, vp, vp, vp, vp, C.POINTER(ip)]
+        L.hedft_statics_init.argtypes = [vp, d, dp]
+        L.hedft_statics_iteration.argtypes = [vp, d, ip, d, dp, dp]
+        L.hedft_tddft_iteration.argtypes = [vp, d, ip, ip, dp, dp, dp]
+        L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
+        L.hedft_last_elapsed.argtypes = [vp, dp]
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        raise HedftError(f"{what}: {lib().hedft_strerror(rc).decode()} ({rc})")
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().hedft_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("tmean", "tsurf"):
+            for a in range(3):
+                getattr(p, k)[a] = v[a]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def grid_supported(n: int) -> bool:
+    return bool(lib().hedft_grid_supported(n))
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Droplet:
+    """One droplet context on one GPU (module state of the reference on the device)."""
+
+    def __init__(self, nx, ny, nz, xmax, ymax, zmax, params: Params | None = None,
+                 mode: int = MODE_TDDFT, device: int = 0):
+        self.grid = Grid(nx, ny, nz, 0, xmax, ymax, zmax)
+        self.params = params if params is not None else default_params()
+        self.mode = mode
+        self.shape = (nz, ny, nx)
+        self.hshape = (nz, ny, nx // 2 + 1)
+        self.nimp = 0
+        h = C.c_void_p()
+        check(lib().hedft_create(C.byref(self.grid), C.byref(self.params), mode, device, C.byref(h)),
+              "hedft_create")
+        self._c = h
+
+    def close(self):
+        if getattr(self, "_c", None):
+            lib().hedft_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- fields ---------------------------------------------------------------
+    def _layout(self, name):
+        nb = lib().hedft_field_bytes(self._c, FIELD[name])
+        n = int(np.prod(self.shape))
+        nh = int(np.prod(self.hshape))
+        if nb == 8 * n:
+            return np.float64, self.shape
+        if nb == 16 * n:
+            return np.complex128, self.shape
+        if nb == 8 * nh:
+            return np.float64, self.hshape
+        return np.complex128, self.hshape
+
+    def field(self, name: str) -> np.ndarray:
+        dt, shp = self._layout(name)
+        out = np.empty(shp, dtype=dt)
+        check(lib().hedft_download(self._c, FIELD[name], _ptr(out), out.nbytes), f"download {name}")
+        return out
+
+    def set(self, name: str, value):
+        dt, shp = self._layout(name)
+        a = np.empty(shp, dtype=dt)
+        a[...] = value
+        check(lib().hedft_upload(self._c, FIELD[name], _ptr(a), a.nbytes), f"upload {name}")
+
+    def sync(self):
+        check(lib().hedft_synchronize(self._c), "sync")
+
+    # ---- routines ---------------------------------------------------------------
+    def derivnd(self, id_, iv, src="WORK1", dst="WORK2"):
+        check(lib().hedft_derivnD(self._c, id_, iv, FIELD[src], FIELD[dst]), "derivnD")
+
+    def fftfw(self, src="WORK1", dst="WORKK1"):
+        check(lib().hedft_fftfw(self._c, FIELD[src], FIELD[dst]), "fftfw")
+
+    def fftbk(self, src="WORKK1", dst="WORK2"):
+        check(lib().hedft_fftbk(self._c, FIELD[src], FIELD[dst]), "fftbk")
+
+    def derden(self): check(lib().hedft_derden(self._c), "derden")
+    def poten(self): check(lib().hedft_poten(self._c), "poten")
+    def term_alfa(self): check(lib().hedft_term_alfa(self._c), "term_alfa")
+    def potenimp(self): check(lib().hedft_potenimp(self._c), "potenimp")
+    def forceimp(self): check(lib().hedft_forceimp(self._c), "forceimp")
+    def update_aimp(self): check(lib().hedft_update_aimp(self._c), "update_aimp")
+    def tddft_init(self): check(lib().hedft_tddft_init(self._c), "tddft_init")
+
+    def steprkr(self, dt, mu4, n4):
+        m, e = C.c_double(mu4), C.c_double(0)
+        check(lib().hedft_steprkr(self._c, dt, C.byref(m), C.byref(e), n4), "steprkr")
+        return m.value, e.value
+
+    def steppcr(self, dt, mu4, n4):
+        m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
+        check(lib().hedft_steppcr(self._c, dt, C.byref(m), C.byref(e), n4, C.byref(pc)), "steppcr")
+        return m.value, e.value, pc.value
+
+    def steprk(self, dt): check(lib().hedft_steprk(self._c, dt), "steprk")
+
+    def steppc(self, dt):
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        check(lib().hedft_steppc(self._c, dt, C.byref(a), C.byref(b), C.byref(c)), "steppc")
+        return a.value, b.value, c.value
+
+    def set_impurities(self, potion, delinter, rmax, mass):
+        potion = np.ascontiguousarray(np.atleast_2d(potion), dtype=np.float64)
+        mass = np.ascontiguousarray(np.atleast_1d(mass), dtype=np.float64)
+        self.nimp = potion.shape[0]
+        check(lib().hedft_set_impurities(self._c, self.nimp, potion.shape[1], delinter, rmax,
+                                         _ptr(potion), _ptr(mass)), "set_impurities")
+
+    def set_impurity_state(self, rimp=None, vimp=None, aimp=None):
+        keep = []
+
+        def conv(a):
+            if a is None:
+                return None
+            flat = np.ascontiguousarray(
+                np.asarray(a, dtype=np.float64).reshape(self.nimp, 3).ravel(order="F"))
+            keep.append(flat)
+            return _ptr(flat)
+        check(lib().hedft_set_impurity_state(self._c, conv(rimp), conv(vimp), conv(aimp)),
+              "set_impurity_state")
+
+    def impurity_state(self):
+        m = self.nimp * 3
+        r, v, a, f = (np.zeros(max(m, 1)) for _ in range(4))
+        ls = C.c_int32(0)
+        check(lib().hedft_get_impurity_state(self._c, _ptr(r), _ptr(v), _ptr(a), _ptr(f),
+                                             C.byref(ls)), "get_impurity_state")
+        shp = lambda x: x[:m].reshape(3, self.nimp).T.copy()
+        return {"rimp": shp(r), "vimp": shp(v), "aimp": shp(a), "force": shp(f), "lstopimp": ls.value}
+
+    def energy(self) -> dict:
+        e = Energies()
+        check(lib().hedft_energy(self._c, C.byref(e)), "energy")
+        return {n: getattr(e, n) for n, _ in Energies._fields_}
+
+    def statics_init(self, n4):
+        m = C.c_double(0)
+        check(lib().hedft_statics_init(self._c, n4, C.byref(m)), "statics_init")
+        return m.value
+
+    def statics_iteration(self, dt, iterrk, n4, mu4):
+        m, e = C.c_double(mu4), C.c_double(0)
+        check(lib().hedft_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e)),
+              "statics_iteration")
+        return m.value, e.value
+
+    def tddft_iteration(self, dt, it, nrk):
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        check(lib().hedft_tddft_iteration(self._c, dt, it, nrk, C.byref(a), C.byref(b), C.byref(c)),
+              "tddft_iteration")
+        return a.value, b.value, c.value
+
+    def run_pc(self, dt, nsteps, use_graph=True) -> float:
+        ms = C.c_double(0)
+        check(lib().hedft_tddft_run_pc(self._c, dt, nsteps, int(use_graph), C.byref(ms)), "run_pc")
+        return ms.value

@@ -1,0 +1,75 @@
+"""Seeded synthetic droplets and the impurity pair-potential table.
+
+The reference reads its initial density from a file (readen.f90:74-84) and
+tabulates the He-impurity potential with Select_Pot (potenimpini.f90:31-59).
+The workloads here use:
+
+* a Woods-Saxon droplet rho(r) = 0.0218/(1+exp((r-R)/0.7)), R = 2.22 N^(1/3),
+  times (1 + 1e-3 U) with U ~ Uniform[0,1) drawn by numpy's default_rng(seed)
+  in C order over (iz, iy, ix), normalised to N atoms;
+* a Lennard-Jones He-X pair potential V = 4 eps ((s/r)^12 - (s/r)^6) with the
+  Select_Pot core convention (r < r_cutoff -> umax, He_statics/V_impur.f90:
+  189-194), an entry the reference's library does not have.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PSTOK = 7.63822291        # ps <-> K^-1 conversion of the reference (DFT4He3d.f90:242)
+MP_U = 0.02061484         # proton mass in K^-1 A^-2 (Dynamics_TDDFT/modules.f90: mp_u)
+
+
+def grid_axes(n: int, xmax: float) -> np.ndarray:
+    """x_i = -xmax + h (i-1), h = 2 xmax / n (BCN4HeDFT.f90:293-327)."""
+    h = 2.0 * xmax / n
+    return -xmax + h * np.arange(n)
+
+
+def woods_saxon(nx, ny, nz, xmax, ymax, zmax, n4, seed=0, rho0=0.0218, a=0.7, noise=1e-3,
+                center=(0.0, 0.0, 0.0)) -> np.ndarray:
+    x, y, z = grid_axes(nx, xmax), grid_axes(ny, ymax), grid_axes(nz, zmax)
+    r = np.sqrt((x[None, None, :] - center[0]) ** 2 + (y[None, :, None] - center[1]) ** 2
+                + (z[:, None, None] - center[2]) ** 2)
+    R = 2.22 * n4 ** (1.0 / 3.0)
+    rho = rho0 / (1.0 + np.exp((r - R) / a))
+    rng = np.random.default_rng(seed)
+    rho = rho * (1.0 + noise * rng.random(rho.shape))
+    dxyz = (2 * xmax / nx) * (2 * ymax / ny) * (2 * zmax / nz)
+    return np.ascontiguousarray(rho * (n4 / (rho.sum() * dxyz)))
+
+
+def select_pot_lj(r, eps=20.0, sigma=4.0, r_cutoff=3.0, umax=None):
+    """LJ entry with Select_Pot semantics: umax inside r_cutoff.  By default
+    umax = V(r_cutoff), so the capped core is continuous."""
+    r = np.asarray(r, dtype=np.float64)
+    if umax is None:
+        sc6 = (sigma / r_cutoff) ** 6
+        umax = 4.0 * eps * (sc6 * sc6 - sc6)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sr6 = (sigma / np.where(r > 0, r, 1.0)) ** 6
+        v = 4.0 * eps * (sr6 * sr6 - sr6)
+    return np.where(r < r_cutoff, umax, v)
+
+
+def impurity_table(n_max: int, box_max: float, **lj):
+    """potenimpini (potenimpini.f90:154-170): npot = 100 max(n) + 1 points on
+    [0, 4 max(xmax)], DelInter = rmax/(npot-1).  Returns (potion, delinter, rmax)."""
+    npot = 100 * n_max + 1
+    rmax = 4.0 * box_max
+    delinter = rmax / (npot - 1)
+    r = delinter * np.arange(npot)
+    return np.ascontiguousarray(select_pot_lj(r, **lj)), delinter, rmax
+
+
+def tddft_setup(drop, n, xmax, n4, seed=0, mass_amu=108.0, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 3.0),
+                lj=None):
+    """Load a Woods-Saxon droplet and one LJ impurity into a context (a Droplet or
+    an oracle.Oracle: both expose the same methods).  Velocities in A/ps are
+    converted with PSTOK (DFT4He3d.f90:299)."""
+    lj = lj or {}
+    den = woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=seed)
+    drop.set("DEN", den)
+    potion, delinter, rmax = impurity_table(n, xmax, **lj)
+    drop.set_impurities(potion[None, :], delinter, rmax, [mass_amu * MP_U])
+    drop.set_impurity_state(np.array(r0)[None, :], np.array(v_aps)[None, :] * PSTOK, np.zeros((1, 3)))
+    return den

@@ -1,0 +1,228 @@
+/*
+ * hedft.h — C-ABI of the MI355X-native BCN-4He-DFT time-step hot path.
+ *
+ * The reference (s-kesh/He_DFT_2024) has no FFI; its "API" is a set of Fortran
+ * subroutines that communicate through module arrays.  Every entry point below
+ * replaces one of those subroutines (cited file:line, paths relative to the
+ * reference root) and keeps its argument meaning.  Module arrays live on the
+ * device inside a context; hedft_upload/hedft_download copy them to and from
+ * the host in the reference's own layout (Fortran column-major, x fastest:
+ * idx = ix + nx*(iy + ny*iz); complex(8) = interleaved re,im; half spectra are
+ * (nx/2+1, ny, nz)), so the Fortran wrapper passes c_loc() of its arrays.
+ *
+ * Conventions: every function returns int (HEDFT_OK = 0 on success, a negative
+ * HEDFT_E* code otherwise; hedft_strerror() names it).  No exceptions cross the
+ * ABI.  A context owns one HIP stream; calls on one context are not
+ * thread-safe, distinct contexts are independent.  Functions that return host
+ * scalars synchronise the context's stream; all others only enqueue work.
+ */
+#ifndef HEDFT_H
+#define HEDFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HEDFT_OK 0
+#define HEDFT_EINVAL -1     /* bad argument (size, id, mode)                 */
+#define HEDFT_EGRID -2      /* grid edge not supported (tstgrid.f90 values)  */
+#define HEDFT_EHIP -3       /* HIP runtime error                             */
+#define HEDFT_ENOMEM -4     /* device allocation failed                      */
+#define HEDFT_ESTATE -5     /* call out of order (e.g. PC before RK start)   */
+#define HEDFT_EBOUNDS -6    /* impurity left the interpolation table          */
+
+/* ---- modes ------------------------------------------------------------- */
+#define HEDFT_MODE_STATICS 0 /* real psi, imaginary time (He_statics)         */
+#define HEDFT_MODE_TDDFT 1   /* complex psi, real time (Dynamics_TDDFT)       */
+
+#define HEDFT_PREC_FP64 0    /* plain FP64 path (the drop-in)                 */
+#define HEDFT_PREC_MIXED 1   /* FP32 mean field/stencils, FP64 psi + sums     */
+
+#define HEDFT_MAX_IMP 4
+
+/* ---- field identifiers (module arrays of the reference) ------------------ */
+enum hedft_field {
+  HEDFT_DEN = 0,      /* rho:            real  N   (module rho)              */
+  HEDFT_PSI,          /* psi:   real N (statics) / complex N (TDDFT)         */
+  HEDFT_POT4,         /* field: pot4     real  N                             */
+  HEDFT_DENCG,        /* rho: dencg      real  N                             */
+  HEDFT_DELJ4,        /* lenard4: delj4  real  N                             */
+  HEDFT_DENALF,       /* alphasterm      real  N                             */
+  HEDFT_FALFS,        /*                 real  N                             */
+  HEDFT_INTXALF,      /*                 real  N                             */
+  HEDFT_INTYALF,      /*                 real  N                             */
+  HEDFT_INTZALF,      /*                 real  N                             */
+  HEDFT_UALPHAS,      /*                 real  N                             */
+  HEDFT_DXDEN,        /* deriva          real  N                             */
+  HEDFT_DYDEN,        /*                 real  N                             */
+  HEDFT_DZDEN,        /*                 real  N                             */
+  HEDFT_UIMP,         /* classicimp uimp real  N (evaluated on download)     */
+  HEDFT_UEXT,         /* field: uext     real  N (statics, lexternal)        */
+  HEDFT_TIMEC,        /* statics: real N (local step); TDDFT: complex N      */
+  HEDFT_PSIOLD1,      /* rho: psiold(:,:,:,k) in PHYSICAL slot k = 1..3      */
+  HEDFT_PSIOLD2,
+  HEDFT_PSIOLD3,
+  HEDFT_HPSIOLD1,     /* rho: hpsiold(:,:,:,k), k = 1..2                     */
+  HEDFT_HPSIOLD2,
+  HEDFT_PC,           /* rkpc: pc                                            */
+  HEDFT_Q,            /* rkpc: q                                             */
+  HEDFT_FDEN,         /* rho: fden   complex (nx/2+1,ny,nz)                  */
+  HEDFT_FVLJ4,        /* lenard4: fvlj4 real (nx/2+1,ny,nz)                  */
+  HEDFT_WCGK,         /* rho: wcgk      real (nx/2+1,ny,nz)                  */
+  HEDFT_KALFS,        /* alphasterm: kalfs real (nx/2+1,ny,nz)               */
+  HEDFT_WORK1,        /* scratch real N fields for the derivnD / FFT hooks   */
+  HEDFT_WORK2,
+  HEDFT_WORKC1,       /* scratch complex N field                             */
+  HEDFT_WORKK1,       /* scratch complex half spectrum                       */
+  HEDFT_NFIELDS
+};
+
+/* Grid: x_i = -xmax + h*(i-1), h = 2*xmax/nx (He_statics/BCN4HeDFT.f90:293-327). */
+typedef struct hedft_grid {
+  int32_t nx, ny, nz;
+  int32_t pad_;
+  double xmax, ymax, zmax;
+} hedft_grid;
+
+/* Functional and method parameters.  Defaults are the reference's module
+ * initialisers (He_statics/modules.f90:15-19,109-133,214-226,297-299;
+ * Dynamics_TDDFT/modules.f90 he4, DFT4He3d.f90:136-138); hedft_default_params
+ * fills them. */
+typedef struct hedft_params {
+  double h2o2m4, cp4, cpp4, alphas, l, den0s;   /* he4                      */
+  double csolid, beta_solid, den_m;             /* he4: C, beta, den_m      */
+  double h4, eps4, sigma4;                      /* lenard4 (h4 = h4ot)      */
+  int32_t core_otc;  /* 1: core4 = 'OTC' (alpha_s term), 0: 'OT '            */
+  int32_t lsolid;    /* solid penalty; the drivers force core4='OT ' with it */
+  int32_t npd;       /* stencil points: 13 (deriva: npd)                     */
+  int32_t icon;      /* boundary mode of derivnD: 0, 8, 13, 18, 20, 24      */
+  double dtmin, sdmu, dmumax, psimin;           /* statics local time step  */
+  double lambdah, tmean[3], tsurf[3];           /* TDDFT absorbing layer    */
+  int32_t precision; /* HEDFT_PREC_FP64 | HEDFT_PREC_MIXED                  */
+  int32_t lexternal; /* statics: add uext to pot4 (He_statics/poten.f90:277) */
+} hedft_params;
+
+/* Per-term energies, He_statics/energy.f90:12-224 and
+ * Dynamics_TDDFT/energy.f90:12-182 (module energies). */
+typedef struct hedft_energies {
+  double ekin4, elj4, ecor4, ealphas, esolid, eext; /* helium terms          */
+  double etot4;                                     /* helium total          */
+  double ekinx, eHeX, eimpu_impu, eimpu;            /* classical impurities  */
+  double etot;                                      /* etot4 + eimpu         */
+  double norm;                                      /* sum(den)*dxyz         */
+} hedft_energies;
+
+typedef struct hedft_ctx hedft_ctx;
+
+const char* hedft_strerror(int rc);
+/* Fills the reference defaults (OT functional, npd=13, icon=13, core 'OT ',
+ * lsolid=.false., Dtmin=1e-10, Sdmu=0.1, Dmumax=100, Lambdah=2, t*mean=1000,
+ * t*surf=2, FP64). */
+void hedft_default_params(hedft_params* p);
+/* 1 when n is an accepted grid edge (tstgrid.f90:1-23, the FFT sizes this
+ * library plans), else 0. */
+int hedft_grid_supported(int32_t n);
+
+/* dimen.f90 + fftini (fft.f90:5-91) + fforma/initcg/kalfs kernel set-up
+ * (fforma.f90:11-61, initcg.f90:32-49, BCN4HeDFT.f90:849).  Allocates every
+ * module array of the chosen mode on device `device`. */
+int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode,
+                 int32_t device, hedft_ctx** out);
+int hedft_destroy(hedft_ctx* c);
+int hedft_upload(hedft_ctx* c, int32_t field, const void* host, size_t bytes);
+int hedft_download(hedft_ctx* c, int32_t field, void* host, size_t bytes);
+size_t hedft_field_bytes(const hedft_ctx* c, int32_t field);
+int hedft_synchronize(hedft_ctx* c);
+
+/* ---- L1: numerical kernels (parity hooks) ----------------------------- */
+/* derivnD(id,nn,h,iv,f,df,icon), init_deriv_parallel.f90:373-648 (real) and
+ * :933-1209 (complex): id-th derivative along axis iv (1..3) of field `in`
+ * into field `out`, with the context's icon and the grid step of that axis. */
+int hedft_derivnD(hedft_ctx* c, int32_t id, int32_t iv, int32_t in, int32_t out);
+/* fftfw(a,b) / fftfw_den: unnormalised R2C, e^{-i} (He_statics/fft.f90:37-59,
+ * Dynamics_TDDFT/fft.f90:96-114).  in: real N field, out: spectrum field. */
+int hedft_fftfw(hedft_ctx* c, int32_t in, int32_t out);
+/* fftbk(c,d) / fftbk_*: C2R scaled by 1/(nx*ny*nz) (He_statics/fft.f90:62-89,
+ * Dynamics_TDDFT/fft.f90:118-168).  The input spectrum is destroyed. */
+int hedft_fftbk(hedft_ctx* c, int32_t in, int32_t out);
+
+/* ---- L2: physics operators ------------------------------------------- */
+/* derden(): grad(den) -> dxden,dyden,dzden (He_statics/derden.f90:5-25). */
+int hedft_derden(hedft_ctx* c);
+/* poten(): statics semantics (He_statics/poten.f90:5-221) use the stored
+ * fden/dencg/denalf/grad(den); TDDFT semantics (Dynamics_TDDFT/poten.f90:124-221)
+ * recompute them from den.  Includes term_alfa when core_otc. */
+int hedft_poten(hedft_ctx* c);
+/* term_alfa() alone (He_statics/term_alfa.f90:5-102); needs dencg/denalf and
+ * grad(den) current.  Writes falfs, int*alf, ualphas. */
+int hedft_term_alfa(hedft_ctx* c);
+/* energy(): per-term energies (He_statics/energy.f90:12-224,
+ * Dynamics_TDDFT/energy.f90:12-182). Synchronises. */
+int hedft_energy(hedft_ctx* c, hedft_energies* e);
+
+/* ---- L3: integrators -------------------------------------------------- */
+/* STEPRKR(deltat,mu4,mu4err,n4), He_statics/steprkr.f90:1-96. */
+int hedft_steprkr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4);
+/* STEPPCR(deltat,mu4,mu4err,n4), He_statics/steppcr.f90:1-108; errpc is the
+ * module variable ErrPC (may be NULL). */
+int hedft_steppcr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4,
+                  double* errpc);
+/* STEPRK(deltat), Dynamics_TDDFT/steprk.f90:1-144. */
+int hedft_steprk(hedft_ctx* c, double deltat);
+/* STEPPC(deltat,errHe,errimp,errvimp), Dynamics_TDDFT/steppc.f90:1-180.
+ * Any of the error pointers may be NULL; when all are NULL nothing syncs. */
+int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, double* errvimp);
+
+/* ---- classical impurities (Dynamics_TDDFT/potenimp*.f90) -------------- */
+/* potenimpini (potenimpini.f90:31-59): radial tables potion(k,1:npot) with
+ * r_i = (i-1)*delinter, rows contiguous per impurity; mass in K^-1 A^-2
+ * (m_imp = m_imp_u*mp_u, DFT4He3d.f90:298). */
+int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delinter,
+                         double rmaxinterpol, const double* potion, const double* mass);
+/* rimp, vimp, aimp as (nimp,3) Fortran arrays (k fastest); NULL skips one. */
+int hedft_set_impurity_state(hedft_ctx* c, const double* rimp, const double* vimp,
+                             const double* aimp);
+int hedft_get_impurity_state(hedft_ctx* c, double* rimp, double* vimp, double* aimp,
+                             double* force, int32_t* lstopimp);
+/* potenimp()/updatepoten (potenimp.f90:1-19,83-123): uimp follows rimp; the
+ * device evaluates it on the fly from the tables, so this only validates. */
+int hedft_potenimp(hedft_ctx* c);
+/* forceimp() (potenimp.f90:23-67): F = -sum(grad(den)*uimp_k)*dxyz. */
+int hedft_forceimp(hedft_ctx* c);
+/* aimp = F/m (DFT4He3d.f90:692-696). */
+int hedft_update_aimp(hedft_ctx* c);
+
+/* ---- driver glue (order of DFT4He3d.f90:677-696, BCN4HeDFT.f90:1113-1160) */
+/* Statics start (BCN4HeDFT.f90:724-874): normalise den to n4, psi=sqrt(den),
+ * fden, dencg, denalf, derden, term_alfa, poten; *mu4 = <psi|H|psi>/n4. */
+int hedft_statics_init(hedft_ctx* c, double n4, double* mu4);
+/* One statics iteration: STEPRKR when iterrk<=4 else STEPPCR, then derden,
+ * fftfw(den), dencg/denalf, poten (BCN4HeDFT.f90:1113-1160). */
+int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double n4,
+                            double* mu4, double* mu4err);
+/* TDDFT start (DFT4He3d.f90:525-601): psi=sqrt(den) (real), den=|psi|^2,
+ * timec, potenimp, poten, forceimp, aimp. */
+int hedft_tddft_init(hedft_ctx* c);
+/* One TDDFT iteration: STEPRK for the first nrk iterations (the driver uses 3,
+ * DFT4He3d.f90:677) else STEPPC, then potenimp, poten, forceimp, aimp=F/m.
+ * iter counts from 1.  Errors may be NULL (then nothing syncs). */
+int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk,
+                          double* errHe, double* errimp, double* errvimp);
+/* nsteps PC iterations (after the RK start) with the per-step work captured
+ * once in a HIP graph and replayed; returns the elapsed device time of the
+ * replays measured with HIP events on the context's stream (ms). */
+int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph,
+                       double* elapsed_ms);
+
+/* Device time (ms, HIP events on the context stream) of the last
+ * hedft_tddft_run_pc / hedft_time_kernel call. */
+int hedft_last_elapsed(hedft_ctx* c, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEDFT_H */

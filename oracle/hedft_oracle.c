@@ -1,0 +1,1305 @@
+/*
+ * hedft_oracle.c — CPU FP64 restatement of the BCN-4He-DFT time-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke check
+ * and the CPU-baseline leg of bench.py link it as the checker.  The product
+ * library (he_dft_2024_amd/) never includes, links or calls it.
+ *
+ * It restates, in plain C with OpenMP loops, the reference's algorithm
+ * (s-kesh/He_DFT_2024; citations are relative to the reference root):
+ *   stencils   init_deriv_parallel.f90:51-83,373-648,933-1209, redef.f:1-324
+ *   FFT        He_statics/fft.f90:5-89, Dynamics_TDDFT/fft.f90:5-168
+ *   kernels    fforma.f90:11-280, initcg.f90:32-49, BCN4HeDFT.f90:849
+ *   mean field He_statics/poten.f90, Dynamics_TDDFT/poten.f90, term_alfa.f90, derden.f90
+ *   steppers   He_statics/steprkr.f90, steppcr.f90; Dynamics_TDDFT/steprk.f90, steppc.f90
+ *   impurity   Dynamics_TDDFT/potenimp.f90, potenimpini.f90
+ *   energies   He_statics/energy.f90, Dynamics_TDDFT/energy.f90
+ * The FFT is this file's own mixed-radix DFT (the reference links MKL's FFTW3,
+ * which is not available here); the sine integral uses the series /
+ * continued-fraction method of Abramowitz & Stegun 5.2 (the reference uses a
+ * NAG-style Chebyshev routine, s13adf.f90).
+ *
+ * Reductions are summed per z-plane and then plane by plane, so results do not
+ * depend on the OpenMP thread count.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ---- parameter blocks: same layout as include/hedft.h (ctypes shares them) */
+typedef struct {
+  int32_t nx, ny, nz, pad_;
+  double xmax, ymax, zmax;
+} ogrid;
+
+typedef struct {
+  double h2o2m4, cp4, cpp4, alphas, l, den0s;
+  double csolid, beta_solid, den_m;
+  double h4, eps4, sigma4;
+  int32_t core_otc, lsolid, npd, icon;
+  double dtmin, sdmu, dmumax, psimin;
+  double lambdah, tmean[3], tsurf[3];
+  int32_t precision, lexternal;
+} oparams;
+
+enum { MODE_STATICS = 0, MODE_TDDFT = 1 };
+#define MAXIMP 4
+
+/* field ids: identical numbering to enum hedft_field */
+enum {
+  F_DEN = 0, F_PSI, F_POT4, F_DENCG, F_DELJ4, F_DENALF, F_FALFS, F_INTX, F_INTY, F_INTZ,
+  F_UALPHAS, F_DXDEN, F_DYDEN, F_DZDEN, F_UIMP, F_UEXT, F_TIMEC, F_PSIOLD1, F_PSIOLD2,
+  F_PSIOLD3, F_HPSIOLD1, F_HPSIOLD2, F_PC, F_Q, F_FDEN, F_FVLJ4, F_WCGK, F_KALFS, F_WORK1,
+  F_WORK2, F_WORKC1, F_WORKK1, F_NFIELDS
+};
+
+typedef struct {
+  ogrid g;
+  oparams p;
+  int mode;
+  int nx, ny, nz, nxh;
+  size_t n, nh;
+  double hx, hy, hz, dxyz, renor;
+  double *x, *y, *z;
+  /* real N */
+  double *den, *pot4, *dencg, *delj4, *denalf, *falfs, *intx, *inty, *intz, *ualphas;
+  double *dxden, *dyden, *dzden, *uimp, *uext, *work1, *work2;
+  double *sto1, *sto2, *sto3, *sto4, *sto5, *sto6, *penalty;
+  /* psi and history: real (statics) or complex (TDDFT) stored as doubles */
+  double *psi, *timec, *psiold[3], *hpsiold[2], *pc, *q, *hpsi;
+  double *workc1;
+  /* half spectra */
+  cplx *fden, *wk1, *wk2, *wk3, *workk1;
+  double *fvlj4, *wcgk, *kalfs, *pmod;
+  int ioldp[3], ioldh[2];
+  /* classical impurities (module classicimp / interpol) */
+  int nimp, npot;
+  double delinter, rmaxinterpol;
+  double *potion;
+  double mimp[MAXIMP];
+  double rimp[MAXIMP * 3], vimp[MAXIMP * 3], aimp[MAXIMP * 3], force[MAXIMP * 3];
+  double qr[MAXIMP * 3], qv[MAXIMP * 3], pcr[MAXIMP * 3], pcv[MAXIMP * 3];
+  double rimpold[3][MAXIMP * 3], vimpold[3][MAXIMP * 3], aimpold[2][MAXIMP * 3];
+  int ioldr[3], ioldv[3], iolda[2];
+  int lstopimp;
+} octx;
+
+/* ======================================================================= */
+/*  Sine integral Si(x): Abramowitz & Stegun 5.2.14 (power series) for      */
+/*  |x| <= 4, and Si = pi/2 - f cos x - g sin x with f, g from the          */
+/*  continued fraction of E1(ix) (A&S 5.1.22, evaluated by modified Lentz)  */
+/*  otherwise.  Returns pi/2 - Si(x) for x > 0 in *comp without cancellation. */
+/*  Evaluated in long double: the LJ form factor built from it is a          */
+/*  difference of nearly equal terms at large |p| (fforma.f90:96-100).       */
+/* ======================================================================= */
+typedef long double ldbl;
+static const ldbl HALFPI_L = 1.570796326794896619231321691639751442L;
+static const ldbl PI_L = 3.141592653589793238462643383279502884L;
+
+static void sine_integral(ldbl x, ldbl* si, ldbl* comp) {
+  ldbl ax = fabsl(x);
+  if (ax <= 4.0L) {
+    ldbl x2 = ax * ax, term = ax, sum = ax;
+    for (int k = 1; k < 80; ++k) {
+      term *= -x2 / ((2.0L * k) * (2.0L * k + 1.0L));
+      ldbl t = term / (2.0L * k + 1.0L);
+      sum += t;
+      if (fabsl(t) < 1e-22L * fabsl(sum)) break;
+    }
+    ldbl s = (x < 0) ? -sum : sum;
+    *si = s;
+    *comp = HALFPI_L - s;
+    return;
+  }
+  /* E1(i x) continued fraction: E1(z) = e^{-z} / (z + 1/(1 + 1/(z + 2/(1 + ...)))) */
+  long double complex b = 1.0L + I * ax, c = 1.0L / 1e-300L, d = 1.0L / b, h = d;
+  for (int i = 1; i < 400; ++i) {
+    ldbl a = -(ldbl)i * i;
+    b += 2.0L;
+    d = 1.0L / (a * d + b);
+    c = b + a / c;
+    long double complex del = c * d;
+    h *= del;
+    if (cabsl(del - 1.0L) < 1e-21L) break;
+  }
+  h *= cosl(ax) - I * sinl(ax);
+  /* h = E1(i ax) = -Ci(ax) + i(Si(ax) - pi/2) */
+  ldbl pim = cimagl(h); /* Si - pi/2 */
+  ldbl s = HALFPI_L + pim;
+  *si = (x < 0) ? -s : s;
+  *comp = (x < 0) ? HALFPI_L + s : -pim;
+}
+
+/* fforma.f90:73-102 sinx11(a,x) = int_x^inf sin(a t)/t^11 dt */
+static ldbl sinx11(ldbl a, ldbl x) {
+  ldbl y = a * x, y2 = y * y;
+  ldbl b[5] = {40320.0L, -720.0L, 24.0L, -2.0L, 1.0L};
+  ldbl c[5] = {362880.0L, -5040.0L, 120.0L, -6.0L, 1.0L};
+  ldbl d[5] = {1.0L, y2, y2 * y2, y2 * y2 * y2, y2 * y2 * y2 * y2};
+  ldbl bd = 0, cd = 0;
+  for (int i = 0; i < 5; ++i) { bd += b[i] * d[i]; cd += c[i] * d[i]; }
+  ldbl res1 = (a * bd * cosl(y) + cd * sinl(y) / x) / (3628800.0L * powl(x, 9));
+  ldbl si, comp;
+  sine_integral(y, &si, &comp);
+  ldbl sinint = (a > 0.0L) ? comp : (HALFPI_L + si);
+  ldbl res2 = -(powl(a, 10) * sinint) / 3628800.0L;
+  return res1 + res2;
+}
+
+/* fforma.f90:108-137 sinx5(a,x) = int_x^inf sin(a t)/t^5 dt */
+static ldbl sinx5(ldbl a, ldbl x) {
+  ldbl y = a * x, y2 = y * y;
+  ldbl bd = -2.0L + y2, cd = -6.0L + y2;
+  ldbl res1 = -(a * bd * cosl(y) + cd * sinl(y) / x) / (24.0L * x * x * x);
+  ldbl si, comp;
+  sine_integral(y, &si, &comp);
+  ldbl sinint = (a > 0.0L) ? comp : (HALFPI_L + si);
+  ldbl res2 = (powl(a, 4) * sinint) / 24.0L;
+  return res1 + res2;
+}
+
+/* fforma.f90:256-280 bforce, Orsay-Trento branch: p = 0 value of the kernel */
+static ldbl bforce_ot(ldbl eps, ldbl sigma, ldbl hc) {
+  ldbl pi4 = 4.0L * PI_L, eps4 = 4.0L * eps;
+  ldbl s6 = powl(sigma, 6), s12 = s6 * s6;
+  return pi4 * eps4 * s12 / (9.0L * powl(hc, 9)) - pi4 * eps4 * s6 / (3.0L * powl(hc, 3));
+}
+
+double oracle_sine_integral(double x) { ldbl s, c; sine_integral(x, &s, &c); return (double)s; }
+double oracle_fvlj_radial(double p, double eps, double sigma, double hc) {
+  if (fabs(p) < 1e-10) return (double)bforce_ot(eps, sigma, hc);
+  ldbl a = 2.0L * PI_L * p, s6 = powl((ldbl)sigma, 6);
+  return (double)(8.0L * eps * s6 * (s6 * sinx11(a, hc) - sinx5(a, hc)) / p);
+}
+
+/* ======================================================================= */
+/*  Stencil coefficients: Bickley M-point matrices (rows = node i of the     */
+/*  13-point window), computed with Fornberg's algorithm (Math. Comp. 51,    */
+/*  699 (1988)) in long double; newder.f:99-130 reads the same matrices from  */
+/*  the mates.f table.                                                        */
+/* ======================================================================= */
+static void fornberg(int m, int npts, long double z, const long double* xs, long double* w) {
+  /* weights w[j] of the m-th derivative at z on the nodes xs[0..npts-1] */
+  long double cw[32][16];
+  memset(cw, 0, sizeof cw);
+  long double c1 = 1.0L, c4 = xs[0] - z;
+  cw[0][0] = 1.0L;
+  for (int i = 1; i < npts; ++i) {
+    int mn = i < m ? i : m;
+    long double c2 = 1.0L, c5 = c4;
+    c4 = xs[i] - z;
+    for (int j = 0; j < i; ++j) {
+      long double c3 = xs[i] - xs[j];
+      c2 *= c3;
+      if (j == i - 1) {
+        for (int k = mn; k >= 1; --k) cw[i][k] = c1 * (k * cw[i - 1][k - 1] - c5 * cw[i - 1][k]) / c2;
+        cw[i][0] = -c1 * c5 * cw[i - 1][0] / c2;
+      }
+      for (int k = mn; k >= 1; --k) cw[j][k] = (c4 * cw[j][k] - k * cw[j][k - 1]) / c3;
+      cw[j][0] = c4 * cw[j][0] / c3;
+    }
+    c1 = c2;
+  }
+  for (int j = 0; j < npts; ++j) w[j] = cw[j][m];
+}
+
+/* redef.f:1-324 restated for the boundary modes this path uses.  cd is the
+ * K x K matrix, row-major cd[i*K+j] (0-based).  Mode composition follows the
+ * reference's TORNA chaining (13 -> 12, 24 -> 4, 25 -> 5, 6 -> 4, 7 -> 5). */
+static void redef(int K, double* cd, int icon0) {
+  int ko2 = K / 2; /* 0-based central row index = ko2 */
+  int icon = icon0;
+  for (;;) {
+    int again = 0;
+    if (icon == 0) return;
+    if (icon == 4 || icon == 5) {
+      int ip = (icon == 4) ? 1 : -1;
+      double c0[32];
+      for (int j = 0; j < K; ++j) c0[j] = cd[ko2 * K + j];
+      for (int i = 1; i <= ko2; ++i) {
+        double* row = cd + (i - 1) * K;
+        int j0 = ko2 + 1 - i + 1, j1 = j0;
+        row[0] = c0[j0 - 1];
+        for (int j = 2; j <= K; ++j) {
+          j0 += 1; j1 -= 1;
+          if (j0 <= K) row[j - 1] = (j1 >= 1) ? c0[j0 - 1] + ip * c0[j1 - 1] : c0[j0 - 1];
+          else row[j - 1] = 0.0;
+        }
+      }
+    } else if (icon == 12) {
+      for (int i = ko2 + 2; i <= K; ++i) {
+        for (int j = K; j >= 2; --j) cd[(i - 1) * K + j - 1] = cd[(i - 2) * K + j - 2];
+        cd[(i - 1) * K] = 0.0;
+      }
+    } else if (icon == 13) {
+      for (int i = ko2; i >= 1; --i) {
+        for (int j = 1; j <= K - 1; ++j) cd[(i - 1) * K + j - 1] = cd[i * K + j];
+        cd[(i - 1) * K + K - 1] = 0.0;
+      }
+      icon = 12; again = 1;
+    } else if (icon == 20 || icon == 24 || icon == 25) {
+      double c0[32];
+      for (int j = 0; j < K; ++j) c0[j] = cd[ko2 * K + j];
+      for (int i = K; i >= ko2 + 2; --i) {
+        double* row = cd + (i - 1) * K;
+        int j0 = ko2 + 1 - i + K, j1 = j0;
+        row[K - 1] = c0[j0 - 1];
+        for (int j = K - 1; j >= 1; --j) {
+          j0 -= 1; j1 += 1;
+          if (j0 >= 1) row[j - 1] = (j1 <= K) ? c0[j0 - 1] + c0[j1 - 1] : c0[j0 - 1];
+          else row[j - 1] = 0.0;
+        }
+      }
+      if (icon == 24) { icon = 4; again = 1; }
+      else if (icon == 25) { icon = 5; again = 1; }
+    } else {
+      return; /* 8/18 handled by wrapping; other modes are not on the path */
+    }
+    if (!again) return;
+  }
+}
+
+/* Init_deriv_p + the coefficient set-up of Deriv3D_p (init_deriv_parallel.f90:
+ * 51-83, 399-414): K x K matrix for derivative id, boundary mode icon, step h */
+void oracle_stencil_matrix(int id, int K, int icon, double h, double* cd) {
+  long double xs[32], w[32];
+  for (int j = 0; j < K; ++j) xs[j] = (long double)j;
+  for (int i = 0; i < K; ++i) {
+    fornberg(id, K, (long double)i, xs, w);
+    for (int j = 0; j < K; ++j) cd[i * K + j] = (double)w[j];
+  }
+  if (icon != 0 && icon != 8 && icon != 18) redef(K, cd, icon);
+  double sto = 1.0 / pow(h, id);
+  for (int i = 0; i < K * K; ++i) cd[i] *= sto;
+}
+
+/* Deriv3D_p / Deriv3D_pc (init_deriv_parallel.f90:373-648, 933-1209): derivative
+ * along axis iv (1..3) of f (ncomp = 1 real, 2 complex interleaved). */
+void oracle_derivnd(int id, int nx, int ny, int nz, double h, int iv, const double* f, double* df,
+                    int icon, int ncomp) {
+  const int K = 13, ko2 = 6;
+  double cd[169];
+  int nn[3] = {nx, ny, nz};
+  int n = nn[iv - 1];
+  size_t stride = (iv == 1) ? 1 : (iv == 2) ? (size_t)nx : (size_t)nx * ny;
+  size_t nlines = (size_t)nx * ny * nz / n;
+  if (n < K) {
+    memset(df, 0, sizeof(double) * ncomp * (size_t)nx * ny * nz);
+    return;
+  }
+  oracle_stencil_matrix(id, K, icon, h, cd);
+  const double* cc = cd + ko2 * K;
+  int periodic = (icon == 8 || icon == 18);
+  double is = (icon == 18) ? -1.0 : 1.0;
+#pragma omp parallel for schedule(static)
+  for (size_t line = 0; line < nlines; ++line) {
+    size_t base;
+    if (iv == 1) base = line * nx;
+    else if (iv == 2) base = (line % nx) + (line / nx) * (size_t)nx * ny;
+    else base = line;
+    for (int comp = 0; comp < ncomp; ++comp) {
+#define F(i) f[(base + (size_t)(i) * stride) * ncomp + comp]
+      for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        if (periodic) {
+          for (int j = 0; j < K; ++j) {
+            int ii = i + j - ko2;
+            double sg = 1.0;
+            if (ii < 0) { ii += n; sg = is; }
+            else if (ii >= n) { ii -= n; sg = is; }
+            s += cc[j] * sg * F(ii);
+          }
+        } else if (i < ko2) {
+          for (int j = 0; j < K; ++j) s += cd[i * K + j] * F(j);
+        } else if (i >= n - ko2) {
+          int r = i - n + K;
+          for (int j = 0; j < K; ++j) s += cd[r * K + j] * F(n - K + j);
+        } else {
+          for (int j = 0; j < K; ++j) s += cc[j] * F(i - ko2 + j);
+        }
+        df[(base + (size_t)i * stride) * ncomp + comp] = s;
+      }
+#undef F
+    }
+  }
+}
+
+/* ======================================================================= */
+/*  FFT: recursive mixed-radix decimation in time over the prime factors of  */
+/*  n (2, 3, 5, ...), twiddles from a per-length table.                      */
+/* ======================================================================= */
+typedef struct { int n; cplx* w; } fft_table;
+
+static void fft_table_init(fft_table* t, int n) {
+  t->n = n;
+  t->w = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+  for (int j = 0; j < n; ++j) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
+    t->w[j] = (double)cosl(a) + I * (double)sinl(a);
+  }
+}
+
+static void fft_rec(const fft_table* t, int n, const cplx* in, size_t is, cplx* out, cplx* tmp,
+                    int sign) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int p = 2;
+  while (n % p) ++p;
+  int m = n / p;
+  for (int r = 0; r < p; ++r) fft_rec(t, m, in + r * is, is * p, tmp + r * m, out + r * m, sign);
+  int step = t->n / n;
+  for (int k = 0; k < m; ++k) {
+    for (int qq = 0; qq < p; ++qq) {
+      int kk = k + qq * m;
+      cplx s = 0;
+      for (int r = 0; r < p; ++r) {
+        int e = (int)(((long)r * kk) % n) * step;
+        cplx w = t->w[e];
+        if (sign > 0) w = conj(w);
+        s += w * tmp[r * m + k];
+      }
+      out[kk] = s;
+    }
+  }
+  /* copy back sub-results are consumed; tmp reused by caller */
+}
+
+static void fft1d(const fft_table* t, cplx* data, size_t stride, cplx* buf, int sign) {
+  int n = t->n;
+  cplx* a = buf; cplx* b = buf + n; cplx* c = buf + 2 * n;
+  for (int i = 0; i < n; ++i) a[i] = data[(size_t)i * stride];
+  fft_rec(t, n, a, 1, b, c, sign);
+  for (int i = 0; i < n; ++i) data[(size_t)i * stride] = b[i];
+}
+
+/* 3-D R2C, unnormalised, e^{-i}: dfftw_plan_dft_r2c_3d semantics */
+void oracle_fft_r2c(int nx, int ny, int nz, const double* in, cplx* out) {
+  int nxh = nx / 2 + 1;
+  fft_table tx, ty, tz;
+  fft_table_init(&tx, nx); fft_table_init(&ty, ny); fft_table_init(&tz, nz);
+  int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+#pragma omp parallel
+  {
+    cplx* buf = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)nmax);
+    cplx* row = (cplx*)malloc(sizeof(cplx) * (size_t)nx);
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)ny * nz; ++l) {
+      for (int i = 0; i < nx; ++i) row[i] = in[(size_t)l * nx + i];
+      fft1d(&tx, row, 1, buf, -1);
+      for (int i = 0; i < nxh; ++i) out[(size_t)l * nxh + i] = row[i];
+    }
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nxh * nz; ++l) {
+      int ix = l % nxh, iz = l / nxh;
+      fft1d(&ty, out + (size_t)iz * nxh * ny + ix, nxh, buf, -1);
+    }
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nxh * ny; ++l) fft1d(&tz, out + l, (size_t)nxh * ny, buf, -1);
+    free(buf); free(row);
+  }
+  free(tx.w); free(ty.w); free(tz.w);
+}
+
+/* 3-D C2R scaled by 1/(nx ny nz) (fftbk_*: execute then *renor).  Destroys in. */
+void oracle_fft_c2r(int nx, int ny, int nz, cplx* in, double* out) {
+  int nxh = nx / 2 + 1;
+  double renor = 1.0 / ((double)nx * ny * nz);
+  fft_table tx, ty, tz;
+  fft_table_init(&tx, nx); fft_table_init(&ty, ny); fft_table_init(&tz, nz);
+  int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+#pragma omp parallel
+  {
+    cplx* buf = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)nmax);
+    cplx* row = (cplx*)malloc(sizeof(cplx) * (size_t)nx);
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nxh * ny; ++l) fft1d(&tz, in + l, (size_t)nxh * ny, buf, +1);
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nxh * nz; ++l) {
+      int ix = l % nxh, iz = l / nxh;
+      fft1d(&ty, in + (size_t)iz * nxh * ny + ix, nxh, buf, +1);
+    }
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)ny * nz; ++l) {
+      const cplx* h = in + (size_t)l * nxh;
+      for (int i = 0; i < nxh; ++i) row[i] = h[i];
+      for (int i = nxh; i < nx; ++i) row[i] = conj(h[nx - i]);
+      fft1d(&tx, row, 1, buf, +1);
+      for (int i = 0; i < nx; ++i) out[(size_t)l * nx + i] = creal(row[i]) * renor;
+    }
+    free(buf); free(row);
+  }
+  free(tx.w); free(ty.w); free(tz.w);
+}
+
+/* ======================================================================= */
+/*  Context                                                                 */
+/* ======================================================================= */
+static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
+
+static void sum_init(double* acc, int nz) { memset(acc, 0, sizeof(double) * (size_t)nz); }
+static double sum_finish(const double* acc, int nz) {
+  double s = 0;
+  for (int i = 0; i < nz; ++i) s += acc[i];
+  return s;
+}
+
+octx* oracle_create(const ogrid* g, const oparams* p, int mode) {
+  octx* c = (octx*)calloc(1, sizeof(octx));
+  c->g = *g; c->p = *p; c->mode = mode;
+  if (c->p.lsolid) c->p.core_otc = 0; /* If(Lsolid) core4='OT ' (BCN4HeDFT.f90:654) */
+  c->nx = g->nx; c->ny = g->ny; c->nz = g->nz; c->nxh = g->nx / 2 + 1;
+  c->n = (size_t)c->nx * c->ny * c->nz;
+  c->nh = (size_t)c->nxh * c->ny * c->nz;
+  c->hx = 2.0 * fabs(g->xmax) / c->nx;
+  c->hy = 2.0 * fabs(g->ymax) / c->ny;
+  c->hz = 2.0 * fabs(g->zmax) / c->nz;
+  c->dxyz = c->hx * c->hy * c->hz;
+  c->renor = 1.0 / ((double)c->nx * c->ny * c->nz);
+  c->x = dalloc(c->nx); c->y = dalloc(c->ny); c->z = dalloc(c->nz);
+  for (int i = 0; i < c->nx; ++i) c->x[i] = -g->xmax + c->hx * i;
+  for (int i = 0; i < c->ny; ++i) c->y[i] = -g->ymax + c->hy * i;
+  for (int i = 0; i < c->nz; ++i) c->z[i] = -g->zmax + c->hz * i;
+  size_t n = c->n, nh = c->nh;
+  int cw = (mode == MODE_TDDFT) ? 2 : 1;
+  c->den = dalloc(n); c->pot4 = dalloc(n); c->dencg = dalloc(n); c->delj4 = dalloc(n);
+  c->denalf = dalloc(n); c->falfs = dalloc(n); c->intx = dalloc(n); c->inty = dalloc(n);
+  c->intz = dalloc(n); c->ualphas = dalloc(n); c->dxden = dalloc(n); c->dyden = dalloc(n);
+  c->dzden = dalloc(n); c->uimp = dalloc(n); c->uext = dalloc(n); c->work1 = dalloc(n);
+  c->work2 = dalloc(n); c->sto1 = dalloc(n); c->sto2 = dalloc(n); c->sto3 = dalloc(n);
+  c->sto4 = dalloc(n); c->sto5 = dalloc(n); c->sto6 = dalloc(n); c->penalty = dalloc(n);
+  c->psi = dalloc(n * cw); c->timec = dalloc(n * cw); c->pc = dalloc(n * cw); c->q = dalloc(n * cw);
+  c->hpsi = dalloc(n * cw);
+  for (int k = 0; k < 3; ++k) c->psiold[k] = dalloc(n * cw);
+  for (int k = 0; k < 2; ++k) c->hpsiold[k] = dalloc(n * cw);
+  c->workc1 = dalloc(2 * n);
+  c->fden = (cplx*)dalloc(2 * nh); c->wk1 = (cplx*)dalloc(2 * nh); c->wk2 = (cplx*)dalloc(2 * nh);
+  c->wk3 = (cplx*)dalloc(2 * nh); c->workk1 = (cplx*)dalloc(2 * nh);
+  c->fvlj4 = dalloc(nh); c->wcgk = dalloc(nh); c->kalfs = dalloc(nh); c->pmod = dalloc(nh);
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = k;
+  for (int k = 0; k < 3; ++k) { c->ioldr[k] = k; c->ioldv[k] = k; }
+  for (int k = 0; k < 2; ++k) c->iolda[k] = k;
+
+  /* p-grid in FFTW order and |p| (BCN4HeDFT.f90:337-371) */
+  int nx = c->nx, ny = c->ny, nz = c->nz, nxh = c->nxh;
+  double hpx = 1.0 / (nx * c->hx), hpy = 1.0 / (ny * c->hy), hpz = 1.0 / (nz * c->hz);
+  double pmaxx = 1.0 / (2.0 * c->hx), pmaxy = 1.0 / (2.0 * c->hy), pmaxz = 1.0 / (2.0 * c->hz);
+  double* px = dalloc(nx); double* py = dalloc(ny); double* pz = dalloc(nz);
+  for (int i = 1; i <= nx / 2 + 1; ++i) px[i - 1] = hpx * (i - 1);
+  for (int i = nx / 2 + 2; i <= nx; ++i) px[i - 1] = -pmaxx + hpx * (i - (nx / 2 + 1));
+  for (int i = 1; i <= ny / 2 + 1; ++i) py[i - 1] = hpy * (i - 1);
+  for (int i = ny / 2 + 2; i <= ny; ++i) py[i - 1] = -pmaxy + hpy * (i - (ny / 2 + 1));
+  for (int i = 1; i <= nz / 2 + 1; ++i) pz[i - 1] = hpz * (i - 1);
+  for (int i = nz / 2 + 2; i <= nz; ++i) pz[i - 1] = -pmaxz + hpz * (i - (nz / 2 + 1));
+  const double twopi = 2.0 * M_PI;
+  double h4 = p->h4;
+  /* fforma (fforma.f90:36-60, 'OT' branch since core is character(len=2)),
+   * initcg (initcg.f90:314-331) with h_cg = h4 in both branches, kalfs
+   * (BCN4HeDFT.f90:849) */
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < nz; ++iz) {
+    for (int iy = 0; iy < ny; ++iy) {
+      for (int ix = 0; ix < nxh; ++ix) {
+        size_t k = (size_t)ix + (size_t)nxh * (iy + (size_t)ny * iz);
+        double pm = sqrt(px[ix] * px[ix] + py[iy] * py[iy] + pz[iz] * pz[iz]);
+        c->pmod[k] = pm;
+        c->fvlj4[k] = oracle_fvlj_radial(pm, p->eps4, p->sigma4, h4);
+        double w = 1.0;
+        if (pm > 0.0) {
+          double arg = twopi * pm * h4;
+          double aux = sin(arg) - arg * cos(arg);
+          w = (fabs(aux) < 1e-10) ? 1.0 : 3.0 * aux / (arg * arg * arg);
+        }
+        c->wcgk[k] = w;
+        double e = M_PI * p->l * pm;
+        c->kalfs[k] = exp(-e * e);
+      }
+    }
+  }
+  free(px); free(py); free(pz);
+  return c;
+}
+
+void oracle_destroy(octx* c) {
+  if (!c) return;
+  double* list[] = {c->x, c->y, c->z, c->den, c->pot4, c->dencg, c->delj4, c->denalf, c->falfs,
+                    c->intx, c->inty, c->intz, c->ualphas, c->dxden, c->dyden, c->dzden, c->uimp,
+                    c->uext, c->work1, c->work2, c->sto1, c->sto2, c->sto3, c->sto4, c->sto5,
+                    c->sto6, c->penalty, c->psi, c->timec, c->pc, c->q, c->hpsi, c->psiold[0],
+                    c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->workc1,
+                    (double*)c->fden, (double*)c->wk1, (double*)c->wk2, (double*)c->wk3,
+                    (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion};
+  for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); ++i) free(list[i]);
+  free(c);
+}
+
+/* pointer + element count (doubles) of a field, for the Python binding */
+double* oracle_field(octx* c, int id, size_t* ndoubles) {
+  size_t n = c->n, nh = c->nh;
+  int cw = (c->mode == MODE_TDDFT) ? 2 : 1;
+  double* ptr = NULL; size_t cnt = n;
+  switch (id) {
+    case F_DEN: ptr = c->den; break;
+    case F_PSI: ptr = c->psi; cnt = n * cw; break;
+    case F_POT4: ptr = c->pot4; break;
+    case F_DENCG: ptr = c->dencg; break;
+    case F_DELJ4: ptr = c->delj4; break;
+    case F_DENALF: ptr = c->denalf; break;
+    case F_FALFS: ptr = c->falfs; break;
+    case F_INTX: ptr = c->intx; break;
+    case F_INTY: ptr = c->inty; break;
+    case F_INTZ: ptr = c->intz; break;
+    case F_UALPHAS: ptr = c->ualphas; break;
+    case F_DXDEN: ptr = c->dxden; break;
+    case F_DYDEN: ptr = c->dyden; break;
+    case F_DZDEN: ptr = c->dzden; break;
+    case F_UIMP: ptr = c->uimp; break;
+    case F_UEXT: ptr = c->uext; break;
+    case F_TIMEC: ptr = c->timec; cnt = n * cw; break;
+    case F_PSIOLD1: case F_PSIOLD2: case F_PSIOLD3: ptr = c->psiold[id - F_PSIOLD1]; cnt = n * cw; break;
+    case F_HPSIOLD1: case F_HPSIOLD2: ptr = c->hpsiold[id - F_HPSIOLD1]; cnt = n * cw; break;
+    case F_PC: ptr = c->pc; cnt = n * cw; break;
+    case F_Q: ptr = c->q; cnt = n * cw; break;
+    case F_FDEN: ptr = (double*)c->fden; cnt = 2 * nh; break;
+    case F_FVLJ4: ptr = c->fvlj4; cnt = nh; break;
+    case F_WCGK: ptr = c->wcgk; cnt = nh; break;
+    case F_KALFS: ptr = c->kalfs; cnt = nh; break;
+    case F_WORK1: ptr = c->work1; break;
+    case F_WORK2: ptr = c->work2; break;
+    case F_WORKC1: ptr = c->workc1; cnt = 2 * n; break;
+    case F_WORKK1: ptr = (double*)c->workk1; cnt = 2 * nh; break;
+    default: ptr = NULL; cnt = 0;
+  }
+  if (ndoubles) *ndoubles = cnt;
+  return ptr;
+}
+
+/* ---- helpers on the context ------------------------------------------- */
+static void deriv(octx* c, int id, int iv, const double* f, double* df, int ncomp) {
+  double h = (iv == 1) ? c->hx : (iv == 2) ? c->hy : c->hz;
+  oracle_derivnd(id, c->nx, c->ny, c->nz, h, iv, f, df, c->p.icon, ncomp);
+}
+static void fftfw(octx* c, const double* in, cplx* out) { oracle_fft_r2c(c->nx, c->ny, c->nz, in, out); }
+static void fftbk(octx* c, cplx* in, double* out) { oracle_fft_c2r(c->nx, c->ny, c->nz, in, out); }
+static void kmul(octx* c, const cplx* a, const double* k, cplx* out) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->nh; ++i) out[i] = a[i] * k[i];
+}
+
+/* derden.f90:5-25 */
+void oracle_derden(octx* c) {
+  deriv(c, 1, 1, c->den, c->dxden, 1);
+  deriv(c, 1, 2, c->den, c->dyden, 1);
+  deriv(c, 1, 3, c->den, c->dzden, 1);
+}
+
+/* term_alfa: He_statics/term_alfa.f90:5-102 (= Dynamics_TDDFT/term_alfa.f90:5-117) */
+void oracle_term_alfa(octx* c) {
+  size_t n = c->n;
+  const oparams* p = &c->p;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    c->falfs[i] = 1.0 - c->denalf[i] / p->den0s;
+    c->sto1[i] = c->falfs[i] * c->dxden[i];
+    c->sto2[i] = c->falfs[i] * c->dyden[i];
+    c->sto3[i] = c->falfs[i] * c->dzden[i];
+  }
+  fftfw(c, c->sto1, c->wk1); fftfw(c, c->sto2, c->wk2); fftfw(c, c->sto3, c->wk3);
+  kmul(c, c->wk1, c->kalfs, c->wk1); kmul(c, c->wk2, c->kalfs, c->wk2); kmul(c, c->wk3, c->kalfs, c->wk3);
+  fftbk(c, c->wk1, c->intx); fftbk(c, c->wk2, c->inty); fftbk(c, c->wk3, c->intz);
+  if (!p->core_otc) return;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i)
+    c->sto1[i] = c->dxden[i] * c->intx[i] + c->dyden[i] * c->inty[i] + c->dzden[i] * c->intz[i];
+  fftfw(c, c->sto1, c->wk1);
+  kmul(c, c->wk1, c->kalfs, c->wk1);
+  fftbk(c, c->wk1, c->ualphas);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    c->ualphas[i] = c->ualphas[i] / p->den0s;
+    c->sto1[i] = c->falfs[i] * c->intx[i];
+    c->sto2[i] = c->falfs[i] * c->inty[i];
+    c->sto3[i] = c->falfs[i] * c->intz[i];
+  }
+  deriv(c, 1, 1, c->sto1, c->sto4, 1);
+  deriv(c, 1, 2, c->sto2, c->sto5, 1);
+  deriv(c, 1, 3, c->sto3, c->sto6, 1);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i)
+    c->ualphas[i] = (c->ualphas[i] + c->sto4[i] + c->sto5[i] + c->sto6[i]) * p->h2o2m4 * p->alphas;
+}
+
+/* correlation + LJ + solid + alpha_s assembly common to both poten variants:
+ * He_statics/poten.f90:27-85,290-319 and Dynamics_TDDFT/poten.f90:150-218 */
+static void poten_tail(octx* c) {
+  size_t n = c->n;
+  const oparams* p = &c->p;
+  kmul(c, c->fden, c->fvlj4, c->wk1);
+  fftbk(c, c->wk1, c->delj4);
+  double a0 = p->cp4 / 2.0, a1 = p->cpp4 / 3.0;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i)
+    c->sto1[i] = c->den[i] * c->dencg[i] * (p->cp4 + p->cpp4 * c->dencg[i]);
+  fftfw(c, c->sto1, c->wk1);
+  kmul(c, c->wk1, c->wcgk, c->wk2);
+  fftbk(c, c->wk2, c->sto1);
+  if (p->lsolid) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+      double t = tanh(p->beta_solid * (c->den[i] - p->den_m));
+      c->penalty[i] = p->csolid * (1.0 + t + p->beta_solid * c->den[i] * (1.0 - t * t));
+    }
+  }
+  int ext = (c->mode == MODE_STATICS) && p->lexternal;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double cg = c->dencg[i];
+    c->pot4[i] = c->delj4[i] + cg * cg * (a0 + a1 * cg) + c->sto1[i] + (ext ? c->uext[i] : 0.0);
+  }
+  if (p->core_otc) {
+    if (c->mode == MODE_TDDFT) {
+      /* Dynamics_TDDFT/poten.f90:200-206: denalf from the fresh fden */
+      kmul(c, c->fden, c->kalfs, c->wk1);
+      fftbk(c, c->wk1, c->denalf);
+    }
+    oracle_term_alfa(c);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) c->pot4[i] += c->ualphas[i];
+  }
+  if (p->lsolid) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) c->pot4[i] += c->penalty[i];
+  }
+}
+
+/* poten(): statics (He_statics/poten.f90:5-221, state prepared by the caller)
+ * or TDDFT (Dynamics_TDDFT/poten.f90:124-221, recomputed from den). */
+void oracle_poten(octx* c) {
+  if (c->mode == MODE_TDDFT) {
+    fftfw(c, c->den, c->fden);
+    kmul(c, c->fden, c->wcgk, c->wk1);
+    fftbk(c, c->wk1, c->dencg);
+    oracle_derden(c);
+  }
+  poten_tail(c);
+}
+
+/* the statics driver's refresh after a step (BCN4HeDFT.f90:1131-1149):
+ * derden, fden = F(den), dencg, denalf */
+void oracle_statics_refresh(octx* c) {
+  oracle_derden(c);
+  fftfw(c, c->den, c->fden);
+  kmul(c, c->fden, c->wcgk, c->wk1);
+  fftbk(c, c->wk1, c->dencg);
+  if (c->p.core_otc) {
+    kmul(c, c->fden, c->kalfs, c->wk2);
+    fftbk(c, c->wk2, c->denalf);
+  }
+}
+
+/* ======================================================================= */
+/*  Statics integrators (real psi)                                          */
+/* ======================================================================= */
+static const double RK_A[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 1.0 / 6.0};
+static const double RK_B[4] = {2.0, 1.0, 1.0, 2.0};
+static const double RK_C[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 0.5};
+
+/* hpsi = -(lap psi) h2o2m4 + pot4 psi; returns sum(psi*hpsi) (per plane order) */
+static double hpsi_real(octx* c, const double* psi, double* hpsi) {
+  deriv(c, 2, 1, psi, c->sto1, 1);
+  deriv(c, 2, 2, psi, c->sto2, 1);
+  deriv(c, 2, 3, psi, c->sto3, 1);
+  size_t plane = (size_t)c->nx * c->ny;
+  double* acc = dalloc(c->nz);
+  double h = c->p.h2o2m4;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < c->nz; ++iz) {
+    double s = 0;
+    for (size_t j = 0; j < plane; ++j) {
+      size_t i = iz * plane + j;
+      hpsi[i] = -(c->sto1[i] + c->sto2[i] + c->sto3[i]) * h + c->pot4[i] * psi[i];
+      s += psi[i] * hpsi[i];
+    }
+    acc[iz] = s;
+  }
+  double s = sum_finish(acc, c->nz);
+  free(acc);
+  return s;
+}
+
+static double sumsq_plane(const octx* c, const double* a) {
+  size_t plane = (size_t)c->nx * c->ny;
+  double* acc = dalloc(c->nz);
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < c->nz; ++iz) {
+    double s = 0;
+    for (size_t j = 0; j < plane; ++j) { double v = a[iz * plane + j]; s += v * v; }
+    acc[iz] = s;
+  }
+  double s = sum_finish(acc, c->nz);
+  free(acc);
+  return s;
+}
+
+/* psi = |psi| * sqrt(n4/(sum psi^2 dxyz)), psimin clamp, den = psi^2 */
+static void renorm_real(octx* c, double n4, int clamp_before) {
+  size_t n = c->n;
+  double pmin = c->p.psimin;
+  if (clamp_before && pmin > 0.0) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) if (c->psi[i] < pmin) c->psi[i] = pmin;
+  }
+  double cnorm = sqrt(n4 / (sumsq_plane(c, c->psi) * c->dxyz));
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double v = fabs(c->psi[i]) * cnorm;
+    if (!clamp_before && pmin > 0.0 && v < pmin) v = pmin;
+    c->psi[i] = v;
+    c->den[i] = v * v;
+  }
+}
+
+/* STEPRKR (He_statics/steprkr.f90:1-96) */
+void oracle_steprkr(octx* c, double deltat, double* mu4, double* mu4err, double n4) {
+  size_t n = c->n;
+  double hmean = 0.0, merr = 0.0;
+  for (int jrun = 0; jrun < 4; ++jrun) {
+    merr = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
+    if (jrun == 0) {
+      hmean = merr;
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < (long)n; ++i)
+        c->timec[i] = deltat / (1.0 + exp((fabs(c->hpsi[i] / c->psi[i]) - c->p.dmumax) / c->p.sdmu)) +
+                      c->p.dtmin;
+    }
+    double a = RK_A[jrun], b = RK_B[jrun], cc = RK_C[jrun];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+      double s4 = -(c->hpsi[i] - merr * c->psi[i]);
+      double s1 = a * (s4 - b * c->q[i]);
+      c->q[i] = c->q[i] + 3.0 * s1 - cc * s4;
+      if (jrun == 0) {
+        c->hpsiold[1][i] = c->hpsiold[0][i];
+        c->hpsiold[0][i] = s4;
+        c->psiold[2][i] = c->psiold[1][i];
+        c->psiold[1][i] = c->psiold[0][i];
+        c->psiold[0][i] = c->psi[i];
+      }
+      c->psi[i] = c->psi[i] + c->timec[i] * s1;
+    }
+    renorm_real(c, n4, 1);
+    if (jrun <= 2) oracle_poten(c);
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = k;
+  *mu4err = fabs(1.0 - *mu4 / hmean);
+  *mu4 = hmean;
+}
+
+/* STEPPCR (He_statics/steppcr.f90:1-108) */
+void oracle_steppcr(octx* c, double deltat, double* mu4, double* mu4err, double n4, double* errpc) {
+  size_t n = c->n;
+  const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
+  double hmean = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
+  double* po3 = c->psiold[c->ioldp[2]];
+  double* ho1 = c->hpsiold[c->ioldh[0]];
+  double* ho2 = c->hpsiold[c->ioldh[1]];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double tc = deltat / (1.0 + exp((fabs(c->hpsi[i] / c->psi[i]) - c->p.dmumax) / c->p.sdmu)) + c->p.dtmin;
+    c->timec[i] = tc;
+    double s4 = -(c->hpsi[i] - hmean * c->psi[i]);
+    double s1 = po3[i] + c4o3 * tc * (2.0 * s4 - ho1[i] + 2.0 * ho2[i]);
+    po3[i] = c->psi[i];
+    c->psi[i] = s1 - c112 * c->pc[i];
+    c->pc[i] = s1;
+    ho2[i] = s4;
+  }
+  renorm_real(c, n4, 0);
+  { int t = c->ioldh[1]; c->ioldh[1] = c->ioldh[0]; c->ioldh[0] = t; }
+  oracle_poten(c);
+  double merr = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
+  po3 = c->psiold[c->ioldp[2]];
+  double* po2 = c->psiold[c->ioldp[1]];
+  ho1 = c->hpsiold[c->ioldh[0]];
+  ho2 = c->hpsiold[c->ioldh[1]];
+  size_t plane = (size_t)c->nx * c->ny;
+  double* acc = dalloc(c->nz);
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < c->nz; ++iz) {
+    double s = 0;
+    for (size_t j = 0; j < plane; ++j) {
+      size_t i = iz * plane + j;
+      double s4 = -(c->hpsi[i] - merr * c->psi[i]);
+      double s5 = 0.125 * (9.0 * po3[i] - po2[i] + 3.0 * c->timec[i] * (s4 + 2.0 * ho1[i] - ho2[i]));
+      c->pc[i] = c->pc[i] - s5;
+      c->psi[i] = s5 + c9 * c->pc[i];
+      s += fabs(c->pc[i]);
+    }
+    acc[iz] = s;
+  }
+  double epc = c9 * sum_finish(acc, c->nz);
+  free(acc);
+  renorm_real(c, n4, 0);
+  epc /= (double)n;
+  { int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t; }
+  *mu4err = fabs(1.0 - *mu4 / hmean);
+  *mu4 = hmean;
+  if (errpc) *errpc = epc;
+}
+
+/* ======================================================================= */
+/*  Classical impurities (Dynamics_TDDFT/potenimp.f90)                      */
+/* ======================================================================= */
+void oracle_set_impurities(octx* c, int nimp, int npot, double delinter, double rmax,
+                           const double* potion, const double* mass) {
+  c->nimp = nimp; c->npot = npot; c->delinter = delinter; c->rmaxinterpol = rmax;
+  free(c->potion);
+  c->potion = dalloc((size_t)nimp * npot);
+  memcpy(c->potion, potion, sizeof(double) * (size_t)nimp * npot);
+  for (int k = 0; k < nimp; ++k) c->mimp[k] = mass[k];
+}
+
+/* rimp etc. are (nimp,3) Fortran arrays: element (k,a) at k + nimp*a */
+void oracle_set_impurity_state(octx* c, const double* r, const double* v, const double* a) {
+  int m = c->nimp * 3;
+  if (r) memcpy(c->rimp, r, sizeof(double) * m);
+  if (v) memcpy(c->vimp, v, sizeof(double) * m);
+  if (a) memcpy(c->aimp, a, sizeof(double) * m);
+}
+void oracle_get_impurity_state(octx* c, double* r, double* v, double* a, double* f, int* lstop) {
+  int m = c->nimp * 3;
+  if (r) memcpy(r, c->rimp, sizeof(double) * m);
+  if (v) memcpy(v, c->vimp, sizeof(double) * m);
+  if (a) memcpy(a, c->aimp, sizeof(double) * m);
+  if (f) memcpy(f, c->force, sizeof(double) * m);
+  if (lstop) *lstop = c->lstopimp;
+}
+
+/* updatepoten (potenimp.f90:83-123): linear interpolation of each table at
+ * |r - R_k|; values beyond rmaxinterpol are clamped to it and flagged */
+static double uimp_k_at(octx* c, int k, double xx, double yy, double zz) {
+  double rx = c->rimp[k], ry = c->rimp[k + c->nimp], rz = c->rimp[k + 2 * c->nimp];
+  double zt = (zz - rz) * (zz - rz);
+  double yt = (yy - ry) * (yy - ry) + zt;
+  double r = sqrt((xx - rx) * (xx - rx) + yt);
+  if (r > c->rmaxinterpol) r = c->rmaxinterpol;
+  int ir = (int)(r / c->delinter); /* 0-based: reference ir-1 */
+  if (ir > c->npot - 2) ir = c->npot - 2;
+  double rmod = fmod(r, c->delinter) / c->delinter;
+  const double* tab = c->potion + (size_t)k * c->npot;
+  return tab[ir] * (1.0 - rmod) + tab[ir + 1] * rmod;
+}
+
+void oracle_potenimp(octx* c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  int flag = 0;
+  for (int k = 0; k < c->nimp; ++k) {
+    double rx = c->rimp[k], ry = c->rimp[k + c->nimp], rz = c->rimp[k + 2 * c->nimp];
+    for (int iz = 0; iz < nz; iz += nz - 1)
+      for (int iy = 0; iy < ny; iy += ny - 1)
+        for (int ix = 0; ix < nx; ix += nx - 1) {
+          double dx = c->x[ix] - rx, dy = c->y[iy] - ry, dz = c->z[iz] - rz;
+          if (sqrt(dx * dx + dy * dy + dz * dz) > c->rmaxinterpol) flag = 1;
+        }
+  }
+  if (flag) c->lstopimp = 1;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        double u = 0;
+        for (int k = 0; k < c->nimp; ++k) u += uimp_k_at(c, k, c->x[ix], c->y[iy], c->z[iz]);
+        c->uimp[ix + (size_t)nx * (iy + (size_t)ny * iz)] = u;
+      }
+}
+
+/* forceimp (potenimp.f90:23-67) without impurity-impurity pair terms */
+void oracle_forceimp(octx* c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int k = 0; k < c->nimp; ++k) {
+    double* acc = dalloc(3 * (size_t)nz);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < nz; ++iz) {
+      double s0 = 0, s1 = 0, s2 = 0;
+      for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+          size_t i = ix + (size_t)nx * (iy + (size_t)ny * iz);
+          double u = uimp_k_at(c, k, c->x[ix], c->y[iy], c->z[iz]);
+          s0 += c->dxden[i] * u; s1 += c->dyden[i] * u; s2 += c->dzden[i] * u;
+        }
+      acc[3 * iz] = s0; acc[3 * iz + 1] = s1; acc[3 * iz + 2] = s2;
+    }
+    double f[3] = {0, 0, 0};
+    for (int iz = 0; iz < nz; ++iz) for (int a = 0; a < 3; ++a) f[a] += acc[3 * iz + a];
+    free(acc);
+    for (int a = 0; a < 3; ++a) c->force[k + c->nimp * a] = -f[a] * c->dxyz;
+  }
+}
+
+void oracle_update_aimp(octx* c) {
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < c->nimp; ++k) c->aimp[k + c->nimp * a] = c->force[k + c->nimp * a] / c->mimp[k];
+}
+
+/* ======================================================================= */
+/*  TDDFT integrators (complex psi)                                         */
+/* ======================================================================= */
+/* k = timec*((lap psi)*h2o2m4 - pot4*psi) - i*uimp*psi (steprk.f90:67) into out */
+static void hpsi_complex(octx* c, const cplx* psi, cplx* out) {
+  deriv(c, 2, 1, (const double*)psi, c->workc1, 2);
+  cplx* lap = (cplx*)dalloc(2 * c->n);
+  memcpy(lap, c->workc1, sizeof(cplx) * c->n);
+  deriv(c, 2, 2, (const double*)psi, c->workc1, 2);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) lap[i] += ((cplx*)c->workc1)[i];
+  deriv(c, 2, 3, (const double*)psi, c->workc1, 2);
+  const cplx* tc = (const cplx*)c->timec;
+  double h = c->p.h2o2m4;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) {
+    cplx l = lap[i] + ((cplx*)c->workc1)[i];
+    out[i] = tc[i] * (l * h - c->pot4[i] * psi[i]) - I * c->uimp[i] * psi[i];
+  }
+  free(lap);
+}
+
+static void den_from_psi(octx* c) {
+  const cplx* psi = (const cplx*)c->psi;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) {
+    double a = cabs(psi[i]);
+    c->den[i] = a * a;
+  }
+}
+
+/* STEPRK (Dynamics_TDDFT/steprk.f90:1-144) */
+void oracle_steprk(octx* c, double deltat) {
+  size_t n = c->n;
+  cplx* psi = (cplx*)c->psi;
+  cplx* q = (cplx*)c->q;
+  cplx* k4 = (cplx*)c->hpsi;
+  cplx* po[3] = {(cplx*)c->psiold[0], (cplx*)c->psiold[1], (cplx*)c->psiold[2]};
+  cplx* ho[2] = {(cplx*)c->hpsiold[0], (cplx*)c->hpsiold[1]};
+  int m = c->nimp * 3;
+  for (int jrun = 0; jrun < 4; ++jrun) {
+    hpsi_complex(c, psi, k4);
+    double a = RK_A[jrun], b = RK_B[jrun], cc = RK_C[jrun];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+      cplx s1 = a * (k4[i] - b * q[i]);
+      q[i] = q[i] + 3.0 * s1 - cc * k4[i];
+      if (jrun == 0) {
+        ho[1][i] = ho[0][i]; ho[0][i] = k4[i];
+        po[2][i] = po[1][i]; po[1][i] = po[0][i]; po[0][i] = psi[i];
+      }
+      psi[i] = psi[i] + deltat * s1;
+    }
+    den_from_psi(c);
+    /* impurity positions and velocities, same scheme (steprk.f90:330-354) */
+    for (int i = 0; i < m; ++i) {
+      double st = a * (c->vimp[i] - b * c->qr[i]);
+      c->qr[i] = c->qr[i] + 3.0 * st - cc * c->vimp[i];
+      if (jrun == 0) {
+        c->rimpold[2][i] = c->rimpold[1][i]; c->rimpold[1][i] = c->rimpold[0][i];
+        c->rimpold[0][i] = c->rimp[i];
+      }
+      c->rimp[i] = c->rimp[i] + deltat * st;
+    }
+    for (int i = 0; i < m; ++i) {
+      double st = a * (c->aimp[i] - b * c->qv[i]);
+      c->qv[i] = c->qv[i] + 3.0 * st - cc * c->aimp[i];
+      if (jrun == 0) {
+        c->aimpold[1][i] = c->aimpold[0][i]; c->aimpold[0][i] = c->aimp[i];
+        c->vimpold[2][i] = c->vimpold[1][i]; c->vimpold[1][i] = c->vimpold[0][i];
+        c->vimpold[0][i] = c->vimp[i];
+      }
+      c->vimp[i] = c->vimp[i] + deltat * st;
+    }
+    if (jrun <= 2) {
+      oracle_potenimp(c);
+      oracle_poten(c);
+      oracle_forceimp(c);
+    }
+  }
+  for (int k = 0; k < 3; ++k) { c->ioldp[k] = k; c->ioldr[k] = k; c->ioldv[k] = k; }
+  for (int k = 0; k < 2; ++k) { c->ioldh[k] = k; c->iolda[k] = k; }
+}
+
+/* STEPPC (Dynamics_TDDFT/steppc.f90:1-180) */
+void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double* errvimp) {
+  size_t n = c->n;
+  const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
+  cplx* psi = (cplx*)c->psi;
+  cplx* pc = (cplx*)c->pc;
+  cplx* k4 = (cplx*)c->hpsi;
+  int m = c->nimp * 3;
+  hpsi_complex(c, psi, k4);
+  {
+    cplx* po3 = (cplx*)c->psiold[c->ioldp[2]];
+    cplx* ho1 = (cplx*)c->hpsiold[c->ioldh[0]];
+    cplx* ho2 = (cplx*)c->hpsiold[c->ioldh[1]];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+      cplx s1 = po3[i] + c4o3 * deltat * (2.0 * k4[i] - ho1[i] + 2.0 * ho2[i]);
+      po3[i] = psi[i];
+      psi[i] = s1 - c112 * pc[i];
+      pc[i] = s1;
+      ho2[i] = k4[i];
+    }
+    den_from_psi(c);
+    int t = c->ioldh[1]; c->ioldh[1] = c->ioldh[0]; c->ioldh[0] = t;
+  }
+  /* impurity predictor + modifier (steppc.f90:461-480) */
+  {
+    double st[MAXIMP * 3];
+    for (int i = 0; i < m; ++i)
+      st[i] = c->rimpold[c->ioldr[2]][i] +
+              c4o3 * deltat * (2.0 * c->vimp[i] - c->vimpold[c->ioldv[0]][i] + 2.0 * c->vimpold[c->ioldv[1]][i]);
+    for (int i = 0; i < m; ++i) {
+      c->rimpold[c->ioldr[2]][i] = c->rimp[i];
+      c->rimp[i] = st[i] - c112 * c->pcr[i];
+      c->pcr[i] = st[i];
+    }
+    for (int i = 0; i < m; ++i)
+      st[i] = c->vimpold[c->ioldv[2]][i] +
+              c4o3 * deltat * (2.0 * c->aimp[i] - c->aimpold[c->iolda[0]][i] + 2.0 * c->aimpold[c->iolda[1]][i]);
+    for (int i = 0; i < m; ++i) {
+      c->vimpold[c->ioldv[2]][i] = st[i] - c112 * c->pcv[i];
+      c->pcv[i] = st[i];
+    }
+    for (int i = 0; i < m; ++i) c->aimpold[c->iolda[1]][i] = c->aimp[i];
+    int t = c->iolda[1]; c->iolda[1] = c->iolda[0]; c->iolda[0] = t;
+  }
+  oracle_potenimp(c);
+  oracle_poten(c);
+  oracle_forceimp(c);
+  hpsi_complex(c, psi, k4);
+  double ehe;
+  {
+    cplx* po3 = (cplx*)c->psiold[c->ioldp[2]];
+    cplx* po2 = (cplx*)c->psiold[c->ioldp[1]];
+    cplx* ho1 = (cplx*)c->hpsiold[c->ioldh[0]];
+    cplx* ho2 = (cplx*)c->hpsiold[c->ioldh[1]];
+    size_t plane = (size_t)c->nx * c->ny;
+    double* acc = dalloc(c->nz);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < c->nz; ++iz) {
+      double s = 0;
+      for (size_t j = 0; j < plane; ++j) {
+        size_t i = iz * plane + j;
+        cplx s5 = 0.125 * (9.0 * po3[i] - po2[i] + 3.0 * deltat * (k4[i] + 2.0 * ho1[i] - ho2[i]));
+        pc[i] = pc[i] - s5;
+        psi[i] = s5 + c9 * pc[i];
+        s += cabs(pc[i]);
+      }
+      acc[iz] = s;
+    }
+    ehe = c9 * sum_finish(acc, c->nz) / (double)n;
+    free(acc);
+    den_from_psi(c);
+    int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t;
+  }
+  /* impurity corrector (steppc.f90:530-555) */
+  double eimp = 0, evimp = 0;
+  {
+    double st[MAXIMP * 3];
+    for (int i = 0; i < m; ++i)
+      st[i] = 0.125 * (9.0 * c->rimpold[c->ioldr[2]][i] - c->rimpold[c->ioldr[1]][i] +
+                       3.0 * deltat * (c->vimpold[c->ioldv[2]][i] + 2.0 * c->vimp[i] - c->vimpold[c->ioldv[0]][i]));
+    for (int i = 0; i < m; ++i) {
+      c->pcr[i] = c->pcr[i] - st[i];
+      c->rimp[i] = st[i] + c9 * c->pcr[i];
+      eimp += fabs(c9 * c->pcr[i]);
+    }
+    eimp = eimp * 0.3333333333 / c->nimp;
+    { int t = c->ioldr[2]; c->ioldr[2] = c->ioldr[1]; c->ioldr[1] = c->ioldr[0]; c->ioldr[0] = t; }
+    for (int i = 0; i < m; ++i)
+      st[i] = 0.125 * (9.0 * c->vimp[i] - c->vimpold[c->ioldv[1]][i] +
+                       3.0 * deltat * (c->aimp[i] + 2.0 * c->aimpold[c->iolda[0]][i] - c->aimpold[c->iolda[1]][i]));
+    for (int i = 0; i < m; ++i) {
+      c->pcv[i] = c->pcv[i] - st[i];
+      c->vimpold[c->ioldv[2]][i] = c->vimp[i];
+      c->vimp[i] = st[i] + c9 * c->pcv[i];
+      evimp += fabs(c9 * c->pcv[i]);
+    }
+    evimp = evimp * 0.3333333333 / c->nimp;
+    { int t = c->ioldv[2]; c->ioldv[2] = c->ioldv[1]; c->ioldv[1] = c->ioldv[0]; c->ioldv[0] = t; }
+  }
+  if (errHe) *errHe = ehe;
+  if (errimp) *errimp = eimp;
+  if (errvimp) *errvimp = evimp;
+}
+
+/* ======================================================================= */
+/*  Energies                                                                */
+/* ======================================================================= */
+/* out: ekin4, elj4, ecor4, ealphas, esolid, eext, etot4, ekinx, eHeX,
+ *      eimpu_impu, eimpu, etot, norm  (struct hedft_energies order) */
+void oracle_energy(octx* c, double* out) {
+  const oparams* p = &c->p;
+  int ncomp = (c->mode == MODE_TDDFT) ? 2 : 1;
+  double* d1 = dalloc(c->n * ncomp);
+  double* d2 = dalloc(c->n * ncomp);
+  double* d3 = dalloc(c->n * ncomp);
+  deriv(c, 1, 1, c->psi, d1, ncomp);
+  deriv(c, 1, 2, c->psi, d2, ncomp);
+  deriv(c, 1, 3, c->psi, d3, ncomp);
+  double a1 = 0.5 * p->cp4, a2 = p->cpp4 / 3.0;
+  size_t plane = (size_t)c->nx * c->ny;
+  int nz = c->nz;
+  double* acc = dalloc(8 * (size_t)nz);
+  int tddft = (c->mode == MODE_TDDFT);
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < nz; ++iz) {
+    double ek = 0, elj = 0, ec = 0, ea = 0, es = 0, ex = 0, eh = 0, nr = 0;
+    for (size_t j = 0; j < plane; ++j) {
+      size_t i = iz * plane + j;
+      double d = c->den[i], cg = c->dencg[i];
+      double g = 0;
+      for (int q = 0; q < ncomp; ++q) {
+        double v1 = d1[i * ncomp + q], v2 = d2[i * ncomp + q], v3 = d3[i * ncomp + q];
+        g += v1 * v1 + v2 * v2 + v3 * v3;
+      }
+      ek += g;
+      elj += c->delj4[i] * d;
+      ec += d * cg * cg * (a1 + a2 * cg);
+      ea += c->falfs[i] * (c->dxden[i] * c->intx[i] + c->dyden[i] * c->inty[i] + c->dzden[i] * c->intz[i]);
+      es += d * (1.0 + tanh(p->beta_solid * (d - p->den_m)));
+      ex += c->uext[i] * d;
+      eh += c->uimp[i] * d;
+      nr += d;
+    }
+    double* a = acc + 8 * iz;
+    a[0] = ek; a[1] = elj; a[2] = ec; a[3] = ea; a[4] = es; a[5] = ex; a[6] = eh; a[7] = nr;
+  }
+  double s[8] = {0};
+  for (int iz = 0; iz < nz; ++iz) for (int k = 0; k < 8; ++k) s[k] += acc[8 * iz + k];
+  free(acc); free(d1); free(d2); free(d3);
+  double dv = c->dxyz;
+  double ekin4 = p->h2o2m4 * s[0] * dv, elj4 = 0.5 * s[1] * dv, ecor4 = s[2] * dv;
+  double ealphas = p->core_otc ? -p->h2o2m4 * 0.5 * p->alphas * s[3] * dv : 0.0;
+  double esolid = p->lsolid ? p->csolid * s[4] * dv : 0.0;
+  double etot4 = ekin4 + elj4 + ecor4 + esolid + ealphas;
+  double eext = 0, ekinx = 0, eHeX = 0, eimpu = 0, etot;
+  if (tddft) {
+    eext = s[5] * dv;
+    etot4 += eext;
+    for (int k = 0; k < c->nimp; ++k) {
+      double v2 = 0;
+      for (int a = 0; a < 3; ++a) { double v = c->vimp[k + c->nimp * a]; v2 += v * v; }
+      ekinx += 0.5 * c->mimp[k] * v2;
+    }
+    eHeX = s[6] * dv;
+    eimpu = ekinx + eHeX;
+    etot = etot4 + eimpu;
+  } else {
+    eext = p->lexternal ? s[5] * dv : 0.0;
+    eimpu = eext;
+    etot = etot4 + eimpu;
+  }
+  out[0] = ekin4; out[1] = elj4; out[2] = ecor4; out[3] = ealphas; out[4] = esolid; out[5] = eext;
+  out[6] = etot4; out[7] = ekinx; out[8] = eHeX; out[9] = 0.0; out[10] = eimpu; out[11] = etot;
+  out[12] = s[7] * dv;
+}
+
+/* ======================================================================= */
+/*  Driver glue                                                             */
+/* ======================================================================= */
+/* BCN4HeDFT.f90:724-874: normalise, psi = sqrt(den), fden, dencg, denalf,
+ * derden, term_alfa, poten; mu4 = sum(psi*hpsi)*dxyz/n4 (evolo(0,...)) */
+void oracle_statics_init(octx* c, double n4, double* mu4) {
+  size_t n = c->n;
+  double s = 0;
+  {
+    size_t plane = (size_t)c->nx * c->ny;
+    double* acc = dalloc(c->nz);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < c->nz; ++iz) {
+      double t = 0;
+      for (size_t j = 0; j < plane; ++j) t += c->den[iz * plane + j];
+      acc[iz] = t;
+    }
+    s = sum_finish(acc, c->nz);
+    free(acc);
+  }
+  double cnorm = n4 / (s * c->dxyz);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    c->den[i] *= cnorm;
+    c->psi[i] = sqrt(c->den[i]);
+  }
+  fftfw(c, c->den, c->fden);
+  kmul(c, c->fden, c->wcgk, c->wk1);
+  fftbk(c, c->wk1, c->dencg);
+  if (c->p.core_otc) {
+    kmul(c, c->fden, c->kalfs, c->wk1);
+    fftbk(c, c->wk1, c->denalf);
+  }
+  oracle_derden(c);
+  if (c->p.core_otc) oracle_term_alfa(c);
+  oracle_poten(c);
+  *mu4 = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
+}
+
+void oracle_statics_iteration(octx* c, double deltat, int iterrk, double n4, double* mu4,
+                              double* mu4err, double* errpc) {
+  if (iterrk <= 4) oracle_steprkr(c, deltat, mu4, mu4err, n4);
+  else oracle_steppcr(c, deltat, mu4, mu4err, n4, errpc);
+  oracle_statics_refresh(c);
+  oracle_poten(c);
+}
+
+/* DFT4He3d.f90:525-601 for mode 0 with no initial boost */
+void oracle_tddft_init(octx* c) {
+  size_t n = c->n;
+  cplx* psi = (cplx*)c->psi;
+  cplx* tc = (cplx*)c->timec;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) psi[i] = sqrt(c->den[i]);
+  den_from_psi(c);
+  const oparams* p = &c->p;
+  for (int iz = 0; iz < c->nz; ++iz)
+    for (int iy = 0; iy < c->ny; ++iy)
+      for (int ix = 0; ix < c->nx; ++ix) {
+        double lam = p->lambdah * (1.0 + tanh((fabs(c->x[ix]) - p->tmean[0]) / p->tsurf[0]) + 1.0 +
+                                   tanh((fabs(c->y[iy]) - p->tmean[1]) / p->tsurf[1]) + 1.0 +
+                                   tanh((fabs(c->z[iz]) - p->tmean[2]) / p->tsurf[2]));
+        tc[ix + (size_t)c->nx * (iy + (size_t)c->ny * iz)] = lam + I * 1.0;
+      }
+  oracle_potenimp(c);
+  oracle_poten(c);
+  oracle_forceimp(c);
+  oracle_update_aimp(c);
+  memset(c->qr, 0, sizeof c->qr); memset(c->qv, 0, sizeof c->qv);
+  memset(c->rimpold, 0, sizeof c->rimpold); memset(c->vimpold, 0, sizeof c->vimpold);
+  memset(c->aimpold, 0, sizeof c->aimpold);
+}
+
+void oracle_tddft_iteration(octx* c, double deltat, int iter, int nrk, double* errHe, double* errimp,
+                            double* errvimp) {
+  if (iter <= nrk) {
+    oracle_steprk(c, deltat);
+    if (errHe) *errHe = 0; if (errimp) *errimp = 0; if (errvimp) *errvimp = 0;
+  } else {
+    oracle_steppc(c, deltat, errHe, errimp, errvimp);
+  }
+  oracle_potenimp(c);
+  oracle_poten(c);
+  oracle_forceimp(c);
+  oracle_update_aimp(c);
+}
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
