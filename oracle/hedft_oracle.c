@@ -1118,7 +1118,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
       c->rimp[i] = st[i] + c9 * c->pcr[i];
       eimp += fabs(c9 * c->pcr[i]);
     }
-    eimp = eimp * 0.3333333333 / c->nimp;
+    eimp = c->nimp ? eimp * 0.3333333333 / c->nimp : 0.0;
     { int t = c->ioldr[2]; c->ioldr[2] = c->ioldr[1]; c->ioldr[1] = c->ioldr[0]; c->ioldr[0] = t; }
     for (int i = 0; i < m; ++i)
       st[i] = 0.125 * (9.0 * c->vimp[i] - c->vimpold[c->ioldv[1]][i] +
@@ -1129,7 +1129,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
       c->vimp[i] = st[i] + c9 * c->pcv[i];
       evimp += fabs(c9 * c->pcv[i]);
     }
-    evimp = evimp * 0.3333333333 / c->nimp;
+    evimp = c->nimp ? evimp * 0.3333333333 / c->nimp : 0.0;
     { int t = c->ioldv[2]; c->ioldv[2] = c->ioldv[1]; c->ioldv[1] = c->ioldv[0]; c->ioldv[0] = t; }
   }
   if (errHe) *errHe = ehe;
