@@ -207,7 +207,7 @@ enum SInOp { SIN_PLAIN = 0, SIN_FINT = 1 };
 // plain derivnD of a field (real or complex) along axis (0..2)
 int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int is_complex);
 // grad(den) -> dxden,dyden,dzden with impurity force partials (forceimp)
-int derden_force(hedft_ctx* c, int want_force);
+int derden_force(hedft_ctx* c, int want_force, int update_aimp);
 // divergence of (1-denalf/den0s)*int_a, then ualphas and pot4 assembly
 int alpha_div_pot4(hedft_ctx* c);
 // Laplacian of psi (x then y into c->lap); the z pass is the fused step kernel

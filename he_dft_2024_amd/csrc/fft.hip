@@ -163,6 +163,57 @@ __device__ __forceinline__ void run_passes(double2* col, int stride, int t, cons
   if constexpr (sizeof...(Rest) > 0) run_passes<L, INV, T, P * R, Rest...>(col, stride, t, tw);
 }
 
+// Stockham pass over C columns of length L at once, all NT threads of the
+// block sharing the C*L/R butterflies.  Element e of column c lives at
+// base[c*cs + e*es].  CF: consecutive threads take consecutive columns
+// (columns interleaved in LDS) instead of consecutive butterflies of one column
+// (columns contiguous), so that a wavefront reads consecutive LDS addresses.
+template <int L, int R, int P, bool INV, int NT, int C, bool CF>
+__device__ __forceinline__ void mpass(double2* base, int cs, int es, const double2* __restrict__ tw) {
+  constexpr int NB = L / R, ITEMS = C * NB, PER = (ITEMS + NT - 1) / NT;
+  double2 u[PER][R];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int item = threadIdx.x + b * NT;
+    if (PER * NT == ITEMS || item < ITEMS) {
+      const int c = CF ? item % C : item / NB, i = CF ? item / C : item % NB;
+      const double2* col = base + c * cs;
+#pragma unroll
+      for (int j = 0; j < R; ++j) u[b][j] = col[(i + j * NB) * es];
+      if constexpr (P > 1) {
+        const int k = i % P;
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+          double2 w = tw[(k * j) * (L / (P * R))];
+          if (INV) w.y = -w.y;
+          u[b][j] = cmul(u[b][j], w);
+        }
+      }
+      dft<R, INV>(u[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int item = threadIdx.x + b * NT;
+    if (PER * NT == ITEMS || item < ITEMS) {
+      const int c = CF ? item % C : item / NB, i = CF ? item / C : item % NB;
+      double2* col = base + c * cs;
+      const int k = i % P;
+      const int o = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) col[(o + j * P) * es] = u[b][j];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, bool INV, int NT, int C, bool CF, int P, int R, int... Rest>
+__device__ __forceinline__ void run_mpasses(double2* base, int cs, int es, const double2* tw) {
+  mpass<L, R, P, INV, NT, C, CF>(base, cs, es, tw);
+  if constexpr (sizeof...(Rest) > 0) run_mpasses<L, INV, NT, C, CF, P * R, Rest...>(base, cs, es, tw);
+}
+
 // Factorisations (first radix largest keeps the thread count T = L / 8).
 template <int L>
 struct Rad;
@@ -173,6 +224,10 @@ struct Rad;
     template <bool INV>                                                                  \
     __device__ static __forceinline__ void run(double2* c, int s, int t, const double2* tw) { \
       run_passes<LEN, INV, TT, 1, __VA_ARGS__>(c, s, t, tw);                             \
+    }                                                                                    \
+    template <bool INV, int NT, int C, bool CF>                                          \
+    __device__ static __forceinline__ void run_multi(double2* b, int cs, int es, const double2* tw) { \
+      run_mpasses<LEN, INV, NT, C, CF, 1, __VA_ARGS__>(b, cs, es, tw);                   \
     }                                                                                    \
   };
 HEDFT_RAD(16, 4, 4, 4)
@@ -222,24 +277,155 @@ __device__ __forceinline__ void xout_apply(const XOut& o, size_t i, double v) {
   }
 }
 
+constexpr int MAXB = 4;  // arrays per batched launch
 struct XInBatch {
-  XIn in[3];
-  double2* out[3];
+  XIn in[MAXB];
+  double2* out[MAXB];
 };
 struct XOutBatch {
-  const double2* in[3];
-  XOut out[3];
+  const double2* in[MAXB];
+  XOut out[MAXB];
 };
 struct SpecBatch {
-  double2* p[3];
+  double2* p[MAXB];
 };
 struct ZConvArgs {
-  const double2* in[3];
-  double2* out[9];
-  double2* store[3];
-  const double* kern[3];
+  const double2* in[MAXB];
+  double2* out[MAXB * 3];
+  double2* store[MAXB];
+  const double* kern[MAXB * 3];  // kern[i*NOUT + o]
   int do_fwd;
 };
+
+// ---------------------------------------------------------------------------
+// fused x + y passes of one z-plane held entirely in LDS (NXH x NY complex,
+// up to 160 KB on gfx950): forward = rows R2C then columns FFT; inverse =
+// columns inverse FFT then rows C2R.  One workgroup per (plane, array).
+// ---------------------------------------------------------------------------
+template <int NX, int NY>
+struct Plane {
+  static constexpr int M = NX / 2, NXH = M + 1, PAD = NXH;  // odd row pitch
+  static constexpr int ITX = NY * (M / 2), ITY = NXH * (NY / 2);
+  static constexpr int NT0 = (ITX > ITY ? ITX : ITY) / 4;
+  static constexpr int NT = NT0 >= 1024 ? 1024 : (NT0 <= 64 ? 64 : ((NT0 + 63) / 64) * 64);
+  static constexpr size_t LDS = sizeof(double2) * NY * PAD;
+};
+
+template <int NX, int NY>
+__global__ void __launch_bounds__((Plane<NX, NY>::NT))
+    k_xyfwd(XInBatch a, Dims d, const double2* __restrict__ twx, const double2* __restrict__ twp,
+            const double2* __restrict__ twy) {
+  using PL = Plane<NX, NY>;
+  constexpr int M = PL::M, NXH = PL::NXH, PAD = PL::PAD, NT = PL::NT;
+  extern __shared__ unsigned char smem_raw[];
+  double2* P = reinterpret_cast<double2*>(smem_raw);
+  const XIn in = a.in[blockIdx.y];
+  double2* out = a.out[blockIdx.y];
+  const size_t row0 = (size_t)blockIdx.x * NY;
+  for (int e = threadIdx.x; e < NY * M; e += NT) {
+    const int y = e / M, j = e - y * M;
+    const size_t i = (row0 + y) * NX + 2 * j;
+    double2 v;
+    if (in.op == XIN_COPY) {
+      v = *reinterpret_cast<const double2*>(in.a[0] + i);
+    } else {
+      v.x = xin_eval(in, i);
+      v.y = xin_eval(in, i + 1);
+    }
+    P[y * PAD + j] = v;
+  }
+  __syncthreads();
+  Rad<M>::template run_multi<false, NT, NY, true>(P, PAD, 1, twx);
+  // real-to-half-complex split of every row: X_k, k = 0..M
+  constexpr int NPOST = NY * (M + 1), PPER = (NPOST + NT - 1) / NT;
+  double2 r[PPER];
+#pragma unroll
+  for (int b = 0; b < PPER; ++b) {
+    const int e = threadIdx.x + b * NT;
+    if (e < NPOST) {
+      const int y = e / (M + 1), k = e - y * (M + 1);
+      const double2 zk = P[y * PAD + (k == M ? 0 : k)];
+      const double2 zm = cconj(P[y * PAD + (k == 0 ? 0 : M - k)]);
+      const double2 ev = cscale(cadd(zk, zm), 0.5);
+      const double2 od = cscale(csub(zk, zm), 0.5);
+      const double2 wo = cmul(twp[k], od);
+      r[b] = make_double2(ev.x + wo.y, ev.y - wo.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < PPER; ++b) {
+    const int e = threadIdx.x + b * NT;
+    if (e < NPOST) {
+      const int y = e / (M + 1), k = e - y * (M + 1);
+      P[y * PAD + k] = r[b];
+    }
+  }
+  __syncthreads();
+  Rad<NY>::template run_multi<false, NT, NXH, true>(P, 1, PAD, twy);
+  double2* o = out + (size_t)blockIdx.x * NY * NXH;
+  for (int e = threadIdx.x; e < NY * NXH; e += NT) {
+    const int y = e / NXH, k = e - y * NXH;
+    o[e] = P[y * PAD + k];
+  }
+}
+
+template <int NX, int NY>
+__global__ void __launch_bounds__((Plane<NX, NY>::NT))
+    k_xyinv(XOutBatch a, Dims d, const double2* __restrict__ twx, const double2* __restrict__ twp,
+            const double2* __restrict__ twy, double renor) {
+  using PL = Plane<NX, NY>;
+  constexpr int M = PL::M, NXH = PL::NXH, PAD = PL::PAD, NT = PL::NT;
+  extern __shared__ unsigned char smem_raw[];
+  double2* P = reinterpret_cast<double2*>(smem_raw);
+  const double2* in = a.in[blockIdx.y] + (size_t)blockIdx.x * NY * NXH;
+  const XOut out = a.out[blockIdx.y];
+  for (int e = threadIdx.x; e < NY * NXH; e += NT) {
+    const int y = e / NXH, k = e - y * NXH;
+    P[y * PAD + k] = in[e];
+  }
+  __syncthreads();
+  Rad<NY>::template run_multi<true, NT, NXH, true>(P, 1, PAD, twy);
+  // half-complex-to-real merge: Z'_k = (X_k + X*_{M-k}) + i W^-k (X_k - X*_{M-k}),
+  // imaginary parts of X_0 and X_M dropped (c2r semantics)
+  constexpr int NPRE = NY * M, QPER = (NPRE + NT - 1) / NT;
+  double2 r[QPER];
+#pragma unroll
+  for (int b = 0; b < QPER; ++b) {
+    const int e = threadIdx.x + b * NT;
+    if (e < NPRE) {
+      const int y = e / M, k = e - y * M;
+      double2 xk = P[y * PAD + k], xm = P[y * PAD + M - k];
+      if (k == 0) {
+        xk.y = 0.0;
+        xm.y = 0.0;
+      }
+      const double2 xmc = cconj(xm);
+      const double2 sm = cadd(xk, xmc), df = csub(xk, xmc);
+      const double2 wd = cmul(cconj(twp[k]), df);
+      r[b] = make_double2(sm.x - wd.y, sm.y + wd.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < QPER; ++b) {
+    const int e = threadIdx.x + b * NT;
+    if (e < NPRE) {
+      const int y = e / M, k = e - y * M;
+      P[y * PAD + k] = r[b];
+    }
+  }
+  __syncthreads();
+  Rad<M>::template run_multi<true, NT, NY, true>(P, PAD, 1, twx);
+  const size_t row0 = (size_t)blockIdx.x * NY;
+  for (int e = threadIdx.x; e < NY * M; e += NT) {
+    const int y = e / M, j = e - y * M;
+    const double2 v = P[y * PAD + j];
+    const size_t i = (row0 + y) * NX + 2 * j;
+    xout_apply(out, i, v.x * renor);
+    xout_apply(out, i + 1, v.y * renor);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // x R2C: rows of nx = 2M reals -> M+1 complex
@@ -330,7 +516,11 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
 }
 
 // ---------------------------------------------------------------------------
-// column passes over a half spectrum.  AX = 1 (y) or 2 (z).
+// column passes over a half spectrum.  The columns of one axis are numbered
+// flat: y pass c = kx + NXH*kz (a tile of TX consecutive c is contiguous
+// except across a plane boundary), z pass c = kx + NXH*ky (contiguous).  TX
+// divides nothing in particular: the last tile is partial only at the very
+// end, so no tile is wasted on the single Nyquist column of every row.
 // ---------------------------------------------------------------------------
 template <int L>
 struct ColTile {
@@ -339,71 +529,71 @@ struct ColTile {
   static constexpr int NT = T * TX;
 };
 
-template <int L, int AX>
-__device__ __forceinline__ void col_geom(const Dims& d, size_t& base, size_t& stride) {
-  const int x0 = blockIdx.x * ColTile<L>::TX;
+template <int AX>
+__device__ __forceinline__ size_t col_addr(const Dims& d, long long c, size_t& stride) {
   if (AX == 1) {
-    base = (size_t)blockIdx.y * d.nxh * d.ny + x0;
     stride = d.nxh;
-  } else {
-    base = (size_t)blockIdx.y * d.nxh + x0;
-    stride = (size_t)d.nxh * d.ny;
+    const long long z = c / d.nxh, kx = c - z * d.nxh;
+    return (size_t)z * d.nxh * d.ny + kx;
   }
+  stride = (size_t)d.nxh * d.ny;
+  return (size_t)c;
 }
 
 template <int L, int AX, bool INV>
 __global__ void __launch_bounds__(ColTile<L>::NT)
-    k_colfft(SpecBatch a, Dims d, const double2* __restrict__ tw) {
+    k_colfft(SpecBatch a, Dims d, long long ncol, const double2* __restrict__ tw) {
   constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
   __shared__ double2 lds[L * TX];
-  double2* s = a.p[blockIdx.z];
-  size_t base, stride;
-  col_geom<L, AX>(d, base, stride);
-  const int x0 = blockIdx.x * TX;
+  double2* s = a.p[blockIdx.y];
+  const long long c0 = (long long)blockIdx.x * TX;
   for (int e = threadIdx.x; e < TX * L; e += NT) {
     const int xl = e % TX, j = e / TX;
-    lds[e] = (x0 + xl < d.nxh) ? s[base + j * stride + xl] : make_double2(0.0, 0.0);
+    size_t st;
+    const long long c = c0 + xl;
+    lds[e] = (c < ncol) ? s[col_addr<AX>(d, c, st) + j * st] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   Rad<L>::template run<INV>(lds + threadIdx.x % TX, TX, threadIdx.x / TX, tw);
   for (int e = threadIdx.x; e < TX * L; e += NT) {
     const int xl = e % TX, j = e / TX;
-    if (x0 + xl < d.nxh) s[base + j * stride + xl] = lds[e];
+    size_t st;
+    const long long c = c0 + xl;
+    if (c < ncol) s[col_addr<AX>(d, c, st) + j * st] = lds[e];
   }
 }
 
-// z convolution: [forward z] -> (x kern_o -> inverse z -> out_o) for o < NOUT
+// z convolution: [forward z] -> (x kern_{b,o} -> inverse z -> out_{b,o}) for o < NOUT
 template <int L, int NOUT>
 __global__ void __launch_bounds__(ColTile<L>::NT)
-    k_zconv(ZConvArgs a, Dims d, const double2* __restrict__ tw) {
+    k_zconv(ZConvArgs a, Dims d, long long ncol, const double2* __restrict__ tw) {
   constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
   __shared__ double2 src[L * TX];
   __shared__ double2 wk[(NOUT > 1) ? L * TX : 1];
-  const int b = blockIdx.z;
+  const int b = blockIdx.y;
   const double2* in = a.in[b];
-  size_t base, stride;
-  col_geom<L, 2>(d, base, stride);
-  const int x0 = blockIdx.x * TX;
+  const size_t stride = (size_t)d.nxh * d.ny;
+  const long long c0 = (long long)blockIdx.x * TX;
   const int xl0 = threadIdx.x % TX, t0 = threadIdx.x / TX;
   for (int e = threadIdx.x; e < TX * L; e += NT) {
     const int xl = e % TX, j = e / TX;
-    src[e] = (x0 + xl < d.nxh) ? in[base + j * stride + xl] : make_double2(0.0, 0.0);
+    src[e] = (c0 + xl < ncol) ? in[c0 + xl + j * stride] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   if (a.do_fwd) Rad<L>::template run<false>(src + xl0, TX, t0, tw);
   if (a.store[b]) {
     for (int e = threadIdx.x; e < TX * L; e += NT) {
       const int xl = e % TX, j = e / TX;
-      if (x0 + xl < d.nxh) a.store[b][base + j * stride + xl] = src[e];
+      if (c0 + xl < ncol) a.store[b][c0 + xl + j * stride] = src[e];
     }
   }
 #pragma unroll 1
   for (int o = 0; o < NOUT; ++o) {
     double2* buf = (NOUT > 1) ? wk : src;
-    const double* kern = a.kern[o];
+    const double* kern = a.kern[b * NOUT + o];
     for (int e = threadIdx.x; e < TX * L; e += NT) {
       const int xl = e % TX, j = e / TX;
-      const double kv = (x0 + xl < d.nxh) ? kern[base + j * stride + xl] : 0.0;
+      const double kv = (c0 + xl < ncol) ? kern[c0 + xl + j * stride] : 0.0;
       buf[e] = cscale(src[e], kv);
     }
     __syncthreads();
@@ -411,7 +601,7 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
     double2* out = a.out[b * NOUT + o];
     for (int e = threadIdx.x; e < TX * L; e += NT) {
       const int xl = e % TX, j = e / TX;
-      if (x0 + xl < d.nxh) out[base + j * stride + xl] = buf[e];
+      if (c0 + xl < ncol) out[c0 + xl + j * stride] = buf[e];
     }
     __syncthreads();
   }
@@ -446,9 +636,24 @@ static int dispatch_len(int L) {
     default: return HEDFT_EGRID;              \
   }
 
+// square planes small enough for one workgroup's LDS take the fused x+y path
+#define HEDFT_PLANE_SWITCH(NXV, NYV, ...)                                       \
+  if (NXV == NYV) {                                                              \
+    switch (NXV) {                                                               \
+      case 32: { constexpr int PX_ = 32, PY_ = 32; __VA_ARGS__; } return HEDFT_OK;   \
+      case 48: { constexpr int PX_ = 48, PY_ = 48; __VA_ARGS__; } return HEDFT_OK;   \
+      case 64: { constexpr int PX_ = 64, PY_ = 64; __VA_ARGS__; } return HEDFT_OK;   \
+      case 96: { constexpr int PX_ = 96, PY_ = 96; __VA_ARGS__; } return HEDFT_OK;   \
+      case 128: { constexpr int PX_ = 128, PY_ = 128; __VA_ARGS__; } return HEDFT_OK; \
+      default: break;                                                            \
+    }                                                                            \
+  }
+
 static inline const double2* twp(const hedft_ctx* c, size_t off) {
   return reinterpret_cast<const double2*>(c->tw) + off;
 }
+
+static bool g_plane_attr_set[8];
 
 int fft_setup(hedft_ctx* c) {
   const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, M = nx / 2;
@@ -468,6 +673,25 @@ int fft_setup(hedft_ctx* c) {
   c->tw_z = add(nz, nz);
   HEDFT_CHECK(hipMalloc(&c->tw, h.size() * sizeof(double2)));
   HEDFT_CHECK(hipMemcpy(c->tw, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+  // plane kernels use more than 64 KB of dynamic LDS
+  const int sizes[5] = {32, 48, 64, 96, 128};
+  for (int k = 0; k < 5; ++k) {
+    if (g_plane_attr_set[k]) continue;
+    hipError_t e1 = hipSuccess, e2 = hipSuccess;
+    switch (sizes[k]) {
+#define HEDFT_SETATTR(N)                                                                                    \
+  case N:                                                                                                   \
+    e1 = hipFuncSetAttribute((const void*)k_xyfwd<N, N>, hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                             (int)Plane<N, N>::LDS);                                                       \
+    e2 = hipFuncSetAttribute((const void*)k_xyinv<N, N>, hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                             (int)Plane<N, N>::LDS);                                                       \
+    break;
+      HEDFT_SETATTR(32) HEDFT_SETATTR(48) HEDFT_SETATTR(64) HEDFT_SETATTR(96) HEDFT_SETATTR(128)
+#undef HEDFT_SETATTR
+    }
+    if (e1 != hipSuccess || e2 != hipSuccess) return HEDFT_EHIP;
+    g_plane_attr_set[k] = true;
+  }
   return HEDFT_OK;
 }
 
@@ -499,16 +723,18 @@ template <int AX, bool INV>
 static int launch_col(hedft_ctx* c, const SpecBatch& a, int nb) {
   const int L = (AX == 1) ? c->d.ny : c->d.nz;
   const size_t off = (AX == 1) ? c->tw_y : c->tw_z;
+  const long long ncol = (long long)c->d.nxh * ((AX == 1) ? c->d.nz : c->d.ny);
   HEDFT_LEN_SWITCH(L, {
     constexpr int TX = ColTile<L_>::TX;
-    dim3 grid((c->d.nxh + TX - 1) / TX, (AX == 1) ? c->d.nz : c->d.ny, nb);
-    hipLaunchKernelGGL((k_colfft<L_, AX, INV>), grid, dim3(ColTile<L_>::NT), 0, c->stream, a, c->d,
+    dim3 grid((unsigned)((ncol + TX - 1) / TX), nb);
+    hipLaunchKernelGGL((k_colfft<L_, AX, INV>), grid, dim3(ColTile<L_>::NT), 0, c->stream, a, c->d, ncol,
                        twp(c, off));
   });
   return HEDFT_OK;
 }
 
 int fwd_xy(hedft_ctx* c, int nb, const XIn* in, double2* const* spec) {
+  if (nb < 1 || nb > MAXB) return HEDFT_EINVAL;
   XInBatch a{};
   SpecBatch s{};
   for (int b = 0; b < nb; ++b) {
@@ -516,35 +742,46 @@ int fwd_xy(hedft_ctx* c, int nb, const XIn* in, double2* const* spec) {
     a.out[b] = spec[b];
     s.p[b] = spec[b];
   }
+  HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
+    using PL_ = Plane<PX_, PY_>;
+    constexpr size_t lds_ = PL_::LDS;
+    constexpr int nt_ = PL_::NT;
+    hipLaunchKernelGGL((k_xyfwd<PX_, PY_>), dim3(c->d.nz, nb), dim3(nt_), lds_, c->stream, a, c->d,
+                       twp(c, c->tw_x), twp(c, c->tw_xpost), twp(c, c->tw_y));
+  });
   HEDFT_TRY(launch_xr2c(c, a, nb));
   return launch_col<1, false>(c, s, nb);
 }
 
 int zconv(hedft_ctx* c, int nin, double2* const* in, int nout, const double* const* kern,
           double2* const* out, int do_fwd, double2* const* store_fwd) {
+  if (nin < 1 || nin > MAXB || nout < 1 || nout > 3) return HEDFT_EINVAL;
   ZConvArgs a{};
   for (int i = 0; i < nin; ++i) {
     a.in[i] = in[i];
     a.store[i] = store_fwd ? store_fwd[i] : nullptr;
-    for (int o = 0; o < nout; ++o) a.out[i * nout + o] = out[i * nout + o];
+    for (int o = 0; o < nout; ++o) {
+      a.out[i * nout + o] = out[i * nout + o];
+      a.kern[i * nout + o] = kern[i * nout + o];
+    }
   }
-  for (int o = 0; o < nout; ++o) a.kern[o] = kern[o];
   a.do_fwd = do_fwd;
   const int L = c->d.nz;
+  const long long ncol = (long long)c->d.nxh * c->d.ny;
   HEDFT_LEN_SWITCH(L, {
     constexpr int TX = ColTile<L_>::TX;
-    dim3 grid((c->d.nxh + TX - 1) / TX, c->d.ny, nin);
+    dim3 grid((unsigned)((ncol + TX - 1) / TX), nin);
     dim3 blk(ColTile<L_>::NT);
     const double2* tw = twp(c, c->tw_z);
-    if (nout == 1) hipLaunchKernelGGL((k_zconv<L_, 1>), grid, blk, 0, c->stream, a, c->d, tw);
-    else if (nout == 2) hipLaunchKernelGGL((k_zconv<L_, 2>), grid, blk, 0, c->stream, a, c->d, tw);
-    else if (nout == 3) hipLaunchKernelGGL((k_zconv<L_, 3>), grid, blk, 0, c->stream, a, c->d, tw);
-    else return HEDFT_EINVAL;
+    if (nout == 1) hipLaunchKernelGGL((k_zconv<L_, 1>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
+    else if (nout == 2) hipLaunchKernelGGL((k_zconv<L_, 2>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
+    else hipLaunchKernelGGL((k_zconv<L_, 3>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
   });
   return HEDFT_OK;
 }
 
 int inv_yx(hedft_ctx* c, int nb, double2* const* spec, const XOut* out) {
+  if (nb < 1 || nb > MAXB) return HEDFT_EINVAL;
   SpecBatch s{};
   XOutBatch a{};
   for (int b = 0; b < nb; ++b) {
@@ -552,6 +789,13 @@ int inv_yx(hedft_ctx* c, int nb, double2* const* spec, const XOut* out) {
     a.in[b] = spec[b];
     a.out[b] = out[b];
   }
+  HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
+    using PL_ = Plane<PX_, PY_>;
+    constexpr size_t lds_ = PL_::LDS;
+    constexpr int nt_ = PL_::NT;
+    hipLaunchKernelGGL((k_xyinv<PX_, PY_>), dim3(c->d.nz, nb), dim3(nt_), lds_, c->stream, a, c->d,
+                       twp(c, c->tw_x), twp(c, c->tw_xpost), twp(c, c->tw_y), c->renor);
+  });
   HEDFT_TRY((launch_col<1, true>(c, s, nb)));
   return launch_xc2r(c, a, nb);
 }

@@ -59,7 +59,7 @@ ImpDev make_impdev(hedft_ctx* c) {
 }
 
 static size_t partials_region(hedft_ctx* c) { return c->partials_cap / 8; }
-double* force_partials(hedft_ctx* c, int axis) { return c->partials + (size_t)(1 + axis) * partials_region(c); }
+double* force_partials(hedft_ctx* c) { return c->partials + partials_region(c); }
 static double* gen_partials(hedft_ctx* c) { return c->partials; }
 
 // ---- reductions -------------------------------------------------------------------
@@ -92,42 +92,35 @@ static void finalize(hedft_ctx* c, const double* part, size_t nblocks, int slot,
                      c->scal + slot, scale);
 }
 
-// F_k,a = -sum_blocks(partials_a) * dxyz; optionally aimp = F/m (DFT4He3d.f90:692-696)
-__global__ void k_force_finalize(const double* p0, int n0, const double* p1, int n1, const double* p2,
-                                 int n2, double* imp, int nimp, double dxyz, int update_aimp) {
-  __shared__ double sh[3][HEDFT_MAX_IMP][256];
-  const double* ps[3] = {p0, p1, p2};
-  const int ns[3] = {n0, n1, n2};
-  for (int a = 0; a < 3; ++a)
-    for (int k = 0; k < HEDFT_MAX_IMP; ++k) {
-      double s = 0.0;
-      for (int b = threadIdx.x; b < ns[a]; b += 256) s += ps[a][(size_t)b * HEDFT_MAX_IMP + k];
-      sh[a][k][threadIdx.x] = s;
-    }
+// F_k,a = -sum_blocks(partials[3k+a]) * dxyz; optionally aimp = F/m (DFT4He3d.f90:692-696)
+__global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, double* imp,
+                                                        int nimp, double dxyz, int update_aimp) {
+  constexpr int NR = 3 * HEDFT_MAX_IMP;
+  __shared__ double sh[NR][256];
+  for (int r = 0; r < NR; ++r) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += part[(size_t)b * NR + r];
+    sh[r][threadIdx.x] = s;
+  }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w)
-      for (int a = 0; a < 3; ++a)
-        for (int k = 0; k < HEDFT_MAX_IMP; ++k) sh[a][k][threadIdx.x] += sh[a][k][threadIdx.x + w];
+      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    for (int a = 0; a < 3; ++a)
-      for (int k = 0; k < nimp; ++k) {
-        const double f = -sh[a][k][0] * dxyz;
+    for (int k = 0; k < nimp; ++k)
+      for (int a = 0; a < 3; ++a) {
+        const double f = -sh[3 * k + a][0] * dxyz;
         imp[IMP_F + k + nimp * a] = f;
         if (update_aimp) imp[IMP_A + k + nimp * a] = f / imp[IMP_MASS + k];
       }
   }
 }
 
-static int g_update_aimp = 0;
-int force_finalize(hedft_ctx* c) {
-  const size_t n0 = pencil_geom<double>(c->d, 0).nblocks, n1 = pencil_geom<double>(c->d, 1).nblocks,
-               n2 = pencil_geom<double>(c->d, 2).nblocks;
-  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, c->stream, force_partials(c, 0), (int)n0,
-                     force_partials(c, 1), (int)n1, force_partials(c, 2), (int)n2, c->imp, c->nimp, c->dxyz,
-                     g_update_aimp);
+int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp) {
+  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, c->imp,
+                     c->nimp, c->dxyz, update_aimp);
   return HEDFT_OK;
 }
 
@@ -271,6 +264,16 @@ struct OutSq {
   const double* acc;
   __device__ __forceinline__ void store(size_t i, int, int, int, V v, double*) const {
     out[i] = (acc ? acc[i] : 0.0) + vsq<V>(v);
+  }
+};
+
+template <typename V>
+struct OutSqXY {
+  static constexpr int NR = 0;
+  double* partials;
+  double* out;
+  __device__ __forceinline__ void store(size_t i, int, int, int, V dx, V dy, double*) const {
+    out[i] = vsq<V>(dx) + vsq<V>(dy);
   }
 };
 
@@ -670,24 +673,57 @@ static XOut xo_pot4(hedft_ctx* c) {
   return o;
 }
 
-// correlation convolution F^-1[wcgk F(den dencg (cp4+cpp4 dencg))]; for the OT core the
-// C2R epilogue assembles pot4 directly
-static int correlation(hedft_ctx* c) {
-  XIn ci{};
-  ci.op = XIN_CORR;
-  ci.a[0] = c->den;
-  ci.a[1] = c->dencg;
-  ci.c0 = c->p.cp4;
-  ci.c1 = c->p.cpp4;
-  double2* s0 = c->spec[0];
-  HEDFT_TRY(fwd_xy(c, 1, &ci, &s0));
-  const double* kw = c->wcgk;
-  HEDFT_TRY(zconv(c, 1, &s0, 1, &kw, &s0, 1, nullptr));
-  XOut xo = c->p.core_otc ? xo_store(c->sto1) : xo_pot4(c);
-  return inv_yx(c, 1, &s0, &xo);
+// Stage B of the mean field, one batched convolution of up to four arrays:
+//   corr  = F^-1[wcgk F(den dencg (cp4+cpp4 dencg))]          (poten.f90:166-178)
+//   int_a = F^-1[kalfs F((1-denalf/den0s) d_a den)], a = x,y,z  (term_alfa.f90:26-48)
+// For the OT core only corr is needed and its C2R epilogue assembles pot4.
+static int stage_b(hedft_ctx* c) {
+  XIn in[4];
+  in[0] = XIn{};
+  in[0].op = XIN_CORR;
+  in[0].a[0] = c->den;
+  in[0].a[1] = c->dencg;
+  in[0].c0 = c->p.cp4;
+  in[0].c1 = c->p.cpp4;
+  const double* grads[3] = {c->dxden, c->dyden, c->dzden};
+  for (int a = 0; a < 3; ++a) {
+    in[1 + a] = XIn{};
+    in[1 + a].op = XIN_FGRAD;
+    in[1 + a].a[0] = c->denalf;
+    in[1 + a].a[1] = grads[a];
+    in[1 + a].c0 = c->p.den0s;
+  }
+  const int nb = c->p.core_otc ? 4 : 1;
+  double2* sp[4] = {c->spec[0], c->spec[1], c->spec[2], c->spec[3]};
+  HEDFT_TRY(fwd_xy(c, nb, in, sp));
+  const double* kk[4] = {c->wcgk, c->kalfs, c->kalfs, c->kalfs};
+  HEDFT_TRY(zconv(c, nb, sp, 1, kk, sp, 1, nullptr));
+  XOut xo[4] = {c->p.core_otc ? xo_store(c->sto1) : xo_pot4(c), xo_store(c->intx), xo_store(c->inty),
+                xo_store(c->intz)};
+  return inv_yx(c, nb, sp, xo);
 }
 
-// term_alfa (term_alfa.f90:5-102) + pot4 assembly
+// Stage C: ualphas' convolution F^-1[kalfs F(grad(den).int)] (term_alfa.f90:61-73),
+// then the divergence term and the pot4 assembly (term_alfa.f90:80-98, poten.f90:194-211)
+static int stage_c(hedft_ctx* c) {
+  XIn dot{};
+  dot.op = XIN_DOT3;
+  dot.a[0] = c->dxden;
+  dot.a[1] = c->dyden;
+  dot.a[2] = c->dzden;
+  dot.a[3] = c->intx;
+  dot.a[4] = c->inty;
+  dot.a[5] = c->intz;
+  double2* s0 = c->spec[0];
+  HEDFT_TRY(fwd_xy(c, 1, &dot, &s0));
+  const double* ka = c->kalfs;
+  HEDFT_TRY(zconv(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
+  XOut xu = xo_store(c->ualph_conv);
+  HEDFT_TRY(inv_yx(c, 1, &s0, &xu));
+  return alpha_div_pot4(c);
+}
+
+// term_alfa() on its own (B restricted to int_a, then C)
 static int alpha_term(hedft_ctx* c) {
   XIn fg[3];
   const double* grads[3] = {c->dxden, c->dyden, c->dzden};
@@ -700,24 +736,11 @@ static int alpha_term(hedft_ctx* c) {
   }
   double2* s3[3] = {c->spec[1], c->spec[2], c->spec[3]};
   HEDFT_TRY(fwd_xy(c, 3, fg, s3));
-  const double* ka = c->kalfs;
-  HEDFT_TRY(zconv(c, 3, s3, 1, &ka, s3, 1, nullptr));
+  const double* ka[3] = {c->kalfs, c->kalfs, c->kalfs};
+  HEDFT_TRY(zconv(c, 3, s3, 1, ka, s3, 1, nullptr));
   XOut xo3[3] = {xo_store(c->intx), xo_store(c->inty), xo_store(c->intz)};
   HEDFT_TRY(inv_yx(c, 3, s3, xo3));
-  XIn dot{};
-  dot.op = XIN_DOT3;
-  dot.a[0] = c->dxden;
-  dot.a[1] = c->dyden;
-  dot.a[2] = c->dzden;
-  dot.a[3] = c->intx;
-  dot.a[4] = c->inty;
-  dot.a[5] = c->intz;
-  double2* s0 = c->spec[0];
-  HEDFT_TRY(fwd_xy(c, 1, &dot, &s0));
-  HEDFT_TRY(zconv(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
-  XOut xu = xo_store(c->ualph_conv);
-  HEDFT_TRY(inv_yx(c, 1, &s0, &xu));
-  return alpha_div_pot4(c);
+  return stage_c(c);
 }
 
 // Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden)
@@ -740,10 +763,9 @@ static int poten_tddft(hedft_ctx* c, int want_force) {
     XOut xo[2] = {xo_store(c->dencg), xo_store(c->delj4)};
     HEDFT_TRY(inv_yx(c, 2, so, xo));
   }
-  g_update_aimp = 0;
-  HEDFT_TRY(derden_force(c, want_force));
-  HEDFT_TRY(correlation(c));
-  if (c->p.core_otc) HEDFT_TRY(alpha_term(c));
+  HEDFT_TRY(derden_force(c, want_force, want_force));
+  HEDFT_TRY(stage_b(c));
+  if (c->p.core_otc) HEDFT_TRY(stage_c(c));
   return HEDFT_OK;
 }
 
@@ -755,14 +777,14 @@ static int poten_statics(hedft_ctx* c) {
   HEDFT_TRY(zconv(c, 1, &fd, 1, &kf, &s2, 0, nullptr));
   XOut xo = xo_store(c->delj4);
   HEDFT_TRY(inv_yx(c, 1, &s2, &xo));
-  HEDFT_TRY(correlation(c));
-  if (c->p.core_otc) HEDFT_TRY(alpha_term(c));
+  HEDFT_TRY(stage_b(c));
+  if (c->p.core_otc) HEDFT_TRY(stage_c(c));
   return HEDFT_OK;
 }
 
 // driver refresh after a statics step (BCN4HeDFT.f90:1131-1149): derden, fden, dencg, denalf
 static int statics_refresh(hedft_ctx* c) {
-  HEDFT_TRY(derden_force(c, 0));
+  HEDFT_TRY(derden_force(c, 0, 0));
   XIn xi{};
   xi.op = XIN_COPY;
   xi.a[0] = c->den;
@@ -902,17 +924,9 @@ static int steppc_impl(hedft_ctx* c, double dt) {
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 
-// driver part after each step (DFT4He3d.f90:686-696): potenimp, poten, forceimp, aimp = F/m
-static int tddft_after_step(hedft_ctx* c) {
-  HEDFT_TRY(poten_tddft(c, 1));
-  if (c->nimp > 0) {
-    // derden_force issued the force reduction without aimp; redo the finalize with aimp
-    g_update_aimp = 1;
-    HEDFT_TRY(force_finalize(c));
-    g_update_aimp = 0;
-  }
-  return HEDFT_OK;
-}
+// driver part after each step (DFT4He3d.f90:686-696): potenimp, poten, forceimp,
+// aimp = F/m -- the force reduction and aimp are fused into poten's gradient pass
+static int tddft_after_step(hedft_ctx* c) { return poten_tddft(c, 1); }
 
 // ---- statics -------------------------------------------------------------------------------------
 static int hpsi_statics(hedft_ctx* c, double n4) {
@@ -995,8 +1009,8 @@ static int steppcr_impl(hedft_ctx* c, double dt, double n4) {
 template <typename V>
 static int energy_sums(hedft_ctx* c, double* s) {
   const V* psi = (const V*)c->psi;
-  launch_pencil<V, 0>(c->stream, c->d, c->coef[0][0], InPlain<V>{psi}, OutSq<V>{nullptr, c->work1, nullptr});
-  launch_pencil<V, 1>(c->stream, c->d, c->coef[0][1], InPlain<V>{psi}, OutSq<V>{nullptr, c->work1, c->work1});
+  launch_pencil_xy<V, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<V>{psi},
+                            OutSqXY<V>{nullptr, c->work1});
   OutEnergy<V> o;
   o.partials = gen_partials(c);
   o.acc = c->work1;
@@ -1150,7 +1164,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
     maxb = std::max(maxb, pencil_geom<double>(d, a).nblocks);
   }
   maxb = std::max(maxb, (size_t)EW_BLOCKS);
-  c->partials_cap = 8 * maxb * 8;
+  c->partials_cap = 8 * maxb * 16;  // 8 regions of 16 values per block
   rc = rc ? rc : alloc(c, (void**)&c->partials, c->partials_cap * sizeof(double));
   rc = rc ? rc : alloc(c, (void**)&c->scal, NSCAL * sizeof(double));
   rc = rc ? rc : alloc(c, (void**)&c->imp, IMP_SIZE * sizeof(double));
@@ -1352,7 +1366,7 @@ int hedft_fftbk(hedft_ctx* c, int32_t in, int32_t out) {
 }
 
 int hedft_derden(hedft_ctx* c) {
-  HEDFT_TRY(derden_force(c, 0));
+  HEDFT_TRY(derden_force(c, 0, 0));
   return done(c);
 }
 
@@ -1517,32 +1531,31 @@ int hedft_potenimp(hedft_ctx* c) {
   return done(c);
 }
 
-struct OutForce {  // standalone forceimp from the stored gradient (potenimp.f90:49-52)
-  static constexpr int NR = HEDFT_MAX_IMP;
-  double* partials;
-  const double* g;
-  ImpDev im;
-  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double, double* red) const {
+// standalone forceimp from the stored gradient (potenimp.f90:49-52)
+__global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const double* gx, const double* gy,
+                                                  const double* gz, double* partials) {
+  double red[3 * HEDFT_MAX_IMP];
+  for (int r = 0; r < 3 * HEDFT_MAX_IMP; ++r) red[r] = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % d.nx), iy = (int)((i / d.nx) % d.ny), iz = (int)(i / ((long long)d.nx * d.ny));
     const double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
-    const double v = g[i];
-    for (int k = 0; k < im.nimp; ++k) red[k] += v * imp_u_k(im, k, xx, yy, zz, false);
+    for (int k = 0; k < im.nimp; ++k) {
+      const double u = imp_u_k(im, k, xx, yy, zz, false);
+      red[3 * k] += gx[i] * u;
+      red[3 * k + 1] += gy[i] * u;
+      red[3 * k + 2] += gz[i] * u;
+    }
   }
-};
+  block_reduce_store<3 * HEDFT_MAX_IMP>(red, partials);
+}
 
 int hedft_forceimp(hedft_ctx* c) {
   if (!c) return HEDFT_EINVAL;
   if (c->nimp == 0) return HEDFT_OK;
-  // reuse the pencil geometry so that force_finalize sees the same partial counts
-  const double* gs[3] = {c->dxden, c->dyden, c->dzden};
-  ImpDev im = make_impdev(c);
-  for (int a = 0; a < 3; ++a) {
-    OutForce o{force_partials(c, a), gs[a], im};
-    if (a == 0) launch_pencil<double, 0>(c->stream, c->d, c->coef[0][0], InPlain<double>{gs[a]}, o);
-    else if (a == 1) launch_pencil<double, 1>(c->stream, c->d, c->coef[0][1], InPlain<double>{gs[a]}, o);
-    else launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], InPlain<double>{gs[a]}, o);
-  }
-  g_update_aimp = 0;
-  HEDFT_TRY(force_finalize(c));
+  hipLaunchKernelGGL(k_force_ew, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c), c->dxden, c->dyden,
+                     c->dzden, force_partials(c));
+  HEDFT_TRY(force_finalize(c, EW_BLOCKS, 0));
   return done(c);
 }
 

@@ -130,11 +130,13 @@ int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int i
   return deriv_plain_t<double>(c, id, axis, (const double*)in, (double*)out);
 }
 
-// grad(den) with the He part of the impurity force F_k = -sum(d_a den * U_k) dxyz
-// (potenimp.f90:49-52) accumulated per workgroup in the same pass.
-struct OutGradForce {
-  static constexpr int NR = HEDFT_MAX_IMP;
+// grad(den): x and y in one fused pass, z in a second pass that also
+// accumulates the He part of the impurity force F_k = -sum(grad(den) U_k) dxyz
+// (potenimp.f90:49-52), evaluating U_k once per point.
+struct OutGradZForce {
+  static constexpr int NR = 3 * HEDFT_MAX_IMP;
   double* out;
+  const double *gx, *gy;
   double* partials;
   ImpDev im;
   int want;
@@ -142,29 +144,30 @@ struct OutGradForce {
     out[i] = v;
     if (want) {
       const double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
-      for (int k = 0; k < im.nimp; ++k) red[k] += v * imp_u_k(im, k, xx, yy, zz, false);
+      const double a = gx[i], b = gy[i];
+      for (int k = 0; k < im.nimp; ++k) {
+        const double u = imp_u_k(im, k, xx, yy, zz, false);
+        red[3 * k] += a * u;
+        red[3 * k + 1] += b * u;
+        red[3 * k + 2] += v * u;
+      }
     }
   }
 };
 
 ImpDev make_impdev(hedft_ctx* c);
-double* force_partials(hedft_ctx* c, int axis);
-int force_finalize(hedft_ctx* c);
+double* force_partials(hedft_ctx* c);
+int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp);
 
-int derden_force(hedft_ctx* c, int want_force) {
+int derden_force(hedft_ctx* c, int want_force, int update_aimp) {
   want_force = want_force && c->nimp > 0;
-  ImpDev im = make_impdev(c);
   InPlain<double> ip{c->den};
-  double* outs[3] = {c->dxden, c->dyden, c->dzden};
-  for (int a = 0; a < 3; ++a) {
-    OutGradForce op{outs[a], force_partials(c, a), im, want_force};
-    const StencilCoef& cf = c->coef[0][a];
-    if (a == 0) launch_pencil<double, 0>(c->stream, c->d, cf, ip, op);
-    else if (a == 1) launch_pencil<double, 1>(c->stream, c->d, cf, ip, op);
-    else launch_pencil<double, 2>(c->stream, c->d, cf, ip, op);
-  }
+  launch_pencil_xy<double, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ip,
+                                 OutTwo{c->dxden, c->dyden, nullptr});
+  OutGradZForce op{c->dzden, c->dxden, c->dyden, force_partials(c), make_impdev(c), want_force};
+  launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], ip, op);
   if (hipGetLastError() != hipSuccess) return HEDFT_EHIP;
-  return want_force ? force_finalize(c) : HEDFT_OK;
+  return want_force ? force_finalize(c, pencil_geom<double>(c->d, 2).nblocks, update_aimp) : HEDFT_OK;
 }
 
 // z pass of the alpha_s divergence with the mean-field assembly:
@@ -190,10 +193,10 @@ struct OutDivPot4 {
 
 int alpha_div_pot4(hedft_ctx* c) {
   const double den0s = c->p.den0s;
-  InFInt ix{c->denalf, c->intx, den0s}, iy{c->denalf, c->inty, den0s}, iz{c->denalf, c->intz, den0s};
-  launch_pencil<double, 0>(c->stream, c->d, c->coef[0][0], ix, OutStore<double>{c->work1, nullptr});
-  launch_pencil<double, 1>(c->stream, c->d, c->coef[0][1], iy,
-                           OutAcc<double>{c->work1, c->work1, nullptr});
+  InFInt2 ixy{c->denalf, c->intx, c->inty, den0s};
+  InFInt iz{c->denalf, c->intz, den0s};
+  launch_pencil_xy<double, false>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ixy,
+                                  OutSumXY<double>{c->work1, nullptr});
   OutDivPot4 op;
   op.partials = nullptr;
   op.acc = c->work1;
@@ -214,21 +217,13 @@ int alpha_div_pot4(hedft_ctx* c) {
 }
 
 int lap_xy(hedft_ctx* c, const void* psi, int is_complex) {
-  if (is_complex) {
-    const double2* p = (const double2*)psi;
-    double2* l = (double2*)c->lap;
-    launch_pencil<double2, 0>(c->stream, c->d, c->coef[1][0], InPlain<double2>{p},
-                              OutStore<double2>{l, nullptr});
-    launch_pencil<double2, 1>(c->stream, c->d, c->coef[1][1], InPlain<double2>{p},
-                              OutAcc<double2>{l, l, nullptr});
-  } else {
-    const double* p = (const double*)psi;
-    double* l = (double*)c->lap;
-    launch_pencil<double, 0>(c->stream, c->d, c->coef[1][0], InPlain<double>{p},
-                             OutStore<double>{l, nullptr});
-    launch_pencil<double, 1>(c->stream, c->d, c->coef[1][1], InPlain<double>{p},
-                             OutAcc<double>{l, l, nullptr});
-  }
+  if (is_complex)
+    launch_pencil_xy<double2, true>(c->stream, c->d, c->coef[1][0], c->coef[1][1],
+                                    InPlain<double2>{(const double2*)psi},
+                                    OutSumXY<double2>{(double2*)c->lap, nullptr});
+  else
+    launch_pencil_xy<double, true>(c->stream, c->d, c->coef[1][0], c->coef[1][1],
+                                   InPlain<double>{(const double*)psi}, OutSumXY<double>{(double*)c->lap, nullptr});
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 

@@ -150,17 +150,156 @@ inline void launch_pencil(hipStream_t s, const Dims& d, const StencilCoef& cf, c
   hipLaunchKernelGGL((k_pencil<V, AX, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
 }
 
+// ---- fused x + y derivatives ------------------------------------------------
+// A workgroup holds a tile of one z-plane: tx columns plus a 6-point halo on
+// each side in x, all ny rows.  The x derivative reads its 13-point window along
+// the tile row, the y derivative the whole column; boundary rows follow the
+// same rules as stencil_at (periodic halos are loaded wrapped, with the sign
+// flip of antiperiodic icon=18 applied to the wrapped values).
+template <typename V>
+__device__ __forceinline__ V stencil_tile_x(const StencilCoef& cf, const V* row, int gx, int nx, int x0) {
+  V acc = vzero<V>();
+  const int off = 6 - x0;  // LDS position of global index g is g + off
+  if (cf.mode != 0 || (gx >= 6 && gx < nx - 6)) {
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.c[j], row[gx - 6 + j + off], acc);
+  } else if (gx < 6) {
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.rows[gx][j], row[j + off], acc);
+  } else {
+    const int r = gx - nx + 12;
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.rows[r][j], row[nx - 13 + j + off], acc);
+  }
+  return acc;
+}
+
+// In::load2(i, a, b): a feeds the x derivative, b the y derivative (SAME: a == b)
+template <typename V, bool SAME, class In, class Out>
+__global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef cfx, StencilCoef cfy, Dims d,
+                                                   int tx) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int NR = Out::NR;
+  double red[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int r = 0; r < (NR > 0 ? NR : 1); ++r) red[r] = 0.0;
+  const int TW = tx + 12, nx = d.nx, ny = d.ny;
+  V* A = reinterpret_cast<V*>(smem_raw);
+  V* B = SAME ? A : A + TW * ny;
+  const int x0 = blockIdx.x * tx, z = blockIdx.y;
+  const size_t zb = (size_t)z * nx * ny;
+  for (int e = threadIdx.x; e < TW * ny; e += blockDim.x) {
+    const int xl = e % TW, y = e / TW;
+    int gx = x0 - 6 + xl;
+    double sg = 1.0;
+    bool ok = true;
+    if (gx < 0 || gx >= nx) {
+      if (cfx.mode != 0) {
+        sg = (cfx.mode == 2) ? -1.0 : 1.0;
+        gx += (gx < 0) ? nx : -nx;
+      } else {
+        ok = false;
+      }
+    }
+    V va = vzero<V>(), vb = vzero<V>();
+    if (ok) in.load2(zb + (size_t)y * nx + gx, va, vb);
+    A[e] = vmul(sg, va);
+    if (!SAME) B[e] = vb;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < tx * ny; e += blockDim.x) {
+    const int xl = e % tx, y = e / tx;
+    const int gx = x0 + xl;
+    if (gx < nx) {
+      const V dx = stencil_tile_x<V>(cfx, A + y * TW, gx, nx, x0);
+      const V dy = stencil_at<V>(cfy, B + xl + 6, TW, y, ny);
+      out.store(zb + (size_t)y * nx + gx, gx, y, z, dx, dy, red);
+    }
+  }
+  if constexpr (NR > 0) block_reduce_store<NR>(red, out.partials);
+}
+
+template <typename V, bool SAME>
+inline PencilGeom pencil_xy_geom(const Dims& d) {
+  PencilGeom g;
+  g.block = dim3(256);
+  int tx = (sizeof(V) == 8) ? 32 : 16;
+  auto lds = [&](int t) { return sizeof(V) * (size_t)(t + 12) * d.ny * (SAME ? 1 : 2); };
+  while (tx > 8 && lds(tx) > 64 * 1024) tx /= 2;
+  // the last tile must start at or before nx-7 so that the one-sided rows of the
+  // last six points read inside the tile
+  while (tx > 8 && d.nx % tx != 0 && d.nx % tx < 7) tx /= 2;
+  if (tx > d.nx) tx = d.nx;
+  g.tx = tx;
+  g.grid = dim3((d.nx + tx - 1) / tx, d.nz);
+  g.lds = lds(tx);
+  g.nblocks = (size_t)g.grid.x * g.grid.y;
+  return g;
+}
+
+template <typename V, bool SAME, class In, class Out>
+inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cfx, const StencilCoef& cfy,
+                             const In& in, const Out& out) {
+  PencilGeom g = pencil_xy_geom<V, SAME>(d);
+  static bool attr = false;
+  if (!attr && g.lds > 64 * 1024) {
+    hipFuncSetAttribute((const void*)k_pencil_xy<V, SAME, In, Out>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_pencil_xy<V, SAME, In, Out>), g.grid, g.block, g.lds, s, in, out, cfx, cfy, d, g.tx);
+}
+
 // ---- common functors ---------------------------------------------------------
 template <typename V>
 struct InPlain {
   const V* a;
   __device__ __forceinline__ V load(size_t i) const { return a[i]; }
+  __device__ __forceinline__ void load2(size_t i, V& x, V& y) const { x = y = a[i]; }
 };
 // (1 - denalf/den0s) * int_a   (term_alfa.f90:80-85)
 struct InFInt {
   const double *denalf, *ia;
   double den0s;
   __device__ __forceinline__ double load(size_t i) const { return (1.0 - denalf[i] / den0s) * ia[i]; }
+};
+// (f int_x, f int_y) for the fused x + y divergence pass
+struct InFInt2 {
+  const double *denalf, *ix, *iy;
+  double den0s;
+  __device__ __forceinline__ void load2(size_t i, double& a, double& b) const {
+    const double f = 1.0 - denalf[i] / den0s;
+    a = f * ix[i];
+    b = f * iy[i];
+  }
+};
+// out = dx + dy (Laplacian / divergence partial sums)
+template <typename V>
+struct OutSumXY {
+  static constexpr int NR = 0;
+  V* out;
+  double* partials;
+  __device__ __forceinline__ void store(size_t i, int, int, int, V dx, V dy, double*) const;
+};
+template <>
+__device__ __forceinline__ void OutSumXY<double>::store(size_t i, int, int, int, double dx, double dy,
+                                                        double*) const {
+  out[i] = dx + dy;
+}
+template <>
+__device__ __forceinline__ void OutSumXY<double2>::store(size_t i, int, int, int, double2 dx, double2 dy,
+                                                         double*) const {
+  out[i] = make_double2(dx.x + dy.x, dx.y + dy.y);
+}
+// two separate outputs (derden)
+struct OutTwo {
+  static constexpr int NR = 0;
+  double *ox, *oy;
+  double* partials;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double dx, double dy, double*) const {
+    ox[i] = dx;
+    oy[i] = dy;
+  }
 };
 template <typename V>
 struct OutStore {
