@@ -96,6 +96,24 @@ __device__ inline double imp_u(const ImpDev& im, int ix, int iy, int iz) {
   return u;
 }
 
+// ---- LDS fill with memory-level parallelism -------------------------------------
+// lds[e] = f(e) for e in [0, total), strided over the NT threads of the block.
+// f must load unconditionally from a valid address (callers clamp indices; tile
+// slots that no output reads may hold any value): the U loads of a thread are
+// then issued back to back before the first LDS store, keeping U requests in
+// flight per wavefront instead of one.
+template <int U, int NT, typename V, typename F>
+__device__ __forceinline__ void lds_fill(V* lds, int total, F f) {
+  for (int e0 = threadIdx.x; e0 < total; e0 += NT * U) {
+    V tmp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) tmp[u] = f(min(e0 + u * NT, total - 1));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * NT < total) lds[e0 + u * NT] = tmp[u];
+  }
+}
+
 // ---- block reduction into per-block partials ----------------------------------
 template <int NR>
 __device__ inline void block_reduce_store(double (&v)[NR], double* partials) {
@@ -126,8 +144,9 @@ struct hedft_ctx {
   hedft_params p;
   int mode;
   int device;
-  hipStream_t stream;
-  hipEvent_t ev0, ev1;
+  hipStream_t stream;  // every launch goes to this stream
+  hipStream_t main_stream, side_stream;  // stream is main_stream except inside a side branch
+  hipEvent_t ev0, ev1, ev_fork, ev_join;
   hedft::Dims d;
   double hx, hy, hz, dxyz, renor;
   // real N fields
@@ -155,6 +174,10 @@ struct hedft_ctx {
   // PC slot bookkeeping (host-side; rkpc: ioldp, ioldh; classicimp: ioldr, ioldv, iolda)
   int ioldp[3], ioldh[2], ioldr[3], ioldv[3], iolda[2];
   int rk_done;
+  // lap holds the x+y Laplacian of the current psi (computed on the side
+  // stream by the mean-field pass that follows a step); cleared whenever psi
+  // may change otherwise
+  int lap_fresh;
   // classical impurities
   int nimp, npot;
   double delinter, rmaxinterpol;
@@ -209,7 +232,8 @@ int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int i
 // grad(den) -> dxden,dyden,dzden with impurity force partials (forceimp)
 int derden_force(hedft_ctx* c, int want_force, int update_aimp);
 // divergence of (1-denalf/den0s)*int_a, then ualphas and pot4 assembly
-int alpha_div_pot4(hedft_ctx* c);
+int alpha_div_xy(hedft_ctx* c);      // x+y divergence of f int_a into work1
+int alpha_div_z_pot4(hedft_ctx* c);  // z divergence + ualphas + pot4 assembly
 // Laplacian of psi (x then y into c->lap); the z pass is the fused step kernel
 int lap_xy(hedft_ctx* c, const void* psi, int is_complex);
 }  // namespace hedft

@@ -302,6 +302,21 @@ struct ZConvArgs {
 // up to 160 KB on gfx950): forward = rows R2C then columns FFT; inverse =
 // columns inverse FFT then rows C2R.  One workgroup per (plane, array).
 // ---------------------------------------------------------------------------
+// X_k of the half-complex split: ((z + zc) - i W^k (z - zc)) / 2 with zc = conj(Z_{M-k})
+__device__ __forceinline__ double2 split_one(double2 z, double2 zc, double2 w) {
+  const double2 ev = cscale(cadd(z, zc), 0.5);
+  const double2 wo = cmul(w, cscale(csub(z, zc), 0.5));
+  return make_double2(ev.x + wo.y, ev.y - wo.x);
+}
+
+// Z'_k of the c2r merge: (x + conj(xm)) + i conj(W^k) (x - conj(xm))
+__device__ __forceinline__ double2 merge_one(double2 x, double2 xm, double2 w) {
+  const double2 xmc = cconj(xm);
+  const double2 sm = cadd(x, xmc);
+  const double2 wd = cmul(cconj(w), csub(x, xmc));
+  return make_double2(sm.x - wd.y, sm.y + wd.x);
+}
+
 template <int NX, int NY>
 struct Plane {
   static constexpr int M = NX / 2, NXH = M + 1, PAD = NXH;  // odd row pitch
@@ -312,54 +327,78 @@ struct Plane {
 };
 
 template <int NX, int NY>
-__global__ void __launch_bounds__((Plane<NX, NY>::NT))
+__global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_waves_per_eu(1, 4)))
     k_xyfwd(XInBatch a, Dims d, const double2* __restrict__ twx, const double2* __restrict__ twp,
             const double2* __restrict__ twy) {
   using PL = Plane<NX, NY>;
   constexpr int M = PL::M, NXH = PL::NXH, PAD = PL::PAD, NT = PL::NT;
   extern __shared__ unsigned char smem_raw[];
   double2* P = reinterpret_cast<double2*>(smem_raw);
-  const XIn in = a.in[blockIdx.y];
+  const XIn& in = a.in[blockIdx.y];
   double2* out = a.out[blockIdx.y];
   const size_t row0 = (size_t)blockIdx.x * NY;
-  for (int e = threadIdx.x; e < NY * M; e += NT) {
-    const int y = e / M, j = e - y * M;
-    const size_t i = (row0 + y) * NX + 2 * j;
-    double2 v;
-    if (in.op == XIN_COPY) {
-      v = *reinterpret_cast<const double2*>(in.a[0] + i);
-    } else {
-      v.x = xin_eval(in, i);
-      v.y = xin_eval(in, i + 1);
+  auto fill = [&](auto getv) {
+    for (int e0 = threadIdx.x; e0 < NY * M; e0 += NT * 4) {
+      double2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + u * NT, NY * M - 1);
+        const int y = e / M, j = e - y * M;
+        v[u] = getv((row0 + y) * NX + 2 * j);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + u * NT;
+        if (e < NY * M) {
+          const int y = e / M, j = e - y * M;
+          P[y * PAD + j] = v[u];
+        }
+      }
     }
-    P[y * PAD + j] = v;
+  };
+  switch (in.op) {  // uniform per workgroup: one branch-free load loop per operation
+    case XIN_CORR:
+      fill([&](size_t i) {
+        const double2 dn = *reinterpret_cast<const double2*>(in.a[0] + i);
+        const double2 cg = *reinterpret_cast<const double2*>(in.a[1] + i);
+        return make_double2(dn.x * cg.x * (in.c0 + in.c1 * cg.x), dn.y * cg.y * (in.c0 + in.c1 * cg.y));
+      });
+      break;
+    case XIN_FGRAD:
+      fill([&](size_t i) {
+        const double2 al = *reinterpret_cast<const double2*>(in.a[0] + i);
+        const double2 gr = *reinterpret_cast<const double2*>(in.a[1] + i);
+        return make_double2((1.0 - al.x / in.c0) * gr.x, (1.0 - al.y / in.c0) * gr.y);
+      });
+      break;
+    case XIN_DOT3:
+      fill([&](size_t i) {
+        const double2 g0 = *reinterpret_cast<const double2*>(in.a[0] + i);
+        const double2 g1 = *reinterpret_cast<const double2*>(in.a[1] + i);
+        const double2 g2 = *reinterpret_cast<const double2*>(in.a[2] + i);
+        const double2 i0 = *reinterpret_cast<const double2*>(in.a[3] + i);
+        const double2 i1 = *reinterpret_cast<const double2*>(in.a[4] + i);
+        const double2 i2 = *reinterpret_cast<const double2*>(in.a[5] + i);
+        return make_double2(g0.x * i0.x + g1.x * i1.x + g2.x * i2.x, g0.y * i0.y + g1.y * i1.y + g2.y * i2.y);
+      });
+      break;
+    default:
+      fill([&](size_t i) { return *reinterpret_cast<const double2*>(in.a[0] + i); });
   }
   __syncthreads();
   Rad<M>::template run_multi<false, NT, NY, true>(P, PAD, 1, twx);
-  // real-to-half-complex split of every row: X_k, k = 0..M
-  constexpr int NPOST = NY * (M + 1), PPER = (NPOST + NT - 1) / NT;
-  double2 r[PPER];
-#pragma unroll
-  for (int b = 0; b < PPER; ++b) {
-    const int e = threadIdx.x + b * NT;
-    if (e < NPOST) {
-      const int y = e / (M + 1), k = e - y * (M + 1);
-      const double2 zk = P[y * PAD + (k == M ? 0 : k)];
-      const double2 zm = cconj(P[y * PAD + (k == 0 ? 0 : M - k)]);
-      const double2 ev = cscale(cadd(zk, zm), 0.5);
-      const double2 od = cscale(csub(zk, zm), 0.5);
-      const double2 wo = cmul(twp[k], od);
-      r[b] = make_double2(ev.x + wo.y, ev.y - wo.x);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < PPER; ++b) {
-    const int e = threadIdx.x + b * NT;
-    if (e < NPOST) {
-      const int y = e / (M + 1), k = e - y * (M + 1);
-      P[y * PAD + k] = r[b];
-    }
+  // real-to-half-complex split of every row, in place: X_k, k = 0..M, from
+  // Z_k = x_2k + i x_2k+1 (Z_M = Z_0).  One thread owns the pair (k, M-k), so
+  // the split needs no register staging and no extra barrier.
+  constexpr int MH = M / 2 + 1;
+  for (int e = threadIdx.x; e < NY * MH; e += NT) {
+    const int y = e / MH, k = e - y * MH;
+    double2* row = P + y * PAD;
+    const double2 a = row[k], b = row[k == 0 ? 0 : M - k];
+    const double2 xk = split_one(a, cconj(b), twp[k]);
+    const double2 xm = split_one(b, cconj(a), twp[M - k]);
+    row[k] = xk;
+    row[M - k] = xm;
   }
   __syncthreads();
   Rad<NY>::template run_multi<false, NT, NXH, true>(P, 1, PAD, twy);
@@ -371,7 +410,7 @@ __global__ void __launch_bounds__((Plane<NX, NY>::NT))
 }
 
 template <int NX, int NY>
-__global__ void __launch_bounds__((Plane<NX, NY>::NT))
+__global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_waves_per_eu(1, 4)))
     k_xyinv(XOutBatch a, Dims d, const double2* __restrict__ twx, const double2* __restrict__ twp,
             const double2* __restrict__ twy, double renor) {
   using PL = Plane<NX, NY>;
@@ -379,41 +418,27 @@ __global__ void __launch_bounds__((Plane<NX, NY>::NT))
   extern __shared__ unsigned char smem_raw[];
   double2* P = reinterpret_cast<double2*>(smem_raw);
   const double2* in = a.in[blockIdx.y] + (size_t)blockIdx.x * NY * NXH;
-  const XOut out = a.out[blockIdx.y];
-  for (int e = threadIdx.x; e < NY * NXH; e += NT) {
-    const int y = e / NXH, k = e - y * NXH;
-    P[y * PAD + k] = in[e];
-  }
+  const XOut& out = a.out[blockIdx.y];
+  static_assert(PAD == NXH, "plane rows are stored contiguously");
+  lds_fill<8, NT>(P, NY * NXH, [&](int e) { return in[e]; });
   __syncthreads();
   Rad<NY>::template run_multi<true, NT, NXH, true>(P, 1, PAD, twy);
-  // half-complex-to-real merge: Z'_k = (X_k + X*_{M-k}) + i W^-k (X_k - X*_{M-k}),
-  // imaginary parts of X_0 and X_M dropped (c2r semantics)
-  constexpr int NPRE = NY * M, QPER = (NPRE + NT - 1) / NT;
-  double2 r[QPER];
-#pragma unroll
-  for (int b = 0; b < QPER; ++b) {
-    const int e = threadIdx.x + b * NT;
-    if (e < NPRE) {
-      const int y = e / M, k = e - y * M;
-      double2 xk = P[y * PAD + k], xm = P[y * PAD + M - k];
-      if (k == 0) {
-        xk.y = 0.0;
-        xm.y = 0.0;
-      }
-      const double2 xmc = cconj(xm);
-      const double2 sm = cadd(xk, xmc), df = csub(xk, xmc);
-      const double2 wd = cmul(cconj(twp[k]), df);
-      r[b] = make_double2(sm.x - wd.y, sm.y + wd.x);
+  // half-complex-to-real merge, in place: Z'_k = (X_k + X*_{M-k}) + i W^-k
+  // (X_k - X*_{M-k}), k = 0..M-1, imaginary parts of X_0 and X_M dropped (c2r
+  // semantics).  One thread owns the pair (k, M-k).
+  constexpr int MH = M / 2 + 1;
+  for (int e = threadIdx.x; e < NY * MH; e += NT) {
+    const int y = e / MH, k = e - y * MH;
+    double2* row = P + y * PAD;
+    double2 a = row[k], b = row[M - k];
+    if (k == 0) {
+      a.y = 0.0;
+      b.y = 0.0;
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < QPER; ++b) {
-    const int e = threadIdx.x + b * NT;
-    if (e < NPRE) {
-      const int y = e / M, k = e - y * M;
-      P[y * PAD + k] = r[b];
-    }
+    const double2 zk = merge_one(a, b, twp[k]);
+    const double2 zm = merge_one(b, a, twp[M - k]);
+    row[k] = zk;
+    if (k != 0) row[M - k] = zm;
   }
   __syncthreads();
   Rad<M>::template run_multi<true, NT, NY, true>(P, PAD, 1, twx);
@@ -435,15 +460,15 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
     k_xr2c(XInBatch a, Dims d, const double2* __restrict__ tw, const double2* __restrict__ twp) {
   constexpr int T = Rad<M>::T;
   __shared__ double2 lds[ROWS][M + 1];
-  const XIn in = a.in[blockIdx.y];
+  const XIn& in = a.in[blockIdx.y];
   double2* out = a.out[blockIdx.y];
   const int nrows = d.ny * d.nz;
   const int row0 = blockIdx.x * ROWS;
-  for (int e = threadIdx.x; e < ROWS * M; e += T * ROWS) {
-    const int r = e / M, j = e - r * M;
+  lds_fill<4, T * ROWS>(&lds[0][0], ROWS * (M + 1), [&](int e) {
+    const int r = e / (M + 1), j = e - r * (M + 1);
     const int row = row0 + r;
     double2 v = make_double2(0.0, 0.0);
-    if (row < nrows) {
+    if (row < nrows && j < M) {
       const size_t i = (size_t)row * (2 * M) + 2 * j;
       if (in.op == XIN_COPY) {
         v = *reinterpret_cast<const double2*>(in.a[0] + i);
@@ -452,8 +477,8 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
         v.y = xin_eval(in, i + 1);
       }
     }
-    lds[r][j] = v;
-  }
+    return v;
+  });
   __syncthreads();
   const int t = threadIdx.x % T, r = threadIdx.x / T;
   Rad<M>::template run<false>(&lds[r][0], 1, t, tw);
@@ -480,14 +505,14 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
   constexpr int T = Rad<M>::T;
   __shared__ double2 lds[ROWS][M + 1];
   const double2* in = a.in[blockIdx.y];
-  const XOut out = a.out[blockIdx.y];
+  const XOut& out = a.out[blockIdx.y];
   const int nrows = d.ny * d.nz;
   const int row0 = blockIdx.x * ROWS;
-  for (int e = threadIdx.x; e < ROWS * M; e += T * ROWS) {
-    const int r = e / M, k = e - r * M;
+  lds_fill<4, T * ROWS>(&lds[0][0], ROWS * (M + 1), [&](int e) {
+    const int r = e / (M + 1), k = e - r * (M + 1);
     const int row = row0 + r;
     double2 z = make_double2(0.0, 0.0);
-    if (row < nrows) {
+    if (row < nrows && k < M) {
       const double2* h = in + (size_t)row * (M + 1);
       double2 xk = h[k];
       double2 xm = h[M - k];
@@ -499,8 +524,8 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
       const double2 wd = cmul(w, df);
       z = make_double2(s.x - wd.y, s.y + wd.x);
     }
-    lds[r][k] = z;
-  }
+    return z;
+  });
   __syncthreads();
   const int t = threadIdx.x % T, r = threadIdx.x / T;
   Rad<M>::template run<true>(&lds[r][0], 1, t, tw);
@@ -547,12 +572,12 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
   __shared__ double2 lds[L * TX];
   double2* s = a.p[blockIdx.y];
   const long long c0 = (long long)blockIdx.x * TX;
-  for (int e = threadIdx.x; e < TX * L; e += NT) {
+  lds_fill<8, NT>(lds, TX * L, [&](int e) {
     const int xl = e % TX, j = e / TX;
     size_t st;
-    const long long c = c0 + xl;
-    lds[e] = (c < ncol) ? s[col_addr<AX>(d, c, st) + j * st] : make_double2(0.0, 0.0);
-  }
+    const long long c = min(c0 + xl, ncol - 1);  // clamped columns are never stored
+    return s[col_addr<AX>(d, c, st) + j * st];
+  });
   __syncthreads();
   Rad<L>::template run<INV>(lds + threadIdx.x % TX, TX, threadIdx.x / TX, tw);
   for (int e = threadIdx.x; e < TX * L; e += NT) {
@@ -575,10 +600,10 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
   const size_t stride = (size_t)d.nxh * d.ny;
   const long long c0 = (long long)blockIdx.x * TX;
   const int xl0 = threadIdx.x % TX, t0 = threadIdx.x / TX;
-  for (int e = threadIdx.x; e < TX * L; e += NT) {
+  lds_fill<8, NT>(src, TX * L, [&](int e) {
     const int xl = e % TX, j = e / TX;
-    src[e] = (c0 + xl < ncol) ? in[c0 + xl + j * stride] : make_double2(0.0, 0.0);
-  }
+    return in[min(c0 + xl, ncol - 1) + j * stride];  // clamped columns are never stored
+  });
   __syncthreads();
   if (a.do_fwd) Rad<L>::template run<false>(src + xl0, TX, t0, tw);
   if (a.store[b]) {
@@ -591,10 +616,19 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
   for (int o = 0; o < NOUT; ++o) {
     double2* buf = (NOUT > 1) ? wk : src;
     const double* kern = a.kern[b * NOUT + o];
-    for (int e = threadIdx.x; e < TX * L; e += NT) {
-      const int xl = e % TX, j = e / TX;
-      const double kv = (c0 + xl < ncol) ? kern[c0 + xl + j * stride] : 0.0;
-      buf[e] = cscale(src[e], kv);
+    for (int e0 = threadIdx.x; e0 < TX * L; e0 += NT * 8) {
+      double kv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = min(e0 + u * NT, TX * L - 1);
+        const int xl = e % TX, j = e / TX;
+        kv[u] = kern[min(c0 + xl, ncol - 1) + j * stride];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * NT;
+        if (e < TX * L) buf[e] = cscale(src[e], kv[u]);
+      }
     }
     __syncthreads();
     Rad<L>::template run<true>(buf + xl0, TX, t0, tw);

@@ -158,8 +158,15 @@ struct HpsiC {
   }
 };
 
+// Outputs per sliding window in the fused stepper epilogues (each output
+// touches 6-8 complex arrays, so fewer outputs are kept in flight).
+#ifndef HEDFT_KB_STEP
+#define HEDFT_KB_STEP 1
+#endif
+
 struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
   static constexpr int NR = 0;
+  static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
   HpsiC h;
   double2 *psi, *q, *po0, *po1, *po2, *ho0, *ho1;
@@ -187,6 +194,7 @@ struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
 
 struct OutPred {  // predictor + modifier (steppc.f90:432-442)
   static constexpr int NR = 0;
+  static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
   HpsiC h;
   double2 *psi, *pc, *po3, *ho1, *ho2;
@@ -211,6 +219,7 @@ struct OutPred {  // predictor + modifier (steppc.f90:432-442)
 
 struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
   static constexpr int NR = 1;
+  static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
   HpsiC h;
   double2 *psi, *pc, *po3, *po2, *ho1, *ho2;
@@ -648,6 +657,29 @@ static void kernel_tables(const hedft_ctx* c, std::vector<double>& fv, std::vect
 }
 
 // ---- mean field pipelines --------------------------------------------------------------------
+// ---- two-branch sections: work launched between fork() and join() inside a
+// SideBranch scope runs on the side stream, concurrently with what the main
+// stream does in the meantime; join() makes the main stream wait for it.  The
+// branches must touch disjoint buffers.  Under stream capture the same calls
+// record a graph with two parallel branches.
+static int fork(hedft_ctx* c) {
+  HEDFT_CHECK(hipEventRecord(c->ev_fork, c->main_stream));
+  HEDFT_CHECK(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
+  return HEDFT_OK;
+}
+
+static int join(hedft_ctx* c) {
+  HEDFT_CHECK(hipEventRecord(c->ev_join, c->side_stream));
+  HEDFT_CHECK(hipStreamWaitEvent(c->main_stream, c->ev_join, 0));
+  return HEDFT_OK;
+}
+
+struct SideBranch {
+  hedft_ctx* c;
+  explicit SideBranch(hedft_ctx* cc) : c(cc) { c->stream = c->side_stream; }
+  ~SideBranch() { c->stream = c->main_stream; }
+};
+
 static XOut xo_store(double* out) {
   XOut o{};
   o.op = XOUT_STORE;
@@ -706,6 +738,13 @@ static int stage_b(hedft_ctx* c) {
 // Stage C: ualphas' convolution F^-1[kalfs F(grad(den).int)] (term_alfa.f90:61-73),
 // then the divergence term and the pot4 assembly (term_alfa.f90:80-98, poten.f90:194-211)
 static int stage_c(hedft_ctx* c) {
+  // side branch: x+y divergence (denalf, int_x, int_y -> work1); main branch:
+  // the convolution (grad den, int -> spec[0] -> ualph_conv)
+  HEDFT_TRY(fork(c));
+  {
+    SideBranch sb(c);
+    HEDFT_TRY(alpha_div_xy(c));
+  }
   XIn dot{};
   dot.op = XIN_DOT3;
   dot.a[0] = c->dxden;
@@ -720,7 +759,8 @@ static int stage_c(hedft_ctx* c) {
   HEDFT_TRY(zconv(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
   XOut xu = xo_store(c->ualph_conv);
   HEDFT_TRY(inv_yx(c, 1, &s0, &xu));
-  return alpha_div_pot4(c);
+  HEDFT_TRY(join(c));
+  return alpha_div_z_pot4(c);
 }
 
 // term_alfa() on its own (B restricted to int_a, then C)
@@ -743,8 +783,17 @@ static int alpha_term(hedft_ctx* c) {
   return stage_c(c);
 }
 
-// Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden)
-static int poten_tddft(hedft_ctx* c, int want_force) {
+// Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden).
+// grad(den) and the impurity force run on the side branch while the main
+// branch convolves den; with_lap also computes the x+y Laplacian of psi there
+// for the step that follows (psi is not changed by the mean field).
+static int poten_tddft(hedft_ctx* c, int want_force, int with_lap = 0) {
+  HEDFT_TRY(fork(c));
+  {
+    SideBranch sb(c);
+    HEDFT_TRY(derden_force(c, want_force, want_force));
+    if (with_lap) HEDFT_TRY(lap_xy(c, c->psi, 1));
+  }
   XIn xi{};
   xi.op = XIN_COPY;
   xi.a[0] = c->den;
@@ -763,7 +812,8 @@ static int poten_tddft(hedft_ctx* c, int want_force) {
     XOut xo[2] = {xo_store(c->dencg), xo_store(c->delj4)};
     HEDFT_TRY(inv_yx(c, 2, so, xo));
   }
-  HEDFT_TRY(derden_force(c, want_force, want_force));
+  HEDFT_TRY(join(c));
+  if (with_lap) c->lap_fresh = 1;
   HEDFT_TRY(stage_b(c));
   if (c->p.core_otc) HEDFT_TRY(stage_c(c));
   return HEDFT_OK;
@@ -784,7 +834,11 @@ static int poten_statics(hedft_ctx* c) {
 
 // driver refresh after a statics step (BCN4HeDFT.f90:1131-1149): derden, fden, dencg, denalf
 static int statics_refresh(hedft_ctx* c) {
-  HEDFT_TRY(derden_force(c, 0, 0));
+  HEDFT_TRY(fork(c));
+  {
+    SideBranch sb(c);
+    HEDFT_TRY(derden_force(c, 0, 0));
+  }
   XIn xi{};
   xi.op = XIN_COPY;
   xi.a[0] = c->den;
@@ -796,13 +850,15 @@ static int statics_refresh(hedft_ctx* c) {
     double2* so[2] = {c->spec[1], c->spec[3]};
     HEDFT_TRY(zconv(c, 1, &s0, 2, kk, so, 1, &fd));
     XOut xo[2] = {xo_store(c->dencg), xo_store(c->denalf)};
-    return inv_yx(c, 2, so, xo);
+    HEDFT_TRY(inv_yx(c, 2, so, xo));
+  } else {
+    const double* kw = c->wcgk;
+    double2* s1 = c->spec[1];
+    HEDFT_TRY(zconv(c, 1, &s0, 1, &kw, &s1, 1, &fd));
+    XOut xo = xo_store(c->dencg);
+    HEDFT_TRY(inv_yx(c, 1, &s1, &xo));
   }
-  const double* kw = c->wcgk;
-  double2* s1 = c->spec[1];
-  HEDFT_TRY(zconv(c, 1, &s0, 1, &kw, &s1, 1, &fd));
-  XOut xo = xo_store(c->dencg);
-  return inv_yx(c, 1, &s1, &xo);
+  return join(c);
 }
 
 int poten(hedft_ctx* c, int want_force) {
@@ -837,7 +893,7 @@ static inline double* rz(void* p) { return (double*)p; }
 // STEPRK (Dynamics_TDDFT/steprk.f90:1-144)
 static int steprk_impl(hedft_ctx* c, double dt) {
   for (int jrun = 0; jrun < 4; ++jrun) {
-    HEDFT_TRY(lap_xy(c, c->psi, 1));
+    if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
     OutRK o;
     o.partials = nullptr;
     o.h = make_hpsic(c);
@@ -855,10 +911,11 @@ static int steprk_impl(hedft_ctx* c, double dt) {
     o.dt = dt;
     o.first = (jrun == 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    c->lap_fresh = 0;
     if (c->nimp > 0)
       hipLaunchKernelGGL(k_imp_rk, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, (int)(jrun == 0), RK_A[jrun],
                          RK_B[jrun], RK_C[jrun], dt);
-    if (jrun <= 2) HEDFT_TRY(poten_tddft(c, 0));
+    if (jrun <= 2) HEDFT_TRY(poten_tddft(c, 0, 1));
   }
   for (int k = 0; k < 3; ++k) c->ioldp[k] = c->ioldr[k] = c->ioldv[k] = k;
   for (int k = 0; k < 2; ++k) c->ioldh[k] = c->iolda[k] = k;
@@ -868,7 +925,7 @@ static int steprk_impl(hedft_ctx* c, double dt) {
 
 // STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC]
 static int steppc_impl(hedft_ctx* c, double dt) {
-  HEDFT_TRY(lap_xy(c, c->psi, 1));
+  if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
   {
     OutPred o;
     o.partials = nullptr;
@@ -881,6 +938,7 @@ static int steppc_impl(hedft_ctx* c, double dt) {
     o.den = c->den;
     o.dt = dt;
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    c->lap_fresh = 0;
     std::swap(c->ioldh[0], c->ioldh[1]);
   }
   if (c->nimp > 0) {
@@ -888,8 +946,8 @@ static int steppc_impl(hedft_ctx* c, double dt) {
                        c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1]);
     std::swap(c->iolda[0], c->iolda[1]);
   }
-  HEDFT_TRY(poten_tddft(c, 0));
-  HEDFT_TRY(lap_xy(c, c->psi, 1));
+  HEDFT_TRY(poten_tddft(c, 0, 1));
+  if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
   {
     OutCorr o;
     o.partials = gen_partials(c);
@@ -903,6 +961,7 @@ static int steppc_impl(hedft_ctx* c, double dt) {
     o.den = c->den;
     o.dt = dt;
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    c->lap_fresh = 0;
     finalize<1>(c, gen_partials(c), pencil_geom<double2>(c->d, 2).nblocks, S_ERRPC, 1.0);
     const int t = c->ioldp[2];
     c->ioldp[2] = c->ioldp[1];
@@ -926,7 +985,9 @@ static int steppc_impl(hedft_ctx* c, double dt) {
 
 // driver part after each step (DFT4He3d.f90:686-696): potenimp, poten, forceimp,
 // aimp = F/m -- the force reduction and aimp are fused into poten's gradient pass
-static int tddft_after_step(hedft_ctx* c) { return poten_tddft(c, 1); }
+// driver's poten + forceimp after a step; also prepares the x+y Laplacian of the
+// new psi for the next step
+static int tddft_after_step(hedft_ctx* c) { return poten_tddft(c, 1, 1); }
 
 // ---- statics -------------------------------------------------------------------------------------
 static int hpsi_statics(hedft_ctx* c, double n4) {
@@ -1119,12 +1180,22 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   if (c->p.lsolid) c->p.core_otc = 0;  // If(Lsolid) core4='OT ' (BCN4HeDFT.f90:654, DFT4He3d.f90:456)
   c->mode = mode;
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  // two streams at the default priority: the main one and a side one for
+  // independent branches of the mean field (joined by events, also under capture)
+  if (hipStreamCreateWithFlags(&c->main_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return HEDFT_EHIP;
   }
+  if (hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking) != hipSuccess) {
+    hipStreamDestroy(c->main_stream);
+    delete c;
+    return HEDFT_EHIP;
+  }
+  c->stream = c->main_stream;
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
+  hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   Dims& d = c->d;
   d.nx = g->nx;
   d.ny = g->ny;
@@ -1205,7 +1276,8 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
 int hedft_destroy(hedft_ctx* c) {
   if (!c) return HEDFT_OK;
   hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->side_stream) hipStreamSynchronize(c->side_stream);
+  if (c->main_stream) hipStreamSynchronize(c->main_stream);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   if (c->graph) hipGraphDestroy(c->graph);
   void* ptrs[] = {c->den, c->pot4, c->dencg, c->delj4, c->denalf, c->intx, c->inty, c->intz, c->ualphas,
@@ -1219,7 +1291,10 @@ int hedft_destroy(hedft_ctx* c) {
   if (c->host_scal) hipHostFree(c->host_scal);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side_stream) hipStreamDestroy(c->side_stream);
+  if (c->main_stream) hipStreamDestroy(c->main_stream);
   delete c;
   return HEDFT_OK;
 }
@@ -1290,6 +1365,7 @@ int hedft_upload(hedft_ctx* c, int32_t field, const void* host, size_t bytes) {
   HEDFT_CHECK(hipMemcpyAsync(p, host, b, hipMemcpyHostToDevice, c->stream));
   HEDFT_CHECK(hipStreamSynchronize(c->stream));
   c->graph_valid = 0;
+  c->lap_fresh = 0;
   return HEDFT_OK;
 }
 
@@ -1339,6 +1415,7 @@ int hedft_derivnD(hedft_ctx* c, int32_t id, int32_t iv, int32_t in, int32_t out)
   if (ci || co || bi != bo || pi == po) return HEDFT_EINVAL;
   const size_t rN = sizeof(double) * c->d.n;
   if (bi != rN && bi != 2 * rN) return HEDFT_EINVAL;
+  c->lap_fresh = 0;
   HEDFT_TRY(deriv_plain(c, id, iv - 1, pi, po, bi == 2 * rN));
   return done(c);
 }
@@ -1361,6 +1438,7 @@ int hedft_fftbk(hedft_ctx* c, int32_t in, int32_t out) {
   HEDFT_TRY(field_ptr(c, in, &pi, &bi, &ci));
   HEDFT_TRY(field_ptr(c, out, &po, &bo, &co));
   if (ci || co || bo != sizeof(double) * c->d.n || bi != sizeof(double2) * c->d.nh) return HEDFT_EINVAL;
+  c->lap_fresh = 0;
   HEDFT_TRY(fftbk(c, (double2*)pi, (double*)po));
   return done(c);
 }
@@ -1593,6 +1671,7 @@ int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double 
 
 int hedft_tddft_init(hedft_ctx* c) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  c->lap_fresh = 0;
   hipLaunchKernelGGL(k_tddft_psi0, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den, (double2*)c->psi);
   // qr = qv = rimpold = vimpold = aimpold = 0 (DFT4He3d.f90:625-629)
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_QR, 0, sizeof(double) * 24, c->stream));
@@ -1628,13 +1707,17 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
   if (!c->rk_done) return HEDFT_ESTATE;
   hipSetDevice(c->device);
   const int period = 6;
+  // the captured iterations start from a prepared x+y Laplacian of psi (as every
+  // iteration leaves it), so prepare it now if an upload invalidated it
+  if (!c->lap_fresh) {
+    HEDFT_TRY(lap_xy(c, c->psi, 1));
+    c->lap_fresh = 1;
+  }
   if (use_graph && (!c->graph_valid || c->graph_dt != deltat)) {
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
-    int saved[13];
-    std::memcpy(saved, c->ioldp, sizeof(int) * 3);
     HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = HEDFT_OK;
     for (int s = 0; s < period && rc == HEDFT_OK; ++s) {
@@ -1649,8 +1732,8 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
     HEDFT_CHECK(hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0));
     c->graph_valid = 1;
     c->graph_dt = deltat;
-    (void)saved;
-    // capture advanced the host-side slot indices by one full period: back to the start
+    // capture advanced the host-side slot indices by one full period, i.e. back
+    // to where they started
   }
   HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
   int done_steps = 0;

@@ -191,12 +191,18 @@ struct OutDivPot4 {
   }
 };
 
-int alpha_div_pot4(hedft_ctx* c) {
-  const double den0s = c->p.den0s;
-  InFInt2 ixy{c->denalf, c->intx, c->inty, den0s};
-  InFInt iz{c->denalf, c->intz, den0s};
+// x + y part of the divergence sum_a d_a(f int_a), f = 1 - denalf/den0s, into work1
+int alpha_div_xy(hedft_ctx* c) {
+  InFInt2 ixy{c->denalf, c->intx, c->inty, c->p.den0s};
   launch_pencil_xy<double, false>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ixy,
                                   OutSumXY<double>{c->work1, nullptr});
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+// z part of the divergence, fused with the ualphas and pot4 assembly
+int alpha_div_z_pot4(hedft_ctx* c) {
+  const double den0s = c->p.den0s;
+  InFInt iz{c->denalf, c->intz, den0s};
   OutDivPot4 op;
   op.partials = nullptr;
   op.acc = c->work1;

@@ -8,6 +8,8 @@
 // to an epilogue functor that stores it or fuses the consumer's arithmetic.
 // Input functors fuse the producer's arithmetic into the LDS fill.
 #pragma once
+#include <type_traits>
+
 #include "common.h"
 
 namespace hedft {
@@ -53,10 +55,55 @@ __device__ __forceinline__ V stencil_at(const StencilCoef& cf, const V* P, int s
   return acc;
 }
 
+// Sliding-window evaluation of KB consecutive outputs i0..i0+KB-1 of a pencil
+// (element stride s, length n): the 12+KB inputs are read once into registers.
+// Outputs whose 13-point window needs a boundary row fall back to stencil_at.
+template <typename V, int KB>
+__device__ __forceinline__ void stencil_run(const StencilCoef& cf, const V* P, int s, int i0, int cnt, int n,
+                                            V (&out)[KB]) {
+  const bool interior = (cf.mode != 0) || (i0 >= 6 && i0 + KB <= n - 6);
+  if (interior && cnt == KB) {
+    V w[KB + 12];
+    const double sg = (cf.mode == 2) ? -1.0 : 1.0;
+#pragma unroll
+    for (int q = 0; q < KB + 12; ++q) {
+      int ii = i0 - 6 + q;
+      double f = 1.0;
+      if (ii < 0) { ii += n; f = sg; }
+      else if (ii >= n) { ii -= n; f = sg; }
+      w[q] = vmul(f, P[ii * s]);
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      V acc = vzero<V>();
+#pragma unroll
+      for (int j = 0; j < 13; ++j) acc = vfma(cf.c[j], w[k + j], acc);
+      out[k] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KB; ++k)
+      if (k < cnt) out[k] = stencil_at<V>(cf, P, s, i0 + k, n);
+  }
+}
+
+// Outputs per sliding window: Out::KB where the epilogue declares it (heavy
+// epilogues keep fewer outputs in flight to stay within registers), else 8.
+template <class Out, class = void>
+struct out_kb {
+  static constexpr int value = 8;
+};
+template <class Out>
+struct out_kb<Out, std::void_t<decltype(Out::KB)>> {
+  static constexpr int value = Out::KB;
+};
+
 // Generic pencil kernel.  AX: 0 = x, 1 = y, 2 = z.  In::load(idx) -> V,
 // Out::store(idx, ix, iy, iz, d, red) with Out::NR per-thread partial sums
-// reduced per workgroup into Out::partials.
-template <typename V, int AX, class In, class Out>
+// reduced per workgroup into Out::partials.  For AX = 1, 2 a workgroup owns TX
+// consecutive x columns of whole pencils; each thread evaluates a contiguous
+// run of outputs along the pencil with a register window.
+template <typename V, int AX, int TX, class In, class Out>
 __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, Dims d, int tx) {
   extern __shared__ unsigned char smem_raw[];
   V* lds = reinterpret_cast<V*>(smem_raw);
@@ -84,7 +131,8 @@ __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, 
       }
     }
   } else {
-    const int x0 = blockIdx.x * tx;
+    constexpr int TPC = 256 / TX, KB = out_kb<Out>::value;
+    const int x0 = blockIdx.x * TX;
     size_t base, stride;
     if (AX == 1) {
       base = (size_t)blockIdx.y * d.nx * d.ny + x0;
@@ -93,19 +141,30 @@ __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, 
       base = (size_t)blockIdx.y * d.nx + x0;
       stride = (size_t)d.nx * d.ny;
     }
-    for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-      const int xl = e % tx, j = e / tx;
-      lds[e] = (x0 + xl < d.nx) ? in.load(base + j * stride + xl) : vzero<V>();
-    }
+    const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+    const bool col_ok = x0 + xl < d.nx;
+    // element e = j*TX + xl of the tile; thread order keeps x fastest (coalesced)
+    const int cmax = d.nx - 1 - x0;  // columns past the grid edge are loaded clamped and never stored
+    lds_fill<8, 256>(lds, n * TX, [&](int e) {
+      const int j = e / TX, c = min(e % TX, cmax);
+      return in.load(base + j * stride + c);
+    });
     __syncthreads();
-    for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-      const int xl = e % tx, i = e / tx;
-      if (x0 + xl < d.nx) {
-        const V v = stencil_at<V>(cf, lds + xl, tx, i, n);
-        const int ix = x0 + xl;
-        const int iy = (AX == 1) ? i : (int)blockIdx.y;
-        const int iz = (AX == 1) ? (int)blockIdx.y : i;
-        out.store(base + i * stride + xl, ix, iy, iz, v, red);
+    if (col_ok) {
+      const int run = n / TPC, i_beg = t * run;
+      for (int i0 = i_beg; i0 < i_beg + run; i0 += KB) {
+        const int cnt = (i_beg + run - i0 < KB) ? i_beg + run - i0 : KB;
+        V v[KB];
+        stencil_run<V, KB>(cf, lds + xl, TX, i0, cnt, n, v);
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          if (k < cnt) {
+            const int i = i0 + k;
+            const int iy = (AX == 1) ? i : (int)blockIdx.y;
+            const int iz = (AX == 1) ? (int)blockIdx.y : i;
+            out.store(base + i * stride + xl, x0 + xl, iy, iz, v[k], red);
+          }
+        }
       }
     }
   }
@@ -118,6 +177,14 @@ struct PencilGeom {
   int tx;
   size_t nblocks;
 };
+
+// column-tile width of the y/z pencil kernels: 32 real / 16 complex columns
+// (256 B rows), 16 complex columns need 64 KB of LDS at n = 256, so 8 there
+template <typename V>
+inline int pencil_tx(int n) {
+  if (sizeof(V) == 8) return 32;
+  return (n > 128) ? 8 : 16;
+}
 
 template <typename V>
 inline PencilGeom pencil_geom(const Dims& d, int ax) {
@@ -132,9 +199,7 @@ inline PencilGeom pencil_geom(const Dims& d, int ax) {
     g.lds = sizeof(V) * (size_t)rows * d.nx;
   } else {
     const int n = (ax == 1) ? d.ny : d.nz;
-    int tx = (sizeof(V) == 8) ? 32 : 16;
-    while (tx > 4 && sizeof(V) * (size_t)tx * n > 48 * 1024) tx /= 2;  // LDS per workgroup
-    if (tx > d.nx) tx = d.nx;
+    const int tx = pencil_tx<V>(n);
     g.tx = tx;
     g.grid = dim3((d.nx + tx - 1) / tx, (ax == 1) ? d.nz : d.ny);
     g.lds = sizeof(V) * (size_t)tx * n;
@@ -147,7 +212,12 @@ template <typename V, int AX, class In, class Out>
 inline void launch_pencil(hipStream_t s, const Dims& d, const StencilCoef& cf, const In& in,
                           const Out& out) {
   PencilGeom g = pencil_geom<V>(d, AX);
-  hipLaunchKernelGGL((k_pencil<V, AX, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
+  if (AX == 0 || g.tx == 32)
+    hipLaunchKernelGGL((k_pencil<V, AX, 32, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
+  else if (g.tx == 16)
+    hipLaunchKernelGGL((k_pencil<V, AX, 16, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
+  else
+    hipLaunchKernelGGL((k_pencil<V, AX, 8, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
 }
 
 // ---- fused x + y derivatives ------------------------------------------------
@@ -175,45 +245,72 @@ __device__ __forceinline__ V stencil_tile_x(const StencilCoef& cf, const V* row,
 }
 
 // In::load2(i, a, b): a feeds the x derivative, b the y derivative (SAME: a == b)
-template <typename V, bool SAME, class In, class Out>
+template <typename V, bool SAME, int TX, class In, class Out>
 __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef cfx, StencilCoef cfy, Dims d,
-                                                   int tx) {
+                                                   int tx_unused) {
   extern __shared__ unsigned char smem_raw[];
   constexpr int NR = Out::NR;
   double red[NR > 0 ? NR : 1];
 #pragma unroll
   for (int r = 0; r < (NR > 0 ? NR : 1); ++r) red[r] = 0.0;
-  const int TW = tx + 12, nx = d.nx, ny = d.ny;
+  constexpr int TW = TX + 12, TPC = 256 / TX, KB = out_kb<Out>::value;
+  const int nx = d.nx, ny = d.ny;
   V* A = reinterpret_cast<V*>(smem_raw);
   V* B = SAME ? A : A + TW * ny;
-  const int x0 = blockIdx.x * tx, z = blockIdx.y;
+  const int x0 = blockIdx.x * TX, z = blockIdx.y;
   const size_t zb = (size_t)z * nx * ny;
-  for (int e = threadIdx.x; e < TW * ny; e += blockDim.x) {
-    const int xl = e % TW, y = e / TW;
-    int gx = x0 - 6 + xl;
-    double sg = 1.0;
-    bool ok = true;
-    if (gx < 0 || gx >= nx) {
-      if (cfx.mode != 0) {
-        sg = (cfx.mode == 2) ? -1.0 : 1.0;
-        gx += (gx < 0) ? nx : -nx;
-      } else {
-        ok = false;
+  {
+    // Halo slots outside the box (non-periodic modes) are never read by any
+    // output, so they are filled from a clamped in-range address; periodic
+    // halos hold the wrapped value times the antiperiodic sign.
+    constexpr int U = SAME ? 8 : 4;
+    const double sgw = (cfx.mode == 2) ? -1.0 : 1.0;
+    for (int e0 = threadIdx.x; e0 < TW * ny; e0 += 256 * U) {
+      V ta[U], tb[U];
+      double sg[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = min(e0 + u * 256, TW * ny - 1);
+        const int y = e / TW, xl = e - y * TW;
+        int gx = x0 - 6 + xl;
+        sg[u] = 1.0;
+        if (gx < 0 || gx >= nx) {
+          if (cfx.mode != 0) {
+            sg[u] = sgw;
+            gx += (gx < 0) ? nx : -nx;
+          } else {
+            gx = min(max(gx, 0), nx - 1);
+          }
+        }
+        in.load2(zb + (size_t)y * nx + gx, ta[u], tb[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * 256;
+        if (e < TW * ny) {
+          A[e] = vmul(sg[u], ta[u]);
+          if (!SAME) B[e] = tb[u];
+        }
       }
     }
-    V va = vzero<V>(), vb = vzero<V>();
-    if (ok) in.load2(zb + (size_t)y * nx + gx, va, vb);
-    A[e] = vmul(sg, va);
-    if (!SAME) B[e] = vb;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < tx * ny; e += blockDim.x) {
-    const int xl = e % tx, y = e / tx;
-    const int gx = x0 + xl;
-    if (gx < nx) {
-      const V dx = stencil_tile_x<V>(cfx, A + y * TW, gx, nx, x0);
-      const V dy = stencil_at<V>(cfy, B + xl + 6, TW, y, ny);
-      out.store(zb + (size_t)y * nx + gx, gx, y, z, dx, dy, red);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int gx = x0 + xl;
+  if (gx < nx) {
+    const int run = ny / TPC, i_beg = t * run;
+    for (int y0 = i_beg; y0 < i_beg + run; y0 += KB) {
+      const int cnt = (i_beg + run - y0 < KB) ? i_beg + run - y0 : KB;
+      V dy[KB];
+      stencil_run<V, KB>(cfy, B + xl + 6, TW, y0, cnt, ny, dy);
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        if (k < cnt) {
+          const int y = y0 + k;
+          const V dx = stencil_tile_x<V>(cfx, A + y * TW, gx, nx, x0);
+          out.store(zb + (size_t)y * nx + gx, gx, y, z, dx, dy[k], red);
+        }
+      }
     }
   }
   if constexpr (NR > 0) block_reduce_store<NR>(red, out.partials);
@@ -227,9 +324,7 @@ inline PencilGeom pencil_xy_geom(const Dims& d) {
   auto lds = [&](int t) { return sizeof(V) * (size_t)(t + 12) * d.ny * (SAME ? 1 : 2); };
   while (tx > 8 && lds(tx) > 64 * 1024) tx /= 2;
   // the last tile must start at or before nx-7 so that the one-sided rows of the
-  // last six points read inside the tile
-  while (tx > 8 && d.nx % tx != 0 && d.nx % tx < 7) tx /= 2;
-  if (tx > d.nx) tx = d.nx;
+  // last six points read inside the tile (true for every supported edge)
   g.tx = tx;
   g.grid = dim3((d.nx + tx - 1) / tx, d.nz);
   g.lds = lds(tx);
@@ -237,17 +332,25 @@ inline PencilGeom pencil_xy_geom(const Dims& d) {
   return g;
 }
 
+template <typename V, bool SAME, int TX, class In, class Out>
+inline void launch_pencil_xy_t(hipStream_t s, const PencilGeom& g, const Dims& d, const StencilCoef& cfx,
+                               const StencilCoef& cfy, const In& in, const Out& out) {
+  static bool attr = false;
+  if (!attr && g.lds > 64 * 1024) {
+    hipFuncSetAttribute((const void*)k_pencil_xy<V, SAME, TX, In, Out>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_pencil_xy<V, SAME, TX, In, Out>), g.grid, g.block, g.lds, s, in, out, cfx, cfy, d, g.tx);
+}
+
 template <typename V, bool SAME, class In, class Out>
 inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cfx, const StencilCoef& cfy,
                              const In& in, const Out& out) {
   PencilGeom g = pencil_xy_geom<V, SAME>(d);
-  static bool attr = false;
-  if (!attr && g.lds > 64 * 1024) {
-    hipFuncSetAttribute((const void*)k_pencil_xy<V, SAME, In, Out>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        160 * 1024);
-    attr = true;
-  }
-  hipLaunchKernelGGL((k_pencil_xy<V, SAME, In, Out>), g.grid, g.block, g.lds, s, in, out, cfx, cfy, d, g.tx);
+  if (g.tx == 32) launch_pencil_xy_t<V, SAME, 32>(s, g, d, cfx, cfy, in, out);
+  else if (g.tx == 16) launch_pencil_xy_t<V, SAME, 16>(s, g, d, cfx, cfy, in, out);
+  else launch_pencil_xy_t<V, SAME, 8>(s, g, d, cfx, cfy, in, out);
 }
 
 // ---- common functors ---------------------------------------------------------

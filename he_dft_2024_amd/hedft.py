@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libhedft.so")
+# HEDFT_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("HEDFT_LIB") or os.path.join(HERE, "_build", "libhedft.so")
 
 MODE_STATICS, MODE_TDDFT = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
