@@ -38,7 +38,11 @@ def parse():
     ap.add_argument("--n4", type=float, default=1000.0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="PC iterations in the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="PC iterations in the CPU sample")
+    ap.add_argument("--probe-steps", type=int, default=24,
+                    help="eager PC iterations timed per kernel for the roofline object")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 512 MiB device copy (counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -55,8 +59,44 @@ def algorithmic_bytes_per_pc_step(n: int, w: int = 8) -> float:
     return 2 * b_poten + 2 * b_pc
 
 
+def bytes_fused_step_kernel(n: int, w: int = 8) -> float:
+    """Algorithmic bytes of one fused H psi + predictor (or corrector) launch:
+    B_pred = B_corr = 8 (2w) N + 2 w N (complex psi history reads/writes, pot4
+    in, den out); the Laplacian's x+y partial sum is an intermediate of the
+    two-pass stencil, not algorithmic traffic."""
+    N = n ** 3
+    return 8 * (2 * w) * N + 2 * w * N
+
+
+def load_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's counters), as committed
+    next to the profiles; None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(f"{kernel}@{n}")
+    except (OSError, ValueError):
+        return None
+
+
+def copy_calibration():
+    import torch
+    x = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_calib:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py: this build runs one droplet per GPU on a single GPU; "
               f"--gpus {a.gpus} is not supported (the multi-GPU design is documented, not built).",
@@ -89,6 +129,21 @@ def main():
     bytes_step = algorithmic_bytes_per_pc_step(n)
     step_gbs = bytes_step * steps_per_s / 1e9
 
+    # dominant kernel: the fused H psi + predictor / corrector z-pass, timed
+    # live with HIP events on the library's main stream over real PC steps
+    roofline = None
+    if a.probe_steps > 0:
+        ms_pred, ms_corr = drop.probe_pc(dt, a.probe_steps)
+        kname, kms = ("fused_hpsi_corrector", ms_corr) if ms_corr >= ms_pred else ("fused_hpsi_predictor", ms_pred)
+        kb = bytes_fused_step_kernel(n)
+        ach = kb / (kms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname, n),
+                    "kernel": kname, "avg_ms": round(kms, 5), "algorithmic_bytes": kb,
+                    "avg_ms_pred": round(ms_pred, 5), "avg_ms_corr": round(ms_corr, 5),
+                    "launches": a.probe_steps}
+        e = drop.energy()
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
@@ -111,7 +166,8 @@ def main():
                    "grid": n, "n4": a.n4, "dt_fs": 0.5, "graph": use_graph},
         "step_algorithmic_GBps": round(step_gbs, 1),
         "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-        "roofline": None,
+        "step_algorithmic_bytes": bytes_step,
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "energy_per_atom_K": e["etot4"] / a.n4,
     }

@@ -186,6 +186,10 @@ struct hedft_ctx {
   int* lstop;       // device flag
   double mass[HEDFT_MAX_IMP];
   double last_ms;
+  // kernel-duration probe: events around the fused predictor (0) and
+  // corrector (1) launches while kt_on
+  int kt_on, kt_cap, kt_n[2];
+  hipEvent_t* kt_ev[2];  // kt_cap pairs each
   // graph of one PC iteration
   hipGraph_t graph;
   hipGraphExec_t graph_exec;

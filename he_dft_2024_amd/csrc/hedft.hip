@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 #include <utility>
 
 #include "stencil_kernel.h"
@@ -890,6 +891,13 @@ static HpsiC make_hpsic(hedft_ctx* c) {
 static inline double2* cz(void* p) { return (double2*)p; }
 static inline double* rz(void* p) { return (double*)p; }
 
+// kernel-duration probe around one fused stepper launch (slot 0 pred, 1 corr)
+static void kt_mark(hedft_ctx* c, int slot, int end) {
+  if (!c->kt_on || c->kt_n[slot] >= c->kt_cap) return;
+  hipEventRecord(c->kt_ev[slot][2 * c->kt_n[slot] + end], c->stream);
+  if (end) c->kt_n[slot]++;
+}
+
 // STEPRK (Dynamics_TDDFT/steprk.f90:1-144)
 static int steprk_impl(hedft_ctx* c, double dt) {
   for (int jrun = 0; jrun < 4; ++jrun) {
@@ -937,7 +945,9 @@ static int steppc_impl(hedft_ctx* c, double dt) {
     o.ho2 = cz(c->hpsiold[c->ioldh[1]]);
     o.den = c->den;
     o.dt = dt;
+    kt_mark(c, 0, 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    kt_mark(c, 0, 1);
     c->lap_fresh = 0;
     std::swap(c->ioldh[0], c->ioldh[1]);
   }
@@ -960,7 +970,9 @@ static int steppc_impl(hedft_ctx* c, double dt) {
     o.ho2 = cz(c->hpsiold[c->ioldh[1]]);
     o.den = c->den;
     o.dt = dt;
+    kt_mark(c, 1, 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    kt_mark(c, 1, 1);
     c->lap_fresh = 0;
     finalize<1>(c, gen_partials(c), pencil_geom<double2>(c->d, 2).nblocks, S_ERRPC, 1.0);
     const int t = c->ioldp[2];
@@ -1750,6 +1762,54 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
   HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   if (elapsed_ms) *elapsed_ms = ms;
+  return done(c);
+}
+
+int hedft_tddft_probe_pc(hedft_ctx* c, double deltat, int32_t nsteps, double* avg_ms_pred,
+                         double* avg_ms_corr) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT || nsteps <= 0 || nsteps > 100000) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  hipSetDevice(c->device);
+  std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
+  int rc = HEDFT_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      rc = HEDFT_EHIP;
+      break;
+    }
+  if (rc == HEDFT_OK) {
+    c->kt_ev[0] = ev.data();
+    c->kt_ev[1] = ev.data() + 2 * (size_t)nsteps;
+    c->kt_cap = nsteps;
+    c->kt_n[0] = c->kt_n[1] = 0;
+    c->kt_on = 1;
+    if (!c->lap_fresh) rc = lap_xy(c, c->psi, 1);
+    if (rc == HEDFT_OK) c->lap_fresh = 1;
+    for (int s = 0; s < nsteps && rc == HEDFT_OK; ++s) {
+      rc = steppc_impl(c, deltat);
+      if (rc == HEDFT_OK) rc = tddft_after_step(c);
+    }
+    c->kt_on = 0;
+    if (rc == HEDFT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = HEDFT_EHIP;
+    double* outs[2] = {avg_ms_pred, avg_ms_corr};
+    for (int slot = 0; slot < 2 && rc == HEDFT_OK; ++slot) {
+      double sum = 0.0;
+      for (int k = 0; k < c->kt_n[slot]; ++k) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->kt_ev[slot][2 * k], c->kt_ev[slot][2 * k + 1]) != hipSuccess) {
+          rc = HEDFT_EHIP;
+          break;
+        }
+        sum += ms;
+      }
+      if (outs[slot]) *outs[slot] = c->kt_n[slot] ? sum / c->kt_n[slot] : 0.0;
+    }
+    c->kt_ev[0] = c->kt_ev[1] = nullptr;
+    c->kt_cap = 0;
+  }
+  for (auto& e : ev)
+    if (e) hipEventDestroy(e);
+  if (rc != HEDFT_OK) return rc;
   return done(c);
 }
 

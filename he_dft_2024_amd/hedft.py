@@ -97,6 +97,7 @@ def lib():
         L.hedft_tddft_iteration.argtypes = [vp, d, ip, ip, dp, dp, dp]
         L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
         L.hedft_last_elapsed.argtypes = [vp, dp]
+        L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
         _lib = L
     return _lib
 
@@ -278,3 +279,10 @@ class Droplet:
         ms = C.c_double(0)
         check(lib().hedft_tddft_run_pc(self._c, dt, nsteps, int(use_graph), C.byref(ms)), "run_pc")
         return ms.value
+
+    def probe_pc(self, dt, nsteps):
+        """nsteps eager PC iterations; average device ms of the fused predictor
+        and corrector kernels (HIP events on the context's main stream)."""
+        a, b = C.c_double(0), C.c_double(0)
+        check(lib().hedft_tddft_probe_pc(self._c, dt, nsteps, C.byref(a), C.byref(b)), "probe_pc")
+        return a.value, b.value

@@ -218,8 +218,16 @@ int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk
 int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph,
                        double* elapsed_ms);
 
+/* Measurement hook (no counterpart in the reference): nsteps eager PC
+ * iterations, exactly as hedft_tddft_run_pc(use_graph=0) runs them, with HIP
+ * events on the context's main stream around each launch of the fused
+ * H psi + predictor kernel and the fused H psi + corrector kernel; returns
+ * their average device durations (ms). */
+int hedft_tddft_probe_pc(hedft_ctx* c, double deltat, int32_t nsteps, double* avg_ms_pred,
+                         double* avg_ms_corr);
+
 /* Device time (ms, HIP events on the context stream) of the last
- * hedft_tddft_run_pc / hedft_time_kernel call. */
+ * hedft_tddft_run_pc call. */
 int hedft_last_elapsed(hedft_ctx* c, double* ms);
 
 #ifdef __cplusplus
