@@ -72,7 +72,11 @@ struct ImpDev {
   double x0, y0, z0, hx, hy, hz;
 };
 
+// Products and sums here are rounded separately (no FMA contraction), as a
+// plain CPU build evaluates x(i) = -xmax + h*(i-1) and r; the table lookup is
+// discontinuous where r/DelInter is an integer, so r must match to the last bit.
 __device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, double zz, bool flag) {
+#pragma clang fp contract(off)
   const double rx = im.rimp[k], ry = im.rimp[k + im.nimp], rz = im.rimp[k + 2 * im.nimp];
   const double zt = (zz - rz) * (zz - rz);
   const double yt = (yy - ry) * (yy - ry) + zt;
@@ -81,16 +85,33 @@ __device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, 
     if (flag) *im.lstop = 1;
     r = im.rmax;
   }
-  int ir = (int)(r / im.delinter);
+  // ir = int(r/DelInter), rmod = mod(r,DelInter)/DelInter (potenimp.f90:101-111).
+  // mod is the exact remainder: when r/DelInter rounds up to an integer the
+  // remainder is just below DelInter while ir already points one node on (the
+  // reference behaves so at such points, e.g. grid points exactly 4 A from an
+  // impurity sitting on a grid line).
+  const double q = trunc(r / im.delinter);
+  int ir = (int)q;
   if (ir > im.npot - 2) ir = im.npot - 2;
-  const double q = r / im.delinter;
-  const double rmod = (r - (double)(long long)q * im.delinter) / im.delinter;
+  double rem = fma(-q, im.delinter, r);
+  if (rem < 0.0) rem += im.delinter;
+  else if (rem >= im.delinter) rem -= im.delinter;
+  const double rmod = rem / im.delinter;
   const double* tab = im.potion + (size_t)k * im.npot;
   return tab[ir] * (1.0 - rmod) + tab[ir + 1] * rmod;
 }
 
+// grid coordinates x_i = -xmax + h*i, rounded like the CPU evaluation
+__device__ inline void imp_xyz(const ImpDev& im, int ix, int iy, int iz, double& xx, double& yy, double& zz) {
+#pragma clang fp contract(off)
+  xx = im.x0 + im.hx * ix;
+  yy = im.y0 + im.hy * iy;
+  zz = im.z0 + im.hz * iz;
+}
+
 __device__ inline double imp_u(const ImpDev& im, int ix, int iy, int iz) {
-  double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
+  double xx, yy, zz;
+  imp_xyz(im, ix, iy, iz, xx, yy, zz);
   double u = 0.0;
   for (int k = 0; k < im.nimp; ++k) u += imp_u_k(im, k, xx, yy, zz, false);
   return u;

@@ -1559,6 +1559,7 @@ int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, dou
 int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delinter, double rmaxinterpol,
                          const double* potion, const double* mass) {
   if (!c || nimp < 0 || nimp > HEDFT_MAX_IMP || (nimp > 0 && (!potion || !mass || npot < 2))) return HEDFT_EINVAL;
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old table
   if (c->potion) hipFree(c->potion);
   c->potion = nullptr;
   c->nimp = nimp;
@@ -1579,9 +1580,11 @@ int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delint
 int hedft_set_impurity_state(hedft_ctx* c, const double* rimp, const double* vimp, const double* aimp) {
   if (!c) return HEDFT_EINVAL;
   const size_t b = sizeof(double) * 3 * c->nimp;
-  if (rimp) HEDFT_CHECK(hipMemcpy(c->imp + IMP_R, rimp, b, hipMemcpyHostToDevice));
-  if (vimp) HEDFT_CHECK(hipMemcpy(c->imp + IMP_V, vimp, b, hipMemcpyHostToDevice));
-  if (aimp) HEDFT_CHECK(hipMemcpy(c->imp + IMP_A, aimp, b, hipMemcpyHostToDevice));
+  // ordered after the work already queued on the context's stream
+  if (rimp) HEDFT_CHECK(hipMemcpyAsync(c->imp + IMP_R, rimp, b, hipMemcpyHostToDevice, c->stream));
+  if (vimp) HEDFT_CHECK(hipMemcpyAsync(c->imp + IMP_V, vimp, b, hipMemcpyHostToDevice, c->stream));
+  if (aimp) HEDFT_CHECK(hipMemcpyAsync(c->imp + IMP_A, aimp, b, hipMemcpyHostToDevice, c->stream));
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
   return HEDFT_OK;
 }
 
@@ -1609,8 +1612,8 @@ __global__ void k_check_bounds(ImpDev im, Dims d) {
     for (int cz_ = 0; cz_ < 2; ++cz_)
       for (int cy = 0; cy < 2; ++cy)
         for (int cx = 0; cx < 2; ++cx) {
-          const double xx = im.x0 + im.hx * (cx ? d.nx - 1 : 0), yy = im.y0 + im.hy * (cy ? d.ny - 1 : 0),
-                       zz = im.z0 + im.hz * (cz_ ? d.nz - 1 : 0);
+          double xx, yy, zz;
+          imp_xyz(im, cx ? d.nx - 1 : 0, cy ? d.ny - 1 : 0, cz_ ? d.nz - 1 : 0, xx, yy, zz);
           imp_u_k(im, k, xx, yy, zz, true);
         }
 }
@@ -1629,7 +1632,8 @@ __global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const doubl
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
        i += (long long)gridDim.x * blockDim.x) {
     const int ix = (int)(i % d.nx), iy = (int)((i / d.nx) % d.ny), iz = (int)(i / ((long long)d.nx * d.ny));
-    const double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
+    double xx, yy, zz;
+    imp_xyz(im, ix, iy, iz, xx, yy, zz);
     for (int k = 0; k < im.nimp; ++k) {
       const double u = imp_u_k(im, k, xx, yy, zz, false);
       red[3 * k] += gx[i] * u;

@@ -143,7 +143,8 @@ struct OutGradZForce {
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double v, double* red) const {
     out[i] = v;
     if (want) {
-      const double xx = im.x0 + im.hx * ix, yy = im.y0 + im.hy * iy, zz = im.z0 + im.hz * iz;
+      double xx, yy, zz;
+      imp_xyz(im, ix, iy, iz, xx, yy, zz);
       const double a = gx[i], b = gy[i];
       for (int k = 0; k < im.nimp; ++k) {
         const double u = imp_u_k(im, k, xx, yy, zz, false);

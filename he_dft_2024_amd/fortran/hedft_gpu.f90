@@ -1,0 +1,264 @@
+!> hedft_gpu — ISO_C_BINDING interface of libhedft (include/hedft.h) for the
+!> Fortran drivers of BCN-4He-DFT.  Every interface binds one C entry point;
+!> the derived types mirror hedft_grid, hedft_params and hedft_energies
+!> field by field.  hedft_check() stops with the library's message, the
+!> reference's own error convention (He_statics/BCN4HeDFT.f90:233-235).
+module hedft_gpu
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  ! status codes and modes
+  integer(c_int), parameter :: HEDFT_OK = 0, HEDFT_EINVAL = -1, HEDFT_EGRID = -2, &
+       HEDFT_EHIP = -3, HEDFT_ENOMEM = -4, HEDFT_ESTATE = -5, HEDFT_EBOUNDS = -6
+  integer(c_int32_t), parameter :: HEDFT_MODE_STATICS = 0, HEDFT_MODE_TDDFT = 1
+  integer(c_int32_t), parameter :: HEDFT_PREC_FP64 = 0, HEDFT_PREC_MIXED = 1
+  integer(c_int32_t), parameter :: HEDFT_MAX_IMP = 4
+
+  ! field identifiers (enum hedft_field)
+  integer(c_int32_t), parameter :: HEDFT_DEN = 0, HEDFT_PSI = 1, HEDFT_POT4 = 2, &
+       HEDFT_DENCG = 3, HEDFT_DELJ4 = 4, HEDFT_DENALF = 5, HEDFT_FALFS = 6, &
+       HEDFT_INTXALF = 7, HEDFT_INTYALF = 8, HEDFT_INTZALF = 9, HEDFT_UALPHAS = 10, &
+       HEDFT_DXDEN = 11, HEDFT_DYDEN = 12, HEDFT_DZDEN = 13, HEDFT_UIMP = 14, &
+       HEDFT_UEXT = 15, HEDFT_TIMEC = 16, HEDFT_PSIOLD1 = 17, HEDFT_PSIOLD2 = 18, &
+       HEDFT_PSIOLD3 = 19, HEDFT_HPSIOLD1 = 20, HEDFT_HPSIOLD2 = 21, HEDFT_PC = 22, &
+       HEDFT_Q = 23, HEDFT_FDEN = 24, HEDFT_FVLJ4 = 25, HEDFT_WCGK = 26, &
+       HEDFT_KALFS = 27, HEDFT_WORK1 = 28, HEDFT_WORK2 = 29, HEDFT_WORKC1 = 30, &
+       HEDFT_WORKK1 = 31
+
+  type, bind(C) :: hedft_grid_t
+    integer(c_int32_t) :: nx, ny, nz, pad_
+    real(c_double) :: xmax, ymax, zmax
+  end type hedft_grid_t
+
+  type, bind(C) :: hedft_params_t
+    real(c_double) :: h2o2m4, cp4, cpp4, alphas, l, den0s
+    real(c_double) :: csolid, beta_solid, den_m
+    real(c_double) :: h4, eps4, sigma4
+    integer(c_int32_t) :: core_otc, lsolid, npd, icon
+    real(c_double) :: dtmin, sdmu, dmumax, psimin
+    real(c_double) :: lambdah, tmean(3), tsurf(3)
+    integer(c_int32_t) :: precision, lexternal
+  end type hedft_params_t
+
+  type, bind(C) :: hedft_energies_t
+    real(c_double) :: ekin4, elj4, ecor4, ealphas, esolid, eext, etot4
+    real(c_double) :: ekinx, eHeX, eimpu_impu, eimpu, etot, norm
+  end type hedft_energies_t
+
+  interface
+    function hedft_strerror_c(rc) bind(C, name="hedft_strerror")
+      import :: c_int, c_ptr
+      integer(c_int), value :: rc
+      type(c_ptr) :: hedft_strerror_c
+    end function
+    subroutine hedft_default_params(p) bind(C, name="hedft_default_params")
+      import :: hedft_params_t
+      type(hedft_params_t), intent(out) :: p
+    end subroutine
+    integer(c_int) function hedft_grid_supported(n) bind(C, name="hedft_grid_supported")
+      import :: c_int, c_int32_t
+      integer(c_int32_t), value :: n
+    end function
+    integer(c_int) function hedft_create(g, p, mode, device, ctx) bind(C, name="hedft_create")
+      import :: c_int, c_int32_t, c_ptr, hedft_grid_t, hedft_params_t
+      type(hedft_grid_t), intent(in) :: g
+      type(hedft_params_t), intent(in) :: p
+      integer(c_int32_t), value :: mode, device
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function hedft_destroy(ctx) bind(C, name="hedft_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    ! host: any contiguous array in the module layout (assumed type -> void*)
+    integer(c_int) function hedft_upload(ctx, field, host, bytes) bind(C, name="hedft_upload")
+      import :: c_int, c_int32_t, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: field
+      type(*), intent(in) :: host(*)
+      integer(c_size_t), value :: bytes
+    end function
+    integer(c_int) function hedft_download(ctx, field, host, bytes) bind(C, name="hedft_download")
+      import :: c_int, c_int32_t, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: field
+      type(*), intent(inout) :: host(*)
+      integer(c_size_t), value :: bytes
+    end function
+    integer(c_size_t) function hedft_field_bytes(ctx, field) bind(C, name="hedft_field_bytes")
+      import :: c_int32_t, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: field
+    end function
+    integer(c_int) function hedft_synchronize(ctx) bind(C, name="hedft_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_derivnD(ctx, id, iv, fin, fout) bind(C, name="hedft_derivnD")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: id, iv, fin, fout
+    end function
+    integer(c_int) function hedft_fftfw(ctx, fin, fout) bind(C, name="hedft_fftfw")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: fin, fout
+    end function
+    integer(c_int) function hedft_fftbk(ctx, fin, fout) bind(C, name="hedft_fftbk")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: fin, fout
+    end function
+    integer(c_int) function hedft_derden(ctx) bind(C, name="hedft_derden")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_poten(ctx) bind(C, name="hedft_poten")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_term_alfa(ctx) bind(C, name="hedft_term_alfa")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_energy(ctx, e) bind(C, name="hedft_energy")
+      import :: c_int, c_ptr, hedft_energies_t
+      type(c_ptr), value :: ctx
+      type(hedft_energies_t), intent(out) :: e
+    end function
+    integer(c_int) function hedft_steprkr(ctx, deltat, mu4, mu4err, n4) bind(C, name="hedft_steprkr")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err
+    end function
+    integer(c_int) function hedft_steppcr(ctx, deltat, mu4, mu4err, n4, errpc) bind(C, name="hedft_steppcr")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err, errpc
+    end function
+    integer(c_int) function hedft_steprk(ctx, deltat) bind(C, name="hedft_steprk")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+    end function
+    integer(c_int) function hedft_steppc(ctx, deltat, errHe, errimp, errvimp) bind(C, name="hedft_steppc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+      real(c_double), intent(out) :: errHe, errimp, errvimp
+    end function
+    integer(c_int) function hedft_set_impurities(ctx, nimp, npot, delinter, rmaxinterpol, potion, mass) &
+         bind(C, name="hedft_set_impurities")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nimp, npot
+      real(c_double), value :: delinter, rmaxinterpol
+      real(c_double), intent(in) :: potion(*), mass(*)
+    end function
+    ! (N_imp,3) arrays; an absent argument is passed as NULL (left unchanged)
+    integer(c_int) function hedft_set_impurity_state(ctx, rimp, vimp, aimp) bind(C, name="hedft_set_impurity_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in), optional :: rimp(*), vimp(*), aimp(*)
+    end function
+    integer(c_int) function hedft_get_impurity_state(ctx, rimp, vimp, aimp, force, lstopimp) &
+         bind(C, name="hedft_get_impurity_state")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out), optional :: rimp(*), vimp(*), aimp(*), force(*)
+      integer(c_int32_t), intent(out), optional :: lstopimp
+    end function
+    integer(c_int) function hedft_potenimp(ctx) bind(C, name="hedft_potenimp")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_forceimp(ctx) bind(C, name="hedft_forceimp")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_update_aimp(ctx) bind(C, name="hedft_update_aimp")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_statics_init(ctx, n4, mu4) bind(C, name="hedft_statics_init")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: n4
+      real(c_double), intent(out) :: mu4
+    end function
+    integer(c_int) function hedft_statics_iteration(ctx, deltat, iterrk, n4, mu4, mu4err) &
+         bind(C, name="hedft_statics_iteration")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      integer(c_int32_t), value :: iterrk
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err
+    end function
+    integer(c_int) function hedft_tddft_init(ctx) bind(C, name="hedft_tddft_init")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_tddft_iteration(ctx, deltat, iter, nrk, errHe, errimp, errvimp) &
+         bind(C, name="hedft_tddft_iteration")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+      integer(c_int32_t), value :: iter, nrk
+      real(c_double), intent(out) :: errHe, errimp, errvimp
+    end function
+    integer(c_int) function hedft_tddft_run_pc(ctx, deltat, nsteps, use_graph, elapsed_ms) &
+         bind(C, name="hedft_tddft_run_pc")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+      integer(c_int32_t), value :: nsteps, use_graph
+      real(c_double), intent(out) :: elapsed_ms
+    end function
+  end interface
+
+contains
+
+  !> Stops with the library's message when rc is not HEDFT_OK.
+  subroutine hedft_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= HEDFT_OK) then
+      write(6, '(A,A,A,I0,A,A)') 'hedft: ', what, ' failed (', rc, '): ', hedft_strerror(rc)
+      stop 'hedft error'
+    end if
+  end subroutine hedft_check
+
+  function hedft_strerror(rc) result(msg)
+    integer(c_int), intent(in) :: rc
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: cp
+    integer :: n, i
+    cp = hedft_strerror_c(rc)
+    n = c_strlen(cp)
+    call c_f_pointer(cp, p, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = p(i)
+    end do
+  end function hedft_strerror
+
+  integer function c_strlen(cp)
+    type(c_ptr), intent(in) :: cp
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    call c_f_pointer(cp, p, [4096])
+    c_strlen = 0
+    do i = 1, 4096
+      if (p(i) == c_null_char) exit
+      c_strlen = i
+    end do
+  end function c_strlen
+
+end module hedft_gpu

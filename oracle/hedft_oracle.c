@@ -21,6 +21,17 @@
  *
  * Reductions are summed per z-plane and then plane by plane, so results do not
  * depend on the OpenMP thread count.
+ *
+ * Pins (tests/test_oracle.py): stencil matrices equal the reference's own
+ * (compiled from its sources into oracle/_ref), FFT equals numpy and the DFT
+ * definition, kernels equal their defining integrals / closed forms, the mean
+ * field satisfies the uniform-density invariants and dE/drho = pot4, the
+ * steppers show their order on exactly solvable problems, the impurity force
+ * is -dE/dR.  Parity unpinned (restated only): the statics stale-state
+ * sequencing and renormalisation order (steprkr.f90:77-83,
+ * BCN4HeDFT.f90:1131-1160), the impurity PC velocity-slot quirk
+ * (Dynamics_TDDFT/steppc.f90:92) with the step-start aimp, and the absorbing
+ * layer profile (DFT4He3d.f90:592-601).
  */
 #include <complex.h>
 #include <math.h>

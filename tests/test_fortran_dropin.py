@@ -1,0 +1,63 @@
+"""The Fortran side of the drop-in boundary: module hedft_gpu (ISO_C_BINDING)
+driving the library the way the TDDFT driver loop does.  The check program
+(he_dft_2024_amd/fortran/hedft_fortran_check, built by build()) and the Python
+binding run the same RK start + PC iterations on the same inputs; the results
+must be identical, since both call the same C entry points in the same order."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from he_dft_2024_amd import inputs as I
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FDIR = os.path.join(ROOT, "he_dft_2024_amd", "fortran")
+EXE = os.path.join(FDIR, "_build", "hedft_fortran_check")
+
+
+def test_fortran_sources_present():
+    for f in ("hedft_gpu.f90", "hedft_fortran_check.f90", "hedft_dropin_tddft.f90", "Makefile"):
+        assert os.path.exists(os.path.join(FDIR, f)), f
+
+
+@pytest.mark.gpu
+def test_fortran_driver_matches_python_binding(gpu, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran check program not built (amdflang absent at build time)")
+    n, xmax, n4, niter, nrk = 32, 6.4, 60.0, 6, 3
+    dt = 5e-4 / I.PSTOK
+    print("[case] fortran drop-in 32^3 3 RK + 3 PC", flush=True)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=4)
+    potion, delinter, rmax = I.impurity_table(n, xmax)
+    mass = 108.0 * I.MP_U
+    r0 = np.array([0.0, 0.0, -4.0])
+    v0 = np.array([0.0, 0.0, 3.0]) * I.PSTOK
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([n, niter, nrk], dtype=np.int32).tofile(f)
+        np.array([xmax, n4, dt, delinter, rmax, mass], dtype=np.float64).tofile(f)
+        np.array([potion.size], dtype=np.int32).tofile(f)
+        np.asarray(potion, dtype=np.float64).tofile(f)
+        r0.tofile(f)
+        v0.tofile(f)
+        np.ascontiguousarray(den, dtype=np.float64).tofile(f)
+    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    err_f, etot_f, ekin_f, norm_f = raw[:4]
+    psi_f = raw[4:].view(np.complex128).reshape(n, n, n)
+
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+    d.set("DEN", den)
+    d.set_impurities(potion[None, :], delinter, rmax, [mass])
+    d.set_impurity_state(r0[None, :], v0[None, :], np.zeros((1, 3)))
+    d.tddft_init()
+    for it in range(1, niter + 1):
+        e = d.tddft_iteration(dt, it, nrk)
+    en = d.energy()
+    psi = d.field("PSI")
+    d.close()
+    assert err_f == e[0]
+    assert etot_f == en["etot"] and ekin_f == en["ekin4"] and norm_f == en["norm"]
+    assert np.array_equal(psi_f, psi)

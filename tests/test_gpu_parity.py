@@ -196,3 +196,24 @@ def test_graph_replay_matches_eager(gpu):
         d.close()
     assert relerr(outs[1][0], outs[0][0]) < 1e-13
     assert relerr(outs[1][1], outs[0][1]) < 1e-13
+
+
+def test_uimp_on_grid_line_vs_oracle(gpu):
+    """An impurity on a grid line puts grid points at integral r/DelInter
+    (r = 4 A exactly here), where the reference's int()/mod() interpolation
+    (potenimp.f90:101-111) jumps by one table node; both sides take the same
+    branch, and one RK + one PC step stay in parity."""
+    n, xmax, n4 = 32, 6.4, 60.0
+    _say("uimp on grid line")
+    d, o = _pair(n, xmax, gpu.MODE_TDDFT, core_otc=1)
+    dt = 5e-4 / I.PSTOK
+    for x in (d, o):
+        I.tddft_setup(x, n, xmax, n4, seed=0, r0=(0.0, 0.0, -4.0))
+        x.tddft_init()
+    assert relerr(d.field("UIMP"), o.field("UIMP")) < 1e-14
+    for x in (d, o):
+        x.tddft_iteration(dt, 1, 1)
+        x.tddft_iteration(dt, 2, 1)
+    for name in ("PSI", "DEN", "POT4", "HPSIOLD1", "HPSIOLD2", "PC"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    d.close()
