@@ -40,12 +40,38 @@ template <bool INV>
 __host__ __device__ inline double2 mul_mi(double2 a) {
   return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
+template <bool INV>
+__host__ __device__ inline float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __host__ __device__ inline float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// complex type of a real type (float2 / double2) and a generic constructor
+template <typename R>
+struct cvec;
+template <>
+struct cvec<double> {
+  using type = double2;
+};
+template <>
+struct cvec<float> {
+  using type = float2;
+};
+template <typename R>
+using cplx_t = typename cvec<R>::type;
+template <typename C, typename T>
+__host__ __device__ inline C mkc(T x, T y) {
+  C r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
 
 // ---- launch geometry -----------------------------------------------------------
 struct Dims {
@@ -183,6 +209,7 @@ struct hedft_ctx {
   double2 *fden, *spec[4], *workk1;
   double *fvlj4, *wcgk, *kalfs;
   double* tw;       // twiddle tables (complex, interleaved)
+  float* twf;       // the same tables rounded to FP32 (mixed path)
   double* lamtab;   // TDDFT absorbing layer: tanh((|x_a|-mean_a)/surf_a) per axis (nx+ny+nz)
   size_t tw_x, tw_xpost, tw_y, tw_z;  // offsets (in complex elements)
   // reductions
@@ -227,27 +254,36 @@ int fftbk(hedft_ctx* c, double2* in, double* out);
 // x-pass prologue (R2C input) and epilogue (C2R output) operations
 enum XInOp { XIN_COPY = 0, XIN_CORR = 1, XIN_FGRAD = 2, XIN_DOT3 = 3 };
 enum XOutOp { XOUT_STORE = 0, XOUT_POT4 = 1 };
-struct XIn {
+// R: storage type of the real fields (double: FP64 path, float: mixed path);
+// the pointwise arithmetic of prologues and epilogues is done in double.
+template <typename R>
+struct XInT {
   int op;
-  const double* a[6];
+  const R* a[6];
   double c0, c1;
 };
-struct XOut {
+template <typename R>
+struct XOutT {
   int op;
-  double* out;
-  const double *delj4, *dencg, *den, *uext;
+  R* out;
+  const R *delj4, *dencg, *den, *uext;
   double a0, a1, csolid, beta, den_m;
   int lsolid, lext;
 };
-// forward x R2C of up to 3 real inputs into spectra, then y forward
-int fwd_xy(hedft_ctx* c, int nb, const XIn* in, double2* const* spec);
+using XIn = XInT<double>;
+using XOut = XOutT<double>;
+// forward x R2C of up to 4 real inputs into spectra, then y forward
+template <typename R>
+int fwd_xy(hedft_ctx* c, int nb, const XInT<R>* in, cplx_t<R>* const* spec);
 // z pass: optional forward, then for each output o: multiply by kern[o], inverse z.
-// nin inputs share the kernels; out[i*nout+o].  store_fwd[i] (may be null)
+// nin inputs; kern[i*nout+o], out[i*nout+o].  store_fwd[i] (may be null)
 // receives the full forward spectrum.
-int zconv(hedft_ctx* c, int nin, double2* const* in, int nout, const double* const* kern,
-          double2* const* out, int do_fwd, double2* const* store_fwd);
+template <typename R>
+int zconv(hedft_ctx* c, int nin, cplx_t<R>* const* in, int nout, const R* const* kern,
+          cplx_t<R>* const* out, int do_fwd, cplx_t<R>* const* store_fwd);
 // y inverse then x C2R with epilogue for nb spectra
-int inv_yx(hedft_ctx* c, int nb, double2* const* spec, const XOut* out);
+template <typename R>
+int inv_yx(hedft_ctx* c, int nb, cplx_t<R>* const* spec, const XOutT<R>* out);
 
 // stencil.hip
 void stencil_setup(hedft_ctx* c);

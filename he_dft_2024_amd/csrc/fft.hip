@@ -23,48 +23,31 @@
 namespace hedft {
 
 // ---------------------------------------------------------------------------
-// radix-R DFT in registers; INV selects e^{+i}
+// radix-R DFT in registers on C = float2 / double2; INV selects e^{+i}
 // ---------------------------------------------------------------------------
-template <int R, bool INV>
-__device__ __forceinline__ void dft(double2* u);
-
-template <>
-__device__ __forceinline__ void dft<2, false>(double2* u) {
-  double2 a = u[0];
-  u[0] = cadd(a, u[1]);
-  u[1] = csub(a, u[1]);
-}
-template <>
-__device__ __forceinline__ void dft<2, true>(double2* u) {
-  dft<2, false>(u);
-}
-
-template <bool INV>
-__device__ __forceinline__ void dft4(double2& u0, double2& u1, double2& u2, double2& u3) {
-  double2 t0 = cadd(u0, u2), t1 = csub(u0, u2), t2 = cadd(u1, u3), t3 = mul_mi<INV>(csub(u1, u3));
+template <bool INV, typename C>
+__device__ __forceinline__ void dft4(C& u0, C& u1, C& u2, C& u3) {
+  C t0 = cadd(u0, u2), t1 = csub(u0, u2), t2 = cadd(u1, u3), t3 = mul_mi<INV>(csub(u1, u3));
   u0 = cadd(t0, t2);
   u2 = csub(t0, t2);
   u1 = cadd(t1, t3);
   u3 = csub(t1, t3);
 }
-template <>
-__device__ __forceinline__ void dft<4, false>(double2* u) { dft4<false>(u[0], u[1], u[2], u[3]); }
-template <>
-__device__ __forceinline__ void dft<4, true>(double2* u) { dft4<true>(u[0], u[1], u[2], u[3]); }
 
-template <bool INV>
-__device__ __forceinline__ void dft8(double2* u) {
-  double2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
-  double2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+template <bool INV, typename C>
+__device__ __forceinline__ void dft8(C* u) {
+  using T = decltype(u[0].x);
+  C e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+  C o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
   dft4<INV>(e0, e1, e2, e3);
   dft4<INV>(o0, o1, o2, o3);
-  const double r = 0.70710678118654752440;
+  const T r = (T)0.70710678118654752440;
   if (!INV) {
-    o1 = make_double2((o1.x + o1.y) * r, (o1.y - o1.x) * r);
-    o3 = make_double2((o3.y - o3.x) * r, (-o3.x - o3.y) * r);
+    o1 = mkc<C>((o1.x + o1.y) * r, (o1.y - o1.x) * r);
+    o3 = mkc<C>((o3.y - o3.x) * r, (-o3.x - o3.y) * r);
   } else {
-    o1 = make_double2((o1.x - o1.y) * r, (o1.x + o1.y) * r);
-    o3 = make_double2((-o3.x - o3.y) * r, (o3.x - o3.y) * r);
+    o1 = mkc<C>((o1.x - o1.y) * r, (o1.x + o1.y) * r);
+    o3 = mkc<C>((-o3.x - o3.y) * r, (o3.x - o3.y) * r);
   }
   o2 = mul_mi<INV>(o2);
   u[0] = cadd(e0, o0); u[4] = csub(e0, o0);
@@ -72,59 +55,67 @@ __device__ __forceinline__ void dft8(double2* u) {
   u[2] = cadd(e2, o2); u[6] = csub(e2, o2);
   u[3] = cadd(e3, o3); u[7] = csub(e3, o3);
 }
-template <>
-__device__ __forceinline__ void dft<8, false>(double2* u) { dft8<false>(u); }
-template <>
-__device__ __forceinline__ void dft<8, true>(double2* u) { dft8<true>(u); }
 
-template <bool INV>
-__device__ __forceinline__ void dft3(double2* u) {
-  const double s3 = 0.86602540378443864676;
-  double2 s = cadd(u[1], u[2]);
-  double2 t = make_double2(u[0].x - 0.5 * s.x, u[0].y - 0.5 * s.y);
-  double2 d = mul_mi<INV>(csub(u[1], u[2]));
+template <bool INV, typename C>
+__device__ __forceinline__ void dft3(C* u) {
+  using T = decltype(u[0].x);
+  const T s3 = (T)0.86602540378443864676;
+  C s = cadd(u[1], u[2]);
+  C t = mkc<C>(u[0].x - (T)0.5 * s.x, u[0].y - (T)0.5 * s.y);
+  C d = mul_mi<INV>(csub(u[1], u[2]));
   d = cscale(d, s3);
   u[0] = cadd(u[0], s);
   u[1] = cadd(t, d);
   u[2] = csub(t, d);
 }
-template <>
-__device__ __forceinline__ void dft<3, false>(double2* u) { dft3<false>(u); }
-template <>
-__device__ __forceinline__ void dft<3, true>(double2* u) { dft3<true>(u); }
 
-template <bool INV>
-__device__ __forceinline__ void dft5(double2* u) {
-  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-  const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-  double2 a1 = cadd(u[1], u[4]), d1 = csub(u[1], u[4]);
-  double2 a2 = cadd(u[2], u[3]), d2 = csub(u[2], u[3]);
-  double2 y0 = cadd(u[0], cadd(a1, a2));
-  double2 p1 = make_double2(u[0].x + c1 * a1.x + c2 * a2.x, u[0].y + c1 * a1.y + c2 * a2.y);
-  double2 p2 = make_double2(u[0].x + c2 * a1.x + c1 * a2.x, u[0].y + c2 * a1.y + c1 * a2.y);
-  double2 b1 = mul_mi<INV>(make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
-  double2 b2 = mul_mi<INV>(make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+template <bool INV, typename C>
+__device__ __forceinline__ void dft5(C* u) {
+  using T = decltype(u[0].x);
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+  C a1 = cadd(u[1], u[4]), d1 = csub(u[1], u[4]);
+  C a2 = cadd(u[2], u[3]), d2 = csub(u[2], u[3]);
+  C y0 = cadd(u[0], cadd(a1, a2));
+  C p1 = mkc<C>(u[0].x + c1 * a1.x + c2 * a2.x, u[0].y + c1 * a1.y + c2 * a2.y);
+  C p2 = mkc<C>(u[0].x + c2 * a1.x + c1 * a2.x, u[0].y + c2 * a1.y + c1 * a2.y);
+  C b1 = mul_mi<INV>(mkc<C>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+  C b2 = mul_mi<INV>(mkc<C>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
   u[0] = y0;
   u[1] = cadd(p1, b1);
   u[4] = csub(p1, b1);
   u[2] = cadd(p2, b2);
   u[3] = csub(p2, b2);
 }
-template <>
-__device__ __forceinline__ void dft<5, false>(double2* u) { dft5<false>(u); }
-template <>
-__device__ __forceinline__ void dft<5, true>(double2* u) { dft5<true>(u); }
+
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void dft(C* u) {
+  if constexpr (R == 2) {
+    C a = u[0];
+    u[0] = cadd(a, u[1]);
+    u[1] = csub(a, u[1]);
+  } else if constexpr (R == 3) {
+    dft3<INV>(u);
+  } else if constexpr (R == 4) {
+    dft4<INV>(u[0], u[1], u[2], u[3]);
+  } else if constexpr (R == 5) {
+    dft5<INV>(u);
+  } else {
+    static_assert(R == 8, "radix");
+    dft8<INV>(u);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // One Stockham autosort pass (radix R, P = product of earlier radices) over a
 // column of length L held in LDS with element stride `stride`, executed by T
 // threads (t = 0..T-1).  tw[m] = exp(-2 pi i m / L).
 // ---------------------------------------------------------------------------
-template <int L, int R, int P, bool INV, int T>
-__device__ __forceinline__ void spass(double2* col, int stride, int t, const double2* __restrict__ tw) {
+template <int L, int R, int P, bool INV, int T, typename C>
+__device__ __forceinline__ void spass(C* col, int stride, int t, const C* __restrict__ tw) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
-  double2 u[PER][R];
+  C u[PER][R];
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     const int i = t + b * T;
@@ -135,7 +126,7 @@ __device__ __forceinline__ void spass(double2* col, int stride, int t, const dou
         const int k = i % P;
 #pragma unroll
         for (int j = 1; j < R; ++j) {
-          double2 w = tw[(k * j) * (L / (P * R))];
+          C w = tw[(k * j) * (L / (P * R))];
           if (INV) w.y = -w.y;
           u[b][j] = cmul(u[b][j], w);
         }
@@ -157,8 +148,8 @@ __device__ __forceinline__ void spass(double2* col, int stride, int t, const dou
   __syncthreads();
 }
 
-template <int L, bool INV, int T, int P, int R, int... Rest>
-__device__ __forceinline__ void run_passes(double2* col, int stride, int t, const double2* tw) {
+template <int L, bool INV, int T, int P, int R, int... Rest, typename C>
+__device__ __forceinline__ void run_passes(C* col, int stride, int t, const C* tw) {
   spass<L, R, P, INV, T>(col, stride, t, tw);
   if constexpr (sizeof...(Rest) > 0) run_passes<L, INV, T, P * R, Rest...>(col, stride, t, tw);
 }
@@ -168,23 +159,23 @@ __device__ __forceinline__ void run_passes(double2* col, int stride, int t, cons
 // base[c*cs + e*es].  CF: consecutive threads take consecutive columns
 // (columns interleaved in LDS) instead of consecutive butterflies of one column
 // (columns contiguous), so that a wavefront reads consecutive LDS addresses.
-template <int L, int R, int P, bool INV, int NT, int C, bool CF>
-__device__ __forceinline__ void mpass(double2* base, int cs, int es, const double2* __restrict__ tw) {
-  constexpr int NB = L / R, ITEMS = C * NB, PER = (ITEMS + NT - 1) / NT;
-  double2 u[PER][R];
+template <int L, int R, int P, bool INV, int NT, int NC, bool CF, typename C>
+__device__ __forceinline__ void mpass(C* base, int cs, int es, const C* __restrict__ tw) {
+  constexpr int NB = L / R, ITEMS = NC * NB, PER = (ITEMS + NT - 1) / NT;
+  C u[PER][R];
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     const int item = threadIdx.x + b * NT;
     if (PER * NT == ITEMS || item < ITEMS) {
-      const int c = CF ? item % C : item / NB, i = CF ? item / C : item % NB;
-      const double2* col = base + c * cs;
+      const int c = CF ? item % NC : item / NB, i = CF ? item / NC : item % NB;
+      const C* col = base + c * cs;
 #pragma unroll
       for (int j = 0; j < R; ++j) u[b][j] = col[(i + j * NB) * es];
       if constexpr (P > 1) {
         const int k = i % P;
 #pragma unroll
         for (int j = 1; j < R; ++j) {
-          double2 w = tw[(k * j) * (L / (P * R))];
+          C w = tw[(k * j) * (L / (P * R))];
           if (INV) w.y = -w.y;
           u[b][j] = cmul(u[b][j], w);
         }
@@ -197,8 +188,8 @@ __device__ __forceinline__ void mpass(double2* base, int cs, int es, const doubl
   for (int b = 0; b < PER; ++b) {
     const int item = threadIdx.x + b * NT;
     if (PER * NT == ITEMS || item < ITEMS) {
-      const int c = CF ? item % C : item / NB, i = CF ? item / C : item % NB;
-      double2* col = base + c * cs;
+      const int c = CF ? item % NC : item / NB, i = CF ? item / NC : item % NB;
+      C* col = base + c * cs;
       const int k = i % P;
       const int o = (i - k) * R + k;
 #pragma unroll
@@ -208,10 +199,10 @@ __device__ __forceinline__ void mpass(double2* base, int cs, int es, const doubl
   __syncthreads();
 }
 
-template <int L, bool INV, int NT, int C, bool CF, int P, int R, int... Rest>
-__device__ __forceinline__ void run_mpasses(double2* base, int cs, int es, const double2* tw) {
-  mpass<L, R, P, INV, NT, C, CF>(base, cs, es, tw);
-  if constexpr (sizeof...(Rest) > 0) run_mpasses<L, INV, NT, C, CF, P * R, Rest...>(base, cs, es, tw);
+template <int L, bool INV, int NT, int NC, bool CF, int P, int R, int... Rest, typename C>
+__device__ __forceinline__ void run_mpasses(C* base, int cs, int es, const C* tw) {
+  mpass<L, R, P, INV, NT, NC, CF>(base, cs, es, tw);
+  if constexpr (sizeof...(Rest) > 0) run_mpasses<L, INV, NT, NC, CF, P * R, Rest...>(base, cs, es, tw);
 }
 
 // Factorisations (first radix largest keeps the thread count T = L / 8).
@@ -221,13 +212,13 @@ struct Rad;
   template <>                                                                            \
   struct Rad<LEN> {                                                                      \
     static constexpr int T = TT;                                                         \
-    template <bool INV>                                                                  \
-    __device__ static __forceinline__ void run(double2* c, int s, int t, const double2* tw) { \
+    template <bool INV, typename C>                                                      \
+    __device__ static __forceinline__ void run(C* c, int s, int t, const C* tw) {       \
       run_passes<LEN, INV, TT, 1, __VA_ARGS__>(c, s, t, tw);                             \
     }                                                                                    \
-    template <bool INV, int NT, int C, bool CF>                                          \
-    __device__ static __forceinline__ void run_multi(double2* b, int cs, int es, const double2* tw) { \
-      run_mpasses<LEN, INV, NT, C, CF, 1, __VA_ARGS__>(b, cs, es, tw);                   \
+    template <bool INV, int NT, int NC, bool CF, typename C>                             \
+    __device__ static __forceinline__ void run_multi(C* b, int cs, int es, const C* tw) { \
+      run_mpasses<LEN, INV, NT, NC, CF, 1, __VA_ARGS__>(b, cs, es, tw);                  \
     }                                                                                    \
   };
 HEDFT_RAD(16, 4, 4, 4)
@@ -242,58 +233,65 @@ HEDFT_RAD(256, 32, 8, 8, 4)
 #undef HEDFT_RAD
 
 // ---------------------------------------------------------------------------
-// x-pass prologue / epilogue
+// x-pass prologue / epilogue (arithmetic in double, storage in R)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double xin_eval(const XIn& in, size_t i) {
+template <typename R>
+__device__ __forceinline__ double xin_eval(const XInT<R>& in, size_t i) {
   switch (in.op) {
     case XIN_CORR: {  // den*dencg*(cp4+cpp4*dencg)   (poten.f90:166-168)
       const double cg = in.a[1][i];
-      return in.a[0][i] * cg * (in.c0 + in.c1 * cg);
+      return (double)in.a[0][i] * cg * (in.c0 + in.c1 * cg);
     }
     case XIN_FGRAD:  // falfs*d_a(den), falfs = 1 - denalf/den0s  (term_alfa.f90:26-34)
-      return (1.0 - in.a[0][i] / in.c0) * in.a[1][i];
+      return (1.0 - (double)in.a[0][i] / in.c0) * (double)in.a[1][i];
     case XIN_DOT3:  // grad(den).int_alf  (term_alfa.f90:61-65)
-      return in.a[0][i] * in.a[3][i] + in.a[1][i] * in.a[4][i] + in.a[2][i] * in.a[5][i];
+      return (double)in.a[0][i] * in.a[3][i] + (double)in.a[1][i] * in.a[4][i] +
+             (double)in.a[2][i] * in.a[5][i];
     default:
       return in.a[0][i];
   }
 }
 
-__device__ __forceinline__ void xout_apply(const XOut& o, size_t i, double v) {
+template <typename R>
+__device__ __forceinline__ void xout_apply(const XOutT<R>& o, size_t i, double v) {
   if (o.op == XOUT_POT4) {
     // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + conv [+ uext] [+ penalty]
     // (He_statics/poten.f90:277-289,315-319; Dynamics_TDDFT/poten.f90:194-218)
     const double cg = o.dencg[i];
-    double p = o.delj4[i] + cg * cg * (o.a0 + o.a1 * cg) + v;
+    double p = (double)o.delj4[i] + cg * cg * (o.a0 + o.a1 * cg) + v;
     if (o.lext) p += o.uext[i];
     if (o.lsolid) {
       const double dn = o.den[i];
       const double th = tanh(o.beta * (dn - o.den_m));
       p += o.csolid * (1.0 + th + o.beta * dn * (1.0 - th * th));
     }
-    o.out[i] = p;
+    o.out[i] = (R)p;
   } else {
-    o.out[i] = v;
+    o.out[i] = (R)v;
   }
 }
 
 constexpr int MAXB = 4;  // arrays per batched launch
+template <typename R>
 struct XInBatch {
-  XIn in[MAXB];
-  double2* out[MAXB];
+  XInT<R> in[MAXB];
+  cplx_t<R>* out[MAXB];
 };
+template <typename R>
 struct XOutBatch {
-  const double2* in[MAXB];
-  XOut out[MAXB];
+  const cplx_t<R>* in[MAXB];
+  XOutT<R> out[MAXB];
 };
+template <typename R>
 struct SpecBatch {
-  double2* p[MAXB];
+  cplx_t<R>* p[MAXB];
 };
+template <typename R>
 struct ZConvArgs {
-  const double2* in[MAXB];
-  double2* out[MAXB * 3];
-  double2* store[MAXB];
-  const double* kern[MAXB * 3];  // kern[i*NOUT + o]
+  const cplx_t<R>* in[MAXB];
+  cplx_t<R>* out[MAXB * 3];
+  cplx_t<R>* store[MAXB];
+  const R* kern[MAXB * 3];  // kern[i*NOUT + o]
   int do_fwd;
 };
 
@@ -303,43 +301,47 @@ struct ZConvArgs {
 // columns inverse FFT then rows C2R.  One workgroup per (plane, array).
 // ---------------------------------------------------------------------------
 // X_k of the half-complex split: ((z + zc) - i W^k (z - zc)) / 2 with zc = conj(Z_{M-k})
-__device__ __forceinline__ double2 split_one(double2 z, double2 zc, double2 w) {
-  const double2 ev = cscale(cadd(z, zc), 0.5);
-  const double2 wo = cmul(w, cscale(csub(z, zc), 0.5));
-  return make_double2(ev.x + wo.y, ev.y - wo.x);
+template <typename C>
+__device__ __forceinline__ C split_one(C z, C zc, C w) {
+  using T = decltype(z.x);
+  const C ev = cscale(cadd(z, zc), (T)0.5);
+  const C wo = cmul(w, cscale(csub(z, zc), (T)0.5));
+  return mkc<C>(ev.x + wo.y, ev.y - wo.x);
 }
 
 // Z'_k of the c2r merge: (x + conj(xm)) + i conj(W^k) (x - conj(xm))
-__device__ __forceinline__ double2 merge_one(double2 x, double2 xm, double2 w) {
-  const double2 xmc = cconj(xm);
-  const double2 sm = cadd(x, xmc);
-  const double2 wd = cmul(cconj(w), csub(x, xmc));
-  return make_double2(sm.x - wd.y, sm.y + wd.x);
+template <typename C>
+__device__ __forceinline__ C merge_one(C x, C xm, C w) {
+  const C xmc = cconj(xm);
+  const C sm = cadd(x, xmc);
+  const C wd = cmul(cconj(w), csub(x, xmc));
+  return mkc<C>(sm.x - wd.y, sm.y + wd.x);
 }
 
-template <int NX, int NY>
+template <int NX, int NY, typename R>
 struct Plane {
   static constexpr int M = NX / 2, NXH = M + 1, PAD = NXH;  // odd row pitch
   static constexpr int ITX = NY * (M / 2), ITY = NXH * (NY / 2);
   static constexpr int NT0 = (ITX > ITY ? ITX : ITY) / 4;
   static constexpr int NT = NT0 >= 1024 ? 1024 : (NT0 <= 64 ? 64 : ((NT0 + 63) / 64) * 64);
-  static constexpr size_t LDS = sizeof(double2) * NY * PAD;
+  static constexpr size_t LDS = sizeof(cplx_t<R>) * NY * PAD;
 };
 
-template <int NX, int NY>
-__global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_waves_per_eu(1, 4)))
-    k_xyfwd(XInBatch a, Dims d, const double2* __restrict__ twx, const double2* __restrict__ twp,
-            const double2* __restrict__ twy) {
-  using PL = Plane<NX, NY>;
+template <int NX, int NY, typename R>
+__global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_waves_per_eu(1, 4)))
+    k_xyfwd(XInBatch<R> a, Dims d, const cplx_t<R>* __restrict__ twx, const cplx_t<R>* __restrict__ twp,
+            const cplx_t<R>* __restrict__ twy) {
+  using C = cplx_t<R>;
+  using PL = Plane<NX, NY, R>;
   constexpr int M = PL::M, NXH = PL::NXH, PAD = PL::PAD, NT = PL::NT;
   extern __shared__ unsigned char smem_raw[];
-  double2* P = reinterpret_cast<double2*>(smem_raw);
-  const XIn& in = a.in[blockIdx.y];
-  double2* out = a.out[blockIdx.y];
+  C* P = reinterpret_cast<C*>(smem_raw);
+  const XInT<R>& in = a.in[blockIdx.y];
+  C* out = a.out[blockIdx.y];
   const size_t row0 = (size_t)blockIdx.x * NY;
   auto fill = [&](auto getv) {
     for (int e0 = threadIdx.x; e0 < NY * M; e0 += NT * 4) {
-      double2 v[4];
+      C v[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int e = min(e0 + u * NT, NY * M - 1);
@@ -356,34 +358,32 @@ __global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_wav
       }
     }
   };
+  // consecutive reals (2j, 2j+1) of a row as one complex
+  auto ld2 = [](const R* p, size_t i) { return *reinterpret_cast<const C*>(p + i); };
   switch (in.op) {  // uniform per workgroup: one branch-free load loop per operation
     case XIN_CORR:
       fill([&](size_t i) {
-        const double2 dn = *reinterpret_cast<const double2*>(in.a[0] + i);
-        const double2 cg = *reinterpret_cast<const double2*>(in.a[1] + i);
-        return make_double2(dn.x * cg.x * (in.c0 + in.c1 * cg.x), dn.y * cg.y * (in.c0 + in.c1 * cg.y));
+        const C dn = ld2(in.a[0], i), cg = ld2(in.a[1], i);
+        return mkc<C>((R)((double)dn.x * cg.x * (in.c0 + in.c1 * cg.x)),
+                      (R)((double)dn.y * cg.y * (in.c0 + in.c1 * cg.y)));
       });
       break;
     case XIN_FGRAD:
       fill([&](size_t i) {
-        const double2 al = *reinterpret_cast<const double2*>(in.a[0] + i);
-        const double2 gr = *reinterpret_cast<const double2*>(in.a[1] + i);
-        return make_double2((1.0 - al.x / in.c0) * gr.x, (1.0 - al.y / in.c0) * gr.y);
+        const C al = ld2(in.a[0], i), gr = ld2(in.a[1], i);
+        return mkc<C>((R)((1.0 - (double)al.x / in.c0) * gr.x), (R)((1.0 - (double)al.y / in.c0) * gr.y));
       });
       break;
     case XIN_DOT3:
       fill([&](size_t i) {
-        const double2 g0 = *reinterpret_cast<const double2*>(in.a[0] + i);
-        const double2 g1 = *reinterpret_cast<const double2*>(in.a[1] + i);
-        const double2 g2 = *reinterpret_cast<const double2*>(in.a[2] + i);
-        const double2 i0 = *reinterpret_cast<const double2*>(in.a[3] + i);
-        const double2 i1 = *reinterpret_cast<const double2*>(in.a[4] + i);
-        const double2 i2 = *reinterpret_cast<const double2*>(in.a[5] + i);
-        return make_double2(g0.x * i0.x + g1.x * i1.x + g2.x * i2.x, g0.y * i0.y + g1.y * i1.y + g2.y * i2.y);
+        const C g0 = ld2(in.a[0], i), g1 = ld2(in.a[1], i), g2 = ld2(in.a[2], i);
+        const C i0 = ld2(in.a[3], i), i1 = ld2(in.a[4], i), i2 = ld2(in.a[5], i);
+        return mkc<C>((R)((double)g0.x * i0.x + (double)g1.x * i1.x + (double)g2.x * i2.x),
+                      (R)((double)g0.y * i0.y + (double)g1.y * i1.y + (double)g2.y * i2.y));
       });
       break;
     default:
-      fill([&](size_t i) { return *reinterpret_cast<const double2*>(in.a[0] + i); });
+      fill([&](size_t i) { return ld2(in.a[0], i); });
   }
   __syncthreads();
   Rad<M>::template run_multi<false, NT, NY, true>(P, PAD, 1, twx);
@@ -393,32 +393,33 @@ __global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_wav
   constexpr int MH = M / 2 + 1;
   for (int e = threadIdx.x; e < NY * MH; e += NT) {
     const int y = e / MH, k = e - y * MH;
-    double2* row = P + y * PAD;
-    const double2 a = row[k], b = row[k == 0 ? 0 : M - k];
-    const double2 xk = split_one(a, cconj(b), twp[k]);
-    const double2 xm = split_one(b, cconj(a), twp[M - k]);
+    C* row = P + y * PAD;
+    const C a0 = row[k], b0 = row[k == 0 ? 0 : M - k];
+    const C xk = split_one(a0, cconj(b0), twp[k]);
+    const C xm = split_one(b0, cconj(a0), twp[M - k]);
     row[k] = xk;
     row[M - k] = xm;
   }
   __syncthreads();
   Rad<NY>::template run_multi<false, NT, NXH, true>(P, 1, PAD, twy);
-  double2* o = out + (size_t)blockIdx.x * NY * NXH;
+  C* o = out + (size_t)blockIdx.x * NY * NXH;
   for (int e = threadIdx.x; e < NY * NXH; e += NT) {
     const int y = e / NXH, k = e - y * NXH;
     o[e] = P[y * PAD + k];
   }
 }
 
-template <int NX, int NY>
-__global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_waves_per_eu(1, 4)))
-    k_xyinv(XOutBatch a, Dims d, const double2* __restrict__ twx, const double2* __restrict__ twp,
-            const double2* __restrict__ twy, double renor) {
-  using PL = Plane<NX, NY>;
+template <int NX, int NY, typename R>
+__global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_waves_per_eu(1, 4)))
+    k_xyinv(XOutBatch<R> a, Dims d, const cplx_t<R>* __restrict__ twx, const cplx_t<R>* __restrict__ twp,
+            const cplx_t<R>* __restrict__ twy, double renor) {
+  using C = cplx_t<R>;
+  using PL = Plane<NX, NY, R>;
   constexpr int M = PL::M, NXH = PL::NXH, PAD = PL::PAD, NT = PL::NT;
   extern __shared__ unsigned char smem_raw[];
-  double2* P = reinterpret_cast<double2*>(smem_raw);
-  const double2* in = a.in[blockIdx.y] + (size_t)blockIdx.x * NY * NXH;
-  const XOut& out = a.out[blockIdx.y];
+  C* P = reinterpret_cast<C*>(smem_raw);
+  const C* in = a.in[blockIdx.y] + (size_t)blockIdx.x * NY * NXH;
+  const XOutT<R>& out = a.out[blockIdx.y];
   static_assert(PAD == NXH, "plane rows are stored contiguously");
   lds_fill<8, NT>(P, NY * NXH, [&](int e) { return in[e]; });
   __syncthreads();
@@ -429,14 +430,14 @@ __global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_wav
   constexpr int MH = M / 2 + 1;
   for (int e = threadIdx.x; e < NY * MH; e += NT) {
     const int y = e / MH, k = e - y * MH;
-    double2* row = P + y * PAD;
-    double2 a = row[k], b = row[M - k];
+    C* row = P + y * PAD;
+    C a0 = row[k], b0 = row[M - k];
     if (k == 0) {
-      a.y = 0.0;
-      b.y = 0.0;
+      a0.y = 0;
+      b0.y = 0;
     }
-    const double2 zk = merge_one(a, b, twp[k]);
-    const double2 zm = merge_one(b, a, twp[M - k]);
+    const C zk = merge_one(a0, b0, twp[k]);
+    const C zm = merge_one(b0, a0, twp[M - k]);
     row[k] = zk;
     if (k != 0) row[M - k] = zm;
   }
@@ -445,36 +446,37 @@ __global__ void __launch_bounds__((Plane<NX, NY>::NT)) __attribute__((amdgpu_wav
   const size_t row0 = (size_t)blockIdx.x * NY;
   for (int e = threadIdx.x; e < NY * M; e += NT) {
     const int y = e / M, j = e - y * M;
-    const double2 v = P[y * PAD + j];
+    const C v = P[y * PAD + j];
     const size_t i = (row0 + y) * NX + 2 * j;
-    xout_apply(out, i, v.x * renor);
-    xout_apply(out, i + 1, v.y * renor);
+    xout_apply(out, i, (double)v.x * renor);
+    xout_apply(out, i + 1, (double)v.y * renor);
   }
 }
 
 // ---------------------------------------------------------------------------
 // x R2C: rows of nx = 2M reals -> M+1 complex
 // ---------------------------------------------------------------------------
-template <int M, int ROWS>
+template <int M, int ROWS, typename R>
 __global__ void __launch_bounds__(Rad<M>::T * ROWS)
-    k_xr2c(XInBatch a, Dims d, const double2* __restrict__ tw, const double2* __restrict__ twp) {
+    k_xr2c(XInBatch<R> a, Dims d, const cplx_t<R>* __restrict__ tw, const cplx_t<R>* __restrict__ twp) {
+  using C = cplx_t<R>;
   constexpr int T = Rad<M>::T;
-  __shared__ double2 lds[ROWS][M + 1];
-  const XIn& in = a.in[blockIdx.y];
-  double2* out = a.out[blockIdx.y];
+  __shared__ C lds[ROWS][M + 1];
+  const XInT<R>& in = a.in[blockIdx.y];
+  C* out = a.out[blockIdx.y];
   const int nrows = d.ny * d.nz;
   const int row0 = blockIdx.x * ROWS;
   lds_fill<4, T * ROWS>(&lds[0][0], ROWS * (M + 1), [&](int e) {
     const int r = e / (M + 1), j = e - r * (M + 1);
     const int row = row0 + r;
-    double2 v = make_double2(0.0, 0.0);
+    C v = mkc<C>((R)0, (R)0);
     if (row < nrows && j < M) {
       const size_t i = (size_t)row * (2 * M) + 2 * j;
       if (in.op == XIN_COPY) {
-        v = *reinterpret_cast<const double2*>(in.a[0] + i);
+        v = *reinterpret_cast<const C*>(in.a[0] + i);
       } else {
-        v.x = xin_eval(in, i);
-        v.y = xin_eval(in, i + 1);
+        v.x = (R)xin_eval(in, i);
+        v.y = (R)xin_eval(in, i + 1);
       }
     }
     return v;
@@ -485,12 +487,9 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
   const int row = row0 + r;
   if (row < nrows) {
     for (int k = t; k <= M; k += T) {
-      const double2 zk = lds[r][k == M ? 0 : k];
-      const double2 zm = cconj(lds[r][k == 0 ? 0 : M - k]);
-      const double2 e = cscale(cadd(zk, zm), 0.5);
-      const double2 o = cscale(csub(zk, zm), 0.5);
-      const double2 wo = cmul(twp[k], o);
-      out[(size_t)row * (M + 1) + k] = make_double2(e.x + wo.y, e.y - wo.x);
+      const C zk = lds[r][k == M ? 0 : k];
+      const C zm = cconj(lds[r][k == 0 ? 0 : M - k]);
+      out[(size_t)row * (M + 1) + k] = split_one(zk, zm, twp[k]);
     }
   }
 }
@@ -498,31 +497,27 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
 // ---------------------------------------------------------------------------
 // x C2R: M+1 complex -> nx = 2M reals (imaginary parts of X_0, X_M ignored)
 // ---------------------------------------------------------------------------
-template <int M, int ROWS>
+template <int M, int ROWS, typename R>
 __global__ void __launch_bounds__(Rad<M>::T * ROWS)
-    k_xc2r(XOutBatch a, Dims d, const double2* __restrict__ tw, const double2* __restrict__ twp,
+    k_xc2r(XOutBatch<R> a, Dims d, const cplx_t<R>* __restrict__ tw, const cplx_t<R>* __restrict__ twp,
            double renor) {
+  using C = cplx_t<R>;
   constexpr int T = Rad<M>::T;
-  __shared__ double2 lds[ROWS][M + 1];
-  const double2* in = a.in[blockIdx.y];
-  const XOut& out = a.out[blockIdx.y];
+  __shared__ C lds[ROWS][M + 1];
+  const C* in = a.in[blockIdx.y];
+  const XOutT<R>& out = a.out[blockIdx.y];
   const int nrows = d.ny * d.nz;
   const int row0 = blockIdx.x * ROWS;
   lds_fill<4, T * ROWS>(&lds[0][0], ROWS * (M + 1), [&](int e) {
     const int r = e / (M + 1), k = e - r * (M + 1);
     const int row = row0 + r;
-    double2 z = make_double2(0.0, 0.0);
+    C z = mkc<C>((R)0, (R)0);
     if (row < nrows && k < M) {
-      const double2* h = in + (size_t)row * (M + 1);
-      double2 xk = h[k];
-      double2 xm = h[M - k];
-      if (k == 0) { xk.y = 0.0; xm.y = 0.0; }
-      const double2 xmc = cconj(xm);
-      const double2 s = cadd(xk, xmc);
-      const double2 df = csub(xk, xmc);
-      const double2 w = cconj(twp[k]);
-      const double2 wd = cmul(w, df);
-      z = make_double2(s.x - wd.y, s.y + wd.x);
+      const C* h = in + (size_t)row * (M + 1);
+      C xk = h[k];
+      C xm = h[M - k];
+      if (k == 0) { xk.y = 0; xm.y = 0; }
+      z = merge_one(xk, xm, twp[k]);
     }
     return z;
   });
@@ -532,10 +527,10 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
   const int row = row0 + r;
   if (row < nrows) {
     for (int j = t; j < M; j += T) {
-      const double2 v = lds[r][j];
+      const C v = lds[r][j];
       const size_t i = (size_t)row * (2 * M) + 2 * j;
-      xout_apply(out, i, v.x * renor);
-      xout_apply(out, i + 1, v.y * renor);
+      xout_apply(out, i, (double)v.x * renor);
+      xout_apply(out, i + 1, (double)v.y * renor);
     }
   }
 }
@@ -565,12 +560,13 @@ __device__ __forceinline__ size_t col_addr(const Dims& d, long long c, size_t& s
   return (size_t)c;
 }
 
-template <int L, int AX, bool INV>
+template <int L, int AX, bool INV, typename R>
 __global__ void __launch_bounds__(ColTile<L>::NT)
-    k_colfft(SpecBatch a, Dims d, long long ncol, const double2* __restrict__ tw) {
+    k_colfft(SpecBatch<R> a, Dims d, long long ncol, const cplx_t<R>* __restrict__ tw) {
+  using C = cplx_t<R>;
   constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
-  __shared__ double2 lds[L * TX];
-  double2* s = a.p[blockIdx.y];
+  __shared__ C lds[L * TX];
+  C* s = a.p[blockIdx.y];
   const long long c0 = (long long)blockIdx.x * TX;
   lds_fill<8, NT>(lds, TX * L, [&](int e) {
     const int xl = e % TX, j = e / TX;
@@ -588,15 +584,18 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
   }
 }
 
-// z convolution: [forward z] -> (x kern_{b,o} -> inverse z -> out_{b,o}) for o < NOUT
-template <int L, int NOUT>
+// z convolution: [forward z] -> (x kern_{b,o} -> inverse z -> out_{b,o}) for o < NOUT.
+// One LDS tile: with several outputs the forward spectrum is kept in registers
+// (L/T values per thread) and re-multiplied for each kernel, so the tile needs
+// no second buffer and more workgroups fit per CU.
+template <int L, int NOUT, typename R>
 __global__ void __launch_bounds__(ColTile<L>::NT)
-    k_zconv(ZConvArgs a, Dims d, long long ncol, const double2* __restrict__ tw) {
-  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
-  __shared__ double2 src[L * TX];
-  __shared__ double2 wk[(NOUT > 1) ? L * TX : 1];
+    k_zconv(ZConvArgs<R> a, Dims d, long long ncol, const cplx_t<R>* __restrict__ tw) {
+  using C = cplx_t<R>;
+  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT, PER = (TX * L + NT - 1) / NT;
+  __shared__ C src[L * TX];
   const int b = blockIdx.y;
-  const double2* in = a.in[b];
+  const C* in = a.in[b];
   const size_t stride = (size_t)d.nxh * d.ny;
   const long long c0 = (long long)blockIdx.x * TX;
   const int xl0 = threadIdx.x % TX, t0 = threadIdx.x / TX;
@@ -612,32 +611,35 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
       if (c0 + xl < ncol) a.store[b][c0 + xl + j * stride] = src[e];
     }
   }
+  C keep[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (e < TX * L) keep[u] = src[e];
+  }
 #pragma unroll 1
   for (int o = 0; o < NOUT; ++o) {
-    double2* buf = (NOUT > 1) ? wk : src;
-    const double* kern = a.kern[b * NOUT + o];
-    for (int e0 = threadIdx.x; e0 < TX * L; e0 += NT * 8) {
-      double kv[8];
+    const R* kern = a.kern[b * NOUT + o];
+    R kv[PER];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = min(e0 + u * NT, TX * L - 1);
-        const int xl = e % TX, j = e / TX;
-        kv[u] = kern[min(c0 + xl, ncol - 1) + j * stride];
-      }
+    for (int u = 0; u < PER; ++u) {
+      const int e = min(threadIdx.x + u * NT, TX * L - 1);
+      const int xl = e % TX, j = e / TX;
+      kv[u] = kern[min(c0 + xl, ncol - 1) + j * stride];
+    }
+    if (o > 0) __syncthreads();  // the previous output's stores have read the tile
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * NT;
-        if (e < TX * L) buf[e] = cscale(src[e], kv[u]);
-      }
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * NT;
+      if (e < TX * L) src[e] = cscale(keep[u], kv[u]);
     }
     __syncthreads();
-    Rad<L>::template run<true>(buf + xl0, TX, t0, tw);
-    double2* out = a.out[b * NOUT + o];
+    Rad<L>::template run<true>(src + xl0, TX, t0, tw);
+    C* out = a.out[b * NOUT + o];
     for (int e = threadIdx.x; e < TX * L; e += NT) {
       const int xl = e % TX, j = e / TX;
-      if (c0 + xl < ncol) out[c0 + xl + j * stride] = buf[e];
+      if (c0 + xl < ncol) out[c0 + xl + j * stride] = src[e];
     }
-    __syncthreads();
   }
 }
 
@@ -683,11 +685,22 @@ static int dispatch_len(int L) {
     }                                                                            \
   }
 
-static inline const double2* twp(const hedft_ctx* c, size_t off) {
-  return reinterpret_cast<const double2*>(c->tw) + off;
+template <typename R>
+static inline const cplx_t<R>* twp(const hedft_ctx* c, size_t off) {
+  if constexpr (sizeof(R) == 8) return reinterpret_cast<const double2*>(c->tw) + off;
+  else return reinterpret_cast<const float2*>(c->twf) + off;
 }
 
 static bool g_plane_attr_set[8];
+
+template <int N, typename R>
+static hipError_t plane_attrs() {
+  hipError_t e = hipFuncSetAttribute((const void*)k_xyfwd<N, N, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)Plane<N, N, R>::LDS);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_xyinv<N, N, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)Plane<N, N, R>::LDS);
+}
 
 int fft_setup(hedft_ctx* c) {
   const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, M = nx / 2;
@@ -705,92 +718,98 @@ int fft_setup(hedft_ctx* c) {
   c->tw_xpost = add(nx, M + 1);
   c->tw_y = add(ny, ny);
   c->tw_z = add(nz, nz);
+  std::vector<float2> hf(h.size());
+  for (size_t i = 0; i < h.size(); ++i) hf[i] = make_float2((float)h[i].x, (float)h[i].y);
   HEDFT_CHECK(hipMalloc(&c->tw, h.size() * sizeof(double2)));
   HEDFT_CHECK(hipMemcpy(c->tw, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HEDFT_CHECK(hipMalloc(&c->twf, hf.size() * sizeof(float2)));
+  HEDFT_CHECK(hipMemcpy(c->twf, hf.data(), hf.size() * sizeof(float2), hipMemcpyHostToDevice));
   // plane kernels use more than 64 KB of dynamic LDS
   const int sizes[5] = {32, 48, 64, 96, 128};
   for (int k = 0; k < 5; ++k) {
     if (g_plane_attr_set[k]) continue;
-    hipError_t e1 = hipSuccess, e2 = hipSuccess;
+    hipError_t e = hipSuccess;
     switch (sizes[k]) {
-#define HEDFT_SETATTR(N)                                                                                    \
-  case N:                                                                                                   \
-    e1 = hipFuncSetAttribute((const void*)k_xyfwd<N, N>, hipFuncAttributeMaxDynamicSharedMemorySize,        \
-                             (int)Plane<N, N>::LDS);                                                       \
-    e2 = hipFuncSetAttribute((const void*)k_xyinv<N, N>, hipFuncAttributeMaxDynamicSharedMemorySize,        \
-                             (int)Plane<N, N>::LDS);                                                       \
+#define HEDFT_SETATTR(N)                                 \
+  case N:                                                \
+    e = plane_attrs<N, double>();                        \
+    if (e == hipSuccess) e = plane_attrs<N, float>();    \
     break;
       HEDFT_SETATTR(32) HEDFT_SETATTR(48) HEDFT_SETATTR(64) HEDFT_SETATTR(96) HEDFT_SETATTR(128)
 #undef HEDFT_SETATTR
     }
-    if (e1 != hipSuccess || e2 != hipSuccess) return HEDFT_EHIP;
+    if (e != hipSuccess) return HEDFT_EHIP;
     g_plane_attr_set[k] = true;
   }
   return HEDFT_OK;
 }
 
-static int launch_xr2c(hedft_ctx* c, const XInBatch& a, int nb) {
+template <typename R>
+static int launch_xr2c(hedft_ctx* c, const XInBatch<R>& a, int nb) {
   const int M = c->d.nx / 2;
   const int nrows = c->d.ny * c->d.nz;
   HEDFT_LEN_SWITCH(M, {
     constexpr int ROWS = xrows<L_>();
     dim3 grid((nrows + ROWS - 1) / ROWS, nb);
-    hipLaunchKernelGGL((k_xr2c<L_, ROWS>), grid, dim3(Rad<L_>::T * ROWS), 0, c->stream, a, c->d,
-                       twp(c, c->tw_x), twp(c, c->tw_xpost));
+    hipLaunchKernelGGL((k_xr2c<L_, ROWS, R>), grid, dim3(Rad<L_>::T * ROWS), 0, c->stream, a, c->d,
+                       twp<R>(c, c->tw_x), twp<R>(c, c->tw_xpost));
   });
   return HEDFT_OK;
 }
 
-static int launch_xc2r(hedft_ctx* c, const XOutBatch& a, int nb) {
+template <typename R>
+static int launch_xc2r(hedft_ctx* c, const XOutBatch<R>& a, int nb) {
   const int M = c->d.nx / 2;
   const int nrows = c->d.ny * c->d.nz;
   HEDFT_LEN_SWITCH(M, {
     constexpr int ROWS = xrows<L_>();
     dim3 grid((nrows + ROWS - 1) / ROWS, nb);
-    hipLaunchKernelGGL((k_xc2r<L_, ROWS>), grid, dim3(Rad<L_>::T * ROWS), 0, c->stream, a, c->d,
-                       twp(c, c->tw_x), twp(c, c->tw_xpost), c->renor);
+    hipLaunchKernelGGL((k_xc2r<L_, ROWS, R>), grid, dim3(Rad<L_>::T * ROWS), 0, c->stream, a, c->d,
+                       twp<R>(c, c->tw_x), twp<R>(c, c->tw_xpost), c->renor);
   });
   return HEDFT_OK;
 }
 
-template <int AX, bool INV>
-static int launch_col(hedft_ctx* c, const SpecBatch& a, int nb) {
+template <int AX, bool INV, typename R>
+static int launch_col(hedft_ctx* c, const SpecBatch<R>& a, int nb) {
   const int L = (AX == 1) ? c->d.ny : c->d.nz;
   const size_t off = (AX == 1) ? c->tw_y : c->tw_z;
   const long long ncol = (long long)c->d.nxh * ((AX == 1) ? c->d.nz : c->d.ny);
   HEDFT_LEN_SWITCH(L, {
     constexpr int TX = ColTile<L_>::TX;
     dim3 grid((unsigned)((ncol + TX - 1) / TX), nb);
-    hipLaunchKernelGGL((k_colfft<L_, AX, INV>), grid, dim3(ColTile<L_>::NT), 0, c->stream, a, c->d, ncol,
-                       twp(c, off));
+    hipLaunchKernelGGL((k_colfft<L_, AX, INV, R>), grid, dim3(ColTile<L_>::NT), 0, c->stream, a, c->d, ncol,
+                       twp<R>(c, off));
   });
   return HEDFT_OK;
 }
 
-int fwd_xy(hedft_ctx* c, int nb, const XIn* in, double2* const* spec) {
+template <typename R>
+int fwd_xy(hedft_ctx* c, int nb, const XInT<R>* in, cplx_t<R>* const* spec) {
   if (nb < 1 || nb > MAXB) return HEDFT_EINVAL;
-  XInBatch a{};
-  SpecBatch s{};
+  XInBatch<R> a{};
+  SpecBatch<R> s{};
   for (int b = 0; b < nb; ++b) {
     a.in[b] = in[b];
     a.out[b] = spec[b];
     s.p[b] = spec[b];
   }
   HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
-    using PL_ = Plane<PX_, PY_>;
+    using PL_ = Plane<PX_, PY_, R>;
     constexpr size_t lds_ = PL_::LDS;
     constexpr int nt_ = PL_::NT;
-    hipLaunchKernelGGL((k_xyfwd<PX_, PY_>), dim3(c->d.nz, nb), dim3(nt_), lds_, c->stream, a, c->d,
-                       twp(c, c->tw_x), twp(c, c->tw_xpost), twp(c, c->tw_y));
+    hipLaunchKernelGGL((k_xyfwd<PX_, PY_, R>), dim3(c->d.nz, nb), dim3(nt_), lds_, c->stream, a, c->d,
+                       twp<R>(c, c->tw_x), twp<R>(c, c->tw_xpost), twp<R>(c, c->tw_y));
   });
-  HEDFT_TRY(launch_xr2c(c, a, nb));
-  return launch_col<1, false>(c, s, nb);
+  HEDFT_TRY(launch_xr2c<R>(c, a, nb));
+  return launch_col<1, false, R>(c, s, nb);
 }
 
-int zconv(hedft_ctx* c, int nin, double2* const* in, int nout, const double* const* kern,
-          double2* const* out, int do_fwd, double2* const* store_fwd) {
+template <typename R>
+int zconv(hedft_ctx* c, int nin, cplx_t<R>* const* in, int nout, const R* const* kern,
+          cplx_t<R>* const* out, int do_fwd, cplx_t<R>* const* store_fwd) {
   if (nin < 1 || nin > MAXB || nout < 1 || nout > 3) return HEDFT_EINVAL;
-  ZConvArgs a{};
+  ZConvArgs<R> a{};
   for (int i = 0; i < nin; ++i) {
     a.in[i] = in[i];
     a.store[i] = store_fwd ? store_fwd[i] : nullptr;
@@ -806,52 +825,62 @@ int zconv(hedft_ctx* c, int nin, double2* const* in, int nout, const double* con
     constexpr int TX = ColTile<L_>::TX;
     dim3 grid((unsigned)((ncol + TX - 1) / TX), nin);
     dim3 blk(ColTile<L_>::NT);
-    const double2* tw = twp(c, c->tw_z);
-    if (nout == 1) hipLaunchKernelGGL((k_zconv<L_, 1>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
-    else if (nout == 2) hipLaunchKernelGGL((k_zconv<L_, 2>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
-    else hipLaunchKernelGGL((k_zconv<L_, 3>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
+    const cplx_t<R>* tw = twp<R>(c, c->tw_z);
+    if (nout == 1) hipLaunchKernelGGL((k_zconv<L_, 1, R>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
+    else if (nout == 2) hipLaunchKernelGGL((k_zconv<L_, 2, R>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
+    else hipLaunchKernelGGL((k_zconv<L_, 3, R>), grid, blk, 0, c->stream, a, c->d, ncol, tw);
   });
   return HEDFT_OK;
 }
 
-int inv_yx(hedft_ctx* c, int nb, double2* const* spec, const XOut* out) {
+template <typename R>
+int inv_yx(hedft_ctx* c, int nb, cplx_t<R>* const* spec, const XOutT<R>* out) {
   if (nb < 1 || nb > MAXB) return HEDFT_EINVAL;
-  SpecBatch s{};
-  XOutBatch a{};
+  SpecBatch<R> s{};
+  XOutBatch<R> a{};
   for (int b = 0; b < nb; ++b) {
     s.p[b] = spec[b];
     a.in[b] = spec[b];
     a.out[b] = out[b];
   }
   HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
-    using PL_ = Plane<PX_, PY_>;
+    using PL_ = Plane<PX_, PY_, R>;
     constexpr size_t lds_ = PL_::LDS;
     constexpr int nt_ = PL_::NT;
-    hipLaunchKernelGGL((k_xyinv<PX_, PY_>), dim3(c->d.nz, nb), dim3(nt_), lds_, c->stream, a, c->d,
-                       twp(c, c->tw_x), twp(c, c->tw_xpost), twp(c, c->tw_y), c->renor);
+    hipLaunchKernelGGL((k_xyinv<PX_, PY_, R>), dim3(c->d.nz, nb), dim3(nt_), lds_, c->stream, a, c->d,
+                       twp<R>(c, c->tw_x), twp<R>(c, c->tw_xpost), twp<R>(c, c->tw_y), c->renor);
   });
-  HEDFT_TRY((launch_col<1, true>(c, s, nb)));
-  return launch_xc2r(c, a, nb);
+  HEDFT_TRY((launch_col<1, true, R>(c, s, nb)));
+  return launch_xc2r<R>(c, a, nb);
 }
+
+template int fwd_xy<double>(hedft_ctx*, int, const XInT<double>*, double2* const*);
+template int fwd_xy<float>(hedft_ctx*, int, const XInT<float>*, float2* const*);
+template int zconv<double>(hedft_ctx*, int, double2* const*, int, const double* const*, double2* const*, int,
+                           double2* const*);
+template int zconv<float>(hedft_ctx*, int, float2* const*, int, const float* const*, float2* const*, int,
+                          float2* const*);
+template int inv_yx<double>(hedft_ctx*, int, double2* const*, const XOutT<double>*);
+template int inv_yx<float>(hedft_ctx*, int, float2* const*, const XOutT<float>*);
 
 int fftfw(hedft_ctx* c, const double* in, double2* out) {
   XIn xi{};
   xi.op = XIN_COPY;
   xi.a[0] = in;
-  HEDFT_TRY(fwd_xy(c, 1, &xi, &out));
-  SpecBatch s{};
+  HEDFT_TRY(fwd_xy<double>(c, 1, &xi, &out));
+  SpecBatch<double> s{};
   s.p[0] = out;
-  return launch_col<2, false>(c, s, 1);
+  return launch_col<2, false, double>(c, s, 1);
 }
 
 int fftbk(hedft_ctx* c, double2* in, double* out) {
-  SpecBatch s{};
+  SpecBatch<double> s{};
   s.p[0] = in;
-  HEDFT_TRY((launch_col<2, true>(c, s, 1)));
+  HEDFT_TRY((launch_col<2, true, double>(c, s, 1)));
   XOut xo{};
   xo.op = XOUT_STORE;
   xo.out = out;
-  return inv_yx(c, 1, &in, &xo);
+  return inv_yx<double>(c, 1, &in, &xo);
 }
 
 }  // namespace hedft
