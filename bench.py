@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--n4", type=float, default=1000.0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=600,
+                    help="PC iterations timed on the mixed-precision path (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=12, help="PC iterations in the CPU sample")
     ap.add_argument("--probe-steps", type=int, default=24,
@@ -144,6 +146,34 @@ def main():
                     "launches": a.probe_steps}
         e = drop.energy()
 
+    # mixed-precision path on the same workload: same RK start, warmup, timed
+    # and probe iteration counts, so its energy per atom is compared with the
+    # FP64 path's after the same number of steps
+    mixed = None
+    if a.mixed_steps > 0:
+        pm = H.default_params(core_otc=1, lsolid=0, precision=H.PREC_MIXED)
+        dm = H.Droplet(n, n, n, xmax, xmax, xmax, pm, mode=H.MODE_TDDFT)
+        I.tddft_setup(dm, n, xmax, a.n4, seed=1, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 3.0))
+        dm.tddft_init()
+        for it in range(1, nrk + 1):
+            dm.tddft_iteration(dt, it, nrk)
+        if a.warmup:
+            dm.run_pc(dt, a.warmup, use_graph)
+        msm = dm.run_pc(dt, a.mixed_steps, use_graph)
+        left = a.steps + a.probe_steps - a.mixed_steps
+        if left > 0:
+            dm.run_pc(dt, left, use_graph)
+        em = dm.energy()
+        dm.close()
+        compared = left >= 0
+        mixed = {"value": round(a.mixed_steps / (msm / 1000.0), 3), "unit": "PC steps/s",
+                 "ms_per_step": round(msm / a.mixed_steps, 4), "steps": a.mixed_steps, "dtype": "f32 field / f64 psi",
+                 "speedup_vs_fp64": round((a.mixed_steps / msm) / (a.steps / ms), 3),
+                 "energy_per_atom_rel_err_vs_fp64": (abs(em["etot4"] - e["etot4"]) / abs(e["etot4"])
+                                                     if compared else None),
+                 "norm_rel_err_vs_fp64": (abs(em["norm"] - e["norm"]) / abs(e["norm"]) if compared else None),
+                 "pc_steps_compared": a.warmup + a.steps + a.probe_steps if compared else None}
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
@@ -170,6 +200,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "energy_per_atom_K": e["etot4"] / a.n4,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 

@@ -2,5 +2,5 @@
 13-point stencils, RK-Gill / Hamming predictor-corrector) behind the C-ABI in
 include/hedft.h.  The HIP library lives in he_dft_2024_amd/_build/libhedft.so;
 the Fortran drop-in module is he_dft_2024_amd/fortran/hedft_gpu.f90."""
-from .hedft import (Droplet, HedftError, MODE_STATICS, MODE_TDDFT, default_params,  # noqa: F401
-                    grid_supported, lib)
+from .hedft import (Droplet, HedftError, MODE_STATICS, MODE_TDDFT, PREC_FP64, PREC_MIXED,  # noqa: F401
+                    default_params, grid_supported, lib)

@@ -190,6 +190,9 @@ struct hedft_ctx {
   hedft_grid g;
   hedft_params p;
   int mode;
+  int mixed;  // HEDFT_PREC_MIXED: the real mean-field fields, spectra and kernel
+              // tables are stored as float (the double* members then point to
+              // float storage); psi and its PC history stay double
   int device;
   hipStream_t stream;  // every launch goes to this stream
   hipStream_t main_stream, side_stream;  // stream is main_stream except inside a side branch

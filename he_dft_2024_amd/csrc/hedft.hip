@@ -137,10 +137,11 @@ struct Lam {
 
 // ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
 // k = timec*((lap)*h2o2m4 - pot4*psi) - i*uimp*psi   (steprk.f90:67, steppc.f90:432,499)
+template <typename R>
 struct HpsiC {
   const double2* lapxy;
   const double2* psi;
-  const double* pot4;
+  const R* pot4;
   Lam lam;
   ImpDev im;
   double h2o2m4;
@@ -148,7 +149,7 @@ struct HpsiC {
     const double2 l0 = lapxy[i];
     const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
     ps = psi[i];
-    const double v = pot4[i];
+    const double v = (double)pot4[i];
     const double2 t = make_double2(lap.x * h2o2m4 - v * ps.x, lap.y * h2o2m4 - v * ps.y);
     const double lr = lam.at(ix, iy, iz);
     // (lr + i) * t
@@ -165,13 +166,14 @@ struct HpsiC {
 #define HEDFT_KB_STEP 1
 #endif
 
+template <typename R>
 struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
-  HpsiC h;
+  HpsiC<R> h;
   double2 *psi, *q, *po0, *po1, *po2, *ho0, *ho1;
-  double* den;
+  R* den;
   double a, b, cc, dt;
   int first;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double*) const {
@@ -189,17 +191,18 @@ struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
     }
     const double2 np = make_double2(ps.x + dt * s1.x, ps.y + dt * s1.y);
     psi[i] = np;
-    den[i] = np.x * np.x + np.y * np.y;
+    den[i] = (R)(np.x * np.x + np.y * np.y);
   }
 };
 
+template <typename R>
 struct OutPred {  // predictor + modifier (steppc.f90:432-442)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
-  HpsiC h;
+  HpsiC<R> h;
   double2 *psi, *pc, *po3, *ho1, *ho2;
-  double* den;
+  R* den;
   double dt;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double*) const {
     double2 ps;
@@ -214,17 +217,18 @@ struct OutPred {  // predictor + modifier (steppc.f90:432-442)
     psi[i] = np;
     pc[i] = s1;
     ho2[i] = k4;
-    den[i] = np.x * np.x + np.y * np.y;
+    den[i] = (R)(np.x * np.x + np.y * np.y);
   }
 };
 
+template <typename R>
 struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
   static constexpr int NR = 1;
   static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
-  HpsiC h;
+  HpsiC<R> h;
   double2 *psi, *pc, *po3, *po2, *ho1, *ho2;
-  double* den;
+  R* den;
   double dt;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double* red) const {
     double2 ps;
@@ -238,7 +242,7 @@ struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
     pc[i] = p;
     const double2 np = make_double2(s5.x + C9 * p.x, s5.y + C9 * p.y);
     psi[i] = np;
-    den[i] = np.x * np.x + np.y * np.y;
+    den[i] = (R)(np.x * np.x + np.y * np.y);
     red[0] += sqrt(p.x * p.x + p.y * p.y);
   }
 };
@@ -287,23 +291,25 @@ struct OutSqXY {
   }
 };
 
-template <typename V>
+template <typename V, typename R>
 struct OutEnergy {
   static constexpr int NR = 8;
   double* partials;
-  const double *acc, *den, *dencg, *delj4, *denalf, *dxden, *dyden, *dzden, *intx, *inty, *intz, *uext;
+  const double* acc;
+  const R *den, *dencg, *delj4, *denalf, *dxden, *dyden, *dzden, *intx, *inty, *intz, *uext;
   ImpDev im;
   double a1, a2, den0s, beta, den_m;
   int otc;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, V v, double* red) const {
     const double d = den[i], cg = dencg[i];
     red[0] += acc[i] + vsq<V>(v);
-    red[1] += delj4[i] * d;
+    red[1] += (double)delj4[i] * d;
     red[2] += d * cg * cg * (a1 + a2 * cg);
     if (otc)
-      red[3] += (1.0 - denalf[i] / den0s) * (dxden[i] * intx[i] + dyden[i] * inty[i] + dzden[i] * intz[i]);
+      red[3] += (1.0 - (double)denalf[i] / den0s) * ((double)dxden[i] * intx[i] + (double)dyden[i] * inty[i] +
+                                                     (double)dzden[i] * intz[i]);
     red[4] += d * (1.0 + tanh(beta * (d - den_m)));
-    red[5] += uext[i] * d;
+    red[5] += (double)uext[i] * d;
     if (im.nimp > 0) red[6] += imp_u(im, ix, iy, iz) * d;
     red[7] += d;
   }
@@ -501,19 +507,28 @@ __global__ void k_eval_timec(Dims d, Lam lam, double2* out) {
     out[i] = make_double2(lam.at(ix, iy, iz), 1.0);
   }
 }
-__global__ void k_eval_falfs(long long n, const double* denalf, double den0s, double* out) {
+template <typename R>
+__global__ void k_eval_falfs(long long n, const R* denalf, double den0s, double* out) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     out[i] = 1.0 - denalf[i] / den0s;
 }
 // TDDFT start: psi = sqrt(den) (real), den = |psi|^2 (DFT4He3d.f90:541-556)
-__global__ void k_tddft_psi0(long long n, double* den, double2* psi) {
+template <typename R>
+__global__ void k_tddft_psi0(long long n, R* den, double2* psi) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const double s = sqrt(den[i]);
+    const double s = sqrt((double)den[i]);
     psi[i] = make_double2(s, 0.0);
-    den[i] = s * s;
+    den[i] = (R)(s * s);
   }
+}
+// precision conversion for uploads / downloads of FP32-stored fields
+template <typename A, typename B>
+__global__ void k_convert(long long n, const A* in, B* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = (B)in[i];
 }
 // statics start: den *= n4/(sum den dxyz), psi = sqrt(den) (BCN4HeDFT.f90:730-734)
 __global__ void k_sum1(long long n, const double* a, double* partials) {
@@ -681,21 +696,33 @@ struct SideBranch {
   ~SideBranch() { c->stream = c->main_stream; }
 };
 
-static XOut xo_store(double* out) {
-  XOut o{};
+// field pointers as stored: the double* members hold float storage on the mixed path
+template <typename R>
+static inline R* fld(double* p) {
+  return reinterpret_cast<R*>(p);
+}
+template <typename R>
+static inline cplx_t<R>* spc(double2* p) {
+  return reinterpret_cast<cplx_t<R>*>(p);
+}
+
+template <typename R>
+static XOutT<R> xo_store(double* out) {
+  XOutT<R> o{};
   o.op = XOUT_STORE;
-  o.out = out;
+  o.out = fld<R>(out);
   return o;
 }
 
-static XOut xo_pot4(hedft_ctx* c) {
-  XOut o{};
+template <typename R>
+static XOutT<R> xo_pot4(hedft_ctx* c) {
+  XOutT<R> o{};
   o.op = XOUT_POT4;
-  o.out = c->pot4;
-  o.delj4 = c->delj4;
-  o.dencg = c->dencg;
-  o.den = c->den;
-  o.uext = c->uext;
+  o.out = fld<R>(c->pot4);
+  o.delj4 = fld<R>(c->delj4);
+  o.dencg = fld<R>(c->dencg);
+  o.den = fld<R>(c->den);
+  o.uext = fld<R>(c->uext);
   o.a0 = c->p.cp4 / 2.0;
   o.a1 = c->p.cpp4 / 3.0;
   o.csolid = c->p.csolid;
@@ -710,34 +737,36 @@ static XOut xo_pot4(hedft_ctx* c) {
 //   corr  = F^-1[wcgk F(den dencg (cp4+cpp4 dencg))]          (poten.f90:166-178)
 //   int_a = F^-1[kalfs F((1-denalf/den0s) d_a den)], a = x,y,z  (term_alfa.f90:26-48)
 // For the OT core only corr is needed and its C2R epilogue assembles pot4.
+template <typename R>
 static int stage_b(hedft_ctx* c) {
-  XIn in[4];
-  in[0] = XIn{};
+  XInT<R> in[4];
+  in[0] = XInT<R>{};
   in[0].op = XIN_CORR;
-  in[0].a[0] = c->den;
-  in[0].a[1] = c->dencg;
+  in[0].a[0] = fld<R>(c->den);
+  in[0].a[1] = fld<R>(c->dencg);
   in[0].c0 = c->p.cp4;
   in[0].c1 = c->p.cpp4;
-  const double* grads[3] = {c->dxden, c->dyden, c->dzden};
+  const R* grads[3] = {fld<R>(c->dxden), fld<R>(c->dyden), fld<R>(c->dzden)};
   for (int a = 0; a < 3; ++a) {
-    in[1 + a] = XIn{};
+    in[1 + a] = XInT<R>{};
     in[1 + a].op = XIN_FGRAD;
-    in[1 + a].a[0] = c->denalf;
+    in[1 + a].a[0] = fld<R>(c->denalf);
     in[1 + a].a[1] = grads[a];
     in[1 + a].c0 = c->p.den0s;
   }
   const int nb = c->p.core_otc ? 4 : 1;
-  double2* sp[4] = {c->spec[0], c->spec[1], c->spec[2], c->spec[3]};
-  HEDFT_TRY(fwd_xy(c, nb, in, sp));
-  const double* kk[4] = {c->wcgk, c->kalfs, c->kalfs, c->kalfs};
-  HEDFT_TRY(zconv(c, nb, sp, 1, kk, sp, 1, nullptr));
-  XOut xo[4] = {c->p.core_otc ? xo_store(c->sto1) : xo_pot4(c), xo_store(c->intx), xo_store(c->inty),
-                xo_store(c->intz)};
-  return inv_yx(c, nb, sp, xo);
+  cplx_t<R>* sp[4] = {spc<R>(c->spec[0]), spc<R>(c->spec[1]), spc<R>(c->spec[2]), spc<R>(c->spec[3])};
+  HEDFT_TRY(fwd_xy<R>(c, nb, in, sp));
+  const R* kk[4] = {fld<R>(c->wcgk), fld<R>(c->kalfs), fld<R>(c->kalfs), fld<R>(c->kalfs)};
+  HEDFT_TRY(zconv<R>(c, nb, sp, 1, kk, sp, 1, nullptr));
+  XOutT<R> xo[4] = {c->p.core_otc ? xo_store<R>(c->sto1) : xo_pot4<R>(c), xo_store<R>(c->intx),
+                    xo_store<R>(c->inty), xo_store<R>(c->intz)};
+  return inv_yx<R>(c, nb, sp, xo);
 }
 
 // Stage C: ualphas' convolution F^-1[kalfs F(grad(den).int)] (term_alfa.f90:61-73),
 // then the divergence term and the pot4 assembly (term_alfa.f90:80-98, poten.f90:194-211)
+template <typename R>
 static int stage_c(hedft_ctx* c) {
   // side branch: x+y divergence (denalf, int_x, int_y -> work1); main branch:
   // the convolution (grad den, int -> spec[0] -> ualph_conv)
@@ -746,78 +775,84 @@ static int stage_c(hedft_ctx* c) {
     SideBranch sb(c);
     HEDFT_TRY(alpha_div_xy(c));
   }
-  XIn dot{};
+  XInT<R> dot{};
   dot.op = XIN_DOT3;
-  dot.a[0] = c->dxden;
-  dot.a[1] = c->dyden;
-  dot.a[2] = c->dzden;
-  dot.a[3] = c->intx;
-  dot.a[4] = c->inty;
-  dot.a[5] = c->intz;
-  double2* s0 = c->spec[0];
-  HEDFT_TRY(fwd_xy(c, 1, &dot, &s0));
-  const double* ka = c->kalfs;
-  HEDFT_TRY(zconv(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
-  XOut xu = xo_store(c->ualph_conv);
-  HEDFT_TRY(inv_yx(c, 1, &s0, &xu));
+  dot.a[0] = fld<R>(c->dxden);
+  dot.a[1] = fld<R>(c->dyden);
+  dot.a[2] = fld<R>(c->dzden);
+  dot.a[3] = fld<R>(c->intx);
+  dot.a[4] = fld<R>(c->inty);
+  dot.a[5] = fld<R>(c->intz);
+  cplx_t<R>* s0 = spc<R>(c->spec[0]);
+  HEDFT_TRY(fwd_xy<R>(c, 1, &dot, &s0));
+  const R* ka = fld<R>(c->kalfs);
+  HEDFT_TRY(zconv<R>(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
+  XOutT<R> xu = xo_store<R>(c->ualph_conv);
+  HEDFT_TRY(inv_yx<R>(c, 1, &s0, &xu));
   HEDFT_TRY(join(c));
   return alpha_div_z_pot4(c);
 }
 
 // term_alfa() on its own (B restricted to int_a, then C)
+template <typename R>
 static int alpha_term(hedft_ctx* c) {
-  XIn fg[3];
-  const double* grads[3] = {c->dxden, c->dyden, c->dzden};
+  XInT<R> fg[3];
+  const R* grads[3] = {fld<R>(c->dxden), fld<R>(c->dyden), fld<R>(c->dzden)};
   for (int a = 0; a < 3; ++a) {
-    fg[a] = XIn{};
+    fg[a] = XInT<R>{};
     fg[a].op = XIN_FGRAD;
-    fg[a].a[0] = c->denalf;
+    fg[a].a[0] = fld<R>(c->denalf);
     fg[a].a[1] = grads[a];
     fg[a].c0 = c->p.den0s;
   }
-  double2* s3[3] = {c->spec[1], c->spec[2], c->spec[3]};
-  HEDFT_TRY(fwd_xy(c, 3, fg, s3));
-  const double* ka[3] = {c->kalfs, c->kalfs, c->kalfs};
-  HEDFT_TRY(zconv(c, 3, s3, 1, ka, s3, 1, nullptr));
-  XOut xo3[3] = {xo_store(c->intx), xo_store(c->inty), xo_store(c->intz)};
-  HEDFT_TRY(inv_yx(c, 3, s3, xo3));
-  return stage_c(c);
+  cplx_t<R>* s3[3] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), spc<R>(c->spec[3])};
+  HEDFT_TRY(fwd_xy<R>(c, 3, fg, s3));
+  const R* ka[3] = {fld<R>(c->kalfs), fld<R>(c->kalfs), fld<R>(c->kalfs)};
+  HEDFT_TRY(zconv<R>(c, 3, s3, 1, ka, s3, 1, nullptr));
+  XOutT<R> xo3[3] = {xo_store<R>(c->intx), xo_store<R>(c->inty), xo_store<R>(c->intz)};
+  HEDFT_TRY(inv_yx<R>(c, 3, s3, xo3));
+  return stage_c<R>(c);
 }
 
 // Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden).
 // grad(den) and the impurity force run on the side branch while the main
 // branch convolves den; with_lap also computes the x+y Laplacian of psi there
 // for the step that follows (psi is not changed by the mean field).
-static int poten_tddft(hedft_ctx* c, int want_force, int with_lap = 0) {
+template <typename R>
+static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   HEDFT_TRY(fork(c));
   {
     SideBranch sb(c);
     HEDFT_TRY(derden_force(c, want_force, want_force));
     if (with_lap) HEDFT_TRY(lap_xy(c, c->psi, 1));
   }
-  XIn xi{};
+  XInT<R> xi{};
   xi.op = XIN_COPY;
-  xi.a[0] = c->den;
-  double2* s0 = c->spec[0];
-  HEDFT_TRY(fwd_xy(c, 1, &xi, &s0));
+  xi.a[0] = fld<R>(c->den);
+  cplx_t<R>* s0 = spc<R>(c->spec[0]);
+  HEDFT_TRY(fwd_xy<R>(c, 1, &xi, &s0));
   if (c->p.core_otc) {
-    const double* kk[3] = {c->wcgk, c->fvlj4, c->kalfs};
-    double2* so[3] = {c->spec[1], c->spec[2], c->spec[3]};
-    HEDFT_TRY(zconv(c, 1, &s0, 3, kk, so, 1, nullptr));
-    XOut xo[3] = {xo_store(c->dencg), xo_store(c->delj4), xo_store(c->denalf)};
-    HEDFT_TRY(inv_yx(c, 3, so, xo));
+    const R* kk[3] = {fld<R>(c->wcgk), fld<R>(c->fvlj4), fld<R>(c->kalfs)};
+    cplx_t<R>* so[3] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), spc<R>(c->spec[3])};
+    HEDFT_TRY(zconv<R>(c, 1, &s0, 3, kk, so, 1, nullptr));
+    XOutT<R> xo[3] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4), xo_store<R>(c->denalf)};
+    HEDFT_TRY(inv_yx<R>(c, 3, so, xo));
   } else {
-    const double* kk[2] = {c->wcgk, c->fvlj4};
-    double2* so[2] = {c->spec[1], c->spec[2]};
-    HEDFT_TRY(zconv(c, 1, &s0, 2, kk, so, 1, nullptr));
-    XOut xo[2] = {xo_store(c->dencg), xo_store(c->delj4)};
-    HEDFT_TRY(inv_yx(c, 2, so, xo));
+    const R* kk[2] = {fld<R>(c->wcgk), fld<R>(c->fvlj4)};
+    cplx_t<R>* so[2] = {spc<R>(c->spec[1]), spc<R>(c->spec[2])};
+    HEDFT_TRY(zconv<R>(c, 1, &s0, 2, kk, so, 1, nullptr));
+    XOutT<R> xo[2] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4)};
+    HEDFT_TRY(inv_yx<R>(c, 2, so, xo));
   }
   HEDFT_TRY(join(c));
   if (with_lap) c->lap_fresh = 1;
-  HEDFT_TRY(stage_b(c));
-  if (c->p.core_otc) HEDFT_TRY(stage_c(c));
+  HEDFT_TRY(stage_b<R>(c));
+  if (c->p.core_otc) HEDFT_TRY(stage_c<R>(c));
   return HEDFT_OK;
+}
+
+static int poten_tddft(hedft_ctx* c, int want_force, int with_lap = 0) {
+  return c->mixed ? poten_tddft_t<float>(c, want_force, with_lap) : poten_tddft_t<double>(c, want_force, with_lap);
 }
 
 // He_statics/poten.f90:5-221: fden, dencg, denalf and grad(den) as left by the caller
@@ -826,10 +861,10 @@ static int poten_statics(hedft_ctx* c) {
   double2* s2 = c->spec[2];
   double2* fd = c->fden;
   HEDFT_TRY(zconv(c, 1, &fd, 1, &kf, &s2, 0, nullptr));
-  XOut xo = xo_store(c->delj4);
+  XOut xo = xo_store<double>(c->delj4);
   HEDFT_TRY(inv_yx(c, 1, &s2, &xo));
-  HEDFT_TRY(stage_b(c));
-  if (c->p.core_otc) HEDFT_TRY(stage_c(c));
+  HEDFT_TRY(stage_b<double>(c));
+  if (c->p.core_otc) HEDFT_TRY(stage_c<double>(c));
   return HEDFT_OK;
 }
 
@@ -850,13 +885,13 @@ static int statics_refresh(hedft_ctx* c) {
     const double* kk[2] = {c->wcgk, c->kalfs};
     double2* so[2] = {c->spec[1], c->spec[3]};
     HEDFT_TRY(zconv(c, 1, &s0, 2, kk, so, 1, &fd));
-    XOut xo[2] = {xo_store(c->dencg), xo_store(c->denalf)};
+    XOut xo[2] = {xo_store<double>(c->dencg), xo_store<double>(c->denalf)};
     HEDFT_TRY(inv_yx(c, 2, so, xo));
   } else {
     const double* kw = c->wcgk;
     double2* s1 = c->spec[1];
     HEDFT_TRY(zconv(c, 1, &s0, 1, &kw, &s1, 1, &fd));
-    XOut xo = xo_store(c->dencg);
+    XOut xo = xo_store<double>(c->dencg);
     HEDFT_TRY(inv_yx(c, 1, &s1, &xo));
   }
   return join(c);
@@ -877,11 +912,12 @@ static Lam make_lam(hedft_ctx* c) {
   return l;
 }
 
-static HpsiC make_hpsic(hedft_ctx* c) {
-  HpsiC h;
+template <typename R>
+static HpsiC<R> make_hpsic(hedft_ctx* c) {
+  HpsiC<R> h;
   h.lapxy = (const double2*)c->lap;
   h.psi = (const double2*)c->psi;
-  h.pot4 = c->pot4;
+  h.pot4 = fld<R>(c->pot4);
   h.lam = make_lam(c);
   h.im = make_impdev(c);
   h.h2o2m4 = c->p.h2o2m4;
@@ -899,12 +935,13 @@ static void kt_mark(hedft_ctx* c, int slot, int end) {
 }
 
 // STEPRK (Dynamics_TDDFT/steprk.f90:1-144)
-static int steprk_impl(hedft_ctx* c, double dt) {
+template <typename R>
+static int steprk_t(hedft_ctx* c, double dt) {
   for (int jrun = 0; jrun < 4; ++jrun) {
     if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
-    OutRK o;
+    OutRK<R> o;
     o.partials = nullptr;
-    o.h = make_hpsic(c);
+    o.h = make_hpsic<R>(c);
     o.psi = cz(c->psi);
     o.q = cz(c->q);
     o.po0 = cz(c->psiold[0]);
@@ -912,7 +949,7 @@ static int steprk_impl(hedft_ctx* c, double dt) {
     o.po2 = cz(c->psiold[2]);
     o.ho0 = cz(c->hpsiold[0]);
     o.ho1 = cz(c->hpsiold[1]);
-    o.den = c->den;
+    o.den = fld<R>(c->den);
     o.a = RK_A[jrun];
     o.b = RK_B[jrun];
     o.cc = RK_C[jrun];
@@ -932,18 +969,19 @@ static int steprk_impl(hedft_ctx* c, double dt) {
 }
 
 // STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC]
-static int steppc_impl(hedft_ctx* c, double dt) {
+template <typename R>
+static int steppc_t(hedft_ctx* c, double dt) {
   if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
   {
-    OutPred o;
+    OutPred<R> o;
     o.partials = nullptr;
-    o.h = make_hpsic(c);
+    o.h = make_hpsic<R>(c);
     o.psi = cz(c->psi);
     o.pc = cz(c->pc);
     o.po3 = cz(c->psiold[c->ioldp[2]]);
     o.ho1 = cz(c->hpsiold[c->ioldh[0]]);
     o.ho2 = cz(c->hpsiold[c->ioldh[1]]);
-    o.den = c->den;
+    o.den = fld<R>(c->den);
     o.dt = dt;
     kt_mark(c, 0, 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
@@ -959,16 +997,16 @@ static int steppc_impl(hedft_ctx* c, double dt) {
   HEDFT_TRY(poten_tddft(c, 0, 1));
   if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
   {
-    OutCorr o;
+    OutCorr<R> o;
     o.partials = gen_partials(c);
-    o.h = make_hpsic(c);
+    o.h = make_hpsic<R>(c);
     o.psi = cz(c->psi);
     o.pc = cz(c->pc);
     o.po3 = cz(c->psiold[c->ioldp[2]]);
     o.po2 = cz(c->psiold[c->ioldp[1]]);
     o.ho1 = cz(c->hpsiold[c->ioldh[0]]);
     o.ho2 = cz(c->hpsiold[c->ioldh[1]]);
-    o.den = c->den;
+    o.den = fld<R>(c->den);
     o.dt = dt;
     kt_mark(c, 1, 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
@@ -1000,6 +1038,9 @@ static int steppc_impl(hedft_ctx* c, double dt) {
 // driver's poten + forceimp after a step; also prepares the x+y Laplacian of the
 // new psi for the next step
 static int tddft_after_step(hedft_ctx* c) { return poten_tddft(c, 1, 1); }
+
+static int steprk_impl(hedft_ctx* c, double dt) { return c->mixed ? steprk_t<float>(c, dt) : steprk_t<double>(c, dt); }
+static int steppc_impl(hedft_ctx* c, double dt) { return c->mixed ? steppc_t<float>(c, dt) : steppc_t<double>(c, dt); }
 
 // ---- statics -------------------------------------------------------------------------------------
 static int hpsi_statics(hedft_ctx* c, double n4) {
@@ -1079,25 +1120,25 @@ static int steppcr_impl(hedft_ctx* c, double dt, double n4) {
 }
 
 // ---- energy -----------------------------------------------------------------------------------
-template <typename V>
+template <typename V, typename R>
 static int energy_sums(hedft_ctx* c, double* s) {
   const V* psi = (const V*)c->psi;
   launch_pencil_xy<V, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<V>{psi},
-                            OutSqXY<V>{nullptr, c->work1});
-  OutEnergy<V> o;
+                            OutSqXY<V>{nullptr, (double*)c->workc1});
+  OutEnergy<V, R> o;
   o.partials = gen_partials(c);
-  o.acc = c->work1;
-  o.den = c->den;
-  o.dencg = c->dencg;
-  o.delj4 = c->delj4;
-  o.denalf = c->denalf;
-  o.dxden = c->dxden;
-  o.dyden = c->dyden;
-  o.dzden = c->dzden;
-  o.intx = c->intx;
-  o.inty = c->inty;
-  o.intz = c->intz;
-  o.uext = c->uext;
+  o.acc = (const double*)c->workc1;
+  o.den = fld<R>(c->den);
+  o.dencg = fld<R>(c->dencg);
+  o.delj4 = fld<R>(c->delj4);
+  o.denalf = fld<R>(c->denalf);
+  o.dxden = fld<R>(c->dxden);
+  o.dyden = fld<R>(c->dyden);
+  o.dzden = fld<R>(c->dzden);
+  o.intx = fld<R>(c->intx);
+  o.inty = fld<R>(c->inty);
+  o.intz = fld<R>(c->intz);
+  o.uext = fld<R>(c->uext);
   o.im = make_impdev(c);
   o.a1 = 0.5 * c->p.cp4;
   o.a2 = c->p.cpp4 / 3.0;
@@ -1183,7 +1224,9 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   if (p->npd != 13) return HEDFT_EINVAL;
   const int icon = p->icon;
   if (!(icon == 0 || icon == 8 || icon == 13 || icon == 18 || icon == 20 || icon == 24)) return HEDFT_EINVAL;
-  if (p->precision != HEDFT_PREC_FP64) return HEDFT_EINVAL;
+  if (p->precision != HEDFT_PREC_FP64 && p->precision != HEDFT_PREC_MIXED) return HEDFT_EINVAL;
+  // the mixed path covers real-time dynamics; imaginary-time relaxation is FP64
+  if (p->precision == HEDFT_PREC_MIXED && mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return HEDFT_EHIP;
   hedft_ctx* c = new hedft_ctx();
   std::memset((void*)c, 0, sizeof(hedft_ctx));
@@ -1191,6 +1234,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   c->p = *p;
   if (c->p.lsolid) c->p.core_otc = 0;  // If(Lsolid) core4='OT ' (BCN4HeDFT.f90:654, DFT4He3d.f90:456)
   c->mode = mode;
+  c->mixed = (p->precision == HEDFT_PREC_MIXED);
   c->device = device;
   // two streams at the default priority: the main one and a side one for
   // independent branches of the mean field (joined by events, also under capture)
@@ -1220,7 +1264,9 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   c->hz = 2.0 * std::fabs(g->zmax) / d.nz;
   c->dxyz = c->hx * c->hy * c->hz;
   c->renor = 1.0 / ((double)d.nx * d.ny * d.nz);
-  const size_t rN = sizeof(double) * d.n, cN = 2 * rN, hN = sizeof(double2) * d.nh, kN = sizeof(double) * d.nh;
+  // mean-field storage: double, or float on the mixed path; psi side always double
+  const size_t fw = c->mixed ? sizeof(float) : sizeof(double);
+  const size_t rN = fw * d.n, cN = 2 * sizeof(double) * d.n, hN = 2 * fw * d.nh, kN = fw * d.nh;
   const size_t pN = (mode == HEDFT_MODE_TDDFT) ? cN : rN;
   int rc = HEDFT_OK;
   double** reals[] = {&c->den, &c->pot4, &c->dencg, &c->delj4, &c->denalf, &c->intx, &c->inty, &c->intz,
@@ -1270,9 +1316,19 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   for (int a = 0, o = 0; a < 3; o += nn[a], ++a)
     for (int i = 0; i < nn[a]; ++i)
       lam[o + i] = std::tanh((std::fabs(-mx[a] + hh[a] * i) - c->p.tmean[a]) / c->p.tsurf[a]);
-  if (hipMemcpyAsync(c->fvlj4, fv.data(), kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipMemcpyAsync(c->wcgk, wc.data(), kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipMemcpyAsync(c->kalfs, ka.data(), kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+  std::vector<float> fvf, wcf, kaf;
+  const void *pfv = fv.data(), *pwc = wc.data(), *pka = ka.data();
+  if (c->mixed) {  // tables computed in long double / double, stored rounded to FP32
+    fvf.assign(fv.begin(), fv.end());
+    wcf.assign(wc.begin(), wc.end());
+    kaf.assign(ka.begin(), ka.end());
+    pfv = fvf.data();
+    pwc = wcf.data();
+    pka = kaf.data();
+  }
+  if (hipMemcpyAsync(c->fvlj4, pfv, kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->wcgk, pwc, kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->kalfs, pka, kN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(c->lamtab, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
           hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1296,7 +1352,7 @@ int hedft_destroy(hedft_ctx* c) {
                   c->dxden, c->dyden, c->dzden, c->uext, c->sto1, c->ualph_conv, c->work1, c->work2,
                   c->timec_r, c->hpsi_r, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
                   c->hpsiold[1], c->pc, c->q, c->lap, c->workc1, c->fden, c->spec[0], c->spec[1], c->spec[2],
-                  c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->lamtab, c->partials, c->scal,
+                  c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
                   c->imp, c->lstop, c->potion};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -1312,7 +1368,10 @@ int hedft_destroy(hedft_ctx* c) {
 }
 
 // device pointer, byte size and whether the field is computed on download
-static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* computed) {
+// bytes: the host-side (reference layout, FP64) size.  On the mixed path every
+// field except psi, its PC history and the computed ones is stored as float on
+// the device; *low reports that (uploads and downloads then convert).
+static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* computed, int* low = nullptr) {
   const size_t rN = sizeof(double) * c->d.n, cN = 2 * rN, hN = sizeof(double2) * c->d.nh,
                kN = sizeof(double) * c->d.nh;
   const bool td = (c->mode == HEDFT_MODE_TDDFT);
@@ -1355,6 +1414,12 @@ static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* comput
     case HEDFT_WORKK1: *ptr = c->workk1; *bytes = hN; break;
     default: return HEDFT_EINVAL;
   }
+  if (*computed && f != HEDFT_TIMEC) *ptr = c->workc1;  // evaluated into double scratch
+  if (low) {
+    const bool psi_side = (f == HEDFT_PSI || (f >= HEDFT_PSIOLD1 && f <= HEDFT_Q) || f == HEDFT_WORKC1 ||
+                           (f == HEDFT_TIMEC && td));
+    *low = c->mixed && !*computed && !psi_side;
+  }
   return HEDFT_OK;
 }
 
@@ -1371,10 +1436,17 @@ int hedft_upload(hedft_ctx* c, int32_t field, const void* host, size_t bytes) {
   void* p;
   size_t b;
   int comp;
-  HEDFT_TRY(field_ptr(c, field, &p, &b, &comp));
+  int low = 0;
+  HEDFT_TRY(field_ptr(c, field, &p, &b, &comp, &low));
   if (comp || bytes != b) return HEDFT_EINVAL;
   hipSetDevice(c->device);
-  HEDFT_CHECK(hipMemcpyAsync(p, host, b, hipMemcpyHostToDevice, c->stream));
+  if (low) {  // FP64 host data -> double scratch -> float field
+    HEDFT_CHECK(hipMemcpyAsync(c->workc1, host, b, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, c->stream, (long long)(b / 8),
+                       (const double*)c->workc1, (float*)p);
+  } else {
+    HEDFT_CHECK(hipMemcpyAsync(p, host, b, hipMemcpyHostToDevice, c->stream));
+  }
   HEDFT_CHECK(hipStreamSynchronize(c->stream));
   c->graph_valid = 0;
   c->lap_fresh = 0;
@@ -1386,15 +1458,25 @@ int hedft_download(hedft_ctx* c, int32_t field, void* host, size_t bytes) {
   void* p;
   size_t b;
   int comp;
-  HEDFT_TRY(field_ptr(c, field, &p, &b, &comp));
+  int low = 0;
+  HEDFT_TRY(field_ptr(c, field, &p, &b, &comp, &low));
   if (bytes != b) return HEDFT_EINVAL;
   hipSetDevice(c->device);
+  const dim3 gr(1024), bl(256);
   if (comp) {
-    const dim3 gr(1024), bl(256);
-    if (field == HEDFT_UIMP) hipLaunchKernelGGL(k_eval_uimp, gr, bl, 0, c->stream, c->d, make_impdev(c), c->work2);
+    double* w = (double*)c->workc1;
+    if (field == HEDFT_UIMP) hipLaunchKernelGGL(k_eval_uimp, gr, bl, 0, c->stream, c->d, make_impdev(c), w);
+    else if (field == HEDFT_FALFS && c->mixed)
+      hipLaunchKernelGGL(k_eval_falfs<float>, gr, bl, 0, c->stream, c->d.n, (const float*)c->denalf,
+                         c->p.den0s, w);
     else if (field == HEDFT_FALFS)
-      hipLaunchKernelGGL(k_eval_falfs, gr, bl, 0, c->stream, c->d.n, c->denalf, c->p.den0s, c->work2);
+      hipLaunchKernelGGL(k_eval_falfs<double>, gr, bl, 0, c->stream, c->d.n, c->denalf, c->p.den0s, w);
     else hipLaunchKernelGGL(k_eval_timec, gr, bl, 0, c->stream, c->d, make_lam(c), (double2*)c->workc1);
+  }
+  if (low) {  // float field -> double scratch -> FP64 host data
+    hipLaunchKernelGGL((k_convert<float, double>), gr, bl, 0, c->stream, (long long)(b / 8), (const float*)p,
+                       (double*)c->workc1);
+    p = c->workc1;
   }
   HEDFT_CHECK(hipMemcpyAsync(host, p, b, hipMemcpyDeviceToHost, c->stream));
   HEDFT_CHECK(hipStreamSynchronize(c->stream));
@@ -1418,7 +1500,7 @@ static int done(hedft_ctx* c) {
 }
 
 int hedft_derivnD(hedft_ctx* c, int32_t id, int32_t iv, int32_t in, int32_t out) {
-  if (!c || id < 1 || id > 2 || iv < 1 || iv > 3) return HEDFT_EINVAL;
+  if (!c || id < 1 || id > 2 || iv < 1 || iv > 3 || c->mixed) return HEDFT_EINVAL;
   void *pi, *po;
   size_t bi, bo;
   int ci, co;
@@ -1433,6 +1515,7 @@ int hedft_derivnD(hedft_ctx* c, int32_t id, int32_t iv, int32_t in, int32_t out)
 }
 
 int hedft_fftfw(hedft_ctx* c, int32_t in, int32_t out) {
+  if (!c || c->mixed) return HEDFT_EINVAL;  // FP64 parity hook
   void *pi, *po;
   size_t bi, bo;
   int ci, co;
@@ -1444,6 +1527,7 @@ int hedft_fftfw(hedft_ctx* c, int32_t in, int32_t out) {
 }
 
 int hedft_fftbk(hedft_ctx* c, int32_t in, int32_t out) {
+  if (!c || c->mixed) return HEDFT_EINVAL;  // FP64 parity hook
   void *pi, *po;
   size_t bi, bo;
   int ci, co;
@@ -1467,19 +1551,24 @@ int hedft_poten(hedft_ctx* c) {
 
 int hedft_term_alfa(hedft_ctx* c) {
   if (!c->p.core_otc) return HEDFT_OK;
-  // pot4 is re-assembled by alpha_div_pot4 from the stored delj4/dencg/sto1; keep it
-  // untouched for a standalone call by saving and restoring it
-  HEDFT_CHECK(hipMemcpyAsync(c->work2, c->pot4, sizeof(double) * c->d.n, hipMemcpyDeviceToDevice, c->stream));
-  HEDFT_TRY(alpha_term(c));
-  HEDFT_CHECK(hipMemcpyAsync(c->pot4, c->work2, sizeof(double) * c->d.n, hipMemcpyDeviceToDevice, c->stream));
+  // pot4 is re-assembled by the divergence pass from the stored delj4/dencg/sto1;
+  // keep it untouched for a standalone call by saving and restoring it
+  const size_t b = (c->mixed ? sizeof(float) : sizeof(double)) * c->d.n;
+  HEDFT_CHECK(hipMemcpyAsync(c->work2, c->pot4, b, hipMemcpyDeviceToDevice, c->stream));
+  HEDFT_TRY(c->mixed ? alpha_term<float>(c) : alpha_term<double>(c));
+  HEDFT_CHECK(hipMemcpyAsync(c->pot4, c->work2, b, hipMemcpyDeviceToDevice, c->stream));
   return done(c);
 }
 
 int hedft_energy(hedft_ctx* c, hedft_energies* e) {
   if (!c || !e) return HEDFT_EINVAL;
   double s[8];
-  if (c->mode == HEDFT_MODE_TDDFT) HEDFT_TRY(energy_sums<double2>(c, s));
-  else HEDFT_TRY(energy_sums<double>(c, s));
+  if (c->mode == HEDFT_MODE_TDDFT) {
+    if (c->mixed) HEDFT_TRY((energy_sums<double2, float>(c, s)));
+    else HEDFT_TRY((energy_sums<double2, double>(c, s)));
+  } else {
+    HEDFT_TRY((energy_sums<double, double>(c, s)));
+  }
   const hedft_params& p = c->p;
   const double dv = c->dxyz;
   std::memset(e, 0, sizeof *e);
@@ -1625,8 +1714,10 @@ int hedft_potenimp(hedft_ctx* c) {
 }
 
 // standalone forceimp from the stored gradient (potenimp.f90:49-52)
-__global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const double* gx, const double* gy,
-                                                  const double* gz, double* partials) {
+extern "C++" {
+template <typename R>
+__global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const R* gx, const R* gy, const R* gz,
+                                                  double* partials) {
   double red[3 * HEDFT_MAX_IMP];
   for (int r = 0; r < 3 * HEDFT_MAX_IMP; ++r) red[r] = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
@@ -1636,19 +1727,24 @@ __global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const doubl
     imp_xyz(im, ix, iy, iz, xx, yy, zz);
     for (int k = 0; k < im.nimp; ++k) {
       const double u = imp_u_k(im, k, xx, yy, zz, false);
-      red[3 * k] += gx[i] * u;
-      red[3 * k + 1] += gy[i] * u;
-      red[3 * k + 2] += gz[i] * u;
+      red[3 * k] += (double)gx[i] * u;
+      red[3 * k + 1] += (double)gy[i] * u;
+      red[3 * k + 2] += (double)gz[i] * u;
     }
   }
   block_reduce_store<3 * HEDFT_MAX_IMP>(red, partials);
 }
+}  // extern "C++"
 
 int hedft_forceimp(hedft_ctx* c) {
   if (!c) return HEDFT_EINVAL;
   if (c->nimp == 0) return HEDFT_OK;
-  hipLaunchKernelGGL(k_force_ew, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c), c->dxden, c->dyden,
-                     c->dzden, force_partials(c));
+  if (c->mixed)
+    hipLaunchKernelGGL(k_force_ew<float>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
+                       fld<float>(c->dxden), fld<float>(c->dyden), fld<float>(c->dzden), force_partials(c));
+  else
+    hipLaunchKernelGGL(k_force_ew<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
+                       c->dxden, c->dyden, c->dzden, force_partials(c));
   HEDFT_TRY(force_finalize(c, EW_BLOCKS, 0));
   return done(c);
 }
@@ -1688,7 +1784,12 @@ int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double 
 int hedft_tddft_init(hedft_ctx* c) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
   c->lap_fresh = 0;
-  hipLaunchKernelGGL(k_tddft_psi0, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den, (double2*)c->psi);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_tddft_psi0<float>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, fld<float>(c->den),
+                       (double2*)c->psi);
+  else
+    hipLaunchKernelGGL(k_tddft_psi0<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den,
+                       (double2*)c->psi);
   // qr = qv = rimpold = vimpold = aimpold = 0 (DFT4He3d.f90:625-629)
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_QR, 0, sizeof(double) * 24, c->stream));
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_ROLD, 0, sizeof(double) * (IMP_MASS - IMP_ROLD), c->stream));

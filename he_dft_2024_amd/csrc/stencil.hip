@@ -132,16 +132,18 @@ int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int i
 
 // grad(den): x and y in one fused pass, z in a second pass that also
 // accumulates the He part of the impurity force F_k = -sum(grad(den) U_k) dxyz
-// (potenimp.f90:49-52), evaluating U_k once per point.
+// (potenimp.f90:49-52), evaluating U_k once per point.  R is the storage type
+// of the density fields (double, or float on the mixed path).
+template <typename R>
 struct OutGradZForce {
   static constexpr int NR = 3 * HEDFT_MAX_IMP;
-  double* out;
-  const double *gx, *gy;
+  R* out;
+  const R *gx, *gy;
   double* partials;
   ImpDev im;
   int want;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double v, double* red) const {
-    out[i] = v;
+    out[i] = (R)v;
     if (want) {
       double xx, yy, zz;
       imp_xyz(im, ix, iy, iz, xx, yy, zz);
@@ -160,67 +162,87 @@ ImpDev make_impdev(hedft_ctx* c);
 double* force_partials(hedft_ctx* c);
 int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp);
 
-int derden_force(hedft_ctx* c, int want_force, int update_aimp) {
+template <typename R>
+static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp) {
   want_force = want_force && c->nimp > 0;
-  InPlain<double> ip{c->den};
-  launch_pencil_xy<double, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ip,
-                                 OutTwo{c->dxden, c->dyden, nullptr});
-  OutGradZForce op{c->dzden, c->dxden, c->dyden, force_partials(c), make_impdev(c), want_force};
-  launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], ip, op);
+  R* den = reinterpret_cast<R*>(c->den);
+  R* dx = reinterpret_cast<R*>(c->dxden);
+  R* dy = reinterpret_cast<R*>(c->dyden);
+  R* dz = reinterpret_cast<R*>(c->dzden);
+  InPlain<R> ip{den};
+  launch_pencil_xy<R, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ip, OutTwo<R>{dx, dy, nullptr});
+  OutGradZForce<R> op{dz, dx, dy, force_partials(c), make_impdev(c), want_force};
+  launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip, op);
   if (hipGetLastError() != hipSuccess) return HEDFT_EHIP;
-  return want_force ? force_finalize(c, pencil_geom<double>(c->d, 2).nblocks, update_aimp) : HEDFT_OK;
+  return want_force ? force_finalize(c, pencil_geom<R>(c->d, 2).nblocks, update_aimp) : HEDFT_OK;
+}
+
+int derden_force(hedft_ctx* c, int want_force, int update_aimp) {
+  return c->mixed ? derden_force_t<float>(c, want_force, update_aimp)
+                  : derden_force_t<double>(c, want_force, update_aimp);
 }
 
 // z pass of the alpha_s divergence with the mean-field assembly:
 // ualphas = (conv/den0s + sum_a d_a(f int_a)) h2o2m4 alphas   (term_alfa.f90:80-98)
 // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ uext] + ualphas
 // (He_statics/poten.f90:277-296, Dynamics_TDDFT/poten.f90:194-211)
+template <typename R>
 struct OutDivPot4 {
   static constexpr int NR = 0;
   double* partials;
-  const double *acc, *conv, *delj4, *dencg, *corr, *uext;
-  double *ualphas, *pot4;
+  const R *acc, *conv, *delj4, *dencg, *corr, *uext;
+  R *ualphas, *pot4;
   double den0s, halpha, a0, a1;
   int lext;
   __device__ __forceinline__ void store(size_t i, int, int, int, double v, double*) const {
-    const double ua = (conv[i] / den0s + (acc[i] + v)) * halpha;
-    ualphas[i] = ua;
+    const double ua = ((double)conv[i] / den0s + ((double)acc[i] + v)) * halpha;
+    ualphas[i] = (R)ua;
     const double cg = dencg[i];
-    double p = delj4[i] + cg * cg * (a0 + a1 * cg) + corr[i];
+    double p = (double)delj4[i] + cg * cg * (a0 + a1 * cg) + (double)corr[i];
     if (lext) p += uext[i];
-    pot4[i] = p + ua;
+    pot4[i] = (R)(p + ua);
   }
 };
 
 // x + y part of the divergence sum_a d_a(f int_a), f = 1 - denalf/den0s, into work1
-int alpha_div_xy(hedft_ctx* c) {
-  InFInt2 ixy{c->denalf, c->intx, c->inty, c->p.den0s};
-  launch_pencil_xy<double, false>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ixy,
-                                  OutSumXY<double>{c->work1, nullptr});
+template <typename R>
+static int alpha_div_xy_t(hedft_ctx* c) {
+  auto F = [](double* p) { return reinterpret_cast<R*>(p); };
+  InFInt2<R> ixy{F(c->denalf), F(c->intx), F(c->inty), c->p.den0s};
+  launch_pencil_xy<R, false>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ixy,
+                             OutSumXY<R>{F(c->work1), nullptr});
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 
+int alpha_div_xy(hedft_ctx* c) { return c->mixed ? alpha_div_xy_t<float>(c) : alpha_div_xy_t<double>(c); }
+
 // z part of the divergence, fused with the ualphas and pot4 assembly
-int alpha_div_z_pot4(hedft_ctx* c) {
+template <typename R>
+static int alpha_div_z_pot4_t(hedft_ctx* c) {
+  auto F = [](double* p) { return reinterpret_cast<R*>(p); };
   const double den0s = c->p.den0s;
-  InFInt iz{c->denalf, c->intz, den0s};
-  OutDivPot4 op;
+  InFInt<R> iz{F(c->denalf), F(c->intz), den0s};
+  OutDivPot4<R> op;
   op.partials = nullptr;
-  op.acc = c->work1;
-  op.conv = c->ualph_conv;
-  op.delj4 = c->delj4;
-  op.dencg = c->dencg;
-  op.corr = c->sto1;
-  op.uext = c->uext;
-  op.ualphas = c->ualphas;
-  op.pot4 = c->pot4;
+  op.acc = F(c->work1);
+  op.conv = F(c->ualph_conv);
+  op.delj4 = F(c->delj4);
+  op.dencg = F(c->dencg);
+  op.corr = F(c->sto1);
+  op.uext = F(c->uext);
+  op.ualphas = F(c->ualphas);
+  op.pot4 = F(c->pot4);
   op.den0s = den0s;
   op.halpha = c->p.h2o2m4 * c->p.alphas;
   op.a0 = c->p.cp4 / 2.0;
   op.a1 = c->p.cpp4 / 3.0;
   op.lext = (c->mode == HEDFT_MODE_STATICS) && c->p.lexternal;
-  launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], iz, op);
+  launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], iz, op);
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+int alpha_div_z_pot4(hedft_ctx* c) {
+  return c->mixed ? alpha_div_z_pot4_t<float>(c) : alpha_div_z_pot4_t<double>(c);
 }
 
 int lap_xy(hedft_ctx* c, const void* psi, int is_complex) {

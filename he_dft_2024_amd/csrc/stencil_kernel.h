@@ -14,23 +14,51 @@
 
 namespace hedft {
 
+// V: element type staged in LDS (double, double2, or float, float2 on the mixed
+// path); A = acc_t<V>: the derivative is always accumulated in double.
+template <typename V>
+struct acc_of {
+  using type = V;
+};
+template <>
+struct acc_of<float> {
+  using type = double;
+};
+template <>
+struct acc_of<float2> {
+  using type = double2;
+};
+template <typename V>
+using acc_t = typename acc_of<V>::type;
+
 template <typename V>
 __device__ __forceinline__ V vzero();
 template <>
 __device__ __forceinline__ double vzero<double>() { return 0.0; }
 template <>
 __device__ __forceinline__ double2 vzero<double2>() { return make_double2(0.0, 0.0); }
+template <>
+__device__ __forceinline__ float vzero<float>() { return 0.0f; }
+template <>
+__device__ __forceinline__ float2 vzero<float2>() { return make_float2(0.0f, 0.0f); }
 __device__ __forceinline__ double vfma(double c, double v, double acc) { return fma(c, v, acc); }
 __device__ __forceinline__ double2 vfma(double c, double2 v, double2 acc) {
   return make_double2(fma(c, v.x, acc.x), fma(c, v.y, acc.y));
 }
+__device__ __forceinline__ double vfma(double c, float v, double acc) { return fma(c, (double)v, acc); }
+__device__ __forceinline__ double2 vfma(double c, float2 v, double2 acc) {
+  return make_double2(fma(c, (double)v.x, acc.x), fma(c, (double)v.y, acc.y));
+}
 __device__ __forceinline__ double vmul(double c, double v) { return c * v; }
 __device__ __forceinline__ double2 vmul(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
+// sign flips of periodic halos (c = +-1, exact)
+__device__ __forceinline__ float vmul(double c, float v) { return (float)c * v; }
+__device__ __forceinline__ float2 vmul(double c, float2 v) { return make_float2((float)c * v.x, (float)c * v.y); }
 
 // Derivative at position i of a pencil P (element stride s) of length n.
 template <typename V>
-__device__ __forceinline__ V stencil_at(const StencilCoef& cf, const V* P, int s, int i, int n) {
-  V acc = vzero<V>();
+__device__ __forceinline__ acc_t<V> stencil_at(const StencilCoef& cf, const V* P, int s, int i, int n) {
+  acc_t<V> acc = vzero<acc_t<V>>();
   if (cf.mode != 0) {
     const double sg = (cf.mode == 2) ? -1.0 : 1.0;
 #pragma unroll
@@ -60,7 +88,7 @@ __device__ __forceinline__ V stencil_at(const StencilCoef& cf, const V* P, int s
 // Outputs whose 13-point window needs a boundary row fall back to stencil_at.
 template <typename V, int KB>
 __device__ __forceinline__ void stencil_run(const StencilCoef& cf, const V* P, int s, int i0, int cnt, int n,
-                                            V (&out)[KB]) {
+                                            acc_t<V> (&out)[KB]) {
   const bool interior = (cf.mode != 0) || (i0 >= 6 && i0 + KB <= n - 6);
   if (interior && cnt == KB) {
     V w[KB + 12];
@@ -75,7 +103,7 @@ __device__ __forceinline__ void stencil_run(const StencilCoef& cf, const V* P, i
     }
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      V acc = vzero<V>();
+      acc_t<V> acc = vzero<acc_t<V>>();
 #pragma unroll
       for (int j = 0; j < 13; ++j) acc = vfma(cf.c[j], w[k + j], acc);
       out[k] = acc;
@@ -126,7 +154,7 @@ __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, 
       const int r = e / n, i = e - r * n;
       const int row = row0 + r;
       if (row < nrows) {
-        const V v = stencil_at<V>(cf, lds + r * n, 1, i, n);
+        const acc_t<V> v = stencil_at<V>(cf, lds + r * n, 1, i, n);
         out.store((size_t)row * n + i, i, row % d.ny, row / d.ny, v, red);
       }
     }
@@ -154,7 +182,7 @@ __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, 
       const int run = n / TPC, i_beg = t * run;
       for (int i0 = i_beg; i0 < i_beg + run; i0 += KB) {
         const int cnt = (i_beg + run - i0 < KB) ? i_beg + run - i0 : KB;
-        V v[KB];
+        acc_t<V> v[KB];
         stencil_run<V, KB>(cf, lds + xl, TX, i0, cnt, n, v);
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
@@ -178,10 +206,12 @@ struct PencilGeom {
   size_t nblocks;
 };
 
-// column-tile width of the y/z pencil kernels: 32 real / 16 complex columns
-// (256 B rows), 16 complex columns need 64 KB of LDS at n = 256, so 8 there
+// column-tile width of the y/z pencil kernels: 256 B rows (64 float, 32
+// double / float2, 16 double2 columns); 16 double2 columns need 64 KB of LDS
+// at n = 256, so 8 there
 template <typename V>
 inline int pencil_tx(int n) {
+  if (sizeof(V) == 4) return 64;
   if (sizeof(V) == 8) return 32;
   return (n > 128) ? 8 : 16;
 }
@@ -212,7 +242,9 @@ template <typename V, int AX, class In, class Out>
 inline void launch_pencil(hipStream_t s, const Dims& d, const StencilCoef& cf, const In& in,
                           const Out& out) {
   PencilGeom g = pencil_geom<V>(d, AX);
-  if (AX == 0 || g.tx == 32)
+  if (AX != 0 && g.tx == 64)
+    hipLaunchKernelGGL((k_pencil<V, AX, 64, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
+  else if (AX == 0 || g.tx == 32)
     hipLaunchKernelGGL((k_pencil<V, AX, 32, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
   else if (g.tx == 16)
     hipLaunchKernelGGL((k_pencil<V, AX, 16, In, Out>), g.grid, g.block, g.lds, s, in, out, cf, d, g.tx);
@@ -227,8 +259,8 @@ inline void launch_pencil(hipStream_t s, const Dims& d, const StencilCoef& cf, c
 // same rules as stencil_at (periodic halos are loaded wrapped, with the sign
 // flip of antiperiodic icon=18 applied to the wrapped values).
 template <typename V>
-__device__ __forceinline__ V stencil_tile_x(const StencilCoef& cf, const V* row, int gx, int nx, int x0) {
-  V acc = vzero<V>();
+__device__ __forceinline__ acc_t<V> stencil_tile_x(const StencilCoef& cf, const V* row, int gx, int nx, int x0) {
+  acc_t<V> acc = vzero<acc_t<V>>();
   const int off = 6 - x0;  // LDS position of global index g is g + off
   if (cf.mode != 0 || (gx >= 6 && gx < nx - 6)) {
 #pragma unroll
@@ -244,7 +276,8 @@ __device__ __forceinline__ V stencil_tile_x(const StencilCoef& cf, const V* row,
   return acc;
 }
 
-// In::load2(i, a, b): a feeds the x derivative, b the y derivative (SAME: a == b)
+// In::loadx(i) feeds the x derivative, In::loady(i) the y derivative (SAME:
+// the same operand, staged once)
 template <typename V, bool SAME, int TX, class In, class Out>
 __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef cfx, StencilCoef cfy, Dims d,
                                                    int tx_unused) {
@@ -262,11 +295,13 @@ __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef c
   {
     // Halo slots outside the box (non-periodic modes) are never read by any
     // output, so they are filled from a clamped in-range address; periodic
-    // halos hold the wrapped value times the antiperiodic sign.
-    constexpr int U = SAME ? 8 : 4;
+    // halos hold the wrapped value times the antiperiodic sign.  The y operand
+    // (B, when it differs from the x operand) is only needed on the TX centre
+    // columns.
+    constexpr int U = 8;
     const double sgw = (cfx.mode == 2) ? -1.0 : 1.0;
     for (int e0 = threadIdx.x; e0 < TW * ny; e0 += 256 * U) {
-      V ta[U], tb[U];
+      V ta[U];
       double sg[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -282,14 +317,30 @@ __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef c
             gx = min(max(gx, 0), nx - 1);
           }
         }
-        in.load2(zb + (size_t)y * nx + gx, ta[u], tb[u]);
+        ta[u] = in.loadx(zb + (size_t)y * nx + gx);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = e0 + u * 256;
-        if (e < TW * ny) {
-          A[e] = vmul(sg[u], ta[u]);
-          if (!SAME) B[e] = tb[u];
+        if (e < TW * ny) A[e] = vmul(sg[u], ta[u]);
+      }
+    }
+    if constexpr (!SAME) {
+      for (int e0 = threadIdx.x; e0 < TX * ny; e0 += 256 * U) {
+        V tb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = min(e0 + u * 256, TX * ny - 1);
+          const int y = e / TX, xl = e - y * TX;
+          tb[u] = in.loady(zb + (size_t)y * nx + min(x0 + xl, nx - 1));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = e0 + u * 256;
+          if (e < TX * ny) {
+            const int y = e / TX, xl = e - y * TX;
+            B[y * TW + xl + 6] = tb[u];
+          }
         }
       }
     }
@@ -301,13 +352,13 @@ __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef c
     const int run = ny / TPC, i_beg = t * run;
     for (int y0 = i_beg; y0 < i_beg + run; y0 += KB) {
       const int cnt = (i_beg + run - y0 < KB) ? i_beg + run - y0 : KB;
-      V dy[KB];
+      acc_t<V> dy[KB];
       stencil_run<V, KB>(cfy, B + xl + 6, TW, y0, cnt, ny, dy);
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         if (k < cnt) {
           const int y = y0 + k;
-          const V dx = stencil_tile_x<V>(cfx, A + y * TW, gx, nx, x0);
+          const acc_t<V> dx = stencil_tile_x<V>(cfx, A + y * TW, gx, nx, x0);
           out.store(zb + (size_t)y * nx + gx, gx, y, z, dx, dy[k], red);
         }
       }
@@ -320,7 +371,7 @@ template <typename V, bool SAME>
 inline PencilGeom pencil_xy_geom(const Dims& d) {
   PencilGeom g;
   g.block = dim3(256);
-  int tx = (sizeof(V) == 8) ? 32 : 16;
+  int tx = (sizeof(V) == 4) ? 64 : (sizeof(V) == 8) ? 32 : 16;
   auto lds = [&](int t) { return sizeof(V) * (size_t)(t + 12) * d.ny * (SAME ? 1 : 2); };
   while (tx > 8 && lds(tx) > 64 * 1024) tx /= 2;
   // the last tile must start at or before nx-7 so that the one-sided rows of the
@@ -348,60 +399,67 @@ template <typename V, bool SAME, class In, class Out>
 inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cfx, const StencilCoef& cfy,
                              const In& in, const Out& out) {
   PencilGeom g = pencil_xy_geom<V, SAME>(d);
-  if (g.tx == 32) launch_pencil_xy_t<V, SAME, 32>(s, g, d, cfx, cfy, in, out);
+  if (g.tx == 64) launch_pencil_xy_t<V, SAME, 64>(s, g, d, cfx, cfy, in, out);
+  else if (g.tx == 32) launch_pencil_xy_t<V, SAME, 32>(s, g, d, cfx, cfy, in, out);
   else if (g.tx == 16) launch_pencil_xy_t<V, SAME, 16>(s, g, d, cfx, cfy, in, out);
   else launch_pencil_xy_t<V, SAME, 8>(s, g, d, cfx, cfy, in, out);
 }
 
 // ---- common functors ---------------------------------------------------------
+// Loads return the LDS staging type; stores receive the double accumulator and
+// round to the storage type.
 template <typename V>
 struct InPlain {
   const V* a;
   __device__ __forceinline__ V load(size_t i) const { return a[i]; }
-  __device__ __forceinline__ void load2(size_t i, V& x, V& y) const { x = y = a[i]; }
+  __device__ __forceinline__ V loadx(size_t i) const { return a[i]; }
 };
 // (1 - denalf/den0s) * int_a   (term_alfa.f90:80-85)
+template <typename R>
 struct InFInt {
-  const double *denalf, *ia;
+  const R *denalf, *ia;
   double den0s;
-  __device__ __forceinline__ double load(size_t i) const { return (1.0 - denalf[i] / den0s) * ia[i]; }
+  __device__ __forceinline__ R load(size_t i) const { return (R)((1.0 - (double)denalf[i] / den0s) * ia[i]); }
 };
 // (f int_x, f int_y) for the fused x + y divergence pass
+template <typename R>
 struct InFInt2 {
-  const double *denalf, *ix, *iy;
+  const R *denalf, *ix, *iy;
   double den0s;
-  __device__ __forceinline__ void load2(size_t i, double& a, double& b) const {
-    const double f = 1.0 - denalf[i] / den0s;
-    a = f * ix[i];
-    b = f * iy[i];
-  }
+  __device__ __forceinline__ R loadx(size_t i) const { return (R)((1.0 - (double)denalf[i] / den0s) * ix[i]); }
+  __device__ __forceinline__ R loady(size_t i) const { return (R)((1.0 - (double)denalf[i] / den0s) * iy[i]); }
 };
+__device__ __forceinline__ double2 vadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double vadd(double a, double b) { return a + b; }
+template <typename V>
+__device__ __forceinline__ V vround(acc_t<V> a);
+template <>
+__device__ __forceinline__ double vround<double>(double a) { return a; }
+template <>
+__device__ __forceinline__ double2 vround<double2>(double2 a) { return a; }
+template <>
+__device__ __forceinline__ float vround<float>(double a) { return (float)a; }
+template <>
+__device__ __forceinline__ float2 vround<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
 // out = dx + dy (Laplacian / divergence partial sums)
 template <typename V>
 struct OutSumXY {
   static constexpr int NR = 0;
   V* out;
   double* partials;
-  __device__ __forceinline__ void store(size_t i, int, int, int, V dx, V dy, double*) const;
+  __device__ __forceinline__ void store(size_t i, int, int, int, acc_t<V> dx, acc_t<V> dy, double*) const {
+    out[i] = vround<V>(vadd(dx, dy));
+  }
 };
-template <>
-__device__ __forceinline__ void OutSumXY<double>::store(size_t i, int, int, int, double dx, double dy,
-                                                        double*) const {
-  out[i] = dx + dy;
-}
-template <>
-__device__ __forceinline__ void OutSumXY<double2>::store(size_t i, int, int, int, double2 dx, double2 dy,
-                                                         double*) const {
-  out[i] = make_double2(dx.x + dy.x, dx.y + dy.y);
-}
 // two separate outputs (derden)
+template <typename R>
 struct OutTwo {
   static constexpr int NR = 0;
-  double *ox, *oy;
+  R *ox, *oy;
   double* partials;
   __device__ __forceinline__ void store(size_t i, int, int, int, double dx, double dy, double*) const {
-    ox[i] = dx;
-    oy[i] = dy;
+    ox[i] = (R)dx;
+    oy[i] = (R)dy;
   }
 };
 template <typename V>
@@ -409,24 +467,9 @@ struct OutStore {
   static constexpr int NR = 0;
   V* out;
   double* partials;
-  __device__ __forceinline__ void store(size_t i, int, int, int, V v, double*) const { out[i] = v; }
+  __device__ __forceinline__ void store(size_t i, int, int, int, acc_t<V> v, double*) const {
+    out[i] = vround<V>(v);
+  }
 };
-template <typename V>
-struct OutAcc {  // out = acc + d
-  static constexpr int NR = 0;
-  V* out;
-  const V* acc;
-  double* partials;
-  __device__ __forceinline__ void store(size_t i, int, int, int, V v, double*) const;
-};
-template <>
-__device__ __forceinline__ void OutAcc<double>::store(size_t i, int, int, int, double v, double*) const {
-  out[i] = acc[i] + v;
-}
-template <>
-__device__ __forceinline__ void OutAcc<double2>::store(size_t i, int, int, int, double2 v, double*) const {
-  const double2 a = acc[i];
-  out[i] = make_double2(a.x + v.x, a.y + v.y);
-}
 
 }  // namespace hedft
