@@ -41,7 +41,7 @@ def build(verbose: bool = True, out: str = OUT, extra: tuple = ()) -> str:
         objs = list(ex.map(lambda s: _compile(s, out, tuple(extra)), SOURCES))
     lib = os.path.join(out, "libhedft.so")
     tmp = lib + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")

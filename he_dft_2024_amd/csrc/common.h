@@ -246,6 +246,7 @@ struct hedft_ctx {
   hipGraphExec_t graph_exec;
   double graph_dt;
   int graph_valid;
+  int graph_slots[13];  // PC slot indices the graph was captured from
 };
 
 namespace hedft {

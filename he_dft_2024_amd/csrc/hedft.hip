@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <thread>
 #include <vector>
 #include <utility>
 
@@ -635,8 +637,8 @@ static double fvlj_radial(double p, double eps, double sigma, double hc) {  // f
   return (double)(8.0L * (ldbl)eps * s6 * (s6 * sinx11(a, (ldbl)hc) - sinx5(a, (ldbl)hc)) / (ldbl)p);
 }
 
-static void kernel_tables(const hedft_ctx* c, std::vector<double>& fv, std::vector<double>& wc,
-                          std::vector<double>& ka) {
+static void kernel_tables_compute(const hedft_ctx* c, std::vector<double>& fv, std::vector<double>& wc,
+                                  std::vector<double>& ka) {
   const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz, nxh = c->d.nxh;
   const double hpx = 1.0 / (nx * c->hx), hpy = 1.0 / (ny * c->hy), hpz = 1.0 / (nz * c->hz);
   const double pmx = 1.0 / (2.0 * c->hx), pmy = 1.0 / (2.0 * c->hy), pmz = 1.0 / (2.0 * c->hz);
@@ -653,23 +655,62 @@ static void kernel_tables(const hedft_ctx* c, std::vector<double>& fv, std::vect
   wc.resize(c->d.nh);
   ka.resize(c->d.nh);
   const double h4 = c->p.h4, twopi = 2.0 * M_PI;
-#pragma omp parallel for schedule(static)
-  for (int iz = 0; iz < nz; ++iz)
-    for (int iy = 0; iy < ny; ++iy)
-      for (int ix = 0; ix < nxh; ++ix) {
-        const size_t k = (size_t)ix + (size_t)nxh * (iy + (size_t)ny * iz);
-        const double pm = std::sqrt(px[ix] * px[ix] + py[iy] * py[iy] + pz[iz] * pz[iz]);
-        fv[k] = fvlj_radial(pm, c->p.eps4, c->p.sigma4, h4);
-        double w = 1.0;  // initcg.f90:314-331
-        if (pm > 0.0) {
-          const double arg = twopi * pm * h4;
-          const double aux = std::sin(arg) - arg * std::cos(arg);
-          w = (std::fabs(aux) < 1e-10) ? 1.0 : 3.0 * aux / (arg * arg * arg);
+  auto planes = [&](int z0, int z1) {
+    for (int iz = z0; iz < z1; ++iz)
+      for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nxh; ++ix) {
+          const size_t k = (size_t)ix + (size_t)nxh * (iy + (size_t)ny * iz);
+          const double pm = std::sqrt(px[ix] * px[ix] + py[iy] * py[iy] + pz[iz] * pz[iz]);
+          fv[k] = fvlj_radial(pm, c->p.eps4, c->p.sigma4, h4);
+          double w = 1.0;  // initcg.f90:314-331
+          if (pm > 0.0) {
+            const double arg = twopi * pm * h4;
+            const double aux = std::sin(arg) - arg * std::cos(arg);
+            w = (std::fabs(aux) < 1e-10) ? 1.0 : 3.0 * aux / (arg * arg * arg);
+          }
+          wc[k] = w;
+          const double e = M_PI * c->p.l * pm;  // BCN4HeDFT.f90:849
+          ka[k] = std::exp(-e * e);
         }
-        wc[k] = w;
-        const double e = M_PI * c->p.l * pm;  // BCN4HeDFT.f90:849
-        ka[k] = std::exp(-e * e);
+  };
+  // z-planes split over host threads (each value depends only on its own p)
+  const int nt = std::max(1, std::min({(int)std::thread::hardware_concurrency(), 16, nz}));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t) pool.emplace_back(planes, nz * t / nt, nz * (t + 1) / nt);
+  for (auto& th : pool) th.join();
+}
+
+// The tables depend only on the grid and the functional constants; contexts
+// with the same ones (an ensemble, a serving loop) share one host computation.
+static void kernel_tables(const hedft_ctx* c, std::vector<double>& fv, std::vector<double>& wc,
+                          std::vector<double>& ka) {
+  struct Entry {
+    double key[10];
+    std::vector<double> fv, wc, ka;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  const double key[10] = {(double)c->d.nx, (double)c->d.ny, (double)c->d.nz, c->hx, c->hy, c->hz,
+                          c->p.h4, c->p.eps4, c->p.sigma4, c->p.l};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry& e : cache)
+      if (std::memcmp(e.key, key, sizeof key) == 0) {
+        fv = e.fv;
+        wc = e.wc;
+        ka = e.ka;
+        return;
       }
+  }
+  kernel_tables_compute(c, fv, wc, ka);
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 8) cache.erase(cache.begin());
+  Entry e;
+  std::memcpy(e.key, key, sizeof key);
+  e.fv = fv;
+  e.wc = wc;
+  e.ka = ka;
+  cache.push_back(std::move(e));
 }
 
 // ---- mean field pipelines --------------------------------------------------------------------
@@ -1819,48 +1860,85 @@ int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk
 
 // One PC iteration per slot phase; the slot rotation has period 6, so six
 // consecutive iterations are captured into one graph and replayed.
-int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph, double* elapsed_ms) {
-  if (!c || c->mode != HEDFT_MODE_TDDFT || nsteps < 0) return HEDFT_EINVAL;
-  if (!c->rk_done) return HEDFT_ESTATE;
-  hipSetDevice(c->device);
-  const int period = 6;
+}  // extern "C"
+
+namespace hedft {
+static constexpr int PC_PERIOD = 6;  // slot rotation period of the PC history
+
+static void slot_state(const hedft_ctx* c, int* s) {
+  std::memcpy(s, c->ioldp, 3 * sizeof(int));
+  std::memcpy(s + 3, c->ioldh, 2 * sizeof(int));
+  std::memcpy(s + 5, c->ioldr, 3 * sizeof(int));
+  std::memcpy(s + 8, c->ioldv, 3 * sizeof(int));
+  std::memcpy(s + 11, c->iolda, 2 * sizeof(int));
+}
+
+// Prepares a context for graph replay: x+y Laplacian of psi current, and a
+// graph of PC_PERIOD iterations captured from the current slot state and dt.
+static int ensure_graph(hedft_ctx* c, double deltat) {
   // the captured iterations start from a prepared x+y Laplacian of psi (as every
   // iteration leaves it), so prepare it now if an upload invalidated it
   if (!c->lap_fresh) {
     HEDFT_TRY(lap_xy(c, c->psi, 1));
     c->lap_fresh = 1;
   }
-  if (use_graph && (!c->graph_valid || c->graph_dt != deltat)) {
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
-    HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = HEDFT_OK;
-    for (int s = 0; s < period && rc == HEDFT_OK; ++s) {
-      rc = steppc_impl(c, deltat);
-      if (rc == HEDFT_OK) rc = tddft_after_step(c);
-    }
-    hipGraph_t gr = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &gr);
-    if (rc != HEDFT_OK) return rc;
-    if (e != hipSuccess) return HEDFT_EHIP;
-    c->graph = gr;
-    HEDFT_CHECK(hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0));
-    c->graph_valid = 1;
-    c->graph_dt = deltat;
-    // capture advanced the host-side slot indices by one full period, i.e. back
-    // to where they started
+  int slots[13];
+  slot_state(c, slots);
+  if (c->graph_valid && c->graph_dt == deltat && std::memcmp(slots, c->graph_slots, sizeof slots) == 0)
+    return HEDFT_OK;
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = 0;
+  HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = HEDFT_OK;
+  for (int s = 0; s < PC_PERIOD && rc == HEDFT_OK; ++s) {
+    rc = steppc_impl(c, deltat);
+    if (rc == HEDFT_OK) rc = tddft_after_step(c);
   }
-  HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  if (rc != HEDFT_OK) {
+    if (gr) hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return HEDFT_EHIP;
+  c->graph = gr;
+  HEDFT_CHECK(hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0));
+  // capture advanced the host-side slot indices by one full period, i.e. back
+  // to where they started
+  c->graph_valid = 1;
+  c->graph_dt = deltat;
+  std::memcpy(c->graph_slots, slots, sizeof slots);
+  return HEDFT_OK;
+}
+
+// enqueue nsteps PC iterations: graph replays for whole periods, eager steps for the rest
+static int enqueue_pc(hedft_ctx* c, double deltat, int nsteps, int use_graph) {
   int done_steps = 0;
-  if (use_graph) {
-    for (; done_steps + period <= nsteps; done_steps += period) HEDFT_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  if (use_graph && nsteps >= PC_PERIOD) {
+    HEDFT_TRY(ensure_graph(c, deltat));
+    for (; done_steps + PC_PERIOD <= nsteps; done_steps += PC_PERIOD)
+      HEDFT_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
   }
   for (; done_steps < nsteps; ++done_steps) {
     HEDFT_TRY(steppc_impl(c, deltat));
     HEDFT_TRY(tddft_after_step(c));
   }
+  return HEDFT_OK;
+}
+}  // namespace hedft
+
+extern "C" {
+
+int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph, double* elapsed_ms) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT || nsteps < 0) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  hipSetDevice(c->device);
+  if (use_graph && nsteps >= PC_PERIOD) HEDFT_TRY(ensure_graph(c, deltat));  // capture outside the timing
+  HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
+  HEDFT_TRY(enqueue_pc(c, deltat, nsteps, use_graph));
   HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
   HEDFT_CHECK(hipEventSynchronize(c->ev1));
   float ms = 0.f;
@@ -1868,6 +1946,55 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
   c->last_ms = ms;
   if (elapsed_ms) *elapsed_ms = ms;
   return done(c);
+}
+
+int hedft_batch_run_pc(hedft_ctx* const* ctxs, int32_t nctx, const double* deltat, int32_t nsteps,
+                       int32_t chunk, double* elapsed_ms) {
+  if (!ctxs || nctx <= 0 || !deltat || nsteps < 0 || chunk < 0) return HEDFT_EINVAL;
+  const int dev = ctxs[0] ? ctxs[0]->device : -1;
+  for (int i = 0; i < nctx; ++i) {
+    const hedft_ctx* c = ctxs[i];
+    if (!c || c->mode != HEDFT_MODE_TDDFT || c->device != dev) return HEDFT_EINVAL;
+    if (!c->rk_done) return HEDFT_ESTATE;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return HEDFT_EINVAL;  // a context appears once
+  }
+  hipSetDevice(dev);
+  const int ch = chunk > 0 ? ((chunk + PC_PERIOD - 1) / PC_PERIOD) * PC_PERIOD : PC_PERIOD;
+  for (int i = 0; i < nctx; ++i)
+    if (nsteps >= PC_PERIOD) HEDFT_TRY(ensure_graph(ctxs[i], deltat[i]));
+  // every context's main stream waits for a common start marker; chunks are
+  // issued round-robin so that the droplets advance together, each on its own
+  // streams, and their kernels overlap on the device
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  HEDFT_CHECK(hipEventCreate(&t0));
+  HEDFT_CHECK(hipEventCreate(&t1));
+  int rc = HEDFT_OK;
+  hipStream_t s0 = ctxs[0]->main_stream;
+  if (hipEventRecord(t0, s0) != hipSuccess) rc = HEDFT_EHIP;
+  for (int i = 1; i < nctx && rc == HEDFT_OK; ++i)
+    if (hipStreamWaitEvent(ctxs[i]->main_stream, t0, 0) != hipSuccess) rc = HEDFT_EHIP;
+  for (int done_steps = 0; done_steps < nsteps && rc == HEDFT_OK;) {
+    const int k = std::min(ch, nsteps - done_steps);
+    for (int i = 0; i < nctx && rc == HEDFT_OK; ++i) rc = enqueue_pc(ctxs[i], deltat[i], k, 1);
+    done_steps += k;
+  }
+  for (int i = 1; i < nctx && rc == HEDFT_OK; ++i) {
+    hedft_ctx* c = ctxs[i];
+    if (hipEventRecord(c->ev1, c->main_stream) != hipSuccess || hipStreamWaitEvent(s0, c->ev1, 0) != hipSuccess)
+      rc = HEDFT_EHIP;
+  }
+  float ms = 0.f;
+  if (rc == HEDFT_OK && (hipEventRecord(t1, s0) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+    rc = HEDFT_EHIP;
+  for (int i = 0; i < nctx; ++i) hipStreamSynchronize(ctxs[i]->main_stream);
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  if (rc != HEDFT_OK) return rc;
+  if (elapsed_ms) *elapsed_ms = ms;
+  for (int i = 0; i < nctx; ++i) HEDFT_TRY(done(ctxs[i]));
+  return HEDFT_OK;
 }
 
 int hedft_tddft_probe_pc(hedft_ctx* c, double deltat, int32_t nsteps, double* avg_ms_pred,

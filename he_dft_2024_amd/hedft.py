@@ -98,6 +98,7 @@ def lib():
         L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
         L.hedft_last_elapsed.argtypes = [vp, dp]
         L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
+        L.hedft_batch_run_pc.argtypes = [C.POINTER(vp), ip, dp, ip, ip, dp]
         _lib = L
     return _lib
 
@@ -286,3 +287,16 @@ class Droplet:
         a, b = C.c_double(0), C.c_double(0)
         check(lib().hedft_tddft_probe_pc(self._c, dt, nsteps, C.byref(a), C.byref(b)), "probe_pc")
         return a.value, b.value
+
+
+def batch_run_pc(droplets, dt, nsteps, chunk=6):
+    """Advance several TDDFT droplets by nsteps PC iterations each in one
+    batched call (hedft_batch_run_pc); dt is a scalar or one value per droplet.
+    Returns the device time of the whole batch in ms."""
+    n = len(droplets)
+    ptrs = (C.c_void_p * n)(*[dr._c for dr in droplets])
+    dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)).copy()
+    ms = C.c_double(0)
+    check(lib().hedft_batch_run_pc(ptrs, n, dts.ctypes.data_as(C.POINTER(C.c_double)), nsteps, chunk,
+                                   C.byref(ms)), "batch_run_pc")
+    return ms.value

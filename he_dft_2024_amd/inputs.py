@@ -73,3 +73,23 @@ def tddft_setup(drop, n, xmax, n4, seed=0, mass_amu=108.0, r0=(0.0, 0.0, -20.0),
     drop.set_impurities(potion[None, :], delinter, rmax, [mass_amu * MP_U])
     drop.set_impurity_state(np.array(r0)[None, :], np.array(v_aps)[None, :] * PSTOK, np.zeros((1, 3)))
     return den
+
+
+def ensemble(count=48, seed=3, n_lo=200.0, n_hi=2000.0, v_lo=1.0, v_hi=8.0, grid=128, h=0.4, z0=-20.0):
+    """Seeded droplet ensemble: N log-uniform in [n_lo, n_hi], impurity speed
+    uniform in [v_lo, v_hi] A/ps toward the centre along +z from z0, impact
+    parameter b uniform in [0, R] along x, R = 2.22 N^(1/3).  The grid edge is
+    clamped to `grid` (2R+16 A exceeds 128 h for N > ~500; those droplets are
+    truncated by the box).  Returns one dict per droplet."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n4 = float(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        v = float(rng.uniform(v_lo, v_hi))
+        R = 2.22 * n4 ** (1.0 / 3.0)
+        b = float(rng.uniform(0.0, R))
+        need = int(np.ceil((2 * R + 16.0) / h))
+        out.append({"n4": n4, "v": v, "b": b, "grid": grid, "xmax": 0.5 * grid * h,
+                    "clamped": need > grid, "seed": int(rng.integers(0, 2**31 - 1))})
+    return out
+

@@ -218,6 +218,17 @@ int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk
 int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph,
                        double* elapsed_ms);
 
+/* Batched ensemble (no counterpart in the reference; the reference runs one
+ * droplet per process): advances nctx independent TDDFT contexts (any grid,
+ * precision and impurity set-up, RK start done, all on one device) by nsteps
+ * PC iterations each.  Each context's captured 6-iteration graph is replayed
+ * on its own streams, chunk iterations at a time round-robin (chunk rounded up
+ * to a multiple of 6; 0 = 6), so the droplets' kernels overlap on the GPU.
+ * deltat[i] is the step of context i; *elapsed_ms is the device time of the
+ * whole batch (HIP events).  Results equal running each context alone. */
+int hedft_batch_run_pc(hedft_ctx* const* ctxs, int32_t nctx, const double* deltat, int32_t nsteps,
+                       int32_t chunk, double* elapsed_ms);
+
 /* Measurement hook (no counterpart in the reference): nsteps eager PC
  * iterations, exactly as hedft_tddft_run_pc(use_graph=0) runs them, with HIP
  * events on the context's main stream around each launch of the fused
