@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=48,
                     help="droplets in the mixed-precision batched ensemble (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=60, help="PC iterations per droplet in the batched leg")
+    ap.add_argument("--serve-jobs", type=int, default=24, help="jobs in the serving leg (0 skips it)")
+    ap.add_argument("--serve-steps", type=int, default=120, help="PC iterations per serving job")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=12, help="PC iterations in the CPU sample")
     ap.add_argument("--probe-steps", type=int, default=24,
@@ -182,6 +184,10 @@ def main():
     if a.batch > 0:
         batched = batched_leg(H, I, a, dt, steps_per_s)
 
+    serving = None
+    if a.serve_jobs > 0:
+        serving = serving_leg(H, I, a, dt)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
@@ -210,6 +216,7 @@ def main():
         "energy_per_atom_K": e["etot4"] / a.n4,
         "mixed": mixed,
         "mixed_batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
@@ -238,6 +245,26 @@ def batched_leg(H, I, a, dt, plain_steps_per_s):
             "steps_each": a.batch_steps, "ms": round(ms, 3), "dtype": "f32 field / f64 psi",
             "speedup_vs_fp64_plain": round(agg / plain_steps_per_s, 3),
             "clamped_droplets": sum(1 for e in ens if e["clamped"]), "nonfinite": bad}
+
+
+def serving_leg(H, I, a, dt):
+    """Continuous-batching loop: Poisson arrivals at 20 jobs/s (seed 4) of
+    droplets drawn like the ensemble (N 200-2000, 1-8 A/ps, impact parameter
+    0-R, 128^3, mixed precision), admitted by a 64 GB HBM budget, each run for
+    a.serve_steps PC iterations; latency percentiles per 1000 steps."""
+    rng = np.random.default_rng(4)
+    arr = np.cumsum(rng.exponential(1.0 / 20.0, size=a.serve_jobs))
+    ens = I.ensemble(count=a.serve_jobs, seed=4, grid=a.grid)
+    jobs = [H.make_job(arrival_s=float(t), n=e["grid"], h=0.4, n4=e["n4"], nsteps=a.serve_steps, nrk=4, dt=dt,
+                       precision=H.PREC_MIXED, r0=(e["b"], 0.0, -20.0), v0=(0.0, 0.0, e["v"]), seed=e["seed"])
+            for t, e in zip(arr, ens)]
+    res, st = H.serve(jobs, hbm_budget=64e9, chunk=12)
+    ok = sum(1 for r in res if r["status"] == 0 and np.isfinite(r["etot"]))
+    return {"jobs": a.serve_jobs, "steps_each": a.serve_steps, "arrival_rate_per_s": 20.0,
+            "completed_ok": ok, "p50_ms_per_1000_steps": round(st["p50_ms_per_1000"], 2),
+            "p99_ms_per_1000_steps": round(st["p99_ms_per_1000"], 2),
+            "aggregate_steps_per_s": round(st["steps_per_s"], 2), "max_concurrent": st["max_concurrent"],
+            "wall_s": round(st["wall_s"], 3), "note": "classical-impurity jobs only"}
 
 
 def cpu_baseline(n, xmax, n4, dt, nsteps):

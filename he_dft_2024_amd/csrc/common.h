@@ -289,6 +289,11 @@ int zconv(hedft_ctx* c, int nin, cplx_t<R>* const* in, int nout, const R* const*
 template <typename R>
 int inv_yx(hedft_ctx* c, int nb, cplx_t<R>* const* spec, const XOutT<R>* out);
 
+// hedft.hip: queue nsteps PC iterations without waiting (graph replays for
+// whole periods); device bytes of one context
+int enqueue_pc_async(hedft_ctx* c, double deltat, int nsteps);
+size_t ctx_footprint(int n, int mode, int precision);
+
 // stencil.hip
 void stencil_setup(hedft_ctx* c);
 enum SInOp { SIN_PLAIN = 0, SIN_FINT = 1 };

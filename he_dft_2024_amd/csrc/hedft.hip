@@ -1928,6 +1928,22 @@ static int enqueue_pc(hedft_ctx* c, double deltat, int nsteps, int use_graph) {
   }
   return HEDFT_OK;
 }
+// non-blocking: nsteps PC iterations queued on the context's stream
+int enqueue_pc_async(hedft_ctx* c, double deltat, int nsteps) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT || nsteps < 0) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  return enqueue_pc(c, deltat, nsteps, 1);
+}
+
+// device bytes hedft_create allocates for a cubic grid edge n
+size_t ctx_footprint(int n, int mode, int precision) {
+  const size_t N = (size_t)n * n * n, Nh = (size_t)(n / 2 + 1) * n * n;
+  const size_t fw = (precision == HEDFT_PREC_MIXED) ? 4 : 8;
+  const size_t pw = (mode == HEDFT_MODE_TDDFT) ? 16 : 8;
+  size_t b = 17 * fw * N + 9 * pw * N + 16 * N + 6 * 2 * fw * Nh + 3 * fw * Nh;
+  if (mode == HEDFT_MODE_STATICS) b += 2 * 8 * N;
+  return b + (8u << 20);  // partials, tables, twiddles, impurity block
+}
 }  // namespace hedft
 
 extern "C" {

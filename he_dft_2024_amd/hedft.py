@@ -52,6 +52,32 @@ class Energies(C.Structure):
         "eimpu_impu", "eimpu", "etot", "norm")]
 
 
+class Job(C.Structure):
+    """hedft_job: one droplet + classical-impurity TDDFT run for the serving loop."""
+    _fields_ = [("arrival_s", C.c_double), ("n", C.c_int32), ("precision", C.c_int32),
+                ("nsteps", C.c_int32), ("nrk", C.c_int32), ("h", C.c_double), ("n4", C.c_double),
+                ("dt", C.c_double), ("r0", C.c_double * 3), ("v0", C.c_double * 3),
+                ("mass_amu", C.c_double), ("eps", C.c_double), ("sigma", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("pad_", C.c_int32), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("latency_s", C.c_double), ("etot", C.c_double), ("etot4", C.c_double), ("ekin4", C.c_double),
+                ("norm", C.c_double), ("rimp", C.c_double * 3)]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("chunk", C.c_int32), ("hbm_budget", C.c_double),
+                ("max_active", C.c_int32), ("realtime", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("steps_per_s", C.c_double), ("p50_ms_per_1000", C.c_double),
+                ("p99_ms_per_1000", C.c_double), ("peak_bytes", C.c_double), ("max_concurrent", C.c_int32),
+                ("completed", C.c_int32)]
+
+
 class HedftError(RuntimeError):
     pass
 
@@ -99,6 +125,10 @@ def lib():
         L.hedft_last_elapsed.argtypes = [vp, dp]
         L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
         L.hedft_batch_run_pc.argtypes = [C.POINTER(vp), ip, dp, ip, ip, dp]
+        L.hedft_job_bytes.restype = C.c_size_t
+        L.hedft_job_bytes.argtypes = [C.POINTER(Job)]
+        L.hedft_serve.argtypes = [C.POINTER(Job), ip, C.POINTER(ServeOpts), C.POINTER(JobResult),
+                                  C.POINTER(ServeStats)]
         _lib = L
     return _lib
 
@@ -300,3 +330,29 @@ def batch_run_pc(droplets, dt, nsteps, chunk=6):
     check(lib().hedft_batch_run_pc(ptrs, n, dts.ctypes.data_as(C.POINTER(C.c_double)), nsteps, chunk,
                                    C.byref(ms)), "batch_run_pc")
     return ms.value
+
+
+def make_job(arrival_s=0.0, n=32, h=0.4, n4=60.0, nsteps=30, nrk=3, dt=5e-4 / 7.63822291, precision=PREC_FP64,
+             r0=(0.0, 0.0, -4.0), v0=(0.0, 0.0, 3.0), mass_amu=108.0, eps=20.0, sigma=4.0, seed=0):
+    return Job(arrival_s, n, precision, nsteps, nrk, h, n4, dt, (C.c_double * 3)(*r0), (C.c_double * 3)(*v0),
+               mass_amu, eps, sigma, seed)
+
+
+def job_bytes(job) -> int:
+    return int(lib().hedft_job_bytes(C.byref(job)))
+
+
+def serve(jobs, hbm_budget, chunk=6, device=0, max_active=0, realtime=True):
+    """Run the continuous-batching loop (hedft_serve) over `jobs`; returns
+    (results, stats) as lists of dicts / a dict."""
+    n = len(jobs)
+    arr = (Job * n)(*jobs)
+    res = (JobResult * n)()
+    st = ServeStats()
+    opts = ServeOpts(device, chunk, float(hbm_budget), max_active, int(realtime))
+    check(lib().hedft_serve(arr, n, C.byref(opts), res, C.byref(st)), "serve")
+    out = [{"status": r.status, "admit_s": r.admit_s, "done_s": r.done_s, "latency_s": r.latency_s,
+            "etot": r.etot, "etot4": r.etot4, "ekin4": r.ekin4, "norm": r.norm, "rimp": list(r.rimp)} for r in res]
+    stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
+    return out, stats
+

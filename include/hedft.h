@@ -229,6 +229,64 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
 int hedft_batch_run_pc(hedft_ctx* const* ctxs, int32_t nctx, const double* deltat, int32_t nsteps,
                        int32_t chunk, double* elapsed_ms);
 
+/* ---- serving: continuous batching of independent droplet jobs ----------
+ * No counterpart in the reference (one droplet per process there).  A job is a
+ * Woods-Saxon droplet (rho0 = 0.0218, a = 0.7 A, R = 2.22 N^(1/3), seeded
+ * 1e-3 noise, normalised to N) on a cubic grid with one classical impurity
+ * (LJ pair potential capped inside 3 A), started as the TDDFT driver does
+ * (DFT4He3d.f90:525-601, nrk RK-Gill iterations) and advanced nsteps PC
+ * iterations.  Classical impurities only. */
+typedef struct hedft_job {
+  double arrival_s;          /* arrival time, seconds after hedft_serve starts */
+  int32_t n;                 /* cubic grid edge */
+  int32_t precision;         /* HEDFT_PREC_FP64 | HEDFT_PREC_MIXED */
+  int32_t nsteps;            /* PC iterations after the RK start */
+  int32_t nrk;               /* RK-Gill start iterations (>= 3) */
+  double h;                  /* grid step (A); xmax = n h / 2 */
+  double n4;                 /* helium atoms */
+  double dt;                 /* time step (K^-1) */
+  double r0[3];              /* impurity start (A) */
+  double v0[3];              /* impurity velocity (A/ps) */
+  double mass_amu, eps, sigma; /* impurity mass, LJ epsilon (K) and sigma (A) */
+  uint64_t seed;             /* density noise seed */
+} hedft_job;
+
+typedef struct hedft_job_result {
+  int32_t status;            /* HEDFT_OK, the job's error, or HEDFT_ESTATE if never run */
+  int32_t pad_;
+  double admit_s, done_s;    /* seconds since the start of hedft_serve */
+  double latency_s;          /* done - arrival */
+  double etot, etot4, ekin4, norm; /* energies after the last step */
+  double rimp[3];
+} hedft_job_result;
+
+typedef struct hedft_serve_opts {
+  int32_t device;
+  int32_t chunk;             /* PC iterations per scheduling quantum (rounded up to 6) */
+  double hbm_budget;         /* bytes of device memory the admitted jobs may hold */
+  int32_t max_active;        /* 0: limited by the budget only */
+  int32_t realtime;          /* 1: admit a job no earlier than its arrival time */
+} hedft_serve_opts;
+
+typedef struct hedft_serve_stats {
+  double wall_s;             /* host time of the whole loop */
+  double steps_per_s;        /* aggregate PC iterations per second */
+  double p50_ms_per_1000;    /* latency percentiles, normalised to 1000 steps */
+  double p99_ms_per_1000;
+  double peak_bytes;         /* largest footprint held at once */
+  int32_t max_concurrent;    /* most jobs resident at once */
+  int32_t completed;
+} hedft_serve_stats;
+
+/* Device bytes a job holds while resident. */
+size_t hedft_job_bytes(const hedft_job* job);
+/* Runs the request loop until every job has finished; results[i] belongs to
+ * jobs[i].  Jobs are admitted in arrival order when their footprint fits the
+ * budget; resident jobs advance chunk iterations at a time on their own
+ * streams.  stats may be NULL. */
+int hedft_serve(const hedft_job* jobs, int32_t njobs, const hedft_serve_opts* opts,
+                hedft_job_result* results, hedft_serve_stats* stats);
+
 /* Measurement hook (no counterpart in the reference): nsteps eager PC
  * iterations, exactly as hedft_tddft_run_pc(use_graph=0) runs them, with HIP
  * events on the context's main stream around each launch of the fused
