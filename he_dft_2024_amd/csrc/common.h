@@ -247,6 +247,9 @@ struct hedft_ctx {
   double graph_dt;
   int graph_valid;
   int graph_slots[13];  // PC slot indices the graph was captured from
+  // tuning: x+y FFT launches with at most this many arrays use separate row and
+  // column passes instead of one whole-plane workgroup per z-plane
+  int xy_split_nb[2];  // [forward, inverse]
 };
 
 namespace hedft {

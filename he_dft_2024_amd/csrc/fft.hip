@@ -88,6 +88,43 @@ __device__ __forceinline__ void dft5(C* u) {
   u[3] = csub(p2, b2);
 }
 
+// 16 = 4 x 4: u[4 n1 + n2] -> X[k1 + 4 k2]
+template <bool INV, typename C>
+__device__ __forceinline__ void dft16(C* u) {
+  using T = decltype(u[0].x);
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, r = (T)0.70710678118654752440;
+  C a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    a[n2][0] = u[n2];
+    a[n2][1] = u[4 + n2];
+    a[n2][2] = u[8 + n2];
+    a[n2][3] = u[12 + n2];
+    dft4<INV>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+  }
+  // twiddles W16^(n2 k1) (conjugated for the inverse)
+  const T sg = INV ? (T)1 : (T)-1;
+  auto tw = [&](C v, T wr, T wi) { return mkc<C>(v.x * wr - v.y * wi, v.x * wi + v.y * wr); };
+  a[1][1] = tw(a[1][1], c1, sg * s1);
+  a[1][2] = tw(a[1][2], r, sg * r);
+  a[1][3] = tw(a[1][3], s1, sg * c1);
+  a[2][1] = tw(a[2][1], r, sg * r);
+  a[2][2] = mul_mi<INV>(a[2][2]);
+  a[2][3] = tw(a[2][3], -r, sg * r);
+  a[3][1] = tw(a[3][1], s1, sg * c1);
+  a[3][2] = tw(a[3][2], -r, sg * r);
+  a[3][3] = tw(a[3][3], -c1, -sg * s1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    C b0 = a[0][k1], b1 = a[1][k1], b2 = a[2][k1], b3 = a[3][k1];
+    dft4<INV>(b0, b1, b2, b3);
+    u[k1] = b0;
+    u[k1 + 4] = b1;
+    u[k1 + 8] = b2;
+    u[k1 + 12] = b3;
+  }
+}
+
 template <int R, bool INV, typename C>
 __device__ __forceinline__ void dft(C* u) {
   if constexpr (R == 2) {
@@ -100,6 +137,8 @@ __device__ __forceinline__ void dft(C* u) {
     dft4<INV>(u[0], u[1], u[2], u[3]);
   } else if constexpr (R == 5) {
     dft5<INV>(u);
+  } else if constexpr (R == 16) {
+    dft16<INV>(u);
   } else {
     static_assert(R == 8, "radix");
     dft8<INV>(u);
@@ -231,6 +270,18 @@ HEDFT_RAD(128, 16, 8, 4, 4)
 HEDFT_RAD(192, 24, 8, 8, 3)
 HEDFT_RAD(256, 32, 8, 8, 4)
 #undef HEDFT_RAD
+
+// Factorisations of the column FFTs inside the whole-plane kernels: fewer,
+// wider passes (each pass is a full LDS round trip of the plane and a barrier).
+template <int L>
+struct RadP : Rad<L> {};
+template <>
+struct RadP<128> {
+  template <bool INV, int NT, int NC, bool CF, typename C>
+  __device__ static __forceinline__ void run_multi(C* b, int cs, int es, const C* tw) {
+    run_mpasses<128, INV, NT, NC, CF, 1, 16, 8>(b, cs, es, tw);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // x-pass prologue / epilogue (arithmetic in double, storage in R)
@@ -401,7 +452,7 @@ __global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_
     row[M - k] = xm;
   }
   __syncthreads();
-  Rad<NY>::template run_multi<false, NT, NXH, true>(P, 1, PAD, twy);
+  RadP<NY>::template run_multi<false, NT, NXH, true>(P, 1, PAD, twy);
   C* o = out + (size_t)blockIdx.x * NY * NXH;
   for (int e = threadIdx.x; e < NY * NXH; e += NT) {
     const int y = e / NXH, k = e - y * NXH;
@@ -423,7 +474,7 @@ __global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_
   static_assert(PAD == NXH, "plane rows are stored contiguously");
   lds_fill<8, NT>(P, NY * NXH, [&](int e) { return in[e]; });
   __syncthreads();
-  Rad<NY>::template run_multi<true, NT, NXH, true>(P, 1, PAD, twy);
+  RadP<NY>::template run_multi<true, NT, NXH, true>(P, 1, PAD, twy);
   // half-complex-to-real merge, in place: Z'_k = (X_k + X*_{M-k}) + i W^-k
   // (X_k - X*_{M-k}), k = 0..M-1, imaginary parts of X_0 and X_M dropped (c2r
   // semantics).  One thread owns the pair (k, M-k).
@@ -794,7 +845,7 @@ int fwd_xy(hedft_ctx* c, int nb, const XInT<R>* in, cplx_t<R>* const* spec) {
     a.out[b] = spec[b];
     s.p[b] = spec[b];
   }
-  HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
+  if (nb > c->xy_split_nb[0]) HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
     using PL_ = Plane<PX_, PY_, R>;
     constexpr size_t lds_ = PL_::LDS;
     constexpr int nt_ = PL_::NT;
@@ -843,7 +894,7 @@ int inv_yx(hedft_ctx* c, int nb, cplx_t<R>* const* spec, const XOutT<R>* out) {
     a.in[b] = spec[b];
     a.out[b] = out[b];
   }
-  HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
+  if (nb > c->xy_split_nb[1]) HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
     using PL_ = Plane<PX_, PY_, R>;
     constexpr size_t lds_ = PL_::LDS;
     constexpr int nt_ = PL_::NT;

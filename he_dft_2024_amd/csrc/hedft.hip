@@ -1276,6 +1276,11 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   if (c->p.lsolid) c->p.core_otc = 0;  // If(Lsolid) core4='OT ' (BCN4HeDFT.f90:654, DFT4He3d.f90:456)
   c->mode = mode;
   c->mixed = (p->precision == HEDFT_PREC_MIXED);
+  if (const char* e = std::getenv("HEDFT_XY_SPLIT_NB")) {  // tuning override: "fwd,inv"
+    c->xy_split_nb[0] = std::atoi(e);
+    const char* comma = std::strchr(e, ',');
+    c->xy_split_nb[1] = comma ? std::atoi(comma + 1) : c->xy_split_nb[0];
+  }
   c->device = device;
   // two streams at the default priority: the main one and a side one for
   // independent branches of the mean field (joined by events, also under capture)
