@@ -229,6 +229,13 @@ struct hedft_ctx {
   // stream by the mean-field pass that follows a step); cleared whenever psi
   // may change otherwise
   int lap_fresh;
+  // small serial kernels queued by a step for the start of the next mean
+  // field's side branch (off the critical path): 1 = impurity predictor,
+  // 2 = impurity corrector, 4 = errHe reduction
+  int side_pre;
+  double side_dt;
+  int side_args[9];
+  size_t side_nblocks;
   // classical impurities
   int nimp, npot;
   double delinter, rmaxinterpol;

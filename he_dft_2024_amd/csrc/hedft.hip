@@ -855,6 +855,21 @@ static int alpha_term(hedft_ctx* c) {
   return stage_c<R>(c);
 }
 
+// the serial kernels a step left for the side branch (impurity Newton PC,
+// errHe reduction); also called directly where no mean field follows
+static void flush_side_pre(hedft_ctx* c) {
+  const int* a = c->side_args;
+  if (c->side_pre & 4)
+    finalize<1>(c, gen_partials(c), c->side_nblocks, S_ERRPC, 1.0);
+  if (c->side_pre & 1)
+    hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
+                       a[3], a[4], a[5]);
+  if (c->side_pre & 2)
+    hipLaunchKernelGGL(k_imp_corr, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
+                       a[3], a[4], a[5], a[6], c->scal);
+  c->side_pre = 0;
+}
+
 // Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden).
 // grad(den) and the impurity force run on the side branch while the main
 // branch convolves den; with_lap also computes the x+y Laplacian of psi there
@@ -864,6 +879,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   HEDFT_TRY(fork(c));
   {
     SideBranch sb(c);
+    flush_side_pre(c);  // impurity PC and errHe of the step just queued
     HEDFT_TRY(derden_force(c, want_force, want_force));
     if (with_lap) HEDFT_TRY(lap_xy(c, c->psi, 1));
   }
@@ -1030,9 +1046,11 @@ static int steppc_t(hedft_ctx* c, double dt) {
     c->lap_fresh = 0;
     std::swap(c->ioldh[0], c->ioldh[1]);
   }
-  if (c->nimp > 0) {
-    hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, dt, c->ioldr[2], c->ioldv[0],
-                       c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1]);
+  if (c->nimp > 0) {  // runs at the start of the mean field's side branch
+    c->side_pre |= 1;
+    c->side_dt = dt;
+    const int a[9] = {c->ioldr[2], c->ioldv[0], c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1], 0, 0, 0};
+    std::memcpy(c->side_args, a, sizeof a);
     std::swap(c->iolda[0], c->iolda[1]);
   }
   HEDFT_TRY(poten_tddft(c, 0, 1));
@@ -1053,15 +1071,19 @@ static int steppc_t(hedft_ctx* c, double dt) {
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
     kt_mark(c, 1, 1);
     c->lap_fresh = 0;
-    finalize<1>(c, gen_partials(c), pencil_geom<double2>(c->d, 2).nblocks, S_ERRPC, 1.0);
+    c->side_pre |= 4;  // errHe reduction, at the start of the next side branch
+    c->side_nblocks = pencil_geom<double2>(c->d, 2).nblocks;
     const int t = c->ioldp[2];
     c->ioldp[2] = c->ioldp[1];
     c->ioldp[1] = c->ioldp[0];
     c->ioldp[0] = t;
   }
-  if (c->nimp > 0) {
-    hipLaunchKernelGGL(k_imp_corr, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, dt, c->ioldr[1], c->ioldr[2],
-                       c->ioldv[0], c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1], c->scal);
+  if (c->nimp > 0) {  // runs at the start of the after-step mean field's side branch
+    c->side_pre |= 2;
+    c->side_dt = dt;
+    const int a[9] = {c->ioldr[1], c->ioldr[2], c->ioldv[0], c->ioldv[1], c->ioldv[2], c->iolda[0], c->iolda[1],
+                      0, 0};
+    std::memcpy(c->side_args, a, sizeof a);
     int t = c->ioldr[2];
     c->ioldr[2] = c->ioldr[1];
     c->ioldr[1] = c->ioldr[0];
@@ -1687,6 +1709,7 @@ int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, dou
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
   if (!c->rk_done) return HEDFT_ESTATE;
   HEDFT_TRY(steppc_impl(c, deltat));
+  flush_side_pre(c);  // STEPPC ends with the impurity corrector and the errHe sum
   HEDFT_TRY(read_pc_errors(c, errHe, errimp, errvimp));
   return done(c);
 }
