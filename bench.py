@@ -235,6 +235,20 @@ def batched_leg(H, I, a, dt, plain_steps_per_s):
         for it in range(1, 5):
             d.tddft_iteration(dt, it, 4)
         drops.append(d)
+    # autotune the launch structure on a scratch droplet of the same grid and
+    # precision, then apply the choice to the whole ensemble
+    g0 = ens[0]["grid"]
+    scratch = H.Droplet(g0, g0, g0, ens[0]["xmax"], ens[0]["xmax"], ens[0]["xmax"],
+                        H.default_params(core_otc=1, precision=H.PREC_MIXED))
+    I.tddft_setup(scratch, g0, ens[0]["xmax"], ens[0]["n4"], seed=ens[0]["seed"], r0=(ens[0]["b"], 0.0, -20.0),
+                  v_aps=(0.0, 0.0, ens[0]["v"]))
+    scratch.tddft_init()
+    for it in range(1, 5):
+        scratch.tddft_iteration(dt, it, 4)
+    tuning = scratch.autotune(dt, 12)
+    scratch.close()
+    for d in drops:
+        d.set_tuning(tuning)
     H.batch_run_pc(drops, dt, 6, chunk=6)  # warmup (graph capture, caches)
     ms = H.batch_run_pc(drops, dt, a.batch_steps, chunk=6)
     agg = len(drops) * a.batch_steps / (ms / 1000.0)
@@ -244,7 +258,8 @@ def batched_leg(H, I, a, dt, plain_steps_per_s):
     return {"value": round(agg, 2), "unit": "PC steps/s (aggregate)", "n_droplets": len(drops),
             "steps_each": a.batch_steps, "ms": round(ms, 3), "dtype": "f32 field / f64 psi",
             "speedup_vs_fp64_plain": round(agg / plain_steps_per_s, 3),
-            "clamped_droplets": sum(1 for e in ens if e["clamped"]), "nonfinite": bad}
+            "clamped_droplets": sum(1 for e in ens if e["clamped"]), "nonfinite": bad,
+            "autotuned": {k: tuning[k] for k in ("xy_split_fwd", "xy_split_inv", "no_side", "prefer_eager")}}
 
 
 def serving_leg(H, I, a, dt):

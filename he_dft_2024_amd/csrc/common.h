@@ -257,6 +257,8 @@ struct hedft_ctx {
   // tuning: x+y FFT launches with at most this many arrays use separate row and
   // column passes instead of one whole-plane workgroup per z-plane
   int xy_split_nb[2];  // [forward, inverse]
+  int no_side;         // 1: run the side branches on the main stream
+  int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
 };
 
 namespace hedft {

@@ -720,12 +720,14 @@ static void kernel_tables(const hedft_ctx* c, std::vector<double>& fv, std::vect
 // branches must touch disjoint buffers.  Under stream capture the same calls
 // record a graph with two parallel branches.
 static int fork(hedft_ctx* c) {
+  if (c->no_side) return HEDFT_OK;
   HEDFT_CHECK(hipEventRecord(c->ev_fork, c->main_stream));
   HEDFT_CHECK(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
   return HEDFT_OK;
 }
 
 static int join(hedft_ctx* c) {
+  if (c->no_side) return HEDFT_OK;
   HEDFT_CHECK(hipEventRecord(c->ev_join, c->side_stream));
   HEDFT_CHECK(hipStreamWaitEvent(c->main_stream, c->ev_join, 0));
   return HEDFT_OK;
@@ -733,7 +735,7 @@ static int join(hedft_ctx* c) {
 
 struct SideBranch {
   hedft_ctx* c;
-  explicit SideBranch(hedft_ctx* cc) : c(cc) { c->stream = c->side_stream; }
+  explicit SideBranch(hedft_ctx* cc) : c(cc) { c->stream = c->no_side ? c->main_stream : c->side_stream; }
   ~SideBranch() { c->stream = c->main_stream; }
 };
 
@@ -1945,6 +1947,7 @@ static int ensure_graph(hedft_ctx* c, double deltat) {
 // enqueue nsteps PC iterations: graph replays for whole periods, eager steps for the rest
 static int enqueue_pc(hedft_ctx* c, double deltat, int nsteps, int use_graph) {
   int done_steps = 0;
+  use_graph = use_graph && !c->prefer_eager;
   if (use_graph && nsteps >= PC_PERIOD) {
     HEDFT_TRY(ensure_graph(c, deltat));
     for (; done_steps + PC_PERIOD <= nsteps; done_steps += PC_PERIOD)
@@ -2086,6 +2089,61 @@ int hedft_tddft_probe_pc(hedft_ctx* c, double deltat, int32_t nsteps, double* av
   for (auto& e : ev)
     if (e) hipEventDestroy(e);
   if (rc != HEDFT_OK) return rc;
+  return done(c);
+}
+
+static void apply_tuning(hedft_ctx* c, const hedft_tuning* t) {
+  c->xy_split_nb[0] = t->xy_split_fwd;
+  c->xy_split_nb[1] = t->xy_split_inv;
+  c->no_side = t->no_side;
+  c->prefer_eager = t->prefer_eager;
+  c->graph_valid = 0;  // the captured graph encodes the launch structure
+}
+
+int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t) {
+  if (!c || !t || t->xy_split_fwd < 0 || t->xy_split_inv < 0) return HEDFT_EINVAL;
+  apply_tuning(c, t);
+  return HEDFT_OK;
+}
+
+int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t) {
+  if (!c || !t) return HEDFT_EINVAL;
+  t->xy_split_fwd = c->xy_split_nb[0];
+  t->xy_split_inv = c->xy_split_nb[1];
+  t->no_side = c->no_side;
+  t->prefer_eager = c->prefer_eager;
+  t->ms_per_step = 0.0;
+  return HEDFT_OK;
+}
+
+int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tuning* best) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT || trial_steps < 1) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  hipSetDevice(c->device);
+  const int k = ((trial_steps + PC_PERIOD - 1) / PC_PERIOD) * PC_PERIOD;
+  hedft_tuning cur{}, win{};
+  win.ms_per_step = 1e300;
+  const int split[4][2] = {{0, 0}, {1, 1}, {1, 0}, {0, 1}};
+  for (int sp = 0; sp < 4; ++sp)
+    for (int ns = 0; ns < 2; ++ns)
+      for (int eg = 0; eg < 2; ++eg) {
+        cur.xy_split_fwd = split[sp][0];
+        cur.xy_split_inv = split[sp][1];
+        cur.no_side = ns;
+        cur.prefer_eager = eg;
+        apply_tuning(c, &cur);
+        HEDFT_TRY(enqueue_pc(c, deltat, PC_PERIOD, 1));  // warm-up incl. graph capture
+        HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
+        HEDFT_TRY(enqueue_pc(c, deltat, k, 1));
+        HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
+        HEDFT_CHECK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        cur.ms_per_step = ms / k;
+        if (cur.ms_per_step < win.ms_per_step) win = cur;
+      }
+  apply_tuning(c, &win);
+  if (best) *best = win;
   return done(c);
 }
 

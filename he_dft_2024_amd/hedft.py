@@ -78,6 +78,11 @@ class ServeStats(C.Structure):
                 ("completed", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("xy_split_fwd", C.c_int32), ("xy_split_inv", C.c_int32), ("no_side", C.c_int32),
+                ("prefer_eager", C.c_int32), ("ms_per_step", C.c_double)]
+
+
 class HedftError(RuntimeError):
     pass
 
@@ -125,6 +130,9 @@ def lib():
         L.hedft_last_elapsed.argtypes = [vp, dp]
         L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
         L.hedft_batch_run_pc.argtypes = [C.POINTER(vp), ip, dp, ip, ip, dp]
+        L.hedft_autotune.argtypes = [vp, d, ip, C.POINTER(Tuning)]
+        L.hedft_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.hedft_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.hedft_job_bytes.restype = C.c_size_t
         L.hedft_job_bytes.argtypes = [C.POINTER(Job)]
         L.hedft_serve.argtypes = [C.POINTER(Job), ip, C.POINTER(ServeOpts), C.POINTER(JobResult),
@@ -310,6 +318,22 @@ class Droplet:
         ms = C.c_double(0)
         check(lib().hedft_tddft_run_pc(self._c, dt, nsteps, int(use_graph), C.byref(ms)), "run_pc")
         return ms.value
+
+    def autotune(self, dt, trial_steps=12):
+        """Time the launch-structure choices on this droplet (it advances) and
+        keep the fastest; returns the chosen tuning as a dict."""
+        t = Tuning()
+        check(lib().hedft_autotune(self._c, dt, trial_steps, C.byref(t)), "autotune")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, t):
+        tt = Tuning(*[int(t[k]) for k in ("xy_split_fwd", "xy_split_inv", "no_side", "prefer_eager")], 0.0)
+        check(lib().hedft_set_tuning(self._c, C.byref(tt)), "set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        check(lib().hedft_get_tuning(self._c, C.byref(t)), "get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def probe_pc(self, dt, nsteps):
         """nsteps eager PC iterations; average device ms of the fused predictor

@@ -287,6 +287,24 @@ size_t hedft_job_bytes(const hedft_job* job);
 int hedft_serve(const hedft_job* jobs, int32_t njobs, const hedft_serve_opts* opts,
                 hedft_job_result* results, hedft_serve_stats* stats);
 
+/* ---- autotuning (no counterpart in the reference) ----------------------
+ * Launch-structure choices of one context, picked by timing: x+y FFT launches
+ * with at most xy_split_fwd / xy_split_inv arrays use separate row and column
+ * passes instead of whole-plane workgroups; no_side runs the independent
+ * branches of the mean field on the main stream; prefer_eager replays no
+ * graph.  no_side and prefer_eager leave every result bit-identical; the
+ * split FFT passes use another radix order, so results agree to rounding. */
+typedef struct hedft_tuning {
+  int32_t xy_split_fwd, xy_split_inv, no_side, prefer_eager;
+  double ms_per_step;        /* measured time of the winning choice (autotune) */
+} hedft_tuning;
+/* Times trial_steps PC iterations (rounded up to 6, after a 6-iteration
+ * warm-up) for each of the 16 combinations on this context and keeps the
+ * fastest.  The context advances by 16 (6 + trial) iterations. */
+int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tuning* best);
+int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t);
+int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t);
+
 /* Measurement hook (no counterpart in the reference): nsteps eager PC
  * iterations, exactly as hedft_tddft_run_pc(use_graph=0) runs them, with HIP
  * events on the context's main stream around each launch of the fused

@@ -57,3 +57,36 @@ def test_graph_replay_after_partial_period(gpu):
         outs.append(d.field("PSI"))
         d.close()
     assert relerr(outs[1], outs[0]) < 1e-13
+
+
+def test_tuning_choices_keep_results(gpu):
+    """Every launch-structure choice the autotuner can make gives the same
+    trajectory: bit-identical for the stream/graph choices, to rounding for
+    the split FFT passes."""
+    print("[case] tuning choices", flush=True)
+    ref = None
+    for t in ({"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0},
+              {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 1, "prefer_eager": 1},
+              {"xy_split_fwd": 4, "xy_split_inv": 4, "no_side": 0, "prefer_eager": 0}):
+        d = _make(gpu, CASES[1])
+        d.set_tuning(t)
+        d.run_pc(DT, 12, True)
+        psi = d.field("PSI")
+        d.close()
+        if ref is None:
+            ref = psi
+        elif t["xy_split_fwd"] == 0:
+            assert np.array_equal(psi, ref)
+        else:
+            assert relerr(psi, ref) < 1e-12
+
+
+def test_autotune_picks_a_measured_choice(gpu):
+    print("[case] autotune", flush=True)
+    d = _make(gpu, CASES[0])
+    t = d.autotune(DT, 6)
+    assert t["ms_per_step"] > 0
+    assert d.get_tuning()["no_side"] == t["no_side"]
+    d.run_pc(DT, 6, True)
+    assert np.isfinite(d.energy()["etot"])
+    d.close()
