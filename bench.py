@@ -45,6 +45,10 @@ def parse():
                     help="droplets in the mixed-precision batched ensemble (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=60, help="PC iterations per droplet in the batched leg")
     ap.add_argument("--serve-jobs", type=int, default=24, help="jobs in the serving leg (0 skips it)")
+    ap.add_argument("--large-steps", type=int, default=48,
+                    help="PC iterations of the 256^3 N=5000 mixed-precision droplet (0 skips it)")
+    ap.add_argument("--statics-steps", type=int, default=200,
+                    help="imaginary-time iterations of the 64^3 N=300 relaxation (0 skips it)")
     ap.add_argument("--serve-steps", type=int, default=120, help="PC iterations per serving job")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=12, help="PC iterations in the CPU sample")
@@ -188,6 +192,14 @@ def main():
     if a.serve_jobs > 0:
         serving = serving_leg(H, I, a, dt)
 
+    large = None
+    if a.large_steps > 0:
+        large = large_leg(H, I, a, dt)
+
+    statics = None
+    if a.statics_steps > 0:
+        statics = statics_leg(H, I, a)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
@@ -217,6 +229,8 @@ def main():
         "mixed": mixed,
         "mixed_batched": batched,
         "serving": serving,
+        "mixed_256": large,
+        "statics_64": statics,
     }
     print(json.dumps(out), flush=True)
 
@@ -280,6 +294,46 @@ def serving_leg(H, I, a, dt):
             "p99_ms_per_1000_steps": round(st["p99_ms_per_1000"], 2),
             "aggregate_steps_per_s": round(st["steps_per_s"], 2), "max_concurrent": st["max_concurrent"],
             "wall_s": round(st["wall_s"], 3), "note": "classical-impurity jobs only"}
+
+
+def large_leg(H, I, a, dt):
+    """Single large droplet: N=5000 on 256^3 (h=0.4 A), impurity at 5 A/ps,
+    mixed precision; PC steps/s and its algorithmic bandwidth."""
+    n = 256
+    xm = 0.5 * n * 0.4
+    d = H.Droplet(n, n, n, xm, xm, xm, H.default_params(core_otc=1, precision=H.PREC_MIXED))
+    I.tddft_setup(d, n, xm, 5000.0, seed=7, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 5.0))
+    d.tddft_init()
+    for it in range(1, 5):
+        d.tddft_iteration(dt, it, 4)
+    d.run_pc(dt, 6, True)
+    ms = d.run_pc(dt, a.large_steps, True)
+    e = d.energy()
+    d.close()
+    sps = a.large_steps / (ms / 1000.0)
+    N, Nh = n ** 3, (n // 2 + 1) * n * n
+    b = 544 * N + 736 * Nh  # mixed: FP32 field, FP64 psi (same counting convention)
+    return {"value": round(sps, 3), "unit": "PC steps/s", "grid": n, "n4": 5000.0, "steps": a.large_steps,
+            "algorithmic_GBps": round(b * sps / 1e9, 1), "energy_per_atom_K": e["etot4"] / 5000.0,
+            "norm": e["norm"]}
+
+
+def statics_leg(H, I, a):
+    """Imaginary-time relaxation: N=300 on 64^3 (h=0.4 A), OTC, 4 RK-Gill +
+    Hamming PC iterations with the local time step, dtau = 1e-4 (FP64)."""
+    n, xm, n4 = 64, 12.8, 300.0
+    d = H.Droplet(n, n, n, xm, xm, xm, H.default_params(core_otc=1), mode=H.MODE_STATICS)
+    d.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=0))
+    mu = d.statics_init(n4)
+    t0 = time.perf_counter()
+    for it in range(1, a.statics_steps + 1):
+        mu = d.statics_iteration(1e-4, it, n4, mu)[0]
+    el = time.perf_counter() - t0
+    e = d.energy()
+    d.close()
+    return {"value": round(a.statics_steps / el, 2), "unit": "iterations/s (host-timed, syncs each step)",
+            "grid": n, "n4": n4, "steps": a.statics_steps, "E_per_N_K": e["etot"] / n4, "mu_K": mu,
+            "norm": e["norm"]}
 
 
 def cpu_baseline(n, xmax, n4, dt, nsteps):

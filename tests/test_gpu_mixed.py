@@ -82,3 +82,26 @@ def test_mixed_graph_replay_matches_eager(gpu):
         outs.append(d.field("PSI"))
         d.close()
     assert relerr(outs[1], outs[0]) < 1e-13
+
+
+def test_mixed_256_against_fp64_path(gpu):
+    """256^3 (row/column FFT passes, 8-column complex tiles): the mixed path
+    tracks the FP64 path, which the 32^3 trajectory tests pin to the oracle."""
+    n, h, n4 = 256, 0.4, 5000.0
+    xmax = 0.5 * n * h
+    dt = 5e-4 / I.PSTOK
+    print("[case] 256^3 fp64 vs mixed", flush=True)
+    res = []
+    for prec in (gpu.PREC_FP64, gpu.PREC_MIXED):
+        d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1, precision=prec))
+        I.tddft_setup(d, n, xmax, n4, seed=7, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 5.0))
+        d.tddft_init()
+        for it in range(1, 5):
+            d.tddft_iteration(dt, it, 4)
+        d.run_pc(dt, 12, True)
+        res.append(d.energy())
+        d.close()
+    fp, mx = res
+    assert abs(mx["etot"] - fp["etot"]) <= 1e-6 * abs(fp["etot"])
+    assert abs(mx["norm"] - fp["norm"]) <= 1e-9 * fp["norm"]
+    assert abs(fp["norm"] - n4) <= 1e-6 * n4
