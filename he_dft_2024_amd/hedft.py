@@ -78,6 +78,12 @@ class ServeStats(C.Structure):
                 ("completed", C.c_int32)]
 
 
+class DenFile(C.Structure):
+    _fields_ = [("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double), ("hx", C.c_double),
+                ("hy", C.c_double), ("hz", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("limp", C.c_int32), ("ximp", C.c_double), ("yimp", C.c_double), ("zimp", C.c_double)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("xy_split_fwd", C.c_int32), ("xy_split_inv", C.c_int32), ("no_side", C.c_int32),
                 ("prefer_eager", C.c_int32), ("ms_per_step", C.c_double)]
@@ -133,6 +139,12 @@ def lib():
         L.hedft_autotune.argtypes = [vp, d, ip, C.POINTER(Tuning)]
         L.hedft_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.hedft_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.hedft_read_density.argtypes = [C.c_char_p, C.c_char, C.POINTER(DenFile), vp, C.c_size_t]
+        L.hedft_write_density.argtypes = [C.c_char_p, C.POINTER(DenFile), vp, C.c_size_t, C.c_char_p]
+        L.hedft_read_wavefunction.argtypes = [C.c_char_p, C.c_char, C.POINTER(DenFile), C.POINTER(C.c_int32), vp,
+                                              vp, ip, vp, C.c_size_t]
+        L.hedft_write_wavefunction.argtypes = [C.c_char_p, C.POINTER(DenFile), ip, vp, vp, vp, C.c_size_t,
+                                               C.c_char_p]
         L.hedft_job_bytes.restype = C.c_size_t
         L.hedft_job_bytes.argtypes = [C.POINTER(Job)]
         L.hedft_serve.argtypes = [C.POINTER(Job), ip, C.POINTER(ServeOpts), C.POINTER(JobResult),
@@ -379,4 +391,64 @@ def serve(jobs, hbm_budget, chunk=6, device=0, max_active=0, realtime=True):
             "etot": r.etot, "etot4": r.etot4, "ekin4": r.ekin4, "norm": r.norm, "rimp": list(r.rimp)} for r in res]
     stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
     return out, stats
+
+
+# ---- density / wave-function files (readen.f90, readenc.f90, printout*.f90) ----
+def _hdr_dict(h):
+    return {k: getattr(h, k) for k, _ in DenFile._fields_}
+
+
+def _hdr_struct(d):
+    h = DenFile()
+    for k, _ in DenFile._fields_:
+        setattr(h, k, d.get(k, 0))
+    return h
+
+
+def read_density(path, cchar=b"#"):
+    """Statics density file -> (header dict, den[nz,ny,nx])."""
+    h = DenFile()
+    b = str(path).encode()
+    check(lib().hedft_read_density(b, cchar, C.byref(h), None, 0), "read_density")
+    den = np.zeros((h.nz, h.ny, h.nx))
+    check(lib().hedft_read_density(b, cchar, C.byref(h), den.ctypes.data_as(C.c_void_p), den.size),
+          "read_density")
+    return _hdr_dict(h), den
+
+
+def write_density(path, header, den, comment=None):
+    den = np.ascontiguousarray(den, dtype=np.float64)
+    h = _hdr_struct(header)
+    check(lib().hedft_write_density(str(path).encode(), C.byref(h), den.ctypes.data_as(C.c_void_p), den.size,
+                                    comment.encode() if comment else None), "write_density")
+
+
+def read_wavefunction(path, cchar=b"#", max_imp=MAX_IMP):
+    """TDDFT wave-function file -> (header dict, rimp[nimp,3], vimp[nimp,3], psi[nz,ny,nx])."""
+    h = DenFile()
+    ni = C.c_int32(0)
+    b = str(path).encode()
+    r = np.zeros((3, max_imp))
+    v = np.zeros((3, max_imp))
+    check(lib().hedft_read_wavefunction(b, cchar, C.byref(h), C.byref(ni), None, None, max_imp, None, 0),
+          "read_wavefunction")
+    psi = np.zeros((h.nz, h.ny, h.nx), dtype=np.complex128)
+    rr = np.zeros(3 * max_imp)
+    vv = np.zeros(3 * max_imp)
+    check(lib().hedft_read_wavefunction(b, cchar, C.byref(h), C.byref(ni), rr.ctypes.data_as(C.c_void_p),
+                                        vv.ctypes.data_as(C.c_void_p), max_imp, psi.ctypes.data_as(C.c_void_p),
+                                        psi.size), "read_wavefunction")
+    k = ni.value
+    return _hdr_dict(h), rr[:3 * k].reshape(3, k).T.copy(), vv[:3 * k].reshape(3, k).T.copy(), psi
+
+
+def write_wavefunction(path, header, rimp, vimp, psi, comment=None):
+    """rimp, vimp: (nimp, 3) arrays (written in the Fortran (N_imp,3) order)."""
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    r = np.ascontiguousarray(np.asarray(rimp, dtype=np.float64).T)
+    v = np.ascontiguousarray(np.asarray(vimp, dtype=np.float64).T)
+    h = _hdr_struct(header)
+    check(lib().hedft_write_wavefunction(str(path).encode(), C.byref(h), r.shape[1], r.ctypes.data_as(C.c_void_p),
+                                         v.ctypes.data_as(C.c_void_p), psi.ctypes.data_as(C.c_void_p), psi.size,
+                                         comment.encode() if comment else None), "write_wavefunction")
 

@@ -229,6 +229,31 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
 int hedft_batch_run_pc(hedft_ctx* const* ctxs, int32_t nctx, const double* deltat, int32_t nsteps,
                        int32_t chunk, double* elapsed_ms);
 
+/* ---- density / wave-function files (host only; no GPU needed) ----------
+ * Text files of the drivers, read as their Fortran list-directed READs do:
+ * leading comment lines starting with cchar skipped (titols.f90:12-42), a
+ * header record, then the values (x fastest).  Statics density:
+ * He_statics/readen.f90:74-84 (written by printout.f90:52-86); TDDFT wave
+ * function: Dynamics_TDDFT/readenc.f90:94-110 (written by printoutc.f90:64-74). */
+typedef struct hedft_denfile {
+  double xmax, ymax, zmax, hx, hy, hz;
+  int32_t nx, ny, nz;
+  int32_t limp;              /* statics header only (logical) */
+  double ximp, yimp, zimp;   /* statics header only */
+} hedft_denfile;
+/* den may be NULL (header only); otherwise n must be nx*ny*nz. */
+int hedft_read_density(const char* path, char cchar, hedft_denfile* hdr, double* den, size_t n);
+/* comment: lines written after '#' (may be NULL); values at full precision. */
+int hedft_write_density(const char* path, const hedft_denfile* hdr, const double* den, size_t n,
+                        const char* comment);
+/* psi: n interleaved complex values (2n doubles), may be NULL; rimp/vimp:
+ * (nimp,3) with k fastest, up to max_imp impurities. */
+int hedft_read_wavefunction(const char* path, char cchar, hedft_denfile* hdr, int32_t* nimp,
+                            double* rimp, double* vimp, int32_t max_imp, double* psi, size_t n);
+int hedft_write_wavefunction(const char* path, const hedft_denfile* hdr, int32_t nimp,
+                             const double* rimp, const double* vimp, const double* psi, size_t n,
+                             const char* comment);
+
 /* ---- serving: continuous batching of independent droplet jobs ----------
  * No counterpart in the reference (one droplet per process there).  A job is a
  * Woods-Saxon droplet (rho0 = 0.0218, a = 0.7 A, R = 2.22 N^(1/3), seeded

@@ -128,6 +128,9 @@ def ref():
         R.ref_initcg.argtypes = [ip, ip, ip, vp, vp, d]
         R.ref_si.restype = d
         R.ref_si.argtypes = [d]
+        R.ref_read_density.argtypes = [C.c_char_p, ip, C.c_char, vp, vp, ip, C.POINTER(C.c_int)]
+        R.ref_read_wavefunction.argtypes = [C.c_char_p, ip, C.c_char, vp, C.POINTER(C.c_int), vp, vp, ip, vp, ip,
+                                            C.POINTER(C.c_int)]
         R.ref_init_deriv(13, 2, 1)
         _ref = R
     return _ref
@@ -155,6 +158,38 @@ def ref_derivnd(id_, f: np.ndarray, h: float, iv: int, icon: int) -> np.ndarray:
     df = np.zeros_like(f)
     R.ref_deriv3d(id_, nx, ny, nz, h, iv, _ptr(f), _ptr(df), icon, 1 if np.iscomplexobj(f) else 0)
     return df
+
+
+def ref_read_density(path, n, cchar=b"#"):
+    """The reference's readen.f90 records read by the Fortran runtime
+    (oracle/_ref): returns (header[13], den[n])."""
+    R = ref()
+    hdr = np.zeros(13)
+    den = np.zeros(n)
+    err = C.c_int(0)
+    b = path.encode()
+    R.ref_read_density(b, len(b), cchar, _ptr(hdr), _ptr(den), n, C.byref(err))
+    if err.value != 0:
+        raise IOError(f"Fortran read failed (iostat {err.value})")
+    return hdr, den
+
+
+def ref_read_wavefunction(path, n, maximp=4, cchar=b"#"):
+    """readenc.f90 mode 2 records via the Fortran runtime: (header[9], nimp,
+    rimp[maximp,3] Fortran order, vimp, psi[n])."""
+    R = ref()
+    hdr = np.zeros(9)
+    ni = C.c_int(0)
+    rimp = np.zeros((3, maximp))
+    vimp = np.zeros((3, maximp))
+    psi = np.zeros(n, dtype=np.complex128)
+    err = C.c_int(0)
+    b = path.encode()
+    R.ref_read_wavefunction(b, len(b), cchar, _ptr(hdr), C.byref(ni), _ptr(rimp), _ptr(vimp), maximp, _ptr(psi),
+                            n, C.byref(err))
+    if err.value != 0:
+        raise IOError(f"Fortran read failed (iostat {err.value})")
+    return hdr, ni.value, rimp, vimp, psi
 
 
 def stencil_matrix(id_, icon, h=1.0, K=13):

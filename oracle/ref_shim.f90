@@ -80,4 +80,73 @@ function ref_si(x) bind(C, name="ref_si") result(r)
   r = s13adf(x8, ifail)
 end function
 
+
+! readen.f90:74-84 records (mode 0): titols, the header, den -- list-directed
+! READs executed by the Fortran runtime on the reference's own titols
+subroutine ref_read_density(path, plen, cchar, hdr, den, n, ierr) bind(C, name="ref_read_density")
+  integer(c_int), value :: plen, n
+  character(kind=c_char), intent(in) :: path(plen)
+  character(kind=c_char), value :: cchar
+  real(c_double), intent(out) :: hdr(13)
+  real(c_double), intent(out) :: den(n)
+  integer(c_int), intent(out) :: ierr
+  character(len=plen) :: fname
+  character(len=1) :: cc
+  integer :: i, isalto, nxp, nyp, nzp
+  logical :: limp
+  real(kind=8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, ximp, yimp, zimp
+  do i = 1, plen
+    fname(i:i) = path(i)
+  end do
+  cc = cchar
+  ierr = 0
+  open(unit=91, file=fname, status='old', iostat=ierr)
+  if (ierr /= 0) return
+  call titols(91, cc, isalto)
+  read(91, *, iostat=ierr) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp, limp, ximp, yimp, zimp
+  if (ierr == 0) read(91, *, iostat=ierr) den
+  close(91)
+  hdr = (/ xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, dble(nxp), dble(nyp), dble(nzp), merge(1d0, 0d0, limp), &
+           ximp, yimp, zimp /)
+end subroutine
+
+! readenc.f90:94-110 records (mode 2): titols, the header, N_imp, rimp, vimp, psi
+subroutine ref_read_wavefunction(path, plen, cchar, hdr, nimp, rimp, vimp, maximp, psi, n, ierr) &
+    bind(C, name="ref_read_wavefunction")
+  integer(c_int), value :: plen, n, maximp
+  character(kind=c_char), intent(in) :: path(plen)
+  character(kind=c_char), value :: cchar
+  real(c_double), intent(out) :: hdr(9)
+  integer(c_int), intent(out) :: nimp, ierr
+  real(c_double), intent(out) :: rimp(maximp, 3), vimp(maximp, 3)
+  complex(c_double_complex), intent(out) :: psi(n)
+  character(len=plen) :: fname
+  character(len=1) :: cc
+  integer :: i, isalto, nxp, nyp, nzp, ni
+  real(kind=8) :: xmaxp, ymaxp, zmaxp, hxp, hyp, hzp
+  real(kind=8), allocatable :: r(:,:), v(:,:)
+  do i = 1, plen
+    fname(i:i) = path(i)
+  end do
+  cc = cchar
+  open(unit=92, file=fname, status='old', iostat=ierr)
+  if (ierr /= 0) return
+  call titols(92, cc, isalto)
+  read(92, *, iostat=ierr) xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, nxp, nyp, nzp
+  if (ierr == 0) read(92, *, iostat=ierr) ni
+  if (ierr == 0 .and. ni >= 1 .and. ni <= maximp) then
+    allocate(r(ni, 3), v(ni, 3))
+    read(92, *, iostat=ierr) r
+    if (ierr == 0) read(92, *, iostat=ierr) v
+    if (ierr == 0) then
+      rimp(1:ni, :) = r
+      vimp(1:ni, :) = v
+    end if
+  end if
+  if (ierr == 0) read(92, *, iostat=ierr) psi
+  close(92)
+  nimp = ni
+  hdr = (/ xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, dble(nxp), dble(nyp), dble(nzp) /)
+end subroutine
+
 end module ref_shim
