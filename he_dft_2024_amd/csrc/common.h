@@ -204,6 +204,7 @@ struct hedft_ctx {
   double *dxden, *dyden, *dzden, *uext, *sto1, *ualph_conv, *work1, *work2;
   double* timec_r;  // statics local time step
   double* hpsi_r;   // statics H psi
+  double *psi1s, *psi2s;  // statics steepest descent: psi one and two steps back
   // psi and PC history: double (statics) or double2 (TDDFT)
   void *psi, *psiold[3], *hpsiold[2], *pc, *q;
   void* lap;        // Laplacian accumulator (same type as psi)
@@ -258,6 +259,7 @@ struct hedft_ctx {
   // column passes instead of one whole-plane workgroup per z-plane
   int xy_split_nb[2];  // [forward, inverse]
   int no_side;         // 1: run the side branches on the main stream
+  int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
 };
 

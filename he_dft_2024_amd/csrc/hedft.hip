@@ -364,6 +364,51 @@ __global__ void __launch_bounds__(256) k_renorm(long long n, double* psi, double
   }
 }
 
+// ---- steepest descent (He_statics/evolo.f90:5-96, ironing.f90:4-33)
+// g = a1 (hpsi - mu psi), a1 = deltat/(1 - deltat (hmean - mu)) with mu := hmean
+__global__ void __launch_bounds__(256) k_evolo_g(long long n, const double* hpsi, const double* psi,
+                                                 const double* scal, double deltat, double* g) {
+  const double hm = scal[S_MU];
+  const double a1 = deltat / (1.0 - deltat * (hm - hm));
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    g[i] = a1 * (hpsi[i] - hm * psi[i]);
+}
+// one smoothing: interior points = (six neighbours)/12 + centre/2; boundary copied
+__global__ void __launch_bounds__(256) k_ironing(Dims d, const double* in, double* out) {
+#pragma clang fp contract(off)
+  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % d.nx), iy = (int)((i / d.nx) % d.ny), iz = (int)(i / sz);
+    if (ix == 0 || iy == 0 || iz == 0 || ix == d.nx - 1 || iy == d.ny - 1 || iz == d.nz - 1) {
+      out[i] = in[i];
+    } else {
+      double t = in[i - 1] + in[i + 1];
+      t = t + in[i - sy];
+      t = t + in[i + sy];
+      t = t + in[i - sz];
+      t = t + in[i + sz];
+      out[i] = 8.333333333333333e-2 * t + 0.5 * in[i];
+    }
+  }
+}
+// psinw = psi - g + vdt1 (psi - psi1) + vdt2 (psi1 - psi2); shift the history; sum psinw^2
+__global__ void __launch_bounds__(256) k_evolo_new(long long n, double* psi, double* psi1, double* psi2,
+                                                   const double* g, double vdt1, double vdt2, double* partials) {
+  double red[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double p = psi[i], p1 = psi1[i], p2 = psi2[i];
+    const double nw = p - g[i] + vdt1 * (p - p1) + vdt2 * (p1 - p2);
+    psi2[i] = p1;
+    psi1[i] = p;
+    psi[i] = nw;
+    red[0] += nw * nw;
+  }
+  block_reduce_store<1>(red, partials);
+}
+
 // STEPPCR predictor + modifier (steppcr.f90:140-153), sum(psi^2)
 __global__ void __launch_bounds__(256) k_pcr_pred(long long n, double* psi, const double* hpsi, double* pc,
                                                   double* timec, double* po3, const double* ho1, double* ho2,
@@ -893,13 +938,17 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   if (c->p.core_otc) {
     const R* kk[3] = {fld<R>(c->wcgk), fld<R>(c->fvlj4), fld<R>(c->kalfs)};
     cplx_t<R>* so[3] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), spc<R>(c->spec[3])};
-    HEDFT_TRY(zconv<R>(c, 1, &s0, 3, kk, so, 1, nullptr));
+    // one workgroup per (column tile, kernel): the forward z transform is
+    // repeated per kernel, but three times as many workgroups are in flight
+    cplx_t<R>* si[3] = {s0, s0, s0};
+    HEDFT_TRY(c->zconv_fused ? zconv<R>(c, 1, &s0, 3, kk, so, 1, nullptr) : zconv<R>(c, 3, si, 1, kk, so, 1, nullptr));
     XOutT<R> xo[3] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4), xo_store<R>(c->denalf)};
     HEDFT_TRY(inv_yx<R>(c, 3, so, xo));
   } else {
     const R* kk[2] = {fld<R>(c->wcgk), fld<R>(c->fvlj4)};
     cplx_t<R>* so[2] = {spc<R>(c->spec[1]), spc<R>(c->spec[2])};
-    HEDFT_TRY(zconv<R>(c, 1, &s0, 2, kk, so, 1, nullptr));
+    cplx_t<R>* si[2] = {s0, s0};
+    HEDFT_TRY(c->zconv_fused ? zconv<R>(c, 1, &s0, 2, kk, so, 1, nullptr) : zconv<R>(c, 2, si, 1, kk, so, 1, nullptr));
     XOutT<R> xo[2] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4)};
     HEDFT_TRY(inv_yx<R>(c, 2, so, xo));
   }
@@ -1128,6 +1177,13 @@ static void renorm(hedft_ctx* c, double n4) {
                      c->dxyz, c->p.psimin);
 }
 
+// renormalisation without the psimin floor (evolo.f90:84-94)
+static void renorm_nomin(hedft_ctx* c, double n4) {
+  finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+  hipLaunchKernelGGL(k_renorm, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->den, c->scal, n4,
+                     c->dxyz, -1.0);
+}
+
 static int read_scal(hedft_ctx* c, int first, int count, double* out) {
   HEDFT_CHECK(hipMemcpyAsync(c->host_scal, c->scal + first, count * sizeof(double), hipMemcpyDeviceToHost,
                              c->stream));
@@ -1300,6 +1356,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   if (c->p.lsolid) c->p.core_otc = 0;  // If(Lsolid) core4='OT ' (BCN4HeDFT.f90:654, DFT4He3d.f90:456)
   c->mode = mode;
   c->mixed = (p->precision == HEDFT_PREC_MIXED);
+  c->zconv_fused = 1;
   if (const char* e = std::getenv("HEDFT_XY_SPLIT_NB")) {  // tuning override: "fwd,inv"
     c->xy_split_nb[0] = std::atoi(e);
     const char* comma = std::strchr(e, ',');
@@ -1346,6 +1403,8 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   if (mode == HEDFT_MODE_STATICS) {
     rc = rc ? rc : alloc(c, (void**)&c->timec_r, rN);
     rc = rc ? rc : alloc(c, (void**)&c->hpsi_r, rN);
+    rc = rc ? rc : alloc(c, (void**)&c->psi1s, rN);
+    rc = rc ? rc : alloc(c, (void**)&c->psi2s, rN);
   }
   void** psis[] = {&c->psi, &c->psiold[0], &c->psiold[1], &c->psiold[2], &c->hpsiold[0], &c->hpsiold[1],
                    &c->pc, &c->q, &c->lap};
@@ -1420,7 +1479,7 @@ int hedft_destroy(hedft_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   void* ptrs[] = {c->den, c->pot4, c->dencg, c->delj4, c->denalf, c->intx, c->inty, c->intz, c->ualphas,
                   c->dxden, c->dyden, c->dzden, c->uext, c->sto1, c->ualph_conv, c->work1, c->work2,
-                  c->timec_r, c->hpsi_r, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
+                  c->timec_r, c->hpsi_r, c->psi1s, c->psi2s, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
                   c->hpsiold[1], c->pc, c->q, c->lap, c->workc1, c->fden, c->spec[0], c->spec[1], c->spec[2],
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
                   c->imp, c->lstop, c->potion};
@@ -1835,10 +1894,44 @@ int hedft_statics_init(hedft_ctx* c, double n4, double* mu4) {
   HEDFT_TRY(statics_refresh(c));
   HEDFT_TRY(poten_statics(c));
   HEDFT_TRY(hpsi_statics(c, n4));
+  // psi1 = psi2 = psi for the steepest-descent method (BCN4HeDFT.f90:861-862)
+  HEDFT_CHECK(hipMemcpyAsync(c->psi1s, c->psi, sizeof(double) * c->d.n, hipMemcpyDeviceToDevice, c->stream));
+  HEDFT_CHECK(hipMemcpyAsync(c->psi2s, c->psi, sizeof(double) * c->d.n, hipMemcpyDeviceToDevice, c->stream));
   double m;
   HEDFT_TRY(read_scal(c, S_MU, 1, &m));
   if (mu4) *mu4 = m;
   c->rk_done = 0;
+  return done(c);
+}
+
+int hedft_evolo(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t iron, double n4, double* mu4,
+                double* mu4err) {
+  if (!c || c->mode != HEDFT_MODE_STATICS || iron < 0 || !mu4) return HEDFT_EINVAL;
+  HEDFT_TRY(hpsi_statics(c, n4));
+  double* g = c->work1;
+  hipLaunchKernelGGL(k_evolo_g, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->hpsi_r, rz(c->psi), c->scal,
+                     deltat, g);
+  for (int k = 0; k < iron; ++k) {  // ping-pong between work1 and work2
+    double* dst = (g == c->work1) ? c->work2 : c->work1;
+    hipLaunchKernelGGL(k_ironing, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, g, dst);
+    g = dst;
+  }
+  hipLaunchKernelGGL(k_evolo_new, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->psi1s, c->psi2s,
+                     g, vdt1, vdt2, gen_partials(c));
+  renorm_nomin(c, n4);
+  double hmean;
+  HEDFT_TRY(read_scal(c, S_MU, 1, &hmean));
+  if (mu4err) *mu4err = std::fabs(1.0 - *mu4 / hmean);
+  *mu4 = hmean;
+  return done(c);
+}
+
+int hedft_statics_iteration_sd(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t iron, double n4,
+                               double* mu4, double* mu4err) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  HEDFT_TRY(hedft_evolo(c, deltat, vdt1, vdt2, iron, n4, mu4, mu4err));
+  HEDFT_TRY(statics_refresh(c));
+  HEDFT_TRY(poten_statics(c));
   return done(c);
 }
 
@@ -1972,7 +2065,7 @@ size_t ctx_footprint(int n, int mode, int precision) {
   const size_t fw = (precision == HEDFT_PREC_MIXED) ? 4 : 8;
   const size_t pw = (mode == HEDFT_MODE_TDDFT) ? 16 : 8;
   size_t b = 17 * fw * N + 9 * pw * N + 16 * N + 6 * 2 * fw * Nh + 3 * fw * Nh;
-  if (mode == HEDFT_MODE_STATICS) b += 2 * 8 * N;
+  if (mode == HEDFT_MODE_STATICS) b += 4 * 8 * N;
   return b + (8u << 20);  // partials, tables, twiddles, impurity block
 }
 }  // namespace hedft

@@ -46,6 +46,17 @@ module hedft_gpu
     real(c_double) :: ekinx, eHeX, eimpu_impu, eimpu, etot, norm
   end type hedft_energies_t
 
+  type, bind(C) :: hedft_tuning_t
+    integer(c_int32_t) :: xy_split_fwd, xy_split_inv, no_side, prefer_eager
+    real(c_double) :: ms_per_step
+  end type hedft_tuning_t
+
+  type, bind(C) :: hedft_denfile_t
+    real(c_double) :: xmax, ymax, zmax, hx, hy, hz
+    integer(c_int32_t) :: nx, ny, nz, limp
+    real(c_double) :: ximp, yimp, zimp
+  end type hedft_denfile_t
+
   interface
     function hedft_strerror_c(rc) bind(C, name="hedft_strerror")
       import :: c_int, c_ptr
@@ -219,6 +230,72 @@ module hedft_gpu
       real(c_double), value :: deltat
       integer(c_int32_t), value :: nsteps, use_graph
       real(c_double), intent(out) :: elapsed_ms
+    end function
+    integer(c_int) function hedft_evolo(ctx, deltat, vdt1, vdt2, iron, n4, mu4, mu4err) bind(C, name="hedft_evolo")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, vdt1, vdt2, n4
+      integer(c_int32_t), value :: iron
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err
+    end function
+    integer(c_int) function hedft_statics_iteration_sd(ctx, deltat, vdt1, vdt2, iron, n4, mu4, mu4err) &
+         bind(C, name="hedft_statics_iteration_sd")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, vdt1, vdt2, n4
+      integer(c_int32_t), value :: iron
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err
+    end function
+    integer(c_int) function hedft_autotune(ctx, deltat, trial_steps, best) bind(C, name="hedft_autotune")
+      import :: c_int, c_int32_t, c_ptr, c_double, hedft_tuning_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+      integer(c_int32_t), value :: trial_steps
+      type(hedft_tuning_t), intent(out) :: best
+    end function
+    integer(c_int) function hedft_set_tuning(ctx, t) bind(C, name="hedft_set_tuning")
+      import :: c_int, c_ptr, hedft_tuning_t
+      type(c_ptr), value :: ctx
+      type(hedft_tuning_t), intent(in) :: t
+    end function
+    integer(c_int) function hedft_read_density(path, cchar, hdr, den, n) bind(C, name="hedft_read_density")
+      import :: c_int, c_char, c_size_t, c_double, hedft_denfile_t
+      character(kind=c_char), intent(in) :: path(*)
+      character(kind=c_char), value :: cchar
+      type(hedft_denfile_t), intent(out) :: hdr
+      real(c_double), intent(out) :: den(*)
+      integer(c_size_t), value :: n
+    end function
+    integer(c_int) function hedft_write_density(path, hdr, den, n, comment) bind(C, name="hedft_write_density")
+      import :: c_int, c_char, c_size_t, c_double, hedft_denfile_t
+      character(kind=c_char), intent(in) :: path(*), comment(*)
+      type(hedft_denfile_t), intent(in) :: hdr
+      real(c_double), intent(in) :: den(*)
+      integer(c_size_t), value :: n
+    end function
+    integer(c_int) function hedft_read_wavefunction(path, cchar, hdr, nimp, rimp, vimp, max_imp, psi, n) &
+         bind(C, name="hedft_read_wavefunction")
+      import :: c_int, c_int32_t, c_char, c_size_t, c_double, c_double_complex, hedft_denfile_t
+      character(kind=c_char), intent(in) :: path(*)
+      character(kind=c_char), value :: cchar
+      type(hedft_denfile_t), intent(out) :: hdr
+      integer(c_int32_t), intent(out) :: nimp
+      real(c_double), intent(out) :: rimp(*), vimp(*)
+      integer(c_int32_t), value :: max_imp
+      complex(c_double_complex), intent(out) :: psi(*)
+      integer(c_size_t), value :: n
+    end function
+    integer(c_int) function hedft_write_wavefunction(path, hdr, nimp, rimp, vimp, psi, n, comment) &
+         bind(C, name="hedft_write_wavefunction")
+      import :: c_int, c_int32_t, c_char, c_size_t, c_double, c_double_complex, hedft_denfile_t
+      character(kind=c_char), intent(in) :: path(*), comment(*)
+      type(hedft_denfile_t), intent(in) :: hdr
+      integer(c_int32_t), value :: nimp
+      real(c_double), intent(in) :: rimp(*), vimp(*)
+      complex(c_double_complex), intent(in) :: psi(*)
+      integer(c_size_t), value :: n
     end function
   end interface
 

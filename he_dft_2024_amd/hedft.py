@@ -136,6 +136,8 @@ def lib():
         L.hedft_last_elapsed.argtypes = [vp, dp]
         L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
         L.hedft_batch_run_pc.argtypes = [C.POINTER(vp), ip, dp, ip, ip, dp]
+        L.hedft_evolo.argtypes = [vp, d, d, d, ip, d, dp, dp]
+        L.hedft_statics_iteration_sd.argtypes = [vp, d, d, d, ip, d, dp, dp]
         L.hedft_autotune.argtypes = [vp, d, ip, C.POINTER(Tuning)]
         L.hedft_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.hedft_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -318,6 +320,17 @@ class Droplet:
         m, e = C.c_double(mu4), C.c_double(0)
         check(lib().hedft_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e)),
               "statics_iteration")
+        return m.value, e.value
+
+    def evolo(self, dt, n4, mu4, vdt=(0.6, 0.1), iron=1):
+        m, e = C.c_double(mu4), C.c_double(0)
+        check(lib().hedft_evolo(self._c, dt, vdt[0], vdt[1], iron, n4, C.byref(m), C.byref(e)), "evolo")
+        return m.value, e.value
+
+    def statics_iteration_sd(self, dt, n4, mu4, vdt=(0.6, 0.1), iron=1):
+        m, e = C.c_double(mu4), C.c_double(0)
+        check(lib().hedft_statics_iteration_sd(self._c, dt, vdt[0], vdt[1], iron, n4, C.byref(m), C.byref(e)),
+              "statics_iteration_sd")
         return m.value, e.value
 
     def tddft_iteration(self, dt, it, nrk):

@@ -204,6 +204,18 @@ int hedft_statics_init(hedft_ctx* c, double n4, double* mu4);
  * fftfw(den), dencg/denalf, poten (BCN4HeDFT.f90:1113-1160). */
 int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double n4,
                             double* mu4, double* mu4err);
+/* evolo(deltat,mu4,mu4err,n4), He_statics/evolo.f90:5-96: steepest descent
+ * psi <- |psi - g + vdt1 (psi - psi1) + vdt2 (psi1 - psi2)| renormalised, with
+ * g = deltat (H - mu) psi smoothed iron times by ironing.f90:4-33 (the module
+ * values vdt = (0.6, 0.1), iron = 1 are the reference defaults; bulk mode not
+ * covered).  psi1 = psi2 = psi are set by hedft_statics_init
+ * (BCN4HeDFT.f90:861-862).  *mu4 in: previous mu, out: <psi|H|psi>/n4. */
+int hedft_evolo(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t iron, double n4,
+                double* mu4, double* mu4err);
+/* Statics iteration with the steepest-descent method (Lrkpc = .false.):
+ * evolo, then derden, fftfw(den), dencg/denalf, poten (BCN4HeDFT.f90:1114-1160). */
+int hedft_statics_iteration_sd(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t iron,
+                               double n4, double* mu4, double* mu4err);
 /* TDDFT start (DFT4He3d.f90:525-601): psi=sqrt(den) (real), den=|psi|^2,
  * timec, potenimp, poten, forceimp, aimp. */
 int hedft_tddft_init(hedft_ctx* c);

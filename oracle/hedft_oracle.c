@@ -85,6 +85,7 @@ typedef struct {
   double *sto1, *sto2, *sto3, *sto4, *sto5, *sto6, *penalty;
   /* psi and history: real (statics) or complex (TDDFT) stored as doubles */
   double *psi, *timec, *psiold[3], *hpsiold[2], *pc, *q, *hpsi;
+  double *psi1s, *psi2s;  /* statics steepest descent: psi one and two steps back */
   double *workc1;
   /* half spectra */
   cplx *fden, *wk1, *wk2, *wk3, *workk1;
@@ -486,6 +487,7 @@ octx* oracle_create(const ogrid* g, const oparams* p, int mode) {
   c->sto4 = dalloc(n); c->sto5 = dalloc(n); c->sto6 = dalloc(n); c->penalty = dalloc(n);
   c->psi = dalloc(n * cw); c->timec = dalloc(n * cw); c->pc = dalloc(n * cw); c->q = dalloc(n * cw);
   c->hpsi = dalloc(n * cw);
+  c->psi1s = dalloc(n); c->psi2s = dalloc(n);
   for (int k = 0; k < 3; ++k) c->psiold[k] = dalloc(n * cw);
   for (int k = 0; k < 2; ++k) c->hpsiold[k] = dalloc(n * cw);
   c->workc1 = dalloc(2 * n);
@@ -543,7 +545,7 @@ void oracle_destroy(octx* c) {
                     c->intx, c->inty, c->intz, c->ualphas, c->dxden, c->dyden, c->dzden, c->uimp,
                     c->uext, c->work1, c->work2, c->sto1, c->sto2, c->sto3, c->sto4, c->sto5,
                     c->sto6, c->penalty, c->psi, c->timec, c->pc, c->q, c->hpsi, c->psiold[0],
-                    c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->workc1,
+                    c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->psi1s, c->psi2s, c->workc1,
                     (double*)c->fden, (double*)c->wk1, (double*)c->wk2, (double*)c->wk3,
                     (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion};
   for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); ++i) free(list[i]);
@@ -1257,6 +1259,64 @@ void oracle_statics_init(octx* c, double n4, double* mu4) {
   if (c->p.core_otc) oracle_term_alfa(c);
   oracle_poten(c);
   *mu4 = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
+  /* psi1 = psi2 = psi for the steepest-descent method (BCN4HeDFT.f90:861-862) */
+  memcpy(c->psi1s, c->psi, n * sizeof(double));
+  memcpy(c->psi2s, c->psi, n * sizeof(double));
+}
+
+/* ironing (He_statics/ironing.f90:4-33): f = (sum of the six neighbours)/12 +
+ * f/2 on interior points, boundary points unchanged */
+void oracle_ironing(int nx, int ny, int nz, double* f) {
+  size_t n = (size_t)nx * ny * nz, sx = 1, sy = nx, sz = (size_t)nx * ny;
+  double* aux = dalloc(n);
+  memcpy(aux, f, n * sizeof(double));
+  const double a0 = 8.333333333333333e-2, a1 = 0.5;
+#pragma omp parallel for schedule(static)
+  for (int iz = 1; iz < nz - 1; ++iz)
+    for (int iy = 1; iy < ny - 1; ++iy)
+      for (int ix = 1; ix < nx - 1; ++ix) {
+        size_t i = ix + sy * iy + sz * iz;
+        double t = aux[i - sx] + aux[i + sx];
+        t = t + aux[i - sy];
+        t = t + aux[i + sy];
+        t = t + aux[i - sz];
+        t = t + aux[i + sz];
+        f[i] = a0 * t + a1 * aux[i];
+      }
+  free(aux);
+}
+
+/* evolo (He_statics/evolo.f90:5-96), non-bulk: steepest descent with the
+ * vdt momentum terms, iron smoothings of (H - mu) psi, renormalisation */
+void oracle_evolo(octx* c, double deltat, double vdt1, double vdt2, int iron, double n4, double* mu4,
+                  double* mu4err) {
+  size_t n = c->n;
+  double hmean = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
+  *mu4err = fabs(1.0 - *mu4 / hmean);
+  *mu4 = hmean;
+  double a1 = 1.0 - deltat * (hmean - *mu4);
+  a1 = deltat / a1;
+  double* g = c->sto4;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) g[i] = a1 * (c->hpsi[i] - *mu4 * c->psi[i]);
+  for (int k = 0; k < iron; ++k) oracle_ironing(c->nx, c->ny, c->nz, g);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double p = c->psi[i], p1 = c->psi1s[i], p2 = c->psi2s[i];
+    double nw = p - g[i] + vdt1 * (p - p1) + vdt2 * (p1 - p2);
+    c->psi2s[i] = p1;
+    c->psi1s[i] = p;
+    c->psi[i] = nw;
+  }
+  renorm_real(c, n4, 0);
+}
+
+/* driver iteration with the steepest-descent method (BCN4HeDFT.f90:1114-1160) */
+void oracle_statics_iteration_sd(octx* c, double deltat, double vdt1, double vdt2, int iron, double n4,
+                                 double* mu4, double* mu4err) {
+  oracle_evolo(c, deltat, vdt1, vdt2, iron, n4, mu4, mu4err);
+  oracle_statics_refresh(c);
+  oracle_poten(c);
 }
 
 void oracle_statics_iteration(octx* c, double deltat, int iterrk, double n4, double* mu4,

@@ -110,6 +110,9 @@ def lib():
         L.oracle_energy.argtypes = [vp, vp]
         L.oracle_statics_init.argtypes = [vp, C.c_double, dp]
         L.oracle_statics_iteration.argtypes = [vp, C.c_double, ip, C.c_double, dp, dp, dp]
+        L.oracle_evolo.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
+        L.oracle_statics_iteration_sd.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
+        L.oracle_ironing.argtypes = [ip, ip, ip, vp]
         L.oracle_tddft_iteration.argtypes = [vp, C.c_double, ip, ip, dp, dp, dp]
         L.oracle_num_threads.restype = ip
         _lib = L
@@ -128,6 +131,8 @@ def ref():
         R.ref_initcg.argtypes = [ip, ip, ip, vp, vp, d]
         R.ref_si.restype = d
         R.ref_si.argtypes = [d]
+        R.ref_evolo.argtypes = [ip, ip, ip, d, d, d, ip, vp, vp, vp, vp, d, d, d, ip, d, C.POINTER(d),
+                                C.POINTER(d), vp, vp]
         R.ref_read_density.argtypes = [C.c_char_p, ip, C.c_char, vp, vp, ip, C.POINTER(C.c_int)]
         R.ref_read_wavefunction.argtypes = [C.c_char_p, ip, C.c_char, vp, C.POINTER(C.c_int), vp, vp, ip, vp, ip,
                                             C.POINTER(C.c_int)]
@@ -158,6 +163,28 @@ def ref_derivnd(id_, f: np.ndarray, h: float, iv: int, icon: int) -> np.ndarray:
     df = np.zeros_like(f)
     R.ref_deriv3d(id_, nx, ny, nz, h, iv, _ptr(f), _ptr(df), icon, 1 if np.iscomplexobj(f) else 0)
     return df
+
+
+def ironing(f: np.ndarray) -> np.ndarray:
+    """One smoothing pass of ironing.f90 (oracle restatement) on a copy."""
+    g = np.ascontiguousarray(f, dtype=np.float64).copy()
+    nz, ny, nx = g.shape
+    lib().oracle_ironing(nx, ny, nz, _ptr(g))
+    return g
+
+
+def ref_evolo(psi, pot4, psi1, psi2, h, icon, deltat, n4, mu4, vdt=(0.6, 0.1), iron=1):
+    """The reference's evolo (oracle/_ref) on copies of the given arrays:
+    returns (psi, psi1, psi2, den, hpsi, mu4, mu4err)."""
+    R = ref()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in (psi, pot4, psi1, psi2)]
+    nz, ny, nx = arrs[0].shape
+    den = np.zeros_like(arrs[0])
+    hp = np.zeros_like(arrs[0])
+    m, e = C.c_double(mu4), C.c_double(0)
+    R.ref_evolo(nx, ny, nz, h[0], h[1], h[2], icon, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(arrs[3]),
+                deltat, vdt[0], vdt[1], iron, n4, C.byref(m), C.byref(e), _ptr(den), _ptr(hp))
+    return arrs[0], arrs[2], arrs[3], den, hp, m.value, e.value
 
 
 def ref_read_density(path, n, cchar=b"#"):
@@ -244,7 +271,7 @@ class Oracle:
     def field(self, name: str) -> np.ndarray:
         cnt = C.c_size_t()
         p = lib().oracle_field(self._c, FIELD[name], C.byref(cnt))
-        flat = np.ctypeslib.as_array(p, shape=(cnt.value,))
+        flat = np.ctypeslib.as_array(p, shape=(cnt.value,))  # a view of the state: copy to keep a snapshot
         n = self.shape[0] * self.shape[1] * self.shape[2]
         nh = self.hshape[0] * self.hshape[1] * self.hshape[2]
         if cnt.value == n:
@@ -326,6 +353,16 @@ class Oracle:
         m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
         lib().oracle_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e), C.byref(pc))
         return m.value, e.value, pc.value
+
+    def evolo(self, dt, n4, mu4, vdt=(0.6, 0.1), iron=1):
+        m, e = C.c_double(mu4), C.c_double(0)
+        lib().oracle_evolo(self._c, dt, vdt[0], vdt[1], iron, n4, C.byref(m), C.byref(e))
+        return m.value, e.value
+
+    def statics_iteration_sd(self, dt, n4, mu4, vdt=(0.6, 0.1), iron=1):
+        m, e = C.c_double(mu4), C.c_double(0)
+        lib().oracle_statics_iteration_sd(self._c, dt, vdt[0], vdt[1], iron, n4, C.byref(m), C.byref(e))
+        return m.value, e.value
 
     def tddft_iteration(self, dt, it, nrk):
         a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)

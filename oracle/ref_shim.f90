@@ -149,4 +149,39 @@ subroutine ref_read_wavefunction(path, plen, cchar, hdr, nimp, rimp, vimp, maxim
   hdr = (/ xmaxp, ymaxp, zmaxp, hxp, hyp, hzp, dble(nxp), dble(nyp), dble(nzp) /)
 end subroutine
 
+
+! evolo(deltat,mu4,mu4err,n4) (He_statics/evolo.f90:5-96, steepest descent with
+! the vdt momentum terms and ironing.f90 smoothing) on module arrays set up
+! here from the caller's psi, pot4, psi1, psi2; results copied back
+subroutine ref_evolo(nx_, ny_, nz_, hx_, hy_, hz_, icon_, psi_, pot4_, psi1_, psi2_, deltat, vdt1, vdt2, iron_, &
+                     n4r, mu4, mu4err, den_, hpsi_) bind(C, name="ref_evolo")
+  use deriva, only: icon
+  use field, only: hpsi, pot4, iron
+  use grid, only: nx, ny, nz, hx, hy, hz, nxyz, dxyz
+  use he4, only: lbulk
+  use rho, only: den, psi, psinw, psi1, psi2
+  use util1, only: vdt, nn
+  use work1, only: sto1, sto2, sto3, sto4
+  integer(c_int), value :: nx_, ny_, nz_, icon_, iron_
+  real(c_double), value :: hx_, hy_, hz_, deltat, vdt1, vdt2, n4r
+  real(c_double), intent(inout) :: psi_(nx_, ny_, nz_), psi1_(nx_, ny_, nz_), psi2_(nx_, ny_, nz_), mu4
+  real(c_double), intent(in) :: pot4_(nx_, ny_, nz_)
+  real(c_double), intent(out) :: mu4err, den_(nx_, ny_, nz_), hpsi_(nx_, ny_, nz_)
+  integer(kind=4) :: n4
+  real(kind=8) :: d8, m8, e8
+  nx = nx_; ny = ny_; nz = nz_; hx = hx_; hy = hy_; hz = hz_
+  nxyz = nx * ny * nz; dxyz = hx * hy * hz
+  nn = (/ nx, ny, nz /)
+  icon = icon_; iron = iron_; vdt = (/ vdt1, vdt2 /); lbulk = .false.
+  if (allocated(psi)) deallocate(psi, psinw, psi1, psi2, den, hpsi, pot4, sto1, sto2, sto3, sto4)
+  allocate(psi(nx,ny,nz), psinw(nx,ny,nz), psi1(nx,ny,nz), psi2(nx,ny,nz), den(nx,ny,nz), hpsi(nx,ny,nz), &
+           pot4(nx,ny,nz), sto1(nx,ny,nz), sto2(nx,ny,nz), sto3(nx,ny,nz), sto4(nx,ny,nz))
+  psi = psi_; psi1 = psi1_; psi2 = psi2_; pot4 = pot4_; den = psi**2
+  n4 = nint(n4r)
+  d8 = deltat; m8 = mu4
+  call evolo(d8, m8, e8, n4)
+  mu4 = m8; mu4err = e8
+  psi_ = psi; psi1_ = psi1; psi2_ = psi2; den_ = den; hpsi_ = hpsi
+end subroutine
+
 end module ref_shim

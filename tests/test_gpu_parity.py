@@ -217,3 +217,30 @@ def test_uimp_on_grid_line_vs_oracle(gpu):
     for name in ("PSI", "DEN", "POT4", "HPSIOLD1", "HPSIOLD2", "PC"):
         assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
     d.close()
+
+
+def test_statics_steepest_descent_vs_oracle(gpu):
+    """evolo + ironing driver iterations (the statics default when Lrkpc is
+    off) against the oracle, whose evolo is pinned to the reference's own."""
+    n, xmax, n4 = 32, 6.4, 50.0
+    _say("statics steepest descent 10 iterations")
+    d, o = _pair(n, xmax, gpu.MODE_STATICS, core_otc=1)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=8)
+    dt = 0.015 * 0.4 ** 2 / (4 * 6.05969638298)
+    hist = []
+    for x in (d, o):
+        x.set("DEN", den)
+        mu = x.statics_init(n4)
+        mu, _ = x.evolo(0.0, n4, mu)  # first chemical potential (BCN4HeDFT.f90:1050)
+        h = [mu]
+        for it in range(10):
+            mu, err = x.statics_iteration_sd(dt, n4, mu)
+            h.append(mu)
+        hist.append(h)
+    assert relerr(hist[0], hist[1]) < RTOL_TRAJ
+    for name in ("PSI", "DEN", "POT4", "DENCG", "UALPHAS"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    eg, eo = d.energy(), o.energy()
+    for k in ("ekin4", "elj4", "ecor4", "ealphas", "etot", "norm"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
+    d.close()

@@ -323,3 +323,49 @@ def test_tddft_norm_conserved_and_energy_converged_without_absorber():
         assert abs(e1["etot"] - e0["etot"]) < 3e-4 * abs(e0["ekin4"])
         ends.append(e1["etot"])
     assert abs(ends[0] - ends[1]) < 1e-9 * abs(ends[0])
+
+
+# ---------------------------------------------------------------- steepest descent
+def test_ironing_closed_forms():
+    """ironing.f90 keeps constants and linear fields on interior points
+    (1/12 of the six neighbours + 1/2 the centre), maps x^2 to x^2 + h^2/6,
+    and leaves the boundary planes untouched."""
+    n = 9
+    x = np.arange(n, dtype=float)
+    X = np.broadcast_to(x[None, None, :], (n, n, n))
+    Y = np.broadcast_to(x[None, :, None], (n, n, n))
+    lin = 2.0 * X - 3.0 * Y + 0.5
+    assert np.allclose(O.ironing(np.full((n, n, n), 1.5)), 1.5, rtol=0, atol=1e-15)
+    assert np.allclose(O.ironing(lin), lin, rtol=0, atol=1e-13)
+    q = X ** 2
+    out = O.ironing(q)
+    inner = (slice(1, -1),) * 3
+    assert np.allclose(out[inner], (q + 1.0 / 6.0)[inner], rtol=0, atol=1e-12)
+    rnd = np.random.default_rng(3).random((n, n, n))
+    out = O.ironing(rnd)
+    for sl in ((0, slice(None), slice(None)), (slice(None), 0, slice(None)), (slice(None), slice(None), -1)):
+        assert np.array_equal(out[sl], rnd[sl])
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_PATH), reason="oracle/_ref not built (reference tree absent)")
+def test_evolo_matches_reference_evolo():
+    """Two steepest-descent steps of the oracle against the reference's own
+    evolo.f90 + ironing.f90 (compiled into oracle/_ref) on the same psi,
+    pot4 and history."""
+    n, xmax, n4 = 16, 3.2, 20.0
+    o = O.Oracle(n, n, n, xmax, xmax, xmax, O.default_params(core_otc=1), O.MODE_STATICS)
+    o.set("DEN", I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=4))
+    mu = o.statics_init(n4)
+    psi, pot4 = o.field("PSI").copy(), o.field("POT4").copy()  # field() is a view of the oracle state
+    psi1 = psi.copy()
+    psi2 = psi.copy()
+    h = (2 * xmax / n,) * 3
+    dt = 0.05 * h[0] ** 2 / (4 * 6.05969638298)
+    mu_ref = mu
+    for step in range(2):
+        mu, err = o.evolo(dt, n4, mu)
+        psi, psi1, psi2, den, hp, mu_ref, err_ref = O.ref_evolo(psi, pot4, psi1, psi2, h, 13, dt, n4, mu_ref)
+        assert abs(mu - mu_ref) <= 1e-12 * abs(mu_ref), step
+        assert abs(err - err_ref) <= 1e-9 * max(abs(err_ref), 1e-300) + 1e-15, step
+        assert relerr(o.field("PSI"), psi) < 1e-12, step
+        assert relerr(o.field("DEN"), den) < 1e-12, step
