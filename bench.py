@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--statics-steps", type=int, default=200,
                     help="imaginary-time iterations of the 64^3 N=300 relaxation (0 skips it)")
     ap.add_argument("--serve-steps", type=int, default=120, help="PC iterations per serving job")
+    ap.add_argument("--quantum-steps", type=int, default=48,
+                    help="PC iterations in the 256^3 quantum-impurity leg (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=12, help="PC iterations in the CPU sample")
     ap.add_argument("--probe-steps", type=int, default=24,
@@ -200,6 +202,10 @@ def main():
     if a.statics_steps > 0:
         statics = statics_leg(H, I, a)
 
+    quantum = None
+    if a.quantum_steps > 0:
+        quantum = quantum_leg(H, I, a)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
@@ -231,6 +237,7 @@ def main():
         "serving": serving,
         "mixed_256": large,
         "statics_64": statics,
+        "quantum_256": quantum,
     }
     print(json.dumps(out), flush=True)
 
@@ -334,6 +341,38 @@ def statics_leg(H, I, a):
     return {"value": round(a.statics_steps / el, 2), "unit": "iterations/s (host-timed, syncs each step)",
             "grid": n, "n4": n4, "steps": a.statics_steps, "E_per_N_K": e["etot"] / n4, "mu_K": mu,
             "norm": e["norm"]}
+
+
+def quantum_leg(H, I, a):
+    """Quantum-impurity TDDFT at the size of the reference's own example
+    (Dynamics_quantum_impurities/input_files/input.0: 256^3, +-32 A, N=1000,
+    Lsolid so core OT, icon 24, t*mean 30 A, deltat 5e-5 ps, impurity mass
+    4.002603 u), FP64.  The X-He kernel is a Lennard-Jones stand-in (eps 1 K,
+    sigma 5.3 A, capped at 3.17502 A) transformed by he_dft_2024_amd.quantum; the
+    reference's He2S_fci table is not shipped.  PC steps/s over graph replays."""
+    from he_dft_2024_amd import quantum as Q
+    n, xm, n4 = 256, 32.0, 1000.0
+    dt = 5e-5 / I.PSTOK
+    p = H.default_params(core_otc=0, lsolid=1, icon=24, tmean=(30.0, 30.0, 30.0))
+    d = H.Droplet(n, n, n, xm, xm, xm, p, mode=H.MODE_TDDFT)
+    v = Q.lennard_jones(1.0, 5.3)
+    t0 = time.perf_counter()
+    uk = Q.uimp_k_table(n, n, n, xm, xm, xm, v, 3.17502, float(v(3.17502)))
+    t_table = time.perf_counter() - t0
+    I.quantum_setup(d, n, xm, n4, uk, seed=11, m_imp_u=4.002603, width=2.0)
+    d.tddft_init()
+    for it in range(1, 4):
+        d.tddft_iteration(dt, it, 3)
+    d.run_pc(dt, 6, True)
+    ms = d.run_pc(dt, a.quantum_steps, True)
+    e = d.energy()
+    nrm, rc = d.quantum_centroid()
+    d.close()
+    sps = a.quantum_steps / (ms / 1000.0)
+    return {"value": round(sps, 3), "unit": "PC steps/s", "grid": n, "n4": n4, "steps": a.quantum_steps,
+            "ms_per_step": round(ms / a.quantum_steps, 3), "dtype": "f64", "uimp_k_table_s": round(t_table, 2),
+            "etot_K": e["etot"], "ekinx_K": e["ekinx"], "eHeX_K": e["eHeX"], "psix_norm": nrm,
+            "psix_centre_A": [round(x, 6) for x in rc]}
 
 
 def cpu_baseline(n, xmax, n4, dt, nsteps):

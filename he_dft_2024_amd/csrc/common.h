@@ -261,16 +261,33 @@ struct hedft_ctx {
   int no_side;         // 1: run the side branches on the main stream
   int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
+  // quantum impurity (Dynamics_quantum_impurities): one impurity wave function
+  // psix with its own PC history, coupled through the real half-spectrum
+  // kernel uimpk: potx4 = F^-1[F(|psix|^2) uimpk] enters pot4, upotx =
+  // F^-1[F(den) uimpk] drives psix
+  int quantum;
+  double h2o2mimp, mu4, m_imp_u;
+  void *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *lapx;  // complex N
+  double *potx4, *upotx;                                     // real N (R storage)
+  double* uimpk;                                             // real half spectrum (R storage)
+  double2* specq[2];                                         // half spectra: |psix|^2, den x uimpk
 };
 
 namespace hedft {
+// field added to pot4 after the correlation term: uext (statics with the
+// external-field option) or potx4 (quantum impurity); null when neither
+inline double* pot4_ext(const hedft_ctx* c) {
+  if (c->mode == HEDFT_MODE_STATICS) return c->p.lexternal ? c->uext : nullptr;
+  return c->quantum ? c->potx4 : nullptr;
+}
+
 // fft.hip
 int fft_setup(hedft_ctx* c);
 int fftfw(hedft_ctx* c, const double* in, double2* out);
 int fftbk(hedft_ctx* c, double2* in, double* out);
 
 // x-pass prologue (R2C input) and epilogue (C2R output) operations
-enum XInOp { XIN_COPY = 0, XIN_CORR = 1, XIN_FGRAD = 2, XIN_DOT3 = 3 };
+enum XInOp { XIN_COPY = 0, XIN_CORR = 1, XIN_FGRAD = 2, XIN_DOT3 = 3, XIN_ABS2 = 4 };
 enum XOutOp { XOUT_STORE = 0, XOUT_POT4 = 1 };
 // R: storage type of the real fields (double: FP64 path, float: mixed path);
 // the pointwise arithmetic of prologues and epilogues is done in double.
@@ -278,13 +295,15 @@ template <typename R>
 struct XInT {
   int op;
   const R* a[6];
+  const double2* z;  // XIN_ABS2: |z|^2 of a complex field
   double c0, c1;
 };
 template <typename R>
 struct XOutT {
   int op;
   R* out;
-  const R *delj4, *dencg, *den, *uext;
+  const R *delj4, *dencg, *den;
+  const R* ext;  // added to pot4 when lext: uext (statics) or potx4 (quantum impurity)
   double a0, a1, csolid, beta, den_m;
   int lsolid, lext;
 };
@@ -318,6 +337,6 @@ int derden_force(hedft_ctx* c, int want_force, int update_aimp);
 // divergence of (1-denalf/den0s)*int_a, then ualphas and pot4 assembly
 int alpha_div_xy(hedft_ctx* c);      // x+y divergence of f int_a into work1
 int alpha_div_z_pot4(hedft_ctx* c);  // z divergence + ualphas + pot4 assembly
-// Laplacian of psi (x then y into c->lap); the z pass is the fused step kernel
-int lap_xy(hedft_ctx* c, const void* psi, int is_complex);
+// x+y Laplacian of psi into out (default c->lap); the z pass is the fused step kernel
+int lap_xy(hedft_ctx* c, const void* psi, int is_complex, void* out = nullptr);
 }  // namespace hedft

@@ -298,6 +298,10 @@ __device__ __forceinline__ double xin_eval(const XInT<R>& in, size_t i) {
     case XIN_DOT3:  // grad(den).int_alf  (term_alfa.f90:61-65)
       return (double)in.a[0][i] * in.a[3][i] + (double)in.a[1][i] * in.a[4][i] +
              (double)in.a[2][i] * in.a[5][i];
+    case XIN_ABS2: {  // psix*conjg(psix)  (Dynamics_quantum_impurities/potenimp.f90:23)
+      const double2 v = in.z[i];
+      return v.x * v.x + v.y * v.y;
+    }
     default:
       return in.a[0][i];
   }
@@ -306,11 +310,12 @@ __device__ __forceinline__ double xin_eval(const XInT<R>& in, size_t i) {
 template <typename R>
 __device__ __forceinline__ void xout_apply(const XOutT<R>& o, size_t i, double v) {
   if (o.op == XOUT_POT4) {
-    // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + conv [+ uext] [+ penalty]
-    // (He_statics/poten.f90:277-289,315-319; Dynamics_TDDFT/poten.f90:194-218)
+    // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + conv [+ uext | potx4] [+ penalty]
+    // (He_statics/poten.f90:277-289,315-319; Dynamics_TDDFT/poten.f90:194-218;
+    // Dynamics_quantum_impurities/poten.f90:116-121)
     const double cg = o.dencg[i];
     double p = (double)o.delj4[i] + cg * cg * (o.a0 + o.a1 * cg) + v;
-    if (o.lext) p += o.uext[i];
+    if (o.lext) p += o.ext[i];
     if (o.lsolid) {
       const double dn = o.den[i];
       const double th = tanh(o.beta * (dn - o.den_m));
@@ -431,6 +436,12 @@ __global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_
         const C i0 = ld2(in.a[3], i), i1 = ld2(in.a[4], i), i2 = ld2(in.a[5], i);
         return mkc<C>((R)((double)g0.x * i0.x + (double)g1.x * i1.x + (double)g2.x * i2.x),
                       (R)((double)g0.y * i0.y + (double)g1.y * i1.y + (double)g2.y * i2.y));
+      });
+      break;
+    case XIN_ABS2:
+      fill([&](size_t i) {
+        const double2 z0 = in.z[i], z1 = in.z[i + 1];
+        return mkc<C>((R)(z0.x * z0.x + z0.y * z0.y), (R)(z1.x * z1.x + z1.y * z1.y));
       });
       break;
     default:

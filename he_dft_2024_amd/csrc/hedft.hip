@@ -29,7 +29,9 @@ enum {
   S_MUOLD = 4,
   S_ERRIMP = 5,
   S_ERRVIMP = 6,
+  S_ERRPCX = 7,  // sum|pcx| of a quantum impurity
   S_ENERGY = 8,  // 8 partial energy sums
+  S_ENERGYQ = 16,  // 2 quantum-impurity energy sums
   NSCAL = 32
 };
 // ---- impurity block layout (doubles) -------------------------------------------
@@ -64,6 +66,7 @@ ImpDev make_impdev(hedft_ctx* c) {
 static size_t partials_region(hedft_ctx* c) { return c->partials_cap / 8; }
 double* force_partials(hedft_ctx* c) { return c->partials + partials_region(c); }
 static double* gen_partials(hedft_ctx* c) { return c->partials; }
+static double* x_partials(hedft_ctx* c) { return c->partials + 2 * partials_region(c); }  // psix corrector
 
 // ---- reductions -------------------------------------------------------------------
 template <int NR>
@@ -138,7 +141,9 @@ struct Lam {
 };
 
 // ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
-// k = timec*((lap)*h2o2m4 - pot4*psi) - i*uimp*psi   (steprk.f90:67, steppc.f90:432,499)
+// k = timec*((lap)*h2o2m4 - (pot4-mu4)*psi) - i*uimp*psi   (steprk.f90:67,
+// steppc.f90:432,499; mu4 and no uimp in Dynamics_quantum_impurities/steprk.f90:66,
+// steppc.f90:48,94; mu4 = 0 for classical impurities)
 template <typename R>
 struct HpsiC {
   const double2* lapxy;
@@ -146,12 +151,12 @@ struct HpsiC {
   const R* pot4;
   Lam lam;
   ImpDev im;
-  double h2o2m4;
+  double h2o2m4, mu4;
   __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2& ps) const {
     const double2 l0 = lapxy[i];
     const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
     ps = psi[i];
-    const double v = (double)pot4[i];
+    const double v = (double)pot4[i] - mu4;
     const double2 t = make_double2(lap.x * h2o2m4 - v * ps.x, lap.y * h2o2m4 - v * ps.y);
     const double lr = lam.at(ix, iy, iz);
     // (lr + i) * t
@@ -162,18 +167,39 @@ struct HpsiC {
   }
 };
 
+// quantum impurity: k = timec*((lap psix)*h2o2mimp - upotx*psix)
+// (Dynamics_quantum_impurities/steprk.f90:101, steppc.f90:66,114)
+template <typename R>
+struct HpsiX {
+  const double2* lapxy;
+  const double2* psi;
+  const R* upot;
+  Lam lam;
+  double h2o2m;
+  __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2& ps) const {
+    const double2 l0 = lapxy[i];
+    const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
+    ps = psi[i];
+    const double v = (double)upot[i];
+    const double2 t = make_double2(lap.x * h2o2m - v * ps.x, lap.y * h2o2m - v * ps.y);
+    const double lr = lam.at(ix, iy, iz);
+    return make_double2(lr * t.x - t.y, lr * t.y + t.x);
+  }
+};
+
 // Outputs per sliding window in the fused stepper epilogues (each output
 // touches 6-8 complex arrays, so fewer outputs are kept in flight).
 #ifndef HEDFT_KB_STEP
 #define HEDFT_KB_STEP 1
 #endif
 
-template <typename R>
+// H: HpsiC (helium) or HpsiX (quantum impurity); den (|psi|^2, R storage) may be null
+template <typename R, typename H = HpsiC<R>>
 struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
-  HpsiC<R> h;
+  H h;
   double2 *psi, *q, *po0, *po1, *po2, *ho0, *ho1;
   R* den;
   double a, b, cc, dt;
@@ -193,16 +219,16 @@ struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
     }
     const double2 np = make_double2(ps.x + dt * s1.x, ps.y + dt * s1.y);
     psi[i] = np;
-    den[i] = (R)(np.x * np.x + np.y * np.y);
+    if (den) den[i] = (R)(np.x * np.x + np.y * np.y);
   }
 };
 
-template <typename R>
+template <typename R, typename H = HpsiC<R>>
 struct OutPred {  // predictor + modifier (steppc.f90:432-442)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
-  HpsiC<R> h;
+  H h;
   double2 *psi, *pc, *po3, *ho1, *ho2;
   R* den;
   double dt;
@@ -219,16 +245,16 @@ struct OutPred {  // predictor + modifier (steppc.f90:432-442)
     psi[i] = np;
     pc[i] = s1;
     ho2[i] = k4;
-    den[i] = (R)(np.x * np.x + np.y * np.y);
+    if (den) den[i] = (R)(np.x * np.x + np.y * np.y);
   }
 };
 
-template <typename R>
+template <typename R, typename H = HpsiC<R>>
 struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
   static constexpr int NR = 1;
   static constexpr int KB = HEDFT_KB_STEP;
   double* partials;
-  HpsiC<R> h;
+  H h;
   double2 *psi, *pc, *po3, *po2, *ho1, *ho2;
   R* den;
   double dt;
@@ -244,7 +270,7 @@ struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
     pc[i] = p;
     const double2 np = make_double2(s5.x + C9 * p.x, s5.y + C9 * p.y);
     psi[i] = np;
-    den[i] = (R)(np.x * np.x + np.y * np.y);
+    if (den) den[i] = (R)(np.x * np.x + np.y * np.y);
     red[0] += sqrt(p.x * p.x + p.y * p.y);
   }
 };
@@ -577,6 +603,32 @@ __global__ void k_convert(long long n, const A* in, B* out) {
        i += (long long)gridDim.x * blockDim.x)
     out[i] = (B)in[i];
 }
+// denx = psix*conjg(psix) (Dynamics_quantum_impurities/steprk.f90:110)
+__global__ void k_abs2(long long n, const double2* z, double* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 v = z[i];
+    out[i] = v.x * v.x + v.y * v.y;
+  }
+}
+// sums of |psix|^2 and |psix|^2 (x,y,z) (DFT4He3d.f90:683-695 of the quantum driver)
+__global__ void __launch_bounds__(256) k_centroid(Dims d, const double2* z, double x0, double y0, double z0,
+                                                  double hx, double hy, double hz, double* partials) {
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % d.nx);
+    const long long r = i / d.nx;
+    const int iy = (int)(r % d.ny), iz = (int)(r / d.ny);
+    const double2 v = z[i];
+    const double w = v.x * v.x + v.y * v.y;
+    red[0] += w;
+    red[1] += w * (x0 + hx * ix);
+    red[2] += w * (y0 + hy * iy);
+    red[3] += w * (z0 + hz * iz);
+  }
+  block_reduce_store<4>(red, partials);
+}
 // statics start: den *= n4/(sum den dxyz), psi = sqrt(den) (BCN4HeDFT.f90:730-734)
 __global__ void k_sum1(long long n, const double* a, double* partials) {
   double red[1] = {0.0};
@@ -810,14 +862,14 @@ static XOutT<R> xo_pot4(hedft_ctx* c) {
   o.delj4 = fld<R>(c->delj4);
   o.dencg = fld<R>(c->dencg);
   o.den = fld<R>(c->den);
-  o.uext = fld<R>(c->uext);
+  o.ext = fld<R>(pot4_ext(c));
   o.a0 = c->p.cp4 / 2.0;
   o.a1 = c->p.cpp4 / 3.0;
   o.csolid = c->p.csolid;
   o.beta = c->p.beta_solid;
   o.den_m = c->p.den_m;
   o.lsolid = c->p.lsolid;
-  o.lext = (c->mode == HEDFT_MODE_STATICS) && c->p.lexternal;
+  o.lext = pot4_ext(c) != nullptr;
   return o;
 }
 
@@ -908,6 +960,8 @@ static void flush_side_pre(hedft_ctx* c) {
   const int* a = c->side_args;
   if (c->side_pre & 4)
     finalize<1>(c, gen_partials(c), c->side_nblocks, S_ERRPC, 1.0);
+  if (c->side_pre & 8)
+    finalize<1>(c, x_partials(c), c->side_nblocks, S_ERRPCX, 1.0);
   if (c->side_pre & 1)
     hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
                        a[3], a[4], a[5]);
@@ -917,10 +971,47 @@ static void flush_side_pre(hedft_ctx* c) {
   c->side_pre = 0;
 }
 
+// potx4 = F^-1[F(|psix|^2) uimp_k] of a quantum impurity
+// (Dynamics_quantum_impurities/potenimp.f90:22-27), one forward, one
+// convolution and one inverse of the impurity density
+template <typename R>
+static int potx4_conv(hedft_ctx* c, double* out) {
+  XInT<R> xa{};
+  xa.op = XIN_ABS2;
+  xa.z = (const double2*)c->psix;
+  cplx_t<R>* sx = spc<R>(c->specq[0]);
+  HEDFT_TRY(fwd_xy<R>(c, 1, &xa, &sx));
+  const R* ku = fld<R>(c->uimpk);
+  HEDFT_TRY(zconv<R>(c, 1, &sx, 1, &ku, &sx, 1, nullptr));
+  XOutT<R> xo = xo_store<R>(out);
+  return inv_yx<R>(c, 1, &sx, &xo);
+}
+
+// potenimp of Dynamics_quantum_impurities (potenimp.f90:20-30) on its own:
+// upotx = F^-1[F(den) uimp_k], potx4
+template <typename R>
+static int potenimp_q(hedft_ctx* c) {
+  XInT<R> xi{};
+  xi.op = XIN_COPY;
+  xi.a[0] = fld<R>(c->den);
+  cplx_t<R>* s0 = spc<R>(c->spec[0]);
+  HEDFT_TRY(fwd_xy<R>(c, 1, &xi, &s0));
+  const R* ku = fld<R>(c->uimpk);
+  cplx_t<R>* s1 = spc<R>(c->specq[1]);
+  HEDFT_TRY(zconv<R>(c, 1, &s0, 1, &ku, &s1, 1, nullptr));
+  XOutT<R> xo = xo_store<R>(c->upotx);
+  HEDFT_TRY(inv_yx<R>(c, 1, &s1, &xo));
+  return potx4_conv<R>(c, c->potx4);
+}
+
 // Dynamics_TDDFT/poten.f90:124-221 (+ forceimp partials fused into derden).
 // grad(den) and the impurity force run on the side branch while the main
 // branch convolves den; with_lap also computes the x+y Laplacian of psi there
 // for the step that follows (psi is not changed by the mean field).
+// With a quantum impurity this is potenimp + poten of
+// Dynamics_quantum_impurities (potenimp.f90:20-30, poten.f90:26-138): upotx =
+// F^-1[F(den) uimp_k] is one more output of the den convolution, potx4 is
+// convolved on the side branch and added in the pot4 assembly.
 template <typename R>
 static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   HEDFT_TRY(fork(c));
@@ -929,29 +1020,43 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     flush_side_pre(c);  // impurity PC and errHe of the step just queued
     HEDFT_TRY(derden_force(c, want_force, want_force));
     if (with_lap) HEDFT_TRY(lap_xy(c, c->psi, 1));
+    if (c->quantum) {
+      if (with_lap) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+      HEDFT_TRY(potx4_conv<R>(c, c->potx4));
+    }
   }
   XInT<R> xi{};
   xi.op = XIN_COPY;
   xi.a[0] = fld<R>(c->den);
   cplx_t<R>* s0 = spc<R>(c->spec[0]);
   HEDFT_TRY(fwd_xy<R>(c, 1, &xi, &s0));
+  // den convolutions: dencg, delj4 [, denalf] [, upotx]
+  const R* kk[4] = {fld<R>(c->wcgk), fld<R>(c->fvlj4), nullptr, nullptr};
+  cplx_t<R>* so[4] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), nullptr, nullptr};
+  XOutT<R> xo[4] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4), {}, {}};
+  int nk = 2;
   if (c->p.core_otc) {
-    const R* kk[3] = {fld<R>(c->wcgk), fld<R>(c->fvlj4), fld<R>(c->kalfs)};
-    cplx_t<R>* so[3] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), spc<R>(c->spec[3])};
-    // one workgroup per (column tile, kernel): the forward z transform is
-    // repeated per kernel, but three times as many workgroups are in flight
-    cplx_t<R>* si[3] = {s0, s0, s0};
-    HEDFT_TRY(c->zconv_fused ? zconv<R>(c, 1, &s0, 3, kk, so, 1, nullptr) : zconv<R>(c, 3, si, 1, kk, so, 1, nullptr));
-    XOutT<R> xo[3] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4), xo_store<R>(c->denalf)};
-    HEDFT_TRY(inv_yx<R>(c, 3, so, xo));
-  } else {
-    const R* kk[2] = {fld<R>(c->wcgk), fld<R>(c->fvlj4)};
-    cplx_t<R>* so[2] = {spc<R>(c->spec[1]), spc<R>(c->spec[2])};
-    cplx_t<R>* si[2] = {s0, s0};
-    HEDFT_TRY(c->zconv_fused ? zconv<R>(c, 1, &s0, 2, kk, so, 1, nullptr) : zconv<R>(c, 2, si, 1, kk, so, 1, nullptr));
-    XOutT<R> xo[2] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4)};
-    HEDFT_TRY(inv_yx<R>(c, 2, so, xo));
+    kk[nk] = fld<R>(c->kalfs);
+    so[nk] = spc<R>(c->spec[3]);
+    xo[nk++] = xo_store<R>(c->denalf);
   }
+  if (c->quantum) {
+    kk[nk] = fld<R>(c->uimpk);
+    so[nk] = spc<R>(c->specq[1]);
+    xo[nk++] = xo_store<R>(c->upotx);
+  }
+  if (c->zconv_fused) {
+    // one workgroup per column tile applies up to three kernels
+    const int n1 = std::min(nk, 3);
+    HEDFT_TRY(zconv<R>(c, 1, &s0, n1, kk, so, 1, nullptr));
+    if (nk > n1) HEDFT_TRY(zconv<R>(c, 1, &s0, nk - n1, kk + n1, so + n1, 1, nullptr));
+  } else {
+    // one workgroup per (column tile, kernel): the forward z transform is
+    // repeated per kernel, but nk times as many workgroups are in flight
+    cplx_t<R>* si[4] = {s0, s0, s0, s0};
+    HEDFT_TRY(zconv<R>(c, nk, si, 1, kk, so, 1, nullptr));
+  }
+  HEDFT_TRY(inv_yx<R>(c, nk, so, xo));
   HEDFT_TRY(join(c));
   if (with_lap) c->lap_fresh = 1;
   HEDFT_TRY(stage_b<R>(c));
@@ -1029,7 +1134,27 @@ static HpsiC<R> make_hpsic(hedft_ctx* c) {
   h.lam = make_lam(c);
   h.im = make_impdev(c);
   h.h2o2m4 = c->p.h2o2m4;
+  h.mu4 = c->quantum ? c->mu4 : 0.0;
   return h;
+}
+
+template <typename R>
+static HpsiX<R> make_hpsix(hedft_ctx* c) {
+  HpsiX<R> h;
+  h.lapxy = (const double2*)c->lapx;
+  h.psi = (const double2*)c->psix;
+  h.upot = fld<R>(c->upotx);
+  h.lam = make_lam(c);
+  h.h2o2m = c->h2o2mimp;
+  return h;
+}
+
+// x+y Laplacians the fused step kernels start from (psi, and psix with a quantum impurity)
+static int prepare_lap(hedft_ctx* c) {
+  if (c->lap_fresh) return HEDFT_OK;
+  HEDFT_TRY(lap_xy(c, c->psi, 1));
+  if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+  return HEDFT_OK;
 }
 
 static inline double2* cz(void* p) { return (double2*)p; }
@@ -1042,11 +1167,34 @@ static void kt_mark(hedft_ctx* c, int slot, int end) {
   if (end) c->kt_n[slot]++;
 }
 
-// STEPRK (Dynamics_TDDFT/steprk.f90:1-144)
+// RK-Gill stage of a quantum impurity (Dynamics_quantum_impurities/steprk.f90:96-110)
+template <typename R>
+static void steprk_x(hedft_ctx* c, double dt, int jrun) {
+  OutRK<R, HpsiX<R>> o;
+  o.partials = nullptr;
+  o.h = make_hpsix<R>(c);
+  o.psi = cz(c->psix);
+  o.q = cz(c->qx);
+  o.po0 = cz(c->psixold[0]);
+  o.po1 = cz(c->psixold[1]);
+  o.po2 = cz(c->psixold[2]);
+  o.ho0 = cz(c->hpsixold[0]);
+  o.ho1 = cz(c->hpsixold[1]);
+  o.den = nullptr;
+  o.a = RK_A[jrun];
+  o.b = RK_B[jrun];
+  o.cc = RK_C[jrun];
+  o.dt = dt;
+  o.first = (jrun == 0);
+  launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psix)}, o);
+}
+
+// STEPRK (Dynamics_TDDFT/steprk.f90:1-144; with a quantum impurity
+// Dynamics_quantum_impurities/steprk.f90:53-124: psix takes the same stage)
 template <typename R>
 static int steprk_t(hedft_ctx* c, double dt) {
   for (int jrun = 0; jrun < 4; ++jrun) {
-    if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
+    HEDFT_TRY(prepare_lap(c));
     OutRK<R> o;
     o.partials = nullptr;
     o.h = make_hpsic<R>(c);
@@ -1064,6 +1212,7 @@ static int steprk_t(hedft_ctx* c, double dt) {
     o.dt = dt;
     o.first = (jrun == 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    if (c->quantum) steprk_x<R>(c, dt, jrun);
     c->lap_fresh = 0;
     if (c->nimp > 0)
       hipLaunchKernelGGL(k_imp_rk, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, (int)(jrun == 0), RK_A[jrun],
@@ -1076,10 +1225,13 @@ static int steprk_t(hedft_ctx* c, double dt) {
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 
-// STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC]
+// STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC].
+// A quantum impurity takes the same predictor and corrector
+// (Dynamics_quantum_impurities/steppc.f90:56-78,106-124) with the same slot
+// indices (ioldpx, ioldhx rotate with ioldp, ioldh); sum|pcx| into scal[S_ERRPCX].
 template <typename R>
 static int steppc_t(hedft_ctx* c, double dt) {
-  if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
+  HEDFT_TRY(prepare_lap(c));
   {
     OutPred<R> o;
     o.partials = nullptr;
@@ -1094,6 +1246,19 @@ static int steppc_t(hedft_ctx* c, double dt) {
     kt_mark(c, 0, 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
     kt_mark(c, 0, 1);
+    if (c->quantum) {
+      OutPred<R, HpsiX<R>> ox;
+      ox.partials = nullptr;
+      ox.h = make_hpsix<R>(c);
+      ox.psi = cz(c->psix);
+      ox.pc = cz(c->pcx);
+      ox.po3 = cz(c->psixold[c->ioldp[2]]);
+      ox.ho1 = cz(c->hpsixold[c->ioldh[0]]);
+      ox.ho2 = cz(c->hpsixold[c->ioldh[1]]);
+      ox.den = nullptr;
+      ox.dt = dt;
+      launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psix)}, ox);
+    }
     c->lap_fresh = 0;
     std::swap(c->ioldh[0], c->ioldh[1]);
   }
@@ -1105,7 +1270,7 @@ static int steppc_t(hedft_ctx* c, double dt) {
     std::swap(c->iolda[0], c->iolda[1]);
   }
   HEDFT_TRY(poten_tddft(c, 0, 1));
-  if (!c->lap_fresh) HEDFT_TRY(lap_xy(c, c->psi, 1));
+  HEDFT_TRY(prepare_lap(c));
   {
     OutCorr<R> o;
     o.partials = gen_partials(c);
@@ -1121,6 +1286,21 @@ static int steppc_t(hedft_ctx* c, double dt) {
     kt_mark(c, 1, 0);
     launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
     kt_mark(c, 1, 1);
+    if (c->quantum) {
+      OutCorr<R, HpsiX<R>> ox;
+      ox.partials = x_partials(c);
+      ox.h = make_hpsix<R>(c);
+      ox.psi = cz(c->psix);
+      ox.pc = cz(c->pcx);
+      ox.po3 = cz(c->psixold[c->ioldp[2]]);
+      ox.po2 = cz(c->psixold[c->ioldp[1]]);
+      ox.ho1 = cz(c->hpsixold[c->ioldh[0]]);
+      ox.ho2 = cz(c->hpsixold[c->ioldh[1]]);
+      ox.den = nullptr;
+      ox.dt = dt;
+      launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psix)}, ox);
+      c->side_pre |= 8;  // errpsix reduction, with errHe's
+    }
     c->lap_fresh = 0;
     c->side_pre |= 4;  // errHe reduction, at the start of the next side branch
     c->side_nblocks = pencil_geom<double2>(c->d, 2).nblocks;
@@ -1271,6 +1451,37 @@ static int energy_sums(hedft_ctx* c, double* s) {
   hipLaunchKernelGGL(k_finalize<8>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
                      (int)pencil_geom<V>(c->d, 2).nblocks, c->scal + S_ENERGY, 1.0);
   return read_scal(c, S_ENERGY, 8, s);
+}
+
+// quantum impurity terms (Dynamics_quantum_impurities/energy.f90:74-96):
+// sum |grad psix|^2 and sum F^-1[F(|psix|^2) uimp_k] den
+template <typename R>
+struct OutEnergyX {
+  static constexpr int NR = 2;
+  double* partials;
+  const double* acc;
+  const R *conv, *den;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double2 v, double* red) const {
+    red[0] += acc[i] + vsq<double2>(v);
+    red[1] += (double)conv[i] * (double)den[i];
+  }
+};
+
+template <typename R>
+static int energy_sums_x(hedft_ctx* c, double* s) {
+  HEDFT_TRY(potx4_conv<R>(c, c->work1));
+  const double2* px = (const double2*)c->psix;
+  launch_pencil_xy<double2, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<double2>{px},
+                                  OutSqXY<double2>{nullptr, (double*)c->workc1});
+  OutEnergyX<R> o;
+  o.partials = gen_partials(c);
+  o.acc = (const double*)c->workc1;
+  o.conv = fld<R>(c->work1);
+  o.den = fld<R>(c->den);
+  launch_pencil<double2, 2>(c->stream, c->d, c->coef[0][2], InPlain<double2>{px}, o);
+  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
+                     (int)pencil_geom<double2>(c->d, 2).nblocks, c->scal + S_ENERGYQ, 1.0);
+  return read_scal(c, S_ENERGYQ, 2, s);
 }
 
 }  // namespace hedft
@@ -1482,7 +1693,9 @@ int hedft_destroy(hedft_ctx* c) {
                   c->timec_r, c->hpsi_r, c->psi1s, c->psi2s, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
                   c->hpsiold[1], c->pc, c->q, c->lap, c->workc1, c->fden, c->spec[0], c->spec[1], c->spec[2],
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
-                  c->imp, c->lstop, c->potion};
+                  c->imp, c->lstop, c->potion, c->psix, c->psixold[0], c->psixold[1], c->psixold[2],
+                  c->hpsixold[0], c->hpsixold[1], c->pcx, c->qx, c->lapx, c->potx4, c->upotx, c->uimpk,
+                  c->specq[0], c->specq[1]};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->host_scal) hipHostFree(c->host_scal);
@@ -1541,12 +1754,28 @@ static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* comput
     case HEDFT_WORK2: *ptr = c->work2; *bytes = rN; break;
     case HEDFT_WORKC1: *ptr = c->workc1; *bytes = cN; break;
     case HEDFT_WORKK1: *ptr = c->workk1; *bytes = hN; break;
+    case HEDFT_PSIX: case HEDFT_POTX4: case HEDFT_UPOTX: case HEDFT_DENX: case HEDFT_UIMPK:
+    case HEDFT_PSIXOLD1: case HEDFT_PSIXOLD2: case HEDFT_PSIXOLD3: case HEDFT_HPSIXOLD1:
+    case HEDFT_HPSIXOLD2: case HEDFT_PCX: case HEDFT_QX:
+      if (!c->quantum) return HEDFT_ESTATE;
+      switch (f) {
+        case HEDFT_PSIX: *ptr = c->psix; *bytes = cN; break;
+        case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
+        case HEDFT_UPOTX: *ptr = c->upotx; *bytes = rN; break;
+        case HEDFT_DENX: *ptr = c->workc1; *bytes = rN; *computed = 1; break;
+        case HEDFT_UIMPK: *ptr = c->uimpk; *bytes = kN; break;
+        case HEDFT_PCX: *ptr = c->pcx; *bytes = cN; break;
+        case HEDFT_QX: *ptr = c->qx; *bytes = cN; break;
+        case HEDFT_HPSIXOLD1: case HEDFT_HPSIXOLD2: *ptr = c->hpsixold[f - HEDFT_HPSIXOLD1]; *bytes = cN; break;
+        default: *ptr = c->psixold[f - HEDFT_PSIXOLD1]; *bytes = cN; break;
+      }
+      break;
     default: return HEDFT_EINVAL;
   }
   if (*computed && f != HEDFT_TIMEC) *ptr = c->workc1;  // evaluated into double scratch
   if (low) {
     const bool psi_side = (f == HEDFT_PSI || (f >= HEDFT_PSIOLD1 && f <= HEDFT_Q) || f == HEDFT_WORKC1 ||
-                           (f == HEDFT_TIMEC && td));
+                           (f == HEDFT_TIMEC && td) || f == HEDFT_PSIX || (f >= HEDFT_PSIXOLD1 && f <= HEDFT_QX));
     *low = c->mixed && !*computed && !psi_side;
   }
   return HEDFT_OK;
@@ -1595,6 +1824,8 @@ int hedft_download(hedft_ctx* c, int32_t field, void* host, size_t bytes) {
   if (comp) {
     double* w = (double*)c->workc1;
     if (field == HEDFT_UIMP) hipLaunchKernelGGL(k_eval_uimp, gr, bl, 0, c->stream, c->d, make_impdev(c), w);
+    else if (field == HEDFT_DENX)
+      hipLaunchKernelGGL(k_abs2, gr, bl, 0, c->stream, c->d.n, (const double2*)c->psix, w);
     else if (field == HEDFT_FALFS && c->mixed)
       hipLaunchKernelGGL(k_eval_falfs<float>, gr, bl, 0, c->stream, c->d.n, (const float*)c->denalf,
                          c->p.den0s, w);
@@ -1718,6 +1949,13 @@ int hedft_energy(hedft_ctx* c, hedft_energies* e) {
       e->ekinx += 0.5 * c->mass[k] * v2;
     }
     e->eHeX = s[6] * dv;
+    if (c->quantum) {
+      double sx[2];
+      if (c->mixed) HEDFT_TRY(energy_sums_x<float>(c, sx));
+      else HEDFT_TRY(energy_sums_x<double>(c, sx));
+      e->ekinx = c->h2o2mimp * sx[0] * dv;
+      e->eHeX = sx[1] * dv;
+    }
     e->eimpu = e->ekinx + e->eHeX;
   } else {
     e->eext = p.lexternal ? s[5] * dv : 0.0;
@@ -1761,7 +1999,7 @@ static int read_pc_errors(hedft_ctx* c, double* errHe, double* errimp, double* e
   double s[8];
   HEDFT_TRY(read_scal(c, 0, 8, s));
   if (errHe) *errHe = (9.0 / 121.0) * s[S_ERRPC] / (double)c->d.n;
-  if (errimp) *errimp = c->nimp ? s[S_ERRIMP] : 0.0;
+  if (errimp) *errimp = c->quantum ? (9.0 / 121.0) * s[S_ERRPCX] / (double)c->d.n : c->nimp ? s[S_ERRIMP] : 0.0;
   if (errvimp) *errvimp = c->nimp ? s[S_ERRVIMP] : 0.0;
   return HEDFT_OK;
 }
@@ -1778,6 +2016,7 @@ int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, dou
 int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delinter, double rmaxinterpol,
                          const double* potion, const double* mass) {
   if (!c || nimp < 0 || nimp > HEDFT_MAX_IMP || (nimp > 0 && (!potion || !mass || npot < 2))) return HEDFT_EINVAL;
+  if (nimp > 0 && c->quantum) return HEDFT_EINVAL;  // one impurity model per context
   HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old table
   if (c->potion) hipFree(c->potion);
   c->potion = nullptr;
@@ -1840,6 +2079,7 @@ __global__ void k_check_bounds(ImpDev im, Dims d) {
 int hedft_potenimp(hedft_ctx* c) {
   if (!c) return HEDFT_EINVAL;
   if (c->nimp > 0) hipLaunchKernelGGL(k_check_bounds, dim3(1), dim3(64), 0, c->stream, make_impdev(c), c->d);
+  if (c->quantum) HEDFT_TRY(c->mixed ? potenimp_q<float>(c) : potenimp_q<double>(c));
   return done(c);
 }
 
@@ -1883,6 +2123,58 @@ int hedft_update_aimp(hedft_ctx* c) {
   if (!c) return HEDFT_EINVAL;
   if (c->nimp > 0) hipLaunchKernelGGL(k_imp_aimp, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp);
   return done(c);
+}
+
+// potenimpini's uimp_k(:,:,:,1) and the mass of one quantum impurity
+// (Dynamics_quantum_impurities/potenimpini.f90:19-29, steprk.f90:97)
+int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u, double mu4) {
+  if (!c || !uimp_k || c->mode != HEDFT_MODE_TDDFT || c->nimp > 0 || !(m_imp_u > 0.0)) return HEDFT_EINVAL;
+  if (bytes != sizeof(double) * (size_t)c->d.nh) return HEDFT_EINVAL;
+  hipSetDevice(c->device);
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old kernel
+  const size_t fw = c->mixed ? sizeof(float) : sizeof(double);
+  const size_t cN = sizeof(double2) * c->d.n, rN = fw * c->d.n, kN = fw * c->d.nh, hN = 2 * fw * c->d.nh;
+  int rc = HEDFT_OK;
+  void** cplx[] = {&c->psix, &c->psixold[0], &c->psixold[1], &c->psixold[2], &c->hpsixold[0], &c->hpsixold[1],
+                   &c->pcx, &c->qx, &c->lapx};
+  for (void** f : cplx)
+    if (!*f) rc = rc ? rc : alloc(c, f, cN);
+  if (!c->potx4) rc = rc ? rc : alloc(c, (void**)&c->potx4, rN);
+  if (!c->upotx) rc = rc ? rc : alloc(c, (void**)&c->upotx, rN);
+  if (!c->uimpk) rc = rc ? rc : alloc(c, (void**)&c->uimpk, kN);
+  for (int k = 0; k < 2; ++k)
+    if (!c->specq[k]) rc = rc ? rc : alloc(c, (void**)&c->specq[k], hN);
+  if (rc != HEDFT_OK) return rc;
+  if (c->mixed) {  // FP64 table -> double scratch -> float kernel
+    HEDFT_CHECK(hipMemcpyAsync(c->workc1, uimp_k, bytes, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, c->stream, (long long)c->d.nh,
+                       (const double*)c->workc1, (float*)c->uimpk);
+  } else {
+    HEDFT_CHECK(hipMemcpyAsync(c->uimpk, uimp_k, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  c->m_imp_u = m_imp_u;
+  c->h2o2mimp = c->p.h2o2m4 * 4.002603 / m_imp_u;
+  c->mu4 = mu4;
+  c->quantum = 1;
+  c->graph_valid = 0;
+  c->lap_fresh = 0;
+  return done(c);
+}
+
+int hedft_quantum_centroid(hedft_ctx* c, double* norm, double* rcm) {
+  if (!c || !c->quantum) return HEDFT_EINVAL;
+  hipLaunchKernelGGL(k_centroid, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, (const double2*)c->psix,
+                     -c->g.xmax, -c->g.ymax, -c->g.zmax, c->hx, c->hy, c->hz, gen_partials(c));
+  hipLaunchKernelGGL(k_finalize<4>, dim3(1), dim3(256), 0, c->stream, gen_partials(c), EW_BLOCKS,
+                     c->scal + S_ENERGYQ, 1.0);
+  double s[4];
+  HEDFT_TRY(read_scal(c, S_ENERGYQ, 4, s));
+  const double nrm = s[0] * c->dxyz;
+  if (norm) *norm = nrm;
+  if (rcm)
+    for (int a = 0; a < 3; ++a) rcm[a] = s[1 + a] * c->dxyz / nrm;
+  return HEDFT_OK;
 }
 
 int hedft_statics_init(hedft_ctx* c, double n4, double* mu4) {
@@ -1957,6 +2249,8 @@ int hedft_tddft_init(hedft_ctx* c) {
   // qr = qv = rimpold = vimpold = aimpold = 0 (DFT4He3d.f90:625-629)
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_QR, 0, sizeof(double) * 24, c->stream));
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_ROLD, 0, sizeof(double) * (IMP_MASS - IMP_ROLD), c->stream));
+  // qx = 0 (Dynamics_quantum_impurities/DFT4He3d.f90:617)
+  if (c->quantum) HEDFT_CHECK(hipMemsetAsync(c->qx, 0, sizeof(double2) * c->d.n, c->stream));
   HEDFT_TRY(hedft_potenimp(c));
   HEDFT_TRY(tddft_after_step(c));
   c->rk_done = 0;
@@ -2001,10 +2295,8 @@ static void slot_state(const hedft_ctx* c, int* s) {
 static int ensure_graph(hedft_ctx* c, double deltat) {
   // the captured iterations start from a prepared x+y Laplacian of psi (as every
   // iteration leaves it), so prepare it now if an upload invalidated it
-  if (!c->lap_fresh) {
-    HEDFT_TRY(lap_xy(c, c->psi, 1));
-    c->lap_fresh = 1;
-  }
+  HEDFT_TRY(prepare_lap(c));
+  c->lap_fresh = 1;
   int slots[13];
   slot_state(c, slots);
   if (c->graph_valid && c->graph_dt == deltat && std::memcmp(slots, c->graph_slots, sizeof slots) == 0)
@@ -2155,7 +2447,7 @@ int hedft_tddft_probe_pc(hedft_ctx* c, double deltat, int32_t nsteps, double* av
     c->kt_cap = nsteps;
     c->kt_n[0] = c->kt_n[1] = 0;
     c->kt_on = 1;
-    if (!c->lap_fresh) rc = lap_xy(c, c->psi, 1);
+    rc = prepare_lap(c);
     if (rc == HEDFT_OK) c->lap_fresh = 1;
     for (int s = 0; s < nsteps && rc == HEDFT_OK; ++s) {
       rc = steppc_impl(c, deltat);

@@ -184,13 +184,14 @@ int derden_force(hedft_ctx* c, int want_force, int update_aimp) {
 
 // z pass of the alpha_s divergence with the mean-field assembly:
 // ualphas = (conv/den0s + sum_a d_a(f int_a)) h2o2m4 alphas   (term_alfa.f90:80-98)
-// pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ uext] + ualphas
-// (He_statics/poten.f90:277-296, Dynamics_TDDFT/poten.f90:194-211)
+// pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ uext | potx4] + ualphas
+// (He_statics/poten.f90:277-296, Dynamics_TDDFT/poten.f90:194-211,
+// Dynamics_quantum_impurities/poten.f90:116-138)
 template <typename R>
 struct OutDivPot4 {
   static constexpr int NR = 0;
   double* partials;
-  const R *acc, *conv, *delj4, *dencg, *corr, *uext;
+  const R *acc, *conv, *delj4, *dencg, *corr, *ext;
   R *ualphas, *pot4;
   double den0s, halpha, a0, a1;
   int lext;
@@ -199,7 +200,7 @@ struct OutDivPot4 {
     ualphas[i] = (R)ua;
     const double cg = dencg[i];
     double p = (double)delj4[i] + cg * cg * (a0 + a1 * cg) + (double)corr[i];
-    if (lext) p += uext[i];
+    if (lext) p += ext[i];
     pot4[i] = (R)(p + ua);
   }
 };
@@ -229,14 +230,14 @@ static int alpha_div_z_pot4_t(hedft_ctx* c) {
   op.delj4 = F(c->delj4);
   op.dencg = F(c->dencg);
   op.corr = F(c->sto1);
-  op.uext = F(c->uext);
+  op.ext = F(pot4_ext(c));
   op.ualphas = F(c->ualphas);
   op.pot4 = F(c->pot4);
   op.den0s = den0s;
   op.halpha = c->p.h2o2m4 * c->p.alphas;
   op.a0 = c->p.cp4 / 2.0;
   op.a1 = c->p.cpp4 / 3.0;
-  op.lext = (c->mode == HEDFT_MODE_STATICS) && c->p.lexternal;
+  op.lext = pot4_ext(c) != nullptr;
   launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], iz, op);
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
@@ -245,14 +246,15 @@ int alpha_div_z_pot4(hedft_ctx* c) {
   return c->mixed ? alpha_div_z_pot4_t<float>(c) : alpha_div_z_pot4_t<double>(c);
 }
 
-int lap_xy(hedft_ctx* c, const void* psi, int is_complex) {
+int lap_xy(hedft_ctx* c, const void* psi, int is_complex, void* out) {
+  if (!out) out = c->lap;
   if (is_complex)
     launch_pencil_xy<double2, true>(c->stream, c->d, c->coef[1][0], c->coef[1][1],
                                     InPlain<double2>{(const double2*)psi},
-                                    OutSumXY<double2>{(double2*)c->lap, nullptr});
+                                    OutSumXY<double2>{(double2*)out, nullptr});
   else
     launch_pencil_xy<double, true>(c->stream, c->d, c->coef[1][0], c->coef[1][1],
-                                   InPlain<double>{(const double*)psi}, OutSumXY<double>{(double*)c->lap, nullptr});
+                                   InPlain<double>{(const double*)psi}, OutSumXY<double>{(double*)out, nullptr});
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 

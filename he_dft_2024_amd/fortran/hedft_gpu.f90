@@ -25,6 +25,11 @@ module hedft_gpu
        HEDFT_Q = 23, HEDFT_FDEN = 24, HEDFT_FVLJ4 = 25, HEDFT_WCGK = 26, &
        HEDFT_KALFS = 27, HEDFT_WORK1 = 28, HEDFT_WORK2 = 29, HEDFT_WORKC1 = 30, &
        HEDFT_WORKK1 = 31
+  ! quantum impurity (module quantal_imp of Dynamics_quantum_impurities)
+  integer(c_int32_t), parameter :: HEDFT_PSIX = 32, HEDFT_POTX4 = 33, HEDFT_UPOTX = 34, &
+       HEDFT_DENX = 35, HEDFT_UIMPK = 36, HEDFT_PSIXOLD1 = 37, HEDFT_PSIXOLD2 = 38, &
+       HEDFT_PSIXOLD3 = 39, HEDFT_HPSIXOLD1 = 40, HEDFT_HPSIXOLD2 = 41, HEDFT_PCX = 42, &
+       HEDFT_QX = 43
 
   type, bind(C) :: hedft_grid_t
     integer(c_int32_t) :: nx, ny, nz, pad_
@@ -195,6 +200,20 @@ module hedft_gpu
     integer(c_int) function hedft_update_aimp(ctx) bind(C, name="hedft_update_aimp")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    ! uimp_k(:,:,:,1) of potenimpini (real, (nx/2+1,ny,nz)), m_imp_u(1), mu4
+    integer(c_int) function hedft_set_quantum_impurity(ctx, uimp_k, nbytes, m_imp_u, mu4) &
+        bind(C, name="hedft_set_quantum_impurity")
+      import :: c_int, c_ptr, c_double, c_size_t
+      type(c_ptr), value :: ctx
+      type(*), intent(in) :: uimp_k(*)
+      integer(c_size_t), value :: nbytes
+      real(c_double), value :: m_imp_u, mu4
+    end function
+    integer(c_int) function hedft_quantum_centroid(ctx, norm, rcm) bind(C, name="hedft_quantum_centroid")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: norm, rcm(3)
     end function
     integer(c_int) function hedft_statics_init(ctx, n4, mu4) bind(C, name="hedft_statics_init")
       import :: c_int, c_ptr, c_double

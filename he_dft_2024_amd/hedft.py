@@ -24,6 +24,8 @@ FIELDS = [
     "UALPHAS", "DXDEN", "DYDEN", "DZDEN", "UIMP", "UEXT", "TIMEC", "PSIOLD1", "PSIOLD2",
     "PSIOLD3", "HPSIOLD1", "HPSIOLD2", "PC", "Q", "FDEN", "FVLJ4", "WCGK", "KALFS", "WORK1",
     "WORK2", "WORKC1", "WORKK1",
+    "PSIX", "POTX4", "UPOTX", "DENX", "UIMPK", "PSIXOLD1", "PSIXOLD2", "PSIXOLD3", "HPSIXOLD1",
+    "HPSIXOLD2", "PCX", "QX",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 
@@ -147,6 +149,8 @@ def lib():
                                               vp, ip, vp, C.c_size_t]
         L.hedft_write_wavefunction.argtypes = [C.c_char_p, C.POINTER(DenFile), ip, vp, vp, vp, C.c_size_t,
                                                C.c_char_p]
+        L.hedft_set_quantum_impurity.argtypes = [vp, vp, C.c_size_t, d, d]
+        L.hedft_quantum_centroid.argtypes = [vp, dp, dp]
         L.hedft_job_bytes.restype = C.c_size_t
         L.hedft_job_bytes.argtypes = [C.POINTER(Job)]
         L.hedft_serve.argtypes = [C.POINTER(Job), ip, C.POINTER(ServeOpts), C.POINTER(JobResult),
@@ -296,6 +300,19 @@ class Droplet:
             return _ptr(flat)
         check(lib().hedft_set_impurity_state(self._c, conv(rimp), conv(vimp), conv(aimp)),
               "set_impurity_state")
+
+    def set_quantum_impurity(self, uimp_k, m_imp_u, mu4=0.0):
+        """One quantum impurity: uimp_k on the half spectrum (nz, ny, nx//2+1), mass in u."""
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(uimp_k, dtype=np.float64), self.hshape))
+        check(lib().hedft_set_quantum_impurity(self._c, _ptr(k), k.nbytes, float(m_imp_u), float(mu4)),
+              "set_quantum_impurity")
+
+    def quantum_centroid(self):
+        """(norm, centre) of |psix|^2."""
+        nrm = C.c_double(0)
+        r = (C.c_double * 3)()
+        check(lib().hedft_quantum_centroid(self._c, C.byref(nrm), r), "quantum_centroid")
+        return nrm.value, np.array(list(r))
 
     def impurity_state(self):
         m = self.nimp * 3

@@ -75,6 +75,29 @@ def tddft_setup(drop, n, xmax, n4, seed=0, mass_amu=108.0, r0=(0.0, 0.0, -20.0),
     return den
 
 
+def gaussian_packet(nx, ny, nz, xmax, ymax, zmax, r0=(0.0, 0.0, 0.0), width=1.5, k=(0.0, 0.0, 0.0)):
+    """Normalised wave packet exp(-|r-r0|^2/(2 width^2) + i k.r): sum |psix|^2 dxyz = 1
+    (the reference reads psix from denX files, readenc.f90)."""
+    x, y, z = grid_axes(nx, xmax), grid_axes(ny, ymax), grid_axes(nz, zmax)
+    X, Y, Z = x[None, None, :], y[None, :, None], z[:, None, None]
+    r2 = (X - r0[0]) ** 2 + (Y - r0[1]) ** 2 + (Z - r0[2]) ** 2
+    psi = np.exp(-r2 / (2.0 * width * width) + 1j * (k[0] * X + k[1] * Y + k[2] * Z))
+    dxyz = (2 * xmax / nx) * (2 * ymax / ny) * (2 * zmax / nz)
+    psi /= np.sqrt((np.abs(psi) ** 2).sum() * dxyz)
+    return np.ascontiguousarray(psi)
+
+
+def quantum_setup(drop, n, xmax, n4, uimp_k, seed=0, m_imp_u=4.002603, mu4=0.0, r0=(0.0, 0.0, 0.0),
+                  width=1.5, k=(0.0, 0.0, 0.0)):
+    """Woods-Saxon droplet plus one quantum impurity (a wave packet) in a context
+    (a Droplet or an oracle.Oracle): den, uimp_k, mass, psix."""
+    den = woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=seed)
+    drop.set("DEN", den)
+    drop.set_quantum_impurity(uimp_k, m_imp_u, mu4)
+    drop.set("PSIX", gaussian_packet(n, n, n, xmax, xmax, xmax, r0, width, k))
+    return den
+
+
 def ensemble(count=48, seed=3, n_lo=200.0, n_hi=2000.0, v_lo=1.0, v_hi=8.0, grid=128, h=0.4, z0=-20.0):
     """Seeded droplet ensemble: N log-uniform in [n_lo, n_hi], impurity speed
     uniform in [v_lo, v_hi] A/ps toward the centre along +z from z0, impact

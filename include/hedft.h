@@ -78,6 +78,20 @@ enum hedft_field {
   HEDFT_WORK2,
   HEDFT_WORKC1,       /* scratch complex N field                             */
   HEDFT_WORKK1,       /* scratch complex half spectrum                       */
+  /* quantum impurity (Dynamics_quantum_impurities, module quantal_imp); these
+   * exist once hedft_set_quantum_impurity has been called                  */
+  HEDFT_PSIX,         /* psix(:,:,:,1)   complex N                           */
+  HEDFT_POTX4,        /* potx4           real  N                             */
+  HEDFT_UPOTX,        /* upotx(:,:,:,1)  real  N                             */
+  HEDFT_DENX,         /* denx = |psix|^2 real  N (evaluated on download)     */
+  HEDFT_UIMPK,        /* uimp_k(:,:,:,1) real (nx/2+1,ny,nz)                 */
+  HEDFT_PSIXOLD1,     /* psixold(:,:,:,k,1) in PHYSICAL slot k = 1..3        */
+  HEDFT_PSIXOLD2,
+  HEDFT_PSIXOLD3,
+  HEDFT_HPSIXOLD1,    /* hpsixold(:,:,:,k,1), k = 1..2                       */
+  HEDFT_HPSIXOLD2,
+  HEDFT_PCX,          /* pcx(:,:,:,1)                                        */
+  HEDFT_QX,           /* qx(:,:,:,1)                                         */
   HEDFT_NFIELDS
 };
 
@@ -111,7 +125,7 @@ typedef struct hedft_params {
 typedef struct hedft_energies {
   double ekin4, elj4, ecor4, ealphas, esolid, eext; /* helium terms          */
   double etot4;                                     /* helium total          */
-  double ekinx, eHeX, eimpu_impu, eimpu;            /* classical impurities  */
+  double ekinx, eHeX, eimpu_impu, eimpu;            /* impurity terms        */
   double etot;                                      /* etot4 + eimpu         */
   double norm;                                      /* sum(den)*dxyz         */
 } hedft_energies;
@@ -195,6 +209,31 @@ int hedft_potenimp(hedft_ctx* c);
 int hedft_forceimp(hedft_ctx* c);
 /* aimp = F/m (DFT4He3d.f90:692-696). */
 int hedft_update_aimp(hedft_ctx* c);
+
+/* ---- quantum impurity (Dynamics_quantum_impurities) -------------------- */
+/* One impurity described by a wave function psix (N_imp = 1, so no
+ * impurity-impurity kernel uimp_k_k).  uimp_k: the X-He interaction's Fourier
+ * transform on the half spectrum (nx/2+1,ny,nz), real, as potenimpini.f90:19-29
+ * fills uimp_k(:,:,:,1) (bytes = 8*(nx/2+1)*ny*nz); m_imp_u: impurity mass in
+ * u (h2o2mimp = h2o2m4*4.002603/m_imp_u, steprk.f90:97); mu4: the chemical
+ * potential subtracted from pot4 in the helium equation (DFT4He3d.f90:183-189,
+ * 0 unless Lbulk).  TDDFT contexts without classical impurities only.
+ * Allocates psix, its PC history, potx4 and upotx (zero) and switches the
+ * context's TDDFT entry points to the quantum-impurity equations:
+ *   hedft_potenimp:  potx4, upotx (potenimp.f90:1-43)
+ *   hedft_poten:     poten with pot4 += potx4 (poten.f90:5-155; fden from den)
+ *   hedft_steprk/steppc: helium and psix together (steprk.f90:53-124,
+ *                    steppc.f90:36-157); steppc's errimp is errpsix, errvimp 0
+ *   hedft_energy:    ekinx, eHeX, eimpu = ekinx + eHeX (energy.f90:74-96)
+ *   hedft_tddft_init / iteration / run_pc: qx = 0 (DFT4He3d.f90:617), then
+ *                    potenimp + poten after each step (DFT4He3d.f90:667-677)
+ * Upload psix (HEDFT_PSIX) before hedft_tddft_init. */
+int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u,
+                               double mu4);
+/* Norm and centre of the impurity density, sum(|psix|^2)*dxyz and
+ * sum(|psix|^2 r)*dxyz/norm (DFT4He3d.f90:681-700, the rimp_k file rows).
+ * Synchronises. */
+int hedft_quantum_centroid(hedft_ctx* c, double* norm, double* rcm);
 
 /* ---- driver glue (order of DFT4He3d.f90:677-696, BCN4HeDFT.f90:1113-1160) */
 /* Statics start (BCN4HeDFT.f90:724-874): normalise den to n4, psi=sqrt(den),

@@ -13,6 +13,8 @@
  *   mean field He_statics/poten.f90, Dynamics_TDDFT/poten.f90, term_alfa.f90, derden.f90
  *   steppers   He_statics/steprkr.f90, steppcr.f90; Dynamics_TDDFT/steprk.f90, steppc.f90
  *   impurity   Dynamics_TDDFT/potenimp.f90, potenimpini.f90
+ *   quantum    Dynamics_quantum_impurities/potenimp.f90, poten.f90, steprk.f90,
+ *              steppc.f90, energy.f90, DFT4He3d.f90:617,681-700 (one impurity)
  *   energies   He_statics/energy.f90, Dynamics_TDDFT/energy.f90
  * The FFT is this file's own mixed-radix DFT (the reference links MKL's FFTW3,
  * which is not available here); the sine integral uses the series /
@@ -68,7 +70,8 @@ enum {
   F_DEN = 0, F_PSI, F_POT4, F_DENCG, F_DELJ4, F_DENALF, F_FALFS, F_INTX, F_INTY, F_INTZ,
   F_UALPHAS, F_DXDEN, F_DYDEN, F_DZDEN, F_UIMP, F_UEXT, F_TIMEC, F_PSIOLD1, F_PSIOLD2,
   F_PSIOLD3, F_HPSIOLD1, F_HPSIOLD2, F_PC, F_Q, F_FDEN, F_FVLJ4, F_WCGK, F_KALFS, F_WORK1,
-  F_WORK2, F_WORKC1, F_WORKK1, F_NFIELDS
+  F_WORK2, F_WORKC1, F_WORKK1, F_PSIX, F_POTX4, F_UPOTX, F_DENX, F_UIMPK, F_PSIXOLD1, F_PSIXOLD2,
+  F_PSIXOLD3, F_HPSIXOLD1, F_HPSIXOLD2, F_PCX, F_QX, F_NFIELDS
 };
 
 typedef struct {
@@ -101,6 +104,13 @@ typedef struct {
   double rimpold[3][MAXIMP * 3], vimpold[3][MAXIMP * 3], aimpold[2][MAXIMP * 3];
   int ioldr[3], ioldv[3], iolda[2];
   int lstopimp;
+  /* one quantum impurity (module quantal_imp, N_imp = 1) */
+  int quantum;
+  double h2o2mimp, mu4;
+  double *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *hpsix; /* complex N */
+  double *potx4, *upotx, *denx;                                  /* real N */
+  double *uimpk;                                                 /* real half spectrum */
+  double errpsix;
 } octx;
 
 /* ======================================================================= */
@@ -547,7 +557,9 @@ void oracle_destroy(octx* c) {
                     c->sto6, c->penalty, c->psi, c->timec, c->pc, c->q, c->hpsi, c->psiold[0],
                     c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->psi1s, c->psi2s, c->workc1,
                     (double*)c->fden, (double*)c->wk1, (double*)c->wk2, (double*)c->wk3,
-                    (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion};
+                    (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion,
+                    c->psix, c->psixold[0], c->psixold[1], c->psixold[2], c->hpsixold[0],
+                    c->hpsixold[1], c->pcx, c->qx, c->hpsix, c->potx4, c->upotx, c->denx, c->uimpk};
   for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); ++i) free(list[i]);
   free(c);
 }
@@ -587,8 +599,18 @@ double* oracle_field(octx* c, int id, size_t* ndoubles) {
     case F_WORK2: ptr = c->work2; break;
     case F_WORKC1: ptr = c->workc1; cnt = 2 * n; break;
     case F_WORKK1: ptr = (double*)c->workk1; cnt = 2 * nh; break;
+    case F_PSIX: ptr = c->psix; cnt = 2 * n; break;
+    case F_POTX4: ptr = c->potx4; break;
+    case F_UPOTX: ptr = c->upotx; break;
+    case F_DENX: ptr = c->denx; break;
+    case F_UIMPK: ptr = c->uimpk; cnt = nh; break;
+    case F_PSIXOLD1: case F_PSIXOLD2: case F_PSIXOLD3: ptr = c->psixold[id - F_PSIXOLD1]; cnt = 2 * n; break;
+    case F_HPSIXOLD1: case F_HPSIXOLD2: ptr = c->hpsixold[id - F_HPSIXOLD1]; cnt = 2 * n; break;
+    case F_PCX: ptr = c->pcx; cnt = 2 * n; break;
+    case F_QX: ptr = c->qx; cnt = 2 * n; break;
     default: ptr = NULL; cnt = 0;
   }
+  if (id >= F_PSIX && !c->quantum) { ptr = NULL; cnt = 0; }
   if (ndoubles) *ndoubles = cnt;
   return ptr;
 }
@@ -669,11 +691,13 @@ static void poten_tail(octx* c) {
       c->penalty[i] = p->csolid * (1.0 + t + p->beta_solid * c->den[i] * (1.0 - t * t));
     }
   }
-  int ext = (c->mode == MODE_STATICS) && p->lexternal;
+  /* external field: uext (statics, lexternal) or potx4 of a quantum impurity
+   * (Dynamics_quantum_impurities/poten.f90:116-121) */
+  const double* ext = (c->mode == MODE_STATICS) ? (p->lexternal ? c->uext : NULL) : (c->quantum ? c->potx4 : NULL);
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)n; ++i) {
     double cg = c->dencg[i];
-    c->pot4[i] = c->delj4[i] + cg * cg * (a0 + a1 * cg) + c->sto1[i] + (ext ? c->uext[i] : 0.0);
+    c->pot4[i] = c->delj4[i] + cg * cg * (a0 + a1 * cg) + c->sto1[i] + (ext ? ext[i] : 0.0);
   }
   if (p->core_otc) {
     if (c->mode == MODE_TDDFT) {
@@ -910,7 +934,9 @@ static double uimp_k_at(octx* c, int k, double xx, double yy, double zz) {
   return tab[ir] * (1.0 - rmod) + tab[ir + 1] * rmod;
 }
 
+static void potenimp_quantum(octx* c);
 void oracle_potenimp(octx* c) {
+  if (c->quantum) { potenimp_quantum(c); return; }
   int nx = c->nx, ny = c->ny, nz = c->nz;
   int flag = 0;
   for (int k = 0; k < c->nimp; ++k) {
@@ -962,9 +988,83 @@ void oracle_update_aimp(octx* c) {
 }
 
 /* ======================================================================= */
+/*  Quantum impurity (Dynamics_quantum_impurities, N_imp = 1)              */
+/* ======================================================================= */
+/* uimp_k(:,:,:,1) as potenimpini.f90:19-29 leaves it; h2o2mimp as steprk.f90:97 */
+void oracle_set_quantum_impurity(octx* c, const double* uimpk, double m_imp_u, double mu4) {
+  size_t n = c->n;
+  if (!c->quantum) {
+    c->psix = dalloc(2 * n); c->pcx = dalloc(2 * n); c->qx = dalloc(2 * n); c->hpsix = dalloc(2 * n);
+    for (int k = 0; k < 3; ++k) c->psixold[k] = dalloc(2 * n);
+    for (int k = 0; k < 2; ++k) c->hpsixold[k] = dalloc(2 * n);
+    c->potx4 = dalloc(n); c->upotx = dalloc(n); c->denx = dalloc(n); c->uimpk = dalloc(c->nh);
+  }
+  memcpy(c->uimpk, uimpk, sizeof(double) * c->nh);
+  c->h2o2mimp = c->p.h2o2m4 * 4.002603 / m_imp_u;
+  c->mu4 = mu4;
+  c->quantum = 1;
+}
+
+/* denx = psix*conjg(psix) */
+static void denx_from_psix(octx* c) {
+  const cplx* px = (const cplx*)c->psix;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) c->denx[i] = creal(px[i] * conj(px[i]));
+}
+
+/* potenimp.f90:20-30 for one impurity: fden = F(den),
+ * potx4 = F^-1[F(|psix|^2) uimp_k], upotx = F^-1[fden uimp_k] */
+static void potenimp_quantum(octx* c) {
+  fftfw(c, c->den, c->fden);
+  denx_from_psix(c);
+  fftfw(c, c->denx, c->wk1);
+  kmul(c, c->wk1, c->uimpk, c->wk2);
+  fftbk(c, c->wk2, c->potx4);
+  kmul(c, c->fden, c->uimpk, c->wk2);
+  fftbk(c, c->wk2, c->upotx);
+}
+
+/* k = timec*((lap psix)*h2o2mimp - upotx*psix)  (steprk.f90:101) */
+static void hpsi_x(octx* c, const cplx* psix, cplx* out) {
+  deriv(c, 2, 1, (const double*)psix, c->workc1, 2);
+  cplx* lap = (cplx*)dalloc(2 * c->n);
+  memcpy(lap, c->workc1, sizeof(cplx) * c->n);
+  deriv(c, 2, 2, (const double*)psix, c->workc1, 2);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) lap[i] += ((cplx*)c->workc1)[i];
+  deriv(c, 2, 3, (const double*)psix, c->workc1, 2);
+  const cplx* tc = (const cplx*)c->timec;
+  double h = c->h2o2mimp;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) {
+    cplx l = lap[i] + ((cplx*)c->workc1)[i];
+    out[i] = tc[i] * (l * h - c->upotx[i] * psix[i]);
+  }
+  free(lap);
+}
+
+/* norm and centre of |psix|^2 (DFT4He3d.f90:681-700 of the quantum driver) */
+void oracle_quantum_centroid(octx* c, double* norm, double* rcm) {
+  const cplx* px = (const cplx*)c->psix;
+  double s[4] = {0, 0, 0, 0};
+  for (int ix = 0; ix < c->nx; ++ix)
+    for (int iy = 0; iy < c->ny; ++iy)
+      for (int iz = 0; iz < c->nz; ++iz) {
+        size_t i = ix + (size_t)c->nx * (iy + (size_t)c->ny * iz);
+        double w = creal(conj(px[i]) * px[i]);
+        s[0] += w; s[1] += w * c->x[ix]; s[2] += w * c->y[iy]; s[3] += w * c->z[iz];
+      }
+  double a = s[0] * c->dxyz;
+  *norm = a;
+  for (int k = 0; k < 3; ++k) rcm[k] = s[1 + k] * c->dxyz / a;
+}
+
+/* ======================================================================= */
 /*  TDDFT integrators (complex psi)                                         */
 /* ======================================================================= */
-/* k = timec*((lap psi)*h2o2m4 - pot4*psi) - i*uimp*psi (steprk.f90:67) into out */
+/* k = timec*((lap psi)*h2o2m4 - pot4*psi) - i*uimp*psi (steprk.f90:67) into out;
+ * with a quantum impurity timec*((lap psi)*h2o2m4 - (pot4-mu4)*psi)
+ * (Dynamics_quantum_impurities/steprk.f90:66; uimp is zero there) */
 static void hpsi_complex(octx* c, const cplx* psi, cplx* out) {
   deriv(c, 2, 1, (const double*)psi, c->workc1, 2);
   cplx* lap = (cplx*)dalloc(2 * c->n);
@@ -978,7 +1078,8 @@ static void hpsi_complex(octx* c, const cplx* psi, cplx* out) {
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)c->n; ++i) {
     cplx l = lap[i] + ((cplx*)c->workc1)[i];
-    out[i] = tc[i] * (l * h - c->pot4[i] * psi[i]) - I * c->uimp[i] * psi[i];
+    double v = c->quantum ? c->pot4[i] - c->mu4 : c->pot4[i];
+    out[i] = tc[i] * (l * h - v * psi[i]) - I * c->uimp[i] * psi[i];
   }
   free(lap);
 }
@@ -1015,6 +1116,24 @@ void oracle_steprk(octx* c, double deltat) {
       psi[i] = psi[i] + deltat * s1;
     }
     den_from_psi(c);
+    if (c->quantum) { /* the same stage for psix (Dynamics_quantum_impurities/steprk.f90:96-111) */
+      cplx* px = (cplx*)c->psix;
+      cplx* qx = (cplx*)c->qx;
+      cplx* kx = (cplx*)c->hpsix;
+      cplx* pxo[3] = {(cplx*)c->psixold[0], (cplx*)c->psixold[1], (cplx*)c->psixold[2]};
+      cplx* hxo[2] = {(cplx*)c->hpsixold[0], (cplx*)c->hpsixold[1]};
+      hpsi_x(c, px, kx);
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < (long)n; ++i) {
+        cplx s1 = a * (kx[i] - b * qx[i]);
+        qx[i] = qx[i] + 3.0 * s1 - cc * kx[i];
+        if (jrun == 0) {
+          hxo[1][i] = hxo[0][i]; hxo[0][i] = kx[i];
+          pxo[2][i] = pxo[1][i]; pxo[1][i] = pxo[0][i]; pxo[0][i] = px[i];
+        }
+        px[i] = px[i] + deltat * s1;
+      }
+    }
     /* impurity positions and velocities, same scheme (steprk.f90:330-354) */
     for (int i = 0; i < m; ++i) {
       double st = a * (c->vimp[i] - b * c->qr[i]);
@@ -1067,6 +1186,23 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
       ho2[i] = k4[i];
     }
     den_from_psi(c);
+    if (c->quantum) { /* psix predictor (Dynamics_quantum_impurities/steppc.f90:60-78) */
+      cplx* px = (cplx*)c->psix;
+      cplx* pcx = (cplx*)c->pcx;
+      cplx* kx = (cplx*)c->hpsix;
+      cplx* xo3 = (cplx*)c->psixold[c->ioldp[2]];
+      cplx* xh1 = (cplx*)c->hpsixold[c->ioldh[0]];
+      cplx* xh2 = (cplx*)c->hpsixold[c->ioldh[1]];
+      hpsi_x(c, px, kx);
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < (long)n; ++i) {
+        cplx s1 = xo3[i] + c4o3 * deltat * (2.0 * kx[i] - xh1[i] + 2.0 * xh2[i]);
+        xo3[i] = px[i];
+        px[i] = s1 - c112 * pcx[i];
+        pcx[i] = s1;
+        xh2[i] = kx[i];
+      }
+    }
     int t = c->ioldh[1]; c->ioldh[1] = c->ioldh[0]; c->ioldh[0] = t;
   }
   /* impurity predictor + modifier (steppc.f90:461-480) */
@@ -1115,8 +1251,31 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
       acc[iz] = s;
     }
     ehe = c9 * sum_finish(acc, c->nz) / (double)n;
-    free(acc);
     den_from_psi(c);
+    if (c->quantum) { /* psix corrector (Dynamics_quantum_impurities/steppc.f90:106-124) */
+      cplx* px = (cplx*)c->psix;
+      cplx* pcx = (cplx*)c->pcx;
+      cplx* kx = (cplx*)c->hpsix;
+      cplx* xo3 = (cplx*)c->psixold[c->ioldp[2]];
+      cplx* xo2 = (cplx*)c->psixold[c->ioldp[1]];
+      cplx* xh1 = (cplx*)c->hpsixold[c->ioldh[0]];
+      cplx* xh2 = (cplx*)c->hpsixold[c->ioldh[1]];
+      hpsi_x(c, px, kx);
+#pragma omp parallel for schedule(static)
+      for (int iz = 0; iz < c->nz; ++iz) {
+        double sx = 0;
+        for (size_t j = 0; j < plane; ++j) {
+          size_t i = iz * plane + j;
+          cplx s5 = 0.125 * (9.0 * xo3[i] - xo2[i] + 3.0 * deltat * (kx[i] + 2.0 * xh1[i] - xh2[i]));
+          pcx[i] = pcx[i] - s5;
+          px[i] = s5 + c9 * pcx[i];
+          sx += cabs(pcx[i]);
+        }
+        acc[iz] = sx;
+      }
+      c->errpsix = c9 * sum_finish(acc, c->nz) / (double)n;
+    }
+    free(acc);
     int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t;
   }
   /* impurity corrector (steppc.f90:530-555) */
@@ -1146,6 +1305,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
     { int t = c->ioldv[2]; c->ioldv[2] = c->ioldv[1]; c->ioldv[1] = c->ioldv[0]; c->ioldv[0] = t; }
   }
   if (errHe) *errHe = ehe;
+  if (c->quantum) eimp = c->errpsix;
   if (errimp) *errimp = eimp;
   if (errvimp) *errvimp = evimp;
 }
@@ -1210,6 +1370,36 @@ void oracle_energy(octx* c, double* out) {
       ekinx += 0.5 * c->mimp[k] * v2;
     }
     eHeX = s[6] * dv;
+    if (c->quantum) {
+      /* Dynamics_quantum_impurities/energy.f90:74-96 with N_imp = 1 */
+      double* g1 = dalloc(2 * c->n);
+      double* g2 = dalloc(2 * c->n);
+      double* g3 = dalloc(2 * c->n);
+      deriv(c, 1, 1, c->psix, g1, 2);
+      deriv(c, 1, 2, c->psix, g2, 2);
+      deriv(c, 1, 3, c->psix, g3, 2);
+      denx_from_psix(c);
+      fftfw(c, c->denx, c->wk1);
+      kmul(c, c->wk1, c->uimpk, c->wk2);
+      fftbk(c, c->wk2, c->sto2);
+      double* qa = dalloc(2 * (size_t)nz);
+#pragma omp parallel for schedule(static)
+      for (int iz = 0; iz < nz; ++iz) {
+        double sk = 0, sh = 0;
+        for (size_t j = 0; j < plane; ++j) {
+          size_t i = iz * plane + j;
+          for (int q = 0; q < 2; ++q)
+            sk += g1[2 * i + q] * g1[2 * i + q] + g2[2 * i + q] * g2[2 * i + q] + g3[2 * i + q] * g3[2 * i + q];
+          sh += c->sto2[i] * c->den[i];
+        }
+        qa[2 * iz] = sk; qa[2 * iz + 1] = sh;
+      }
+      double sk = 0, sh = 0;
+      for (int iz = 0; iz < nz; ++iz) { sk += qa[2 * iz]; sh += qa[2 * iz + 1]; }
+      free(qa); free(g1); free(g2); free(g3);
+      ekinx = c->h2o2mimp * sk * dv;
+      eHeX = sh * dv;
+    }
     eimpu = ekinx + eHeX;
     etot = etot4 + eimpu;
   } else {
@@ -1351,6 +1541,7 @@ void oracle_tddft_init(octx* c) {
   memset(c->qr, 0, sizeof c->qr); memset(c->qv, 0, sizeof c->qv);
   memset(c->rimpold, 0, sizeof c->rimpold); memset(c->vimpold, 0, sizeof c->vimpold);
   memset(c->aimpold, 0, sizeof c->aimpold);
+  if (c->quantum) memset(c->qx, 0, sizeof(double) * 2 * n); /* DFT4He3d.f90:617 (quantum driver) */
 }
 
 void oracle_tddft_iteration(octx* c, double deltat, int iter, int nrk, double* errHe, double* errimp,

@@ -16,6 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libhedft_ref.so")
+REFQ_PATH = os.path.join(HERE, "_ref", "libhedft_refq.so")
 
 MODE_STATICS, MODE_TDDFT = 0, 1
 
@@ -25,6 +26,8 @@ FIELDS = [
     "UALPHAS", "DXDEN", "DYDEN", "DZDEN", "UIMP", "UEXT", "TIMEC", "PSIOLD1", "PSIOLD2",
     "PSIOLD3", "HPSIOLD1", "HPSIOLD2", "PC", "Q", "FDEN", "FVLJ4", "WCGK", "KALFS", "WORK1",
     "WORK2", "WORKC1", "WORKK1",
+    "PSIX", "POTX4", "UPOTX", "DENX", "UIMPK", "PSIXOLD1", "PSIXOLD2", "PSIXOLD3", "HPSIXOLD1",
+    "HPSIXOLD2", "PCX", "QX",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 ENERGY_NAMES = ["ekin4", "elj4", "ecor4", "ealphas", "esolid", "eext", "etot4", "ekinx", "eHeX",
@@ -75,6 +78,7 @@ def default_params(**kw) -> Params:
 
 _lib = None
 _ref = None
+_refq = None
 
 
 def lib():
@@ -114,6 +118,8 @@ def lib():
         L.oracle_statics_iteration_sd.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
         L.oracle_ironing.argtypes = [ip, ip, ip, vp]
         L.oracle_tddft_iteration.argtypes = [vp, C.c_double, ip, ip, dp, dp, dp]
+        L.oracle_set_quantum_impurity.argtypes = [vp, vp, C.c_double, C.c_double]
+        L.oracle_quantum_centroid.argtypes = [vp, dp, vp]
         L.oracle_num_threads.restype = ip
         _lib = L
     return _lib
@@ -139,6 +145,36 @@ def ref():
         R.ref_init_deriv(13, 2, 1)
         _ref = R
     return _ref
+
+
+def ref_q():
+    """The reference's impurity-potential library and FT_V_spline
+    (Dynamics_quantum_impurities, oracle/_ref), or None when not built."""
+    global _refq
+    if _refq is None and os.path.exists(REFQ_PATH):
+        R = C.CDLL(REFQ_PATH)
+        d, ip = C.c_double, C.c_int
+        R.refq_select_pot.restype = d
+        R.refq_select_pot.argtypes = [C.c_char_p, ip, d, d, d]
+        R.refq_ft_v_spline.restype = d
+        R.refq_ft_v_spline.argtypes = [d, d, d, C.c_char_p, ip, d, d, ip]
+        _refq = R
+    return _refq
+
+
+def ref_select_pot(selec: str, r, r_cutoff: float, umax: float) -> np.ndarray:
+    """Select_Pot(selec, r, r_cutoff, umax) of the reference at each r."""
+    R = ref_q()
+    b = selec.encode()
+    return np.array([R.refq_select_pot(b, len(b), float(x), r_cutoff, umax) for x in np.atleast_1d(r)])
+
+
+def ref_ft_v_spline(selec: str, p, pmax: float, hq: float, r_cutoff: float, umax: float, parabolic=False):
+    """FT_V_spline(p, pmax, hq, selec, r_cutoff, umax) of the reference at each p."""
+    R = ref_q()
+    b = selec.encode()
+    return np.array([R.refq_ft_v_spline(float(x), pmax, hq, b, len(b), r_cutoff, umax, int(parabolic))
+                     for x in np.atleast_1d(p)])
 
 
 def _ptr(a: np.ndarray):
@@ -368,6 +404,16 @@ class Oracle:
         a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
         lib().oracle_tddft_iteration(self._c, dt, it, nrk, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
+
+    def set_quantum_impurity(self, uimp_k, m_imp_u, mu4=0.0):
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(uimp_k, dtype=np.float64), self.hshape))
+        lib().oracle_set_quantum_impurity(self._c, _ptr(k), float(m_imp_u), float(mu4))
+
+    def quantum_centroid(self):
+        nrm = C.c_double(0)
+        r = np.zeros(3)
+        lib().oracle_quantum_centroid(self._c, C.byref(nrm), _ptr(r))
+        return nrm.value, r
 
 
 def num_threads() -> int:

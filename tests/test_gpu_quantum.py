@@ -1,0 +1,145 @@
+"""GPU parity of the quantum-impurity path (Dynamics_quantum_impurities) against
+the FP64 oracle, through the C-ABI, on seeded droplets with one impurity wave
+packet and a Lennard-Jones X-He kernel from he_dft_2024_amd.quantum."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import relerr
+from he_dft_2024_amd import inputs as I
+from he_dft_2024_amd import quantum as Q
+from he_dft_2024_amd.hedft import FIELD
+
+pytestmark = pytest.mark.gpu
+
+RTOL_OP = 1e-12
+RTOL_TRAJ = 1e-9
+N, XMAX, N4 = 32, 6.4, 60.0
+M_IMP = 2 * 4.002603
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+_UK = {}
+
+
+def _uimp_k(n=N, xmax=XMAX):
+    if (n, xmax) not in _UK:
+        v = Q.lennard_jones(10.22, 2.556)
+        _UK[(n, xmax)] = Q.uimp_k_table(n, n, n, xmax, xmax, xmax, v, 2.0, float(v(2.0)))
+    return _UK[(n, xmax)]
+
+
+def _pair(gpu, otc, precision=0, **kw):
+    gp = gpu.default_params(core_otc=otc, precision=precision, **kw)
+    op = O.default_params(core_otc=otc, **kw)
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, gp, mode=gpu.MODE_TDDFT)
+    o = O.Oracle(N, N, N, XMAX, XMAX, XMAX, op, mode=O.MODE_TDDFT)
+    for x in (d, o):
+        I.quantum_setup(x, N, XMAX, N4, _uimp_k(), seed=3, m_imp_u=M_IMP, mu4=-1.5, r0=(0.4, -0.8, 1.2),
+                        width=1.3, k=(0.6, 0.0, -0.3))
+    return d, o
+
+
+@pytest.mark.parametrize("otc,lsolid", [(1, 0), (0, 0), (0, 1)])
+def test_quantum_potenimp_poten_vs_oracle(gpu, otc, lsolid):
+    _say("quantum potenimp+poten otc", otc, "lsolid", lsolid)
+    d, o = _pair(gpu, otc, lsolid=lsolid)
+    for x in (d, o):
+        x.potenimp()
+    for name in ("POTX4", "UPOTX"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_OP, name
+    for x in (d, o):
+        x.poten()
+    for name in ("POTX4", "UPOTX", "DENCG", "DELJ4", "POT4") + (("DENALF", "UALPHAS") if otc else ()):
+        assert relerr(d.field(name), o.field(name)) < RTOL_OP, name
+    assert relerr(d.field("DENX"), np.abs(o.field("PSIX")) ** 2) < 1e-15
+    d.close()
+
+
+@pytest.mark.parametrize("otc", [1, 0])
+def test_quantum_trajectory_vs_oracle(gpu, otc):
+    _say("quantum 3 RK + 9 PC otc", otc)
+    d, o = _pair(gpu, otc)
+    dt = 5e-4 / I.PSTOK
+    errs = []
+    for x in (d, o):
+        x.tddft_init()
+        errs.append(np.array([x.tddft_iteration(dt, it, 3) for it in range(1, 13)]))
+    assert relerr(errs[0][3:, :2], errs[1][3:, :2]) < RTOL_TRAJ
+    assert np.all(errs[0][3:, 1] > 0)
+    for name in ("PSI", "DEN", "POT4", "PSIX", "POTX4", "UPOTX", "PSIXOLD1", "PSIXOLD2",
+                 "PSIXOLD3", "HPSIXOLD1", "HPSIXOLD2"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    # pc, pcx end each step as predictor minus corrector, ~1e-10 of psi here: compared on psi's scale
+    for name, ref in (("PC", "PSI"), ("PCX", "PSIX")):
+        assert np.abs(d.field(name) - o.field(name)).max() < RTOL_TRAJ * np.abs(o.field(ref)).max(), name
+    kscale = np.abs(o.field("HPSIXOLD1")).max()
+    assert np.abs(d.field("QX")).max() < 1e-12 * kscale
+    eg, eo = d.energy(), o.energy()
+    for k in ("ekin4", "elj4", "ecor4", "ealphas", "ekinx", "eHeX", "eimpu", "etot", "norm"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
+    (ng, rg), (no, ro) = d.quantum_centroid(), o.quantum_centroid()
+    assert abs(ng - no) < 1e-12 and np.abs(rg - ro).max() < 1e-10
+    d.close()
+
+
+def test_quantum_graph_replay_matches_eager(gpu):
+    _say("quantum graph vs eager")
+    dt = 5e-4 / I.PSTOK
+    outs = []
+    for use_graph in (False, True):
+        d, o = _pair(gpu, 1)
+        o.close()
+        d.tddft_init()
+        for it in range(1, 4):
+            d.tddft_iteration(dt, it, 3)
+        d.run_pc(dt, 13, use_graph)
+        d.run_pc(dt, 5, use_graph)
+        outs.append((d.field("PSI"), d.field("PSIX"), d.field("POT4")))
+        d.close()
+    for a, b in zip(outs[1], outs[0]):
+        assert relerr(a, b) < 1e-13
+
+
+def test_quantum_mixed_close_to_fp64(gpu):
+    _say("quantum mixed vs fp64")
+    dt = 5e-4 / I.PSTOK
+    res = []
+    for prec in (0, 1):
+        d, o = _pair(gpu, 1, precision=prec)
+        o.close()
+        d.tddft_init()
+        for it in range(1, 4):
+            d.tddft_iteration(dt, it, 3)
+        d.run_pc(dt, 24)
+        res.append((d.field("PSI"), d.field("PSIX"), d.energy()))
+        d.close()
+    assert relerr(res[1][0], res[0][0]) < 2e-5
+    assert relerr(res[1][1], res[0][1]) < 2e-5
+    for k in ("etot", "ekinx", "eHeX"):
+        assert abs(res[1][2][k] - res[0][2][k]) < 1e-4 * abs(res[0][2][k]), k
+
+
+def test_quantum_abi_errors(gpu):
+    _say("quantum abi errors")
+    import ctypes as C
+    L = gpu.lib()
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_STATICS)
+    k = np.zeros((N, N, N // 2 + 1))
+    rc = L.hedft_set_quantum_impurity(d._c, k.ctypes.data_as(C.c_void_p), k.nbytes, 4.0, 0.0)
+    assert rc == -1  # statics context
+    d.close()
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_TDDFT)
+    assert L.hedft_field_bytes(d._c, FIELD["PSIX"]) == 0  # not allocated yet
+    assert L.hedft_set_quantum_impurity(d._c, k.ctypes.data_as(C.c_void_p), k.nbytes - 8, 4.0, 0.0) == -1
+    assert L.hedft_set_quantum_impurity(d._c, k.ctypes.data_as(C.c_void_p), k.nbytes, 0.0, 0.0) == -1
+    d.set_quantum_impurity(k, 4.0)
+    assert L.hedft_field_bytes(d._c, FIELD["PSIX"]) == 16 * N ** 3
+    pot = np.zeros((1, 10))
+    m = np.ones(1)
+    rc = L.hedft_set_impurities(d._c, 1, 10, 1.0, 10.0, pot.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p))
+    assert rc == -1  # one impurity model per context
+    d.close()
