@@ -142,8 +142,8 @@ struct Lam {
 
 // ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
 // k = timec*((lap)*h2o2m4 - (pot4-mu4)*psi) - i*uimp*psi   (steprk.f90:67,
-// steppc.f90:432,499; mu4 and no uimp in Dynamics_quantum_impurities/steprk.f90:66,
-// steppc.f90:48,94; mu4 = 0 for classical impurities)
+// steppc.f90:432,499; mu4 and no uimp in Dynamics_quantum_impurities/steprk.f90:68,
+// steppc.f90:46,105; mu4 = 0 for classical impurities)
 template <typename R>
 struct HpsiC {
   const double2* lapxy;
@@ -168,7 +168,7 @@ struct HpsiC {
 };
 
 // quantum impurity: k = timec*((lap psix)*h2o2mimp - upotx*psix)
-// (Dynamics_quantum_impurities/steprk.f90:101, steppc.f90:66,114)
+// (Dynamics_quantum_impurities/steprk.f90:94, steppc.f90:77,127)
 template <typename R>
 struct HpsiX {
   const double2* lapxy;
@@ -603,7 +603,7 @@ __global__ void k_convert(long long n, const A* in, B* out) {
        i += (long long)gridDim.x * blockDim.x)
     out[i] = (B)in[i];
 }
-// denx = psix*conjg(psix) (Dynamics_quantum_impurities/steprk.f90:110)
+// denx = psix*conjg(psix) (Dynamics_quantum_impurities/steprk.f90:105)
 __global__ void k_abs2(long long n, const double2* z, double* out) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -611,7 +611,7 @@ __global__ void k_abs2(long long n, const double2* z, double* out) {
     out[i] = v.x * v.x + v.y * v.y;
   }
 }
-// sums of |psix|^2 and |psix|^2 (x,y,z) (DFT4He3d.f90:683-695 of the quantum driver)
+// sums of |psix|^2 and |psix|^2 (x,y,z) (DFT4He3d.f90:682-700 of the quantum driver)
 __global__ void __launch_bounds__(256) k_centroid(Dims d, const double2* z, double x0, double y0, double z0,
                                                   double hx, double hy, double hz, double* partials) {
   double red[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1167,7 +1167,7 @@ static void kt_mark(hedft_ctx* c, int slot, int end) {
   if (end) c->kt_n[slot]++;
 }
 
-// RK-Gill stage of a quantum impurity (Dynamics_quantum_impurities/steprk.f90:96-110)
+// RK-Gill stage of a quantum impurity (Dynamics_quantum_impurities/steprk.f90:84-106)
 template <typename R>
 static void steprk_x(hedft_ctx* c, double dt, int jrun) {
   OutRK<R, HpsiX<R>> o;
@@ -1227,7 +1227,7 @@ static int steprk_t(hedft_ctx* c, double dt) {
 
 // STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC].
 // A quantum impurity takes the same predictor and corrector
-// (Dynamics_quantum_impurities/steppc.f90:56-78,106-124) with the same slot
+// (Dynamics_quantum_impurities/steppc.f90:64-95,116-139) with the same slot
 // indices (ioldpx, ioldhx rotate with ioldp, ioldh); sum|pcx| into scal[S_ERRPCX].
 template <typename R>
 static int steppc_t(hedft_ctx* c, double dt) {
@@ -1453,7 +1453,7 @@ static int energy_sums(hedft_ctx* c, double* s) {
   return read_scal(c, S_ENERGY, 8, s);
 }
 
-// quantum impurity terms (Dynamics_quantum_impurities/energy.f90:74-96):
+// quantum impurity terms (Dynamics_quantum_impurities/energy.f90:102-132):
 // sum |grad psix|^2 and sum F^-1[F(|psix|^2) uimp_k] den
 template <typename R>
 struct OutEnergyX {
@@ -2126,7 +2126,7 @@ int hedft_update_aimp(hedft_ctx* c) {
 }
 
 // potenimpini's uimp_k(:,:,:,1) and the mass of one quantum impurity
-// (Dynamics_quantum_impurities/potenimpini.f90:19-29, steprk.f90:97)
+// (Dynamics_quantum_impurities/potenimpini.f90:19-29, steprk.f90:86)
 int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u, double mu4) {
   if (!c || !uimp_k || c->mode != HEDFT_MODE_TDDFT || c->nimp > 0 || !(m_imp_u > 0.0)) return HEDFT_EINVAL;
   if (bytes != sizeof(double) * (size_t)c->d.nh) return HEDFT_EINVAL;

@@ -215,7 +215,7 @@ int hedft_update_aimp(hedft_ctx* c);
  * impurity-impurity kernel uimp_k_k).  uimp_k: the X-He interaction's Fourier
  * transform on the half spectrum (nx/2+1,ny,nz), real, as potenimpini.f90:19-29
  * fills uimp_k(:,:,:,1) (bytes = 8*(nx/2+1)*ny*nz); m_imp_u: impurity mass in
- * u (h2o2mimp = h2o2m4*4.002603/m_imp_u, steprk.f90:97); mu4: the chemical
+ * u (h2o2mimp = h2o2m4*4.002603/m_imp_u, steprk.f90:86); mu4: the chemical
  * potential subtracted from pot4 in the helium equation (DFT4He3d.f90:183-189,
  * 0 unless Lbulk).  TDDFT contexts without classical impurities only.
  * Allocates psix, its PC history, potx4 and upotx (zero) and switches the
@@ -224,14 +224,14 @@ int hedft_update_aimp(hedft_ctx* c);
  *   hedft_poten:     poten with pot4 += potx4 (poten.f90:5-155; fden from den)
  *   hedft_steprk/steppc: helium and psix together (steprk.f90:53-124,
  *                    steppc.f90:36-157); steppc's errimp is errpsix, errvimp 0
- *   hedft_energy:    ekinx, eHeX, eimpu = ekinx + eHeX (energy.f90:74-96)
+ *   hedft_energy:    ekinx, eHeX, eimpu = ekinx + eHeX (energy.f90:102-132)
  *   hedft_tddft_init / iteration / run_pc: qx = 0 (DFT4He3d.f90:617), then
  *                    potenimp + poten after each step (DFT4He3d.f90:667-677)
  * Upload psix (HEDFT_PSIX) before hedft_tddft_init. */
 int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u,
                                double mu4);
 /* Norm and centre of the impurity density, sum(|psix|^2)*dxyz and
- * sum(|psix|^2 r)*dxyz/norm (DFT4He3d.f90:681-700, the rimp_k file rows).
+ * sum(|psix|^2 r)*dxyz/norm (DFT4He3d.f90:682-700, the rimp_k file rows).
  * Synchronises. */
 int hedft_quantum_centroid(hedft_ctx* c, double* norm, double* rcm);
 

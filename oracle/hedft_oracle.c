@@ -14,7 +14,7 @@
  *   steppers   He_statics/steprkr.f90, steppcr.f90; Dynamics_TDDFT/steprk.f90, steppc.f90
  *   impurity   Dynamics_TDDFT/potenimp.f90, potenimpini.f90
  *   quantum    Dynamics_quantum_impurities/potenimp.f90, poten.f90, steprk.f90,
- *              steppc.f90, energy.f90, DFT4He3d.f90:617,681-700 (one impurity)
+ *              steppc.f90, energy.f90, DFT4He3d.f90:617,682-700 (one impurity)
  *   energies   He_statics/energy.f90, Dynamics_TDDFT/energy.f90
  * The FFT is this file's own mixed-radix DFT (the reference links MKL's FFTW3,
  * which is not available here); the sine integral uses the series /
@@ -990,7 +990,7 @@ void oracle_update_aimp(octx* c) {
 /* ======================================================================= */
 /*  Quantum impurity (Dynamics_quantum_impurities, N_imp = 1)              */
 /* ======================================================================= */
-/* uimp_k(:,:,:,1) as potenimpini.f90:19-29 leaves it; h2o2mimp as steprk.f90:97 */
+/* uimp_k(:,:,:,1) as potenimpini.f90:19-29 leaves it; h2o2mimp as steprk.f90:86 */
 void oracle_set_quantum_impurity(octx* c, const double* uimpk, double m_imp_u, double mu4) {
   size_t n = c->n;
   if (!c->quantum) {
@@ -1024,7 +1024,7 @@ static void potenimp_quantum(octx* c) {
   fftbk(c, c->wk2, c->upotx);
 }
 
-/* k = timec*((lap psix)*h2o2mimp - upotx*psix)  (steprk.f90:101) */
+/* k = timec*((lap psix)*h2o2mimp - upotx*psix)  (Dynamics_quantum_impurities/steprk.f90:94) */
 static void hpsi_x(octx* c, const cplx* psix, cplx* out) {
   deriv(c, 2, 1, (const double*)psix, c->workc1, 2);
   cplx* lap = (cplx*)dalloc(2 * c->n);
@@ -1043,7 +1043,7 @@ static void hpsi_x(octx* c, const cplx* psix, cplx* out) {
   free(lap);
 }
 
-/* norm and centre of |psix|^2 (DFT4He3d.f90:681-700 of the quantum driver) */
+/* norm and centre of |psix|^2 (DFT4He3d.f90:682-700 of the quantum driver) */
 void oracle_quantum_centroid(octx* c, double* norm, double* rcm) {
   const cplx* px = (const cplx*)c->psix;
   double s[4] = {0, 0, 0, 0};
@@ -1064,7 +1064,7 @@ void oracle_quantum_centroid(octx* c, double* norm, double* rcm) {
 /* ======================================================================= */
 /* k = timec*((lap psi)*h2o2m4 - pot4*psi) - i*uimp*psi (steprk.f90:67) into out;
  * with a quantum impurity timec*((lap psi)*h2o2m4 - (pot4-mu4)*psi)
- * (Dynamics_quantum_impurities/steprk.f90:66; uimp is zero there) */
+ * (Dynamics_quantum_impurities/steprk.f90:68; uimp is zero there) */
 static void hpsi_complex(octx* c, const cplx* psi, cplx* out) {
   deriv(c, 2, 1, (const double*)psi, c->workc1, 2);
   cplx* lap = (cplx*)dalloc(2 * c->n);
@@ -1116,7 +1116,7 @@ void oracle_steprk(octx* c, double deltat) {
       psi[i] = psi[i] + deltat * s1;
     }
     den_from_psi(c);
-    if (c->quantum) { /* the same stage for psix (Dynamics_quantum_impurities/steprk.f90:96-111) */
+    if (c->quantum) { /* the same stage for psix (Dynamics_quantum_impurities/steprk.f90:84-106) */
       cplx* px = (cplx*)c->psix;
       cplx* qx = (cplx*)c->qx;
       cplx* kx = (cplx*)c->hpsix;
@@ -1186,7 +1186,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
       ho2[i] = k4[i];
     }
     den_from_psi(c);
-    if (c->quantum) { /* psix predictor (Dynamics_quantum_impurities/steppc.f90:60-78) */
+    if (c->quantum) { /* psix predictor (Dynamics_quantum_impurities/steppc.f90:64-95) */
       cplx* px = (cplx*)c->psix;
       cplx* pcx = (cplx*)c->pcx;
       cplx* kx = (cplx*)c->hpsix;
@@ -1252,7 +1252,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
     }
     ehe = c9 * sum_finish(acc, c->nz) / (double)n;
     den_from_psi(c);
-    if (c->quantum) { /* psix corrector (Dynamics_quantum_impurities/steppc.f90:106-124) */
+    if (c->quantum) { /* psix corrector (Dynamics_quantum_impurities/steppc.f90:116-139) */
       cplx* px = (cplx*)c->psix;
       cplx* pcx = (cplx*)c->pcx;
       cplx* kx = (cplx*)c->hpsix;
@@ -1371,7 +1371,7 @@ void oracle_energy(octx* c, double* out) {
     }
     eHeX = s[6] * dv;
     if (c->quantum) {
-      /* Dynamics_quantum_impurities/energy.f90:74-96 with N_imp = 1 */
+      /* Dynamics_quantum_impurities/energy.f90:102-132 with N_imp = 1 */
       double* g1 = dalloc(2 * c->n);
       double* g2 = dalloc(2 * c->n);
       double* g3 = dalloc(2 * c->n);

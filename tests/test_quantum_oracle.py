@@ -126,8 +126,14 @@ def test_free_impurity_plane_wave_phase():
     h = 2 * xmax / n
     kappa = -(O.derivnd(2, psi0, h, 1, 8) / psi0).real.mean()  # periodic stencil's symbol
     den = np.full((n, n, n), 0.02)
-    o = _quantum_oracle(n, xmax, uimp_k=0.0, m=m_imp, psix=psi0, den=den, icon=8)
+    mu4 = 3.7
+    o = O.Oracle(n, n, n, xmax, xmax, xmax, O.default_params(icon=8))
+    o.set("DEN", den)
+    o.set_quantum_impurity(0.0, m_imp, mu4)
+    o.set("PSIX", psi0)
     o.tddft_init()
+    pot4 = o.field("POT4").mean()  # uniform helium: a constant mean field
+    assert np.ptp(o.field("POT4")) < 1e-12 * abs(pot4)
     dt = 5e-4 / I.PSTOK
     for it in range(1, 13):
         o.tddft_iteration(dt, it, 3)
@@ -135,6 +141,9 @@ def test_free_impurity_plane_wave_phase():
     exact = psi0 * np.exp(-1j * w * 12 * dt)
     assert w * dt > 5e-4  # a second-order scheme would miss by ~(w dt)^3 per step
     assert relerr(o.field("PSIX"), exact) < 1e-11
+    # the helium rotates with (pot4 - mu4) (Dynamics_quantum_impurities/steprk.f90:68)
+    he = np.sqrt(den) * np.exp(-1j * (pot4 - mu4) * 12 * dt)
+    assert relerr(o.field("PSI"), he) < 1e-11
 
 
 def test_coupled_evolution_conserves_norms():
