@@ -32,6 +32,7 @@ enum {
   S_ERRPCX = 7,  // sum|pcx| of a quantum impurity
   S_ENERGY = 8,  // 8 partial energy sums
   S_ENERGYQ = 16,  // 2 quantum-impurity energy sums
+  S_OBS = 16,      // 13 observable sums (used alone)
   NSCAL = 32
 };
 // ---- impurity block layout (doubles) -------------------------------------------
@@ -1018,7 +1019,9 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   {
     SideBranch sb(c);
     flush_side_pre(c);  // impurity PC and errHe of the step just queued
-    HEDFT_TRY(derden_force(c, want_force, want_force));
+    // the quantum-impurity poten calls derden only for the OTC core
+    // (Dynamics_quantum_impurities/poten.f90:48): nothing else reads grad(den) there
+    if (!c->quantum || c->p.core_otc) HEDFT_TRY(derden_force(c, want_force, want_force));
     if (with_lap) HEDFT_TRY(lap_xy(c, c->psi, 1));
     if (c->quantum) {
       if (with_lap) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
@@ -1482,6 +1485,74 @@ static int energy_sums_x(hedft_ctx* c, double* s) {
   hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
                      (int)pencil_geom<double2>(c->d, 2).nblocks, c->scal + S_ENERGYQ, 1.0);
   return read_scal(c, S_ENERGYQ, 2, s);
+}
+
+// ---- droplet observables (Dynamics_TDDFT/DFT4He3d.f90:709-836, r_cm.f90) ----------------------
+// x and y first derivatives of psi into two complex fields
+struct OutTwoC {
+  static constexpr int NR = 0;
+  double2 *ox, *oy;
+  double* partials;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double2 dx, double2 dy, double*) const {
+    ox[i] = dx;
+    oy[i] = dy;
+  }
+};
+
+// z pass with the sums: den moments (7) and, with g = grad psi,
+// Re[i conj(psi) g_a] (3) and Re[i conj(psi) (r x g)_a] about the origin (3)
+template <typename R>
+struct OutObs {
+  static constexpr int NR = 13;
+  double* partials;
+  const double2 *psi, *dx, *dy;
+  const R* den;
+  double x0, y0, z0, hx, hy, hz;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double* red) const {
+    const double x = x0 + hx * ix, y = y0 + hy * iy, z = z0 + hz * iz;
+    const double r = (double)den[i];
+    red[0] += r;
+    red[1] += r * x;
+    red[2] += r * y;
+    red[3] += r * z;
+    red[4] += r * x * x;
+    red[5] += r * y * y;
+    red[6] += r * z * z;
+    const double2 p = psi[i], gx = dx[i], gy = dy[i];
+    // Re[i conj(p) g] = p.y g.x - p.x g.y
+    const double jx = p.y * gx.x - p.x * gx.y, jy = p.y * gy.x - p.x * gy.y, jz = p.y * dz.x - p.x * dz.y;
+    red[7] += z * jy - y * jz;
+    red[8] += x * jz - z * jx;
+    red[9] += y * jx - x * jy;
+    red[10] += jx;
+    red[11] += jy;
+    red[12] += jz;
+  }
+};
+
+template <typename R>
+static int observable_sums(hedft_ctx* c, double* s) {
+  const double2* psi = (const double2*)c->psi;
+  double2 *gx = (double2*)c->workc1, *gy = (double2*)c->lap;
+  launch_pencil_xy<double2, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<double2>{psi},
+                                  OutTwoC{gx, gy, nullptr});
+  c->lap_fresh = 0;  // the Laplacian buffer held gy
+  OutObs<R> o;
+  o.partials = gen_partials(c);
+  o.psi = psi;
+  o.dx = gx;
+  o.dy = gy;
+  o.den = fld<R>(c->den);
+  o.x0 = -c->g.xmax;
+  o.y0 = -c->g.ymax;
+  o.z0 = -c->g.zmax;
+  o.hx = c->hx;
+  o.hy = c->hy;
+  o.hz = c->hz;
+  launch_pencil<double2, 2>(c->stream, c->d, c->coef[0][2], InPlain<double2>{psi}, o);
+  hipLaunchKernelGGL(k_finalize<13>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
+                     (int)pencil_geom<double2>(c->d, 2).nblocks, c->scal + S_OBS, 1.0);
+  return read_scal(c, S_OBS, 13, s);
 }
 
 }  // namespace hedft
@@ -1963,6 +2034,27 @@ int hedft_energy(hedft_ctx* c, hedft_energies* e) {
   }
   e->etot = e->etot4 + e->eimpu;
   e->norm = s[7] * dv;
+  return HEDFT_OK;
+}
+
+int hedft_compute_observables(hedft_ctx* c, int32_t n4, double mhe, hedft_observables* o) {
+  if (!c || !o || c->mode != HEDFT_MODE_TDDFT || n4 <= 0 || !(mhe > 0.0)) return HEDFT_EINVAL;
+  double s[13];
+  if (c->mixed) HEDFT_TRY(observable_sums<float>(c, s));
+  else HEDFT_TRY(observable_sums<double>(c, s));
+  const double dv = c->dxyz;
+  std::memset(o, 0, sizeof *o);
+  o->norm = s[0] * dv;
+  for (int a = 0; a < 3; ++a) {
+    o->rcm[a] = s[1 + a] * dv / (double)n4;
+    o->r2[a] = s[4 + a] * dv;
+    o->vcom[a] = -s[10 + a] * dv / (o->norm * mhe);
+  }
+  const double xc = o->rcm[0], yc = o->rcm[1], zc = o->rcm[2];
+  const double gx = s[10], gy = s[11], gz = s[12];
+  o->l[0] = (s[7] - zc * gy + yc * gz) * dv;
+  o->l[1] = (s[8] - xc * gz + zc * gx) * dv;
+  o->l[2] = (s[9] - yc * gx + xc * gy) * dv;
   return HEDFT_OK;
 }
 

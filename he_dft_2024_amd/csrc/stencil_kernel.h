@@ -290,7 +290,14 @@ __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef c
   const int nx = d.nx, ny = d.ny;
   V* A = reinterpret_cast<V*>(smem_raw);
   V* B = SAME ? A : A + TW * ny;
-  const int x0 = blockIdx.x * TX, z = blockIdx.y;
+  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+  // XCDs, so physical id p runs on XCD p % 8.  Giving each XCD a contiguous
+  // range of tiles (x tiles of a plane next to each other) lets the 12 halo
+  // columns a tile shares with its x neighbours hit that XCD's L2.
+  const int ntiles = gridDim.x * gridDim.y;
+  int tile = blockIdx.x + gridDim.x * blockIdx.y;
+  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+  const int x0 = (tile % gridDim.x) * TX, z = tile / gridDim.x;
   const size_t zb = (size_t)z * nx * ny;
   {
     // Halo slots outside the box (non-periodic modes) are never read by any

@@ -51,6 +51,11 @@ module hedft_gpu
     real(c_double) :: ekinx, eHeX, eimpu_impu, eimpu, etot, norm
   end type hedft_energies_t
 
+  ! observables block of the TDDFT driver (Dynamics_TDDFT/DFT4He3d.f90:709-836)
+  type, bind(C) :: hedft_observables_t
+    real(c_double) :: norm, rcm(3), l(3), vcom(3), r2(3)
+  end type hedft_observables_t
+
   type, bind(C) :: hedft_tuning_t
     integer(c_int32_t) :: xy_split_fwd, xy_split_inv, no_side, prefer_eager
     real(c_double) :: ms_per_step
@@ -142,6 +147,14 @@ module hedft_gpu
       import :: c_int, c_ptr, hedft_energies_t
       type(c_ptr), value :: ctx
       type(hedft_energies_t), intent(out) :: e
+    end function
+    ! n4: the nominal atom number r_cm divides by; mhe in K^-1 A^-2
+    integer(c_int) function hedft_compute_observables(ctx, n4, mhe, o) bind(C, name="hedft_compute_observables")
+      import :: c_int, c_ptr, c_int32_t, c_double, hedft_observables_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: n4
+      real(c_double), value :: mhe
+      type(hedft_observables_t), intent(out) :: o
     end function
     integer(c_int) function hedft_steprkr(ctx, deltat, mu4, mu4err, n4) bind(C, name="hedft_steprkr")
       import :: c_int, c_ptr, c_double

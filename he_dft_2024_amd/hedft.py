@@ -54,6 +54,15 @@ class Energies(C.Structure):
         "eimpu_impu", "eimpu", "etot", "norm")]
 
 
+class Observables(C.Structure):
+    _fields_ = [("norm", C.c_double), ("rcm", C.c_double * 3), ("l", C.c_double * 3), ("vcom", C.c_double * 3),
+                ("r2", C.c_double * 3)]
+
+
+# helium mass in K^-1 A^-2 (Dynamics_TDDFT/DFT4He3d.f90:103,315: 4.002572164 u times mp_u)
+MHE = 4.002572164 * 0.02061484
+
+
 class Job(C.Structure):
     """hedft_job: one droplet + classical-impurity TDDFT run for the serving loop."""
     _fields_ = [("arrival_s", C.c_double), ("n", C.c_int32), ("precision", C.c_int32),
@@ -149,6 +158,7 @@ def lib():
                                               vp, ip, vp, C.c_size_t]
         L.hedft_write_wavefunction.argtypes = [C.c_char_p, C.POINTER(DenFile), ip, vp, vp, vp, C.c_size_t,
                                                C.c_char_p]
+        L.hedft_compute_observables.argtypes = [vp, ip, d, C.POINTER(Observables)]
         L.hedft_set_quantum_impurity.argtypes = [vp, vp, C.c_size_t, d, d]
         L.hedft_quantum_centroid.argtypes = [vp, dp, dp]
         L.hedft_job_bytes.restype = C.c_size_t
@@ -300,6 +310,13 @@ class Droplet:
             return _ptr(flat)
         check(lib().hedft_set_impurity_state(self._c, conv(rimp), conv(vimp), conv(aimp)),
               "set_impurity_state")
+
+    def observables(self, n4, mhe=MHE) -> dict:
+        """Observables block of the TDDFT driver: norm, rcm, l, vcom, r2."""
+        o = Observables()
+        check(lib().hedft_compute_observables(self._c, int(n4), float(mhe), C.byref(o)), "observables")
+        return {"norm": o.norm, "rcm": np.array(list(o.rcm)), "l": np.array(list(o.l)),
+                "vcom": np.array(list(o.vcom)), "r2": np.array(list(o.r2))}
 
     def set_quantum_impurity(self, uimp_k, m_imp_u, mu4=0.0):
         """One quantum impurity: uimp_k on the half spectrum (nz, ny, nx//2+1), mass in u."""

@@ -130,6 +130,17 @@ typedef struct hedft_energies {
   double norm;                                      /* sum(den)*dxyz         */
 } hedft_energies;
 
+/* Droplet observables printed at the energy points of the TDDFT driver
+ * (Dynamics_TDDFT/DFT4He3d.f90:709-836, r_cm.f90:1-30). */
+typedef struct hedft_observables {
+  double norm;     /* auxn4 = sum(den)*dxyz                                  */
+  double rcm[3];   /* r_cm(den, n4): sum(den*r)*dxyz/n4, n4 the nominal N    */
+  double l[3];     /* <Lx,Ly,Lz> about rcm: real part of sum(i conj(psi)     */
+                   /* ((r-rcm) x grad) psi)*dxyz, first-derivative stencils  */
+  double vcom[3];  /* real part of sum(-i conj(psi) grad psi)*dxyz/(norm*mhe) */
+  double r2[3];    /* sum(den*x^2)*dxyz, y, z (about the origin)             */
+} hedft_observables;
+
 typedef struct hedft_ctx hedft_ctx;
 
 const char* hedft_strerror(int rc);
@@ -177,6 +188,12 @@ int hedft_term_alfa(hedft_ctx* c);
 /* energy(): per-term energies (He_statics/energy.f90:12-224,
  * Dynamics_TDDFT/energy.f90:12-182). Synchronises. */
 int hedft_energy(hedft_ctx* c, hedft_energies* e);
+/* The observables block of the TDDFT driver (Dynamics_TDDFT/DFT4He3d.f90:727-836):
+ * centre of mass (r_cm with the nominal atom number n4), angular momentum about
+ * it, centre-of-mass velocity (mhe = 4.002572164*mp_u in K^-1 A^-2 in the
+ * reference, DFT4He3d.f90:103,315) and <x^2>, <y^2>, <z^2>.  TDDFT contexts.
+ * Synchronises. */
+int hedft_compute_observables(hedft_ctx* c, int32_t n4, double mhe, hedft_observables* o);
 
 /* ---- L3: integrators -------------------------------------------------- */
 /* STEPRKR(deltat,mu4,mu4err,n4), He_statics/steprkr.f90:1-96. */

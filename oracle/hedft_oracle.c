@@ -1413,6 +1413,53 @@ void oracle_energy(octx* c, double* out) {
 }
 
 /* ======================================================================= */
+/*  Droplet observables (Dynamics_TDDFT/DFT4He3d.f90:709-836, r_cm.f90)     */
+/* ======================================================================= */
+/* out: norm, rcm[3], l[3], vcom[3], r2[3] (struct hedft_observables order).
+ * Restated loop by loop: r_cm with the nominal n4, then the three angular
+ * momentum sums about rcm, the three COM velocity sums and <x^2>, <y^2>, <z^2>;
+ * the complex sums keep their real part (the reference stores them in reals). */
+void oracle_observables(octx* c, int n4, double mhe, double* out) {
+  size_t n = c->n;
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  const cplx* psi = (const cplx*)c->psi;
+  cplx* g[3];
+  for (int a = 0; a < 3; ++a) g[a] = (cplx*)dalloc(2 * n);
+  for (int a = 0; a < 3; ++a) deriv(c, 1, a + 1, c->psi, (double*)g[a], 2);
+  double sden = 0, cm[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t i = ix + (size_t)nx * (iy + (size_t)ny * iz);
+        double r = c->den[i];
+        sden += r;
+        cm[0] += r * c->x[ix]; cm[1] += r * c->y[iy]; cm[2] += r * c->z[iz];
+        r2[0] += r * c->x[ix] * c->x[ix]; r2[1] += r * c->y[iy] * c->y[iy]; r2[2] += r * c->z[iz] * c->z[iz];
+      }
+  double dv = c->dxyz;
+  double norm = sden * dv;
+  for (int a = 0; a < 3; ++a) cm[a] = cm[a] * dv / n4;
+  cplx lz = 0, ly = 0, lx = 0, v[3] = {0, 0, 0};
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t i = ix + (size_t)nx * (iy + (size_t)ny * iz);
+        double xx = c->x[ix] - cm[0], yy = c->y[iy] - cm[1], zz = c->z[iz] - cm[2];
+        cplx w = I * conj(psi[i]);
+        lz += w * (yy * g[0][i] - xx * g[1][i]);
+        ly += w * (xx * g[2][i] - zz * g[0][i]);
+        lx += w * (zz * g[1][i] - yy * g[2][i]);
+        for (int a = 0; a < 3; ++a) v[a] += -I * conj(psi[i]) * g[a][i];
+      }
+  out[0] = norm;
+  for (int a = 0; a < 3; ++a) out[1 + a] = cm[a];
+  out[4] = creal(lx) * dv; out[5] = creal(ly) * dv; out[6] = creal(lz) * dv;
+  for (int a = 0; a < 3; ++a) out[7 + a] = creal(v[a]) * dv / (norm * mhe);
+  for (int a = 0; a < 3; ++a) out[10 + a] = r2[a] * dv;
+  for (int a = 0; a < 3; ++a) free(g[a]);
+}
+
+/* ======================================================================= */
 /*  Driver glue                                                             */
 /* ======================================================================= */
 /* BCN4HeDFT.f90:724-874: normalise, psi = sqrt(den), fden, dencg, denalf,

@@ -30,6 +30,7 @@ FIELDS = [
     "HPSIXOLD2", "PCX", "QX",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
+MHE = 4.002572164 * 0.02061484  # helium mass in K^-1 A^-2 (Dynamics_TDDFT/DFT4He3d.f90:103,315)
 ENERGY_NAMES = ["ekin4", "elj4", "ecor4", "ealphas", "esolid", "eext", "etot4", "ekinx", "eHeX",
                 "eimpu_impu", "eimpu", "etot", "norm"]
 
@@ -120,6 +121,7 @@ def lib():
         L.oracle_tddft_iteration.argtypes = [vp, C.c_double, ip, ip, dp, dp, dp]
         L.oracle_set_quantum_impurity.argtypes = [vp, vp, C.c_double, C.c_double]
         L.oracle_quantum_centroid.argtypes = [vp, dp, vp]
+        L.oracle_observables.argtypes = [vp, ip, C.c_double, vp]
         L.oracle_num_threads.restype = ip
         _lib = L
     return _lib
@@ -404,6 +406,12 @@ class Oracle:
         a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
         lib().oracle_tddft_iteration(self._c, dt, it, nrk, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
+
+    def observables(self, n4, mhe=MHE):
+        out = np.zeros(13)
+        lib().oracle_observables(self._c, int(n4), float(mhe), _ptr(out))
+        return {"norm": out[0], "rcm": out[1:4].copy(), "l": out[4:7].copy(), "vcom": out[7:10].copy(),
+                "r2": out[10:13].copy()}
 
     def set_quantum_impurity(self, uimp_k, m_imp_u, mu4=0.0):
         k = np.ascontiguousarray(np.broadcast_to(np.asarray(uimp_k, dtype=np.float64), self.hshape))
