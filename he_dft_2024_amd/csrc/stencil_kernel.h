@@ -8,6 +8,7 @@
 // to an epilogue functor that stores it or fuses the consumer's arithmetic.
 // Input functors fuse the producer's arithmetic into the LDS fill.
 #pragma once
+#include <cstdlib>
 #include <type_traits>
 
 #include "common.h"
@@ -374,6 +375,142 @@ __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef c
   if constexpr (NR > 0) block_reduce_store<NR>(red, out.partials);
 }
 
+// 13-point derivative at global index g of an axis of length n, from a tile
+// whose local element k holds global index g0 - 6 + k (stride s); halos are
+// preloaded (wrapped with the antiperiodic sign), so no index wraps here.
+template <typename V>
+__device__ __forceinline__ acc_t<V> stencil_tile(const StencilCoef& cf, const V* P, int s, int g, int n, int g0) {
+  acc_t<V> acc = vzero<acc_t<V>>();
+  const int off = 6 - g0;
+  if (cf.mode != 0 || (g >= 6 && g < n - 6)) {
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.c[j], P[(g - 6 + j + off) * s], acc);
+  } else if (g < 6) {
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.rows[g][j], P[(j + off) * s], acc);
+  } else {
+    const int r = g - n + 12;
+#pragma unroll
+    for (int j = 0; j < 13; ++j) acc = vfma(cf.rows[r][j], P[(n - 13 + j + off) * s], acc);
+  }
+  return acc;
+}
+
+// x + y derivatives on 2-D tiles: TX columns x TY rows of one z-plane with
+// 6-point halos on all four sides, so that wide tiles (few halo re-reads)
+// fit in LDS at any ny.  SAME: one (TY+12) x (TX+12) rectangle serves both
+// derivatives; otherwise the x operand is staged on TY rows x (TX+12) and the
+// y operand on (TY+12) rows x TX.  Requires ny % TY == 0 (every row chunk then
+// holds the 13 rows a one-sided boundary stencil reads) and Out::NR == 0.
+template <typename V, bool SAME, int TX, int TY, class In, class Out>
+__global__ void __launch_bounds__(256) k_pencil_xy2(In in, Out out, StencilCoef cfx, StencilCoef cfy, Dims d) {
+  extern __shared__ unsigned char smem_raw[];
+  static_assert(Out::NR == 0, "2-D tiles carry no reductions");
+  constexpr int TW = TX + 12, TH = TY + 12, TPC = 256 / TX, RUN = TY / TPC;
+  static_assert(TY % TPC == 0, "rows per thread");
+  const int nx = d.nx, ny = d.ny;
+  V* A = reinterpret_cast<V*>(smem_raw);       // x operand, column 0 <-> x0 - 6
+  V* B = SAME ? A + 6 : A + TY * TW;           // y operand, row 0 <-> y0 - 6
+  constexpr int SB = SAME ? TW : TX;           // row stride of the y operand
+  // XCD-aware tile order (see k_pencil_xy): x tiles, then y tiles, then planes
+  const int ntx = gridDim.x, nty = gridDim.y;
+  const int ntiles = ntx * nty * gridDim.z;
+  int tile = blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z);
+  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+  const int x0 = (tile % ntx) * TX, y0 = ((tile / ntx) % nty) * TY, z = tile / (ntx * nty);
+  const size_t zb = (size_t)z * nx * ny;
+  const double sgx = (cfx.mode == 2) ? -1.0 : 1.0, sgy = (cfy.mode == 2) ? -1.0 : 1.0;
+  // global index -> in-range index and sign: periodic halos wrap (with the
+  // antiperiodic sign), others are clamped (never read by any output)
+  auto fold = [](int g, int n, int mode, double sgw, double& sg) {
+    if (g < 0 || g >= n) {
+      if (mode != 0) {
+        sg *= sgw;
+        return g + ((g < 0) ? n : -n);
+      }
+      return min(max(g, 0), n - 1);
+    }
+    return g;
+  };
+  {
+    constexpr int U = 8;
+    // x operand (SAME: the whole rectangle)
+    constexpr int RA = SAME ? TH : TY;
+    for (int e0 = threadIdx.x; e0 < RA * TW; e0 += 256 * U) {
+      V ta[U];
+      double sg[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = min(e0 + u * 256, RA * TW - 1);
+        const int r = e / TW, c = e - r * TW;
+        sg[u] = 1.0;
+        const int gy = fold(SAME ? y0 - 6 + r : y0 + r, ny, cfy.mode, sgy, sg[u]);
+        const int gx = fold(x0 - 6 + c, nx, cfx.mode, sgx, sg[u]);
+        ta[u] = in.loadx(zb + (size_t)gy * nx + gx);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * 256;
+        if (e < RA * TW) A[e] = vmul(sg[u], ta[u]);
+      }
+    }
+    if constexpr (!SAME) {
+      for (int e0 = threadIdx.x; e0 < TH * TX; e0 += 256 * U) {
+        V tb[U];
+        double sg[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = min(e0 + u * 256, TH * TX - 1);
+          const int r = e / TX, c = e - r * TX;
+          sg[u] = 1.0;
+          const int gy = fold(y0 - 6 + r, ny, cfy.mode, sgy, sg[u]);
+          tb[u] = in.loady(zb + (size_t)gy * nx + min(x0 + c, nx - 1));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = e0 + u * 256;
+          if (e < TH * TX) B[e] = vmul(sg[u], tb[u]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int gx = x0 + xl;
+  if (gx < nx) {
+#pragma unroll 2
+    for (int k = 0; k < RUN; ++k) {
+      const int y = y0 + t * RUN + k;
+      const V* rowx = A + (size_t)(y - y0 + (SAME ? 6 : 0)) * TW;
+      const acc_t<V> dx = stencil_tile<V>(cfx, rowx, 1, gx, nx, x0);
+      const acc_t<V> dy = stencil_tile<V>(cfy, B + xl, SB, y, ny, y0);
+      double* no_red = nullptr;
+      out.store(zb + (size_t)y * nx + gx, gx, y, z, dx, dy, no_red);
+    }
+  }
+}
+
+// tile shape of the 2-D variant for a value type: (TX, TY)
+template <typename V>
+struct XYTile {
+  static constexpr int TX = (sizeof(V) == 4) ? 64 : (sizeof(V) == 8) ? 64 : 32;
+  static constexpr int TY = (sizeof(V) == 4) ? 64 : 32;
+};
+
+// 1 when the 2-D tiles are used for this grid (ny a multiple of TY, and a
+// last x tile of at least 7 columns); HEDFT_XY_TILE=0 in the environment
+// selects the full-column kernel
+template <typename V>
+inline bool use_xy2(const Dims& d) {
+  static const int enabled = [] {
+    const char* e = std::getenv("HEDFT_XY_TILE");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  // the last x tile must hold the 13 columns a one-sided boundary stencil reads
+  const int rx = d.nx % XYTile<V>::TX;
+  return enabled && d.ny % XYTile<V>::TY == 0 && d.nx >= 13 && (rx == 0 || rx >= 7);
+}
+
 template <typename V, bool SAME>
 inline PencilGeom pencil_xy_geom(const Dims& d) {
   PencilGeom g;
@@ -405,6 +542,16 @@ inline void launch_pencil_xy_t(hipStream_t s, const PencilGeom& g, const Dims& d
 template <typename V, bool SAME, class In, class Out>
 inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cfx, const StencilCoef& cfy,
                              const In& in, const Out& out) {
+  if constexpr (Out::NR == 0) {
+    if (use_xy2<V>(d)) {
+      constexpr int TX = XYTile<V>::TX, TY = XYTile<V>::TY;
+      const dim3 grid((d.nx + TX - 1) / TX, d.ny / TY, d.nz);
+      const size_t lds = sizeof(V) * (SAME ? (size_t)(TY + 12) * (TX + 12)
+                                           : (size_t)TY * (TX + 12) + (size_t)(TY + 12) * TX);
+      hipLaunchKernelGGL((k_pencil_xy2<V, SAME, TX, TY, In, Out>), grid, dim3(256), lds, s, in, out, cfx, cfy, d);
+      return;
+    }
+  }
   PencilGeom g = pencil_xy_geom<V, SAME>(d);
   if (g.tx == 64) launch_pencil_xy_t<V, SAME, 64>(s, g, d, cfx, cfy, in, out);
   else if (g.tx == 32) launch_pencil_xy_t<V, SAME, 32>(s, g, d, cfx, cfy, in, out);

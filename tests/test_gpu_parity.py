@@ -244,3 +244,31 @@ def test_statics_steepest_descent_vs_oracle(gpu):
     for k in ("ekin4", "elj4", "ecor4", "ealphas", "etot", "norm"):
         assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
     d.close()
+
+
+@pytest.mark.parametrize("icon", [13, 0, 8, 18, 24])
+def test_xy_stencil_paths_all_icons_vs_oracle(gpu, icon):
+    """The x+y stencil kernels (grad den, alpha_s divergence, Laplacian of psi,
+    |grad psi|^2) on a non-cubic grid with every boundary mode: poten, one RK
+    step and the energy against the oracle."""
+    nx, ny, nz, h, n4 = 64, 96, 32, 0.4, 40.0
+    _say("xy stencils icon", icon, (nx, ny, nz))
+    import he_dft_2024_amd as H
+    xm, ym, zm = 0.5 * nx * h, 0.5 * ny * h, 0.5 * nz * h
+    d = H.Droplet(nx, ny, nz, xm, ym, zm, H.default_params(core_otc=1, icon=icon))
+    o = O.Oracle(nx, ny, nz, xm, ym, zm, O.default_params(core_otc=1, icon=icon))
+    den = I.woods_saxon(nx, ny, nz, xm, ym, zm, n4, seed=9, center=(0.8, -1.2, 0.4))
+    dt = 5e-4 / I.PSTOK
+    for x in (d, o):
+        x.set("DEN", den)
+        x.tddft_init()
+        x.poten()
+    for name in ("DXDEN", "DYDEN", "DZDEN", "UALPHAS", "POT4"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_OP, name
+    for x in (d, o):
+        x.steprk(dt)
+    assert relerr(d.field("PSI"), o.field("PSI")) < RTOL_TRAJ
+    eg, eo = d.energy(), o.energy()
+    for k in ("ekin4", "ealphas", "etot"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]), k
+    d.close()
