@@ -280,7 +280,8 @@ def batched_leg(H, I, a, dt, plain_steps_per_s):
             "steps_each": a.batch_steps, "ms": round(ms, 3), "dtype": "f32 field / f64 psi",
             "speedup_vs_fp64_plain": round(agg / plain_steps_per_s, 3),
             "clamped_droplets": sum(1 for e in ens if e["clamped"]), "nonfinite": bad,
-            "autotuned": {k: tuning[k] for k in ("xy_split_fwd", "xy_split_inv", "no_side", "prefer_eager")}}
+            "autotuned": {k: tuning[k] for k in ("xy_split_fwd", "xy_split_inv", "no_side", "prefer_eager",
+                                                 "lap_late")}}
 
 
 def serving_leg(H, I, a, dt):

@@ -261,6 +261,7 @@ struct hedft_ctx {
   int no_side;         // 1: run the side branches on the main stream
   int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
+  int lap_late;        // 1: the next step's x+y Laplacians run beside stages B/C, 0: beside the den convolutions
   // quantum impurity (Dynamics_quantum_impurities): one impurity wave function
   // psix with its own PC history, coupled through the real half-spectrum
   // kernel uimpk: potx4 = F^-1[F(|psix|^2) uimpk] enters pot4, upotx =

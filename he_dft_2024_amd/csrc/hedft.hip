@@ -145,18 +145,17 @@ struct Lam {
 // k = timec*((lap)*h2o2m4 - (pot4-mu4)*psi) - i*uimp*psi   (steprk.f90:67,
 // steppc.f90:432,499; mu4 and no uimp in Dynamics_quantum_impurities/steprk.f90:68,
 // steppc.f90:46,105; mu4 = 0 for classical impurities)
+// ps is psi at the point, taken from the stencil's LDS tile (not re-read)
 template <typename R>
 struct HpsiC {
   const double2* lapxy;
-  const double2* psi;
   const R* pot4;
   Lam lam;
   ImpDev im;
   double h2o2m4, mu4;
-  __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2& ps) const {
+  __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2 ps) const {
     const double2 l0 = lapxy[i];
     const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
-    ps = psi[i];
     const double v = (double)pot4[i] - mu4;
     const double2 t = make_double2(lap.x * h2o2m4 - v * ps.x, lap.y * h2o2m4 - v * ps.y);
     const double lr = lam.at(ix, iy, iz);
@@ -173,14 +172,12 @@ struct HpsiC {
 template <typename R>
 struct HpsiX {
   const double2* lapxy;
-  const double2* psi;
   const R* upot;
   Lam lam;
   double h2o2m;
-  __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2& ps) const {
+  __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2 ps) const {
     const double2 l0 = lapxy[i];
     const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
-    ps = psi[i];
     const double v = (double)upot[i];
     const double2 t = make_double2(lap.x * h2o2m - v * ps.x, lap.y * h2o2m - v * ps.y);
     const double lr = lam.at(ix, iy, iz);
@@ -199,14 +196,14 @@ template <typename R, typename H = HpsiC<R>>
 struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
+  static constexpr bool CENTRE = true;  // store() receives psi at the point from the LDS tile
   double* partials;
   H h;
   double2 *psi, *q, *po0, *po1, *po2, *ho0, *ho1;
   R* den;
   double a, b, cc, dt;
   int first;
-  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double*) const {
-    double2 ps;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double*) const {
     const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
     const double2 qq = q[i];
     const double2 s1 = make_double2(a * (k4.x - b * qq.x), a * (k4.y - b * qq.y));
@@ -228,13 +225,13 @@ template <typename R, typename H = HpsiC<R>>
 struct OutPred {  // predictor + modifier (steppc.f90:432-442)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
+  static constexpr bool CENTRE = true;  // store() receives psi at the point from the LDS tile
   double* partials;
   H h;
   double2 *psi, *pc, *po3, *ho1, *ho2;
   R* den;
   double dt;
-  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double*) const {
-    double2 ps;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double*) const {
     const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
     const double2 o3 = po3[i], h1 = ho1[i], h2 = ho2[i];
     const double f = C4O3 * dt;
@@ -254,13 +251,13 @@ template <typename R, typename H = HpsiC<R>>
 struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
   static constexpr int NR = 1;
   static constexpr int KB = HEDFT_KB_STEP;
+  static constexpr bool CENTRE = true;  // store() receives psi at the point from the LDS tile
   double* partials;
   H h;
   double2 *psi, *pc, *po3, *po2, *ho1, *ho2;
   R* den;
   double dt;
-  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double* red) const {
-    double2 ps;
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double* red) const {
     const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
     const double2 o3 = po3[i], o2 = po2[i], h1 = ho1[i], h2 = ho2[i];
     const double f = 3.0 * dt;
@@ -1022,10 +1019,10 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     // the quantum-impurity poten calls derden only for the OTC core
     // (Dynamics_quantum_impurities/poten.f90:48): nothing else reads grad(den) there
     if (!c->quantum || c->p.core_otc) HEDFT_TRY(derden_force(c, want_force, want_force));
-    if (with_lap) HEDFT_TRY(lap_xy(c, c->psi, 1));
-    if (c->quantum) {
-      if (with_lap) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
-      HEDFT_TRY(potx4_conv<R>(c, c->potx4));
+    if (c->quantum) HEDFT_TRY(potx4_conv<R>(c, c->potx4));
+    if (with_lap && !c->lap_late) {
+      HEDFT_TRY(lap_xy(c, c->psi, 1));
+      if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
     }
   }
   XInT<R> xi{};
@@ -1061,9 +1058,22 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   }
   HEDFT_TRY(inv_yx<R>(c, nk, so, xo));
   HEDFT_TRY(join(c));
+  // the x+y Laplacians for the next step run beside stages B and C (they read
+  // only psi and psix, which the mean field does not change); stage C's own
+  // branch and its join come after them on the side stream
+  const bool late = with_lap && c->lap_late;
+  if (late) {
+    HEDFT_TRY(fork(c));
+    {
+      SideBranch sb(c);
+      HEDFT_TRY(lap_xy(c, c->psi, 1));
+      if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+    }
+  }
   if (with_lap) c->lap_fresh = 1;
   HEDFT_TRY(stage_b<R>(c));
   if (c->p.core_otc) HEDFT_TRY(stage_c<R>(c));
+  else if (late) HEDFT_TRY(join(c));
   return HEDFT_OK;
 }
 
@@ -1132,7 +1142,6 @@ template <typename R>
 static HpsiC<R> make_hpsic(hedft_ctx* c) {
   HpsiC<R> h;
   h.lapxy = (const double2*)c->lap;
-  h.psi = (const double2*)c->psi;
   h.pot4 = fld<R>(c->pot4);
   h.lam = make_lam(c);
   h.im = make_impdev(c);
@@ -1145,7 +1154,6 @@ template <typename R>
 static HpsiX<R> make_hpsix(hedft_ctx* c) {
   HpsiX<R> h;
   h.lapxy = (const double2*)c->lapx;
-  h.psi = (const double2*)c->psix;
   h.upot = fld<R>(c->upotx);
   h.lam = make_lam(c);
   h.h2o2m = c->h2o2mimp;
@@ -1639,6 +1647,8 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   c->mode = mode;
   c->mixed = (p->precision == HEDFT_PREC_MIXED);
   c->zconv_fused = 1;
+  c->lap_late = 0;  // measured: the Laplacians beside the den convolutions keep stage B's chain shorter
+  if (const char* e = std::getenv("HEDFT_LAP_LATE")) c->lap_late = std::atoi(e);
   if (const char* e = std::getenv("HEDFT_XY_SPLIT_NB")) {  // tuning override: "fwd,inv"
     c->xy_split_nb[0] = std::atoi(e);
     const char* comma = std::strchr(e, ',');
@@ -2574,6 +2584,7 @@ static void apply_tuning(hedft_ctx* c, const hedft_tuning* t) {
   c->xy_split_nb[1] = t->xy_split_inv;
   c->no_side = t->no_side;
   c->prefer_eager = t->prefer_eager;
+  c->lap_late = t->lap_late;
   c->graph_valid = 0;  // the captured graph encodes the launch structure
 }
 
@@ -2589,6 +2600,8 @@ int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t) {
   t->xy_split_inv = c->xy_split_nb[1];
   t->no_side = c->no_side;
   t->prefer_eager = c->prefer_eager;
+  t->lap_late = c->lap_late;
+  t->pad_ = 0;
   t->ms_per_step = 0.0;
   return HEDFT_OK;
 }
@@ -2603,22 +2616,24 @@ int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tunin
   const int split[4][2] = {{0, 0}, {1, 1}, {1, 0}, {0, 1}};
   for (int sp = 0; sp < 4; ++sp)
     for (int ns = 0; ns < 2; ++ns)
-      for (int eg = 0; eg < 2; ++eg) {
-        cur.xy_split_fwd = split[sp][0];
-        cur.xy_split_inv = split[sp][1];
-        cur.no_side = ns;
-        cur.prefer_eager = eg;
-        apply_tuning(c, &cur);
-        HEDFT_TRY(enqueue_pc(c, deltat, PC_PERIOD, 1));  // warm-up incl. graph capture
-        HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
-        HEDFT_TRY(enqueue_pc(c, deltat, k, 1));
-        HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
-        HEDFT_CHECK(hipEventSynchronize(c->ev1));
-        float ms = 0.f;
-        HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        cur.ms_per_step = ms / k;
-        if (cur.ms_per_step < win.ms_per_step) win = cur;
-      }
+      for (int eg = 0; eg < 2; ++eg)
+        for (int ll = 0; ll < 2; ++ll) {
+          cur.xy_split_fwd = split[sp][0];
+          cur.xy_split_inv = split[sp][1];
+          cur.no_side = ns;
+          cur.prefer_eager = eg;
+          cur.lap_late = ll;
+          apply_tuning(c, &cur);
+          HEDFT_TRY(enqueue_pc(c, deltat, PC_PERIOD, 1));  // warm-up incl. graph capture
+          HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
+          HEDFT_TRY(enqueue_pc(c, deltat, k, 1));
+          HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
+          HEDFT_CHECK(hipEventSynchronize(c->ev1));
+          float ms = 0.f;
+          HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+          cur.ms_per_step = ms / k;
+          if (cur.ms_per_step < win.ms_per_step) win = cur;
+        }
   apply_tuning(c, &win);
   if (best) *best = win;
   return done(c);

@@ -134,17 +134,17 @@ int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int i
 // accumulates the He part of the impurity force F_k = -sum(grad(den) U_k) dxyz
 // (potenimp.f90:49-52), evaluating U_k once per point.  R is the storage type
 // of the density fields (double, or float on the mixed path).
-template <typename R>
+// WANT = false: the gradient alone (no accumulators, fewer registers)
+template <typename R, bool WANT>
 struct OutGradZForce {
-  static constexpr int NR = 3 * HEDFT_MAX_IMP;
+  static constexpr int NR = WANT ? 3 * HEDFT_MAX_IMP : 0;
   R* out;
   const R *gx, *gy;
   double* partials;
   ImpDev im;
-  int want;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double v, double* red) const {
     out[i] = (R)v;
-    if (want) {
+    if constexpr (WANT) {
       double xx, yy, zz;
       imp_xyz(im, ix, iy, iz, xx, yy, zz);
       const double a = gx[i], b = gy[i];
@@ -171,8 +171,12 @@ static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp) {
   R* dz = reinterpret_cast<R*>(c->dzden);
   InPlain<R> ip{den};
   launch_pencil_xy<R, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ip, OutTwo<R>{dx, dy, nullptr});
-  OutGradZForce<R> op{dz, dx, dy, force_partials(c), make_impdev(c), want_force};
-  launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip, op);
+  if (want_force)
+    launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip,
+                        OutGradZForce<R, true>{dz, dx, dy, force_partials(c), make_impdev(c)});
+  else
+    launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip,
+                        OutGradZForce<R, false>{dz, dx, dy, nullptr, make_impdev(c)});
   if (hipGetLastError() != hipSuccess) return HEDFT_EHIP;
   return want_force ? force_finalize(c, pencil_geom<R>(c->d, 2).nblocks, update_aimp) : HEDFT_OK;
 }

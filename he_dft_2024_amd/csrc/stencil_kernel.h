@@ -127,6 +127,17 @@ struct out_kb<Out, std::void_t<decltype(Out::KB)>> {
   static constexpr int value = Out::KB;
 };
 
+// Epilogues that declare CENTRE = true receive the pencil's value at the
+// output point (from the LDS tile) with the derivative.
+template <class Out, class = void>
+struct out_centre {
+  static constexpr bool value = false;
+};
+template <class Out>
+struct out_centre<Out, std::void_t<decltype(Out::CENTRE)>> {
+  static constexpr bool value = Out::CENTRE;
+};
+
 // Generic pencil kernel.  AX: 0 = x, 1 = y, 2 = z.  In::load(idx) -> V,
 // Out::store(idx, ix, iy, iz, d, red) with Out::NR per-thread partial sums
 // reduced per workgroup into Out::partials.  For AX = 1, 2 a workgroup owns TX
@@ -156,7 +167,10 @@ __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, 
       const int row = row0 + r;
       if (row < nrows) {
         const acc_t<V> v = stencil_at<V>(cf, lds + r * n, 1, i, n);
-        out.store((size_t)row * n + i, i, row % d.ny, row / d.ny, v, red);
+        if constexpr (out_centre<Out>::value)
+          out.store((size_t)row * n + i, i, row % d.ny, row / d.ny, v, lds[e], red);
+        else
+          out.store((size_t)row * n + i, i, row % d.ny, row / d.ny, v, red);
       }
     }
   } else {
@@ -191,7 +205,10 @@ __global__ void __launch_bounds__(256) k_pencil(In in, Out out, StencilCoef cf, 
             const int i = i0 + k;
             const int iy = (AX == 1) ? i : (int)blockIdx.y;
             const int iz = (AX == 1) ? (int)blockIdx.y : i;
-            out.store(base + i * stride + xl, x0 + xl, iy, iz, v[k], red);
+            if constexpr (out_centre<Out>::value)
+              out.store(base + i * stride + xl, x0 + xl, iy, iz, v[k], lds[i * TX + xl], red);
+            else
+              out.store(base + i * stride + xl, x0 + xl, iy, iz, v[k], red);
           }
         }
       }

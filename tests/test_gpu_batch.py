@@ -65,9 +65,10 @@ def test_tuning_choices_keep_results(gpu):
     the split FFT passes."""
     print("[case] tuning choices", flush=True)
     ref = None
-    for t in ({"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0},
-              {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 1, "prefer_eager": 1},
-              {"xy_split_fwd": 4, "xy_split_inv": 4, "no_side": 0, "prefer_eager": 0}):
+    for t in ({"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0, "lap_late": 0},
+              {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 1, "prefer_eager": 1, "lap_late": 0},
+              {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0, "lap_late": 1},
+              {"xy_split_fwd": 4, "xy_split_inv": 4, "no_side": 0, "prefer_eager": 0, "lap_late": 0}):
         d = _make(gpu, CASES[1])
         d.set_tuning(t)
         d.run_pc(DT, 12, True)
