@@ -14,10 +14,12 @@ from he_dft_2024_amd import inputs as I
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FDIR = os.path.join(ROOT, "he_dft_2024_amd", "fortran")
 EXE = os.path.join(FDIR, "_build", "hedft_fortran_check")
+QEXE = os.path.join(FDIR, "_build", "hedft_fortran_qcheck")
 
 
 def test_fortran_sources_present():
-    for f in ("hedft_gpu.f90", "hedft_fortran_check.f90", "hedft_dropin_tddft.f90", "Makefile"):
+    for f in ("hedft_gpu.f90", "hedft_fortran_check.f90", "hedft_fortran_qcheck.f90", "hedft_dropin_tddft.f90",
+              "hedft_dropin_quantum.f90", "Makefile"):
         assert os.path.exists(os.path.join(FDIR, f)), f
 
 
@@ -61,3 +63,47 @@ def test_fortran_driver_matches_python_binding(gpu, tmp_path):
     assert err_f == e[0]
     assert etot_f == en["etot"] and ekin_f == en["ekin4"] and norm_f == en["norm"]
     assert np.array_equal(psi_f, psi)
+
+
+@pytest.mark.gpu
+def test_fortran_quantum_loop_matches_python_binding(gpu, tmp_path):
+    """The quantum-impurity loop driven from Fortran equals the Python binding's,
+    bit for bit (same entry points, same order)."""
+    if not os.path.exists(QEXE):
+        pytest.skip("Fortran check program not built (amdflang absent at build time)")
+    from he_dft_2024_amd import quantum as Q
+    n, xmax, n4, niter, nrk = 32, 6.4, 60.0, 6, 3
+    dt, m_imp, mu4 = 5e-4 / I.PSTOK, 2 * 4.002603, -0.7
+    print("[case] fortran quantum loop 32^3 3 RK + 3 PC", flush=True)
+    v = Q.lennard_jones(10.22, 2.556)
+    uk = Q.uimp_k_table(n, n, n, xmax, xmax, xmax, v, 2.0, float(v(2.0)))
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=6)
+    psix = I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(0.5, -0.4, 1.0), width=1.3, k=(0.4, 0.0, 0.2))
+    fin, fout = tmp_path / "qin.bin", tmp_path / "qout.bin"
+    with open(fin, "wb") as f:
+        np.array([n, niter, nrk], dtype=np.int32).tofile(f)
+        np.array([xmax, dt, m_imp, mu4], dtype=np.float64).tofile(f)
+        np.ascontiguousarray(uk, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(den, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(psix, dtype=np.complex128).tofile(f)
+    r = subprocess.run([QEXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    scal = raw[:9]
+    psix_f = raw[9:].view(np.complex128).reshape(n, n, n)
+
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(), mode=gpu.MODE_TDDFT)
+    d.set_quantum_impurity(uk, m_imp, mu4)
+    d.set("DEN", den)
+    d.set("PSIX", psix)
+    d.tddft_init()
+    for it in range(1, niter + 1):
+        e = d.tddft_iteration(dt, it, nrk)
+    en = d.energy()
+    nrm, rc = d.quantum_centroid()
+    px = d.field("PSIX")
+    d.close()
+    assert scal[0] == e[0] and scal[1] == e[1]
+    assert scal[2] == en["etot"] and scal[3] == en["ekinx"] and scal[4] == en["eHeX"]
+    assert scal[5] == nrm and np.array_equal(scal[6:9], rc)
+    assert np.array_equal(psix_f, px)
