@@ -64,12 +64,12 @@ MHE = 4.002572164 * 0.02061484
 
 
 class Job(C.Structure):
-    """hedft_job: one droplet + classical-impurity TDDFT run for the serving loop."""
+    """hedft_job: one droplet + impurity TDDFT run for the serving loop."""
     _fields_ = [("arrival_s", C.c_double), ("n", C.c_int32), ("precision", C.c_int32),
                 ("nsteps", C.c_int32), ("nrk", C.c_int32), ("h", C.c_double), ("n4", C.c_double),
                 ("dt", C.c_double), ("r0", C.c_double * 3), ("v0", C.c_double * 3),
                 ("mass_amu", C.c_double), ("eps", C.c_double), ("sigma", C.c_double),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("quantum", C.c_int32), ("pad_", C.c_int32), ("width", C.c_double)]
 
 
 class JobResult(C.Structure):
@@ -418,9 +418,11 @@ def batch_run_pc(droplets, dt, nsteps, chunk=6):
 
 
 def make_job(arrival_s=0.0, n=32, h=0.4, n4=60.0, nsteps=30, nrk=3, dt=5e-4 / 7.63822291, precision=PREC_FP64,
-             r0=(0.0, 0.0, -4.0), v0=(0.0, 0.0, 3.0), mass_amu=108.0, eps=20.0, sigma=4.0, seed=0):
+             r0=(0.0, 0.0, -4.0), v0=(0.0, 0.0, 3.0), mass_amu=108.0, eps=20.0, sigma=4.0, seed=0, quantum=False,
+             width=1.5):
+    """hedft_job: classical impurity, or with quantum=True a wave packet of the given width."""
     return Job(arrival_s, n, precision, nsteps, nrk, h, n4, dt, (C.c_double * 3)(*r0), (C.c_double * 3)(*v0),
-               mass_amu, eps, sigma, seed)
+               mass_amu, eps, sigma, seed, int(bool(quantum)), 0, width)
 
 
 def job_bytes(job) -> int:

@@ -328,7 +328,11 @@ int hedft_write_wavefunction(const char* path, const hedft_denfile* hdr, int32_t
  * 1e-3 noise, normalised to N) on a cubic grid with one classical impurity
  * (LJ pair potential capped inside 3 A), started as the TDDFT driver does
  * (DFT4He3d.f90:525-601, nrk RK-Gill iterations) and advanced nsteps PC
- * iterations.  Classical impurities only. */
+ * iterations.  With quantum = 1 the impurity is a wave packet instead
+ * (Dynamics_quantum_impurities): psix = exp(-|r-r0|^2/(2 width^2) + i k.r),
+ * normalised to 1, k = v0/(2 h2o2mimp) (group velocity v0), mass mass_amu,
+ * coupled through uimp_k = the 3-D Fourier transform of the same capped LJ
+ * potential (potenimpini.f90:19-29; closed-form core + Filon quadrature). */
 typedef struct hedft_job {
   double arrival_s;          /* arrival time, seconds after hedft_serve starts */
   int32_t n;                 /* cubic grid edge */
@@ -342,6 +346,9 @@ typedef struct hedft_job {
   double v0[3];              /* impurity velocity (A/ps) */
   double mass_amu, eps, sigma; /* impurity mass, LJ epsilon (K) and sigma (A) */
   uint64_t seed;             /* density noise seed */
+  int32_t quantum;           /* 0: classical impurity, 1: quantum impurity (wave packet) */
+  int32_t pad_;
+  double width;              /* quantum: Gaussian width of the packet (A) */
 } hedft_job;
 
 typedef struct hedft_job_result {
@@ -350,7 +357,7 @@ typedef struct hedft_job_result {
   double admit_s, done_s;    /* seconds since the start of hedft_serve */
   double latency_s;          /* done - arrival */
   double etot, etot4, ekin4, norm; /* energies after the last step */
-  double rimp[3];
+  double rimp[3];            /* impurity position (quantum: centre of |psix|^2) */
 } hedft_job_result;
 
 typedef struct hedft_serve_opts {

@@ -13,11 +13,16 @@ def _jobs(gpu, count=6, seed=11):
     jobs = []
     for k in range(count):
         n = 32 if k % 2 == 0 else 48
-        jobs.append(gpu.make_job(arrival_s=float(arrivals[k]), n=n, n4=float(rng.uniform(40, 120)),
+        # a quantum-impurity job (mass 4 u, packet width 1.5 A) in the mix; its strong
+        # coupling takes the quantum example's step, 5e-5 ps (input.0 of the reference)
+        quantum = (k % 5 == 4)
+        dt = (5e-5 if quantum else 5e-4) / 7.63822291
+        jobs.append(gpu.make_job(arrival_s=float(arrivals[k]), n=n, n4=float(rng.uniform(40, 120)), dt=dt,
                                  nsteps=int(rng.choice([12, 18, 30])), nrk=3,
                                  precision=gpu.PREC_MIXED if k % 3 == 0 else gpu.PREC_FP64,
                                  r0=(float(rng.uniform(0, 1)), 0.0, -4.0),
-                                 v0=(0.0, 0.0, float(rng.uniform(1, 8))), seed=100 + k))
+                                 v0=(0.0, 0.0, float(rng.uniform(1, 8))), seed=100 + k,
+                                 quantum=quantum, mass_amu=4.0 if quantum else 108.0))
     return jobs
 
 
@@ -46,6 +51,8 @@ def test_serve_respects_hbm_budget(gpu):
     for j in jobs:
         j.arrival_s = 0.0
         j.n = 32  # equal footprints: the budget holds two at a time
+        j.quantum = 0
+        j.mass_amu = 108.0
     per = max(gpu.job_bytes(j) for j in jobs)
     res, st = gpu.serve(jobs, hbm_budget=2.2 * per, chunk=6)
     assert st["completed"] == len(jobs)
@@ -60,3 +67,57 @@ def test_serve_rejects_job_larger_than_budget(gpu):
     j = gpu.make_job(n=64)
     with pytest.raises(gpu.HedftError):
         gpu.serve([j], hbm_budget=gpu.job_bytes(j) / 2)
+
+
+def _splitmix_uniform(seed, count):
+    """The serving loop's density noise (SplitMix64, uniform in [0,1))."""
+    m64 = (1 << 64) - 1
+    out = np.empty(count)
+    st = seed
+    for i in range(count):
+        st = (st + 0x9E3779B97F4A7C15) & m64
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m64
+        z ^= z >> 31
+        out[i] = (z >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def test_serve_quantum_job_matches_python_setup(gpu):
+    """A quantum-impurity job built inside hedft_serve (C++ droplet, uimp_k by
+    Filon quadrature, wave packet) equals the same job set up through the
+    Python binding with he_dft_2024_amd.quantum's transform."""
+    from he_dft_2024_amd import inputs as I
+    from he_dft_2024_amd import quantum as Q
+    print("[case] serve quantum job vs python set-up", flush=True)
+    n, h, n4, nsteps, nrk = 32, 0.4, 70.0, 12, 3
+    job = gpu.make_job(n=n, h=h, n4=n4, nsteps=nsteps, nrk=nrk, dt=5e-5 / I.PSTOK, r0=(0.3, -0.2, -3.0),
+                       v0=(0.0, 1.0, 4.0), seed=77, quantum=True, mass_amu=4.0, width=1.5)
+    res, _ = gpu.serve([job], hbm_budget=4 * gpu.job_bytes(job), chunk=6)
+    r = res[0]
+    assert r["status"] == 0
+    xm = 0.5 * n * h
+    x = -xm + h * np.arange(n)
+    rr = np.sqrt(x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2)
+    rho = 0.0218 / (1.0 + np.exp((rr - 2.22 * n4 ** (1.0 / 3.0)) / 0.7))
+    rho = rho * (1.0 + 1e-3 * _splitmix_uniform(77, n ** 3).reshape(n, n, n))
+    rho *= n4 / (rho.sum() * h ** 3)
+    v = Q.lennard_jones(job.eps, job.sigma)
+    uk = Q.uimp_k_table(n, n, n, xm, xm, xm, v, 3.0, float(v(3.0)))
+    h2o2mimp = 6.05969638298 * 4.002603 / 4.0
+    k = np.array(job.v0[:]) * I.PSTOK / (2.0 * h2o2mimp)
+    d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1))
+    d.set("DEN", rho)
+    d.set_quantum_impurity(uk, 4.0, 0.0)
+    d.set("PSIX", I.gaussian_packet(n, n, n, xm, xm, xm, r0=tuple(job.r0[:]), width=1.5, k=tuple(k)))
+    d.tddft_init()
+    for it in range(1, nrk + 1):
+        d.tddft_iteration(job.dt, it, nrk)
+    d.run_pc(job.dt, nsteps)
+    e = d.energy()
+    nrm, rc = d.quantum_centroid()
+    d.close()
+    for key in ("etot", "etot4", "ekin4", "norm"):
+        assert abs(r[key] - e[key]) <= 1e-9 * abs(e[key]), (key, r[key], e[key])
+    assert np.abs(np.array(r["rimp"]) - rc).max() < 1e-9
