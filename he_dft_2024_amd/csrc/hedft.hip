@@ -100,19 +100,20 @@ static void finalize(hedft_ctx* c, const double* part, size_t nblocks, int slot,
 }
 
 // F_k,a = -sum_blocks(partials[3k+a]) * dxyz; optionally aimp = F/m (DFT4He3d.f90:692-696)
-__global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, double* imp,
-                                                        int nimp, double dxyz, int update_aimp) {
+// (per-block partials hold 3 values per accumulated impurity: stride nr)
+__global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, int nr,
+                                                        double* imp, int nimp, double dxyz, int update_aimp) {
   constexpr int NR = 3 * HEDFT_MAX_IMP;
   __shared__ double sh[NR][256];
-  for (int r = 0; r < NR; ++r) {
+  for (int r = 0; r < nr; ++r) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) s += part[(size_t)b * NR + r];
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += part[(size_t)b * nr + r];
     sh[r][threadIdx.x] = s;
   }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w)
-      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
+      for (int r = 0; r < nr; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -125,8 +126,8 @@ __global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict
   }
 }
 
-int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp) {
-  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, c->imp,
+int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr) {
+  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, nr, c->imp,
                      c->nimp, c->dxyz, update_aimp);
   return HEDFT_OK;
 }
@@ -2217,7 +2218,7 @@ int hedft_forceimp(hedft_ctx* c) {
   else
     hipLaunchKernelGGL(k_force_ew<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
                        c->dxden, c->dyden, c->dzden, force_partials(c));
-  HEDFT_TRY(force_finalize(c, EW_BLOCKS, 0));
+  HEDFT_TRY(force_finalize(c, EW_BLOCKS, 0, 3 * HEDFT_MAX_IMP));
   return done(c);
 }
 

@@ -134,21 +134,25 @@ int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int i
 // accumulates the He part of the impurity force F_k = -sum(grad(den) U_k) dxyz
 // (potenimp.f90:49-52), evaluating U_k once per point.  R is the storage type
 // of the density fields (double, or float on the mixed path).
-// WANT = false: the gradient alone (no accumulators, fewer registers)
-template <typename R, bool WANT>
+// NI impurities with force accumulators: 0 (the gradient alone), 1 (the usual
+// single impurity, fully unrolled) or HEDFT_MAX_IMP (any count, run-time loop)
+template <typename R, int NI>
 struct OutGradZForce {
-  static constexpr int NR = WANT ? 3 * HEDFT_MAX_IMP : 0;
+  static constexpr int NR = 3 * NI;
   R* out;
   const R *gx, *gy;
   double* partials;
   ImpDev im;
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double v, double* red) const {
     out[i] = (R)v;
-    if constexpr (WANT) {
+    if constexpr (NI > 0) {
       double xx, yy, zz;
       imp_xyz(im, ix, iy, iz, xx, yy, zz);
       const double a = gx[i], b = gy[i];
-      for (int k = 0; k < im.nimp; ++k) {
+      const int nk = (NI == 1) ? 1 : im.nimp;
+#pragma unroll
+      for (int k = 0; k < NI; ++k) {
+        if (k >= nk) break;
         const double u = imp_u_k(im, k, xx, yy, zz, false);
         red[3 * k] += a * u;
         red[3 * k + 1] += b * u;
@@ -160,7 +164,7 @@ struct OutGradZForce {
 
 ImpDev make_impdev(hedft_ctx* c);
 double* force_partials(hedft_ctx* c);
-int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp);
+int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr);
 
 template <typename R>
 static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp) {
@@ -171,14 +175,18 @@ static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp) {
   R* dz = reinterpret_cast<R*>(c->dzden);
   InPlain<R> ip{den};
   launch_pencil_xy<R, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], ip, OutTwo<R>{dx, dy, nullptr});
-  if (want_force)
+  if (want_force && c->nimp == 1)
     launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip,
-                        OutGradZForce<R, true>{dz, dx, dy, force_partials(c), make_impdev(c)});
+                        OutGradZForce<R, 1>{dz, dx, dy, force_partials(c), make_impdev(c)});
+  else if (want_force)
+    launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip,
+                        OutGradZForce<R, HEDFT_MAX_IMP>{dz, dx, dy, force_partials(c), make_impdev(c)});
   else
     launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip,
-                        OutGradZForce<R, false>{dz, dx, dy, nullptr, make_impdev(c)});
+                        OutGradZForce<R, 0>{dz, dx, dy, nullptr, make_impdev(c)});
   if (hipGetLastError() != hipSuccess) return HEDFT_EHIP;
-  return want_force ? force_finalize(c, pencil_geom<R>(c->d, 2).nblocks, update_aimp) : HEDFT_OK;
+  const int nr = (c->nimp == 1) ? 3 : 3 * HEDFT_MAX_IMP;  // accumulators per block of the pass above
+  return want_force ? force_finalize(c, pencil_geom<R>(c->d, 2).nblocks, update_aimp, nr) : HEDFT_OK;
 }
 
 int derden_force(hedft_ctx* c, int want_force, int update_aimp) {

@@ -272,3 +272,32 @@ def test_xy_stencil_paths_all_icons_vs_oracle(gpu, icon):
     for k in ("ekin4", "ealphas", "etot"):
         assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]), k
     d.close()
+
+
+def test_tddft_two_impurities_vs_oracle(gpu):
+    """Two classical impurities with different tables and masses (the force
+    pass's run-time impurity loop), and the standalone forceimp."""
+    n, xmax, n4 = 32, 6.4, 60.0
+    _say("tddft two impurities")
+    d, o = _pair(n, xmax, gpu.MODE_TDDFT, core_otc=1)
+    dt = 5e-4 / I.PSTOK
+    potion, delinter, rmax = I.impurity_table(n, xmax)
+    potion2, _, _ = I.impurity_table(n, xmax, eps=12.0, sigma=3.5)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=8)
+    r0 = np.array([[0.0, 0.5, -4.0], [1.0, -0.5, 4.5]])
+    v0 = np.array([[0.0, 0.0, 3.0], [0.5, 0.0, -2.0]]) * I.PSTOK
+    for x in (d, o):
+        x.set("DEN", den)
+        x.set_impurities(np.stack([potion, potion2]), delinter, rmax, [108.0 * I.MP_U, 40.0 * I.MP_U])
+        x.set_impurity_state(r0, v0, np.zeros((2, 3)))
+        x.tddft_init()
+        for it in range(1, 9):
+            x.tddft_iteration(dt, it, 3)
+    assert relerr(d.field("PSI"), o.field("PSI")) < RTOL_TRAJ
+    sg, so = d.impurity_state(), o.impurity_state()
+    for k in ("rimp", "vimp", "aimp", "force"):
+        assert relerr(sg[k], so[k]) < RTOL_TRAJ, k
+    for x in (d, o):
+        x.forceimp()
+    assert relerr(d.impurity_state()["force"], o.impurity_state()["force"]) < RTOL_OP
+    d.close()
