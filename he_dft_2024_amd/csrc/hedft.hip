@@ -822,6 +822,22 @@ static int fork(hedft_ctx* c) {
   return HEDFT_OK;
 }
 
+// A fork in two halves: fork_point() marks the branch point on the main
+// stream, the main branch's first kernel is launched, then fork_side() lets the
+// side stream start.  Both branches still begin at the branch point, but the
+// main kernel (a whole-plane FFT that needs a full CU's LDS) is dispatched
+// before the side branch's kernels spread over the CUs.
+static int fork_point(hedft_ctx* c) {
+  if (c->no_side) return HEDFT_OK;
+  HEDFT_CHECK(hipEventRecord(c->ev_fork, c->main_stream));
+  return HEDFT_OK;
+}
+static int fork_side(hedft_ctx* c) {
+  if (c->no_side) return HEDFT_OK;
+  HEDFT_CHECK(hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
+  return HEDFT_OK;
+}
+
 static int join(hedft_ctx* c) {
   if (c->no_side) return HEDFT_OK;
   HEDFT_CHECK(hipEventRecord(c->ev_join, c->side_stream));
@@ -909,11 +925,7 @@ template <typename R>
 static int stage_c(hedft_ctx* c) {
   // side branch: x+y divergence (denalf, int_x, int_y -> work1); main branch:
   // the convolution (grad den, int -> spec[0] -> ualph_conv)
-  HEDFT_TRY(fork(c));
-  {
-    SideBranch sb(c);
-    HEDFT_TRY(alpha_div_xy(c));
-  }
+  HEDFT_TRY(fork_point(c));
   XInT<R> dot{};
   dot.op = XIN_DOT3;
   dot.a[0] = fld<R>(c->dxden);
@@ -924,6 +936,11 @@ static int stage_c(hedft_ctx* c) {
   dot.a[5] = fld<R>(c->intz);
   cplx_t<R>* s0 = spc<R>(c->spec[0]);
   HEDFT_TRY(fwd_xy<R>(c, 1, &dot, &s0));
+  HEDFT_TRY(fork_side(c));
+  {
+    SideBranch sb(c);
+    HEDFT_TRY(alpha_div_xy(c));
+  }
   const R* ka = fld<R>(c->kalfs);
   HEDFT_TRY(zconv<R>(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
   XOutT<R> xu = xo_store<R>(c->ualph_conv);
@@ -1013,7 +1030,17 @@ static int potenimp_q(hedft_ctx* c) {
 // convolved on the side branch and added in the pot4 assembly.
 template <typename R>
 static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
-  HEDFT_TRY(fork(c));
+  // where the next step's x+y Laplacians run: beside the den convolutions
+  // (early) or beside stages B and C (late); lap_late 2 puts them late only in
+  // the mean field that also carries the impurity force
+  const bool late = with_lap && (c->lap_late == 1 || (c->lap_late == 2 && want_force));
+  HEDFT_TRY(fork_point(c));
+  XInT<R> xi{};
+  xi.op = XIN_COPY;
+  xi.a[0] = fld<R>(c->den);
+  cplx_t<R>* s0 = spc<R>(c->spec[0]);
+  HEDFT_TRY(fwd_xy<R>(c, 1, &xi, &s0));
+  HEDFT_TRY(fork_side(c));
   {
     SideBranch sb(c);
     flush_side_pre(c);  // impurity PC and errHe of the step just queued
@@ -1021,16 +1048,11 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     // (Dynamics_quantum_impurities/poten.f90:48): nothing else reads grad(den) there
     if (!c->quantum || c->p.core_otc) HEDFT_TRY(derden_force(c, want_force, want_force));
     if (c->quantum) HEDFT_TRY(potx4_conv<R>(c, c->potx4));
-    if (with_lap && !c->lap_late) {
+    if (with_lap && !late) {
       HEDFT_TRY(lap_xy(c, c->psi, 1));
       if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
     }
   }
-  XInT<R> xi{};
-  xi.op = XIN_COPY;
-  xi.a[0] = fld<R>(c->den);
-  cplx_t<R>* s0 = spc<R>(c->spec[0]);
-  HEDFT_TRY(fwd_xy<R>(c, 1, &xi, &s0));
   // den convolutions: dencg, delj4 [, denalf] [, upotx]
   const R* kk[4] = {fld<R>(c->wcgk), fld<R>(c->fvlj4), nullptr, nullptr};
   cplx_t<R>* so[4] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), nullptr, nullptr};
@@ -1062,7 +1084,6 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   // the x+y Laplacians for the next step run beside stages B and C (they read
   // only psi and psix, which the mean field does not change); stage C's own
   // branch and its join come after them on the side stream
-  const bool late = with_lap && c->lap_late;
   if (late) {
     HEDFT_TRY(fork(c));
     {
@@ -2618,7 +2639,7 @@ int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tunin
   for (int sp = 0; sp < 4; ++sp)
     for (int ns = 0; ns < 2; ++ns)
       for (int eg = 0; eg < 2; ++eg)
-        for (int ll = 0; ll < 2; ++ll) {
+        for (int ll = 0; ll < 3; ++ll) {
           cur.xy_split_fwd = split[sp][0];
           cur.xy_split_inv = split[sp][1];
           cur.no_side = ns;

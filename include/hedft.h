@@ -396,15 +396,16 @@ int hedft_serve(const hedft_job* jobs, int32_t njobs, const hedft_serve_opts* op
  * split FFT passes use another radix order, so results agree to rounding. */
 typedef struct hedft_tuning {
   int32_t xy_split_fwd, xy_split_inv, no_side, prefer_eager;
-  int32_t lap_late;          /* 1: the next step's x+y Laplacians run beside the
-                                second half of the mean field, 0: beside the
-                                density convolutions (default)              */
+  int32_t lap_late;          /* where the next step's x+y Laplacians run: 0 beside
+                                the density convolutions, 1 beside the second
+                                half of the mean field, 2 the latter only in
+                                the mean field that carries the impurity force */
   int32_t pad_;
   double ms_per_step;        /* measured time of the winning choice (autotune) */
 } hedft_tuning;
 /* Times trial_steps PC iterations (rounded up to 6, after a 6-iteration
- * warm-up) for each of the 32 combinations on this context and keeps the
- * fastest.  The context advances by 32 (6 + trial) iterations. */
+ * warm-up) for each of the 48 combinations on this context and keeps the
+ * fastest.  The context advances by 48 (6 + trial) iterations. */
 int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tuning* best);
 int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t);
 int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t);
