@@ -100,24 +100,31 @@ static void finalize(hedft_ctx* c, const double* part, size_t nblocks, int slot,
 }
 
 // F_k,a = -sum_blocks(partials[3k+a]) * dxyz; optionally aimp = F/m (DFT4He3d.f90:692-696)
-// (per-block partials hold 3 values per accumulated impurity: stride nr)
-__global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, int nr,
-                                                        double* imp, int nimp, double dxyz, int update_aimp) {
-  constexpr int NR = 3 * HEDFT_MAX_IMP;
+// (per-block partials hold NR = 3 values per accumulated impurity).  Each
+// thread first sums its blocks into NR independent registers (loads in flight
+// together), then one tree over the workgroup.
+template <int NR>
+__global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, double* imp,
+                                                        int nimp, double dxyz, int update_aimp) {
   __shared__ double sh[NR][256];
-  for (int r = 0; r < nr; ++r) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) s += part[(size_t)b * nr + r];
-    sh[r][threadIdx.x] = s;
+  double s[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] += part[(size_t)b * NR + r];
   }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = s[r];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w)
-      for (int r = 0; r < nr; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    for (int k = 0; k < nimp; ++k)
+    for (int k = 0; k < nimp && 3 * k < NR; ++k)
       for (int a = 0; a < 3; ++a) {
         const double f = -sh[3 * k + a][0] * dxyz;
         imp[IMP_F + k + nimp * a] = f;
@@ -127,7 +134,12 @@ __global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict
 }
 
 int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr) {
-  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, nr, c->imp,
+  if (nr == 3)
+    hipLaunchKernelGGL(k_force_finalize<3>, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, c->imp,
+                       c->nimp, c->dxyz, update_aimp);
+  else
+    hipLaunchKernelGGL(k_force_finalize<3 * HEDFT_MAX_IMP>, dim3(1), dim3(256), 0, c->stream, force_partials(c),
+                       (int)nblocks, c->imp,
                      c->nimp, c->dxyz, update_aimp);
   return HEDFT_OK;
 }
