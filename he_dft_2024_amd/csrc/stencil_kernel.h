@@ -224,13 +224,25 @@ struct PencilGeom {
   size_t nblocks;
 };
 
+// side-stream stencil passes sized to share CUs with the plane FFTs
+// (HEDFT_SIDE_FIT=0 in the environment selects the wider tiles)
+inline bool side_fit() {
+  static const int enabled = [] {
+    const char* e = std::getenv("HEDFT_SIDE_FIT");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return enabled;
+}
+
 // column-tile width of the y/z pencil kernels: 256 B rows (64 float, 32
 // double / float2, 16 double2 columns); 16 double2 columns need 64 KB of LDS
-// at n = 256, so 8 there
+// at n = 256, so 8 there.  Double columns are 16 wide (128 B rows) up to
+// n = 128 when side passes fit beside the plane FFTs: 16 KiB of LDS instead
+// of 32 KiB, under the 30 KiB a plane workgroup leaves free.
 template <typename V>
 inline int pencil_tx(int n) {
   if (sizeof(V) == 4) return 64;
-  if (sizeof(V) == 8) return 32;
+  if (sizeof(V) == 8) return (side_fit() && n <= 128) ? 16 : 32;
   return (n > 128) ? 8 : 16;
 }
 
@@ -507,25 +519,31 @@ __global__ void __launch_bounds__(256) k_pencil_xy2(In in, Out out, StencilCoef 
   }
 }
 
-// tile shape of the 2-D variant for a value type: (TX, TY)
-template <typename V>
+// tile shape of the 2-D variant for a value type: (TX, TY).  FIT selects
+// tiles whose LDS (at most 30 KiB) fits beside a whole-plane FFT workgroup
+// (130 KiB of the CU's 160 KiB), so that a stencil pass on the side stream
+// can share CUs with the plane transforms of the main stream.
+template <typename V, bool SAME, bool FIT>
 struct XYTile {
-  static constexpr int TX = (sizeof(V) == 4) ? 64 : (sizeof(V) == 8) ? 64 : 32;
-  static constexpr int TY = (sizeof(V) == 4) ? 64 : 32;
+  static constexpr int TX = (sizeof(V) == 4) ? 64 : (sizeof(V) == 8) ? ((FIT && !SAME) ? 32 : 64) : 32;
+  static constexpr int TY = (sizeof(V) == 4) ? 64 : (sizeof(V) == 8) ? 32 : (FIT ? 16 : 32);
+  static constexpr size_t LDS = sizeof(V) * (SAME ? (size_t)(TY + 12) * (TX + 12)
+                                                 : (size_t)TY * (TX + 12) + (size_t)(TY + 12) * TX);
+  static_assert(!FIT || sizeof(V) == 4 || LDS <= 30 * 1024, "FIT tile exceeds the LDS left beside a plane");
 };
 
 // 1 when the 2-D tiles are used for this grid (ny a multiple of TY, and a
 // last x tile of at least 7 columns); HEDFT_XY_TILE=0 in the environment
 // selects the full-column kernel
-template <typename V>
+template <typename V, bool SAME, bool FIT>
 inline bool use_xy2(const Dims& d) {
   static const int enabled = [] {
     const char* e = std::getenv("HEDFT_XY_TILE");
     return (e && e[0] == '0') ? 0 : 1;
   }();
   // the last x tile must hold the 13 columns a one-sided boundary stencil reads
-  const int rx = d.nx % XYTile<V>::TX;
-  return enabled && d.ny % XYTile<V>::TY == 0 && d.nx >= 13 && (rx == 0 || rx >= 7);
+  const int rx = d.nx % XYTile<V, SAME, FIT>::TX;
+  return enabled && d.ny % XYTile<V, SAME, FIT>::TY == 0 && d.nx >= 13 && (rx == 0 || rx >= 7);
 }
 
 template <typename V, bool SAME>
@@ -560,13 +578,17 @@ template <typename V, bool SAME, class In, class Out>
 inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cfx, const StencilCoef& cfy,
                              const In& in, const Out& out) {
   if constexpr (Out::NR == 0) {
-    if (use_xy2<V>(d)) {
-      constexpr int TX = XYTile<V>::TX, TY = XYTile<V>::TY;
-      const dim3 grid((d.nx + TX - 1) / TX, d.ny / TY, d.nz);
-      const size_t lds = sizeof(V) * (SAME ? (size_t)(TY + 12) * (TX + 12)
-                                           : (size_t)TY * (TX + 12) + (size_t)(TY + 12) * TX);
-      hipLaunchKernelGGL((k_pencil_xy2<V, SAME, TX, TY, In, Out>), grid, dim3(256), lds, s, in, out, cfx, cfy, d);
-      return;
+    auto xy2 = [&](auto fit) {
+      constexpr bool FIT = decltype(fit)::value;
+      using T = XYTile<V, SAME, FIT>;
+      const dim3 grid((d.nx + T::TX - 1) / T::TX, d.ny / T::TY, d.nz);
+      hipLaunchKernelGGL((k_pencil_xy2<V, SAME, T::TX, T::TY, In, Out>), grid, dim3(256), T::LDS, s, in, out, cfx,
+                         cfy, d);
+    };
+    if (side_fit()) {
+      if (use_xy2<V, SAME, true>(d)) return xy2(std::true_type{});
+    } else if (use_xy2<V, SAME, false>(d)) {
+      return xy2(std::false_type{});
     }
   }
   PencilGeom g = pencil_xy_geom<V, SAME>(d);
