@@ -17,6 +17,7 @@
 //        tile stays in LDS (the spectrum of a convolution never round-trips
 //        HBM between the forward and inverse z transforms).
 #include <cmath>
+#include <cstdlib>
 
 #include "common.h"
 
@@ -516,6 +517,194 @@ __global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_
 }
 
 // ---------------------------------------------------------------------------
+// 128 x 128 planes with fewer LDS round trips.  The x passes read their
+// operands from global memory straight into registers (forward) or write
+// their results straight from registers (inverse), and the y pass next to
+// global memory does the same, so the plane crosses LDS 6-7 times per
+// transform instead of 12.  Lanes 8c..8c+7 own row c: the 8 x 8 exchange
+// between the two radix-8 x passes, the real-pair split and the c2r merge
+// only read the lane's own row, which lies in its own wavefront, so they are
+// ordered by wave-scope fences instead of workgroup barriers.  Between the
+// two x passes the Stockham position 8a + b of a row is stored at 8a + (b ^ a)
+// (an XOR swizzle that spreads both the radix-8 writes and reads over the
+// LDS banks).  Arithmetic per element is that of k_xyfwd / k_xyinv.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)))
+    k_xyfwd128(XInBatch<R> a, const cplx_t<R>* __restrict__ twx, const cplx_t<R>* __restrict__ twp,
+               const cplx_t<R>* __restrict__ twy) {
+  using C = cplx_t<R>;
+  constexpr int NX = 128, NY = 128, M = 64, NXH = 65, PAD = 65, NT = 1024;
+  extern __shared__ unsigned char smem_raw[];
+  C* P = reinterpret_cast<C*>(smem_raw);
+  const XInT<R>& in = a.in[blockIdx.y];
+  const int c = threadIdx.x >> 3, i = threadIdx.x & 7;
+  // reals 2(i + 8j), 2(i + 8j) + 1 of row c sit at g0 + 16 j
+  const size_t g0 = ((size_t)blockIdx.x * NY + c) * NX + 2 * i;
+  C u[8];
+  auto load = [&](auto getv) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) u[j] = getv(g0 + 16 * j);
+  };
+  auto ld2 = [](const R* p, size_t k) { return *reinterpret_cast<const C*>(p + k); };
+  switch (in.op) {  // uniform per workgroup: one branch-free load sequence per operation
+    case XIN_CORR:
+      load([&](size_t k) {
+        const C dn = ld2(in.a[0], k), cg = ld2(in.a[1], k);
+        return mkc<C>((R)((double)dn.x * cg.x * (in.c0 + in.c1 * cg.x)),
+                      (R)((double)dn.y * cg.y * (in.c0 + in.c1 * cg.y)));
+      });
+      break;
+    case XIN_FGRAD:
+      load([&](size_t k) {
+        const C al = ld2(in.a[0], k), gr = ld2(in.a[1], k);
+        return mkc<C>((R)((1.0 - (double)al.x / in.c0) * gr.x), (R)((1.0 - (double)al.y / in.c0) * gr.y));
+      });
+      break;
+    case XIN_DOT3:
+      load([&](size_t k) {
+        const C g0v = ld2(in.a[0], k), g1 = ld2(in.a[1], k), g2 = ld2(in.a[2], k);
+        const C i0 = ld2(in.a[3], k), i1 = ld2(in.a[4], k), i2 = ld2(in.a[5], k);
+        return mkc<C>((R)((double)g0v.x * i0.x + (double)g1.x * i1.x + (double)g2.x * i2.x),
+                      (R)((double)g0v.y * i0.y + (double)g1.y * i1.y + (double)g2.y * i2.y));
+      });
+      break;
+    case XIN_ABS2:
+      load([&](size_t k) {
+        const double2 z0 = in.z[k], z1 = in.z[k + 1];
+        return mkc<C>((R)(z0.x * z0.x + z0.y * z0.y), (R)(z1.x * z1.x + z1.y * z1.y));
+      });
+      break;
+    default:
+      load([&](size_t k) { return ld2(in.a[0], k); });
+  }
+  // x pass 1 (radix 8, no twiddles): result j of butterfly i is Stockham position 8i + j
+  dft<8, false>(u);
+  C* row = P + c * PAD;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) row[8 * i + (j ^ i)] = u[j];
+  wave_sync();
+  // x pass 2 (radix 8): butterfly i takes positions i + 8j; its results are Z_{i+8j}
+#pragma unroll
+  for (int j = 0; j < 8; ++j) u[j] = row[8 * j + (i ^ j)];
+#pragma unroll
+  for (int j = 1; j < 8; ++j) u[j] = cmul(u[j], twx[i * j]);
+  dft<8, false>(u);
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) row[i + 8 * j] = u[j];
+  wave_sync();
+  // real-to-half-complex split X_k = split(Z_k, conj Z_{M-k}, W^k) for k = i + 8j
+  // (Z_M = Z_0); lane i = 0 also forms X_M from Z_0
+  C x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = i + 8 * j;
+    x[j] = split_one(u[j], cconj(row[(M - k) & (M - 1)]), twp[k]);
+  }
+  const C xnyq = split_one(u[0], cconj(u[0]), twp[M]);
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) row[i + 8 * j] = x[j];
+  if (i == 0) row[M] = xnyq;
+  __syncthreads();
+  // y pass 1 (radix 16) within LDS
+  mpass<NY, 16, 1, false, NT, NXH, true>(P, 1, PAD, twy);
+  // y pass 2 (radix 8, after 16): from LDS to the half spectrum
+  constexpr int RR = 8, PP = 16, NB = NY / RR, ITEMS = NXH * NB, PER = (ITEMS + NT - 1) / NT;
+  C* o = a.out[blockIdx.y] + (size_t)blockIdx.x * NY * NXH;
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int item = threadIdx.x + b * NT;
+    if (item < ITEMS) {
+      const int cc = item % NXH, ii = item / NXH;
+      C v[RR];
+#pragma unroll
+      for (int j = 0; j < RR; ++j) v[j] = P[cc + (ii + j * NB) * PAD];
+      const int k = ii % PP;
+#pragma unroll
+      for (int j = 1; j < RR; ++j) v[j] = cmul(v[j], twy[(k * j) * (NY / (PP * RR))]);
+      dft<RR, false>(v);
+      const int oo = (ii - k) * RR + k;
+#pragma unroll
+      for (int j = 0; j < RR; ++j) o[(size_t)(oo + j * PP) * NXH + cc] = v[j];
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)))
+    k_xyinv128(XOutBatch<R> a, const cplx_t<R>* __restrict__ twx, const cplx_t<R>* __restrict__ twp,
+               const cplx_t<R>* __restrict__ twy, double renor) {
+  using C = cplx_t<R>;
+  constexpr int NX = 128, NY = 128, M = 64, NXH = 65, PAD = 65, NT = 1024;
+  extern __shared__ unsigned char smem_raw[];
+  C* P = reinterpret_cast<C*>(smem_raw);
+  const C* in = a.in[blockIdx.y] + (size_t)blockIdx.x * NY * NXH;
+  const XOutT<R>& out = a.out[blockIdx.y];
+  // y pass 1 (radix 16, no twiddles) from the half spectrum: butterfly ii of
+  // column cc takes rows ii + 8j; result j is Stockham position 16 ii + j
+  {
+    constexpr int NB = NY / 16, ITEMS = NXH * NB;
+    const int item = threadIdx.x;
+    if (item < ITEMS) {
+      const int cc = item % NXH, ii = item / NXH;
+      C v[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = in[(ii + j * NB) * NXH + cc];
+      dft<16, true>(v);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) P[cc + (16 * ii + j) * PAD] = v[j];
+    }
+  }
+  __syncthreads();
+  // y pass 2 (radix 8, after 16) within LDS
+  mpass<NY, 8, 16, true, NT, NXH, true>(P, 1, PAD, twy);
+  // c2r merge fused into x pass 1: Z'_k = merge(X_k, X_{M-k}, W^k), k = i + 8j,
+  // imaginary parts of X_0 and X_M dropped
+  const int c = threadIdx.x >> 3, i = threadIdx.x & 7;
+  C* row = P + c * PAD;
+  C u[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = i + 8 * j;
+    C a0 = row[k], b0 = row[M - k];
+    if (k == 0) {
+      a0.y = 0;
+      b0.y = 0;
+    }
+    u[j] = merge_one(a0, b0, twp[k]);
+  }
+  dft<8, true>(u);
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) row[8 * i + (j ^ i)] = u[j];
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) u[j] = row[8 * j + (i ^ j)];
+#pragma unroll
+  for (int j = 1; j < 8; ++j) {
+    C w = twx[i * j];
+    w.y = -w.y;
+    u[j] = cmul(u[j], w);
+  }
+  dft<8, true>(u);
+  // z_{i+8j} = (x_{2(i+8j)}, x_{2(i+8j)+1}) of row c, through the epilogue
+  const size_t g0 = ((size_t)blockIdx.x * NY + c) * NX + 2 * i;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    xout_apply(out, g0 + 16 * j, (double)u[j].x * renor);
+    xout_apply(out, g0 + 16 * j + 1, (double)u[j].y * renor);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // x R2C: rows of nx = 2M reals -> M+1 complex
 // ---------------------------------------------------------------------------
 template <int M, int ROWS, typename R>
@@ -760,8 +949,28 @@ static hipError_t plane_attrs() {
   hipError_t e = hipFuncSetAttribute((const void*)k_xyfwd<N, N, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)Plane<N, N, R>::LDS);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_xyinv<N, N, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  e = hipFuncSetAttribute((const void*)k_xyinv<N, N, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)Plane<N, N, R>::LDS);
+  if (e != hipSuccess || N != 128) return e;
+  e = hipFuncSetAttribute((const void*)k_xyfwd128<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)Plane<N, N, R>::LDS);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_xyinv128<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)Plane<N, N, R>::LDS);
+}
+
+// the 128 x 128 plane kernels with fewer LDS round trips, for double storage
+// (HEDFT_PLANE128=0 in the environment selects the generic ones).  With float
+// storage the generic kernels stay: their 67 KB planes and 61-64 VGPRs let two
+// workgroups share a CU, which the register-resident passes (84-86 VGPRs) do
+// not, and both measure the same at 128^3.
+template <typename R>
+static bool plane128() {
+  static const int enabled = [] {
+    const char* e = std::getenv("HEDFT_PLANE128");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return enabled && sizeof(R) == 8;
 }
 
 int fft_setup(hedft_ctx* c) {
@@ -856,6 +1065,12 @@ int fwd_xy(hedft_ctx* c, int nb, const XInT<R>* in, cplx_t<R>* const* spec) {
     a.out[b] = spec[b];
     s.p[b] = spec[b];
   }
+  if (nb > c->xy_split_nb[0] && c->d.nx == 128 && c->d.ny == 128 && plane128<R>()) {
+    constexpr size_t lds128 = Plane<128, 128, R>::LDS;
+    hipLaunchKernelGGL((k_xyfwd128<R>), dim3(c->d.nz, nb), dim3(1024), lds128, c->stream, a,
+                       twp<R>(c, c->tw_x), twp<R>(c, c->tw_xpost), twp<R>(c, c->tw_y));
+    return HEDFT_OK;
+  }
   if (nb > c->xy_split_nb[0]) HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
     using PL_ = Plane<PX_, PY_, R>;
     constexpr size_t lds_ = PL_::LDS;
@@ -904,6 +1119,12 @@ int inv_yx(hedft_ctx* c, int nb, cplx_t<R>* const* spec, const XOutT<R>* out) {
     s.p[b] = spec[b];
     a.in[b] = spec[b];
     a.out[b] = out[b];
+  }
+  if (nb > c->xy_split_nb[1] && c->d.nx == 128 && c->d.ny == 128 && plane128<R>()) {
+    constexpr size_t lds128 = Plane<128, 128, R>::LDS;
+    hipLaunchKernelGGL((k_xyinv128<R>), dim3(c->d.nz, nb), dim3(1024), lds128, c->stream, a,
+                       twp<R>(c, c->tw_x), twp<R>(c, c->tw_xpost), twp<R>(c, c->tw_y), c->renor);
+    return HEDFT_OK;
   }
   if (nb > c->xy_split_nb[1]) HEDFT_PLANE_SWITCH(c->d.nx, c->d.ny, {
     using PL_ = Plane<PX_, PY_, R>;
