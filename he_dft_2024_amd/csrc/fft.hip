@@ -194,6 +194,71 @@ __device__ __forceinline__ void run_passes(C* col, int stride, int t, const C* t
   if constexpr (sizeof...(Rest) > 0) run_passes<L, INV, T, P * R, Rest...>(col, stride, t, tw);
 }
 
+// The last pass of a column FFT with its results handed to put(position, value)
+// (a global store) instead of written back to LDS.
+template <int L, int R, int P, bool INV, int T, typename C, typename Put>
+__device__ __forceinline__ void spass_out(const C* col, int stride, int t, const C* __restrict__ tw, Put put) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + T - 1) / T;
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int i = t + b * T;
+    if (PER * T == NB || i < NB) {
+      C u[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) u[j] = col[(i + j * NB) * stride];
+      if constexpr (P > 1) {
+        const int k = i % P;
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+          C w = tw[(k * j) * (L / (P * R))];
+          if (INV) w.y = -w.y;
+          u[j] = cmul(u[j], w);
+        }
+      }
+      dft<R, INV>(u);
+      const int k = i % P;
+      const int base = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) put(base + j * P, u[j]);
+    }
+  }
+}
+
+template <int L, bool INV, int T, int P, int R, int... Rest, typename C, typename Put>
+__device__ __forceinline__ void run_passes_to(C* col, int stride, int t, const C* tw, Put put) {
+  if constexpr (sizeof...(Rest) == 0) {
+    spass_out<L, R, P, INV, T>(col, stride, t, tw, put);
+  } else {
+    spass<L, R, P, INV, T>(col, stride, t, tw);
+    run_passes_to<L, INV, T, P * R, Rest...>(col, stride, t, tw, put);
+  }
+}
+
+// First pass (radix R1 = L / T, no twiddles) on operands already in registers:
+// thread t holds elements t + j T, j < R1, and writes its results to the
+// Stockham positions t R1 + j.
+template <int L, bool INV, int T, int R1, typename C>
+__device__ __forceinline__ void first_pass_regs(C (&u)[R1], C* col, int stride, int t) {
+  static_assert(L / R1 == T, "one first-pass butterfly per thread");
+  dft<R1, INV>(u);
+#pragma unroll
+  for (int j = 0; j < R1; ++j) col[(t * R1 + j) * stride] = u[j];
+  __syncthreads();
+}
+
+template <int L, bool INV, int T, int R1, int... Rest, typename C>
+__device__ __forceinline__ void run_regs_lds(C (&u)[R1], C* col, int stride, int t, const C* tw) {
+  first_pass_regs<L, INV, T>(u, col, stride, t);
+  run_passes<L, INV, T, R1, Rest...>(col, stride, t, tw);
+}
+
+template <int L, bool INV, int T, int R1, int... Rest, typename C, typename Put>
+__device__ __forceinline__ void run_regs_out(C (&u)[R1], C* col, int stride, int t, const C* tw, Put put) {
+  first_pass_regs<L, INV, T>(u, col, stride, t);
+  run_passes_to<L, INV, T, R1, Rest...>(col, stride, t, tw, put);
+}
+
 // Stockham pass over C columns of length L at once, all NT threads of the
 // block sharing the C*L/R butterflies.  Element e of column c lives at
 // base[c*cs + e*es].  CF: consecutive threads take consecutive columns
@@ -259,6 +324,18 @@ struct Rad;
     template <bool INV, int NT, int NC, bool CF, typename C>                             \
     __device__ static __forceinline__ void run_multi(C* b, int cs, int es, const C* tw) { \
       run_mpasses<LEN, INV, NT, NC, CF, 1, __VA_ARGS__>(b, cs, es, tw);                  \
+    }                                                                                    \
+    /* first pass from registers u[j] = element t + j T, result in LDS */              \
+    template <bool INV, typename C>                                                      \
+    __device__ static __forceinline__ void run_from_regs(C (&u)[LEN / TT], C* c, int s, int t, \
+                                                         const C* tw) {                  \
+      run_regs_lds<LEN, INV, TT, __VA_ARGS__>(u, c, s, t, tw);                          \
+    }                                                                                    \
+    /* first pass from registers, last pass to put(position, value) */                 \
+    template <bool INV, typename C, typename Put>                                        \
+    __device__ static __forceinline__ void run_regs_to(C (&u)[LEN / TT], C* c, int s, int t, \
+                                                       const C* tw, Put put) {           \
+      run_regs_out<LEN, INV, TT, __VA_ARGS__>(u, c, s, t, tw, put);                     \
     }                                                                                    \
   };
 HEDFT_RAD(16, 4, 4, 4)
@@ -836,61 +913,52 @@ __global__ void __launch_bounds__(ColTile<L>::NT)
 }
 
 // z convolution: [forward z] -> (x kern_{b,o} -> inverse z -> out_{b,o}) for o < NOUT.
-// One LDS tile: with several outputs the forward spectrum is kept in registers
-// (L/T values per thread) and re-multiplied for each kernel, so the tile needs
-// no second buffer and more workgroups fit per CU.
+// One LDS tile of TX columns.  Thread (column xl0, t0) owns the elements
+// t0 + j T of its column, which are the operands of the first radix pass in
+// both directions: the forward transform's first pass reads them straight
+// from global memory, the forward spectrum is kept in registers (one value per
+// operand) and re-multiplied for each kernel straight into the first inverse
+// pass, and the last inverse pass stores to global memory; only the passes in
+// between go through LDS.
 template <int L, int NOUT, typename R>
 __global__ void __launch_bounds__(ColTile<L>::NT)
     k_zconv(ZConvArgs<R> a, Dims d, long long ncol, const cplx_t<R>* __restrict__ tw) {
   using C = cplx_t<R>;
-  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT, PER = (TX * L + NT - 1) / NT;
+  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT, T = Rad<L>::T, R1 = L / T;
+  static_assert(NT == T * TX, "one thread per (column, first-pass butterfly)");
   __shared__ C src[L * TX];
   const int b = blockIdx.y;
   const C* in = a.in[b];
   const size_t stride = (size_t)d.nxh * d.ny;
   const long long c0 = (long long)blockIdx.x * TX;
   const int xl0 = threadIdx.x % TX, t0 = threadIdx.x / TX;
-  lds_fill<8, NT>(src, TX * L, [&](int e) {
-    const int xl = e % TX, j = e / TX;
-    return in[min(c0 + xl, ncol - 1) + j * stride];  // clamped columns are never stored
-  });
-  __syncthreads();
-  if (a.do_fwd) Rad<L>::template run<false>(src + xl0, TX, t0, tw);
-  if (a.store[b]) {
-    for (int e = threadIdx.x; e < TX * L; e += NT) {
-      const int xl = e % TX, j = e / TX;
-      if (c0 + xl < ncol) a.store[b][c0 + xl + j * stride] = src[e];
-    }
-  }
-  C keep[PER];
+  const long long col = min(c0 + xl0, ncol - 1);  // clamped columns are never stored
+  const bool live = c0 + xl0 < ncol;
+  C keep[R1];
 #pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int e = threadIdx.x + u * NT;
-    if (e < TX * L) keep[u] = src[e];
+  for (int j = 0; j < R1; ++j) keep[j] = in[col + (size_t)(t0 + j * T) * stride];
+  if (a.do_fwd) {
+    Rad<L>::template run_from_regs<false>(keep, src + xl0, TX, t0, tw);
+    if (a.store[b]) {
+      for (int e = threadIdx.x; e < TX * L; e += NT) {
+        const int xl = e % TX, j = e / TX;
+        if (c0 + xl < ncol) a.store[b][c0 + xl + j * stride] = src[e];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R1; ++j) keep[j] = src[xl0 + (t0 + j * T) * TX];
   }
 #pragma unroll 1
   for (int o = 0; o < NOUT; ++o) {
     const R* kern = a.kern[b * NOUT + o];
-    R kv[PER];
+    C u[R1];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = min(threadIdx.x + u * NT, TX * L - 1);
-      const int xl = e % TX, j = e / TX;
-      kv[u] = kern[min(c0 + xl, ncol - 1) + j * stride];
-    }
-    if (o > 0) __syncthreads();  // the previous output's stores have read the tile
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = threadIdx.x + u * NT;
-      if (e < TX * L) src[e] = cscale(keep[u], kv[u]);
-    }
-    __syncthreads();
-    Rad<L>::template run<true>(src + xl0, TX, t0, tw);
+    for (int j = 0; j < R1; ++j) u[j] = cscale(keep[j], kern[col + (size_t)(t0 + j * T) * stride]);
+    if (o > 0 || a.do_fwd) __syncthreads();  // earlier readers of the tile are done
     C* out = a.out[b * NOUT + o];
-    for (int e = threadIdx.x; e < TX * L; e += NT) {
-      const int xl = e % TX, j = e / TX;
-      if (c0 + xl < ncol) out[c0 + xl + j * stride] = src[e];
-    }
+    Rad<L>::template run_regs_to<true>(u, src + xl0, TX, t0, tw, [&](int z, C v) {
+      if (live) out[col + (size_t)z * stride] = v;
+    });
   }
 }
 
