@@ -494,12 +494,16 @@ __global__ void __launch_bounds__(256) k_pcr_corr(long long n, double* psi, cons
 
 __global__ void k_copy_scal(double* s, int from, int to) { s[to] = s[from]; }
 
-// ---- impurity Newton updates (single thread; N_imp*3 <= 12 numbers) ------------------
+// ---- impurity Newton updates (one lane per coordinate; N_imp*3 <= 12 numbers) --------
+// Every coordinate's update reads and writes only its own entries of the state
+// arrays, so lane i carries coordinate i through the reference's loops in
+// their order; launched as one wavefront.
 __global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, double cc, double dt) {
-  const int m = 3 * nimp;
+  const int m = 3 * nimp, i = threadIdx.x;
+  if (i >= m) return;
   double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *QR = imp + IMP_QR, *QV = imp + IMP_QV;
   double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
-  for (int i = 0; i < m; ++i) {  // steprk.f90:330-340
+  {  // steprk.f90:330-340
     const double st = a * (V[i] - b * QR[i]);
     QR[i] = QR[i] + 3.0 * st - cc * V[i];
     if (first) {
@@ -509,7 +513,7 @@ __global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, d
     }
     R[i] = R[i] + dt * st;
   }
-  for (int i = 0; i < m; ++i) {  // steprk.f90:345-354
+  {  // steprk.f90:345-354
     const double st = a * (A[i] - b * QV[i]);
     QV[i] = QV[i] + 3.0 * st - cc * A[i];
     if (first) {
@@ -525,50 +529,51 @@ __global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, d
 
 // steppc.f90:461-480 (slot indices 0-based)
 __global__ void k_imp_pred(double* imp, int nimp, double dt, int r3, int v1, int v2, int v3, int a1, int a2) {
-  const int m = 3 * nimp;
+  const int m = 3 * nimp, i = threadIdx.x;
+  if (i >= m) return;
   double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *PCR = imp + IMP_PCR, *PCV = imp + IMP_PCV;
   double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
-  double st[12];
-  for (int i = 0; i < m; ++i) st[i] = RO[12 * r3 + i] + C4O3 * dt * (2.0 * V[i] - VO[12 * v1 + i] + 2.0 * VO[12 * v2 + i]);
-  for (int i = 0; i < m; ++i) {
-    RO[12 * r3 + i] = R[i];
-    R[i] = st[i] - C112 * PCR[i];
-    PCR[i] = st[i];
-  }
-  for (int i = 0; i < m; ++i) st[i] = VO[12 * v3 + i] + C4O3 * dt * (2.0 * A[i] - AO[12 * a1 + i] + 2.0 * AO[12 * a2 + i]);
-  for (int i = 0; i < m; ++i) {
-    VO[12 * v3 + i] = st[i] - C112 * PCV[i];
-    PCV[i] = st[i];
-  }
-  for (int i = 0; i < m; ++i) AO[12 * a2 + i] = A[i];
+  const double sr = RO[12 * r3 + i] + C4O3 * dt * (2.0 * V[i] - VO[12 * v1 + i] + 2.0 * VO[12 * v2 + i]);
+  RO[12 * r3 + i] = R[i];
+  R[i] = sr - C112 * PCR[i];
+  PCR[i] = sr;
+  const double sv = VO[12 * v3 + i] + C4O3 * dt * (2.0 * A[i] - AO[12 * a1 + i] + 2.0 * AO[12 * a2 + i]);
+  VO[12 * v3 + i] = sv - C112 * PCV[i];
+  PCV[i] = sv;
+  AO[12 * a2 + i] = A[i];
 }
 
-// steppc.f90:530-555; errimp, errvimp into scal
+// steppc.f90:530-555; errimp, errvimp into scal (summed by lane 0 in coordinate order)
 __global__ void k_imp_corr(double* imp, int nimp, double dt, int r2, int r3, int v1, int v2, int v3, int a1,
                            int a2, double* scal) {
-  const int m = 3 * nimp;
-  double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *PCR = imp + IMP_PCR, *PCV = imp + IMP_PCV;
-  double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
-  double st[12];
-  double e1 = 0.0, e2 = 0.0;
-  for (int i = 0; i < m; ++i)
-    st[i] = 0.125 * (9.0 * RO[12 * r3 + i] - RO[12 * r2 + i] +
-                     3.0 * dt * (VO[12 * v3 + i] + 2.0 * V[i] - VO[12 * v1 + i]));
-  for (int i = 0; i < m; ++i) {
-    PCR[i] = PCR[i] - st[i];
-    R[i] = st[i] + C9 * PCR[i];
-    e1 += fabs(C9 * PCR[i]);
-  }
-  for (int i = 0; i < m; ++i)
-    st[i] = 0.125 * (9.0 * V[i] - VO[12 * v2 + i] + 3.0 * dt * (A[i] + 2.0 * AO[12 * a1 + i] - AO[12 * a2 + i]));
-  for (int i = 0; i < m; ++i) {
-    PCV[i] = PCV[i] - st[i];
+  __shared__ double err[2][12];
+  const int m = 3 * nimp, i = threadIdx.x;
+  if (i < m) {
+    double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *PCR = imp + IMP_PCR, *PCV = imp + IMP_PCV;
+    double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
+    const double sr = 0.125 * (9.0 * RO[12 * r3 + i] - RO[12 * r2 + i] +
+                               3.0 * dt * (VO[12 * v3 + i] + 2.0 * V[i] - VO[12 * v1 + i]));
+    const double pr = PCR[i] - sr;
+    PCR[i] = pr;
+    R[i] = sr + C9 * pr;
+    err[0][i] = fabs(C9 * pr);
+    const double sv = 0.125 * (9.0 * V[i] - VO[12 * v2 + i] + 3.0 * dt * (A[i] + 2.0 * AO[12 * a1 + i] - AO[12 * a2 + i]));
+    const double pv = PCV[i] - sv;
+    PCV[i] = pv;
     VO[12 * v3 + i] = V[i];
-    V[i] = st[i] + C9 * PCV[i];
-    e2 += fabs(C9 * PCV[i]);
+    V[i] = sv + C9 * pv;
+    err[1][i] = fabs(C9 * pv);
   }
-  scal[S_ERRIMP] = e1 * 0.3333333333 / nimp;
-  scal[S_ERRVIMP] = e2 * 0.3333333333 / nimp;
+  __syncthreads();
+  if (i == 0) {
+    double e1 = 0.0, e2 = 0.0;
+    for (int k = 0; k < m; ++k) {
+      e1 += err[0][k];
+      e2 += err[1][k];
+    }
+    scal[S_ERRIMP] = e1 * 0.3333333333 / nimp;
+    scal[S_ERRVIMP] = e2 * 0.3333333333 / nimp;
+  }
 }
 
 __global__ void k_imp_aimp(double* imp, int nimp) {
@@ -991,10 +996,10 @@ static void flush_side_pre(hedft_ctx* c) {
   if (c->side_pre & 8)
     finalize<1>(c, x_partials(c), c->side_nblocks, S_ERRPCX, 1.0);
   if (c->side_pre & 1)
-    hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
+    hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(64), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
                        a[3], a[4], a[5]);
   if (c->side_pre & 2)
-    hipLaunchKernelGGL(k_imp_corr, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
+    hipLaunchKernelGGL(k_imp_corr, dim3(1), dim3(64), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
                        a[3], a[4], a[5], a[6], c->scal);
   c->side_pre = 0;
 }
@@ -1260,7 +1265,7 @@ static int steprk_t(hedft_ctx* c, double dt) {
     if (c->quantum) steprk_x<R>(c, dt, jrun);
     c->lap_fresh = 0;
     if (c->nimp > 0)
-      hipLaunchKernelGGL(k_imp_rk, dim3(1), dim3(1), 0, c->stream, c->imp, c->nimp, (int)(jrun == 0), RK_A[jrun],
+      hipLaunchKernelGGL(k_imp_rk, dim3(1), dim3(64), 0, c->stream, c->imp, c->nimp, (int)(jrun == 0), RK_A[jrun],
                          RK_B[jrun], RK_C[jrun], dt);
     if (jrun <= 2) HEDFT_TRY(poten_tddft(c, 0, 1));
   }
