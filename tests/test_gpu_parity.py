@@ -18,7 +18,7 @@ def _say(*a):
     print("[case]", *a, flush=True)
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32), (96, 48, 64)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32), (96, 48, 64), (128, 128, 32)])
 def test_fft_roundtrip_vs_numpy(gpu, shape):
     nx, ny, nz = shape
     _say("fft", shape)
@@ -172,6 +172,60 @@ def test_tddft_trajectory_vs_oracle(gpu, otc):
         assert relerr(sg[k], so[k]) < RTOL_TRAJ, k
     eg, eo = d.energy(), o.energy()
     for k in ("ekin4", "elj4", "ecor4", "ealphas", "eHeX", "ekinx", "etot", "norm"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
+    d.close()
+
+
+# 128 x 128 planes (FP64) take their own plane-FFT kernels; a thin slab in z
+# keeps the oracle's share of these tests to seconds
+SLAB = (128, 128, 32, 25.6, 25.6, 6.4)
+
+
+def _slab_pair(mode, **kw):
+    import he_dft_2024_amd as H
+    nx, ny, nz, xm, ym, zm = SLAB
+    d = H.Droplet(nx, ny, nz, xm, ym, zm, H.default_params(**kw), mode=mode)
+    o = O.Oracle(nx, ny, nz, xm, ym, zm, O.default_params(**kw), mode=mode)
+    return d, o
+
+
+def _slab_setup(x, seed):
+    nx, ny, nz, xm, ym, zm = SLAB
+    x.set("DEN", I.woods_saxon(nx, ny, nz, xm, ym, zm, 40.0, seed=seed, center=(1.3, -0.7, 0.0)))
+    potion, delinter, rmax = I.impurity_table(nx, xm)
+    x.set_impurities(potion[None, :], delinter, rmax, [108.0 * I.MP_U])
+    x.set_impurity_state(np.array([[-2.0, 1.0, 0.2]]), np.array([[3.0, 0.0, 0.0]]) * I.PSTOK, np.zeros((1, 3)))
+
+
+@pytest.mark.parametrize("otc", [1, 0])
+def test_poten_tddft_128_planes_vs_oracle(gpu, otc):
+    _say("poten tddft 128x128x32 otc", otc)
+    d, o = _slab_pair(gpu.MODE_TDDFT, core_otc=otc)
+    for x in (d, o):
+        _slab_setup(x, seed=11)
+        x.poten()
+    for name in MF_FIELDS + (ALPHA_FIELDS if otc else []):
+        assert relerr(d.field(name), o.field(name)) < RTOL_OP, name
+    d.close()
+
+
+def test_tddft_trajectory_128_planes_vs_oracle(gpu):
+    _say("tddft 128x128x32 3 RK + 3 PC otc")
+    d, o = _slab_pair(gpu.MODE_TDDFT, core_otc=1)
+    dt = 5e-4 / I.PSTOK
+    errs = []
+    for x in (d, o):
+        _slab_setup(x, seed=12)
+        x.tddft_init()
+        errs.append(np.array([x.tddft_iteration(dt, it, 3) for it in range(1, 7)]))
+    assert relerr(errs[0][3:, 0], errs[1][3:, 0]) < RTOL_TRAJ
+    for name in ("PSI", "DEN", "POT4", "PSIOLD1", "HPSIOLD1", "UIMP"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    sg, so = d.impurity_state(), o.impurity_state()
+    for k in ("rimp", "vimp", "force"):
+        assert relerr(sg[k], so[k]) < RTOL_TRAJ, k
+    eg, eo = d.energy(), o.energy()
+    for k in ("ekin4", "elj4", "ecor4", "ealphas", "eHeX", "etot"):
         assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
     d.close()
 
