@@ -196,7 +196,7 @@ struct hedft_ctx {
   int device;
   hipStream_t stream;  // every launch goes to this stream
   hipStream_t main_stream, side_stream;  // stream is main_stream except inside a side branch
-  hipEvent_t ev0, ev1, ev_fork, ev_join;
+  hipEvent_t ev0, ev1, ev_fork, ev_join, ev_mid;
   hedft::Dims d;
   double hx, hy, hz, dxyz, renor;
   // real N fields
@@ -261,7 +261,7 @@ struct hedft_ctx {
   int no_side;         // 1: run the side branches on the main stream
   int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
-  int lap_late;        // 1: the next step's x+y Laplacians run beside stages B/C, 0: beside the den convolutions
+  int lap_late;        // where the next step's x+y Laplacians run (hedft_tuning.lap_late)
   // quantum impurity (Dynamics_quantum_impurities): one impurity wave function
   // psix with its own PC history, coupled through the real half-spectrum
   // kernel uimpk: potx4 = F^-1[F(|psix|^2) uimpk] enters pot4, upotx =

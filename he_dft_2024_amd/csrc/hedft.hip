@@ -1050,9 +1050,12 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   // where the next step's x+y Laplacians run: beside the den convolutions
   // (early) or beside stages B and C (late); lap_late 2 puts them late only in
   // the mean field that also carries the impurity force, 3 on the main stream
-  // after the den convolutions (beside the rest of the side branch)
+  // after the den convolutions (beside the rest of the side branch), 4 on the
+  // side stream after grad(den) with stage B waiting only for grad(den) (the
+  // Laplacians are joined with stage C's branch, or after stage B)
   const bool late = with_lap && (c->lap_late == 1 || (c->lap_late == 2 && want_force));
   const bool on_main = with_lap && c->lap_late == 3;
+  const bool mid = with_lap && c->lap_late == 4 && !c->no_side;
   HEDFT_TRY(fork_point(c));
   XInT<R> xi{};
   xi.op = XIN_COPY;
@@ -1067,6 +1070,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     // (Dynamics_quantum_impurities/poten.f90:48): nothing else reads grad(den) there
     if (!c->quantum || c->p.core_otc) HEDFT_TRY(derden_force(c, want_force, want_force));
     if (c->quantum) HEDFT_TRY(potx4_conv<R>(c, c->potx4));
+    if (mid) HEDFT_CHECK(hipEventRecord(c->ev_mid, c->side_stream));
     if (with_lap && !late && !on_main) {
       HEDFT_TRY(lap_xy(c, c->psi, 1));
       if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
@@ -1103,7 +1107,8 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     HEDFT_TRY(lap_xy(c, c->psi, 1));
     if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
   }
-  HEDFT_TRY(join(c));
+  if (mid) HEDFT_CHECK(hipStreamWaitEvent(c->main_stream, c->ev_mid, 0));
+  else HEDFT_TRY(join(c));
   // the x+y Laplacians for the next step run beside stages B and C (they read
   // only psi and psix, which the mean field does not change); stage C's own
   // branch and its join come after them on the side stream
@@ -1118,7 +1123,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   if (with_lap) c->lap_fresh = 1;
   HEDFT_TRY(stage_b<R>(c));
   if (c->p.core_otc) HEDFT_TRY(stage_c<R>(c));
-  else if (late) HEDFT_TRY(join(c));
+  else if (late || mid) HEDFT_TRY(join(c));
   return HEDFT_OK;
 }
 
@@ -1692,9 +1697,9 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   c->mode = mode;
   c->mixed = (p->precision == HEDFT_PREC_MIXED);
   c->zconv_fused = 1;
-  // measured at 128^3: beside the den convolutions for double storage, on the
-  // main stream after them for float storage (3 % faster there)
-  c->lap_late = c->mixed ? 3 : 0;
+  // measured at 128^3: on the side stream after grad(den) for double storage,
+  // on the main stream after the den convolutions for float storage
+  c->lap_late = c->mixed ? 3 : 4;
   if (const char* e = std::getenv("HEDFT_LAP_LATE")) c->lap_late = std::atoi(e);
   if (const char* e = std::getenv("HEDFT_XY_SPLIT_NB")) {  // tuning override: "fwd,inv"
     c->xy_split_nb[0] = std::atoi(e);
@@ -1718,6 +1723,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   hipEventCreate(&c->ev1);
   hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming);
   Dims& d = c->d;
   d.nx = g->nx;
   d.ny = g->ny;
@@ -1831,6 +1837,7 @@ int hedft_destroy(hedft_ctx* c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_mid) hipEventDestroy(c->ev_mid);
   if (c->side_stream) hipStreamDestroy(c->side_stream);
   if (c->main_stream) hipStreamDestroy(c->main_stream);
   delete c;
@@ -2664,7 +2671,7 @@ int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tunin
   for (int sp = 0; sp < 4; ++sp)
     for (int ns = 0; ns < 2; ++ns)
       for (int eg = 0; eg < 2; ++eg)
-        for (int ll = 0; ll < 4; ++ll) {
+        for (int ll = 0; ll < 5; ++ll) {
           cur.xy_split_fwd = split[sp][0];
           cur.xy_split_inv = split[sp][1];
           cur.no_side = ns;

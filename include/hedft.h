@@ -401,13 +401,14 @@ typedef struct hedft_tuning {
                                 half of the mean field, 2 the latter only in
                                 the mean field that carries the impurity force,
                                 3 on the main stream after the density
-                                convolutions */
+                                convolutions, 4 on the side stream after
+                                grad(den), joined after stage B starts */
   int32_t pad_;
   double ms_per_step;        /* measured time of the winning choice (autotune) */
 } hedft_tuning;
 /* Times trial_steps PC iterations (rounded up to 6, after a 6-iteration
- * warm-up) for each of the 64 combinations on this context and keeps the
- * fastest.  The context advances by 64 (6 + trial) iterations. */
+ * warm-up) for each of the 80 combinations on this context and keeps the
+ * fastest.  The context advances by 80 (6 + trial) iterations. */
 int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tuning* best);
 int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t);
 int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t);
