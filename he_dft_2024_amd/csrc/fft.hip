@@ -892,24 +892,24 @@ template <int L, int AX, bool INV, typename R>
 __global__ void __launch_bounds__(ColTile<L>::NT)
     k_colfft(SpecBatch<R> a, Dims d, long long ncol, const cplx_t<R>* __restrict__ tw) {
   using C = cplx_t<R>;
-  constexpr int TX = ColTile<L>::TX, NT = ColTile<L>::NT;
+  constexpr int TX = ColTile<L>::TX, T = Rad<L>::T, R1 = L / T;
   __shared__ C lds[L * TX];
   C* s = a.p[blockIdx.y];
+  // thread (column xl0, t0) loads the first pass's operands t0 + j T straight
+  // from global memory and the last pass stores straight back (in place: the
+  // workgroup owns its columns and has read them all before the first barrier)
+  const int xl0 = threadIdx.x % TX, t0 = threadIdx.x / TX;
   const long long c0 = (long long)blockIdx.x * TX;
-  lds_fill<8, NT>(lds, TX * L, [&](int e) {
-    const int xl = e % TX, j = e / TX;
-    size_t st;
-    const long long c = min(c0 + xl, ncol - 1);  // clamped columns are never stored
-    return s[col_addr<AX>(d, c, st) + j * st];
+  const long long col = min(c0 + xl0, ncol - 1);  // clamped columns are never stored
+  const bool live = c0 + xl0 < ncol;
+  size_t st;
+  const size_t base = col_addr<AX>(d, col, st);
+  C u[R1];
+#pragma unroll
+  for (int j = 0; j < R1; ++j) u[j] = s[base + (size_t)(t0 + j * T) * st];
+  Rad<L>::template run_regs_to<INV>(u, lds + xl0, TX, t0, tw, [&](int z, C v) {
+    if (live) s[base + (size_t)z * st] = v;
   });
-  __syncthreads();
-  Rad<L>::template run<INV>(lds + threadIdx.x % TX, TX, threadIdx.x / TX, tw);
-  for (int e = threadIdx.x; e < TX * L; e += NT) {
-    const int xl = e % TX, j = e / TX;
-    size_t st;
-    const long long c = c0 + xl;
-    if (c < ncol) s[col_addr<AX>(d, c, st) + j * st] = lds[e];
-  }
 }
 
 // z convolution: [forward z] -> (x kern_{b,o} -> inverse z -> out_{b,o}) for o < NOUT.
