@@ -334,7 +334,10 @@ enum SInOp { SIN_PLAIN = 0, SIN_FINT = 1 };
 // plain derivnD of a field (real or complex) along axis (0..2)
 int deriv_plain(hedft_ctx* c, int id, int axis, const void* in, void* out, int is_complex);
 // grad(den) -> dxden,dyden,dzden with impurity force partials (forceimp)
-int derden_force(hedft_ctx* c, int want_force, int update_aimp);
+// grad(den) and, with want_force, the impurity force partials; finalize = 0
+// leaves their reduction to derden_force_finalize (queued later on the stream)
+int derden_force(hedft_ctx* c, int want_force, int update_aimp, int finalize = 1);
+int derden_force_finalize(hedft_ctx* c, int update_aimp);
 // divergence of (1-denalf/den0s)*int_a, then ualphas and pot4 assembly
 int alpha_div_xy(hedft_ctx* c);      // x+y divergence of f int_a into work1
 int alpha_div_z_pot4(hedft_ctx* c);  // z divergence + ualphas + pot4 assembly

@@ -989,19 +989,26 @@ static int alpha_term(hedft_ctx* c) {
 
 // the serial kernels a step left for the side branch (impurity Newton PC,
 // errHe reduction); also called directly where no mean field follows
-static void flush_side_pre(hedft_ctx* c) {
-  const int* a = c->side_args;
+static void flush_side_errors(hedft_ctx* c) {
   if (c->side_pre & 4)
     finalize<1>(c, gen_partials(c), c->side_nblocks, S_ERRPC, 1.0);
   if (c->side_pre & 8)
     finalize<1>(c, x_partials(c), c->side_nblocks, S_ERRPCX, 1.0);
+  c->side_pre &= ~(4 | 8);
+}
+static void flush_side_impurity(hedft_ctx* c) {
+  const int* a = c->side_args;
   if (c->side_pre & 1)
     hipLaunchKernelGGL(k_imp_pred, dim3(1), dim3(64), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
                        a[3], a[4], a[5]);
   if (c->side_pre & 2)
     hipLaunchKernelGGL(k_imp_corr, dim3(1), dim3(64), 0, c->stream, c->imp, c->nimp, c->side_dt, a[0], a[1], a[2],
                        a[3], a[4], a[5], a[6], c->scal);
-  c->side_pre = 0;
+  c->side_pre &= ~(1 | 2);
+}
+static void flush_side_pre(hedft_ctx* c) {
+  flush_side_errors(c);
+  flush_side_impurity(c);
 }
 
 // potx4 = F^-1[F(|psix|^2) uimp_k] of a quantum impurity
@@ -1065,12 +1072,21 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   HEDFT_TRY(fork_side(c));
   {
     SideBranch sb(c);
-    flush_side_pre(c);  // impurity PC and errHe of the step just queued
+    // impurity PC and errHe of the step just queued; with the mid-branch join
+    // the errHe sum and the force reduction (read only by the next step's
+    // impurity PC) wait until grad(den) is done
+    const bool derden = !c->quantum || c->p.core_otc;
+    if (!mid) flush_side_errors(c);
+    flush_side_impurity(c);
     // the quantum-impurity poten calls derden only for the OTC core
     // (Dynamics_quantum_impurities/poten.f90:48): nothing else reads grad(den) there
-    if (!c->quantum || c->p.core_otc) HEDFT_TRY(derden_force(c, want_force, want_force));
+    if (derden) HEDFT_TRY(derden_force(c, want_force, want_force, !mid));
     if (c->quantum) HEDFT_TRY(potx4_conv<R>(c, c->potx4));
-    if (mid) HEDFT_CHECK(hipEventRecord(c->ev_mid, c->side_stream));
+    if (mid) {
+      HEDFT_CHECK(hipEventRecord(c->ev_mid, c->side_stream));
+      if (derden && want_force) HEDFT_TRY(derden_force_finalize(c, want_force));
+      flush_side_errors(c);
+    }
     if (with_lap && !late && !on_main) {
       HEDFT_TRY(lap_xy(c, c->psi, 1));
       if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));

@@ -167,7 +167,7 @@ double* force_partials(hedft_ctx* c);
 int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr);
 
 template <typename R>
-static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp) {
+static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp, int finalize) {
   want_force = want_force && c->nimp > 0;
   R* den = reinterpret_cast<R*>(c->den);
   R* dx = reinterpret_cast<R*>(c->dxden);
@@ -185,13 +185,19 @@ static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp) {
     launch_pencil<R, 2>(c->stream, c->d, c->coef[0][2], ip,
                         OutGradZForce<R, 0>{dz, dx, dy, nullptr, make_impdev(c)});
   if (hipGetLastError() != hipSuccess) return HEDFT_EHIP;
-  const int nr = (c->nimp == 1) ? 3 : 3 * HEDFT_MAX_IMP;  // accumulators per block of the pass above
-  return want_force ? force_finalize(c, pencil_geom<R>(c->d, 2).nblocks, update_aimp, nr) : HEDFT_OK;
+  return (want_force && finalize) ? derden_force_finalize(c, update_aimp) : HEDFT_OK;
 }
 
-int derden_force(hedft_ctx* c, int want_force, int update_aimp) {
-  return c->mixed ? derden_force_t<float>(c, want_force, update_aimp)
-                  : derden_force_t<double>(c, want_force, update_aimp);
+int derden_force(hedft_ctx* c, int want_force, int update_aimp, int finalize) {
+  return c->mixed ? derden_force_t<float>(c, want_force, update_aimp, finalize)
+                  : derden_force_t<double>(c, want_force, update_aimp, finalize);
+}
+
+int derden_force_finalize(hedft_ctx* c, int update_aimp) {
+  if (c->nimp <= 0) return HEDFT_OK;
+  const int nr = (c->nimp == 1) ? 3 : 3 * HEDFT_MAX_IMP;  // accumulators per block of the z pass
+  const size_t nb = c->mixed ? pencil_geom<float>(c->d, 2).nblocks : pencil_geom<double>(c->d, 2).nblocks;
+  return force_finalize(c, nb, update_aimp, nr);
 }
 
 // z pass of the alpha_s divergence with the mean-field assembly:
