@@ -224,25 +224,26 @@ struct PencilGeom {
   size_t nblocks;
 };
 
-// side-stream stencil passes sized to share CUs with the plane FFTs
-// (HEDFT_SIDE_FIT=0 in the environment selects the wider tiles)
-inline bool side_fit() {
+// side-stream stencil passes sized to share CUs with the plane FFTs, on grids
+// whose square planes take the whole-plane kernels (HEDFT_SIDE_FIT=0 in the
+// environment selects the wider tiles everywhere)
+inline bool side_fit(const Dims& d) {
   static const int enabled = [] {
     const char* e = std::getenv("HEDFT_SIDE_FIT");
     return (e && e[0] == '0') ? 0 : 1;
   }();
-  return enabled;
+  return enabled && d.nx == d.ny && d.nx <= 128;
 }
 
 // column-tile width of the y/z pencil kernels: 256 B rows (64 float, 32
 // double / float2, 16 double2 columns); 16 double2 columns need 64 KB of LDS
-// at n = 256, so 8 there.  Double columns are 16 wide (128 B rows) up to
-// n = 128 when side passes fit beside the plane FFTs: 16 KiB of LDS instead
-// of 32 KiB, under the 30 KiB a plane workgroup leaves free.
+// at n = 256, so 8 there.  Double columns are 16 wide (128 B rows) when side
+// passes fit beside the plane FFTs: 16 KiB of LDS at n = 128 instead of
+// 32 KiB, under the 30 KiB a plane workgroup leaves free.
 template <typename V>
-inline int pencil_tx(int n) {
+inline int pencil_tx(const Dims& d, int n) {
   if (sizeof(V) == 4) return 64;
-  if (sizeof(V) == 8) return (side_fit() && n <= 128) ? 16 : 32;
+  if (sizeof(V) == 8) return side_fit(d) ? 16 : 32;
   return (n > 128) ? 8 : 16;
 }
 
@@ -259,7 +260,7 @@ inline PencilGeom pencil_geom(const Dims& d, int ax) {
     g.lds = sizeof(V) * (size_t)rows * d.nx;
   } else {
     const int n = (ax == 1) ? d.ny : d.nz;
-    const int tx = pencil_tx<V>(n);
+    const int tx = pencil_tx<V>(d, n);
     g.tx = tx;
     g.grid = dim3((d.nx + tx - 1) / tx, (ax == 1) ? d.nz : d.ny);
     g.lds = sizeof(V) * (size_t)tx * n;
@@ -585,7 +586,7 @@ inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cf
       hipLaunchKernelGGL((k_pencil_xy2<V, SAME, T::TX, T::TY, In, Out>), grid, dim3(256), T::LDS, s, in, out, cfx,
                          cfy, d);
     };
-    if (side_fit()) {
+    if (side_fit(d)) {
       if (use_xy2<V, SAME, true>(d)) return xy2(std::true_type{});
     } else if (use_xy2<V, SAME, false>(d)) {
       return xy2(std::false_type{});
