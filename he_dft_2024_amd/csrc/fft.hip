@@ -788,31 +788,30 @@ template <int M, int ROWS, typename R>
 __global__ void __launch_bounds__(Rad<M>::T * ROWS)
     k_xr2c(XInBatch<R> a, Dims d, const cplx_t<R>* __restrict__ tw, const cplx_t<R>* __restrict__ twp) {
   using C = cplx_t<R>;
-  constexpr int T = Rad<M>::T;
+  constexpr int T = Rad<M>::T, R1 = M / T;
   __shared__ C lds[ROWS][M + 1];
   const XInT<R>& in = a.in[blockIdx.y];
   C* out = a.out[blockIdx.y];
   const int nrows = d.ny * d.nz;
-  const int row0 = blockIdx.x * ROWS;
-  lds_fill<4, T * ROWS>(&lds[0][0], ROWS * (M + 1), [&](int e) {
-    const int r = e / (M + 1), j = e - r * (M + 1);
-    const int row = row0 + r;
-    C v = mkc<C>((R)0, (R)0);
-    if (row < nrows && j < M) {
-      const size_t i = (size_t)row * (2 * M) + 2 * j;
+  const int t = threadIdx.x % T, r = threadIdx.x / T;
+  const int row = blockIdx.x * ROWS + r;
+  // the first pass's operands (pairs t + j T of the row) straight from global
+  // memory; rows past the end load zeros and store nothing
+  C u[R1];
+#pragma unroll
+  for (int j = 0; j < R1; ++j) {
+    u[j] = mkc<C>((R)0, (R)0);
+    if (row < nrows) {
+      const size_t i = (size_t)row * (2 * M) + 2 * (t + j * T);
       if (in.op == XIN_COPY) {
-        v = *reinterpret_cast<const C*>(in.a[0] + i);
+        u[j] = *reinterpret_cast<const C*>(in.a[0] + i);
       } else {
-        v.x = (R)xin_eval(in, i);
-        v.y = (R)xin_eval(in, i + 1);
+        u[j].x = (R)xin_eval(in, i);
+        u[j].y = (R)xin_eval(in, i + 1);
       }
     }
-    return v;
-  });
-  __syncthreads();
-  const int t = threadIdx.x % T, r = threadIdx.x / T;
-  Rad<M>::template run<false>(&lds[r][0], 1, t, tw);
-  const int row = row0 + r;
+  }
+  Rad<M>::template run_from_regs<false>(u, &lds[r][0], 1, t, tw);
   if (row < nrows) {
     for (int k = t; k <= M; k += T) {
       const C zk = lds[r][k == M ? 0 : k];
@@ -823,44 +822,44 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
 }
 
 // ---------------------------------------------------------------------------
-// x C2R: M+1 complex -> nx = 2M reals (imaginary parts of X_0, X_M ignored)
+// x C2R: M+1 complex -> nx = 2M reals (imaginary parts of X_0, X_M ignored).
+// The c2r merge of the pair (k, M-k) is formed from global memory straight
+// into the first pass's registers, and the last pass writes the reals through
+// the epilogue.
 // ---------------------------------------------------------------------------
 template <int M, int ROWS, typename R>
 __global__ void __launch_bounds__(Rad<M>::T * ROWS)
     k_xc2r(XOutBatch<R> a, Dims d, const cplx_t<R>* __restrict__ tw, const cplx_t<R>* __restrict__ twp,
            double renor) {
   using C = cplx_t<R>;
-  constexpr int T = Rad<M>::T;
+  constexpr int T = Rad<M>::T, R1 = M / T;
   __shared__ C lds[ROWS][M + 1];
   const C* in = a.in[blockIdx.y];
   const XOutT<R>& out = a.out[blockIdx.y];
   const int nrows = d.ny * d.nz;
-  const int row0 = blockIdx.x * ROWS;
-  lds_fill<4, T * ROWS>(&lds[0][0], ROWS * (M + 1), [&](int e) {
-    const int r = e / (M + 1), k = e - r * (M + 1);
-    const int row = row0 + r;
-    C z = mkc<C>((R)0, (R)0);
-    if (row < nrows && k < M) {
-      const C* h = in + (size_t)row * (M + 1);
-      C xk = h[k];
-      C xm = h[M - k];
-      if (k == 0) { xk.y = 0; xm.y = 0; }
-      z = merge_one(xk, xm, twp[k]);
-    }
-    return z;
-  });
-  __syncthreads();
   const int t = threadIdx.x % T, r = threadIdx.x / T;
-  Rad<M>::template run<true>(&lds[r][0], 1, t, tw);
-  const int row = row0 + r;
-  if (row < nrows) {
-    for (int j = t; j < M; j += T) {
-      const C v = lds[r][j];
-      const size_t i = (size_t)row * (2 * M) + 2 * j;
-      xout_apply(out, i, (double)v.x * renor);
-      xout_apply(out, i + 1, (double)v.y * renor);
+  const int row = blockIdx.x * ROWS + r;
+  const bool live = row < nrows;
+  const C* h = in + (size_t)min(row, nrows - 1) * (M + 1);
+  C u[R1];
+#pragma unroll
+  for (int j = 0; j < R1; ++j) {
+    const int k = t + j * T;
+    C xk = h[k];
+    C xm = h[M - k];
+    if (k == 0) {
+      xk.y = 0;
+      xm.y = 0;
     }
+    u[j] = merge_one(xk, xm, twp[k]);
   }
+  const size_t o = (size_t)min(row, nrows - 1) * (2 * M);
+  Rad<M>::template run_regs_to<true>(u, &lds[r][0], 1, t, tw, [&](int z, C v) {
+    if (live) {
+      xout_apply(out, o + 2 * z, (double)v.x * renor);
+      xout_apply(out, o + 2 * z + 1, (double)v.y * renor);
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
