@@ -429,6 +429,47 @@ struct ZConvArgs {
   int do_fwd;
 };
 
+// Calls use(getv) with the loader of the x-pass prologue operation, getv(i)
+// returning the reals i, i+1 of the transformed field as one complex (the
+// operation is uniform per workgroup, so each gets one branch-free load
+// sequence).
+template <typename R, typename Use>
+__device__ __forceinline__ void with_xin_pairs(const XInT<R>& in, Use use) {
+  using C = cplx_t<R>;
+  auto ld2 = [](const R* p, size_t i) { return *reinterpret_cast<const C*>(p + i); };
+  switch (in.op) {
+    case XIN_CORR:
+      use([&](size_t i) {
+        const C dn = ld2(in.a[0], i), cg = ld2(in.a[1], i);
+        return mkc<C>((R)((double)dn.x * cg.x * (in.c0 + in.c1 * cg.x)),
+                      (R)((double)dn.y * cg.y * (in.c0 + in.c1 * cg.y)));
+      });
+      break;
+    case XIN_FGRAD:
+      use([&](size_t i) {
+        const C al = ld2(in.a[0], i), gr = ld2(in.a[1], i);
+        return mkc<C>((R)((1.0 - (double)al.x / in.c0) * gr.x), (R)((1.0 - (double)al.y / in.c0) * gr.y));
+      });
+      break;
+    case XIN_DOT3:
+      use([&](size_t i) {
+        const C g0 = ld2(in.a[0], i), g1 = ld2(in.a[1], i), g2 = ld2(in.a[2], i);
+        const C i0 = ld2(in.a[3], i), i1 = ld2(in.a[4], i), i2 = ld2(in.a[5], i);
+        return mkc<C>((R)((double)g0.x * i0.x + (double)g1.x * i1.x + (double)g2.x * i2.x),
+                      (R)((double)g0.y * i0.y + (double)g1.y * i1.y + (double)g2.y * i2.y));
+      });
+      break;
+    case XIN_ABS2:
+      use([&](size_t i) {
+        const double2 z0 = in.z[i], z1 = in.z[i + 1];
+        return mkc<C>((R)(z0.x * z0.x + z0.y * z0.y), (R)(z1.x * z1.x + z1.y * z1.y));
+      });
+      break;
+    default:
+      use([&](size_t i) { return ld2(in.a[0], i); });
+  }
+}
+
 // ---------------------------------------------------------------------------
 // fused x + y passes of one z-plane held entirely in LDS (NXH x NY complex,
 // up to 160 KB on gfx950): forward = rows R2C then columns FFT; inverse =
@@ -492,39 +533,7 @@ __global__ void __launch_bounds__((Plane<NX, NY, R>::NT)) __attribute__((amdgpu_
       }
     }
   };
-  // consecutive reals (2j, 2j+1) of a row as one complex
-  auto ld2 = [](const R* p, size_t i) { return *reinterpret_cast<const C*>(p + i); };
-  switch (in.op) {  // uniform per workgroup: one branch-free load loop per operation
-    case XIN_CORR:
-      fill([&](size_t i) {
-        const C dn = ld2(in.a[0], i), cg = ld2(in.a[1], i);
-        return mkc<C>((R)((double)dn.x * cg.x * (in.c0 + in.c1 * cg.x)),
-                      (R)((double)dn.y * cg.y * (in.c0 + in.c1 * cg.y)));
-      });
-      break;
-    case XIN_FGRAD:
-      fill([&](size_t i) {
-        const C al = ld2(in.a[0], i), gr = ld2(in.a[1], i);
-        return mkc<C>((R)((1.0 - (double)al.x / in.c0) * gr.x), (R)((1.0 - (double)al.y / in.c0) * gr.y));
-      });
-      break;
-    case XIN_DOT3:
-      fill([&](size_t i) {
-        const C g0 = ld2(in.a[0], i), g1 = ld2(in.a[1], i), g2 = ld2(in.a[2], i);
-        const C i0 = ld2(in.a[3], i), i1 = ld2(in.a[4], i), i2 = ld2(in.a[5], i);
-        return mkc<C>((R)((double)g0.x * i0.x + (double)g1.x * i1.x + (double)g2.x * i2.x),
-                      (R)((double)g0.y * i0.y + (double)g1.y * i1.y + (double)g2.y * i2.y));
-      });
-      break;
-    case XIN_ABS2:
-      fill([&](size_t i) {
-        const double2 z0 = in.z[i], z1 = in.z[i + 1];
-        return mkc<C>((R)(z0.x * z0.x + z0.y * z0.y), (R)(z1.x * z1.x + z1.y * z1.y));
-      });
-      break;
-    default:
-      fill([&](size_t i) { return ld2(in.a[0], i); });
-  }
+  with_xin_pairs(in, fill);
   __syncthreads();
   Rad<M>::template run_multi<false, NT, NY, true>(P, PAD, 1, twx);
   // real-to-half-complex split of every row, in place: X_k, k = 0..M, from
@@ -629,38 +638,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4)
 #pragma unroll
     for (int j = 0; j < 8; ++j) u[j] = getv(g0 + 16 * j);
   };
-  auto ld2 = [](const R* p, size_t k) { return *reinterpret_cast<const C*>(p + k); };
-  switch (in.op) {  // uniform per workgroup: one branch-free load sequence per operation
-    case XIN_CORR:
-      load([&](size_t k) {
-        const C dn = ld2(in.a[0], k), cg = ld2(in.a[1], k);
-        return mkc<C>((R)((double)dn.x * cg.x * (in.c0 + in.c1 * cg.x)),
-                      (R)((double)dn.y * cg.y * (in.c0 + in.c1 * cg.y)));
-      });
-      break;
-    case XIN_FGRAD:
-      load([&](size_t k) {
-        const C al = ld2(in.a[0], k), gr = ld2(in.a[1], k);
-        return mkc<C>((R)((1.0 - (double)al.x / in.c0) * gr.x), (R)((1.0 - (double)al.y / in.c0) * gr.y));
-      });
-      break;
-    case XIN_DOT3:
-      load([&](size_t k) {
-        const C g0v = ld2(in.a[0], k), g1 = ld2(in.a[1], k), g2 = ld2(in.a[2], k);
-        const C i0 = ld2(in.a[3], k), i1 = ld2(in.a[4], k), i2 = ld2(in.a[5], k);
-        return mkc<C>((R)((double)g0v.x * i0.x + (double)g1.x * i1.x + (double)g2.x * i2.x),
-                      (R)((double)g0v.y * i0.y + (double)g1.y * i1.y + (double)g2.y * i2.y));
-      });
-      break;
-    case XIN_ABS2:
-      load([&](size_t k) {
-        const double2 z0 = in.z[k], z1 = in.z[k + 1];
-        return mkc<C>((R)(z0.x * z0.x + z0.y * z0.y), (R)(z1.x * z1.x + z1.y * z1.y));
-      });
-      break;
-    default:
-      load([&](size_t k) { return ld2(in.a[0], k); });
-  }
+  with_xin_pairs(in, load);
   // x pass 1 (radix 8, no twiddles): result j of butterfly i is Stockham position 8i + j
   dft<8, false>(u);
   C* row = P + c * PAD;
