@@ -18,7 +18,8 @@ def _say(*a):
     print("[case]", *a, flush=True)
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32), (96, 48, 64), (128, 128, 32)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32), (96, 48, 64), (128, 128, 32), (192, 192, 32),
+                                   (64, 32, 256)])
 def test_fft_roundtrip_vs_numpy(gpu, shape):
     nx, ny, nz = shape
     _say("fft", shape)
