@@ -237,15 +237,15 @@ inline bool side_fit(const Dims& d) {
 
 // column-tile width of the y/z pencil kernels: 32 float columns (128 B rows;
 // 64 columns left a 128^3 grid only 256 workgroups, measured 4 % slower on
-// the mixed step), 32 double, 16 double2; 16 double2 columns need 64 KB of
-// LDS at n = 256, so 8 there.  Double columns are 16 wide (128 B rows) when
-// side passes fit beside the plane FFTs: 16 KiB of LDS at n = 128 instead of
-// 32 KiB, under the 30 KiB a plane workgroup leaves free.
+// the mixed step), 32 double, 16 double2 (64 KB of LDS at n = 256, where 8
+// columns measured 5-7 % slower on the 256^3 steps).  Double columns are 16
+// wide (128 B rows) when side passes fit beside the plane FFTs: 16 KiB of LDS
+// at n = 128 instead of 32 KiB, under the 30 KiB a plane workgroup leaves free.
 template <typename V>
 inline int pencil_tx(const Dims& d, int n) {
   if (sizeof(V) == 4) return 32;
   if (sizeof(V) == 8) return side_fit(d) ? 16 : 32;
-  return (n > 128) ? 8 : 16;
+  return 16;
 }
 
 template <typename V>

@@ -231,6 +231,31 @@ def test_tddft_trajectory_128_planes_vs_oracle(gpu):
     d.close()
 
 
+def test_tddft_trajectory_long_z_vs_oracle(gpu):
+    """z pencils of length 256 (the 256^3 grids' column tiles) in the fused steppers."""
+    _say("tddft 32x32x256 3 RK + 3 PC otc")
+    import he_dft_2024_amd as H
+    nx, ny, nz, xm, ym, zm = 32, 32, 256, 6.4, 6.4, 51.2
+    d = H.Droplet(nx, ny, nz, xm, ym, zm, H.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+    o = O.Oracle(nx, ny, nz, xm, ym, zm, O.default_params(core_otc=1), mode=O.MODE_TDDFT)
+    dt = 5e-4 / I.PSTOK
+    errs = []
+    for x in (d, o):
+        x.set("DEN", I.woods_saxon(nx, ny, nz, xm, ym, zm, 30.0, seed=13, center=(0.3, -0.2, 4.0)))
+        potion, delinter, rmax = I.impurity_table(nz, zm)
+        x.set_impurities(potion[None, :], delinter, rmax, [108.0 * I.MP_U])
+        x.set_impurity_state(np.array([[0.0, 0.4, -2.0]]), np.array([[0.0, 0.0, 3.0]]) * I.PSTOK, np.zeros((1, 3)))
+        x.tddft_init()
+        errs.append(np.array([x.tddft_iteration(dt, it, 3) for it in range(1, 7)]))
+    assert relerr(errs[0][3:, 0], errs[1][3:, 0]) < RTOL_TRAJ
+    for name in ("PSI", "DEN", "POT4", "PSIOLD1", "HPSIOLD1"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    sg, so = d.impurity_state(), o.impurity_state()
+    for k in ("rimp", "force"):
+        assert relerr(sg[k], so[k]) < RTOL_TRAJ, k
+    d.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     """hedft_tddft_run_pc replays a captured 6-step graph; results equal eager steps."""
     n, xmax, n4 = 32, 6.4, 60.0
