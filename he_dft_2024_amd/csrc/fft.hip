@@ -847,10 +847,13 @@ __global__ void __launch_bounds__(Rad<M>::T * ROWS)
 // divides nothing in particular: the last tile is partial only at the very
 // end, so no tile is wasted on the single Nyquist column of every row.
 // ---------------------------------------------------------------------------
+// 16 columns per tile at every length (256 B of contiguous complex double
+// per row; at L = 256 a 512-thread tile of 64 KB measured 10 % faster on the
+// 256^3 mixed step than 8 columns)
 template <int L>
 struct ColTile {
   static constexpr int T = Rad<L>::T;
-  static constexpr int TX = (T <= 16) ? 16 : 8;
+  static constexpr int TX = 16;
   static constexpr int NT = T * TX;
 };
 
