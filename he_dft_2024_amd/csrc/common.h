@@ -211,6 +211,12 @@ struct hedft_ctx {
   void* workc1;     // complex N scratch
   // half spectra
   double2 *fden, *spec[4], *workk1;
+  // TDDFT: spectra whose inverse waits for stage C (delj4, and upotx with a
+  // quantum impurity, read only after the pot4 assembly)
+  double2* specd;
+  int ndefer;
+  double2* defer_spec[2];
+  double* defer_out[2];
   double *fvlj4, *wcgk, *kalfs;
   double* tw;       // twiddle tables (complex, interleaved)
   float* twf;       // the same tables rounded to FP32 (mixed path)

@@ -960,8 +960,17 @@ static int stage_c(hedft_ctx* c) {
   }
   const R* ka = fld<R>(c->kalfs);
   HEDFT_TRY(zconv<R>(c, 1, &s0, 1, &ka, &s0, 1, nullptr));
-  XOutT<R> xu = xo_store<R>(c->ualph_conv);
-  HEDFT_TRY(inv_yx<R>(c, 1, &s0, &xu));
+  // ualphas' convolution, with the density convolutions the mean field left
+  // for this inverse (delj4 [, upotx])
+  cplx_t<R>* su[3] = {s0, nullptr, nullptr};
+  XOutT<R> xu[3] = {xo_store<R>(c->ualph_conv), {}, {}};
+  int nu = 1;
+  for (int k = 0; k < c->ndefer; ++k) {
+    su[nu] = spc<R>(c->defer_spec[k]);
+    xu[nu++] = xo_store<R>(c->defer_out[k]);
+  }
+  c->ndefer = 0;
+  HEDFT_TRY(inv_yx<R>(c, nu, su, xu));
   HEDFT_TRY(join(c));
   return alpha_div_z_pot4(c);
 }
@@ -1092,9 +1101,13 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
       if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
     }
   }
-  // den convolutions: dencg, delj4 [, denalf] [, upotx]
+  // den convolutions: dencg, delj4 [, denalf] [, upotx].  With the OTC core,
+  // delj4 (read only by the pot4 assembly at the end of stage C) and upotx
+  // (read by the next psix step) are inverted with stage C's convolution, so
+  // that this inverse needs one round of workgroups over the CUs instead of two
+  const bool defer = c->p.core_otc && c->specd;
   const R* kk[4] = {fld<R>(c->wcgk), fld<R>(c->fvlj4), nullptr, nullptr};
-  cplx_t<R>* so[4] = {spc<R>(c->spec[1]), spc<R>(c->spec[2]), nullptr, nullptr};
+  cplx_t<R>* so[4] = {spc<R>(c->spec[1]), spc<R>(defer ? c->specd : c->spec[2]), nullptr, nullptr};
   XOutT<R> xo[4] = {xo_store<R>(c->dencg), xo_store<R>(c->delj4), {}, {}};
   int nk = 2;
   if (c->p.core_otc) {
@@ -1118,7 +1131,24 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     cplx_t<R>* si[4] = {s0, s0, s0, s0};
     HEDFT_TRY(zconv<R>(c, nk, si, 1, kk, so, 1, nullptr));
   }
-  HEDFT_TRY(inv_yx<R>(c, nk, so, xo));
+  c->ndefer = 0;
+  if (defer) {
+    cplx_t<R>* si[4];
+    XOutT<R> xi[4];
+    int ni = 0;
+    for (int k = 0; k < nk; ++k) {
+      if (xo[k].out == fld<R>(c->delj4) || (c->quantum && xo[k].out == fld<R>(c->upotx))) {
+        c->defer_spec[c->ndefer] = reinterpret_cast<double2*>(so[k]);
+        c->defer_out[c->ndefer++] = reinterpret_cast<double*>(xo[k].out);
+      } else {
+        si[ni] = so[k];
+        xi[ni++] = xo[k];
+      }
+    }
+    HEDFT_TRY(inv_yx<R>(c, ni, si, xi));
+  } else {
+    HEDFT_TRY(inv_yx<R>(c, nk, so, xo));
+  }
   if (on_main) {
     HEDFT_TRY(lap_xy(c, c->psi, 1));
     if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
@@ -1773,6 +1803,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   rc = rc ? rc : alloc(c, &c->workc1, cN);
   double2** specs[] = {&c->fden, &c->spec[0], &c->spec[1], &c->spec[2], &c->spec[3], &c->workk1};
   for (double2** f : specs) rc = rc ? rc : alloc(c, (void**)f, hN);
+  if (mode == HEDFT_MODE_TDDFT) rc = rc ? rc : alloc(c, (void**)&c->specd, hN);
   rc = rc ? rc : alloc(c, (void**)&c->fvlj4, kN);
   rc = rc ? rc : alloc(c, (void**)&c->wcgk, kN);
   rc = rc ? rc : alloc(c, (void**)&c->kalfs, kN);
@@ -1845,7 +1876,7 @@ int hedft_destroy(hedft_ctx* c) {
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
                   c->imp, c->lstop, c->potion, c->psix, c->psixold[0], c->psixold[1], c->psixold[2],
                   c->hpsixold[0], c->hpsixold[1], c->pcx, c->qx, c->lapx, c->potx4, c->upotx, c->uimpk,
-                  c->specq[0], c->specq[1]};
+                  c->specq[0], c->specq[1], c->specd};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (c->host_scal) hipHostFree(c->host_scal);
@@ -2530,6 +2561,7 @@ size_t ctx_footprint(int n, int mode, int precision) {
   const size_t pw = (mode == HEDFT_MODE_TDDFT) ? 16 : 8;
   size_t b = 17 * fw * N + 9 * pw * N + 16 * N + 6 * 2 * fw * Nh + 3 * fw * Nh;
   if (mode == HEDFT_MODE_STATICS) b += 4 * 8 * N;
+  else b += 2 * fw * Nh;  // the spectrum whose inverse waits for stage C
   return b + (8u << 20);  // partials, tables, twiddles, impurity block
 }
 }  // namespace hedft
