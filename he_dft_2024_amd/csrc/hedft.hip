@@ -546,7 +546,7 @@ __global__ void k_imp_pred(double* imp, int nimp, double dt, int r3, int v1, int
 // steppc.f90:530-555; errimp, errvimp into scal (summed by lane 0 in coordinate order)
 __global__ void k_imp_corr(double* imp, int nimp, double dt, int r2, int r3, int v1, int v2, int v3, int a1,
                            int a2, double* scal) {
-  __shared__ double err[2][12];
+  __shared__ double err[2][3 * HEDFT_MAX_IMP];
   const int m = 3 * nimp, i = threadIdx.x;
   if (i < m) {
     double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *PCR = imp + IMP_PCR, *PCV = imp + IMP_PCV;
