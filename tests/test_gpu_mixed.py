@@ -46,6 +46,24 @@ def test_mixed_poten_vs_fp64_oracle(gpu, otc):
     d.close()
 
 
+def test_mixed_poten_128_planes_vs_fp64_oracle(gpu):
+    """128 x 128 planes with float storage (the whole-plane FFT kernels of the mixed path)."""
+    nx, ny, nz, xm, ym, zm = 128, 128, 32, 25.6, 25.6, 6.4
+    print("[case] mixed poten 128x128x32", flush=True)
+    d = gpu.Droplet(nx, ny, nz, xm, ym, zm, gpu.default_params(precision=gpu.PREC_MIXED, core_otc=1),
+                    mode=gpu.MODE_TDDFT)
+    o = O.Oracle(nx, ny, nz, xm, ym, zm, O.default_params(core_otc=1), O.MODE_TDDFT)
+    den = I.woods_saxon(nx, ny, nz, xm, ym, zm, 40.0, seed=11, center=(1.3, -0.7, 0.0))
+    for x in (d, o):
+        x.set("DEN", den)
+        x.set_impurities(np.zeros((1, 10)), 1.0, 100.0, [1.0])
+        x.set_impurity_state(np.zeros((1, 3)), np.zeros((1, 3)), np.zeros((1, 3)))
+        x.poten()
+    for name in ["DENCG", "DELJ4", "DXDEN", "DYDEN", "DZDEN", "POT4", "DENALF", "UALPHAS"]:
+        assert relerr(d.field(name), o.field(name)) < RTOL_FIELD, name
+    d.close()
+
+
 def test_mixed_trajectory_energies_vs_fp64_oracle(gpu):
     n, xmax, n4 = 32, 6.4, 60.0
     print("[case] mixed 3 RK + 12 PC", flush=True)
@@ -85,7 +103,7 @@ def test_mixed_graph_replay_matches_eager(gpu):
 
 
 def test_mixed_256_against_fp64_path(gpu):
-    """256^3 (row/column FFT passes, 8-column complex tiles): the mixed path
+    """256^3 (row/column FFT passes, 256-long column tiles): the mixed path
     tracks the FP64 path, which the 32^3 trajectory tests pin to the oracle."""
     n, h, n4 = 256, 0.4, 5000.0
     xmax = 0.5 * n * h
