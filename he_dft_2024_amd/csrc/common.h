@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -125,6 +126,62 @@ __device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, 
   const double rmod = rem / im.delinter;
   const double* tab = im.potion + (size_t)k * im.npot;
   return tab[ir] * (1.0 - rmod) + tab[ir + 1] * rmod;
+}
+
+// ---- impurity-impurity pair terms (Dynamics_TDDFT/potenimp.f90:40-58,
+// energy.f90:168-177) ------------------------------------------------------------
+// The reference evaluates Select_pot(drselec_gs_k_k) for the pair force and
+// Select_pot(selec_gs_k_k) for the pair energy; here both come as radial
+// tables on r_i = i*del (i = 0..np-1), one pair after the other in the order
+// (1,2),(1,3),..,(1,N),(2,3),.., read with updatepoten's interpolation
+// (potenimp.f90:101-112).  r beyond the last node is clamped to it and
+// flagged.
+__host__ __device__ inline double radial_lerp(const double* t, int np, double del, double r, int* flag) {
+#pragma clang fp contract(off)
+  const double rmax = del * (double)(np - 1);
+  if (r > rmax) {
+    if (flag) *flag = 1;
+    r = rmax;
+  }
+  const double q = trunc(r / del);
+  int ir = (int)q;
+  if (ir > np - 2) ir = np - 2;
+  double rem = fma(-q, del, r);
+  if (rem < 0.0) rem += del;
+  else if (rem >= del) rem -= del;
+  const double rmod = rem / del;
+  return t[ir] * (1.0 - rmod) + t[ir + 1] * rmod;
+}
+
+// index of pair (k,m), k < m, among the nimp*(nimp-1)/2 pairs
+__host__ __device__ inline int pair_index(int k, int m, int nimp) { return k * nimp - k * (k + 1) / 2 + (m - k - 1); }
+
+struct PairDev {
+  const double* dv;  // force tables (drselec), npairs x np
+  int np;
+  double del;
+  int* lstop;
+};
+
+// F_ij(k,m,:) = -D_km(r) (R_k - R_m)/r for m > k, F_ij(m,k,:) = -F_ij(k,m,:)
+// (potenimp.f90:40-47); fij[(k*MAXIMP + m)*3 + a]
+__host__ __device__ inline void pair_forces(const double* rimp, int nimp, const double* dv, int np, double del,
+                                            int* flag, double* fij) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < HEDFT_MAX_IMP * HEDFT_MAX_IMP * 3; ++i) fij[i] = 0.0;
+  for (int k = 0; k < nimp; ++k)
+    for (int m = k + 1; m < nimp; ++m) {
+      const double dx = rimp[k] - rimp[m], dy = rimp[k + nimp] - rimp[m + nimp],
+                   dz = rimp[k + 2 * nimp] - rimp[m + 2 * nimp];
+      const double aux1 = sqrt(dx * dx + dy * dy + dz * dz);
+      const double s = radial_lerp(dv + (size_t)pair_index(k, m, nimp) * np, np, del, aux1, flag);
+      const double d[3] = {dx, dy, dz};
+      for (int a = 0; a < 3; ++a) {
+        const double f = -s * d[a] / aux1;
+        fij[(k * HEDFT_MAX_IMP + m) * 3 + a] = f;
+        fij[(m * HEDFT_MAX_IMP + k) * 3 + a] = -f;
+      }
+    }
 }
 
 // grid coordinates x_i = -xmax + h*i, rounded like the CPU evaluation
@@ -250,6 +307,12 @@ struct hedft_ctx {
   double* imp;      // device impurity state block (see physics.hip layout)
   int* lstop;       // device flag
   double mass[HEDFT_MAX_IMP];
+  // impurity-impurity pair tables (hedft_set_impurity_pairs): device force
+  // tables, host copy of the energy tables; required when nimp >= 2
+  int npair;
+  double delpair;
+  double* pairtab;      // device: npairs x npair force values (drselec)
+  double* pair_v_host;  // host: npairs x npair energy values (selec)
   double last_ms;
   // kernel-duration probe: events around the fused predictor (0) and
   // corrector (1) launches while kt_on

@@ -105,7 +105,7 @@ static void finalize(hedft_ctx* c, const double* part, size_t nblocks, int slot,
 // together), then one tree over the workgroup.
 template <int NR>
 __global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, double* imp,
-                                                        int nimp, double dxyz, int update_aimp) {
+                                                        int nimp, double dxyz, int update_aimp, PairDev pr) {
   __shared__ double sh[NR][256];
   double s[NR];
 #pragma unroll
@@ -124,23 +124,48 @@ __global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    // F(k,:) = -sum(grad(den) uimp_k) dxyz + sum_m F_ij(k,m,:)  (potenimp.f90:49-58)
+    double fij[HEDFT_MAX_IMP * HEDFT_MAX_IMP * 3];
+    if (nimp > 1 && pr.dv) pair_forces(imp + IMP_R, nimp, pr.dv, pr.np, pr.del, pr.lstop, fij);
     for (int k = 0; k < nimp && 3 * k < NR; ++k)
       for (int a = 0; a < 3; ++a) {
-        const double f = -sh[3 * k + a][0] * dxyz;
+        double f = -sh[3 * k + a][0] * dxyz;
+        if (nimp > 1 && pr.dv)
+          for (int m = 0; m < nimp; ++m) f += fij[(k * HEDFT_MAX_IMP + m) * 3 + a];
         imp[IMP_F + k + nimp * a] = f;
         if (update_aimp) imp[IMP_A + k + nimp * a] = f / imp[IMP_MASS + k];
       }
   }
 }
 
+static int npairs_of(int nimp) { return nimp * (nimp - 1) / 2; }
+static void clear_pairs(hedft_ctx* c) {
+  if (c->pairtab) hipFree(c->pairtab);
+  std::free(c->pair_v_host);
+  c->pairtab = nullptr;
+  c->pair_v_host = nullptr;
+  c->npair = 0;
+}
+// several classical impurities need their pair tables (the reference always
+// adds the pair force and energy); HEDFT_ESTATE until they are given
+static int imp_ready(const hedft_ctx* c) { return (c->nimp >= 2 && !c->pairtab) ? HEDFT_ESTATE : HEDFT_OK; }
+
+static PairDev make_pairdev(hedft_ctx* c) {
+  PairDev p;
+  p.dv = c->pairtab;
+  p.np = c->npair;
+  p.del = c->delpair;
+  p.lstop = c->lstop;
+  return p;
+}
+
 int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr) {
   if (nr == 3)
     hipLaunchKernelGGL(k_force_finalize<3>, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, c->imp,
-                       c->nimp, c->dxyz, update_aimp);
+                       c->nimp, c->dxyz, update_aimp, make_pairdev(c));
   else
     hipLaunchKernelGGL(k_force_finalize<3 * HEDFT_MAX_IMP>, dim3(1), dim3(256), 0, c->stream, force_partials(c),
-                       (int)nblocks, c->imp,
-                     c->nimp, c->dxyz, update_aimp);
+                       (int)nblocks, c->imp, c->nimp, c->dxyz, update_aimp, make_pairdev(c));
   return HEDFT_OK;
 }
 
@@ -149,8 +174,11 @@ int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr) {
 struct Lam {
   const double *tx, *ty, *tz;
   double lambdah;
+  // The reference assigns cmplx(Lambda,1.d0) without a KIND argument, i.e. a
+  // default-kind (REAL(4)) complex, to timec: Lambda is rounded to single
+  // precision before it enters the complex(8) array.
   __device__ __forceinline__ double at(int ix, int iy, int iz) const {
-    return lambdah * (1.0 + tx[ix] + 1.0 + ty[iy] + 1.0 + tz[iz]);
+    return (double)(float)(lambdah * (1.0 + tx[ix] + 1.0 + ty[iy] + 1.0 + tz[iz]));
   }
 };
 
@@ -1879,6 +1907,7 @@ int hedft_destroy(hedft_ctx* c) {
                   c->specq[0], c->specq[1], c->specd};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  clear_pairs(c);
   if (c->host_scal) hipHostFree(c->host_scal);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -2131,6 +2160,20 @@ int hedft_energy(hedft_ctx* c, hedft_energies* e) {
       e->ekinx += 0.5 * c->mass[k] * v2;
     }
     e->eHeX = s[6] * dv;
+    if (c->nimp >= 2 && c->pair_v_host) {  // energy.f90:168-177
+      double r[3 * HEDFT_MAX_IMP];
+      HEDFT_CHECK(hipMemcpy(r, c->imp + IMP_R, sizeof(double) * 3 * c->nimp, hipMemcpyDeviceToHost));
+      double eij = 0.0;
+      const int ni = c->nimp;
+      for (int k = 0; k < ni; ++k)
+        for (int m = k + 1; m < ni; ++m) {
+          const double dx = r[k] - r[m], dy = r[k + ni] - r[m + ni], dz = r[k + 2 * ni] - r[m + 2 * ni];
+          const double aux1 = std::sqrt(dx * dx + dy * dy + dz * dz);
+          eij += radial_lerp(c->pair_v_host + (size_t)pair_index(k, m, ni) * c->npair, c->npair, c->delpair, aux1,
+                             nullptr);
+        }
+      e->eimpu_impu = eij;
+    }
     if (c->quantum) {
       double sx[2];
       if (c->mixed) HEDFT_TRY(energy_sums_x<float>(c, sx));
@@ -2138,7 +2181,7 @@ int hedft_energy(hedft_ctx* c, hedft_energies* e) {
       e->ekinx = c->h2o2mimp * sx[0] * dv;
       e->eHeX = sx[1] * dv;
     }
-    e->eimpu = e->ekinx + e->eHeX;
+    e->eimpu = e->ekinx + e->eHeX + e->eimpu_impu;
   } else {
     e->eext = p.lexternal ? s[5] * dv : 0.0;
     e->eimpu = e->eext;
@@ -2193,6 +2236,7 @@ int hedft_steppcr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, doub
 
 int hedft_steprk(hedft_ctx* c, double deltat) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  HEDFT_TRY(imp_ready(c));
   HEDFT_TRY(steprk_impl(c, deltat));
   return done(c);
 }
@@ -2209,6 +2253,7 @@ static int read_pc_errors(hedft_ctx* c, double* errHe, double* errimp, double* e
 
 int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, double* errvimp) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  HEDFT_TRY(imp_ready(c));
   if (!c->rk_done) return HEDFT_ESTATE;
   HEDFT_TRY(steppc_impl(c, deltat));
   flush_side_pre(c);  // STEPPC ends with the impurity corrector and the errHe sum
@@ -2221,20 +2266,49 @@ int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delint
   if (!c || nimp < 0 || nimp > HEDFT_MAX_IMP || (nimp > 0 && (!potion || !mass || npot < 2))) return HEDFT_EINVAL;
   if (nimp > 0 && c->quantum) return HEDFT_EINVAL;  // one impurity model per context
   HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old table
+  // a captured graph holds the old impurity count and table pointer
+  c->graph_valid = 0;
   if (c->potion) hipFree(c->potion);
   c->potion = nullptr;
-  c->nimp = nimp;
+  clear_pairs(c);  // pair tables belong to one impurity set
+  c->nimp = 0;
   c->npot = npot;
   c->delinter = delinter;
   c->rmaxinterpol = rmaxinterpol;
   if (nimp == 0) return HEDFT_OK;
-  HEDFT_CHECK(hipMalloc(&c->potion, sizeof(double) * (size_t)nimp * npot));
+  if (hipMalloc(&c->potion, sizeof(double) * (size_t)nimp * npot) != hipSuccess) {
+    c->potion = nullptr;
+    return HEDFT_ENOMEM;
+  }
   HEDFT_CHECK(hipMemcpy(c->potion, potion, sizeof(double) * (size_t)nimp * npot, hipMemcpyHostToDevice));
+  c->nimp = nimp;
   for (int k = 0; k < nimp; ++k) c->mass[k] = mass[k];
   double m[HEDFT_MAX_IMP] = {0};
   for (int k = 0; k < nimp; ++k) m[k] = mass[k];
   HEDFT_CHECK(hipMemcpy(c->imp + IMP_MASS, m, sizeof m, hipMemcpyHostToDevice));
   c->graph_valid = 0;
+  return HEDFT_OK;
+}
+
+int hedft_set_impurity_pairs(hedft_ctx* c, int32_t npair, double delpair, const double* vpair,
+                             const double* dvpair) {
+  if (!c || c->nimp < 2 || npair < 2 || !(delpair > 0.0) || !vpair || !dvpair) return HEDFT_EINVAL;
+  hipSetDevice(c->device);
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old tables
+  c->graph_valid = 0;
+  clear_pairs(c);
+  const size_t cnt = (size_t)npairs_of(c->nimp) * npair;
+  c->pair_v_host = (double*)std::malloc(sizeof(double) * cnt);
+  if (!c->pair_v_host) return HEDFT_ENOMEM;
+  if (hipMalloc(&c->pairtab, sizeof(double) * cnt) != hipSuccess) {
+    c->pairtab = nullptr;
+    clear_pairs(c);
+    return HEDFT_ENOMEM;
+  }
+  std::memcpy(c->pair_v_host, vpair, sizeof(double) * cnt);
+  HEDFT_CHECK(hipMemcpy(c->pairtab, dvpair, sizeof(double) * cnt, hipMemcpyHostToDevice));
+  c->npair = npair;
+  c->delpair = delpair;
   return HEDFT_OK;
 }
 
@@ -2312,6 +2386,7 @@ __global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const R* gx
 int hedft_forceimp(hedft_ctx* c) {
   if (!c) return HEDFT_EINVAL;
   if (c->nimp == 0) return HEDFT_OK;
+  HEDFT_TRY(imp_ready(c));
   if (c->mixed)
     hipLaunchKernelGGL(k_force_ew<float>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
                        fld<float>(c->dxden), fld<float>(c->dyden), fld<float>(c->dzden), force_partials(c));
@@ -2442,6 +2517,7 @@ int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double 
 
 int hedft_tddft_init(hedft_ctx* c) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  HEDFT_TRY(imp_ready(c));
   c->lap_fresh = 0;
   if (c->mixed)
     hipLaunchKernelGGL(k_tddft_psi0<float>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, fld<float>(c->den),
@@ -2463,6 +2539,7 @@ int hedft_tddft_init(hedft_ctx* c) {
 int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk, double* errHe, double* errimp,
                           double* errvimp) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
+  HEDFT_TRY(imp_ready(c));
   if (iter <= nrk) {
     HEDFT_TRY(steprk_impl(c, deltat));
     if (errHe) *errHe = 0.0;
@@ -2570,6 +2647,7 @@ extern "C" {
 
 int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph, double* elapsed_ms) {
   if (!c || c->mode != HEDFT_MODE_TDDFT || nsteps < 0) return HEDFT_EINVAL;
+  HEDFT_TRY(imp_ready(c));
   if (!c->rk_done) return HEDFT_ESTATE;
   hipSetDevice(c->device);
   if (use_graph && nsteps >= PC_PERIOD) HEDFT_TRY(ensure_graph(c, deltat));  // capture outside the timing
@@ -2592,6 +2670,7 @@ int hedft_batch_run_pc(hedft_ctx* const* ctxs, int32_t nctx, const double* delta
     const hedft_ctx* c = ctxs[i];
     if (!c || c->mode != HEDFT_MODE_TDDFT || c->device != dev) return HEDFT_EINVAL;
     if (!c->rk_done) return HEDFT_ESTATE;
+    HEDFT_TRY(imp_ready(c));
     for (int j = 0; j < i; ++j)
       if (ctxs[j] == c) return HEDFT_EINVAL;  // a context appears once
   }

@@ -140,6 +140,7 @@ def lib():
         L.hedft_steppc.argtypes = [vp, d, dp, dp, dp]
         L.hedft_set_impurities.argtypes = [vp, ip, ip, d, d, vp, vp]
         L.hedft_set_impurity_state.argtypes = [vp, vp, vp, vp]
+        L.hedft_set_impurity_pairs.argtypes = [vp, ip, d, vp, vp]
         L.hedft_get_impurity_state.argtypes = [vp, vp, vp, vp, vp, C.POINTER(ip)]
         L.hedft_statics_init.argtypes = [vp, d, dp]
         L.hedft_statics_iteration.argtypes = [vp, d, ip, d, dp, dp]
@@ -298,6 +299,18 @@ class Droplet:
         self.nimp = potion.shape[0]
         check(lib().hedft_set_impurities(self._c, self.nimp, potion.shape[1], delinter, rmax,
                                          _ptr(potion), _ptr(mass)), "set_impurities")
+
+    def set_impurity_pairs(self, vpair, dvpair, delpair):
+        """Impurity-impurity tables (npairs, npair) on r_i = i*delpair: the pair
+        energy (selec_gs_k_k) and the pair force function (drselec_gs_k_k),
+        pairs ordered (1,2),(1,3),..,(2,3),.. (potenimp.f90:40-58,
+        energy.f90:168-177)."""
+        v = np.ascontiguousarray(np.atleast_2d(vpair), dtype=np.float64)
+        dv = np.ascontiguousarray(np.atleast_2d(dvpair), dtype=np.float64)
+        if v.shape != dv.shape or v.shape[0] != self.nimp * (self.nimp - 1) // 2:
+            raise ValueError("pair tables: one row per impurity pair k < m, same shape")
+        check(lib().hedft_set_impurity_pairs(self._c, v.shape[1], delpair, _ptr(v), _ptr(dv)),
+              "set_impurity_pairs")
 
     def set_impurity_state(self, rimp=None, vimp=None, aimp=None):
         keep = []

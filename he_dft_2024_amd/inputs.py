@@ -51,6 +51,32 @@ def select_pot_lj(r, eps=20.0, sigma=4.0, r_cutoff=3.0, umax=None):
     return np.where(r < r_cutoff, umax, v)
 
 
+def select_pot_lj_dr(r, eps=20.0, sigma=4.0, r_cutoff=3.0, drumax=0.0):
+    """dV/dr of the LJ entry, the kind of function the reference's drselec_gs_k_k
+    names for the pair force (potenimp.f90:44): 24 eps/r ((s/r)^6 - 2 (s/r)^12)
+    outside r_cutoff, drumax inside."""
+    r = np.asarray(r, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rr = np.where(r > 0, r, 1.0)
+        sr6 = (sigma / rr) ** 6
+        dv = 24.0 * eps / rr * (sr6 - 2.0 * sr6 * sr6)
+    return np.where(r < r_cutoff, drumax, dv)
+
+
+def pair_tables(nimp: int, rmax: float, npair: int = 200001, pairs=None):
+    """Impurity-impurity tables on r_i = i*delpair over [0, rmax], one row per pair
+    k < m in the order (1,2),(1,3),..,(2,3),..: (vpair, dvpair, delpair).
+    pairs: list of LJ keyword dicts (eps, sigma, r_cutoff) per pair, default
+    one LJ (eps 20 K, sigma 4 A, r_cutoff 3 A) for every pair."""
+    npairs = nimp * (nimp - 1) // 2
+    pairs = pairs or [{}] * npairs
+    delpair = rmax / (npair - 1)
+    r = delpair * np.arange(npair)
+    v = np.stack([select_pot_lj(r, **kw) for kw in pairs])
+    dv = np.stack([select_pot_lj_dr(r, **{k: x for k, x in kw.items() if k != "umax"}) for kw in pairs])
+    return np.ascontiguousarray(v), np.ascontiguousarray(dv), delpair
+
+
 def impurity_table(n_max: int, box_max: float, **lj):
     """potenimpini (potenimpini.f90:154-170): npot = 100 max(n) + 1 points on
     [0, 4 max(xmax)], DelInter = rmax/(npot-1).  Returns (potion, delinter, rmax)."""

@@ -214,6 +214,20 @@ int hedft_steppc(hedft_ctx* c, double deltat, double* errHe, double* errimp, dou
  * (m_imp = m_imp_u*mp_u, DFT4He3d.f90:298). */
 int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delinter,
                          double rmaxinterpol, const double* potion, const double* mass);
+/* Impurity-impurity terms of forceimp and energy (potenimp.f90:40-58,
+ * energy.f90:168-177) for nimp >= 2, replacing the reference's
+ * Select_pot(drselec_gs_k_k(k,m), r, ...) (force) and
+ * Select_pot(selec_gs_k_k(k,m), r, ...) (energy) by radial tables on
+ * r_i = i*delpair, i = 0..npair-1, interpolated as updatepoten interpolates
+ * potion (potenimp.f90:101-112).  One table of npair values per pair k < m, in
+ * the order (1,2),(1,3),..,(1,N),(2,3),..: vpair feeds eimpu_impu, dvpair the
+ * pair force F_ij(k,m,:) = -dv(|R_k-R_m|) (R_k-R_m)/|R_k-R_m|, F_ij(m,k) =
+ * -F_ij(k,m), added to F(k,:).  A distance beyond the last node is clamped and
+ * sets lstopimp.  With nimp >= 2 the integrators, forceimp and the drivers
+ * return HEDFT_ESTATE until the tables are set; hedft_set_impurities clears
+ * them. */
+int hedft_set_impurity_pairs(hedft_ctx* c, int32_t npair, double delpair, const double* vpair,
+                             const double* dvpair);
 /* rimp, vimp, aimp as (nimp,3) Fortran arrays (k fastest); NULL skips one. */
 int hedft_set_impurity_state(hedft_ctx* c, const double* rimp, const double* vimp,
                              const double* aimp);

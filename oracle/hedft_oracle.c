@@ -104,6 +104,10 @@ typedef struct {
   double rimpold[3][MAXIMP * 3], vimpold[3][MAXIMP * 3], aimpold[2][MAXIMP * 3];
   int ioldr[3], ioldv[3], iolda[2];
   int lstopimp;
+  /* impurity-impurity pair tables (selec_gs_k_k, drselec_gs_k_k as tables) */
+  int npair;
+  double delpair;
+  double *pairv, *pairdv;
   /* one quantum impurity (module quantal_imp, N_imp = 1) */
   int quantum;
   double h2o2mimp, mu4;
@@ -559,7 +563,8 @@ void oracle_destroy(octx* c) {
                     (double*)c->fden, (double*)c->wk1, (double*)c->wk2, (double*)c->wk3,
                     (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion,
                     c->psix, c->psixold[0], c->psixold[1], c->psixold[2], c->hpsixold[0],
-                    c->hpsixold[1], c->pcx, c->qx, c->hpsix, c->potx4, c->upotx, c->denx, c->uimpk};
+                    c->hpsixold[1], c->pcx, c->qx, c->hpsix, c->potx4, c->upotx, c->denx, c->uimpk,
+                    c->pairv, c->pairdv};
   for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); ++i) free(list[i]);
   free(c);
 }
@@ -901,6 +906,9 @@ void oracle_set_impurities(octx* c, int nimp, int npot, double delinter, double 
   c->potion = dalloc((size_t)nimp * npot);
   memcpy(c->potion, potion, sizeof(double) * (size_t)nimp * npot);
   for (int k = 0; k < nimp; ++k) c->mimp[k] = mass[k];
+  free(c->pairv); free(c->pairdv);
+  c->pairv = c->pairdv = NULL;
+  c->npair = 0;
 }
 
 /* rimp etc. are (nimp,3) Fortran arrays: element (k,a) at k + nimp*a */
@@ -959,9 +967,51 @@ void oracle_potenimp(octx* c) {
       }
 }
 
-/* forceimp (potenimp.f90:23-67) without impurity-impurity pair terms */
+/* Pair tables on r_i = i*delpair, one per pair k < m in the order (1,2),(1,3),..,
+ * read like updatepoten reads potion (potenimp.f90:101-112): ir = int(r/del),
+ * rmod = mod(r,del)/del; r beyond the last node clamped to it and flagged. */
+void oracle_set_impurity_pairs(octx* c, int npair, double delpair, const double* v, const double* dv) {
+  int np = c->nimp * (c->nimp - 1) / 2;
+  free(c->pairv); free(c->pairdv);
+  c->pairv = dalloc((size_t)np * npair);
+  c->pairdv = dalloc((size_t)np * npair);
+  memcpy(c->pairv, v, sizeof(double) * (size_t)np * npair);
+  memcpy(c->pairdv, dv, sizeof(double) * (size_t)np * npair);
+  c->npair = npair; c->delpair = delpair;
+}
+
+static double pair_at(octx* c, const double* tabs, int k, int m, double r) {
+  int n = c->nimp, idx = k * n - k * (k + 1) / 2 + (m - k - 1);
+  const double* t = tabs + (size_t)idx * c->npair;
+  double rmax = c->delpair * (double)(c->npair - 1);
+  if (r > rmax) { c->lstopimp = 1; r = rmax; }
+  int ir = (int)(r / c->delpair);
+  if (ir > c->npair - 2) ir = c->npair - 2;
+  double rmod = fmod(r, c->delpair) / c->delpair;
+  return t[ir] * (1.0 - rmod) + t[ir + 1] * rmod;
+}
+
+/* |R_k - R_m| as potenimp.f90:43 and energy.f90:172-173 form it */
+static double pair_dist(const octx* c, int k, int m, double d[3]) {
+  int n = c->nimp;
+  for (int a = 0; a < 3; ++a) d[a] = c->rimp[k + n * a] - c->rimp[m + n * a];
+  return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+/* forceimp (potenimp.f90:23-67): F(k,:) = -sum(grad(den) uimp_k) dxyz
+ * + sum_m F_ij(k,m,:), F_ij(k,m,:) = -D_km(r) (R_k-R_m)/r (k < m),
+ * F_ij(m,k,:) = -F_ij(k,m,:) */
 void oracle_forceimp(octx* c) {
   int nx = c->nx, ny = c->ny, nz = c->nz;
+  double fij[MAXIMP][MAXIMP][3];
+  memset(fij, 0, sizeof fij);
+  if (c->nimp > 1 && c->pairdv)
+    for (int k = 0; k < c->nimp; ++k)
+      for (int m = k + 1; m < c->nimp; ++m) {
+        double d[3], r = pair_dist(c, k, m, d);
+        double v = pair_at(c, c->pairdv, k, m, r);
+        for (int a = 0; a < 3; ++a) { fij[k][m][a] = -v * d[a] / r; fij[m][k][a] = -fij[k][m][a]; }
+      }
   for (int k = 0; k < c->nimp; ++k) {
     double* acc = dalloc(3 * (size_t)nz);
 #pragma omp parallel for schedule(static)
@@ -978,7 +1028,11 @@ void oracle_forceimp(octx* c) {
     double f[3] = {0, 0, 0};
     for (int iz = 0; iz < nz; ++iz) for (int a = 0; a < 3; ++a) f[a] += acc[3 * iz + a];
     free(acc);
-    for (int a = 0; a < 3; ++a) c->force[k + c->nimp * a] = -f[a] * c->dxyz;
+    for (int a = 0; a < 3; ++a) {
+      double fk = -f[a] * c->dxyz;
+      for (int m = 0; m < c->nimp; ++m) fk += fij[k][m][a];
+      c->force[k + c->nimp * a] = fk;
+    }
   }
 }
 
@@ -1360,7 +1414,7 @@ void oracle_energy(octx* c, double* out) {
   double ealphas = p->core_otc ? -p->h2o2m4 * 0.5 * p->alphas * s[3] * dv : 0.0;
   double esolid = p->lsolid ? p->csolid * s[4] * dv : 0.0;
   double etot4 = ekin4 + elj4 + ecor4 + esolid + ealphas;
-  double eext = 0, ekinx = 0, eHeX = 0, eimpu = 0, etot;
+  double eext = 0, ekinx = 0, eHeX = 0, eimpu = 0, eimpu_impu = 0, etot;
   if (tddft) {
     eext = s[5] * dv;
     etot4 += eext;
@@ -1400,7 +1454,14 @@ void oracle_energy(octx* c, double* out) {
       ekinx = c->h2o2mimp * sk * dv;
       eHeX = sh * dv;
     }
-    eimpu = ekinx + eHeX;
+    /* energy.f90:168-177 */
+    if (c->nimp > 1 && c->pairv)
+      for (int k = 0; k < c->nimp; ++k)
+        for (int m = k + 1; m < c->nimp; ++m) {
+          double d[3], r = pair_dist(c, k, m, d);
+          eimpu_impu += pair_at(c, c->pairv, k, m, r);
+        }
+    eimpu = ekinx + eHeX + eimpu_impu;
     etot = etot4 + eimpu;
   } else {
     eext = p->lexternal ? s[5] * dv : 0.0;
@@ -1408,7 +1469,7 @@ void oracle_energy(octx* c, double* out) {
     etot = etot4 + eimpu;
   }
   out[0] = ekin4; out[1] = elj4; out[2] = ecor4; out[3] = ealphas; out[4] = esolid; out[5] = eext;
-  out[6] = etot4; out[7] = ekinx; out[8] = eHeX; out[9] = 0.0; out[10] = eimpu; out[11] = etot;
+  out[6] = etot4; out[7] = ekinx; out[8] = eHeX; out[9] = eimpu_impu; out[10] = eimpu; out[11] = etot;
   out[12] = s[7] * dv;
 }
 
@@ -1576,9 +1637,11 @@ void oracle_tddft_init(octx* c) {
   for (int iz = 0; iz < c->nz; ++iz)
     for (int iy = 0; iy < c->ny; ++iy)
       for (int ix = 0; ix < c->nx; ++ix) {
-        double lam = p->lambdah * (1.0 + tanh((fabs(c->x[ix]) - p->tmean[0]) / p->tsurf[0]) + 1.0 +
-                                   tanh((fabs(c->y[iy]) - p->tmean[1]) / p->tsurf[1]) + 1.0 +
-                                   tanh((fabs(c->z[iz]) - p->tmean[2]) / p->tsurf[2]));
+        /* cmplx(Lambda, 1.d0) without a KIND is a default (REAL(4)) complex:
+         * the reference stores Lambda rounded to single precision */
+        double lam = (double)(float)(p->lambdah * (1.0 + tanh((fabs(c->x[ix]) - p->tmean[0]) / p->tsurf[0]) + 1.0 +
+                                                   tanh((fabs(c->y[iy]) - p->tmean[1]) / p->tsurf[1]) + 1.0 +
+                                                   tanh((fabs(c->z[iz]) - p->tmean[2]) / p->tsurf[2])));
         tc[ix + (size_t)c->nx * (iy + (size_t)c->ny * iz)] = lam + I * 1.0;
       }
   oracle_potenimp(c);

@@ -111,6 +111,7 @@ def lib():
         L.oracle_steppc.argtypes = [vp, C.c_double, dp, dp, dp]
         L.oracle_set_impurities.argtypes = [vp, ip, ip, C.c_double, C.c_double, vp, vp]
         L.oracle_set_impurity_state.argtypes = [vp, vp, vp, vp]
+        L.oracle_set_impurity_pairs.argtypes = [vp, ip, C.c_double, vp, vp]
         L.oracle_get_impurity_state.argtypes = [vp, vp, vp, vp, vp, C.POINTER(C.c_int)]
         L.oracle_energy.argtypes = [vp, vp]
         L.oracle_statics_init.argtypes = [vp, C.c_double, dp]
@@ -358,6 +359,14 @@ class Oracle:
         self.nimp = potion.shape[0]
         lib().oracle_set_impurities(self._c, self.nimp, potion.shape[1], delinter, rmax,
                                     _ptr(potion), _ptr(mass))
+
+    def set_impurity_pairs(self, vpair, dvpair, delpair: float):
+        """Pair tables (npairs, npair): energy (selec_gs_k_k) and force
+        (drselec_gs_k_k) functions of |R_k - R_m| on r_i = i*delpair."""
+        v = np.ascontiguousarray(np.atleast_2d(vpair), dtype=np.float64)
+        dv = np.ascontiguousarray(np.atleast_2d(dvpair), dtype=np.float64)
+        assert v.shape == dv.shape and v.shape[0] == self.nimp * (self.nimp - 1) // 2
+        lib().oracle_set_impurity_pairs(self._c, v.shape[1], delpair, _ptr(v), _ptr(dv))
 
     def set_impurity_state(self, rimp=None, vimp=None, aimp=None):
         def conv(a):

@@ -19,7 +19,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32), (96, 48, 64), (128, 128, 32), (192, 192, 32),
-                                   (64, 32, 256)])
+                                   (64, 32, 256), (64, 64, 128), (128, 128, 128)])
 def test_fft_roundtrip_vs_numpy(gpu, shape):
     nx, ny, nz = shape
     _say("fft", shape)
@@ -145,11 +145,20 @@ def test_statics_trajectory_vs_oracle(gpu):
     d.close()
 
 
-@pytest.mark.parametrize("otc", [1, 0])
-def test_tddft_trajectory_vs_oracle(gpu, otc):
+# the absorbing layer of the reference's TDDFT example (Lambdah = 2, t*mean =
+# box half-width - 2 A, t*surf = 2 A; Dynamics_TDDFT/input_files/
+# xe-he1000-hoc-200mps.input:2-4), here on all three axes
+def _absorber(xmax, ymax=None, zmax=None):
+    ymax = xmax if ymax is None else ymax
+    zmax = xmax if zmax is None else zmax
+    return {"lambdah": 2.0, "tmean": (xmax - 2.0, ymax - 2.0, zmax - 2.0), "tsurf": (2.0, 2.0, 2.0)}
+
+
+@pytest.mark.parametrize("otc,absorb", [(1, False), (0, False), (1, True), (0, True)])
+def test_tddft_trajectory_vs_oracle(gpu, otc, absorb):
     n, xmax, n4 = 32, 6.4, 60.0
-    _say("tddft 3 RK + 9 PC otc", otc)
-    d, o = _pair(n, xmax, gpu.MODE_TDDFT, core_otc=otc)
+    _say("tddft 3 RK + 9 PC otc", otc, "absorber", absorb)
+    d, o = _pair(n, xmax, gpu.MODE_TDDFT, core_otc=otc, **(_absorber(xmax) if absorb else {}))
     dt = 5e-4 / I.PSTOK
     errs = []
     for x in (d, o):
@@ -161,8 +170,15 @@ def test_tddft_trajectory_vs_oracle(gpu, otc):
         errs.append(np.array(e))
     assert relerr(errs[0][3:, 0], errs[1][3:, 0]) < RTOL_TRAJ
     for name in ("PSI", "DEN", "POT4", "PC", "PSIOLD1", "PSIOLD2", "PSIOLD3", "HPSIOLD1", "HPSIOLD2",
-                 "UIMP", "TIMEC"):
+                 "UIMP"):
         assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    tc = d.field("TIMEC")
+    assert np.array_equal(tc, o.field("TIMEC"))
+    if absorb:   # the layer is active and removes helium at the walls
+        assert tc.real.max() > 1.0
+        assert d.energy()["norm"] < n4 * (1 - 1e-9)
+    else:
+        assert tc.real.max() == 0.0
     # RK-Gill's q returns to zero after every full 4-stage step in exact arithmetic
     # (stage 4: a=1/6, b=2, c=1/2), so what remains is round-off on both sides
     kscale = np.abs(o.field("HPSIOLD1")).max()
@@ -182,9 +198,11 @@ def test_tddft_trajectory_vs_oracle(gpu, otc):
 SLAB = (128, 128, 32, 25.6, 25.6, 6.4)
 
 
-def _slab_pair(mode, **kw):
+def _slab_pair(mode, absorb=False, **kw):
     import he_dft_2024_amd as H
     nx, ny, nz, xm, ym, zm = SLAB
+    if absorb:
+        kw.update(_absorber(xm, ym, zm))
     d = H.Droplet(nx, ny, nz, xm, ym, zm, H.default_params(**kw), mode=mode)
     o = O.Oracle(nx, ny, nz, xm, ym, zm, O.default_params(**kw), mode=mode)
     return d, o
@@ -210,9 +228,10 @@ def test_poten_tddft_128_planes_vs_oracle(gpu, otc):
     d.close()
 
 
-def test_tddft_trajectory_128_planes_vs_oracle(gpu):
-    _say("tddft 128x128x32 3 RK + 3 PC otc")
-    d, o = _slab_pair(gpu.MODE_TDDFT, core_otc=1)
+@pytest.mark.parametrize("absorb", [False, True])
+def test_tddft_trajectory_128_planes_vs_oracle(gpu, absorb):
+    _say("tddft 128x128x32 3 RK + 3 PC otc absorber", absorb)
+    d, o = _slab_pair(gpu.MODE_TDDFT, absorb=absorb, core_otc=1)
     dt = 5e-4 / I.PSTOK
     errs = []
     for x in (d, o):
@@ -354,29 +373,43 @@ def test_xy_stencil_paths_all_icons_vs_oracle(gpu, icon):
     d.close()
 
 
-def test_tddft_two_impurities_vs_oracle(gpu):
-    """Two classical impurities with different tables and masses (the force
-    pass's run-time impurity loop), and the standalone forceimp."""
+def test_tddft_three_impurities_with_pair_terms_vs_oracle(gpu):
+    """Three classical impurities with different tables, masses and pair
+    potentials: the force pass's run-time impurity loop, the pair force F_ij
+    and the pair energy (potenimp.f90:40-58, energy.f90:168-177), eager steps
+    and graph replay, and the standalone forceimp."""
     n, xmax, n4 = 32, 6.4, 60.0
-    _say("tddft two impurities")
+    _say("tddft three impurities with pair terms")
     d, o = _pair(n, xmax, gpu.MODE_TDDFT, core_otc=1)
     dt = 5e-4 / I.PSTOK
     potion, delinter, rmax = I.impurity_table(n, xmax)
     potion2, _, _ = I.impurity_table(n, xmax, eps=12.0, sigma=3.5)
+    potion3, _, _ = I.impurity_table(n, xmax, eps=30.0, sigma=4.4)
+    v, dv, dl = I.pair_tables(3, 4 * xmax, npair=20001,
+                              pairs=[{}, {"eps": 12.0, "sigma": 3.5}, {"eps": 30.0, "sigma": 4.4}])
     den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=8)
-    r0 = np.array([[0.0, 0.5, -4.0], [1.0, -0.5, 4.5]])
-    v0 = np.array([[0.0, 0.0, 3.0], [0.5, 0.0, -2.0]]) * I.PSTOK
+    r0 = np.array([[0.0, 0.5, -4.0], [1.0, -0.5, 4.5], [-3.0, 2.0, 0.5]])
+    v0 = np.array([[0.0, 0.0, 3.0], [0.5, 0.0, -2.0], [1.0, -1.0, 0.0]]) * I.PSTOK
     for x in (d, o):
         x.set("DEN", den)
-        x.set_impurities(np.stack([potion, potion2]), delinter, rmax, [108.0 * I.MP_U, 40.0 * I.MP_U])
-        x.set_impurity_state(r0, v0, np.zeros((2, 3)))
+        x.set_impurities(np.stack([potion, potion2, potion3]), delinter, rmax,
+                         [108.0 * I.MP_U, 40.0 * I.MP_U, 20.0 * I.MP_U])
+        x.set_impurity_pairs(v, dv, dl)
+        x.set_impurity_state(r0, v0, np.zeros((3, 3)))
         x.tddft_init()
-        for it in range(1, 9):
+        for it in range(1, 10):
             x.tddft_iteration(dt, it, 3)
+    d.run_pc(dt, 6, True)
+    for it in range(10, 16):
+        o.tddft_iteration(dt, it, 3)
     assert relerr(d.field("PSI"), o.field("PSI")) < RTOL_TRAJ
     sg, so = d.impurity_state(), o.impurity_state()
     for k in ("rimp", "vimp", "aimp", "force"):
         assert relerr(sg[k], so[k]) < RTOL_TRAJ, k
+    eg, eo = d.energy(), o.energy()
+    assert abs(eo["eimpu_impu"]) > 1e-3
+    for k in ("eimpu_impu", "ekinx", "eHeX", "eimpu", "etot"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
     for x in (d, o):
         x.forceimp()
     assert relerr(d.impurity_state()["force"], o.impurity_state()["force"]) < RTOL_OP
