@@ -156,6 +156,16 @@ __host__ __device__ inline double radial_lerp(const double* t, int np, double de
 // index of pair (k,m), k < m, among the nimp*(nimp-1)/2 pairs
 __host__ __device__ inline int pair_index(int k, int m, int nimp) { return k * nimp - k * (k + 1) / 2 + (m - k - 1); }
 
+// ---- impurity block layout on the device (doubles) ---------------------------------
+enum {
+  IMP_R = 0, IMP_V = 12, IMP_A = 24, IMP_F = 36, IMP_QR = 48, IMP_QV = 60, IMP_PCR = 72,
+  IMP_PCV = 84, IMP_ROLD = 96, IMP_VOLD = 132, IMP_AOLD = 168, IMP_MASS = 192, IMP_SIZE = 200
+};
+
+// blocks of the grid-stride elementwise kernels (a fixed grid keeps their
+// per-block partial sums, and so the reductions, deterministic)
+static constexpr int EW_BLOCKS = 1024;
+
 struct PairDev {
   const double* dv;  // force tables (drselec), npairs x np
   int np;
@@ -396,6 +406,15 @@ int inv_yx(hedft_ctx* c, int nb, cplx_t<R>* const* spec, const XOutT<R>* out);
 // whole periods); device bytes of one context
 int enqueue_pc_async(hedft_ctx* c, double deltat, int nsteps);
 size_t ctx_footprint(int n, int mode, int precision);
+
+// impurity.hip: force reduction F = -sum(partials) dxyz + pair forces [, aimp = F/m]
+ImpDev make_impdev(hedft_ctx* c);
+double* force_partials(hedft_ctx* c);
+int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr);
+int npairs_of(int nimp);
+void clear_pairs(hedft_ctx* c);
+// several classical impurities need their pair tables: HEDFT_ESTATE until given
+int imp_ready(const hedft_ctx* c);
 
 // stencil.hip
 void stencil_setup(hedft_ctx* c);

@@ -35,37 +35,13 @@ enum {
   S_OBS = 16,      // 13 observable sums (used alone)
   NSCAL = 32
 };
-// ---- impurity block layout (doubles) -------------------------------------------
-enum {
-  IMP_R = 0, IMP_V = 12, IMP_A = 24, IMP_F = 36, IMP_QR = 48, IMP_QV = 60, IMP_PCR = 72,
-  IMP_PCV = 84, IMP_ROLD = 96, IMP_VOLD = 132, IMP_AOLD = 168, IMP_MASS = 192, IMP_SIZE = 200
-};
 
 static const double RK_A[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 1.0 / 6.0};
 static const double RK_B[4] = {2.0, 1.0, 1.0, 2.0};
 static const double RK_C[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 0.5};
 static const double C112 = 112.0 / 121.0, C9 = 9.0 / 121.0, C4O3 = 4.0 / 3.0;
 
-ImpDev make_impdev(hedft_ctx* c) {
-  ImpDev im;
-  im.nimp = c->nimp;
-  im.npot = c->npot;
-  im.delinter = c->delinter;
-  im.rmax = c->rmaxinterpol;
-  im.potion = c->potion;
-  im.rimp = c->imp + IMP_R;
-  im.lstop = c->lstop;
-  im.x0 = -c->g.xmax;
-  im.y0 = -c->g.ymax;
-  im.z0 = -c->g.zmax;
-  im.hx = c->hx;
-  im.hy = c->hy;
-  im.hz = c->hz;
-  return im;
-}
-
 static size_t partials_region(hedft_ctx* c) { return c->partials_cap / 8; }
-double* force_partials(hedft_ctx* c) { return c->partials + partials_region(c); }
 static double* gen_partials(hedft_ctx* c) { return c->partials; }
 static double* x_partials(hedft_ctx* c) { return c->partials + 2 * partials_region(c); }  // psix corrector
 
@@ -97,76 +73,6 @@ template <int NR>
 static void finalize(hedft_ctx* c, const double* part, size_t nblocks, int slot, double scale) {
   hipLaunchKernelGGL((k_finalize<NR>), dim3(1), dim3(256), 0, c->stream, part, (int)nblocks,
                      c->scal + slot, scale);
-}
-
-// F_k,a = -sum_blocks(partials[3k+a]) * dxyz; optionally aimp = F/m (DFT4He3d.f90:692-696)
-// (per-block partials hold NR = 3 values per accumulated impurity).  Each
-// thread first sums its blocks into NR independent registers (loads in flight
-// together), then one tree over the workgroup.
-template <int NR>
-__global__ void __launch_bounds__(256) k_force_finalize(const double* __restrict__ part, int nblocks, double* imp,
-                                                        int nimp, double dxyz, int update_aimp, PairDev pr) {
-  __shared__ double sh[NR][256];
-  double s[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) s[r] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += 256) {
-#pragma unroll
-    for (int r = 0; r < NR; ++r) s[r] += part[(size_t)b * NR + r];
-  }
-#pragma unroll
-  for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = s[r];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-#pragma unroll
-      for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    // F(k,:) = -sum(grad(den) uimp_k) dxyz + sum_m F_ij(k,m,:)  (potenimp.f90:49-58)
-    double fij[HEDFT_MAX_IMP * HEDFT_MAX_IMP * 3];
-    if (nimp > 1 && pr.dv) pair_forces(imp + IMP_R, nimp, pr.dv, pr.np, pr.del, pr.lstop, fij);
-    for (int k = 0; k < nimp && 3 * k < NR; ++k)
-      for (int a = 0; a < 3; ++a) {
-        double f = -sh[3 * k + a][0] * dxyz;
-        if (nimp > 1 && pr.dv)
-          for (int m = 0; m < nimp; ++m) f += fij[(k * HEDFT_MAX_IMP + m) * 3 + a];
-        imp[IMP_F + k + nimp * a] = f;
-        if (update_aimp) imp[IMP_A + k + nimp * a] = f / imp[IMP_MASS + k];
-      }
-  }
-}
-
-static int npairs_of(int nimp) { return nimp * (nimp - 1) / 2; }
-static void clear_pairs(hedft_ctx* c) {
-  if (c->pairtab) hipFree(c->pairtab);
-  std::free(c->pair_v_host);
-  c->pairtab = nullptr;
-  c->pair_v_host = nullptr;
-  c->npair = 0;
-}
-// several classical impurities need their pair tables (the reference always
-// adds the pair force and energy); HEDFT_ESTATE until they are given
-static int imp_ready(const hedft_ctx* c) { return (c->nimp >= 2 && !c->pairtab) ? HEDFT_ESTATE : HEDFT_OK; }
-
-static PairDev make_pairdev(hedft_ctx* c) {
-  PairDev p;
-  p.dv = c->pairtab;
-  p.np = c->npair;
-  p.del = c->delpair;
-  p.lstop = c->lstop;
-  return p;
-}
-
-int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr) {
-  if (nr == 3)
-    hipLaunchKernelGGL(k_force_finalize<3>, dim3(1), dim3(256), 0, c->stream, force_partials(c), (int)nblocks, c->imp,
-                       c->nimp, c->dxyz, update_aimp, make_pairdev(c));
-  else
-    hipLaunchKernelGGL(k_force_finalize<3 * HEDFT_MAX_IMP>, dim3(1), dim3(256), 0, c->stream, force_partials(c),
-                       (int)nblocks, c->imp, c->nimp, c->dxyz, update_aimp, make_pairdev(c));
-  return HEDFT_OK;
 }
 
 // ---- absorbing layer: timec = Lambdah*(1+tanh(..x)+1+tanh(..y)+1+tanh(..z)) + i
@@ -383,7 +289,6 @@ struct OutEnergy {
 };
 
 // ---- statics elementwise kernels (grid-stride, fixed grid for deterministic sums)
-static constexpr int EW_BLOCKS = 1024;
 
 // one RK-Gill stage of STEPRKR after the mean (steprkr.f90:54-73), sum(psi^2)
 __global__ void __launch_bounds__(256) k_rkr_stage(long long n, double* psi, const double* hpsi, double* q,
@@ -2290,28 +2195,6 @@ int hedft_set_impurities(hedft_ctx* c, int32_t nimp, int32_t npot, double delint
   return HEDFT_OK;
 }
 
-int hedft_set_impurity_pairs(hedft_ctx* c, int32_t npair, double delpair, const double* vpair,
-                             const double* dvpair) {
-  if (!c || c->nimp < 2 || npair < 2 || !(delpair > 0.0) || !vpair || !dvpair) return HEDFT_EINVAL;
-  hipSetDevice(c->device);
-  HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old tables
-  c->graph_valid = 0;
-  clear_pairs(c);
-  const size_t cnt = (size_t)npairs_of(c->nimp) * npair;
-  c->pair_v_host = (double*)std::malloc(sizeof(double) * cnt);
-  if (!c->pair_v_host) return HEDFT_ENOMEM;
-  if (hipMalloc(&c->pairtab, sizeof(double) * cnt) != hipSuccess) {
-    c->pairtab = nullptr;
-    clear_pairs(c);
-    return HEDFT_ENOMEM;
-  }
-  std::memcpy(c->pair_v_host, vpair, sizeof(double) * cnt);
-  HEDFT_CHECK(hipMemcpy(c->pairtab, dvpair, sizeof(double) * cnt, hipMemcpyHostToDevice));
-  c->npair = npair;
-  c->delpair = delpair;
-  return HEDFT_OK;
-}
-
 int hedft_set_impurity_state(hedft_ctx* c, const double* rimp, const double* vimp, const double* aimp) {
   if (!c) return HEDFT_EINVAL;
   const size_t b = sizeof(double) * 3 * c->nimp;
@@ -2357,43 +2240,6 @@ int hedft_potenimp(hedft_ctx* c) {
   if (!c) return HEDFT_EINVAL;
   if (c->nimp > 0) hipLaunchKernelGGL(k_check_bounds, dim3(1), dim3(64), 0, c->stream, make_impdev(c), c->d);
   if (c->quantum) HEDFT_TRY(c->mixed ? potenimp_q<float>(c) : potenimp_q<double>(c));
-  return done(c);
-}
-
-// standalone forceimp from the stored gradient (potenimp.f90:49-52)
-extern "C++" {
-template <typename R>
-__global__ void __launch_bounds__(256) k_force_ew(Dims d, ImpDev im, const R* gx, const R* gy, const R* gz,
-                                                  double* partials) {
-  double red[3 * HEDFT_MAX_IMP];
-  for (int r = 0; r < 3 * HEDFT_MAX_IMP; ++r) red[r] = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < d.n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int ix = (int)(i % d.nx), iy = (int)((i / d.nx) % d.ny), iz = (int)(i / ((long long)d.nx * d.ny));
-    double xx, yy, zz;
-    imp_xyz(im, ix, iy, iz, xx, yy, zz);
-    for (int k = 0; k < im.nimp; ++k) {
-      const double u = imp_u_k(im, k, xx, yy, zz, false);
-      red[3 * k] += (double)gx[i] * u;
-      red[3 * k + 1] += (double)gy[i] * u;
-      red[3 * k + 2] += (double)gz[i] * u;
-    }
-  }
-  block_reduce_store<3 * HEDFT_MAX_IMP>(red, partials);
-}
-}  // extern "C++"
-
-int hedft_forceimp(hedft_ctx* c) {
-  if (!c) return HEDFT_EINVAL;
-  if (c->nimp == 0) return HEDFT_OK;
-  HEDFT_TRY(imp_ready(c));
-  if (c->mixed)
-    hipLaunchKernelGGL(k_force_ew<float>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
-                       fld<float>(c->dxden), fld<float>(c->dyden), fld<float>(c->dzden), force_partials(c));
-  else
-    hipLaunchKernelGGL(k_force_ew<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
-                       c->dxden, c->dyden, c->dzden, force_partials(c));
-  HEDFT_TRY(force_finalize(c, EW_BLOCKS, 0, 3 * HEDFT_MAX_IMP));
   return done(c);
 }
 

@@ -162,9 +162,6 @@ struct OutGradZForce {
   }
 };
 
-ImpDev make_impdev(hedft_ctx* c);
-double* force_partials(hedft_ctx* c);
-int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr);
 
 template <typename R>
 static int derden_force_t(hedft_ctx* c, int want_force, int update_aimp, int finalize) {
