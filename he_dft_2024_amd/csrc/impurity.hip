@@ -3,6 +3,7 @@
 // standalone forceimp entry point (Dynamics_TDDFT/potenimp.f90:23-67).
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 
@@ -126,6 +127,39 @@ int force_finalize(hedft_ctx* c, size_t nblocks, int update_aimp, int nr) {
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 
+// ---- scratch of the allocation-free forceimp ----------------------------------
+// One copy per GPU, alive until the process ends: the per-block force partials
+// of k_force_ew (EW_BLOCKS blocks x 3 HEDFT_MAX_IMP sums) and one flag word that
+// takes a clamped pair distance when the caller passes no flag of its own.
+// Every call on a GPU shares that GPU's copy, so calls that overlap on one GPU
+// race.  Each block's partials are written in full before the finalize pass
+// reads them, so nothing needs clearing between calls.
+static constexpr size_t NOALLOC_PARTIALS = (size_t)EW_BLOCKS * 3 * HEDFT_MAX_IMP;
+static constexpr size_t NOALLOC_DOUBLES = NOALLOC_PARTIALS + 32;
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+static std::mutex scratch_mu;
+static double* scratch_dev[NOALLOC_MAX_DEVICES];  // device address of `scratch` per GPU
+
+// The array's name is a host-side stand-in: its device address comes from
+// hipGetSymbolAddress, once per GPU with that GPU current.
+static int scratch_on(int device, double** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return HEDFT_EINVAL;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "hedft: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return HEDFT_EHIP;
+    }
+    scratch_dev[device] = static_cast<double*>(p);
+  }
+  *out = scratch_dev[device];
+  return HEDFT_OK;
+}
+
 }  // namespace hedft
 
 using namespace hedft;
@@ -167,6 +201,63 @@ int hedft_forceimp(hedft_ctx* c) {
     hipLaunchKernelGGL(k_force_ew<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, make_impdev(c),
                        c->dxden, c->dyden, c->dzden, force_partials(c));
   return force_finalize(c, EW_BLOCKS, 0, 3 * HEDFT_MAX_IMP);
+}
+
+int hedft_forceimp_noalloc(const hedft_grid* g, int32_t device, void* stream, const double* dxden,
+                           const double* dyden, const double* dzden, int32_t nimp, int32_t npot, double delinter,
+                           double rmaxinterpol, const double* potion, const double* rimp, int32_t npair,
+                           double delpair, const double* dvpair, double* force, int32_t* lstop) {
+  if (!g || g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || !dxden || !dyden || !dzden || !rimp || !force)
+    return HEDFT_EINVAL;
+  if (nimp < 1 || nimp > HEDFT_MAX_IMP || npot < 2 || !potion || !(delinter > 0.0)) return HEDFT_EINVAL;
+  if (nimp >= 2 && (npair < 2 || !(delpair > 0.0) || !dvpair)) return HEDFT_EINVAL;
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess) return HEDFT_EHIP;
+  if (hipSetDevice(device) != hipSuccess) return HEDFT_EINVAL;
+  double* sc = nullptr;
+  int rc = scratch_on(device, &sc);
+  if (rc == HEDFT_OK) {
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    Dims d;
+    d.nx = g->nx;
+    d.ny = g->ny;
+    d.nz = g->nz;
+    d.nxh = g->nx / 2 + 1;
+    d.n = (long long)d.nx * d.ny * d.nz;
+    d.nh = (long long)d.nxh * d.ny * d.nz;
+    const double hx = 2.0 * std::fabs(g->xmax) / d.nx, hy = 2.0 * std::fabs(g->ymax) / d.ny,
+                 hz = 2.0 * std::fabs(g->zmax) / d.nz;
+    int* flag = lstop ? lstop : reinterpret_cast<int*>(sc + NOALLOC_PARTIALS);
+    ImpDev im;
+    im.nimp = nimp;
+    im.npot = npot;
+    im.delinter = delinter;
+    im.rmax = rmaxinterpol;
+    im.potion = potion;
+    im.rimp = rimp;
+    im.lstop = flag;
+    im.x0 = -g->xmax;
+    im.y0 = -g->ymax;
+    im.z0 = -g->zmax;
+    im.hx = hx;
+    im.hy = hy;
+    im.hz = hz;
+    PairDev pr;
+    pr.dv = nimp >= 2 ? dvpair : nullptr;
+    pr.np = npair;
+    pr.del = delpair;
+    pr.lstop = flag;
+    hipLaunchKernelGGL(k_force_ew<double>, dim3(EW_BLOCKS), dim3(256), 0, st, d, im, dxden, dyden, dzden, sc);
+    hipLaunchKernelGGL(k_force_finalize<3 * HEDFT_MAX_IMP>, dim3(1), dim3(256), 0, st, sc, EW_BLOCKS, rimp, force,
+                       nullptr, nullptr, nimp, hx * hy * hz, 0, pr);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      fprintf(stderr, "hedft: %s\n", hipGetErrorString(e));
+      rc = HEDFT_EHIP;
+    }
+  }
+  hipSetDevice(prev);
+  return rc;
 }
 
 }  // extern "C"

@@ -141,6 +141,8 @@ def lib():
         L.hedft_set_impurities.argtypes = [vp, ip, ip, d, d, vp, vp]
         L.hedft_set_impurity_state.argtypes = [vp, vp, vp, vp]
         L.hedft_set_impurity_pairs.argtypes = [vp, ip, d, vp, vp]
+        L.hedft_forceimp_noalloc.argtypes = [C.POINTER(Grid), ip, vp, vp, vp, vp, ip, ip, d, d, vp, vp, ip, d,
+                                             vp, vp, vp]
         L.hedft_get_impurity_state.argtypes = [vp, vp, vp, vp, vp, C.POINTER(ip)]
         L.hedft_statics_init.argtypes = [vp, d, dp]
         L.hedft_statics_iteration.argtypes = [vp, d, ip, d, dp, dp]

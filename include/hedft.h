@@ -240,6 +240,25 @@ int hedft_potenimp(hedft_ctx* c);
 int hedft_forceimp(hedft_ctx* c);
 /* aimp = F/m (DFT4He3d.f90:692-696). */
 int hedft_update_aimp(hedft_ctx* c);
+/* forceimp() without a context and without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  Same
+ * arithmetic as hedft_forceimp on a context holding the same fields (results
+ * are bit-identical).  Every pointer except g is DEVICE memory of `device`:
+ * dxden/dyden/dzden (nx*ny*nz FP64, x fastest), potion (nimp x npot tables, as
+ * hedft_set_impurities), rimp (nimp,3, k fastest), dvpair (the pair force
+ * tables of hedft_set_impurity_pairs, needed when nimp >= 2), force (out,
+ * (nimp,3)), lstop (int flag set when a pair distance leaves its table; may be
+ * NULL).  Work is enqueued on `stream` (a hipStream_t; NULL = the default
+ * stream) and not waited for.  Scratch: a fixed per-GPU device array of
+ * 1024 x 12 + 32 doubles held until the process ends; nimp above
+ * HEDFT_MAX_IMP is rejected.  Calls that overlap on one GPU share that array
+ * and race: serialise them (one stream, or events). */
+int hedft_forceimp_noalloc(const hedft_grid* g, int32_t device, void* stream, const double* dxden,
+                           const double* dyden, const double* dzden, int32_t nimp, int32_t npot,
+                           double delinter, double rmaxinterpol, const double* potion,
+                           const double* rimp, int32_t npair, double delpair, const double* dvpair,
+                           double* force, int32_t* lstop);
 
 /* ---- quantum impurity (Dynamics_quantum_impurities) -------------------- */
 /* One impurity described by a wave function psix (N_imp = 1, so no
