@@ -389,7 +389,7 @@ template <typename R>
 __device__ __forceinline__ void xout_apply(const XOutT<R>& o, size_t i, double v) {
   if (o.op == XOUT_POT4) {
     // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + conv [+ uext | potx4] [+ penalty]
-    // (He_statics/poten.f90:277-289,315-319; Dynamics_TDDFT/poten.f90:194-218;
+    // (He_statics/poten.f90:73-91,111-115; Dynamics_TDDFT/poten.f90:194-218;
     // Dynamics_quantum_impurities/poten.f90:116-121)
     const double cg = o.dencg[i];
     double p = (double)o.delj4[i] + cg * cg * (o.a0 + o.a1 * cg) + v;

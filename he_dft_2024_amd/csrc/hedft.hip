@@ -90,7 +90,7 @@ struct Lam {
 
 // ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
 // k = timec*((lap)*h2o2m4 - (pot4-mu4)*psi) - i*uimp*psi   (steprk.f90:67,
-// steppc.f90:432,499; mu4 and no uimp in Dynamics_quantum_impurities/steprk.f90:68,
+// steppc.f90:51,118; mu4 and no uimp in Dynamics_quantum_impurities/steprk.f90:68,
 // steppc.f90:46,105; mu4 = 0 for classical impurities)
 // ps is psi at the point, taken from the stencil's LDS tile (not re-read)
 template <typename R>
@@ -140,7 +140,7 @@ struct HpsiX {
 
 // H: HpsiC (helium) or HpsiX (quantum impurity); den (|psi|^2, R storage) may be null
 template <typename R, typename H = HpsiC<R>>
-struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
+struct OutRK {  // one RK-Gill stage (steprk.f90:67-79)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
   static constexpr bool CENTRE = true;  // store() receives psi at the point from the LDS tile
@@ -169,7 +169,7 @@ struct OutRK {  // one RK-Gill stage (steprk.f90:304-316)
 };
 
 template <typename R, typename H = HpsiC<R>>
-struct OutPred {  // predictor + modifier (steppc.f90:432-442)
+struct OutPred {  // predictor + modifier (steppc.f90:51-61)
   static constexpr int NR = 0;
   static constexpr int KB = HEDFT_KB_STEP;
   static constexpr bool CENTRE = true;  // store() receives psi at the point from the LDS tile
@@ -195,7 +195,7 @@ struct OutPred {  // predictor + modifier (steppc.f90:432-442)
 };
 
 template <typename R, typename H = HpsiC<R>>
-struct OutCorr {  // corrector + final value (steppc.f90:499-510), sum|pc|
+struct OutCorr {  // corrector + final value (steppc.f90:118-127), sum|pc|
   static constexpr int NR = 1;
   static constexpr int KB = HEDFT_KB_STEP;
   static constexpr bool CENTRE = true;  // store() receives psi at the point from the LDS tile
@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(256) k_evolo_new(long long n, double* psi, dou
   block_reduce_store<1>(red, partials);
 }
 
-// STEPPCR predictor + modifier (steppcr.f90:140-153), sum(psi^2)
+// STEPPCR predictor + modifier (steppcr.f90:40-61), sum(psi^2)
 __global__ void __launch_bounds__(256) k_pcr_pred(long long n, double* psi, const double* hpsi, double* pc,
                                                   double* timec, double* po3, const double* ho1, double* ho2,
                                                   const double* scal, double deltat, double dtmin, double sdmu,
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(256) k_pcr_pred(long long n, double* psi, cons
   block_reduce_store<1>(red, partials);
 }
 
-// STEPPCR corrector + final (steppcr.f90:174-186): sum(psi^2), sum|pc|
+// STEPPCR corrector + final (steppcr.f90:74-94): sum(psi^2), sum|pc|
 __global__ void __launch_bounds__(256) k_pcr_corr(long long n, double* psi, const double* hpsi, double* pc,
                                                   const double* timec, const double* po3, const double* po2,
                                                   const double* ho1, const double* ho2, const double* scal,
@@ -436,7 +436,7 @@ __global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, d
   if (i >= m) return;
   double *R = imp + IMP_R, *V = imp + IMP_V, *A = imp + IMP_A, *QR = imp + IMP_QR, *QV = imp + IMP_QV;
   double *RO = imp + IMP_ROLD, *VO = imp + IMP_VOLD, *AO = imp + IMP_AOLD;
-  {  // steprk.f90:330-340
+  {  // steprk.f90:93-103
     const double st = a * (V[i] - b * QR[i]);
     QR[i] = QR[i] + 3.0 * st - cc * V[i];
     if (first) {
@@ -446,7 +446,7 @@ __global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, d
     }
     R[i] = R[i] + dt * st;
   }
-  {  // steprk.f90:345-354
+  {  // steprk.f90:108-117
     const double st = a * (A[i] - b * QV[i]);
     QV[i] = QV[i] + 3.0 * st - cc * A[i];
     if (first) {
@@ -460,7 +460,7 @@ __global__ void k_imp_rk(double* imp, int nimp, int first, double a, double b, d
   }
 }
 
-// steppc.f90:461-480 (slot indices 0-based)
+// steppc.f90:79-99 (slot indices 0-based)
 __global__ void k_imp_pred(double* imp, int nimp, double dt, int r3, int v1, int v2, int v3, int a1, int a2) {
   const int m = 3 * nimp, i = threadIdx.x;
   if (i >= m) return;
@@ -476,7 +476,7 @@ __global__ void k_imp_pred(double* imp, int nimp, double dt, int r3, int v1, int
   AO[12 * a2 + i] = A[i];
 }
 
-// steppc.f90:530-555; errimp, errvimp into scal (summed by lane 0 in coordinate order)
+// steppc.f90:148-174; errimp, errvimp into scal (summed by lane 0 in coordinate order)
 __global__ void k_imp_corr(double* imp, int nimp, double dt, int r2, int r3, int v1, int v2, int v3, int a1,
                            int a2, double* scal) {
   __shared__ double err[2][3 * HEDFT_MAX_IMP];
@@ -708,7 +708,7 @@ static void kernel_tables_compute(const hedft_ctx* c, std::vector<double>& fv, s
           const size_t k = (size_t)ix + (size_t)nxh * (iy + (size_t)ny * iz);
           const double pm = std::sqrt(px[ix] * px[ix] + py[iy] * py[iy] + pz[iz] * pz[iz]);
           fv[k] = fvlj_radial(pm, c->p.eps4, c->p.sigma4, h4);
-          double w = 1.0;  // initcg.f90:314-331
+          double w = 1.0;  // initcg.f90:32-49
           if (pm > 0.0) {
             const double arg = twopi * pm * h4;
             const double aux = std::sin(arg) - arg * std::cos(arg);
@@ -1676,6 +1676,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   c->mode = mode;
   c->mixed = (p->precision == HEDFT_PREC_MIXED);
   c->zconv_fused = 1;
+  if (const char* e = std::getenv("HEDFT_ZCONV_FUSED")) c->zconv_fused = std::atoi(e);
   // measured at 128^3: on the side stream after grad(den) for double storage,
   // on the main stream after the den convolutions for float storage
   c->lap_late = c->mixed ? 3 : 4;

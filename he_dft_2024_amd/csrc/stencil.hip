@@ -200,7 +200,7 @@ int derden_force_finalize(hedft_ctx* c, int update_aimp) {
 // z pass of the alpha_s divergence with the mean-field assembly:
 // ualphas = (conv/den0s + sum_a d_a(f int_a)) h2o2m4 alphas   (term_alfa.f90:80-98)
 // pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ uext | potx4] + ualphas
-// (He_statics/poten.f90:277-296, Dynamics_TDDFT/poten.f90:194-211,
+// (He_statics/poten.f90:73-91, Dynamics_TDDFT/poten.f90:194-211,
 // Dynamics_quantum_impurities/poten.f90:116-138)
 template <typename R>
 struct OutDivPot4 {

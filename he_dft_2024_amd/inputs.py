@@ -78,7 +78,7 @@ def pair_tables(nimp: int, rmax: float, npair: int = 200001, pairs=None):
 
 
 def impurity_table(n_max: int, box_max: float, **lj):
-    """potenimpini (potenimpini.f90:154-170): npot = 100 max(n) + 1 points on
+    """potenimpini (potenimpini.f90:31-59): npot = 100 max(n) + 1 points on
     [0, 4 max(xmax)], DelInter = rmax/(npot-1).  Returns (potion, delinter, rmax)."""
     npot = 100 * n_max + 1
     rmax = 4.0 * box_max

@@ -117,7 +117,7 @@ typedef struct hedft_params {
   double dtmin, sdmu, dmumax, psimin;           /* statics local time step  */
   double lambdah, tmean[3], tsurf[3];           /* TDDFT absorbing layer    */
   int32_t precision; /* HEDFT_PREC_FP64 | HEDFT_PREC_MIXED                  */
-  int32_t lexternal; /* statics: add uext to pot4 (He_statics/poten.f90:277) */
+  int32_t lexternal; /* statics: add uext to pot4 (He_statics/poten.f90:73-78) */
 } hedft_params;
 
 /* Per-term energies, He_statics/energy.f90:12-224 and

@@ -527,7 +527,7 @@ octx* oracle_create(const ogrid* g, const oparams* p, int mode) {
   const double twopi = 2.0 * M_PI;
   double h4 = p->h4;
   /* fforma (fforma.f90:36-60, 'OT' branch since core is character(len=2)),
-   * initcg (initcg.f90:314-331) with h_cg = h4 in both branches, kalfs
+   * initcg (initcg.f90:32-49) with h_cg = h4 in both branches, kalfs
    * (BCN4HeDFT.f90:849) */
 #pragma omp parallel for schedule(static)
   for (int iz = 0; iz < nz; ++iz) {
@@ -676,7 +676,7 @@ void oracle_term_alfa(octx* c) {
 }
 
 /* correlation + LJ + solid + alpha_s assembly common to both poten variants:
- * He_statics/poten.f90:27-85,290-319 and Dynamics_TDDFT/poten.f90:150-218 */
+ * He_statics/poten.f90:27-115 and Dynamics_TDDFT/poten.f90:150-218 */
 static void poten_tail(octx* c) {
   size_t n = c->n;
   const oparams* p = &c->p;
@@ -1188,7 +1188,7 @@ void oracle_steprk(octx* c, double deltat) {
         px[i] = px[i] + deltat * s1;
       }
     }
-    /* impurity positions and velocities, same scheme (steprk.f90:330-354) */
+    /* impurity positions and velocities, same scheme (steprk.f90:93-117) */
     for (int i = 0; i < m; ++i) {
       double st = a * (c->vimp[i] - b * c->qr[i]);
       c->qr[i] = c->qr[i] + 3.0 * st - cc * c->vimp[i];
@@ -1259,7 +1259,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
     }
     int t = c->ioldh[1]; c->ioldh[1] = c->ioldh[0]; c->ioldh[0] = t;
   }
-  /* impurity predictor + modifier (steppc.f90:461-480) */
+  /* impurity predictor + modifier (steppc.f90:79-99) */
   {
     double st[MAXIMP * 3];
     for (int i = 0; i < m; ++i)
@@ -1332,7 +1332,7 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
     free(acc);
     int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t;
   }
-  /* impurity corrector (steppc.f90:530-555) */
+  /* impurity corrector (steppc.f90:148-174) */
   double eimp = 0, evimp = 0;
   {
     double st[MAXIMP * 3];

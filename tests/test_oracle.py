@@ -236,7 +236,7 @@ def _mode_decay(dt, nsteps):
 
 def test_imaginary_time_steppers_third_order():
     """STEPRKR/STEPPCR renormalise psi after every stage (steprkr.f90:77-79,
-    steppcr.f90:154-157,187-190); on this problem the scheme converges at third
+    steppcr.f90:58-61,91-94); on this problem the scheme converges at third
     order (error ratio 8 when dt is halved), which the test pins."""
     T = 0.06
     e1 = _mode_decay(T / 12, 12)
