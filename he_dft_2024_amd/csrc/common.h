@@ -272,6 +272,7 @@ struct hedft_ctx {
   double* timec_r;  // statics local time step
   double* hpsi_r;   // statics H psi
   double *psi1s, *psi2s;  // statics steepest descent: psi one and two steps back
+  double* psil;           // statics log-psi steppers: log|psi|
   // psi and PC history: double (statics) or double2 (TDDFT)
   void *psi, *psiold[3], *hpsiold[2], *pc, *q;
   void* lap;        // Laplacian accumulator (same type as psi)
@@ -296,6 +297,8 @@ struct hedft_ctx {
   double* host_scal;    // pinned mirror
   // stencil coefficient sets: [id-1][axis]
   hedft::StencilCoef coef[2][3];
+  // the same with one-sided boundary rows (Icon_local = 0 of the log-psi steppers)
+  hedft::StencilCoef coef0[2][3];
   // PC slot bookkeeping (host-side; rkpc: ioldp, ioldh; classicimp: ioldr, ioldv, iolda)
   int ioldp[3], ioldh[2], ioldr[3], ioldv[3], iolda[2];
   int rk_done;

@@ -425,6 +425,129 @@ __global__ void __launch_bounds__(256) k_pcr_corr(long long n, double* psi, cons
   block_reduce_store<2>(red, partials);
 }
 
+// ---- log-psi steppers (He_statics/steprkrl.f90, steppcrl.f90) --------------------
+static constexpr double UNDERPSIL = -700.0;  // modules.f90:298
+
+// out = acc + v (SQ: acc + v^2): the six derivative terms of H_l summed in the
+// reference's order, (((d2x + d2y) + d2z) + d1x^2) + d1y^2 + d1z^2
+template <bool SQ>
+struct OutAccum {
+  static constexpr int NR = 0;
+  double* partials;
+  double* out;
+  const double* acc;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double v, double*) const {
+    const double t = SQ ? v * v : v;
+    out[i] = acc ? acc[i] + t : t;
+  }
+};
+
+// hpsi = -g h2o2m4 + pot4 (steprkrl.f90:56), sum(den*hpsi)
+__global__ void __launch_bounds__(256) k_hpsil(long long n, const double* g, const double* pot4, const double* den,
+                                               double* hpsi, double h2o2m4, double* partials) {
+  double red[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double hp = -g[i] * h2o2m4 + pot4[i];
+    hpsi[i] = hp;
+    red[0] += den[i] * hp;
+  }
+  block_reduce_store<1>(red, partials);
+}
+
+// psil = max(psil, underpsil); psi = exp(psil); accumulates sum(psi^2)
+__device__ __forceinline__ double psil_exp(double* psil, double* psi, long long i, double v) {
+  if (v < UNDERPSIL) v = UNDERPSIL;
+  psil[i] = v;
+  const double e = exp(v);
+  psi[i] = e;
+  return e * e;
+}
+
+// one RK-Gill stage of STEPRKRL after the mean (steprkrl.f90:61-77)
+__global__ void __launch_bounds__(256) k_rkrl_stage(long long n, double* psil, double* psi, const double* hpsi,
+                                                    double* q, double* po0, double* po1, double* po2, double* ho0,
+                                                    double* ho1, const double* scal, double a, double b, double cc,
+                                                    double deltat, int first, double* partials) {
+  double red[1] = {0.0};
+  const double mu = scal[S_MU];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double pl = psil[i];
+    const double s4 = -(hpsi[i] - mu);
+    const double s1 = a * (s4 - b * q[i]);
+    q[i] = q[i] + 3.0 * s1 - cc * s4;
+    if (first) {
+      ho1[i] = ho0[i];
+      ho0[i] = s4;
+      po2[i] = po1[i];
+      po1[i] = po0[i];
+      po0[i] = pl;
+    }
+    red[0] += psil_exp(psil, psi, i, pl + deltat * s1);
+  }
+  block_reduce_store<1>(red, partials);
+}
+
+// cnorm = sqrt(n4/(sum psi^2 dxyz)); psil += log(cnorm); psi = |psi| cnorm; den = psi^2
+// (steprkrl.f90:78-81)
+__global__ void k_renorm_log(long long n, double* psil, double* psi, double* den, const double* scal, double n4,
+                             double dxyz) {
+  const double cnorm = sqrt(n4 / (scal[S_SUMSQ] * dxyz));
+  const double lc = log(cnorm);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    psil[i] = psil[i] + lc;
+    const double v = fabs(psi[i]) * cnorm;
+    psi[i] = v;
+    den[i] = v * v;
+  }
+}
+
+// STEPPCRL predictor + modifier (steppcrl.f90:46-59), sum(psi^2)
+__global__ void __launch_bounds__(256) k_pcrl_pred(long long n, double* psil, double* psi, const double* hpsi,
+                                                   double* pc, double* po3, const double* ho1, double* ho2,
+                                                   const double* scal, double deltat, double* partials) {
+  double red[1] = {0.0};
+  const double hmean = scal[S_MU];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double s4 = -(hpsi[i] - hmean);
+    const double s1 = po3[i] + C4O3 * deltat * (2.0 * s4 - ho1[i] + 2.0 * ho2[i]);
+    po3[i] = psil[i];
+    const double np = s1 - C112 * pc[i];
+    pc[i] = s1;
+    ho2[i] = s4;
+    red[0] += psil_exp(psil, psi, i, np);
+  }
+  block_reduce_store<1>(red, partials);
+}
+
+// STEPPCRL corrector + final (steppcrl.f90:86-98): sum(psi^2), sum|pc|
+__global__ void __launch_bounds__(256) k_pcrl_corr(long long n, double* psil, double* psi, const double* hpsi,
+                                                   double* pc, const double* po3, const double* po2,
+                                                   const double* ho1, const double* ho2, const double* scal,
+                                                   double deltat, double* partials) {
+  double red[2] = {0.0, 0.0};
+  const double mu = scal[S_MU];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double s4 = -(hpsi[i] - mu);
+    const double s5 = 0.125 * (9.0 * po3[i] - po2[i] + 3.0 * deltat * (s4 + 2.0 * ho1[i] - ho2[i]));
+    const double pcv = pc[i] - s5;
+    pc[i] = pcv;
+    red[0] += psil_exp(psil, psi, i, s5 + C9 * pcv);
+    red[1] += fabs(pcv);
+  }
+  block_reduce_store<2>(red, partials);
+}
+
+__global__ void k_log_abs(long long n, const double* psi, double* psil) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    psil[i] = log(fabs(psi[i]));
+}
+
 __global__ void k_copy_scal(double* s, int from, int to) { s[to] = s[from]; }
 
 // ---- impurity Newton updates (one lane per coordinate; N_imp*3 <= 12 numbers) --------
@@ -1460,6 +1583,73 @@ static int steppcr_impl(hedft_ctx* c, double dt, double n4) {
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 
+// H_l psil (steprkrl.f90:44-57): six derivative passes of psil with one-sided
+// rows, accumulated in work1, then hpsi and sum(den*hpsi)*dxyz/n4 into S_MU
+static int hpsi_log(hedft_ctx* c, double n4) {
+  double* g = c->work1;
+  const double* pl = c->psil;
+  InPlain<double> ip{pl};
+  launch_pencil<double, 0>(c->stream, c->d, c->coef0[1][0], ip, OutAccum<false>{nullptr, g, nullptr});
+  launch_pencil<double, 1>(c->stream, c->d, c->coef0[1][1], ip, OutAccum<false>{nullptr, g, g});
+  launch_pencil<double, 2>(c->stream, c->d, c->coef0[1][2], ip, OutAccum<false>{nullptr, g, g});
+  launch_pencil<double, 0>(c->stream, c->d, c->coef0[0][0], ip, OutAccum<true>{nullptr, g, g});
+  launch_pencil<double, 1>(c->stream, c->d, c->coef0[0][1], ip, OutAccum<true>{nullptr, g, g});
+  launch_pencil<double, 2>(c->stream, c->d, c->coef0[0][2], ip, OutAccum<true>{nullptr, g, g});
+  hipLaunchKernelGGL(k_hpsil, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, g, c->pot4, c->den, c->hpsi_r,
+                     c->p.h2o2m4, gen_partials(c));
+  finalize<1>(c, gen_partials(c), EW_BLOCKS, S_MU, c->dxyz / n4);
+  return HEDFT_OK;
+}
+
+static void renorm_log(hedft_ctx* c, double n4) {
+  hipLaunchKernelGGL(k_renorm_log, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->psil, rz(c->psi), c->den,
+                     c->scal, n4, c->dxyz);
+}
+
+// STEPRKRL (He_statics/steprkrl.f90:1-98)
+static int steprkrl_impl(hedft_ctx* c, double dt, double n4) {
+  for (int jrun = 0; jrun < 4; ++jrun) {
+    HEDFT_TRY(hpsi_log(c, n4));
+    if (jrun == 0) hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, c->stream, c->scal, S_MU, S_HMEAN);
+    hipLaunchKernelGGL(k_rkrl_stage, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->psil, rz(c->psi),
+                       c->hpsi_r, rz(c->q), rz(c->psiold[0]), rz(c->psiold[1]), rz(c->psiold[2]),
+                       rz(c->hpsiold[0]), rz(c->hpsiold[1]), c->scal, RK_A[jrun], RK_B[jrun], RK_C[jrun], dt,
+                       (int)(jrun == 0), gen_partials(c));
+    finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+    renorm_log(c, n4);
+    if (jrun <= 2) HEDFT_TRY(poten_statics(c));
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = k;
+  c->rk_done++;
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+// STEPPCRL (He_statics/steppcrl.f90:1-116)
+static int steppcrl_impl(hedft_ctx* c, double dt, double n4) {
+  HEDFT_TRY(hpsi_log(c, n4));
+  hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, c->stream, c->scal, S_MU, S_HMEAN);
+  hipLaunchKernelGGL(k_pcrl_pred, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->psil, rz(c->psi),
+                     c->hpsi_r, rz(c->pc), rz(c->psiold[c->ioldp[2]]), rz(c->hpsiold[c->ioldh[0]]),
+                     rz(c->hpsiold[c->ioldh[1]]), c->scal, dt, gen_partials(c));
+  finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+  renorm_log(c, n4);
+  std::swap(c->ioldh[0], c->ioldh[1]);
+  HEDFT_TRY(poten_statics(c));
+  HEDFT_TRY(hpsi_log(c, n4));
+  hipLaunchKernelGGL(k_pcrl_corr, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->psil, rz(c->psi),
+                     c->hpsi_r, rz(c->pc), rz(c->psiold[c->ioldp[2]]), rz(c->psiold[c->ioldp[1]]),
+                     rz(c->hpsiold[c->ioldh[0]]), rz(c->hpsiold[c->ioldh[1]]), c->scal, dt, gen_partials(c));
+  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c), EW_BLOCKS,
+                     c->scal + S_SUMSQ, 1.0);  // S_SUMSQ = sum psi^2, S_ERRPC = sum|pc|
+  renorm_log(c, n4);
+  const int t = c->ioldp[2];
+  c->ioldp[2] = c->ioldp[1];
+  c->ioldp[1] = c->ioldp[0];
+  c->ioldp[0] = t;
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
 // ---- energy -----------------------------------------------------------------------------------
 template <typename V, typename R>
 static int energy_sums(hedft_ctx* c, double* s) {
@@ -1729,6 +1919,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
     rc = rc ? rc : alloc(c, (void**)&c->timec_r, rN);
     rc = rc ? rc : alloc(c, (void**)&c->hpsi_r, rN);
     rc = rc ? rc : alloc(c, (void**)&c->psi1s, rN);
+    rc = rc ? rc : alloc(c, (void**)&c->psil, rN);
     rc = rc ? rc : alloc(c, (void**)&c->psi2s, rN);
   }
   void** psis[] = {&c->psi, &c->psiold[0], &c->psiold[1], &c->psiold[2], &c->hpsiold[0], &c->hpsiold[1],
@@ -1805,7 +1996,7 @@ int hedft_destroy(hedft_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   void* ptrs[] = {c->den, c->pot4, c->dencg, c->delj4, c->denalf, c->intx, c->inty, c->intz, c->ualphas,
                   c->dxden, c->dyden, c->dzden, c->uext, c->sto1, c->ualph_conv, c->work1, c->work2,
-                  c->timec_r, c->hpsi_r, c->psi1s, c->psi2s, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
+                  c->timec_r, c->hpsi_r, c->psi1s, c->psi2s, c->psil, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
                   c->hpsiold[1], c->pc, c->q, c->lap, c->workc1, c->fden, c->spec[0], c->spec[1], c->spec[2],
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
                   c->imp, c->lstop, c->potion, c->psix, c->psixold[0], c->psixold[1], c->psixold[2],
@@ -1887,6 +2078,9 @@ static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* comput
         default: *ptr = c->psixold[f - HEDFT_PSIXOLD1]; *bytes = cN; break;
       }
       break;
+    case HEDFT_PSIL:
+      if (td) return HEDFT_EINVAL;
+      *ptr = c->psil; *bytes = rN; break;
     default: return HEDFT_EINVAL;
   }
   if (*computed && f != HEDFT_TIMEC) *ptr = c->workc1;  // evaluated into double scratch
@@ -2140,6 +2334,34 @@ int hedft_steppcr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, doub
   return done(c);
 }
 
+int hedft_statics_log_start(hedft_ctx* c) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  hipLaunchKernelGGL(k_log_abs, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psi), c->psil);
+  return done(c);
+}
+
+int hedft_steprkrl(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  HEDFT_TRY(steprkrl_impl(c, deltat, n4));
+  double h;
+  HEDFT_TRY(read_scal(c, S_HMEAN, 1, &h));
+  if (mu4err) *mu4err = std::fabs(1.0 - (mu4 ? *mu4 : 0.0) / h);
+  if (mu4) *mu4 = h;
+  return done(c);
+}
+
+int hedft_steppcrl(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4, double* errpc) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  if (!c->rk_done) return HEDFT_ESTATE;
+  HEDFT_TRY(steppcrl_impl(c, deltat, n4));
+  double s[4];
+  HEDFT_TRY(read_scal(c, 0, 4, s));
+  if (mu4err) *mu4err = std::fabs(1.0 - (mu4 ? *mu4 : 0.0) / s[S_HMEAN]);
+  if (mu4) *mu4 = s[S_HMEAN];
+  if (errpc) *errpc = (9.0 / 121.0) * s[S_ERRPC] / (double)c->d.n;
+  return done(c);
+}
+
 int hedft_steprk(hedft_ctx* c, double deltat) {
   if (!c || c->mode != HEDFT_MODE_TDDFT) return HEDFT_EINVAL;
   HEDFT_TRY(imp_ready(c));
@@ -2347,6 +2569,20 @@ int hedft_statics_iteration_sd(hedft_ctx* c, double deltat, double vdt1, double 
                                double* mu4, double* mu4err) {
   if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
   HEDFT_TRY(hedft_evolo(c, deltat, vdt1, vdt2, iron, n4, mu4, mu4err));
+  HEDFT_TRY(statics_refresh(c));
+  HEDFT_TRY(poten_statics(c));
+  return done(c);
+}
+
+int hedft_statics_iteration_log(hedft_ctx* c, double deltat, int32_t iterrk, double n4, double* mu4,
+                                double* mu4err, double* errpc) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  if (iterrk <= 4) {
+    HEDFT_TRY(hedft_steprkrl(c, deltat, mu4, mu4err, n4));
+    if (errpc) *errpc = 0.0;
+  } else {
+    HEDFT_TRY(hedft_steppcrl(c, deltat, mu4, mu4err, n4, errpc));
+  }
   HEDFT_TRY(statics_refresh(c));
   HEDFT_TRY(poten_statics(c));
   return done(c);

@@ -107,7 +107,10 @@ static StencilCoef make_coef(int id, int icon, double h) {
 void stencil_setup(hedft_ctx* c) {
   const double h[3] = {c->hx, c->hy, c->hz};
   for (int id = 1; id <= 2; ++id)
-    for (int a = 0; a < 3; ++a) c->coef[id - 1][a] = make_coef(id, c->p.icon, h[a]);
+    for (int a = 0; a < 3; ++a) {
+      c->coef[id - 1][a] = make_coef(id, c->p.icon, h[a]);
+      c->coef0[id - 1][a] = make_coef(id, 0, h[a]);
+    }
 }
 
 template <typename V>

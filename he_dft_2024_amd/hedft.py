@@ -25,7 +25,7 @@ FIELDS = [
     "PSIOLD3", "HPSIOLD1", "HPSIOLD2", "PC", "Q", "FDEN", "FVLJ4", "WCGK", "KALFS", "WORK1",
     "WORK2", "WORKC1", "WORKK1",
     "PSIX", "POTX4", "UPOTX", "DENX", "UIMPK", "PSIXOLD1", "PSIXOLD2", "PSIXOLD3", "HPSIXOLD1",
-    "HPSIXOLD2", "PCX", "QX",
+    "HPSIXOLD2", "PCX", "QX", "PSIL",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 
@@ -146,6 +146,10 @@ def lib():
         L.hedft_get_impurity_state.argtypes = [vp, vp, vp, vp, vp, C.POINTER(ip)]
         L.hedft_statics_init.argtypes = [vp, d, dp]
         L.hedft_statics_iteration.argtypes = [vp, d, ip, d, dp, dp]
+        L.hedft_statics_iteration_log.argtypes = [vp, d, ip, d, dp, dp, dp]
+        L.hedft_steprkrl.argtypes = [vp, d, dp, dp, d]
+        L.hedft_steppcrl.argtypes = [vp, d, dp, dp, d, dp]
+        L.hedft_statics_log_start.argtypes = [vp]
         L.hedft_tddft_iteration.argtypes = [vp, d, ip, ip, dp, dp, dp]
         L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
         L.hedft_last_elapsed.argtypes = [vp, dp]
@@ -371,6 +375,25 @@ class Droplet:
         check(lib().hedft_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e)),
               "statics_iteration")
         return m.value, e.value
+
+    def statics_log_start(self):
+        check(lib().hedft_statics_log_start(self._c), "statics_log_start")
+
+    def steprkrl(self, dt, mu4, n4):
+        m, e = C.c_double(mu4), C.c_double(0)
+        check(lib().hedft_steprkrl(self._c, dt, C.byref(m), C.byref(e), n4), "steprkrl")
+        return m.value, e.value
+
+    def steppcrl(self, dt, mu4, n4):
+        m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
+        check(lib().hedft_steppcrl(self._c, dt, C.byref(m), C.byref(e), n4, C.byref(pc)), "steppcrl")
+        return m.value, e.value, pc.value
+
+    def statics_iteration_log(self, dt, iterrk, n4, mu4):
+        m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
+        check(lib().hedft_statics_iteration_log(self._c, dt, iterrk, n4, C.byref(m), C.byref(e), C.byref(pc)),
+              "statics_iteration_log")
+        return m.value, e.value, pc.value
 
     def evolo(self, dt, n4, mu4, vdt=(0.6, 0.1), iron=1):
         m, e = C.c_double(mu4), C.c_double(0)

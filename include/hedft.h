@@ -92,6 +92,7 @@ enum hedft_field {
   HEDFT_HPSIXOLD2,
   HEDFT_PCX,          /* pcx(:,:,:,1)                                        */
   HEDFT_QX,           /* qx(:,:,:,1)                                         */
+  HEDFT_PSIL,         /* statics: psil = log|psi| of the log-psi steppers    */
   HEDFT_NFIELDS
 };
 
@@ -202,6 +203,21 @@ int hedft_steprkr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, doub
  * module variable ErrPC (may be NULL). */
 int hedft_steppcr(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4,
                   double* errpc);
+/* Log-psi variants of the imaginary-time steppers (Lrkpcl): psil = log|psi|
+ * evolves as dpsil/dtau = -(H_l - mu), H_l = -(lap psil + |grad psil|^2) h2o2m4
+ * + pot4, with one-sided stencils (Icon_local = 0, modules.f90:294), deltat
+ * without the local time step, psil >= underpsil = -700 (modules.f90:298), and
+ * psi = exp(psil) renormalised to n4 after every stage.
+ * hedft_statics_log_start: psil = log|psi| (BCN4HeDFT.f90:1081-1083).
+ * STEPRKRL(deltat,mu4,mu4err,n4), He_statics/steprkrl.f90:1-98.
+ * STEPPCRL(deltat,mu4,mu4err,n4), He_statics/steppcrl.f90:1-116 (errpc: ErrPC).
+ * hedft_statics_iteration_log: STEPRKRL when iterrk <= 4 else STEPPCRL, then the
+ * driver refresh and poten (BCN4HeDFT.f90:1122-1160). */
+int hedft_statics_log_start(hedft_ctx* c);
+int hedft_steprkrl(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4);
+int hedft_steppcrl(hedft_ctx* c, double deltat, double* mu4, double* mu4err, double n4, double* errpc);
+int hedft_statics_iteration_log(hedft_ctx* c, double deltat, int32_t iterrk, double n4, double* mu4,
+                                double* mu4err, double* errpc);
 /* STEPRK(deltat), Dynamics_TDDFT/steprk.f90:1-144. */
 int hedft_steprk(hedft_ctx* c, double deltat);
 /* STEPPC(deltat,errHe,errimp,errvimp), Dynamics_TDDFT/steppc.f90:1-180.

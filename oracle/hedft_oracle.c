@@ -71,7 +71,7 @@ enum {
   F_UALPHAS, F_DXDEN, F_DYDEN, F_DZDEN, F_UIMP, F_UEXT, F_TIMEC, F_PSIOLD1, F_PSIOLD2,
   F_PSIOLD3, F_HPSIOLD1, F_HPSIOLD2, F_PC, F_Q, F_FDEN, F_FVLJ4, F_WCGK, F_KALFS, F_WORK1,
   F_WORK2, F_WORKC1, F_WORKK1, F_PSIX, F_POTX4, F_UPOTX, F_DENX, F_UIMPK, F_PSIXOLD1, F_PSIXOLD2,
-  F_PSIXOLD3, F_HPSIXOLD1, F_HPSIXOLD2, F_PCX, F_QX, F_NFIELDS
+  F_PSIXOLD3, F_HPSIXOLD1, F_HPSIXOLD2, F_PCX, F_QX, F_PSIL, F_NFIELDS
 };
 
 typedef struct {
@@ -89,6 +89,7 @@ typedef struct {
   /* psi and history: real (statics) or complex (TDDFT) stored as doubles */
   double *psi, *timec, *psiold[3], *hpsiold[2], *pc, *q, *hpsi;
   double *psi1s, *psi2s;  /* statics steepest descent: psi one and two steps back */
+  double *psil;           /* statics log-psi variants: log|psi| (module rho: psil) */
   double *workc1;
   /* half spectra */
   cplx *fden, *wk1, *wk2, *wk3, *workk1;
@@ -501,7 +502,7 @@ octx* oracle_create(const ogrid* g, const oparams* p, int mode) {
   c->sto4 = dalloc(n); c->sto5 = dalloc(n); c->sto6 = dalloc(n); c->penalty = dalloc(n);
   c->psi = dalloc(n * cw); c->timec = dalloc(n * cw); c->pc = dalloc(n * cw); c->q = dalloc(n * cw);
   c->hpsi = dalloc(n * cw);
-  c->psi1s = dalloc(n); c->psi2s = dalloc(n);
+  c->psi1s = dalloc(n); c->psi2s = dalloc(n); c->psil = dalloc(n);
   for (int k = 0; k < 3; ++k) c->psiold[k] = dalloc(n * cw);
   for (int k = 0; k < 2; ++k) c->hpsiold[k] = dalloc(n * cw);
   c->workc1 = dalloc(2 * n);
@@ -559,7 +560,7 @@ void oracle_destroy(octx* c) {
                     c->intx, c->inty, c->intz, c->ualphas, c->dxden, c->dyden, c->dzden, c->uimp,
                     c->uext, c->work1, c->work2, c->sto1, c->sto2, c->sto3, c->sto4, c->sto5,
                     c->sto6, c->penalty, c->psi, c->timec, c->pc, c->q, c->hpsi, c->psiold[0],
-                    c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->psi1s, c->psi2s, c->workc1,
+                    c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->psi1s, c->psi2s, c->psil, c->workc1,
                     (double*)c->fden, (double*)c->wk1, (double*)c->wk2, (double*)c->wk3,
                     (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion,
                     c->psix, c->psixold[0], c->psixold[1], c->psixold[2], c->hpsixold[0],
@@ -613,9 +614,10 @@ double* oracle_field(octx* c, int id, size_t* ndoubles) {
     case F_HPSIXOLD1: case F_HPSIXOLD2: ptr = c->hpsixold[id - F_HPSIXOLD1]; cnt = 2 * n; break;
     case F_PCX: ptr = c->pcx; cnt = 2 * n; break;
     case F_QX: ptr = c->qx; cnt = 2 * n; break;
+    case F_PSIL: ptr = c->psil; break;
     default: ptr = NULL; cnt = 0;
   }
-  if (id >= F_PSIX && !c->quantum) { ptr = NULL; cnt = 0; }
+  if (id >= F_PSIX && id <= F_QX && !c->quantum) { ptr = NULL; cnt = 0; }
   if (ndoubles) *ndoubles = cnt;
   return ptr;
 }
@@ -889,6 +891,149 @@ void oracle_steppcr(octx* c, double deltat, double* mu4, double* mu4err, double 
   double epc = c9 * sum_finish(acc, c->nz);
   free(acc);
   renorm_real(c, n4, 0);
+  epc /= (double)n;
+  { int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t; }
+  *mu4err = fabs(1.0 - *mu4 / hmean);
+  *mu4 = hmean;
+  if (errpc) *errpc = epc;
+}
+
+/* ---- log-psi variants (He_statics/steprkrl.f90, steppcrl.f90): psil = log|psi|
+ * evolves with H_l = -(lap psil + |grad psil|^2) h2o2m4 + pot4, the derivatives
+ * taken with Icon_local = 0 (one-sided boundary rows, modules.f90:294) */
+static const double UNDERPSIL = -700.0; /* modules.f90:298 */
+
+/* hpsi = -(d2x+d2y+d2z + d1x^2+d1y^2+d1z^2) h2o2m4 + pot4 (steprkrl.f90:44-56);
+ * returns sum(den*hpsi) */
+static double hpsi_log(octx* c) {
+  double h[3] = {c->hx, c->hy, c->hz};
+  double* d2[3] = {c->sto1, c->sto2, c->sto3};
+  double* d1[3] = {c->sto4, c->sto5, c->sto6};
+  for (int a = 0; a < 3; ++a) {
+    oracle_derivnd(2, c->nx, c->ny, c->nz, h[a], a + 1, c->psil, d2[a], 0, 1);
+    oracle_derivnd(1, c->nx, c->ny, c->nz, h[a], a + 1, c->psil, d1[a], 0, 1);
+  }
+  size_t plane = (size_t)c->nx * c->ny;
+  double* acc = dalloc(c->nz);
+  double hh = c->p.h2o2m4;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < c->nz; ++iz) {
+    double s = 0;
+    for (size_t j = 0; j < plane; ++j) {
+      size_t i = iz * plane + j;
+      double g = c->sto1[i] + c->sto2[i] + c->sto3[i] + c->sto4[i] * c->sto4[i] + c->sto5[i] * c->sto5[i] +
+                 c->sto6[i] * c->sto6[i];
+      c->hpsi[i] = -g * hh + c->pot4[i];
+      s += c->den[i] * c->hpsi[i];
+    }
+    acc[iz] = s;
+  }
+  double s = sum_finish(acc, c->nz);
+  free(acc);
+  return s;
+}
+
+/* psil = max(psil, underpsil); psi = exp(psil); cnorm = sqrt(n4/(sum psi^2 dxyz));
+ * psil += log(cnorm); psi = |psi| cnorm; den = psi^2 (steprkrl.f90:76-81) */
+static void renorm_log(octx* c, double n4) {
+  size_t n = c->n;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    if (c->psil[i] < UNDERPSIL) c->psil[i] = UNDERPSIL;
+    c->psi[i] = exp(c->psil[i]);
+  }
+  double cnorm = sqrt(n4 / (sumsq_plane(c, c->psi) * c->dxyz));
+  double lc = log(cnorm);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    c->psil[i] = c->psil[i] + lc;
+    double v = fabs(c->psi[i]) * cnorm;
+    c->psi[i] = v;
+    c->den[i] = v * v;
+  }
+}
+
+/* psil = log|psi| before the iterations (BCN4HeDFT.f90:1081-1083) */
+void oracle_statics_log_start(octx* c) {
+  size_t n = c->n;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) c->psil[i] = log(fabs(c->psi[i]));
+}
+
+/* STEPRKRL (He_statics/steprkrl.f90:1-98) */
+void oracle_steprkrl(octx* c, double deltat, double* mu4, double* mu4err, double n4) {
+  size_t n = c->n;
+  double hmean = 0.0, merr = 0.0;
+  for (int jrun = 0; jrun < 4; ++jrun) {
+    merr = hpsi_log(c) * c->dxyz / n4;
+    if (jrun == 0) hmean = merr;
+    double a = RK_A[jrun], b = RK_B[jrun], cc = RK_C[jrun];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+      double s4 = -(c->hpsi[i] - merr);
+      double s1 = a * (s4 - b * c->q[i]);
+      c->q[i] = c->q[i] + 3.0 * s1 - cc * s4;
+      if (jrun == 0) {
+        c->hpsiold[1][i] = c->hpsiold[0][i];
+        c->hpsiold[0][i] = s4;
+        c->psiold[2][i] = c->psiold[1][i];
+        c->psiold[1][i] = c->psiold[0][i];
+        c->psiold[0][i] = c->psil[i];
+      }
+      c->psil[i] = c->psil[i] + deltat * s1;
+    }
+    renorm_log(c, n4);
+    if (jrun <= 2) oracle_poten(c);
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = k;
+  for (int k = 0; k < 2; ++k) c->ioldh[k] = k;
+  *mu4err = fabs(1.0 - *mu4 / hmean);
+  *mu4 = hmean;
+}
+
+/* STEPPCRL (He_statics/steppcrl.f90:1-116) */
+void oracle_steppcrl(octx* c, double deltat, double* mu4, double* mu4err, double n4, double* errpc) {
+  size_t n = c->n;
+  const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
+  double hmean = hpsi_log(c) * c->dxyz / n4;
+  double* po3 = c->psiold[c->ioldp[2]];
+  double* ho1 = c->hpsiold[c->ioldh[0]];
+  double* ho2 = c->hpsiold[c->ioldh[1]];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double s4 = -(c->hpsi[i] - hmean);
+    double s1 = po3[i] + c4o3 * deltat * (2.0 * s4 - ho1[i] + 2.0 * ho2[i]);
+    po3[i] = c->psil[i];
+    c->psil[i] = s1 - c112 * c->pc[i];
+    c->pc[i] = s1;
+    ho2[i] = s4;
+  }
+  renorm_log(c, n4);
+  { int t = c->ioldh[1]; c->ioldh[1] = c->ioldh[0]; c->ioldh[0] = t; }
+  oracle_poten(c);
+  double merr = hpsi_log(c) * c->dxyz / n4;
+  po3 = c->psiold[c->ioldp[2]];
+  double* po2 = c->psiold[c->ioldp[1]];
+  ho1 = c->hpsiold[c->ioldh[0]];
+  ho2 = c->hpsiold[c->ioldh[1]];
+  size_t plane = (size_t)c->nx * c->ny;
+  double* acc = dalloc(c->nz);
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < c->nz; ++iz) {
+    double s = 0;
+    for (size_t j = 0; j < plane; ++j) {
+      size_t i = iz * plane + j;
+      double s4 = -(c->hpsi[i] - merr);
+      double s5 = 0.125 * (9.0 * po3[i] - po2[i] + 3.0 * deltat * (s4 + 2.0 * ho1[i] - ho2[i]));
+      c->pc[i] = c->pc[i] - s5;
+      c->psil[i] = s5 + c9 * c->pc[i];
+      s += fabs(c->pc[i]);
+    }
+    acc[iz] = s;
+  }
+  double epc = c9 * sum_finish(acc, c->nz);
+  free(acc);
+  renorm_log(c, n4);
   epc /= (double)n;
   { int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t; }
   *mu4err = fabs(1.0 - *mu4 / hmean);
@@ -1613,6 +1758,15 @@ void oracle_evolo(octx* c, double deltat, double vdt1, double vdt2, int iron, do
 void oracle_statics_iteration_sd(octx* c, double deltat, double vdt1, double vdt2, int iron, double n4,
                                  double* mu4, double* mu4err) {
   oracle_evolo(c, deltat, vdt1, vdt2, iron, n4, mu4, mu4err);
+  oracle_statics_refresh(c);
+  oracle_poten(c);
+}
+
+/* statics iteration with the log-psi steppers (Lrkpcl, BCN4HeDFT.f90:1122-1160) */
+void oracle_statics_iteration_log(octx* c, double deltat, int iterrk, double n4, double* mu4,
+                                  double* mu4err, double* errpc) {
+  if (iterrk <= 4) oracle_steprkrl(c, deltat, mu4, mu4err, n4);
+  else oracle_steppcrl(c, deltat, mu4, mu4err, n4, errpc);
   oracle_statics_refresh(c);
   oracle_poten(c);
 }

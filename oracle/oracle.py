@@ -27,7 +27,7 @@ FIELDS = [
     "PSIOLD3", "HPSIOLD1", "HPSIOLD2", "PC", "Q", "FDEN", "FVLJ4", "WCGK", "KALFS", "WORK1",
     "WORK2", "WORKC1", "WORKK1",
     "PSIX", "POTX4", "UPOTX", "DENX", "UIMPK", "PSIXOLD1", "PSIXOLD2", "PSIXOLD3", "HPSIXOLD1",
-    "HPSIXOLD2", "PCX", "QX",
+    "HPSIXOLD2", "PCX", "QX", "PSIL",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 MHE = 4.002572164 * 0.02061484  # helium mass in K^-1 A^-2 (Dynamics_TDDFT/DFT4He3d.f90:103,315)
@@ -116,6 +116,10 @@ def lib():
         L.oracle_energy.argtypes = [vp, vp]
         L.oracle_statics_init.argtypes = [vp, C.c_double, dp]
         L.oracle_statics_iteration.argtypes = [vp, C.c_double, ip, C.c_double, dp, dp, dp]
+        L.oracle_statics_iteration_log.argtypes = [vp, C.c_double, ip, C.c_double, dp, dp, dp]
+        L.oracle_steprkrl.argtypes = [vp, C.c_double, dp, dp, C.c_double]
+        L.oracle_steppcrl.argtypes = [vp, C.c_double, dp, dp, C.c_double, dp]
+        L.oracle_statics_log_start.argtypes = [vp]
         L.oracle_evolo.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
         L.oracle_statics_iteration_sd.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
         L.oracle_ironing.argtypes = [ip, ip, ip, vp]
@@ -399,6 +403,24 @@ class Oracle:
     def statics_iteration(self, dt, iterrk, n4, mu4):
         m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
         lib().oracle_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e), C.byref(pc))
+        return m.value, e.value, pc.value
+
+    def statics_log_start(self):
+        lib().oracle_statics_log_start(self._c)
+
+    def steprkrl(self, dt, mu4, n4):
+        m, e = C.c_double(mu4), C.c_double(0)
+        lib().oracle_steprkrl(self._c, dt, C.byref(m), C.byref(e), n4)
+        return m.value, e.value
+
+    def steppcrl(self, dt, mu4, n4):
+        m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
+        lib().oracle_steppcrl(self._c, dt, C.byref(m), C.byref(e), n4, C.byref(pc))
+        return m.value, e.value, pc.value
+
+    def statics_iteration_log(self, dt, iterrk, n4, mu4):
+        m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
+        lib().oracle_statics_iteration_log(self._c, dt, iterrk, n4, C.byref(m), C.byref(e), C.byref(pc))
         return m.value, e.value, pc.value
 
     def evolo(self, dt, n4, mu4, vdt=(0.6, 0.1), iron=1):

@@ -165,3 +165,55 @@ def test_absorber_profile_rounded_to_single_precision():
     assert np.array_equal(tc.imag, np.ones_like(lam))
     assert np.abs(tc.real - lam32).max() <= 1e-15 * lam.max()
     assert np.abs(tc.real - lam).max() > 0.0   # the rounding is visible
+
+
+# ------------------------------------------------------------- log-psi steppers
+# In a harmonic trap pot4 = omega r^2 (uext with the functional switched off),
+# log psi = -alpha r^2 + beta stays quadratic under STEPRKRL/STEPPCRL: the
+# one-sided 13-point stencils (Icon_local = 0) are exact on quadratics, so
+# H_l = 6 h alpha + (omega - 4 h alpha^2) r^2 and alpha obeys the Riccati
+# equation d alpha/dtau = omega - 4 h alpha^2, alpha(tau) = alpha* tanh(2 sqrt(omega
+# h) tau + atanh(alpha0/alpha*)), alpha* = sqrt(omega/(4h)); the ground state
+# alpha* is a fixed point with mu = 6 h alpha*.  A wrong sign or factor of the
+# |grad psil|^2 term, of h2o2m4, or a step other than deltat misses this.
+H2O2M4 = 6.05969638298
+
+
+def _log_trap(a0, dt, nsteps, sigma=1.5):
+    n, h = 32, 0.4
+    xm = 0.5 * n * h
+    omega = H2O2M4 / sigma ** 4
+    x = I.grid_axes(n, xm)
+    R2 = x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2
+    p = O.default_params(eps4=0.0, cp4=0.0, cpp4=0.0, core_otc=0, lexternal=1)
+    o = O.Oracle(n, n, n, xm, xm, xm, p, O.MODE_STATICS)
+    o.set("UEXT", omega * R2)
+    o.set("DEN", np.exp(-2 * a0 * R2))
+    n4 = 5.0
+    mu = o.statics_init(n4)
+    o.statics_log_start()
+    for it in range(1, nsteps + 1):
+        mu, _, _ = o.statics_iteration_log(dt, it, n4, mu)
+    pl = o.field("PSIL")
+    inner = (slice(6, -6),) * 3   # the fit away from the one-sided boundary rows
+    A = np.stack([-R2[inner].ravel(), np.ones(R2[inner].size)], 1)
+    alpha = np.linalg.lstsq(A, pl[inner].ravel(), rcond=None)[0][0]
+    return alpha, mu, omega
+
+
+def test_log_steppers_keep_trap_ground_state():
+    sigma = 1.5
+    astar = 1.0 / (2 * sigma ** 2)
+    alpha, mu, omega = _log_trap(astar, 2e-4, 20, sigma)
+    assert abs(alpha - astar) < 1e-12
+    assert abs(mu - 6 * H2O2M4 * astar) < 1e-11 * mu
+
+
+def test_log_steppers_follow_riccati_solution():
+    sigma = 1.5
+    astar = 1.0 / (2 * sigma ** 2)
+    a0, dt, nsteps = 0.6 * astar, 4e-4, 250
+    alpha, _, omega = _log_trap(a0, dt, nsteps, sigma)
+    exact = astar * np.tanh(2 * np.sqrt(omega * H2O2M4) * dt * nsteps + np.arctanh(a0 / astar))
+    assert abs(alpha - a0) > 0.3 * abs(astar - a0)          # it moved
+    assert abs(alpha - exact) < 1e-9 * exact, (alpha, exact)
