@@ -29,7 +29,8 @@ def _compile(src: str, out: str = OUT, extra: tuple = ()) -> str:
     cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-8000:]}")
+        errs = [l for l in r.stderr.splitlines() if "error" in l]
+        raise RuntimeError(f"hipcc failed for {src}:\n" + "\n".join(errs[:20]) + "\n" + r.stderr[-4000:])
     return obj
 
 

@@ -354,14 +354,17 @@ struct hedft_ctx {
   double *potx4, *upotx;                                     // real N (R storage)
   double* uimpk;                                             // real half spectrum (R storage)
   double2* specq[2];                                         // half spectra: |psix|^2, den x uimpk
+  // statics (He_statics with limp): psix is real N, psixold[0..1] hold psi1x,
+  // psi2x of evolox, specq[0] holds fdenx = F(psix^2) as the driver leaves it
 };
 
 namespace hedft {
 // field added to pot4 after the correlation term: uext (statics with the
 // external-field option) or potx4 (quantum impurity); null when neither
 inline double* pot4_ext(const hedft_ctx* c) {
+  if (c->quantum) return c->potx4;
   if (c->mode == HEDFT_MODE_STATICS) return c->p.lexternal ? c->uext : nullptr;
-  return c->quantum ? c->potx4 : nullptr;
+  return nullptr;
 }
 
 // fft.hip

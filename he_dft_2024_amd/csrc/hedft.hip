@@ -542,6 +542,44 @@ __global__ void __launch_bounds__(256) k_pcrl_corr(long long n, double* psil, do
   block_reduce_store<2>(red, partials);
 }
 
+// ---- statics impurity wave function (He_statics/evolox.f90) ----------------------
+__global__ void k_sq(long long n, const double* a, double* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = a[i] * a[i];
+}
+// hpsix = hpsix - epsx psix with the previous eigenvalue (evolox.f90:50-52)
+__global__ void k_evolox_g(long long n, double* hpsix, const double* psix, double epsx) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    hpsix[i] = hpsix[i] - epsx * psix[i];
+}
+// psixnw = psix - deltatx g + vdt1 (psix - psi1x) + vdt2 (psi1x - psi2x), history
+// shift, max(psimin, .) (evolox.f90:63-77); sum psixnw^2
+__global__ void __launch_bounds__(256) k_evolox_new(long long n, double* psix, double* psi1x, double* psi2x,
+                                                    const double* g, double deltatx, double vdt1, double vdt2,
+                                                    double psimin, double* partials) {
+  double red[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double p = psix[i], p1 = psi1x[i], p2 = psi2x[i];
+    double nw = p - deltatx * g[i] + vdt1 * (p - p1) + vdt2 * (p1 - p2);
+    psi2x[i] = p1;
+    psi1x[i] = p;
+    nw = (psimin > nw) ? psimin : nw;
+    psix[i] = nw;
+    red[0] += nw * nw;
+  }
+  block_reduce_store<1>(red, partials);
+}
+// psix = psixnw sqrt(1/(sum psixnw^2 dxyz)) (evolox.f90:78-83)
+__global__ void k_norm1(long long n, double* psix, const double* scal, double dxyz) {
+  const double cnorm = sqrt(1.0 / (scal[S_SUMSQ] * dxyz));
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    psix[i] = psix[i] * cnorm;
+}
+
 __global__ void k_log_abs(long long n, const double* psi, double* psil) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -1234,6 +1272,8 @@ static int poten_tddft(hedft_ctx* c, int want_force, int with_lap = 0) {
 }
 
 // He_statics/poten.f90:5-221: fden, dencg, denalf and grad(den) as left by the caller
+// (with an impurity wave function also fdenx: upotx = F^-1[fden vq], potx4 =
+// F^-1[fdenx vq] enter the pot4 assembly, poten.f90:94-107)
 static int poten_statics(hedft_ctx* c) {
   const double* kf = c->fvlj4;
   double2* s2 = c->spec[2];
@@ -1241,13 +1281,29 @@ static int poten_statics(hedft_ctx* c) {
   HEDFT_TRY(zconv(c, 1, &fd, 1, &kf, &s2, 0, nullptr));
   XOut xo = xo_store<double>(c->delj4);
   HEDFT_TRY(inv_yx(c, 1, &s2, &xo));
+  if (c->quantum) {
+    double2* in[2] = {c->fden, c->specq[0]};
+    const double* kq[2] = {c->uimpk, c->uimpk};
+    double2* so[2] = {c->spec[0], c->spec[1]};
+    HEDFT_TRY(zconv(c, 2, in, 1, kq, so, 0, nullptr));
+    XOut xq[2] = {xo_store<double>(c->upotx), xo_store<double>(c->potx4)};
+    HEDFT_TRY(inv_yx(c, 2, so, xq));
+  }
   HEDFT_TRY(stage_b<double>(c));
   if (c->p.core_otc) HEDFT_TRY(stage_c<double>(c));
   return HEDFT_OK;
 }
 
+// fdenx = F(psix^2) of the statics impurity (BCN4HeDFT.f90:788,1130)
+static int statics_fdenx(hedft_ctx* c) {
+  hipLaunchKernelGGL(k_sq, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, (const double*)c->psix, c->work2);
+  return fftfw(c, c->work2, c->specq[0]);
+}
+
 // driver refresh after a statics step (BCN4HeDFT.f90:1131-1149): derden, fden, dencg, denalf
+// (and fdenx with an impurity wave function, BCN4HeDFT.f90:1128-1130)
 static int statics_refresh(hedft_ctx* c) {
+  if (c->quantum) HEDFT_TRY(statics_fdenx(c));
   HEDFT_TRY(fork(c));
   {
     SideBranch sb(c);
@@ -1669,7 +1725,9 @@ static int energy_sums(hedft_ctx* c, double* s) {
   o.intx = fld<R>(c->intx);
   o.inty = fld<R>(c->inty);
   o.intz = fld<R>(c->intz);
-  o.uext = fld<R>(c->uext);
+  // sum(uext den), or sum(potx4 den) with a statics impurity wave function
+  // (He_statics/energy.f90:91)
+  o.uext = fld<R>((c->quantum && c->mode == HEDFT_MODE_STATICS) ? c->potx4 : c->uext);
   o.im = make_impdev(c);
   o.a1 = 0.5 * c->p.cp4;
   o.a2 = c->p.cpp4 / 3.0;
@@ -1712,6 +1770,27 @@ static int energy_sums_x(hedft_ctx* c, double* s) {
   hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
                      (int)pencil_geom<double2>(c->d, 2).nblocks, c->scal + S_ENERGYQ, 1.0);
   return read_scal(c, S_ENERGYQ, 2, s);
+}
+
+// statics impurity wave function: sum |grad psix|^2 (He_statics/energy.f90:81-90)
+struct OutSumSq {
+  static constexpr int NR = 1;
+  double* partials;
+  const double* acc;
+  __device__ __forceinline__ void store(size_t i, int, int, int, double v, double* red) const {
+    red[0] += acc[i] + v * v;
+  }
+};
+static int energy_sums_xs(hedft_ctx* c, double* s) {
+  const double* px = rz(c->psix);
+  launch_pencil_xy<double, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<double>{px},
+                                 OutSqXY<double>{nullptr, (double*)c->workc1});
+  OutSumSq o;
+  o.partials = gen_partials(c);
+  o.acc = (const double*)c->workc1;
+  launch_pencil<double, 2>(c->stream, c->d, c->coef[0][2], InPlain<double>{px}, o);
+  finalize<1>(c, gen_partials(c), pencil_geom<double>(c->d, 2).nblocks, S_ENERGYQ, 1.0);
+  return read_scal(c, S_ENERGYQ, 1, s);
 }
 
 // ---- droplet observables (Dynamics_TDDFT/DFT4He3d.f90:709-836, r_cm.f90) ----------------------
@@ -2066,6 +2145,17 @@ static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* comput
     case HEDFT_PSIXOLD1: case HEDFT_PSIXOLD2: case HEDFT_PSIXOLD3: case HEDFT_HPSIXOLD1:
     case HEDFT_HPSIXOLD2: case HEDFT_PCX: case HEDFT_QX:
       if (!c->quantum) return HEDFT_ESTATE;
+      if (!td) {  // statics: real psix, its potentials and kernel
+        switch (f) {
+          case HEDFT_PSIX: *ptr = c->psix; *bytes = rN; break;
+          case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
+          case HEDFT_UPOTX: *ptr = c->upotx; *bytes = rN; break;
+          case HEDFT_DENX: *ptr = c->workc1; *bytes = rN; *computed = 1; break;
+          case HEDFT_UIMPK: *ptr = c->uimpk; *bytes = kN; break;
+          default: return HEDFT_EINVAL;
+        }
+        break;
+      }
       switch (f) {
         case HEDFT_PSIX: *ptr = c->psix; *bytes = cN; break;
         case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
@@ -2135,6 +2225,8 @@ int hedft_download(hedft_ctx* c, int32_t field, void* host, size_t bytes) {
   if (comp) {
     double* w = (double*)c->workc1;
     if (field == HEDFT_UIMP) hipLaunchKernelGGL(k_eval_uimp, gr, bl, 0, c->stream, c->d, make_impdev(c), w);
+    else if (field == HEDFT_DENX && c->mode == HEDFT_MODE_STATICS)
+      hipLaunchKernelGGL(k_sq, gr, bl, 0, c->stream, c->d.n, rz(c->psix), w);
     else if (field == HEDFT_DENX)
       hipLaunchKernelGGL(k_abs2, gr, bl, 0, c->stream, c->d.n, (const double2*)c->psix, w);
     else if (field == HEDFT_FALFS && c->mixed)
@@ -2282,6 +2374,15 @@ int hedft_energy(hedft_ctx* c, hedft_energies* e) {
       e->eHeX = sx[1] * dv;
     }
     e->eimpu = e->ekinx + e->eHeX + e->eimpu_impu;
+  } else if (c->quantum) {
+    // He_statics/energy.f90:76-93,137-149: etot = etot4 + ekinx + eimpu
+    double sk;
+    HEDFT_TRY(energy_sums_xs(c, &sk));
+    e->ekinx = c->h2o2mimp * sk * dv;
+    e->eimpu = s[5] * dv;
+    e->etot = e->etot4 + e->ekinx + e->eimpu;
+    e->norm = s[7] * dv;
+    return HEDFT_OK;
   } else {
     e->eext = p.lexternal ? s[5] * dv : 0.0;
     e->eimpu = e->eext;
@@ -2474,11 +2575,33 @@ int hedft_update_aimp(hedft_ctx* c) {
 
 // potenimpini's uimp_k(:,:,:,1) and the mass of one quantum impurity
 // (Dynamics_quantum_impurities/potenimpini.f90:19-29, steprk.f90:86)
+// He_statics with limp: real psix, psi1x, psi2x, the kernel vq (BCN4HeDFT.f90:738-791)
+// and h2o2mx = h2o2m4 (4.002602/mimpur) (BCN4HeDFT.f90:537)
+static int set_quantum_statics(hedft_ctx* c, const double* vq, size_t bytes, double m_imp_u) {
+  if (c->p.lexternal) return HEDFT_EINVAL;  // one external field in the pot4 assembly
+  const size_t rN = sizeof(double) * c->d.n, kN = sizeof(double) * c->d.nh, hN = sizeof(double2) * c->d.nh;
+  int rc = HEDFT_OK;
+  void** reals[] = {&c->psix, &c->psixold[0], &c->psixold[1], (void**)&c->potx4, (void**)&c->upotx};
+  for (void** f : reals)
+    if (!*f) rc = rc ? rc : alloc(c, f, rN);
+  if (!c->uimpk) rc = rc ? rc : alloc(c, (void**)&c->uimpk, kN);
+  if (!c->specq[0]) rc = rc ? rc : alloc(c, (void**)&c->specq[0], hN);
+  if (rc != HEDFT_OK) return rc;
+  HEDFT_CHECK(hipMemcpyAsync(c->uimpk, vq, bytes, hipMemcpyHostToDevice, c->stream));
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));
+  c->m_imp_u = m_imp_u;
+  c->h2o2mimp = c->p.h2o2m4 * (4.002602 / m_imp_u);
+  c->mu4 = 0.0;
+  c->quantum = 1;
+  return done(c);
+}
+
 int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u, double mu4) {
-  if (!c || !uimp_k || c->mode != HEDFT_MODE_TDDFT || c->nimp > 0 || !(m_imp_u > 0.0)) return HEDFT_EINVAL;
+  if (!c || !uimp_k || c->nimp > 0 || !(m_imp_u > 0.0)) return HEDFT_EINVAL;
   if (bytes != sizeof(double) * (size_t)c->d.nh) return HEDFT_EINVAL;
   hipSetDevice(c->device);
   HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old kernel
+  if (c->mode == HEDFT_MODE_STATICS) return set_quantum_statics(c, uimp_k, bytes, m_imp_u);
   const size_t fw = c->mixed ? sizeof(float) : sizeof(double);
   const size_t cN = sizeof(double2) * c->d.n, rN = fw * c->d.n, kN = fw * c->d.nh, hN = 2 * fw * c->d.nh;
   int rc = HEDFT_OK;
@@ -2530,6 +2653,11 @@ int hedft_statics_init(hedft_ctx* c, double n4, double* mu4) {
   finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
   hipLaunchKernelGGL(k_statics_psi0, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->den, rz(c->psi), c->scal,
                      n4, c->dxyz);
+  if (c->quantum) {  // psi1x = psi2x = psix (BCN4HeDFT.f90:781-785)
+    const size_t b = sizeof(double) * c->d.n;
+    HEDFT_CHECK(hipMemcpyAsync(c->psixold[0], c->psix, b, hipMemcpyDeviceToDevice, c->stream));
+    HEDFT_CHECK(hipMemcpyAsync(c->psixold[1], c->psix, b, hipMemcpyDeviceToDevice, c->stream));
+  }
   HEDFT_TRY(statics_refresh(c));
   HEDFT_TRY(poten_statics(c));
   HEDFT_TRY(hpsi_statics(c, n4));
@@ -2562,6 +2690,49 @@ int hedft_evolo(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t i
   HEDFT_TRY(read_scal(c, S_MU, 1, &hmean));
   if (mu4err) *mu4err = std::fabs(1.0 - *mu4 / hmean);
   *mu4 = hmean;
+  return done(c);
+}
+
+// evolox (He_statics/evolox.f90:5-87)
+int hedft_evolox(hedft_ctx* c, double deltatx, double vdt1, double vdt2, int32_t ironx, double* epsx,
+                 double* epsxerr) {
+  if (!c || c->mode != HEDFT_MODE_STATICS || !c->quantum || ironx < 0 || !epsx) return HEDFT_EINVAL;
+  // hpsix = -(lap psix) h2o2mx + upotx psix, sum(psix hpsix) dxyz (evolox.f90:39-48)
+  HEDFT_TRY(lap_xy(c, c->psix, 0));
+  OutHpsiR o;
+  o.partials = gen_partials(c);
+  o.lapxy = rz(c->lap);
+  o.psi = rz(c->psix);
+  o.pot4 = c->upotx;
+  o.hpsi = c->hpsi_r;
+  o.h2o2m4 = c->h2o2mimp;
+  launch_pencil<double, 2>(c->stream, c->d, c->coef[1][2], InPlain<double>{rz(c->psix)}, o);
+  finalize<1>(c, gen_partials(c), pencil_geom<double>(c->d, 2).nblocks, S_MU, c->dxyz);
+  const double eold = *epsx;
+  hipLaunchKernelGGL(k_evolox_g, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->hpsi_r, rz(c->psix), eold);
+  double* g = c->hpsi_r;
+  for (int k = 0; k < ironx; ++k) {  // ping-pong through work1 / work2
+    double* dst = (g == c->work1) ? c->work2 : c->work1;
+    hipLaunchKernelGGL(k_ironing, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, g, dst);
+    g = dst;
+  }
+  hipLaunchKernelGGL(k_evolox_new, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psix),
+                     rz(c->psixold[0]), rz(c->psixold[1]), g, deltatx, vdt1, vdt2, c->p.psimin, gen_partials(c));
+  finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
+  hipLaunchKernelGGL(k_norm1, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psix), c->scal, c->dxyz);
+  double hmean;
+  HEDFT_TRY(read_scal(c, S_MU, 1, &hmean));
+  if (epsxerr) *epsxerr = std::fabs(1.0 - eold / hmean);
+  *epsx = hmean;
+  return done(c);
+}
+
+// the driver's part after the steps (BCN4HeDFT.f90:1125-1160): derden, fden
+// [, fdenx], dencg, denalf, poten
+int hedft_statics_refresh(hedft_ctx* c) {
+  if (!c || c->mode != HEDFT_MODE_STATICS) return HEDFT_EINVAL;
+  HEDFT_TRY(statics_refresh(c));
+  HEDFT_TRY(poten_statics(c));
   return done(c);
 }
 

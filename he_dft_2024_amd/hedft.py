@@ -150,6 +150,8 @@ def lib():
         L.hedft_steprkrl.argtypes = [vp, d, dp, dp, d]
         L.hedft_steppcrl.argtypes = [vp, d, dp, dp, d, dp]
         L.hedft_statics_log_start.argtypes = [vp]
+        L.hedft_evolox.argtypes = [vp, d, d, d, ip, dp, dp]
+        L.hedft_statics_refresh.argtypes = [vp]
         L.hedft_tddft_iteration.argtypes = [vp, d, ip, ip, dp, dp, dp]
         L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
         L.hedft_last_elapsed.argtypes = [vp, dp]
@@ -375,6 +377,14 @@ class Droplet:
         check(lib().hedft_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e)),
               "statics_iteration")
         return m.value, e.value
+
+    def evolox(self, dtx, epsx, vdt=(0.6, 0.1), ironx=1):
+        e, r = C.c_double(epsx), C.c_double(0)
+        check(lib().hedft_evolox(self._c, dtx, vdt[0], vdt[1], ironx, C.byref(e), C.byref(r)), "evolox")
+        return e.value, r.value
+
+    def statics_refresh(self):
+        check(lib().hedft_statics_refresh(self._c), "statics_refresh")
 
     def statics_log_start(self):
         check(lib().hedft_statics_log_start(self._c), "statics_log_start")

@@ -277,13 +277,18 @@ int hedft_forceimp_noalloc(const hedft_grid* g, int32_t device, void* stream, co
                            double* force, int32_t* lstop);
 
 /* ---- quantum impurity (Dynamics_quantum_impurities) -------------------- */
-/* One impurity described by a wave function psix (N_imp = 1, so no
+/* On a statics context this is the He_statics impurity (limp): uimp_k is vq
+ * (BCN4HeDFT.f90:746-760), psix is real (upload HEDFT_PSIX before
+ * hedft_statics_init), poten adds upotx = F^-1[fden vq] and potx4 =
+ * F^-1[fdenx vq] (He_statics/poten.f90:94-107), energy adds ekinx and eimpu
+ * (energy.f90:76-149), mu4 is unused and lexternal is rejected.
+ * On a TDDFT context: one impurity described by a wave function psix (N_imp = 1, so no
  * impurity-impurity kernel uimp_k_k).  uimp_k: the X-He interaction's Fourier
  * transform on the half spectrum (nx/2+1,ny,nz), real, as potenimpini.f90:19-29
  * fills uimp_k(:,:,:,1) (bytes = 8*(nx/2+1)*ny*nz); m_imp_u: impurity mass in
  * u (h2o2mimp = h2o2m4*4.002603/m_imp_u, steprk.f90:86); mu4: the chemical
  * potential subtracted from pot4 in the helium equation (DFT4He3d.f90:183-189,
- * 0 unless Lbulk).  TDDFT contexts without classical impurities only.
+ * 0 unless Lbulk).  Contexts without classical impurities only.
  * Allocates psix, its PC history, potx4 and upotx (zero) and switches the
  * context's TDDFT entry points to the quantum-impurity equations:
  *   hedft_potenimp:  potx4, upotx (potenimp.f90:1-43)
@@ -317,6 +322,22 @@ int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double 
  * (BCN4HeDFT.f90:861-862).  *mu4 in: previous mu, out: <psi|H|psi>/n4. */
 int hedft_evolo(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t iron, double n4,
                 double* mu4, double* mu4err);
+/* evolox(deltatx,epsx,epsxerr), He_statics/evolox.f90:5-87: one steepest-descent
+ * step of the impurity wave function psix of a statics context with a quantum
+ * impurity (hedft_set_quantum_impurity on a statics context: real psix, the
+ * kernel vq, h2o2mx = h2o2m4 4.002602/m_imp_u as BCN4HeDFT.f90:537) in the
+ * potential upotx left by poten: psix <- max(psimin, psix - deltatx g
+ * + vdt1 (psix - psi1x) + vdt2 (psi1x - psi2x)) normalised to 1, with g =
+ * (H - epsx) psix smoothed ironx times.  *epsx in: previous eigenvalue, out:
+ * <psix|H|psix>.  The driver follows it with hedft_statics_refresh. */
+int hedft_evolox(hedft_ctx* c, double deltatx, double vdt1, double vdt2, int32_t ironx, double* epsx,
+                 double* epsxerr);
+/* The driver's part after the steps (BCN4HeDFT.f90:1125-1160): derden,
+ * fden = F(den), fdenx = F(psix^2) with an impurity wave function, dencg,
+ * denalf, poten.  A statics driver with an impurity runs the helium step
+ * (hedft_evolo / hedft_steprkr / hedft_steppcr / hedft_steprkrl /
+ * hedft_steppcrl), then hedft_evolox, then this. */
+int hedft_statics_refresh(hedft_ctx* c);
 /* Statics iteration with the steepest-descent method (Lrkpc = .false.):
  * evolo, then derden, fftfw(den), dencg/denalf, poten (BCN4HeDFT.f90:1114-1160). */
 int hedft_statics_iteration_sd(hedft_ctx* c, double deltat, double vdt1, double vdt2, int32_t iron,

@@ -114,6 +114,8 @@ typedef struct {
   double h2o2mimp, mu4;
   double *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *hpsix; /* complex N */
   double *potx4, *upotx, *denx;                                  /* real N */
+  double *psi1x, *psi2x;   /* statics (He_statics, limp): psix one and two steps back */
+  cplx* fdenx;             /* statics: F(denx) as the driver leaves it */
   double *uimpk;                                                 /* real half spectrum */
   double errpsix;
 } octx;
@@ -565,7 +567,7 @@ void oracle_destroy(octx* c) {
                     (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion,
                     c->psix, c->psixold[0], c->psixold[1], c->psixold[2], c->hpsixold[0],
                     c->hpsixold[1], c->pcx, c->qx, c->hpsix, c->potx4, c->upotx, c->denx, c->uimpk,
-                    c->pairv, c->pairdv};
+                    c->pairv, c->pairdv, c->psi1x, c->psi2x, (double*)c->fdenx};
   for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); ++i) free(list[i]);
   free(c);
 }
@@ -605,7 +607,7 @@ double* oracle_field(octx* c, int id, size_t* ndoubles) {
     case F_WORK2: ptr = c->work2; break;
     case F_WORKC1: ptr = c->workc1; cnt = 2 * n; break;
     case F_WORKK1: ptr = (double*)c->workk1; cnt = 2 * nh; break;
-    case F_PSIX: ptr = c->psix; cnt = 2 * n; break;
+    case F_PSIX: ptr = c->psix; cnt = (c->mode == MODE_STATICS) ? n : 2 * n; break;
     case F_POTX4: ptr = c->potx4; break;
     case F_UPOTX: ptr = c->upotx; break;
     case F_DENX: ptr = c->denx; break;
@@ -716,6 +718,15 @@ static void poten_tail(octx* c) {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)n; ++i) c->pot4[i] += c->ualphas[i];
   }
+  if (c->mode == MODE_STATICS && c->quantum) {
+    /* He_statics/poten.f90:94-107: upotx = F^-1[fden vq], potx4 = F^-1[fdenx vq] */
+    kmul(c, c->fden, c->uimpk, c->wk1);
+    kmul(c, c->fdenx, c->uimpk, c->wk2);
+    fftbk(c, c->wk1, c->upotx);
+    fftbk(c, c->wk2, c->potx4);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) c->pot4[i] += c->potx4[i];
+  }
   if (p->lsolid) {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)n; ++i) c->pot4[i] += c->penalty[i];
@@ -736,9 +747,11 @@ void oracle_poten(octx* c) {
 
 /* the statics driver's refresh after a step (BCN4HeDFT.f90:1131-1149):
  * derden, fden = F(den), dencg, denalf */
+static void statics_fdenx(octx* c);
 void oracle_statics_refresh(octx* c) {
   oracle_derden(c);
   fftfw(c, c->den, c->fden);
+  if (c->quantum) statics_fdenx(c); /* BCN4HeDFT.f90:1129-1130 */
   kmul(c, c->fden, c->wcgk, c->wk1);
   fftbk(c, c->wk1, c->dencg);
   if (c->p.core_otc) {
@@ -1192,6 +1205,20 @@ void oracle_update_aimp(octx* c) {
 /* uimp_k(:,:,:,1) as potenimpini.f90:19-29 leaves it; h2o2mimp as steprk.f90:86 */
 void oracle_set_quantum_impurity(octx* c, const double* uimpk, double m_imp_u, double mu4) {
   size_t n = c->n;
+  if (c->mode == MODE_STATICS) {
+    /* He_statics with limp: real psix, vq (BCN4HeDFT.f90:738-791), h2o2mx
+     * (BCN4HeDFT.f90:537) */
+    if (!c->quantum) {
+      c->psix = dalloc(n); c->psi1x = dalloc(n); c->psi2x = dalloc(n); c->hpsix = dalloc(n);
+      c->potx4 = dalloc(n); c->upotx = dalloc(n); c->denx = dalloc(n); c->uimpk = dalloc(c->nh);
+      c->fdenx = (cplx*)dalloc(2 * c->nh);
+    }
+    memcpy(c->uimpk, uimpk, sizeof(double) * c->nh);
+    c->h2o2mimp = c->p.h2o2m4 * (4.002602 / m_imp_u);
+    c->mu4 = 0.0;
+    c->quantum = 1;
+    return;
+  }
   if (!c->quantum) {
     c->psix = dalloc(2 * n); c->pcx = dalloc(2 * n); c->qx = dalloc(2 * n); c->hpsix = dalloc(2 * n);
     for (int k = 0; k < 3; ++k) c->psixold[k] = dalloc(2 * n);
@@ -1608,6 +1635,29 @@ void oracle_energy(octx* c, double* out) {
         }
     eimpu = ekinx + eHeX + eimpu_impu;
     etot = etot4 + eimpu;
+  } else if (c->quantum) {
+    /* He_statics/energy.f90:76-93,137-149: ekinx = h2o2mx sum|grad psix|^2 dxyz,
+     * eimpu = sum(potx4 den) dxyz, etot = etot4 + ekinx + eimpu */
+    deriv(c, 1, 1, c->psix, c->sto4, 1);
+    deriv(c, 1, 2, c->psix, c->sto5, 1);
+    deriv(c, 1, 3, c->psix, c->sto6, 1);
+    double* qa = dalloc(2 * (size_t)nz);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < nz; ++iz) {
+      double sk = 0, se = 0;
+      for (size_t j = 0; j < plane; ++j) {
+        size_t i = iz * plane + j;
+        sk += c->sto4[i] * c->sto4[i] + c->sto5[i] * c->sto5[i] + c->sto6[i] * c->sto6[i];
+        se += c->potx4[i] * c->den[i];
+      }
+      qa[2 * iz] = sk; qa[2 * iz + 1] = se;
+    }
+    double sk = 0, se = 0;
+    for (int iz = 0; iz < nz; ++iz) { sk += qa[2 * iz]; se += qa[2 * iz + 1]; }
+    free(qa);
+    ekinx = c->h2o2mimp * sk * dv;
+    eimpu = se * dv;
+    etot = etot4 + ekinx + eimpu;
   } else {
     eext = p->lexternal ? s[5] * dv : 0.0;
     eimpu = eext;
@@ -1670,6 +1720,7 @@ void oracle_observables(octx* c, int n4, double mhe, double* out) {
 /* ======================================================================= */
 /* BCN4HeDFT.f90:724-874: normalise, psi = sqrt(den), fden, dencg, denalf,
  * derden, term_alfa, poten; mu4 = sum(psi*hpsi)*dxyz/n4 (evolo(0,...)) */
+static void statics_fdenx(octx* c);
 void oracle_statics_init(octx* c, double n4, double* mu4) {
   size_t n = c->n;
   double s = 0;
@@ -1700,11 +1751,24 @@ void oracle_statics_init(octx* c, double n4, double* mu4) {
   }
   oracle_derden(c);
   if (c->p.core_otc) oracle_term_alfa(c);
+  if (c->quantum) {
+    /* BCN4HeDFT.f90:781-791: psi1x = psi2x = psix, fdenx = F(denx) */
+    memcpy(c->psi1x, c->psix, n * sizeof(double));
+    memcpy(c->psi2x, c->psix, n * sizeof(double));
+    statics_fdenx(c);
+  }
   oracle_poten(c);
   *mu4 = hpsi_real(c, c->psi, c->hpsi) * c->dxyz / n4;
   /* psi1 = psi2 = psi for the steepest-descent method (BCN4HeDFT.f90:861-862) */
   memcpy(c->psi1s, c->psi, n * sizeof(double));
   memcpy(c->psi2s, c->psi, n * sizeof(double));
+}
+
+/* denx = psix^2, fdenx = F(denx) */
+static void statics_fdenx(octx* c) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) c->denx[i] = c->psix[i] * c->psix[i];
+  fftfw(c, c->denx, c->fdenx);
 }
 
 /* ironing (He_statics/ironing.f90:4-33): f = (sum of the six neighbours)/12 +
@@ -1752,6 +1816,53 @@ void oracle_evolo(octx* c, double deltat, double vdt1, double vdt2, int iron, do
     c->psi[i] = nw;
   }
   renorm_real(c, n4, 0);
+}
+
+/* evolox (He_statics/evolox.f90:5-87): steepest descent of the impurity wave
+ * function in the potential upotx, normalised to 1; epsx in: previous
+ * eigenvalue, out: <psix|H|psix> */
+void oracle_evolox(octx* c, double deltatx, double vdt1, double vdt2, int ironx, double* epsx, double* epsxerr) {
+  size_t n = c->n;
+  deriv(c, 2, 1, c->psix, c->sto1, 1);
+  deriv(c, 2, 2, c->psix, c->sto2, 1);
+  deriv(c, 2, 3, c->psix, c->sto3, 1);
+  size_t plane = (size_t)c->nx * c->ny;
+  double* acc = dalloc(c->nz);
+  double hx = c->h2o2mimp;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < c->nz; ++iz) {
+    double t = 0;
+    for (size_t j = 0; j < plane; ++j) {
+      size_t i = iz * plane + j;
+      double s4 = (c->sto1[i] + c->sto2[i] + c->sto3[i]) * hx;
+      c->hpsix[i] = -s4 + c->upotx[i] * c->psix[i];
+      t += c->psix[i] * c->hpsix[i];
+    }
+    acc[iz] = t;
+  }
+  double hmean = sum_finish(acc, c->nz) * c->dxyz;
+  free(acc);
+  double eold = *epsx;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) c->hpsix[i] = c->hpsix[i] - eold * c->psix[i];
+  *epsxerr = fabs(1.0 - eold / hmean);
+  *epsx = hmean;
+  for (int k = 0; k < ironx; ++k) oracle_ironing(c->nx, c->ny, c->nz, c->hpsix);
+  double pmin = c->p.psimin;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double p = c->psix[i], p1 = c->psi1x[i], p2 = c->psi2x[i];
+    double nw = p - deltatx * c->hpsix[i] + vdt1 * (p - p1) + vdt2 * (p1 - p2);
+    c->psi2x[i] = p1;
+    c->psi1x[i] = p;
+    c->psix[i] = (nw > pmin) ? nw : pmin;
+  }
+  double cnorm = sqrt(1.0 / (sumsq_plane(c, c->psix) * c->dxyz));
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    c->psix[i] = c->psix[i] * cnorm;
+    c->denx[i] = c->psix[i] * c->psix[i];
+  }
 }
 
 /* driver iteration with the steepest-descent method (BCN4HeDFT.f90:1114-1160) */

@@ -120,6 +120,7 @@ def lib():
         L.oracle_steprkrl.argtypes = [vp, C.c_double, dp, dp, C.c_double]
         L.oracle_steppcrl.argtypes = [vp, C.c_double, dp, dp, C.c_double, dp]
         L.oracle_statics_log_start.argtypes = [vp]
+        L.oracle_evolox.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, dp, dp]
         L.oracle_evolo.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
         L.oracle_statics_iteration_sd.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, C.c_double, dp, dp]
         L.oracle_ironing.argtypes = [ip, ip, ip, vp]
@@ -404,6 +405,16 @@ class Oracle:
         m, e, pc = C.c_double(mu4), C.c_double(0), C.c_double(0)
         lib().oracle_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e), C.byref(pc))
         return m.value, e.value, pc.value
+
+    def statics_refresh_poten(self):
+        """the driver's refresh and poten after the steps (BCN4HeDFT.f90:1125-1160)"""
+        lib().oracle_statics_refresh(self._c)
+        lib().oracle_poten(self._c)
+
+    def evolox(self, dtx, epsx, vdt=(0.6, 0.1), ironx=1):
+        e, r = C.c_double(epsx), C.c_double(0)
+        lib().oracle_evolox(self._c, dtx, vdt[0], vdt[1], ironx, C.byref(e), C.byref(r))
+        return e.value, r.value
 
     def statics_log_start(self):
         lib().oracle_statics_log_start(self._c)

@@ -127,10 +127,15 @@ def test_quantum_abi_errors(gpu):
     _say("quantum abi errors")
     import ctypes as C
     L = gpu.lib()
-    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_STATICS)
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, gpu.default_params(lexternal=1), mode=gpu.MODE_STATICS)
     k = np.zeros((N, N, N // 2 + 1))
     rc = L.hedft_set_quantum_impurity(d._c, k.ctypes.data_as(C.c_void_p), k.nbytes, 4.0, 0.0)
-    assert rc == -1  # statics context
+    assert rc == -1  # a statics impurity takes the place of the external field
+    d.close()
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_STATICS)
+    d.set_quantum_impurity(k, 4.0)
+    assert L.hedft_field_bytes(d._c, FIELD["PSIX"]) == 8 * N ** 3   # real psix in statics
+    assert L.hedft_field_bytes(d._c, FIELD["PSIXOLD1"]) == 0
     d.close()
     d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_TDDFT)
     assert L.hedft_field_bytes(d._c, FIELD["PSIX"]) == 0  # not allocated yet

@@ -217,3 +217,48 @@ def test_log_steppers_follow_riccati_solution():
     exact = astar * np.tanh(2 * np.sqrt(omega * H2O2M4) * dt * nsteps + np.arctanh(a0 / astar))
     assert abs(alpha - a0) > 0.3 * abs(astar - a0)          # it moved
     assert abs(alpha - exact) < 1e-9 * exact, (alpha, exact)
+
+
+# ---------------------------------------------------------------- evolox
+# Steepest descent of the statics impurity wave function (He_statics/evolox.f90)
+# in a separable trap upotx = omega r^2 converges to the lowest eigenvector of
+# the discrete H_x = -h2o2mx D2 + upotx, whose eigenvalue is three times the
+# lowest eigenvalue of the 1-D matrix -h2o2mx D2 + omega x^2 (D2 the 13-point
+# second-derivative operator of icon 13, built column by column from derivnD):
+# epsx and psix are compared with that eigenpair.  h2o2mx = h2o2m4
+# 4.002602/m (BCN4HeDFT.f90:537); a wrong mass factor, sign or step moves them.
+def test_evolox_converges_to_discrete_ground_state():
+    n, h = 32, 0.4
+    xm = 0.5 * n * h
+    mx = 2 * 4.002602
+    hx = H2O2M4 * 4.002602 / mx
+    sigma = 1.5
+    omega = hx / sigma ** 4
+    x = I.grid_axes(n, xm)
+    R2 = x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2
+    D2 = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros((1, 1, n))
+        e[0, 0, j] = 1.0
+        D2[:, j] = O.derivnd(2, e, h, 1, 13)[0, 0]
+    H1 = -hx * D2 + omega * np.diag(x ** 2)
+    assert np.array_equal(H1, H1.T)
+    w, V = np.linalg.eigh(H1)
+    o = O.Oracle(n, n, n, xm, xm, xm, O.default_params(core_otc=0), O.MODE_STATICS)
+    o.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, 20.0, seed=1))
+    o.set_quantum_impurity(np.zeros(o.hshape), mx)
+    p0 = np.exp(-R2 / (2 * 1.3 ** 2))             # a narrower packet than the ground state,
+    o.set("PSIX", p0 / np.sqrt((p0 ** 2).sum() * h ** 3))   # normalised as evolox keeps it
+    o.statics_init(20.0)
+    o.set("UPOTX", omega * R2)
+    eps, _ = o.evolox(0.0, 0.0)
+    assert abs(eps - 3 * w[0]) > 1e-3
+    for _ in range(300):
+        eps, err = o.evolox(0.5 * h * h / (4 * hx), eps)
+    assert abs(eps - 3 * w[0]) < 1e-12 * eps
+    v = V[:, 0]
+    v3 = v[None, None, :] * v[None, :, None] * v[:, None, None]
+    v3 = v3 * (np.sign(v3.sum()) / np.sqrt((v3 ** 2).sum() * h ** 3))
+    assert np.abs(o.field("PSIX") - v3).max() < 1e-9 * np.abs(v3).max()
+    # the discrete eigenvalue approaches the trap's 3 h2o2mx / sigma^2
+    assert abs(eps - 3 * hx / sigma ** 2) < 1e-6 * eps
