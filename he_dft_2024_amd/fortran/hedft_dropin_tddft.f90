@@ -25,14 +25,17 @@ subroutine hedft_dropin_start(Lambdah, txmean, tymean, tzmean, txsurf, tysurf, t
   use lenard4, only: h4, eps4, sigma4, core4
   use deriva, only: npd, icon
   use rho, only: den, psi
-  use classicimp, only: N_imp, rimp, vimp, aimp, m_imp
+  use classicimp, only: N_imp, rimp, vimp, aimp, m_imp, selec_gs_k_k, r_cutoff_gs_k_k, umax_gs_k_k, &
+                        drselec_gs_k_k, drr_cutoff_gs_k_k, drumax_gs_k_k
   use interpol, only: DelInter, potion, rmaxinterpol, npot
   implicit none
   real(8), intent(in) :: Lambdah, txmean, tymean, tzmean, txsurf, tysurf, tzsurf
   type(hedft_grid_t) :: g
   type(hedft_params_t) :: p
-  real(c_double), allocatable :: tab(:,:)
+  real(c_double), allocatable :: tab(:,:), vp(:,:), dvp(:,:)
   integer(c_size_t) :: n
+  integer :: k, m, i, ip, npair
+  real(8) :: r, delpair, Select_pot
 
   g%nx = nx; g%ny = ny; g%nz = nz; g%pad_ = 0
   g%xmax = xmax; g%ymax = ymax; g%zmax = zmax
@@ -55,6 +58,26 @@ subroutine hedft_dropin_start(Lambdah, txmean, tymean, tzmean, txsurf, tysurf, t
   tab = transpose(potion)
   call hedft_check(hedft_set_impurities(hctx, int(N_imp, c_int32_t), int(npot, c_int32_t), DelInter, &
                    rmaxinterpol, tab, m_imp), 'set_impurities')
+  ! impurity-impurity terms (potenimp.f90:40-58, energy.f90:168-177): the pair
+  ! potentials the driver selected, tabulated 20 times finer than potion
+  if (N_imp >= 2) then
+    npair = 20 * (npot - 1) + 1
+    delpair = rmaxinterpol / dble(npair - 1)
+    allocate(vp(npair, N_imp * (N_imp - 1) / 2), dvp(npair, N_imp * (N_imp - 1) / 2))
+    ip = 0
+    do k = 1, N_imp
+      do m = k + 1, N_imp
+        ip = ip + 1
+        do i = 1, npair
+          r = dble(i - 1) * delpair
+          vp(i, ip) = Select_pot(selec_gs_k_k(k, m), r, r_cutoff_gs_k_k(k, m), umax_gs_k_k(k, m))
+          dvp(i, ip) = Select_pot(drselec_gs_k_k(k, m), r, drr_cutoff_gs_k_k(k, m), drumax_gs_k_k(k, m))
+        end do
+      end do
+    end do
+    call hedft_check(hedft_set_impurity_pairs(hctx, int(npair, c_int32_t), delpair, vp, dvp), &
+                     'set_impurity_pairs')
+  end if
   call hedft_check(hedft_set_impurity_state(hctx, rimp, vimp, aimp), 'set_impurity_state')
 end subroutine hedft_dropin_start
 
@@ -73,13 +96,14 @@ subroutine hedft_dropin_pull()
   call hedft_check(hedft_download(hctx, HEDFT_POT4, pot4, 8_c_size_t * n), 'download pot4')
 end subroutine hedft_dropin_pull
 
-! the driver's aimp (formed on the host) goes to the device before each step
+! the driver's aimp (formed on the host) goes to the device before each step,
+! with rimp and vimp as forceimp last mirrored them (equal to the device's)
 subroutine hedft_dropin_push_aimp()
   use hedft_gpu
   use hedft_dropin_state
-  use classicimp, only: aimp
+  use classicimp, only: rimp, vimp, aimp
   implicit none
-  call hedft_check(hedft_set_impurity_state(hctx, aimp=aimp), 'set aimp')
+  call hedft_check(hedft_set_impurity_state(hctx, rimp, vimp, aimp), 'set aimp')
 end subroutine hedft_dropin_push_aimp
 
 SUBROUTINE STEPRK(deltat)
@@ -120,13 +144,13 @@ subroutine forceimp()
   use iso_c_binding
   use hedft_gpu
   use hedft_dropin_state
-  use classicimp, only: rimp, vimp, F
+  use classicimp, only: N_imp, rimp, vimp, F
   use interpol, only: lstopimp
   implicit none
   integer(c_int32_t) :: ls
+  real(c_double) :: adev(N_imp, 3)   ! the device's aimp: the driver forms its own
   call hedft_check(hedft_forceimp(hctx), 'forceimp')
-  call hedft_check(hedft_get_impurity_state(hctx, rimp=rimp, vimp=vimp, force=F, lstopimp=ls), &
-                   'get_impurity_state')
+  call hedft_check(hedft_get_impurity_state(hctx, rimp, vimp, adev, F, ls), 'get_impurity_state')
   lstopimp = (ls /= 0)
 end subroutine forceimp
 

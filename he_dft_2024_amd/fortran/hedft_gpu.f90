@@ -30,6 +30,8 @@ module hedft_gpu
        HEDFT_DENX = 35, HEDFT_UIMPK = 36, HEDFT_PSIXOLD1 = 37, HEDFT_PSIXOLD2 = 38, &
        HEDFT_PSIXOLD3 = 39, HEDFT_HPSIXOLD1 = 40, HEDFT_HPSIXOLD2 = 41, HEDFT_PCX = 42, &
        HEDFT_QX = 43
+  ! statics log-psi steppers: psil = log|psi| (rkpc: Psil)
+  integer(c_int32_t), parameter :: HEDFT_PSIL = 44
 
   type, bind(C) :: hedft_grid_t
     integer(c_int32_t) :: nx, ny, nz, pad_
@@ -194,14 +196,16 @@ module hedft_gpu
     integer(c_int) function hedft_set_impurity_state(ctx, rimp, vimp, aimp) bind(C, name="hedft_set_impurity_state")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(in), optional :: rimp(*), vimp(*), aimp(*)
+      ! every argument is required: an absent optional argument of a bind(C)
+      ! interface is not passed as a null pointer by every Fortran compiler
+      real(c_double), intent(in) :: rimp(*), vimp(*), aimp(*)
     end function
     integer(c_int) function hedft_get_impurity_state(ctx, rimp, vimp, aimp, force, lstopimp) &
          bind(C, name="hedft_get_impurity_state")
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out), optional :: rimp(*), vimp(*), aimp(*), force(*)
-      integer(c_int32_t), intent(out), optional :: lstopimp
+      real(c_double), intent(out) :: rimp(*), vimp(*), aimp(*), force(*)
+      integer(c_int32_t), intent(out) :: lstopimp
     end function
     integer(c_int) function hedft_potenimp(ctx) bind(C, name="hedft_potenimp")
       import :: c_int, c_ptr
@@ -280,6 +284,44 @@ module hedft_gpu
       integer(c_int32_t), value :: iron
       real(c_double), intent(inout) :: mu4
       real(c_double), intent(out) :: mu4err
+    end function
+    integer(c_int) function hedft_statics_log_start(ctx) bind(C, name="hedft_statics_log_start")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_steprkrl(ctx, deltat, mu4, mu4err, n4) bind(C, name="hedft_steprkrl")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err
+    end function
+    integer(c_int) function hedft_steppcrl(ctx, deltat, mu4, mu4err, n4, errpc) bind(C, name="hedft_steppcrl")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err, errpc
+    end function
+    integer(c_int) function hedft_evolox(ctx, deltatx, vdt1, vdt2, ironx, epsx, epsxerr) bind(C, name="hedft_evolox")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltatx, vdt1, vdt2
+      integer(c_int32_t), value :: ironx
+      real(c_double), intent(inout) :: epsx
+      real(c_double), intent(out) :: epsxerr
+    end function
+    integer(c_int) function hedft_statics_refresh(ctx) bind(C, name="hedft_statics_refresh")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function hedft_set_impurity_pairs(ctx, npair, delpair, vpair, dvpair) &
+         bind(C, name="hedft_set_impurity_pairs")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: npair
+      real(c_double), value :: delpair
+      real(c_double), intent(in) :: vpair(*), dvpair(*)
     end function
     integer(c_int) function hedft_autotune(ctx, deltat, trial_steps, best) bind(C, name="hedft_autotune")
       import :: c_int, c_int32_t, c_ptr, c_double, hedft_tuning_t

@@ -19,7 +19,8 @@ QEXE = os.path.join(FDIR, "_build", "hedft_fortran_qcheck")
 
 def test_fortran_sources_present():
     for f in ("hedft_gpu.f90", "hedft_fortran_check.f90", "hedft_fortran_qcheck.f90", "hedft_dropin_tddft.f90",
-              "hedft_dropin_quantum.f90", "Makefile"):
+              "hedft_dropin_quantum.f90", "hedft_dropin_statics.f90", "hedft_dropin_tddft_run.f90",
+              "hedft_dropin_statics_run.f90", "Makefile"):
         assert os.path.exists(os.path.join(FDIR, f)), f
 
 
@@ -107,3 +108,136 @@ def test_fortran_quantum_loop_matches_python_binding(gpu, tmp_path):
     assert scal[2] == en["etot"] and scal[3] == en["ekinx"] and scal[4] == en["eHeX"]
     assert scal[5] == nrm and np.array_equal(scal[6:9], rc)
     assert np.array_equal(psix_f, px)
+
+
+DEXE = os.path.join(FDIR, "_build", "hedft_dropin_tddft_run")
+SEXE = os.path.join(FDIR, "_build", "hedft_dropin_statics_run")
+
+
+@pytest.mark.gpu
+def test_dropin_tddft_reference_names_match_python_sequence(gpu, tmp_path):
+    """A driver-shaped Fortran program linked with hedft_dropin_tddft.o and the
+    reference's own Dynamics_TDDFT/modules.f90 calls STEPRK / STEPPC / potenimp
+    / poten / forceimp by their reference names in the order of
+    DFT4He3d.f90:671-696 (aimp = F/m on the host).  The same entry points
+    called from Python in that order give bit-identical results, and the
+    graph-ready driver entry (hedft_tddft_iteration, force fused into the
+    gradient pass) agrees to rounding."""
+    if not os.path.exists(DEXE):
+        pytest.skip("drop-in driver program not built (reference tree or amdflang absent at build time)")
+    n, xmax, n4, niter = 32, 6.4, 60.0, 6
+    dt = 5e-4 / I.PSTOK
+    print("[case] fortran drop-in by reference names, 32^3 3 RK + 3 PC", flush=True)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=4)
+    potion, delinter, rmax = I.impurity_table(n, xmax)
+    mass = 108.0 * I.MP_U
+    r0 = np.array([0.0, 0.5, -4.0])
+    v0 = np.array([0.0, 0.0, 3.0]) * I.PSTOK
+    fin, fout = tmp_path / "din.bin", tmp_path / "dout.bin"
+    with open(fin, "wb") as f:
+        np.array([n, niter], dtype=np.int32).tofile(f)
+        np.array([xmax, dt, delinter, rmax, mass], dtype=np.float64).tofile(f)
+        np.array([potion.size], dtype=np.int32).tofile(f)
+        np.asarray(potion, dtype=np.float64).tofile(f)
+        r0.tofile(f)
+        v0.tofile(f)
+        np.ascontiguousarray(den, dtype=np.float64).tofile(f)
+    r = subprocess.run([DEXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    err_f, etot_f, ekin_f, ehex_f = raw[:4]
+    rimp_f, vimp_f, f_f = raw[4:7], raw[7:10], raw[10:13]
+    psi_f = raw[13:].view(np.complex128).reshape(n, n, n)
+
+    # the same calls from Python
+    psi0 = np.sqrt(den).astype(np.complex128)
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+    d.set("DEN", np.abs(psi0) ** 2)
+    d.set("PSI", psi0)
+    d.set_impurities(potion[None, :], delinter, rmax, [mass])
+    d.set_impurity_state(r0[None, :], v0[None, :], np.zeros((1, 3)))
+
+    def after_step():
+        d.potenimp()
+        d.poten()
+        d.forceimp()
+        return d.impurity_state()["force"] / mass
+    aimp = after_step()
+    err = 0.0
+    for it in range(1, niter + 1):
+        d.set_impurity_state(aimp=aimp)
+        if it <= 3:
+            d.steprk(dt)
+        else:
+            err = d.steppc(dt)[0]
+        aimp = after_step()
+    en = d.energy()
+    st = d.impurity_state()
+    assert err_f == err
+    assert etot_f == en["etot"] and ekin_f == en["ekin4"] and ehex_f == en["eHeX"]
+    assert np.array_equal(rimp_f, st["rimp"][0]) and np.array_equal(vimp_f, st["vimp"][0])
+    assert np.array_equal(f_f, st["force"][0])
+    assert np.array_equal(psi_f, d.field("PSI"))
+    d.close()
+    # the driver entry point with the fused force reduction
+    e = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+    e.set("DEN", den)
+    e.set_impurities(potion[None, :], delinter, rmax, [mass])
+    e.set_impurity_state(r0[None, :], v0[None, :], np.zeros((1, 3)))
+    e.tddft_init()
+    for it in range(1, niter + 1):
+        e.tddft_iteration(dt, it, 3)
+    psi_e = e.field("PSI")
+    assert np.abs(psi_e - psi_f).max() < 1e-12 * np.abs(psi_f).max()
+    assert abs(e.energy()["etot"] - etot_f) < 1e-12 * abs(etot_f)
+    e.close()
+
+
+@pytest.mark.gpu
+def test_dropin_statics_reference_names_match_python_sequence(gpu, tmp_path):
+    """The He_statics drop-in (hedft_dropin_statics.o + the reference's own
+    He_statics/modules.f90): STEPRKR / STEPPCR by name with the driver's refresh
+    (BCN4HeDFT.f90:1113-1160), bit-identical to the same entry points from
+    Python and to hedft_statics_iteration."""
+    if not os.path.exists(SEXE):
+        pytest.skip("drop-in driver program not built (reference tree or amdflang absent at build time)")
+    n, xmax, n4, niter = 32, 6.4, 50, 10
+    dt = 0.01 * 0.4 ** 2 / (4 * 6.05969638298)
+    print("[case] fortran statics drop-in by reference names, 32^3 4 RK + 6 PC", flush=True)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, float(n4), seed=3)
+    fin, fout = tmp_path / "sin.bin", tmp_path / "sout.bin"
+    with open(fin, "wb") as f:
+        np.array([n, niter, n4], dtype=np.int32).tofile(f)
+        np.array([xmax, dt], dtype=np.float64).tofile(f)
+        np.ascontiguousarray(den, dtype=np.float64).tofile(f)
+    r = subprocess.run([SEXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    mus_f = raw[:niter]
+    errpc_f, etot_f, ekin_f = raw[niter:niter + 3]
+    psi_f = raw[niter + 3:].reshape(n, n, n)
+
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_STATICS)
+    d.set("DEN", den)
+    mu = d.statics_init(float(n4))
+    mus, epc = [], 0.0
+    for it in range(1, niter + 1):
+        if it <= 4:
+            mu, _ = d.steprkr(dt, mu, float(n4))
+        else:
+            mu, _, epc = d.steppcr(dt, mu, float(n4))
+        d.statics_refresh()
+        mus.append(mu)
+    en = d.energy()
+    assert np.array_equal(mus_f, np.array(mus))
+    assert errpc_f == epc and etot_f == en["etot"] and ekin_f == en["ekin4"]
+    assert np.array_equal(psi_f, d.field("PSI"))
+    d.close()
+    s = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_STATICS)
+    s.set("DEN", den)
+    mu = s.statics_init(float(n4))
+    for it in range(1, niter + 1):
+        mu, _ = s.statics_iteration(dt, it, float(n4), mu)
+    assert mu == mus_f[-1]
+    assert np.array_equal(psi_f, s.field("PSI"))
+    s.close()
