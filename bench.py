@@ -49,7 +49,13 @@ def parse():
                     help="PC iterations of the 256^3 N=5000 mixed-precision droplet (0 skips it)")
     ap.add_argument("--statics-steps", type=int, default=200,
                     help="imaginary-time iterations of the 64^3 N=300 relaxation (0 skips it)")
-    ap.add_argument("--serve-steps", type=int, default=120, help="PC iterations per serving job")
+    ap.add_argument("--serve-steps", type=int, default=1000, help="PC iterations per serving job")
+    ap.add_argument("--ragged-steps", type=int, default=24,
+                    help="PC iterations per droplet of the ragged (128/192) batched ensemble (0 skips it)")
+    ap.add_argument("--accuracy-steps", type=int, default=1002,
+                    help="PC iterations after which the mixed path is compared with the FP64 path")
+    ap.add_argument("--cpu-statics-steps", type=int, default=40,
+                    help="statics iterations of the 64^3 CPU sample (PC iterations 11.. are timed)")
     ap.add_argument("--quantum-steps", type=int, default=48,
                     help="PC iterations in the 256^3 quantum-impurity leg (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -172,23 +178,36 @@ def main():
         if a.warmup:
             dm.run_pc(dt, a.warmup, use_graph)
         msm = dm.run_pc(dt, a.mixed_steps, use_graph)
-        left = a.steps + a.probe_steps - a.mixed_steps
-        if left > 0:
-            dm.run_pc(dt, left, use_graph)
+        # accuracy: both paths advanced to the same fixed number of PC
+        # iterations (at least --accuracy-steps), then compared per energy term
+        # and by the density's relative L2 error
+        done_fp = a.warmup + a.steps + a.probe_steps
+        done_mx = a.warmup + a.mixed_steps
+        horizon = max(a.accuracy_steps, done_fp, done_mx)
+        if horizon > done_fp:
+            drop.run_pc(dt, horizon - done_fp, use_graph)
+        if horizon > done_mx:
+            dm.run_pc(dt, horizon - done_mx, use_graph)
+        e = drop.energy()
         em = dm.energy()
+        den_fp, den_mx = drop.field("DEN"), dm.field("DEN")
         dm.close()
-        compared = left >= 0
+        terms = {k: abs(em[k] - e[k]) / abs(e[k]) for k in ("ekin4", "elj4", "ecor4", "ealphas", "eHeX", "ekinx",
+                                                            "etot4", "etot", "norm") if abs(e[k]) > 0}
         mixed = {"value": round(a.mixed_steps / (msm / 1000.0), 3), "unit": "PC steps/s",
                  "ms_per_step": round(msm / a.mixed_steps, 4), "steps": a.mixed_steps, "dtype": "f32 field / f64 psi",
                  "speedup_vs_fp64": round((a.mixed_steps / msm) / (a.steps / ms), 3),
-                 "energy_per_atom_rel_err_vs_fp64": (abs(em["etot4"] - e["etot4"]) / abs(e["etot4"])
-                                                     if compared else None),
-                 "norm_rel_err_vs_fp64": (abs(em["norm"] - e["norm"]) / abs(e["norm"]) if compared else None),
-                 "pc_steps_compared": a.warmup + a.steps + a.probe_steps if compared else None}
+                 "pc_steps_compared": horizon,
+                 "energy_per_atom_rel_err_vs_fp64": abs(em["etot"] - e["etot"]) / abs(e["etot"]),
+                 "term_rel_err_vs_fp64": {k: float("%.3e" % v) for k, v in terms.items()},
+                 "density_l2_rel_err_vs_fp64": float(np.linalg.norm(den_mx - den_fp) / np.linalg.norm(den_fp))}
 
     batched = None
     if a.batch > 0:
         batched = batched_leg(H, I, a, dt, steps_per_s)
+    ragged = None
+    if a.batch > 0 and a.ragged_steps > 0:
+        ragged = ragged_leg(H, I, a, dt)
 
     serving = None
     if a.serve_jobs > 0:
@@ -209,6 +228,10 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, xmax, a.n4, dt, a.cpu_steps)
+        if a.cpu_statics_steps > 10:
+            cpu["statics_64"] = cpu_statics_baseline(a.cpu_statics_steps)
+            if statics is not None:
+                statics["speedup_vs_cpu"] = round(statics["value"] / cpu["statics_64"]["value"], 1)
 
     out = {
         "metric": "TDDFT steps/s at 128^3 (FP64 plain, one droplet)",
@@ -234,6 +257,7 @@ def main():
         "energy_per_atom_K": e["etot4"] / a.n4,
         "mixed": mixed,
         "mixed_batched": batched,
+        "mixed_batched_ragged": ragged,
         "serving": serving,
         "mixed_256": large,
         "statics_64": statics,
@@ -284,6 +308,35 @@ def batched_leg(H, I, a, dt, plain_steps_per_s):
                                                  "lap_late")}}
 
 
+def ragged_leg(H, I, a, dt):
+    """The ensemble with uneven grids: the same seeded draws, each droplet on the
+    smallest of {128, 192} points that holds 2R + 16 A (19 at 128^3, 29 at 192^3,
+    none truncated), mixed precision, advanced together by hedft_batch_run_pc.
+    Reports droplet PC steps/s and the work-weighted rate in 128^3-droplet
+    steps/s (each droplet weighted by its grid volume / 128^3)."""
+    ens = I.ensemble(count=a.batch, seed=3, grids=(128, 192))
+    drops = []
+    for e in ens:
+        g, xm = e["grid"], e["xmax"]
+        d = H.Droplet(g, g, g, xm, xm, xm, H.default_params(core_otc=1, precision=H.PREC_MIXED))
+        I.tddft_setup(d, g, xm, e["n4"], seed=e["seed"], r0=(e["b"], 0.0, -20.0), v_aps=(0.0, 0.0, e["v"]))
+        d.tddft_init()
+        for it in range(1, 5):
+            d.tddft_iteration(dt, it, 4)
+        drops.append(d)
+    H.batch_run_pc(drops, dt, 6, chunk=6)
+    ms = H.batch_run_pc(drops, dt, a.ragged_steps, chunk=6)
+    agg = len(drops) * a.ragged_steps / (ms / 1000.0)
+    w = sum((e["grid"] / 128.0) ** 3 for e in ens) / len(ens)
+    bad = sum(1 for d in drops if not np.isfinite(d.energy()["etot"]))
+    for d in drops:
+        d.close()
+    return {"value": round(agg, 2), "unit": "PC steps/s (aggregate, droplets of 128^3 and 192^3)",
+            "n_droplets": len(drops), "grids": {str(g): sum(1 for e in ens if e["grid"] == g) for g in (128, 192)},
+            "steps_each": a.ragged_steps, "ms": round(ms, 3),
+            "work_weighted_128_equiv_steps_per_s": round(agg * w, 2), "clamped_droplets": 0, "nonfinite": bad}
+
+
 def serving_leg(H, I, a, dt):
     """Continuous-batching loop: Poisson arrivals at 20 jobs/s (seed 4) of
     droplets drawn like the ensemble (N 200-2000, 1-8 A/ps, impact parameter
@@ -304,12 +357,20 @@ def serving_leg(H, I, a, dt):
             for t, e, q in zip(arr, ens, quantum)]
     res, st = H.serve(jobs, hbm_budget=64e9, chunk=12)
     ok = sum(1 for r in res if r["status"] == 0 and np.isfinite(r["etot"]))
+    # time from admission to completion (context creation, RK start and the PC
+    # iterations), per 1000 PC iterations, beside the arrival-to-completion latency
+    svc = np.array([(r["done_s"] - r["admit_s"]) * 1000.0 / a.serve_steps * 1000.0 for r in res
+                    if r["status"] == 0])
+    first, last = min(r["admit_s"] for r in res), max(r["done_s"] for r in res)
     return {"jobs": a.serve_jobs, "quantum_jobs": int(quantum.sum()), "steps_each": a.serve_steps,
             "arrival_rate_per_s": 20.0, "completed_ok": ok,
             "p50_ms_per_1000_steps": round(st["p50_ms_per_1000"], 2),
             "p99_ms_per_1000_steps": round(st["p99_ms_per_1000"], 2),
-            "aggregate_steps_per_s": round(st["steps_per_s"], 2), "max_concurrent": st["max_concurrent"],
-            "wall_s": round(st["wall_s"], 3)}
+            "p50_service_ms_per_1000_steps": round(float(np.percentile(svc, 50)), 2),
+            "p99_service_ms_per_1000_steps": round(float(np.percentile(svc, 99)), 2),
+            "aggregate_steps_per_s": round(st["steps_per_s"], 2),
+            "aggregate_steps_per_s_first_admit_to_last_done": round(ok * a.serve_steps / (last - first), 2),
+            "max_concurrent": st["max_concurrent"], "wall_s": round(st["wall_s"], 3)}
 
 
 def large_leg(H, I, a, dt):
@@ -317,21 +378,30 @@ def large_leg(H, I, a, dt):
     mixed precision; PC steps/s and its algorithmic bandwidth."""
     n = 256
     xm = 0.5 * n * 0.4
-    d = H.Droplet(n, n, n, xm, xm, xm, H.default_params(core_otc=1, precision=H.PREC_MIXED))
-    I.tddft_setup(d, n, xm, 5000.0, seed=7, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 5.0))
-    d.tddft_init()
-    for it in range(1, 5):
-        d.tddft_iteration(dt, it, 4)
-    d.run_pc(dt, 6, True)
-    ms = d.run_pc(dt, a.large_steps, True)
-    e = d.energy()
-    d.close()
+    res = {}
+    for prec in (H.PREC_MIXED, H.PREC_FP64):
+        d = H.Droplet(n, n, n, xm, xm, xm, H.default_params(core_otc=1, precision=prec))
+        I.tddft_setup(d, n, xm, 5000.0, seed=7, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 5.0))
+        d.tddft_init()
+        for it in range(1, 5):
+            d.tddft_iteration(dt, it, 4)
+        d.run_pc(dt, 6, True)
+        ms = d.run_pc(dt, a.large_steps, True)
+        res[prec] = (ms, d.energy(), d.field("DEN"))
+        d.close()
+    ms, e, den = res[H.PREC_MIXED]
+    ms64, e64, den64 = res[H.PREC_FP64]
     sps = a.large_steps / (ms / 1000.0)
     N, Nh = n ** 3, (n // 2 + 1) * n * n
     b = 544 * N + 736 * Nh  # mixed: FP32 field, FP64 psi (same counting convention)
+    terms = {k: float("%.3e" % (abs(e[k] - e64[k]) / abs(e64[k])))
+             for k in ("ekin4", "elj4", "ecor4", "ealphas", "eHeX", "ekinx", "etot", "norm") if abs(e64[k]) > 0}
     return {"value": round(sps, 3), "unit": "PC steps/s", "grid": n, "n4": 5000.0, "steps": a.large_steps,
             "algorithmic_GBps": round(b * sps / 1e9, 1), "energy_per_atom_K": e["etot4"] / 5000.0,
-            "norm": e["norm"]}
+            "norm": e["norm"], "fp64_steps_per_s": round(a.large_steps / (ms64 / 1000.0), 3),
+            "term_rel_err_vs_fp64": terms,
+            "density_l2_rel_err_vs_fp64": float(np.linalg.norm(den - den64) / np.linalg.norm(den64)),
+            "pc_steps_compared": 6 + a.large_steps}
 
 
 def statics_leg(H, I, a):
@@ -401,6 +471,25 @@ def cpu_baseline(n, xmax, n4, dt, nsteps):
     return {"value": round(nsteps / el, 5), "unit": "PC steps/s", "cores": O.num_threads(),
             "kind": "port", "sample": f"{nsteps} PC iterations of the same {n}^3 workload after a 4-step "
                                       f"RK start (oracle, OpenMP, own mixed-radix FFT)"}
+
+
+def cpu_statics_baseline(nsteps):
+    """The FP64 oracle on the 64^3 N=300 imaginary-time relaxation (4 RK-Gill +
+    Hamming PC, dtau = 1e-4), timed over PC iterations 11..nsteps on the host."""
+    from oracle import oracle as O
+    from he_dft_2024_amd import inputs as I
+    n, xm, n4 = 64, 12.8, 300.0
+    o = O.Oracle(n, n, n, xm, xm, xm, O.default_params(core_otc=1), O.MODE_STATICS)
+    o.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=0))
+    mu = o.statics_init(n4)
+    for it in range(1, 11):
+        mu = o.statics_iteration(1e-4, it, n4, mu)[0]
+    t0 = time.perf_counter()
+    for it in range(11, nsteps + 1):
+        mu = o.statics_iteration(1e-4, it, n4, mu)[0]
+    el = time.perf_counter() - t0
+    return {"value": round((nsteps - 10) / el, 3), "unit": "iterations/s", "cores": O.num_threads(),
+            "kind": "port", "sample": f"statics iterations 11..{nsteps} of the 64^3 N=300 relaxation (oracle)"}
 
 
 if __name__ == "__main__":

@@ -124,12 +124,15 @@ def quantum_setup(drop, n, xmax, n4, uimp_k, seed=0, m_imp_u=4.002603, mu4=0.0, 
     return den
 
 
-def ensemble(count=48, seed=3, n_lo=200.0, n_hi=2000.0, v_lo=1.0, v_hi=8.0, grid=128, h=0.4, z0=-20.0):
+def ensemble(count=48, seed=3, n_lo=200.0, n_hi=2000.0, v_lo=1.0, v_hi=8.0, grid=128, h=0.4, z0=-20.0,
+             grids=None):
     """Seeded droplet ensemble: N log-uniform in [n_lo, n_hi], impurity speed
     uniform in [v_lo, v_hi] A/ps toward the centre along +z from z0, impact
     parameter b uniform in [0, R] along x, R = 2.22 N^(1/3).  The grid edge is
     clamped to `grid` (2R+16 A exceeds 128 h for N > ~500; those droplets are
-    truncated by the box).  Returns one dict per droplet."""
+    truncated by the box), or, with `grids` (e.g. (128, 192)), the smallest
+    edge that holds 2R+16 A (the largest one when none does: clamped).
+    Returns one dict per droplet."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -138,7 +141,8 @@ def ensemble(count=48, seed=3, n_lo=200.0, n_hi=2000.0, v_lo=1.0, v_hi=8.0, grid
         R = 2.22 * n4 ** (1.0 / 3.0)
         b = float(rng.uniform(0.0, R))
         need = int(np.ceil((2 * R + 16.0) / h))
-        out.append({"n4": n4, "v": v, "b": b, "grid": grid, "xmax": 0.5 * grid * h,
-                    "clamped": need > grid, "seed": int(rng.integers(0, 2**31 - 1))})
+        g = grid if grids is None else next((e for e in sorted(grids) if e >= need), max(grids))
+        out.append({"n4": n4, "v": v, "b": b, "grid": g, "xmax": 0.5 * g * h,
+                    "clamped": need > g, "seed": int(rng.integers(0, 2**31 - 1))})
     return out
 
