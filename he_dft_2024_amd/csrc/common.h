@@ -341,6 +341,10 @@ struct hedft_ctx {
   // column passes instead of one whole-plane workgroup per z-plane
   int xy_split_nb[2];  // [forward, inverse]
   int no_side;         // 1: run the side branches on the main stream
+  // > 0: the TDDFT steppers compute the x+y Laplacian themselves (k_lap25,
+  // z chunks of lap25_tz planes) instead of reading a separate x+y pass
+  int lap25_tz;
+  hedft::StencilCoef* coef2_dev;  // device copy of coef[1][0..2] (second derivative, x y z)
   int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
   int lap_late;        // where the next step's x+y Laplacians run (hedft_tuning.lap_late)

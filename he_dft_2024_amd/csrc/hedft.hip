@@ -83,9 +83,17 @@ struct Lam {
   // The reference assigns cmplx(Lambda,1.d0) without a KIND argument, i.e. a
   // default-kind (REAL(4)) complex, to timec: Lambda is rounded to single
   // precision before it enters the complex(8) array.
-  __device__ __forceinline__ double at(int ix, int iy, int iz) const {
-    return (double)(float)(lambdah * (1.0 + tx[ix] + 1.0 + ty[iy] + 1.0 + tz[iz]));
+  __device__ __forceinline__ double at(int ix, int iy, int iz) const { return of(tx[ix], ty[iy], tz[iz]); }
+  // from the three table values
+  __device__ __forceinline__ double of(double ax, double ay, double az) const {
+    return (double)(float)(lambdah * (1.0 + ax + 1.0 + ay + 1.0 + az));
   }
+};
+
+// raw operands of H psi at a point: the potential and the absorber's table
+// values, loaded ahead of the arithmetic by the 2.5-D stepper kernel
+struct HOps {
+  double p, ax, ay, az;
 };
 
 // ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
@@ -100,12 +108,24 @@ struct HpsiC {
   Lam lam;
   ImpDev im;
   double h2o2m4, mu4;
+  __device__ __forceinline__ HOps ops(size_t i, int ix, int iy, int iz) const {
+    return HOps{(double)pot4[i], lam.tx[ix], lam.ty[iy], lam.tz[iz]};
+  }
   __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2 ps) const {
-    const double2 l0 = lapxy[i];
-    const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
-    const double v = (double)pot4[i] - mu4;
+    return kv(i, ix, iy, iz, dz, ps, ops(i, ix, iy, iz));
+  }
+  // o: operands at the point; lapxy null: dz is the whole Laplacian (the
+  // 2.5-D stepper kernel)
+  __device__ __forceinline__ double2 kv(size_t i, int ix, int iy, int iz, double2 dz, double2 ps,
+                                        const HOps& o) const {
+    double2 lap = dz;
+    if (lapxy) {
+      const double2 l0 = lapxy[i];
+      lap = make_double2(l0.x + dz.x, l0.y + dz.y);
+    }
+    const double v = o.p - mu4;
     const double2 t = make_double2(lap.x * h2o2m4 - v * ps.x, lap.y * h2o2m4 - v * ps.y);
-    const double lr = lam.at(ix, iy, iz);
+    const double lr = lam.of(o.ax, o.ay, o.az);
     // (lr + i) * t
     const double2 tt = make_double2(lr * t.x - t.y, lr * t.y + t.x);
     const double u = (im.nimp > 0) ? imp_u(im, ix, iy, iz) : 0.0;
@@ -122,12 +142,22 @@ struct HpsiX {
   const R* upot;
   Lam lam;
   double h2o2m;
+  __device__ __forceinline__ HOps ops(size_t i, int ix, int iy, int iz) const {
+    return HOps{(double)upot[i], lam.tx[ix], lam.ty[iy], lam.tz[iz]};
+  }
   __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2 ps) const {
-    const double2 l0 = lapxy[i];
-    const double2 lap = make_double2(l0.x + dz.x, l0.y + dz.y);
-    const double v = (double)upot[i];
+    return kv(i, ix, iy, iz, dz, ps, ops(i, ix, iy, iz));
+  }
+  __device__ __forceinline__ double2 kv(size_t i, int ix, int iy, int iz, double2 dz, double2 ps,
+                                        const HOps& o) const {
+    double2 lap = dz;
+    if (lapxy) {
+      const double2 l0 = lapxy[i];
+      lap = make_double2(l0.x + dz.x, l0.y + dz.y);
+    }
+    const double v = o.p;
     const double2 t = make_double2(lap.x * h2o2m - v * ps.x, lap.y * h2o2m - v * ps.y);
-    const double lr = lam.at(ix, iy, iz);
+    const double lr = lam.of(o.ax, o.ay, o.az);
     return make_double2(lr * t.x - t.y, lr * t.y + t.x);
   }
 };
@@ -150,16 +180,30 @@ struct OutRK {  // one RK-Gill stage (steprk.f90:67-79)
   R* den;
   double a, b, cc, dt;
   int first;
-  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double*) const {
-    const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
-    const double2 qq = q[i];
+  // the operands read at the point, loaded ahead of the arithmetic by the
+  // 2.5-D kernel (store() loads and applies them at once)
+  struct Ops {
+    double2 qq, h0, p1, p0;
+    HOps v;
+  };
+  // (the history slots are read on every stage, used on the first)
+  __device__ __forceinline__ Ops load(size_t i, int ix, int iy, int iz) const {
+    return Ops{q[i], ho0[i], po1[i], po0[i], h.ops(i, ix, iy, iz)};
+  }
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double* red) const {
+    apply(i, ix, iy, iz, dz, ps, load(i, ix, iy, iz), red);
+  }
+  __device__ __forceinline__ void apply(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, const Ops& o,
+                                        double*) const {
+    const double2 k4 = h.kv(i, ix, iy, iz, dz, ps, o.v);
+    const double2 qq = o.qq;
     const double2 s1 = make_double2(a * (k4.x - b * qq.x), a * (k4.y - b * qq.y));
     q[i] = make_double2(qq.x + 3.0 * s1.x - cc * k4.x, qq.y + 3.0 * s1.y - cc * k4.y);
     if (first) {
-      ho1[i] = ho0[i];
+      ho1[i] = o.h0;
       ho0[i] = k4;
-      po2[i] = po1[i];
-      po1[i] = po0[i];
+      po2[i] = o.p1;
+      po1[i] = o.p0;
       po0[i] = ps;
     }
     const double2 np = make_double2(ps.x + dt * s1.x, ps.y + dt * s1.y);
@@ -178,14 +222,25 @@ struct OutPred {  // predictor + modifier (steppc.f90:51-61)
   double2 *psi, *pc, *po3, *ho1, *ho2;
   R* den;
   double dt;
-  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double*) const {
-    const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
-    const double2 o3 = po3[i], h1 = ho1[i], h2 = ho2[i];
+  struct Ops {
+    double2 o3, h1, h2, p;
+    HOps v;
+  };
+  __device__ __forceinline__ Ops load(size_t i, int ix, int iy, int iz) const {
+    return Ops{po3[i], ho1[i], ho2[i], pc[i], h.ops(i, ix, iy, iz)};
+  }
+  __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double* red) const {
+    apply(i, ix, iy, iz, dz, ps, load(i, ix, iy, iz), red);
+  }
+  __device__ __forceinline__ void apply(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, const Ops& o,
+                                        double*) const {
+    const double2 k4 = h.kv(i, ix, iy, iz, dz, ps, o.v);
+    const double2 o3 = o.o3, h1 = o.h1, h2 = o.h2;
     const double f = C4O3 * dt;
     const double2 s1 = make_double2(o3.x + f * (2.0 * k4.x - h1.x + 2.0 * h2.x),
                                     o3.y + f * (2.0 * k4.y - h1.y + 2.0 * h2.y));
     po3[i] = ps;
-    const double2 p = pc[i];
+    const double2 p = o.p;
     const double2 np = make_double2(s1.x - C112 * p.x, s1.y - C112 * p.y);
     psi[i] = np;
     pc[i] = s1;
@@ -204,13 +259,24 @@ struct OutCorr {  // corrector + final value (steppc.f90:118-127), sum|pc|
   double2 *psi, *pc, *po3, *po2, *ho1, *ho2;
   R* den;
   double dt;
+  struct Ops {
+    double2 o3, o2, h1, h2, p0;
+    HOps v;
+  };
+  __device__ __forceinline__ Ops load(size_t i, int ix, int iy, int iz) const {
+    return Ops{po3[i], po2[i], ho1[i], ho2[i], pc[i], h.ops(i, ix, iy, iz)};
+  }
   __device__ __forceinline__ void store(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, double* red) const {
-    const double2 k4 = h.k(i, ix, iy, iz, dz, ps);
-    const double2 o3 = po3[i], o2 = po2[i], h1 = ho1[i], h2 = ho2[i];
+    apply(i, ix, iy, iz, dz, ps, load(i, ix, iy, iz), red);
+  }
+  __device__ __forceinline__ void apply(size_t i, int ix, int iy, int iz, double2 dz, double2 ps, const Ops& o,
+                                        double* red) const {
+    const double2 k4 = h.kv(i, ix, iy, iz, dz, ps, o.v);
+    const double2 o3 = o.o3, o2 = o.o2, h1 = o.h1, h2 = o.h2;
     const double f = 3.0 * dt;
     const double2 s5 = make_double2(0.125 * (9.0 * o3.x - o2.x + f * (k4.x + 2.0 * h1.x - h2.x)),
                                     0.125 * (9.0 * o3.y - o2.y + f * (k4.y + 2.0 * h1.y - h2.y)));
-    const double2 p0 = pc[i];
+    const double2 p0 = o.p0;
     const double2 p = make_double2(p0.x - s5.x, p0.y - s5.y);
     pc[i] = p;
     const double2 np = make_double2(s5.x + C9 * p.x, s5.y + C9 * p.y);
@@ -1162,10 +1228,13 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   // the mean field that also carries the impurity force, 3 on the main stream
   // after the den convolutions (beside the rest of the side branch), 4 on the
   // side stream after grad(den) with stage B waiting only for grad(den) (the
-  // Laplacians are joined with stage C's branch, or after stage B)
+  // Laplacians are joined with stage C's branch, or after stage B).  With the
+  // 2.5-D steppers there is no Laplacian pass; lap_late 4 still lets stage B
+  // wait only for grad(den).
+  const bool mid = (with_lap || c->lap25_tz > 0) && c->lap_late == 4 && !c->no_side;
+  if (c->lap25_tz > 0) with_lap = 0;
   const bool late = with_lap && (c->lap_late == 1 || (c->lap_late == 2 && want_force));
   const bool on_main = with_lap && c->lap_late == 3;
-  const bool mid = with_lap && c->lap_late == 4 && !c->no_side;
   HEDFT_TRY(fork_point(c));
   XInT<R> xi{};
   xi.op = XIN_COPY;
@@ -1349,7 +1418,7 @@ static Lam make_lam(hedft_ctx* c) {
 template <typename R>
 static HpsiC<R> make_hpsic(hedft_ctx* c) {
   HpsiC<R> h;
-  h.lapxy = (const double2*)c->lap;
+  h.lapxy = c->lap25_tz > 0 ? nullptr : (const double2*)c->lap;
   h.pot4 = fld<R>(c->pot4);
   h.lam = make_lam(c);
   h.im = make_impdev(c);
@@ -1361,7 +1430,7 @@ static HpsiC<R> make_hpsic(hedft_ctx* c) {
 template <typename R>
 static HpsiX<R> make_hpsix(hedft_ctx* c) {
   HpsiX<R> h;
-  h.lapxy = (const double2*)c->lapx;
+  h.lapxy = c->lap25_tz > 0 ? nullptr : (const double2*)c->lapx;
   h.upot = fld<R>(c->upotx);
   h.lam = make_lam(c);
   h.h2o2m = c->h2o2mimp;
@@ -1370,7 +1439,7 @@ static HpsiX<R> make_hpsix(hedft_ctx* c) {
 
 // x+y Laplacians the fused step kernels start from (psi, and psix with a quantum impurity)
 static int prepare_lap(hedft_ctx* c) {
-  if (c->lap_fresh) return HEDFT_OK;
+  if (c->lap_fresh || c->lap25_tz > 0) return HEDFT_OK;
   HEDFT_TRY(lap_xy(c, c->psi, 1));
   if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
   return HEDFT_OK;
@@ -1378,6 +1447,26 @@ static int prepare_lap(hedft_ctx* c) {
 
 static inline double2* cz(void* p) { return (double2*)p; }
 static inline double* rz(void* p) { return (double*)p; }
+
+// one fused H psi + integrator pass over a complex field: the 2.5-D kernel
+// (x, y and z second derivatives in one pass) or the z pass after the
+// separate x+y Laplacian pass.  The 2.5-D kernel reads psi at the halos of
+// neighbouring tiles while the pass runs, so it writes the new psi to the
+// spare buffer (the one the separate pass would hold the Laplacian in), which
+// then becomes psi; a PC iteration swaps twice and an RK step four times.
+template <class Out>
+static void launch_step(hedft_ctx* c, void*& psi, void*& spare, Out o) {
+  if (c->lap25_tz > 0) {
+    o.psi = cz(spare);
+    launch_lap25(c->stream, c->d, c->coef2_dev, cz(psi), o, c->lap25_tz);
+    std::swap(psi, spare);
+  } else {
+    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(psi)}, o);
+  }
+}
+static size_t step_nblocks(hedft_ctx* c) {
+  return c->lap25_tz > 0 ? lap25_geom(c->d, c->lap25_tz).nblocks : pencil_geom<double2>(c->d, 2).nblocks;
+}
 
 // kernel-duration probe around one fused stepper launch (slot 0 pred, 1 corr)
 static void kt_mark(hedft_ctx* c, int slot, int end) {
@@ -1405,7 +1494,7 @@ static void steprk_x(hedft_ctx* c, double dt, int jrun) {
   o.cc = RK_C[jrun];
   o.dt = dt;
   o.first = (jrun == 0);
-  launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psix)}, o);
+  launch_step(c, c->psix, c->lapx, o);
 }
 
 // STEPRK (Dynamics_TDDFT/steprk.f90:1-144; with a quantum impurity
@@ -1430,7 +1519,7 @@ static int steprk_t(hedft_ctx* c, double dt) {
     o.cc = RK_C[jrun];
     o.dt = dt;
     o.first = (jrun == 0);
-    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    launch_step(c, c->psi, c->lap, o);
     if (c->quantum) steprk_x<R>(c, dt, jrun);
     c->lap_fresh = 0;
     if (c->nimp > 0)
@@ -1463,7 +1552,7 @@ static int steppc_t(hedft_ctx* c, double dt) {
     o.den = fld<R>(c->den);
     o.dt = dt;
     kt_mark(c, 0, 0);
-    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    launch_step(c, c->psi, c->lap, o);
     kt_mark(c, 0, 1);
     if (c->quantum) {
       OutPred<R, HpsiX<R>> ox;
@@ -1476,7 +1565,7 @@ static int steppc_t(hedft_ctx* c, double dt) {
       ox.ho2 = cz(c->hpsixold[c->ioldh[1]]);
       ox.den = nullptr;
       ox.dt = dt;
-      launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psix)}, ox);
+      launch_step(c, c->psix, c->lapx, ox);
     }
     c->lap_fresh = 0;
     std::swap(c->ioldh[0], c->ioldh[1]);
@@ -1503,7 +1592,7 @@ static int steppc_t(hedft_ctx* c, double dt) {
     o.den = fld<R>(c->den);
     o.dt = dt;
     kt_mark(c, 1, 0);
-    launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psi)}, o);
+    launch_step(c, c->psi, c->lap, o);
     kt_mark(c, 1, 1);
     if (c->quantum) {
       OutCorr<R, HpsiX<R>> ox;
@@ -1517,12 +1606,12 @@ static int steppc_t(hedft_ctx* c, double dt) {
       ox.ho2 = cz(c->hpsixold[c->ioldh[1]]);
       ox.den = nullptr;
       ox.dt = dt;
-      launch_pencil<double2, 2>(c->stream, c->d, c->coef[1][2], InPlain<double2>{cz(c->psix)}, ox);
+      launch_step(c, c->psix, c->lapx, ox);
       c->side_pre |= 8;  // errpsix reduction, with errHe's
     }
     c->lap_fresh = 0;
     c->side_pre |= 4;  // errHe reduction, at the start of the next side branch
-    c->side_nblocks = pencil_geom<double2>(c->d, 2).nblocks;
+    c->side_nblocks = step_nblocks(c);
     const int t = c->ioldp[2];
     c->ioldp[2] = c->ioldp[1];
     c->ioldp[1] = c->ioldp[0];
@@ -2018,12 +2107,21 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
     maxb = std::max(maxb, pencil_geom<double>(d, a).nblocks);
   }
   maxb = std::max(maxb, (size_t)EW_BLOCKS);
+  // the 2.5-D stepper (TDDFT): z chunks of 16 planes by default, HEDFT_LAP25=<planes>
+  // selects another chunk length and 0 the separate x+y Laplacian pass
+  if (mode == HEDFT_MODE_TDDFT) {
+    int tz = 16;
+    if (const char* e = std::getenv("HEDFT_LAP25")) tz = std::atoi(e);
+    if (tz > 0 && lap25_geom(d, tz).nblocks > 0) c->lap25_tz = tz;
+    maxb = std::max(maxb, lap25_geom(d, 1).nblocks);
+  }
   c->partials_cap = 8 * maxb * 16;  // 8 regions of 16 values per block
   rc = rc ? rc : alloc(c, (void**)&c->partials, c->partials_cap * sizeof(double));
   rc = rc ? rc : alloc(c, (void**)&c->scal, NSCAL * sizeof(double));
   rc = rc ? rc : alloc(c, (void**)&c->imp, IMP_SIZE * sizeof(double));
   rc = rc ? rc : alloc(c, (void**)&c->lstop, sizeof(int));
   rc = rc ? rc : alloc(c, (void**)&c->lamtab, sizeof(double) * (d.nx + d.ny + d.nz));
+  rc = rc ? rc : alloc(c, (void**)&c->coef2_dev, 3 * sizeof(StencilCoef));
   if (rc == HEDFT_OK && hipHostMalloc((void**)&c->host_scal, NSCAL * sizeof(double)) != hipSuccess) rc = HEDFT_ENOMEM;
   if (rc == HEDFT_OK) rc = fft_setup(c);
   if (rc != HEDFT_OK) {
@@ -2031,6 +2129,11 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
     return rc;
   }
   stencil_setup(c);
+  if (hipMemcpyAsync(c->coef2_dev, c->coef[1], 3 * sizeof(StencilCoef), hipMemcpyHostToDevice, c->stream) !=
+      hipSuccess) {
+    hedft_destroy(c);
+    return HEDFT_EHIP;
+  }
   // spectral kernels (fforma, initcg, kalfs) and absorbing-layer tables
   std::vector<double> fv, wc, ka;
   kernel_tables(c, fv, wc, ka);
@@ -2080,7 +2183,7 @@ int hedft_destroy(hedft_ctx* c) {
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
                   c->imp, c->lstop, c->potion, c->psix, c->psixold[0], c->psixold[1], c->psixold[2],
                   c->hpsixold[0], c->hpsixold[1], c->pcx, c->qx, c->lapx, c->potx4, c->upotx, c->uimpk,
-                  c->specq[0], c->specq[1], c->specd};
+                  c->specq[0], c->specq[1], c->specd, c->coef2_dev};
   for (void* q : ptrs)
     if (q) hipFree(q);
   clear_pairs(c);
@@ -3020,11 +3123,14 @@ static void apply_tuning(hedft_ctx* c, const hedft_tuning* t) {
   c->no_side = t->no_side;
   c->prefer_eager = t->prefer_eager;
   c->lap_late = t->lap_late;
+  c->lap25_tz = t->lap25;
+  c->lap_fresh = 0;    // the spare psi buffer does not hold the x+y Laplacian
   c->graph_valid = 0;  // the captured graph encodes the launch structure
 }
 
 int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t) {
-  if (!c || !t || t->xy_split_fwd < 0 || t->xy_split_inv < 0) return HEDFT_EINVAL;
+  if (!c || !t || t->xy_split_fwd < 0 || t->xy_split_inv < 0 || t->lap25 < 0) return HEDFT_EINVAL;
+  if (t->lap25 > 0 && (c->mode != HEDFT_MODE_TDDFT || lap25_geom(c->d, t->lap25).nblocks == 0)) return HEDFT_EGRID;
   apply_tuning(c, t);
   return HEDFT_OK;
 }
@@ -3036,7 +3142,7 @@ int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t) {
   t->no_side = c->no_side;
   t->prefer_eager = c->prefer_eager;
   t->lap_late = c->lap_late;
-  t->pad_ = 0;
+  t->lap25 = c->lap25_tz;
   t->ms_per_step = 0.0;
   return HEDFT_OK;
 }
@@ -3058,6 +3164,7 @@ int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tunin
           cur.no_side = ns;
           cur.prefer_eager = eg;
           cur.lap_late = ll;
+          cur.lap25 = c->lap25_tz;
           apply_tuning(c, &cur);
           HEDFT_TRY(enqueue_pc(c, deltat, PC_PERIOD, 1));  // warm-up incl. graph capture
           HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));

@@ -10,6 +10,7 @@
 #pragma once
 #include <cstdlib>
 #include <type_traits>
+#include <utility>
 
 #include "common.h"
 
@@ -598,6 +599,247 @@ inline void launch_pencil_xy(hipStream_t s, const Dims& d, const StencilCoef& cf
   else if (g.tx == 32) launch_pencil_xy_t<V, SAME, 32>(s, g, d, cfx, cfy, in, out);
   else if (g.tx == 16) launch_pencil_xy_t<V, SAME, 16>(s, g, d, cfx, cfy, in, out);
   else launch_pencil_xy_t<V, SAME, 8>(s, g, d, cfx, cfy, in, out);
+}
+
+// Out types whose H carries classical impurities (Out::h.im)
+template <class Out, class = void>
+struct has_imp : std::false_type {};
+template <class Out>
+struct has_imp<Out, std::void_t<decltype(std::declval<Out&>().h.im)>> : std::true_type {};
+
+// ---- 2.5-D fused Laplacian of a complex field with a z-pass epilogue --------
+// A workgroup owns a TX x TY tile of (x, y) columns over the planes z0..z1-1
+// (z1 - z0 <= tz).  Each thread keeps the 13 planes of its column that the z
+// stencil reads in registers and moves up one plane per step.  Each plane of
+// the tile, with 6-point x and y halos (periodic halos wrapped with the
+// antiperiodic sign, others clamped: no output reads those), is staged in LDS,
+// double-buffered, for the x and y derivatives; the halo and the next queue
+// value of plane z+1 are loaded while plane z is evaluated.  The epilogue
+// (Out::apply, with the operands Out::load read one step ahead)
+// receives (d_x + d_y) + d_z and the value at the point: the same sums, in
+// the same order, as the x+y tile pass (k_pencil_xy2 with OutSumXY) followed
+// by the z pass (k_pencil) produce.  cf3: the x, y, z coefficient sets in
+// device memory.  Requires nx % TX == 0, ny % TY == 0, TX, TY >= 7 (a tile
+// at the box edge then holds the 13 rows a one-sided stencil reads) and
+// n >= 13 on every axis.
+template <int TX, int TY, class Out>
+__global__ void __launch_bounds__(TX * TY) k_lap25(const double2* __restrict__ P, Out out,
+                                                  const StencilCoef* __restrict__ cf3, Dims d, int tz) {
+  static_assert(TX >= 7 && TY >= 7, "edge tiles must hold 13 rows");
+  constexpr int NT = TX * TY, TW = TX + 12, TH = TY + 12;
+  // halo slots: the 6-row bands above and below the centre columns, and the
+  // 6-column bands left and right of the centre rows (the corners are read by
+  // no stencil)
+  constexpr int NHALO = 12 * TX + 12 * TY, HU = (NHALO + NT - 1) / NT;
+  __shared__ double2 buf[2][TH * TW];
+  constexpr int NR = Out::NR;
+  double red[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int r = 0; r < (NR > 0 ? NR : 1); ++r) red[r] = 0.0;
+  // the coefficients live in LDS: read from global memory inside the loop
+  // they would be vector loads, whose waits would also wait for the prefetches
+  __shared__ StencilCoef cfs[3];
+  static_assert(sizeof(StencilCoef) % 8 == 0, "copied as doubles");
+  for (int e = threadIdx.x; e < (int)(3 * sizeof(StencilCoef) / 8); e += NT)
+    reinterpret_cast<double*>(cfs)[e] = reinterpret_cast<const double*>(cf3)[e];
+  __syncthreads();
+  const StencilCoef& cfx = cfs[0];
+  const StencilCoef& cfy = cfs[1];
+  const StencilCoef& cfz = cfs[2];
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int ntx = nx / TX, nty = ny / TY, nzc = (nz + tz - 1) / tz;
+  // XCD-aware order (see k_pencil_xy): each XCD takes a contiguous range of
+  // tiles, x tiles then y tiles then z chunks, so the halos a tile shares
+  // with its x and y neighbours stay in that XCD's L2
+  const int ntiles = ntx * nty * nzc;
+  int tile = blockIdx.x;
+  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+  const int x0 = (tile % ntx) * TX, y0 = ((tile / ntx) % nty) * TY, zc = tile / (ntx * nty);
+  const int z0 = zc * tz, z1 = min(z0 + tz, nz);
+  const int xl = threadIdx.x % TX, yl = threadIdx.x / TX;
+  const int gx = x0 + xl, gy = y0 + yl;
+  const size_t plane = (size_t)nx * ny;
+  const size_t col = (size_t)gy * nx + gx;
+  const int cpos = (yl + 6) * TW + xl + 6;
+  const double sgx = (cfx.mode == 2) ? -1.0 : 1.0, sgy = (cfy.mode == 2) ? -1.0 : 1.0;
+  const double sgz = (cfz.mode == 2) ? -1.0 : 1.0;
+  int hoff[HU];
+  size_t hsrc[HU];
+  double hsg[HU];
+#pragma unroll
+  for (int u = 0; u < HU; ++u) {
+    const int k = min((int)threadIdx.x + u * NT, NHALO - 1);
+    int r, c;
+    if (k < 6 * TX) {
+      r = k / TX;
+      c = 6 + k % TX;
+    } else if (k < 12 * TX) {
+      r = TY + 6 + (k - 6 * TX) / TX;
+      c = 6 + (k - 6 * TX) % TX;
+    } else {
+      const int k3 = k - 12 * TX, cc = k3 % 12;
+      r = 6 + k3 / 12;
+      c = (cc < 6) ? cc : TX + cc;
+    }
+    double sg = 1.0;
+    int yy = y0 - 6 + r, xx = x0 - 6 + c;
+    if (yy < 0 || yy >= ny) {
+      if (cfy.mode != 0) {
+        sg *= sgy;
+        yy += (yy < 0) ? ny : -ny;
+      } else {
+        yy = min(max(yy, 0), ny - 1);
+      }
+    }
+    if (xx < 0 || xx >= nx) {
+      if (cfx.mode != 0) {
+        sg *= sgx;
+        xx += (xx < 0) ? nx : -nx;
+      } else {
+        xx = min(max(xx, 0), nx - 1);
+      }
+    }
+    hoff[u] = ((int)threadIdx.x + u * NT < NHALO) ? r * TW + c : -1;
+    hsrc[u] = (size_t)yy * nx + xx;
+    hsg[u] = sg;
+  }
+  // plane of queue slot 0 at step z (unwrapped; clamped to the box for the
+  // one-sided rows of mode 0)
+  const bool clampz = cfz.mode == 0;
+  auto qbase = [&](int z) { return clampz ? min(max(z - 6, 0), nz - 13) : z - 6; };
+  // unwrapped plane u -> stored plane, and the antiperiodic sign of the wrap
+  auto zwrap = [&](int u, double& f) {
+    f = 1.0;
+    if (u < 0) {
+      f = sgz;
+      return u + nz;
+    }
+    if (u >= nz) {
+      f = sgz;
+      return u - nz;
+    }
+    return u;
+  };
+  double2 q[13];
+  int b = qbase(z0);
+#pragma unroll
+  for (int j = 0; j < 13; ++j) {
+    double f;
+    const int zz = zwrap(b + j, f);
+    q[j] = vmul(f, P[(size_t)zz * plane + col]);
+  }
+  // value of plane zp (queue base bq) at this thread's column, from the queue
+  // (uniform slot: 6 or 7 inside the box, others within 6 planes of its ends
+  // in mode 0; registers only)
+  auto centre = [&](int slot) {
+    double2 v = q[6];
+#pragma unroll
+    for (int j = 0; j < 13; ++j)
+      if (slot == j) v = q[j];
+    return v;
+  };
+  // plane z0 of the tile into buffer 0
+  {
+    double2 hv[HU];
+#pragma unroll
+    for (int u = 0; u < HU; ++u) hv[u] = P[(size_t)z0 * plane + hsrc[u]];
+    buf[0][cpos] = centre(z0 - b);
+#pragma unroll
+    for (int u = 0; u < HU; ++u)
+      if (hoff[u] >= 0) buf[0][hoff[u]] = vmul(hsg[u], hv[u]);
+  }
+  Out o = out;
+  __shared__ double rimp_lds[3 * HEDFT_MAX_IMP];
+  if constexpr (has_imp<Out>::value) {
+    // impurity positions (fixed during the pass) from LDS: no global load
+    // of them inside the loop
+    if ((int)threadIdx.x < 3 * o.h.im.nimp) rimp_lds[threadIdx.x] = o.h.im.rimp[threadIdx.x];
+    o.h.im.rimp = rimp_lds;
+  }
+  // the epilogue's operands at the point, loaded one step ahead
+  typename Out::Ops ops = o.load((size_t)z0 * plane + col, gx, gy, z0);
+  // Per step: the next plane's halo and queue value are loaded before the
+  // barrier and stored into the other LDS buffer after the stencils, the
+  // epilogue then runs on operands loaded in the previous step, and the next
+  // step's operands are loaded last.  Every load is issued unconditionally
+  // (from a clamped, valid address when its value is not needed), so each
+  // wait covers only loads issued well before it.
+  for (int z = z0; z < z1; ++z) {
+    const int zn = z + 1;
+    const bool more = zn < z1;
+    const int bn = more ? qbase(zn) : b;
+    const size_t pn = (size_t)min(zn, nz - 1) * plane;
+    double2 hv[HU];
+#pragma unroll
+    for (int u = 0; u < HU; ++u) hv[u] = P[pn + hsrc[u]];
+    double nqf;
+    const int nqz = zwrap(bn + 12, nqf);
+    const double2 nqr = P[(size_t)nqz * plane + col];
+    __syncthreads();
+    const double2* T = buf[(z - z0) & 1];
+    const int k = z - b;
+    const double2 cen = centre(k);
+    const double2 dx = stencil_tile<double2>(cfx, T + (yl + 6) * TW, 1, gx, nx, x0);
+    const double2 dy = stencil_tile<double2>(cfy, T + 6 + xl, TW, gy, ny, y0);
+    double2 dz = make_double2(0.0, 0.0);
+    if (k == 6) {
+#pragma unroll
+      for (int j = 0; j < 13; ++j) dz = vfma(cfz.c[j], q[j], dz);
+    } else {
+      const int r = (z < 6) ? z : z - nz + 12;
+#pragma unroll
+      for (int j = 0; j < 13; ++j) dz = vfma(cfz.rows[r][j], q[j], dz);
+    }
+    if (more) {
+      double2* Tn = buf[(zn - z0) & 1];
+      Tn[cpos] = centre(zn - b);
+#pragma unroll
+      for (int u = 0; u < HU; ++u)
+        if (hoff[u] >= 0) Tn[hoff[u]] = vmul(hsg[u], hv[u]);
+    }
+    const double2 sxy = make_double2(dx.x + dy.x, dx.y + dy.y);
+    o.apply((size_t)z * plane + col, gx, gy, z, make_double2(sxy.x + dz.x, sxy.y + dz.y), cen, ops, red);
+    ops = o.load(pn + col, gx, gy, min(zn, nz - 1));
+    if (bn != b) {
+#pragma unroll
+      for (int j = 0; j < 12; ++j) q[j] = q[j + 1];
+      q[12] = vmul(nqf, nqr);
+      b = bn;
+    }
+  }
+  if constexpr (NR > 0) block_reduce_store<NR>(red, out.partials);
+}
+
+// geometry of the 2.5-D kernel: tile (TX, TY) = (32, 8) when the planes allow
+// it, else (16, 16); 0 when neither divides the plane
+struct Lap25Geom {
+  int tx, ty, tz;
+  size_t nblocks;
+};
+inline Lap25Geom lap25_geom(const Dims& d, int tz) {
+  Lap25Geom g{0, 0, tz, 0};
+  if (d.nx < 13 || d.ny < 13 || d.nz < 13 || tz < 1) return g;
+  if (d.nx % 32 == 0 && d.ny % 8 == 0) {
+    g.tx = 32;
+    g.ty = 8;
+  } else if (d.nx % 16 == 0 && d.ny % 16 == 0) {
+    g.tx = 16;
+    g.ty = 16;
+  } else {
+    return g;
+  }
+  g.nblocks = (size_t)(d.nx / g.tx) * (d.ny / g.ty) * ((d.nz + tz - 1) / tz);
+  return g;
+}
+
+template <class Out>
+inline void launch_lap25(hipStream_t s, const Dims& d, const StencilCoef* cf3, const double2* psi, const Out& o,
+                         int tz) {
+  const Lap25Geom g = lap25_geom(d, tz);
+  if (g.tx == 32)
+    hipLaunchKernelGGL((k_lap25<32, 8, Out>), dim3(g.nblocks), dim3(256), 0, s, psi, o, cf3, d, tz);
+  else
+    hipLaunchKernelGGL((k_lap25<16, 16, Out>), dim3(g.nblocks), dim3(256), 0, s, psi, o, cf3, d, tz);
 }
 
 // ---- common functors ---------------------------------------------------------

@@ -473,12 +473,20 @@ typedef struct hedft_tuning {
                                 3 on the main stream after the density
                                 convolutions, 4 on the side stream after
                                 grad(den), joined after stage B starts */
-  int32_t pad_;
+  int32_t lap25;             /* TDDFT steppers: > 0 computes the x, y and z
+                                second derivatives in one 2.5-D pass over
+                                chunks of lap25 planes (the default, 16);
+                                0 runs the separate x+y Laplacian pass (kept
+                                for comparison).  Fields are identical; errHe
+                                sums its blocks in another order. */
   double ms_per_step;        /* measured time of the winning choice (autotune) */
 } hedft_tuning;
 /* Times trial_steps PC iterations (rounded up to 6, after a 6-iteration
- * warm-up) for each of the 80 combinations on this context and keeps the
- * fastest.  The context advances by 80 (6 + trial) iterations. */
+ * warm-up) for each of the 80 combinations of the other fields on this
+ * context (lap25 kept) and keeps the fastest.  The context advances by 80
+ * (6 + trial) iterations.  hedft_set_tuning returns HEDFT_EGRID for lap25 > 0
+ * on a grid the 2.5-D tiles do not divide (nx % 16, ny % 16) or a statics
+ * context. */
 int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tuning* best);
 int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t);
 int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t);

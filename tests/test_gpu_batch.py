@@ -74,6 +74,8 @@ def test_tuning_choices_keep_results(gpu):
               {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0, "lap_late": 3},
               {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0, "lap_late": 4},
               {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 1, "prefer_eager": 0, "lap_late": 4},
+              {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 0, "prefer_eager": 0, "lap_late": 4, "lap25": 0},
+              {"xy_split_fwd": 0, "xy_split_inv": 0, "no_side": 1, "prefer_eager": 0, "lap_late": 3, "lap25": 0},
               {"xy_split_fwd": 4, "xy_split_inv": 4, "no_side": 0, "prefer_eager": 0, "lap_late": 0}):
         d = _make(gpu, CASES[1])
         d.set_tuning(t)
