@@ -83,8 +83,9 @@ def algorithmic_bytes_per_pc_step(n: int, w: int = 8) -> float:
 def bytes_fused_step_kernel(n: int, w: int = 8) -> float:
     """Algorithmic bytes of one fused H psi + predictor (or corrector) launch:
     B_pred = B_corr = 8 (2w) N + 2 w N (complex psi history reads/writes, pot4
-    in, den out); the Laplacian's x+y partial sum is an intermediate of the
-    two-pass stencil, not algorithmic traffic."""
+    in, den out).  The kernel is the 2.5-D pass (x, y and z stencils of psi
+    and the update together); its tile halos are re-reads, not algorithmic
+    traffic."""
     N = n ** 3
     return 8 * (2 * w) * N + 2 * w * N
 
@@ -149,7 +150,7 @@ def main():
     bytes_step = algorithmic_bytes_per_pc_step(n)
     step_gbs = bytes_step * steps_per_s / 1e9
 
-    # dominant kernel: the fused H psi + predictor / corrector z-pass, timed
+    # dominant kernel: the fused H psi + predictor / corrector pass, timed
     # live with HIP events on the library's main stream over real PC steps
     roofline = None
     if a.probe_steps > 0:
