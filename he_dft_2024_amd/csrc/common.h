@@ -102,7 +102,9 @@ struct ImpDev {
 // Products and sums here are rounded separately (no FMA contraction), as a
 // plain CPU build evaluates x(i) = -xmax + h*(i-1) and r; the table lookup is
 // discontinuous where r/DelInter is an integer, so r must match to the last bit.
-__device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, double zz, bool flag) {
+// table node ir and weight rmod of impurity k at (xx, yy, zz)
+__device__ inline void imp_node_k(const ImpDev& im, int k, double xx, double yy, double zz, bool flag, int& ir,
+                                  double& rmod) {
 #pragma clang fp contract(off)
   const double rx = im.rimp[k], ry = im.rimp[k + im.nimp], rz = im.rimp[k + 2 * im.nimp];
   const double zt = (zz - rz) * (zz - rz);
@@ -118,14 +120,26 @@ __device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, 
   // reference behaves so at such points, e.g. grid points exactly 4 A from an
   // impurity sitting on a grid line).
   const double q = trunc(r / im.delinter);
-  int ir = (int)q;
+  ir = (int)q;
   if (ir > im.npot - 2) ir = im.npot - 2;
   double rem = fma(-q, im.delinter, r);
   if (rem < 0.0) rem += im.delinter;
   else if (rem >= im.delinter) rem -= im.delinter;
-  const double rmod = rem / im.delinter;
+  rmod = rem / im.delinter;
+}
+
+// the linear interpolation between the two table values
+__device__ inline double imp_lerp(double t0, double t1, double rmod) {
+#pragma clang fp contract(off)
+  return t0 * (1.0 - rmod) + t1 * rmod;
+}
+
+__device__ inline double imp_u_k(const ImpDev& im, int k, double xx, double yy, double zz, bool flag) {
+  int ir;
+  double rmod;
+  imp_node_k(im, k, xx, yy, zz, flag, ir, rmod);
   const double* tab = im.potion + (size_t)k * im.npot;
-  return tab[ir] * (1.0 - rmod) + tab[ir + 1] * rmod;
+  return imp_lerp(tab[ir], tab[ir + 1], rmod);
 }
 
 // ---- impurity-impurity pair terms (Dynamics_TDDFT/potenimp.f90:40-58,

@@ -94,6 +94,7 @@ struct Lam {
 // values, loaded ahead of the arithmetic by the 2.5-D stepper kernel
 struct HOps {
   double p, ax, ay, az;
+  double t0, t1, rm;  // one classical impurity: its two table values and weight
 };
 
 // ---- fused z-pass epilogues of the TDDFT integrators -----------------------------
@@ -108,8 +109,19 @@ struct HpsiC {
   Lam lam;
   ImpDev im;
   double h2o2m4, mu4;
+  // with one impurity its table lookup is part of the operands (loaded
+  // ahead by the 2.5-D kernel); with several it is evaluated in kv
   __device__ __forceinline__ HOps ops(size_t i, int ix, int iy, int iz) const {
-    return HOps{(double)pot4[i], lam.tx[ix], lam.ty[iy], lam.tz[iz]};
+    HOps o{(double)pot4[i], lam.tx[ix], lam.ty[iy], lam.tz[iz], 0.0, 0.0, 0.0};
+    if (im.nimp == 1) {
+      double xx, yy, zz;
+      imp_xyz(im, ix, iy, iz, xx, yy, zz);
+      int ir;
+      imp_node_k(im, 0, xx, yy, zz, false, ir, o.rm);
+      o.t0 = im.potion[ir];
+      o.t1 = im.potion[ir + 1];
+    }
+    return o;
   }
   __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2 ps) const {
     return kv(i, ix, iy, iz, dz, ps, ops(i, ix, iy, iz));
@@ -128,7 +140,9 @@ struct HpsiC {
     const double lr = lam.of(o.ax, o.ay, o.az);
     // (lr + i) * t
     const double2 tt = make_double2(lr * t.x - t.y, lr * t.y + t.x);
-    const double u = (im.nimp > 0) ? imp_u(im, ix, iy, iz) : 0.0;
+    double u = 0.0;
+    if (im.nimp == 1) u += imp_lerp(o.t0, o.t1, o.rm);
+    else if (im.nimp > 1) u = imp_u(im, ix, iy, iz);
     // - i u psi = (u psi.y, -u psi.x)
     return make_double2(tt.x + u * ps.y, tt.y - u * ps.x);
   }
@@ -143,7 +157,7 @@ struct HpsiX {
   Lam lam;
   double h2o2m;
   __device__ __forceinline__ HOps ops(size_t i, int ix, int iy, int iz) const {
-    return HOps{(double)upot[i], lam.tx[ix], lam.ty[iy], lam.tz[iz]};
+    return HOps{(double)upot[i], lam.tx[ix], lam.ty[iy], lam.tz[iz], 0.0, 0.0, 0.0};
   }
   __device__ __forceinline__ double2 k(size_t i, int ix, int iy, int iz, double2 dz, double2 ps) const {
     return kv(i, ix, iy, iz, dz, ps, ops(i, ix, iy, iz));

@@ -642,6 +642,14 @@ __global__ void __launch_bounds__(TX * TY) k_lap25(const double2* __restrict__ P
   static_assert(sizeof(StencilCoef) % 8 == 0, "copied as doubles");
   for (int e = threadIdx.x; e < (int)(3 * sizeof(StencilCoef) / 8); e += NT)
     reinterpret_cast<double*>(cfs)[e] = reinterpret_cast<const double*>(cf3)[e];
+  Out o = out;
+  __shared__ double rimp_lds[3 * HEDFT_MAX_IMP];
+  if constexpr (has_imp<Out>::value) {
+    // impurity positions (fixed during the pass) from LDS: no global load
+    // of them inside the loop
+    if ((int)threadIdx.x < 3 * o.h.im.nimp) rimp_lds[threadIdx.x] = o.h.im.rimp[threadIdx.x];
+    o.h.im.rimp = rimp_lds;
+  }
   __syncthreads();
   const StencilCoef& cfx = cfs[0];
   const StencilCoef& cfy = cfs[1];
@@ -747,14 +755,6 @@ __global__ void __launch_bounds__(TX * TY) k_lap25(const double2* __restrict__ P
 #pragma unroll
     for (int u = 0; u < HU; ++u)
       if (hoff[u] >= 0) buf[0][hoff[u]] = vmul(hsg[u], hv[u]);
-  }
-  Out o = out;
-  __shared__ double rimp_lds[3 * HEDFT_MAX_IMP];
-  if constexpr (has_imp<Out>::value) {
-    // impurity positions (fixed during the pass) from LDS: no global load
-    // of them inside the loop
-    if ((int)threadIdx.x < 3 * o.h.im.nimp) rimp_lds[threadIdx.x] = o.h.im.rimp[threadIdx.x];
-    o.h.im.rimp = rimp_lds;
   }
   // the epilogue's operands at the point, loaded one step ahead
   typename Out::Ops ops = o.load((size_t)z0 * plane + col, gx, gy, z0);
