@@ -351,8 +351,11 @@ __global__ void __launch_bounds__(256) k_pencil_xy(In in, Out out, StencilCoef c
         sg[u] = 1.0;
         if (gx < 0 || gx >= nx) {
           if (cfx.mode != 0) {
+            // one wrap covers every slot an output reads; a tile wider than
+            // the grid (tx > nx) has slots beyond that, which are clamped so
+            // that their (unused) loads stay inside the array
             sg[u] = sgw;
-            gx += (gx < 0) ? nx : -nx;
+            gx = min(max(gx + ((gx < 0) ? nx : -nx), 0), nx - 1);
           } else {
             gx = min(max(gx, 0), nx - 1);
           }
@@ -453,12 +456,15 @@ __global__ void __launch_bounds__(256) k_pencil_xy2(In in, Out out, StencilCoef 
   const size_t zb = (size_t)z * nx * ny;
   const double sgx = (cfx.mode == 2) ? -1.0 : 1.0, sgy = (cfy.mode == 2) ? -1.0 : 1.0;
   // global index -> in-range index and sign: periodic halos wrap (with the
-  // antiperiodic sign), others are clamped (never read by any output)
+  // antiperiodic sign), others are clamped (never read by any output).  One
+  // wrap covers every slot an output reads; a tile wider than the grid (TX >
+  // nx, e.g. 64 columns on a 32-point axis) has slots beyond that, which are
+  // clamped too so that their unused loads stay inside the array.
   auto fold = [](int g, int n, int mode, double sgw, double& sg) {
     if (g < 0 || g >= n) {
       if (mode != 0) {
         sg *= sgw;
-        return g + ((g < 0) ? n : -n);
+        return min(max(g + ((g < 0) ? n : -n), 0), n - 1);
       }
       return min(max(g, 0), n - 1);
     }
