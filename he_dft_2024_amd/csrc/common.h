@@ -359,6 +359,10 @@ struct hedft_ctx {
   // z chunks of lap25_tz planes) instead of reading a separate x+y pass
   int lap25_tz;
   hedft::StencilCoef* coef2_dev;  // device copy of coef[1][0..2] (second derivative, x y z)
+  hedft::StencilCoef* coef1_dev;  // device copy of coef[0][0..2] (first derivative, x y z)
+  // 1: the mean field runs on the main stream alone: the alpha_s divergence
+  // with the pot4 assembly in one 2.5-D pass (no x+y pass beside stage C)
+  int fused;
   int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
   int lap_late;        // where the next step's x+y Laplacians run (hedft_tuning.lap_late)
@@ -453,6 +457,11 @@ int derden_force_finalize(hedft_ctx* c, int update_aimp);
 // divergence of (1-denalf/den0s)*int_a, then ualphas and pot4 assembly
 int alpha_div_xy(hedft_ctx* c);      // x+y divergence of f int_a into work1
 int alpha_div_z_pot4(hedft_ctx* c);  // z divergence + ualphas + pot4 assembly
+// the whole divergence (x, y, z) + ualphas + pot4 assembly in one 2.5-D pass;
+// results equal those of alpha_div_xy followed by alpha_div_z_pot4
+int alpha_div25_pot4(hedft_ctx* c);
+// 1 when the grid takes the 2.5-D tiles (lap25_geom)
+int div25_ok(const hedft_ctx* c);
 // x+y Laplacian of psi into out (default c->lap); the z pass is the fused step kernel
 int lap_xy(hedft_ctx* c, const void* psi, int is_complex, void* out = nullptr);
 }  // namespace hedft

@@ -1112,11 +1112,17 @@ static int stage_b(hedft_ctx* c) {
 
 // Stage C: ualphas' convolution F^-1[kalfs F(grad(den).int)] (term_alfa.f90:61-73),
 // then the divergence term and the pot4 assembly (term_alfa.f90:80-98, poten.f90:194-211)
+// the single-stream mean field (tuning field `fused`): with the 2.5-D steppers
+// on grids the 2.5-D tiles divide
+static bool fused_mf(const hedft_ctx* c) { return c->fused && c->lap25_tz > 0 && div25_ok(c); }
+
 template <typename R>
 static int stage_c(hedft_ctx* c) {
+  const bool fused = fused_mf(c);
   // side branch: x+y divergence (denalf, int_x, int_y -> work1); main branch:
-  // the convolution (grad den, int -> spec[0] -> ualph_conv)
-  HEDFT_TRY(fork_point(c));
+  // the convolution (grad den, int -> spec[0] -> ualph_conv).  Fused: the
+  // divergence is one 2.5-D pass after the convolution, on the main stream.
+  if (!fused) HEDFT_TRY(fork_point(c));
   XInT<R> dot{};
   dot.op = XIN_DOT3;
   dot.a[0] = fld<R>(c->dxden);
@@ -1127,8 +1133,8 @@ static int stage_c(hedft_ctx* c) {
   dot.a[5] = fld<R>(c->intz);
   cplx_t<R>* s0 = spc<R>(c->spec[0]);
   HEDFT_TRY(fwd_xy<R>(c, 1, &dot, &s0));
-  HEDFT_TRY(fork_side(c));
-  {
+  if (!fused) {
+    HEDFT_TRY(fork_side(c));
     SideBranch sb(c);
     HEDFT_TRY(alpha_div_xy(c));
   }
@@ -1145,6 +1151,7 @@ static int stage_c(hedft_ctx* c) {
   }
   c->ndefer = 0;
   HEDFT_TRY(inv_yx<R>(c, nu, su, xu));
+  if (fused) return alpha_div25_pot4(c);
   HEDFT_TRY(join(c));
   return alpha_div_z_pot4(c);
 }
@@ -1245,7 +1252,9 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   // Laplacians are joined with stage C's branch, or after stage B).  With the
   // 2.5-D steppers there is no Laplacian pass; lap_late 4 still lets stage B
   // wait only for grad(den).
-  const bool mid = (with_lap || c->lap25_tz > 0) && c->lap_late == 4 && !c->no_side;
+  // (with the fused stage C no join follows stage B, so the side branch is
+  // joined whole before it)
+  const bool mid = (with_lap || c->lap25_tz > 0) && c->lap_late == 4 && !c->no_side && !fused_mf(c);
   if (c->lap25_tz > 0) with_lap = 0;
   const bool late = with_lap && (c->lap_late == 1 || (c->lap_late == 2 && want_force));
   const bool on_main = with_lap && c->lap_late == 3;
@@ -2049,6 +2058,10 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   c->mixed = (p->precision == HEDFT_PREC_MIXED);
   c->zconv_fused = 1;
   if (const char* e = std::getenv("HEDFT_ZCONV_FUSED")) c->zconv_fused = std::atoi(e);
+  // measured at 128^3 FP64: the fused stage C is 2 % slower (its 2.5-D pass
+  // takes 47-50 us against 35 us for the z pass), so it is off by default
+  c->fused = 0;
+  if (const char* e = std::getenv("HEDFT_FUSED")) c->fused = std::atoi(e);
   // measured at 128^3: on the side stream after grad(den) for double storage,
   // on the main stream after the den convolutions for float storage
   c->lap_late = c->mixed ? 3 : 4;
@@ -2136,6 +2149,7 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   rc = rc ? rc : alloc(c, (void**)&c->lstop, sizeof(int));
   rc = rc ? rc : alloc(c, (void**)&c->lamtab, sizeof(double) * (d.nx + d.ny + d.nz));
   rc = rc ? rc : alloc(c, (void**)&c->coef2_dev, 3 * sizeof(StencilCoef));
+  rc = rc ? rc : alloc(c, (void**)&c->coef1_dev, 3 * sizeof(StencilCoef));
   if (rc == HEDFT_OK && hipHostMalloc((void**)&c->host_scal, NSCAL * sizeof(double)) != hipSuccess) rc = HEDFT_ENOMEM;
   if (rc == HEDFT_OK) rc = fft_setup(c);
   if (rc != HEDFT_OK) {
@@ -2144,7 +2158,9 @@ int hedft_create(const hedft_grid* g, const hedft_params* p, int32_t mode, int32
   }
   stencil_setup(c);
   if (hipMemcpyAsync(c->coef2_dev, c->coef[1], 3 * sizeof(StencilCoef), hipMemcpyHostToDevice, c->stream) !=
-      hipSuccess) {
+          hipSuccess ||
+      hipMemcpyAsync(c->coef1_dev, c->coef[0], 3 * sizeof(StencilCoef), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess) {
     hedft_destroy(c);
     return HEDFT_EHIP;
   }
@@ -2197,7 +2213,7 @@ int hedft_destroy(hedft_ctx* c) {
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
                   c->imp, c->lstop, c->potion, c->psix, c->psixold[0], c->psixold[1], c->psixold[2],
                   c->hpsixold[0], c->hpsixold[1], c->pcx, c->qx, c->lapx, c->potx4, c->upotx, c->uimpk,
-                  c->specq[0], c->specq[1], c->specd, c->coef2_dev};
+                  c->specq[0], c->specq[1], c->specd, c->coef2_dev, c->coef1_dev};
   for (void* q : ptrs)
     if (q) hipFree(q);
   clear_pairs(c);
@@ -3138,12 +3154,14 @@ static void apply_tuning(hedft_ctx* c, const hedft_tuning* t) {
   c->prefer_eager = t->prefer_eager;
   c->lap_late = t->lap_late;
   c->lap25_tz = t->lap25;
+  c->fused = t->fused;
   c->lap_fresh = 0;    // the spare psi buffer does not hold the x+y Laplacian
   c->graph_valid = 0;  // the captured graph encodes the launch structure
 }
 
 int hedft_set_tuning(hedft_ctx* c, const hedft_tuning* t) {
-  if (!c || !t || t->xy_split_fwd < 0 || t->xy_split_inv < 0 || t->lap25 < 0) return HEDFT_EINVAL;
+  if (!c || !t || t->xy_split_fwd < 0 || t->xy_split_inv < 0 || t->lap25 < 0 || t->fused < 0 || t->fused > 1)
+    return HEDFT_EINVAL;
   if (t->lap25 > 0 && (c->mode != HEDFT_MODE_TDDFT || lap25_geom(c->d, t->lap25).nblocks == 0)) return HEDFT_EGRID;
   apply_tuning(c, t);
   return HEDFT_OK;
@@ -3157,6 +3175,7 @@ int hedft_get_tuning(const hedft_ctx* c, hedft_tuning* t) {
   t->prefer_eager = c->prefer_eager;
   t->lap_late = c->lap_late;
   t->lap25 = c->lap25_tz;
+  t->fused = c->fused;
   t->ms_per_step = 0.0;
   return HEDFT_OK;
 }
@@ -3179,6 +3198,7 @@ int hedft_autotune(hedft_ctx* c, double deltat, int32_t trial_steps, hedft_tunin
           cur.prefer_eager = eg;
           cur.lap_late = ll;
           cur.lap25 = c->lap25_tz;
+          cur.fused = c->fused;
           apply_tuning(c, &cur);
           HEDFT_TRY(enqueue_pc(c, deltat, PC_PERIOD, 1));  // warm-up incl. graph capture
           HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));

@@ -214,12 +214,11 @@ struct OutDivPot4 {
   double den0s, halpha, a0, a1;
   int lext;
   __device__ __forceinline__ void store(size_t i, int, int, int, double v, double*) const {
-    const double ua = ((double)conv[i] / den0s + ((double)acc[i] + v)) * halpha;
-    ualphas[i] = (R)ua;
-    const double cg = dencg[i];
-    double p = (double)delj4[i] + cg * cg * (a0 + a1 * cg) + (double)corr[i];
-    if (lext) p += ext[i];
-    pot4[i] = (R)(p + ua);
+    const AlphaPot4 r = alpha_pot4((double)conv[i], den0s, (double)acc[i] + v, halpha, (double)delj4[i],
+                                   (double)dencg[i], (double)corr[i], a0, a1, lext != 0,
+                                   lext ? (double)ext[i] : 0.0);
+    ualphas[i] = (R)r.ua;
+    pot4[i] = (R)r.pot4;
   }
 };
 
@@ -262,6 +261,39 @@ static int alpha_div_z_pot4_t(hedft_ctx* c) {
 
 int alpha_div_z_pot4(hedft_ctx* c) {
   return c->mixed ? alpha_div_z_pot4_t<float>(c) : alpha_div_z_pot4_t<double>(c);
+}
+
+int div25_ok(const hedft_ctx* c) { return lap25_geom(c->d, 16).nblocks > 0; }
+
+// the divergence in one 2.5-D pass (z chunks of 16 planes), with the same
+// assembly as alpha_div_z_pot4
+template <typename R>
+static int alpha_div25_pot4_t(hedft_ctx* c) {
+  auto F = [](double* p) { return reinterpret_cast<R*>(p); };
+  Div25Args<R> a;
+  a.denalf = F(c->denalf);
+  a.ix = F(c->intx);
+  a.iy = F(c->inty);
+  a.iz = F(c->intz);
+  a.conv = F(c->ualph_conv);
+  a.delj4 = F(c->delj4);
+  a.dencg = F(c->dencg);
+  a.corr = F(c->sto1);
+  a.ext = F(pot4_ext(c));
+  a.ualphas = F(c->ualphas);
+  a.pot4 = F(c->pot4);
+  a.den0s = c->p.den0s;
+  a.halpha = c->p.h2o2m4 * c->p.alphas;
+  a.a0 = c->p.cp4 / 2.0;
+  a.a1 = c->p.cpp4 / 3.0;
+  a.lext = pot4_ext(c) != nullptr;
+  launch_div25<R>(c->stream, c->d, c->coef1_dev, a, 16);
+  return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
+}
+
+int alpha_div25_pot4(hedft_ctx* c) {
+  if (!div25_ok(c)) return HEDFT_EGRID;
+  return c->mixed ? alpha_div25_pot4_t<float>(c) : alpha_div25_pot4_t<double>(c);
 }
 
 int lap_xy(hedft_ctx* c, const void* psi, int is_complex, void* out) {

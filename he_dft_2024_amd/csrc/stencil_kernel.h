@@ -915,4 +915,228 @@ struct OutStore {
   }
 };
 
+// ---- 2.5-D divergence of f int_a with the mean-field assembly ---------------
+// div = d_x(f int_x) + d_y(f int_y) + d_z(f int_z), f = 1 - denalf/den0s
+// (term_alfa.f90:80-85), then ualphas = (conv/den0s + div) h2o2m4 alphas and
+// pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ ext] + ualphas
+// (term_alfa.f90:86-98, Dynamics_TDDFT/poten.f90:194-211).  Tiling as k_lap25:
+// a workgroup owns TX x TY (x, y) columns over a chunk of at most tz planes;
+// each thread keeps the 13-plane z window of f int_z (storage type R) in
+// registers, and each plane of f int_x (with 6-column x halos) and f int_y
+// (with 6-row y halos) is staged in LDS, double-buffered.  The operands are
+// rounded to R where the separate passes stage them, dx + dy is rounded to R
+// as the x+y pass stores it, and every sum keeps its order: the results equal
+// those of k_pencil_xy2<InFInt2, OutSumXY> followed by k_pencil<InFInt,
+// OutDivPot4>.  Requires nx % TX == 0, ny % TY == 0, TX, TY >= 7, n >= 13.
+// The mean-field assembly at a point, shared by k_div25 and the z pass of the
+// two-stream mean field so that both round identically (no contraction into
+// fma is left to the compiler, which could fuse differently in the two
+// kernels): ualphas = (conv/den0s + div) halpha and
+// pot4 = delj4 + dencg^2 (cp4/2 + cpp4/3 dencg) + corr [+ ext] + ualphas.
+struct AlphaPot4 {
+  double ua, pot4;
+};
+__device__ __forceinline__ AlphaPot4 alpha_pot4(double conv, double den0s, double div, double halpha,
+                                                double delj4, double cg, double corr, double a0, double a1,
+                                                bool lext, double ext) {
+#pragma clang fp contract(off)
+  const double ua = (conv / den0s + div) * halpha;
+  double p = delj4 + cg * cg * (a0 + a1 * cg) + corr;
+  if (lext) p += ext;
+  return AlphaPot4{ua, p + ua};
+}
+
+template <typename R>
+struct Div25Args {
+  const R *denalf, *ix, *iy, *iz;
+  const R *conv, *delj4, *dencg, *corr, *ext;
+  R *ualphas, *pot4;
+  double den0s, halpha, a0, a1;
+  int lext;
+};
+
+template <int TX, int TY, typename R>
+__global__ void __launch_bounds__(TX * TY) k_div25(Div25Args<R> a, const StencilCoef* __restrict__ cf3, Dims d,
+                                                  int tz) {
+  static_assert(TX >= 7 && TY >= 7, "edge tiles must hold 13 rows");
+  constexpr int NT = TX * TY, TW = TX + 12, TH = TY + 12;
+  // LDS per buffer: the x operand (TY rows x TW, column 0 <-> x0 - 6), then
+  // the y operand (TH rows x TX, row 0 <-> y0 - 6)
+  constexpr int XS = TY * TW, BS = XS + TH * TX;
+  // halo slots: the y operand's 6-row bands above and below the tile, then
+  // the x operand's 6-column bands left and right of it
+  constexpr int NHALO = 12 * TX + 12 * TY, HU = (NHALO + NT - 1) / NT;
+  __shared__ R buf[2][BS];
+  __shared__ StencilCoef cfs[3];
+  static_assert(sizeof(StencilCoef) % 8 == 0, "copied as doubles");
+  for (int e = threadIdx.x; e < (int)(3 * sizeof(StencilCoef) / 8); e += NT)
+    reinterpret_cast<double*>(cfs)[e] = reinterpret_cast<const double*>(cf3)[e];
+  __syncthreads();
+  const StencilCoef& cfx = cfs[0];
+  const StencilCoef& cfy = cfs[1];
+  const StencilCoef& cfz = cfs[2];
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int ntx = nx / TX, nty = ny / TY, nzc = (nz + tz - 1) / tz;
+  // XCD-aware order (as k_lap25)
+  const int ntiles = ntx * nty * nzc;
+  int tile = blockIdx.x;
+  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+  const int x0 = (tile % ntx) * TX, y0 = ((tile / ntx) % nty) * TY, zc = tile / (ntx * nty);
+  const int z0 = zc * tz, z1 = min(z0 + tz, nz);
+  const int xl = threadIdx.x % TX, yl = threadIdx.x / TX;
+  const int gx = x0 + xl, gy = y0 + yl;
+  const size_t plane = (size_t)nx * ny;
+  const size_t col = (size_t)gy * nx + gx;
+  const int xpos = yl * TW + xl + 6, ypos = XS + (yl + 6) * TX + xl;
+  const double den0s = a.den0s;
+  // f int at index i of one operand, rounded to R as the separate passes stage it
+  auto fint = [&](const R* ia, size_t i) { return (R)((1.0 - (double)a.denalf[i] / den0s) * ia[i]); };
+  const double sgx = (cfx.mode == 2) ? -1.0 : 1.0, sgy = (cfy.mode == 2) ? -1.0 : 1.0;
+  const double sgz = (cfz.mode == 2) ? -1.0 : 1.0;
+  int hoff[HU];
+  size_t hsrc[HU];
+  double hsg[HU];
+  const R* hop[HU];
+#pragma unroll
+  for (int u = 0; u < HU; ++u) {
+    const int k = min((int)threadIdx.x + u * NT, NHALO - 1);
+    double sg = 1.0;
+    int xx, yy, off;
+    if (k < 12 * TX) {  // y operand: rows 0..5 and TY+6..TY+11
+      const int r = (k < 6 * TX) ? k / TX : TY + 6 + (k - 6 * TX) / TX, c = k % TX;
+      yy = y0 - 6 + r;
+      xx = x0 + c;
+      if (yy < 0 || yy >= ny) {
+        if (cfy.mode != 0) {
+          sg = sgy;
+          yy += (yy < 0) ? ny : -ny;
+        } else {
+          yy = min(max(yy, 0), ny - 1);
+        }
+      }
+      off = XS + r * TX + c;
+      hop[u] = a.iy;
+    } else {  // x operand: columns 0..5 and TX+6..TX+11
+      const int k3 = k - 12 * TX, r = k3 / 12, cc = k3 % 12, c = (cc < 6) ? cc : TX + cc;
+      yy = y0 + r;
+      xx = x0 - 6 + c;
+      if (xx < 0 || xx >= nx) {
+        if (cfx.mode != 0) {
+          sg = sgx;
+          xx += (xx < 0) ? nx : -nx;
+        } else {
+          xx = min(max(xx, 0), nx - 1);
+        }
+      }
+      off = r * TW + c;
+      hop[u] = a.ix;
+    }
+    hoff[u] = ((int)threadIdx.x + u * NT < NHALO) ? off : -1;
+    hsrc[u] = (size_t)yy * nx + xx;
+    hsg[u] = sg;
+  }
+  const bool clampz = cfz.mode == 0;
+  auto qbase = [&](int z) { return clampz ? min(max(z - 6, 0), nz - 13) : z - 6; };
+  auto zwrap = [&](int u, double& f) {
+    f = 1.0;
+    if (u < 0) {
+      f = sgz;
+      return u + nz;
+    }
+    if (u >= nz) {
+      f = sgz;
+      return u - nz;
+    }
+    return u;
+  };
+  R q[13];
+  int b = qbase(z0);
+#pragma unroll
+  for (int j = 0; j < 13; ++j) {
+    double f;
+    const int zz = zwrap(b + j, f);
+    q[j] = vmul(f, fint(a.iz, (size_t)zz * plane + col));
+  }
+  // plane z0 of the tile into buffer 0
+  {
+    const size_t p0 = (size_t)z0 * plane;
+    R hv[HU];
+#pragma unroll
+    for (int u = 0; u < HU; ++u) hv[u] = fint(hop[u], p0 + hsrc[u]);
+    buf[0][xpos] = fint(a.ix, p0 + col);
+    buf[0][ypos] = fint(a.iy, p0 + col);
+#pragma unroll
+    for (int u = 0; u < HU; ++u)
+      if (hoff[u] >= 0) buf[0][hoff[u]] = vmul(hsg[u], hv[u]);
+  }
+  // the assembly's operands at the point, loaded one step ahead (ext is read
+  // from conv's address when there is no external term)
+  const R* ext = a.lext ? a.ext : a.conv;
+  struct Ops {
+    double cv, dj, cg, co, ex;
+  };
+  auto load_ops = [&](size_t i) {
+    return Ops{(double)a.conv[i], (double)a.delj4[i], (double)a.dencg[i], (double)a.corr[i], (double)ext[i]};
+  };
+  Ops ops = load_ops((size_t)z0 * plane + col);
+  for (int z = z0; z < z1; ++z) {
+    const int zn = z + 1;
+    const bool more = zn < z1;
+    const int bn = more ? qbase(zn) : b;
+    const size_t pn = (size_t)min(zn, nz - 1) * plane;
+    R hv[HU];
+#pragma unroll
+    for (int u = 0; u < HU; ++u) hv[u] = fint(hop[u], pn + hsrc[u]);
+    const R cxn = fint(a.ix, pn + col), cyn = fint(a.iy, pn + col);
+    double nqf;
+    const int nqz = zwrap(bn + 12, nqf);
+    const R nqr = fint(a.iz, (size_t)nqz * plane + col);
+    __syncthreads();
+    const R* T = buf[(z - z0) & 1];
+    const double dx = stencil_tile<R>(cfx, T + yl * TW, 1, gx, nx, x0);
+    const double dy = stencil_tile<R>(cfy, T + XS + xl, TX, gy, ny, y0);
+    const int k = z - b;
+    double dz = 0.0;
+    if (k == 6) {
+#pragma unroll
+      for (int j = 0; j < 13; ++j) dz = vfma(cfz.c[j], q[j], dz);
+    } else {
+      const int r = (z < 6) ? z : z - nz + 12;
+#pragma unroll
+      for (int j = 0; j < 13; ++j) dz = vfma(cfz.rows[r][j], q[j], dz);
+    }
+    if (more) {
+      R* Tn = buf[(zn - z0) & 1];
+      Tn[xpos] = cxn;
+      Tn[ypos] = cyn;
+#pragma unroll
+      for (int u = 0; u < HU; ++u)
+        if (hoff[u] >= 0) Tn[hoff[u]] = vmul(hsg[u], hv[u]);
+    }
+    // OutSumXY's rounding of dx + dy, then OutDivPot4's arithmetic
+    const double sxy = (double)vround<R>(dx + dy);
+    const size_t i = (size_t)z * plane + col;
+    const AlphaPot4 r = alpha_pot4(ops.cv, den0s, sxy + dz, a.halpha, ops.dj, ops.cg, ops.co, a.a0, a.a1,
+                                   a.lext != 0, ops.ex);
+    a.ualphas[i] = (R)r.ua;
+    a.pot4[i] = (R)r.pot4;
+    ops = load_ops(pn + col);
+    if (bn != b) {
+#pragma unroll
+      for (int j = 0; j < 12; ++j) q[j] = q[j + 1];
+      q[12] = vmul(nqf, nqr);
+      b = bn;
+    }
+  }
+}
+
+template <typename R>
+inline void launch_div25(hipStream_t s, const Dims& d, const StencilCoef* cf3, const Div25Args<R>& a, int tz) {
+  const Lap25Geom g = lap25_geom(d, tz);
+  if (g.tx == 32)
+    hipLaunchKernelGGL((k_div25<32, 8, R>), dim3(g.nblocks), dim3(256), 0, s, a, cf3, d, tz);
+  else
+    hipLaunchKernelGGL((k_div25<16, 16, R>), dim3(g.nblocks), dim3(256), 0, s, a, cf3, d, tz);
+}
+
 }  // namespace hedft

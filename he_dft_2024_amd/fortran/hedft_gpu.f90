@@ -60,7 +60,7 @@ module hedft_gpu
 
   type, bind(C) :: hedft_tuning_t
     integer(c_int32_t) :: xy_split_fwd, xy_split_inv, no_side, prefer_eager
-    integer(c_int32_t) :: lap_late, lap25
+    integer(c_int32_t) :: lap_late, lap25, fused
     real(c_double) :: ms_per_step
   end type hedft_tuning_t
 

@@ -98,7 +98,7 @@ class DenFile(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("xy_split_fwd", C.c_int32), ("xy_split_inv", C.c_int32), ("no_side", C.c_int32),
                 ("prefer_eager", C.c_int32), ("lap_late", C.c_int32), ("lap25", C.c_int32),
-                ("ms_per_step", C.c_double)]
+                ("fused", C.c_int32), ("ms_per_step", C.c_double)]
 
 
 class HedftError(RuntimeError):
@@ -435,11 +435,13 @@ class Droplet:
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def set_tuning(self, t):
-        """Apply a tuning dict; a key left out keeps 0, except lap25, which
-        keeps the context's current value."""
-        lap25 = int(t["lap25"]) if "lap25" in t else self.get_tuning()["lap25"]
+        """Apply a tuning dict; a key left out keeps 0, except lap25 and
+        fused, which keep the context's current values."""
+        cur = self.get_tuning()
+        lap25 = int(t["lap25"]) if "lap25" in t else cur["lap25"]
+        fused = int(t["fused"]) if "fused" in t else cur["fused"]
         tt = Tuning(*[int(t.get(k, 0)) for k in ("xy_split_fwd", "xy_split_inv", "no_side", "prefer_eager",
-                                                 "lap_late")], lap25, 0.0)
+                                                 "lap_late")], lap25, fused, 0.0)
         check(lib().hedft_set_tuning(self._c, C.byref(tt)), "set_tuning")
 
     def get_tuning(self):

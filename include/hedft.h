@@ -479,6 +479,13 @@ typedef struct hedft_tuning {
                                 0 runs the separate x+y Laplacian pass (kept
                                 for comparison).  Fields are identical; errHe
                                 sums its blocks in another order. */
+  int32_t fused;             /* 1: with lap25 > 0, the mean field's
+                                alpha_s divergence (x, y and z) and the pot4
+                                assembly are one 2.5-D pass on the main
+                                stream, so stage C has no side branch; 0: the
+                                x+y divergence on the side stream beside
+                                stage C, then a z pass (the default: 2 %
+                                faster at 128^3).  Fields are identical. */
   double ms_per_step;        /* measured time of the winning choice (autotune) */
 } hedft_tuning;
 /* Times trial_steps PC iterations (rounded up to 6, after a 6-iteration
