@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libhedft.so")
-SOURCES = ["fft.hip", "stencil.hip", "hedft.hip", "impurity.hip", "serve.hip", "denio.cpp"]
+SOURCES = ["fft.hip", "stencil.hip", "meanfield.hip", "steppers.hip", "statics.hip", "observables.hip", "hedft.hip",
+           "impurity.hip", "serve.hip", "denio.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HEDFT_ARCH", "gfx950")
 FLAGS = [
