@@ -13,8 +13,9 @@
  *   mean field He_statics/poten.f90, Dynamics_TDDFT/poten.f90, term_alfa.f90, derden.f90
  *   steppers   He_statics/steprkr.f90, steppcr.f90; Dynamics_TDDFT/steprk.f90, steppc.f90
  *   impurity   Dynamics_TDDFT/potenimp.f90, potenimpini.f90
- *   quantum    Dynamics_quantum_impurities/potenimp.f90, poten.f90, steprk.f90,
- *              steppc.f90, energy.f90, DFT4He3d.f90:617,682-700 (one impurity)
+ *   quantum    Dynamics_quantum_impurities/potenimp.f90, potenimpini.f90, poten.f90,
+ *              steprk.f90, steppc.f90, energy.f90, DFT4He3d.f90:617,682-700
+ *              (N_imp wave functions with the uimp_k_k pair kernels)
  *   energies   He_statics/energy.f90, Dynamics_TDDFT/energy.f90
  * The FFT is this file's own mixed-radix DFT (the reference links MKL's FFTW3,
  * which is not available here); the sine integral uses the series /
@@ -74,6 +75,16 @@ enum {
   F_PSIXOLD3, F_HPSIXOLD1, F_HPSIXOLD2, F_PCX, F_QX, F_PSIL, F_NFIELDS
 };
 
+#define MAXQ 4
+/* one quantum impurity's state (Dynamics_quantum_impurities, module quantal_imp) */
+typedef struct {
+  double h2o2mimp;
+  double *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *hpsix; /* complex N (real N in statics) */
+  double *upotx, *denx;                                          /* real N */
+  double *psi1x, *psi2x;   /* statics (He_statics, limp): psix one and two steps back */
+  double* uimpk;           /* uimp_k(:,:,:,i): real half spectrum */
+} qimp;
+
 typedef struct {
   ogrid g;
   oparams p;
@@ -109,14 +120,15 @@ typedef struct {
   int npair;
   double delpair;
   double *pairv, *pairdv;
-  /* one quantum impurity (module quantal_imp, N_imp = 1) */
-  int quantum;
-  double h2o2mimp, mu4;
-  double *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *hpsix; /* complex N */
-  double *potx4, *upotx, *denx;                                  /* real N */
-  double *psi1x, *psi2x;   /* statics (He_statics, limp): psix one and two steps back */
-  cplx* fdenx;             /* statics: F(denx) as the driver leaves it */
-  double *uimpk;                                                 /* real half spectrum */
+  /* quantum impurities (module quantal_imp): quantum = N_imp wave functions
+   * (TDDFT, up to MAXQ; one in statics), each with its own state; qsel picks
+   * the one the field access and the centroid refer to */
+  int quantum, qsel;
+  qimp qi[MAXQ];
+  double* uimpkk[MAXQ][MAXQ]; /* uimp_k_k(:,:,:,i,j), NULL where Selec_gs_k_k is 'null' or i = j */
+  double mu4;
+  double* potx4;              /* real N */
+  cplx* fdenx;                /* statics: F(denx) as the driver leaves it */
   double errpsix;
 } octx;
 
@@ -565,16 +577,22 @@ void oracle_destroy(octx* c) {
                     c->psiold[1], c->psiold[2], c->hpsiold[0], c->hpsiold[1], c->psi1s, c->psi2s, c->psil, c->workc1,
                     (double*)c->fden, (double*)c->wk1, (double*)c->wk2, (double*)c->wk3,
                     (double*)c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->pmod, c->potion,
-                    c->psix, c->psixold[0], c->psixold[1], c->psixold[2], c->hpsixold[0],
-                    c->hpsixold[1], c->pcx, c->qx, c->hpsix, c->potx4, c->upotx, c->denx, c->uimpk,
-                    c->pairv, c->pairdv, c->psi1x, c->psi2x, (double*)c->fdenx};
+                    c->potx4, c->pairv, c->pairdv, (double*)c->fdenx};
   for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); ++i) free(list[i]);
+  for (int k = 0; k < MAXQ; ++k) {
+    qimp* q = &c->qi[k];
+    double* ql[] = {q->psix, q->psixold[0], q->psixold[1], q->psixold[2], q->hpsixold[0], q->hpsixold[1],
+                    q->pcx, q->qx, q->hpsix, q->upotx, q->denx, q->psi1x, q->psi2x, q->uimpk};
+    for (size_t i = 0; i < sizeof(ql) / sizeof(ql[0]); ++i) free(ql[i]);
+    for (int m = 0; m < MAXQ; ++m) free(c->uimpkk[k][m]);
+  }
   free(c);
 }
 
 /* pointer + element count (doubles) of a field, for the Python binding */
 double* oracle_field(octx* c, int id, size_t* ndoubles) {
   size_t n = c->n, nh = c->nh;
+  qimp* Q = &c->qi[c->qsel];
   int cw = (c->mode == MODE_TDDFT) ? 2 : 1;
   double* ptr = NULL; size_t cnt = n;
   switch (id) {
@@ -607,15 +625,15 @@ double* oracle_field(octx* c, int id, size_t* ndoubles) {
     case F_WORK2: ptr = c->work2; break;
     case F_WORKC1: ptr = c->workc1; cnt = 2 * n; break;
     case F_WORKK1: ptr = (double*)c->workk1; cnt = 2 * nh; break;
-    case F_PSIX: ptr = c->psix; cnt = (c->mode == MODE_STATICS) ? n : 2 * n; break;
+    case F_PSIX: ptr = Q->psix; cnt = (c->mode == MODE_STATICS) ? n : 2 * n; break;
     case F_POTX4: ptr = c->potx4; break;
-    case F_UPOTX: ptr = c->upotx; break;
-    case F_DENX: ptr = c->denx; break;
-    case F_UIMPK: ptr = c->uimpk; cnt = nh; break;
-    case F_PSIXOLD1: case F_PSIXOLD2: case F_PSIXOLD3: ptr = c->psixold[id - F_PSIXOLD1]; cnt = 2 * n; break;
-    case F_HPSIXOLD1: case F_HPSIXOLD2: ptr = c->hpsixold[id - F_HPSIXOLD1]; cnt = 2 * n; break;
-    case F_PCX: ptr = c->pcx; cnt = 2 * n; break;
-    case F_QX: ptr = c->qx; cnt = 2 * n; break;
+    case F_UPOTX: ptr = Q->upotx; break;
+    case F_DENX: ptr = Q->denx; break;
+    case F_UIMPK: ptr = Q->uimpk; cnt = nh; break;
+    case F_PSIXOLD1: case F_PSIXOLD2: case F_PSIXOLD3: ptr = Q->psixold[id - F_PSIXOLD1]; cnt = 2 * n; break;
+    case F_HPSIXOLD1: case F_HPSIXOLD2: ptr = Q->hpsixold[id - F_HPSIXOLD1]; cnt = 2 * n; break;
+    case F_PCX: ptr = Q->pcx; cnt = 2 * n; break;
+    case F_QX: ptr = Q->qx; cnt = 2 * n; break;
     case F_PSIL: ptr = c->psil; break;
     default: ptr = NULL; cnt = 0;
   }
@@ -720,9 +738,9 @@ static void poten_tail(octx* c) {
   }
   if (c->mode == MODE_STATICS && c->quantum) {
     /* He_statics/poten.f90:94-107: upotx = F^-1[fden vq], potx4 = F^-1[fdenx vq] */
-    kmul(c, c->fden, c->uimpk, c->wk1);
-    kmul(c, c->fdenx, c->uimpk, c->wk2);
-    fftbk(c, c->wk1, c->upotx);
+    kmul(c, c->fden, c->qi[0].uimpk, c->wk1);
+    kmul(c, c->fdenx, c->qi[0].uimpk, c->wk2);
+    fftbk(c, c->wk1, c->qi[0].upotx);
     fftbk(c, c->wk2, c->potx4);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)n; ++i) c->pot4[i] += c->potx4[i];
@@ -1200,58 +1218,110 @@ void oracle_update_aimp(octx* c) {
 }
 
 /* ======================================================================= */
-/*  Quantum impurity (Dynamics_quantum_impurities, N_imp = 1)              */
+/*  Quantum impurities (Dynamics_quantum_impurities, N_imp wave functions)  */
 /* ======================================================================= */
-/* uimp_k(:,:,:,1) as potenimpini.f90:19-29 leaves it; h2o2mimp as steprk.f90:86 */
-void oracle_set_quantum_impurity(octx* c, const double* uimpk, double m_imp_u, double mu4) {
-  size_t n = c->n;
+/* potenimpini.f90:19-44 leaves uimp_k(:,:,:,i) and uimp_k_k(:,:,:,i,j) (zero
+ * for i = j and for 'null' pairs); h2o2mimp(i) as steprk.f90:86.  uimpkk
+ * holds nq*nq half spectra in the order (i, j) (row i = the impurity that
+ * feels the potential), pair[i*nq+j] = 0 marks a 'null' pair; both may be
+ * NULL for a single impurity. */
+void oracle_set_quantum_impurities(octx* c, int nq, const double* uimpk, const double* uimpkk, const int* pair,
+                                   const double* m_imp_u, double mu4) {
+  size_t n = c->n, nh = c->nh;
   if (c->mode == MODE_STATICS) {
-    /* He_statics with limp: real psix, vq (BCN4HeDFT.f90:738-791), h2o2mx
+    /* He_statics with limp: one real psix, vq (BCN4HeDFT.f90:738-791), h2o2mx
      * (BCN4HeDFT.f90:537) */
+    qimp* q = &c->qi[0];
     if (!c->quantum) {
-      c->psix = dalloc(n); c->psi1x = dalloc(n); c->psi2x = dalloc(n); c->hpsix = dalloc(n);
-      c->potx4 = dalloc(n); c->upotx = dalloc(n); c->denx = dalloc(n); c->uimpk = dalloc(c->nh);
-      c->fdenx = (cplx*)dalloc(2 * c->nh);
+      q->psix = dalloc(n); q->psi1x = dalloc(n); q->psi2x = dalloc(n); q->hpsix = dalloc(n);
+      c->potx4 = dalloc(n); q->upotx = dalloc(n); q->denx = dalloc(n); q->uimpk = dalloc(nh);
+      c->fdenx = (cplx*)dalloc(2 * nh);
     }
-    memcpy(c->uimpk, uimpk, sizeof(double) * c->nh);
-    c->h2o2mimp = c->p.h2o2m4 * (4.002602 / m_imp_u);
+    memcpy(q->uimpk, uimpk, sizeof(double) * nh);
+    q->h2o2mimp = c->p.h2o2m4 * (4.002602 / m_imp_u[0]);
     c->mu4 = 0.0;
     c->quantum = 1;
     return;
   }
-  if (!c->quantum) {
-    c->psix = dalloc(2 * n); c->pcx = dalloc(2 * n); c->qx = dalloc(2 * n); c->hpsix = dalloc(2 * n);
-    for (int k = 0; k < 3; ++k) c->psixold[k] = dalloc(2 * n);
-    for (int k = 0; k < 2; ++k) c->hpsixold[k] = dalloc(2 * n);
-    c->potx4 = dalloc(n); c->upotx = dalloc(n); c->denx = dalloc(n); c->uimpk = dalloc(c->nh);
+  if (!c->potx4) c->potx4 = dalloc(n);
+  for (int k = 0; k < nq; ++k) {
+    qimp* q = &c->qi[k];
+    if (!q->psix) {
+      q->psix = dalloc(2 * n); q->pcx = dalloc(2 * n); q->qx = dalloc(2 * n); q->hpsix = dalloc(2 * n);
+      for (int s = 0; s < 3; ++s) q->psixold[s] = dalloc(2 * n);
+      for (int s = 0; s < 2; ++s) q->hpsixold[s] = dalloc(2 * n);
+      q->upotx = dalloc(n); q->denx = dalloc(n); q->uimpk = dalloc(nh);
+    }
+    memcpy(q->uimpk, uimpk + (size_t)k * nh, sizeof(double) * nh);
+    q->h2o2mimp = c->p.h2o2m4 * 4.002603 / m_imp_u[k];
+    for (int m = 0; m < MAXQ; ++m) {
+      free(c->uimpkk[k][m]);
+      c->uimpkk[k][m] = NULL;
+      if (m < nq && m != k && uimpkk && pair && pair[k * nq + m]) {
+        c->uimpkk[k][m] = dalloc(nh);
+        memcpy(c->uimpkk[k][m], uimpkk + ((size_t)k * nq + m) * nh, sizeof(double) * nh);
+      }
+    }
   }
-  memcpy(c->uimpk, uimpk, sizeof(double) * c->nh);
-  c->h2o2mimp = c->p.h2o2m4 * 4.002603 / m_imp_u;
   c->mu4 = mu4;
-  c->quantum = 1;
+  c->quantum = nq;
+  c->qsel = 0;
 }
 
-/* denx = psix*conjg(psix) */
-static void denx_from_psix(octx* c) {
-  const cplx* px = (const cplx*)c->psix;
+/* one impurity (N_imp = 1) */
+void oracle_set_quantum_impurity(octx* c, const double* uimpk, double m_imp_u, double mu4) {
+  oracle_set_quantum_impurities(c, 1, uimpk, NULL, NULL, &m_imp_u, mu4);
+}
+
+/* the impurity the field access and the centroid refer to */
+void oracle_select_quantum(octx* c, int k) {
+  if (k >= 0 && k < (c->quantum > 0 ? c->quantum : 1)) c->qsel = k;
+}
+
+/* denx = psix*conjg(psix) of impurity k */
+static void denx_from_psix(octx* c, int k) {
+  const cplx* px = (const cplx*)c->qi[k].psix;
+  double* dx = c->qi[k].denx;
 #pragma omp parallel for schedule(static)
-  for (long i = 0; i < (long)c->n; ++i) c->denx[i] = creal(px[i] * conj(px[i]));
+  for (long i = 0; i < (long)c->n; ++i) dx[i] = creal(px[i] * conj(px[i]));
 }
 
-/* potenimp.f90:20-30 for one impurity: fden = F(den),
- * potx4 = F^-1[F(|psix|^2) uimp_k], upotx = F^-1[fden uimp_k] */
+static void add_to(octx* c, double* acc, const double* v) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)c->n; ++i) acc[i] += v[i];
+}
+
+/* potenimp.f90:13-39: fden = F(den); for each impurity i, potx4 +=
+ * F^-1[F(|psix_i|^2) uimp_k_i], upotx_i = F^-1[fden uimp_k_i] +
+ * sum_{j != i, pair not null} F^-1[F(|psix_j|^2) uimp_k_k(i,j)], every term
+ * transformed back on its own and summed in real space, in that order */
 static void potenimp_quantum(octx* c) {
+  const int nq = c->quantum;
   fftfw(c, c->den, c->fden);
-  denx_from_psix(c);
-  fftfw(c, c->denx, c->wk1);
-  kmul(c, c->wk1, c->uimpk, c->wk2);
-  fftbk(c, c->wk2, c->potx4);
-  kmul(c, c->fden, c->uimpk, c->wk2);
-  fftbk(c, c->wk2, c->upotx);
+  memset(c->potx4, 0, sizeof(double) * c->n);
+  for (int i = 0; i < nq; ++i) {
+    qimp* q = &c->qi[i];
+    denx_from_psix(c, i);
+    fftfw(c, q->denx, c->wk1);
+    kmul(c, c->wk1, q->uimpk, c->wk2);
+    fftbk(c, c->wk2, c->sto2);
+    add_to(c, c->potx4, c->sto2);
+    kmul(c, c->fden, q->uimpk, c->wk2);
+    fftbk(c, c->wk2, q->upotx);
+    for (int j = 0; j < nq; ++j) {
+      if (j == i || !c->uimpkk[i][j]) continue;
+      denx_from_psix(c, j);
+      fftfw(c, c->qi[j].denx, c->wk1);
+      kmul(c, c->wk1, c->uimpkk[i][j], c->wk2);
+      fftbk(c, c->wk2, c->sto2);
+      add_to(c, q->upotx, c->sto2);
+    }
+  }
 }
 
-/* k = timec*((lap psix)*h2o2mimp - upotx*psix)  (Dynamics_quantum_impurities/steprk.f90:94) */
-static void hpsi_x(octx* c, const cplx* psix, cplx* out) {
+/* k = timec*((lap psix)*h2o2mimp - upotx*psix) of impurity q
+ * (Dynamics_quantum_impurities/steprk.f90:94) */
+static void hpsi_x(octx* c, const qimp* q, const cplx* psix, cplx* out) {
   deriv(c, 2, 1, (const double*)psix, c->workc1, 2);
   cplx* lap = (cplx*)dalloc(2 * c->n);
   memcpy(lap, c->workc1, sizeof(cplx) * c->n);
@@ -1260,18 +1330,18 @@ static void hpsi_x(octx* c, const cplx* psix, cplx* out) {
   for (long i = 0; i < (long)c->n; ++i) lap[i] += ((cplx*)c->workc1)[i];
   deriv(c, 2, 3, (const double*)psix, c->workc1, 2);
   const cplx* tc = (const cplx*)c->timec;
-  double h = c->h2o2mimp;
+  double h = q->h2o2mimp;
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)c->n; ++i) {
     cplx l = lap[i] + ((cplx*)c->workc1)[i];
-    out[i] = tc[i] * (l * h - c->upotx[i] * psix[i]);
+    out[i] = tc[i] * (l * h - q->upotx[i] * psix[i]);
   }
   free(lap);
 }
 
 /* norm and centre of |psix|^2 (DFT4He3d.f90:682-700 of the quantum driver) */
 void oracle_quantum_centroid(octx* c, double* norm, double* rcm) {
-  const cplx* px = (const cplx*)c->psix;
+  const cplx* px = (const cplx*)c->qi[c->qsel].psix;
   double s[4] = {0, 0, 0, 0};
   for (int ix = 0; ix < c->nx; ++ix)
     for (int iy = 0; iy < c->ny; ++iy)
@@ -1342,13 +1412,15 @@ void oracle_steprk(octx* c, double deltat) {
       psi[i] = psi[i] + deltat * s1;
     }
     den_from_psi(c);
-    if (c->quantum) { /* the same stage for psix (Dynamics_quantum_impurities/steprk.f90:84-106) */
-      cplx* px = (cplx*)c->psix;
-      cplx* qx = (cplx*)c->qx;
-      cplx* kx = (cplx*)c->hpsix;
-      cplx* pxo[3] = {(cplx*)c->psixold[0], (cplx*)c->psixold[1], (cplx*)c->psixold[2]};
-      cplx* hxo[2] = {(cplx*)c->hpsixold[0], (cplx*)c->hpsixold[1]};
-      hpsi_x(c, px, kx);
+    /* the same stage for each psix (Dynamics_quantum_impurities/steprk.f90:84-106) */
+    for (int qk = 0; qk < c->quantum; ++qk) {
+      const qimp* Q = &c->qi[qk];
+      cplx* px = (cplx*)Q->psix;
+      cplx* qx = (cplx*)Q->qx;
+      cplx* kx = (cplx*)Q->hpsix;
+      cplx* pxo[3] = {(cplx*)Q->psixold[0], (cplx*)Q->psixold[1], (cplx*)Q->psixold[2]};
+      cplx* hxo[2] = {(cplx*)Q->hpsixold[0], (cplx*)Q->hpsixold[1]};
+      hpsi_x(c, Q, px, kx);
 #pragma omp parallel for schedule(static)
       for (long i = 0; i < (long)n; ++i) {
         cplx s1 = a * (kx[i] - b * qx[i]);
@@ -1412,14 +1484,17 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
       ho2[i] = k4[i];
     }
     den_from_psi(c);
-    if (c->quantum) { /* psix predictor (Dynamics_quantum_impurities/steppc.f90:64-95) */
-      cplx* px = (cplx*)c->psix;
-      cplx* pcx = (cplx*)c->pcx;
-      cplx* kx = (cplx*)c->hpsix;
-      cplx* xo3 = (cplx*)c->psixold[c->ioldp[2]];
-      cplx* xh1 = (cplx*)c->hpsixold[c->ioldh[0]];
-      cplx* xh2 = (cplx*)c->hpsixold[c->ioldh[1]];
-      hpsi_x(c, px, kx);
+    /* psix predictors (Dynamics_quantum_impurities/steppc.f90:64-95); ioldpx
+     * and ioldhx rotate with ioldp and ioldh for every impurity */
+    for (int qk = 0; qk < c->quantum; ++qk) {
+      const qimp* Q = &c->qi[qk];
+      cplx* px = (cplx*)Q->psix;
+      cplx* pcx = (cplx*)Q->pcx;
+      cplx* kx = (cplx*)Q->hpsix;
+      cplx* xo3 = (cplx*)Q->psixold[c->ioldp[2]];
+      cplx* xh1 = (cplx*)Q->hpsixold[c->ioldh[0]];
+      cplx* xh2 = (cplx*)Q->hpsixold[c->ioldh[1]];
+      hpsi_x(c, Q, px, kx);
 #pragma omp parallel for schedule(static)
       for (long i = 0; i < (long)n; ++i) {
         cplx s1 = xo3[i] + c4o3 * deltat * (2.0 * kx[i] - xh1[i] + 2.0 * xh2[i]);
@@ -1478,15 +1553,19 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
     }
     ehe = c9 * sum_finish(acc, c->nz) / (double)n;
     den_from_psi(c);
-    if (c->quantum) { /* psix corrector (Dynamics_quantum_impurities/steppc.f90:116-139) */
-      cplx* px = (cplx*)c->psix;
-      cplx* pcx = (cplx*)c->pcx;
-      cplx* kx = (cplx*)c->hpsix;
-      cplx* xo3 = (cplx*)c->psixold[c->ioldp[2]];
-      cplx* xo2 = (cplx*)c->psixold[c->ioldp[1]];
-      cplx* xh1 = (cplx*)c->hpsixold[c->ioldh[0]];
-      cplx* xh2 = (cplx*)c->hpsixold[c->ioldh[1]];
-      hpsi_x(c, px, kx);
+    /* psix correctors (Dynamics_quantum_impurities/steppc.f90:115-139):
+     * errPsix = sum_k c9 sum|pcx_k|, divided by nxyz after the loop */
+    double errx = 0.0;
+    for (int qk = 0; qk < c->quantum; ++qk) {
+      const qimp* Q = &c->qi[qk];
+      cplx* px = (cplx*)Q->psix;
+      cplx* pcx = (cplx*)Q->pcx;
+      cplx* kx = (cplx*)Q->hpsix;
+      cplx* xo3 = (cplx*)Q->psixold[c->ioldp[2]];
+      cplx* xo2 = (cplx*)Q->psixold[c->ioldp[1]];
+      cplx* xh1 = (cplx*)Q->hpsixold[c->ioldh[0]];
+      cplx* xh2 = (cplx*)Q->hpsixold[c->ioldh[1]];
+      hpsi_x(c, Q, px, kx);
 #pragma omp parallel for schedule(static)
       for (int iz = 0; iz < c->nz; ++iz) {
         double sx = 0;
@@ -1499,8 +1578,9 @@ void oracle_steppc(octx* c, double deltat, double* errHe, double* errimp, double
         }
         acc[iz] = sx;
       }
-      c->errpsix = c9 * sum_finish(acc, c->nz) / (double)n;
+      errx += c9 * sum_finish(acc, c->nz);
     }
+    if (c->quantum) c->errpsix = errx / (double)n;
     free(acc);
     int t = c->ioldp[2]; c->ioldp[2] = c->ioldp[1]; c->ioldp[1] = c->ioldp[0]; c->ioldp[0] = t;
   }
@@ -1597,36 +1677,70 @@ void oracle_energy(octx* c, double* out) {
     }
     eHeX = s[6] * dv;
     if (c->quantum) {
-      /* Dynamics_quantum_impurities/energy.f90:102-132 with N_imp = 1 */
+      /* Dynamics_quantum_impurities/energy.f90:102-132: ekinx = sum_k h2o2mimp_k
+       * sum|grad psix_k|^2 dxyz; eHeX = sum(F^-1[sum_k F(|psix_k|^2) uimp_k] den)
+       * dxyz; eimpu_impu = sum_{k<m} sum(F^-1[F(|psix_k|^2) uimp_k_k(k,m)]
+       * |psix_m|^2) dxyz */
       double* g1 = dalloc(2 * c->n);
       double* g2 = dalloc(2 * c->n);
       double* g3 = dalloc(2 * c->n);
-      deriv(c, 1, 1, c->psix, g1, 2);
-      deriv(c, 1, 2, c->psix, g2, 2);
-      deriv(c, 1, 3, c->psix, g3, 2);
-      denx_from_psix(c);
-      fftfw(c, c->denx, c->wk1);
-      kmul(c, c->wk1, c->uimpk, c->wk2);
-      fftbk(c, c->wk2, c->sto2);
       double* qa = dalloc(2 * (size_t)nz);
+      cplx* acck = (cplx*)dalloc(2 * c->nh);
+      for (int q = 0; q < c->quantum; ++q) {
+        const qimp* Q = &c->qi[q];
+        deriv(c, 1, 1, Q->psix, g1, 2);
+        deriv(c, 1, 2, Q->psix, g2, 2);
+        deriv(c, 1, 3, Q->psix, g3, 2);
+#pragma omp parallel for schedule(static)
+        for (int iz = 0; iz < nz; ++iz) {
+          double sk = 0;
+          for (size_t j = 0; j < plane; ++j) {
+            size_t i = iz * plane + j;
+            for (int r = 0; r < 2; ++r)
+              sk += g1[2 * i + r] * g1[2 * i + r] + g2[2 * i + r] * g2[2 * i + r] + g3[2 * i + r] * g3[2 * i + r];
+          }
+          qa[iz] = sk;
+        }
+        double sk = 0;
+        for (int iz = 0; iz < nz; ++iz) sk += qa[iz];
+        ekinx += Q->h2o2mimp * sk;
+        denx_from_psix(c, q);
+        fftfw(c, Q->denx, c->wk1);
+        kmul(c, c->wk1, Q->uimpk, c->wk2);
+        for (size_t i = 0; i < c->nh; ++i) acck[i] += c->wk2[i];
+      }
+      ekinx *= dv;
+      fftbk(c, acck, c->sto2);
 #pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nz; ++iz) {
-        double sk = 0, sh = 0;
-        for (size_t j = 0; j < plane; ++j) {
-          size_t i = iz * plane + j;
-          for (int q = 0; q < 2; ++q)
-            sk += g1[2 * i + q] * g1[2 * i + q] + g2[2 * i + q] * g2[2 * i + q] + g3[2 * i + q] * g3[2 * i + q];
-          sh += c->sto2[i] * c->den[i];
-        }
-        qa[2 * iz] = sk; qa[2 * iz + 1] = sh;
+        double sh = 0;
+        for (size_t j = 0; j < plane; ++j) sh += c->sto2[iz * plane + j] * c->den[iz * plane + j];
+        qa[iz] = sh;
       }
-      double sk = 0, sh = 0;
-      for (int iz = 0; iz < nz; ++iz) { sk += qa[2 * iz]; sh += qa[2 * iz + 1]; }
-      free(qa); free(g1); free(g2); free(g3);
-      ekinx = c->h2o2mimp * sk * dv;
+      double sh = 0;
+      for (int iz = 0; iz < nz; ++iz) sh += qa[iz];
       eHeX = sh * dv;
+      double eii = 0;
+      for (int k = 0; k < c->quantum; ++k) {
+        fftfw(c, c->qi[k].denx, c->wk1);
+        for (int m = k + 1; m < c->quantum; ++m) {
+          if (!c->uimpkk[k][m]) continue;
+          kmul(c, c->wk1, c->uimpkk[k][m], c->wk2);
+          fftbk(c, c->wk2, c->sto2);
+          const double* dm = c->qi[m].denx;
+#pragma omp parallel for schedule(static)
+          for (int iz = 0; iz < nz; ++iz) {
+            double se = 0;
+            for (size_t j = 0; j < plane; ++j) se += c->sto2[iz * plane + j] * dm[iz * plane + j];
+            qa[iz] = se;
+          }
+          for (int iz = 0; iz < nz; ++iz) eii += qa[iz];
+        }
+      }
+      eimpu_impu = eii * dv;
+      free(qa); free(g1); free(g2); free(g3); free(acck);
     }
-    /* energy.f90:168-177 */
+    /* energy.f90:168-177 (classical impurities) */
     if (c->nimp > 1 && c->pairv)
       for (int k = 0; k < c->nimp; ++k)
         for (int m = k + 1; m < c->nimp; ++m) {
@@ -1638,9 +1752,9 @@ void oracle_energy(octx* c, double* out) {
   } else if (c->quantum) {
     /* He_statics/energy.f90:76-93,137-149: ekinx = h2o2mx sum|grad psix|^2 dxyz,
      * eimpu = sum(potx4 den) dxyz, etot = etot4 + ekinx + eimpu */
-    deriv(c, 1, 1, c->psix, c->sto4, 1);
-    deriv(c, 1, 2, c->psix, c->sto5, 1);
-    deriv(c, 1, 3, c->psix, c->sto6, 1);
+    deriv(c, 1, 1, c->qi[0].psix, c->sto4, 1);
+    deriv(c, 1, 2, c->qi[0].psix, c->sto5, 1);
+    deriv(c, 1, 3, c->qi[0].psix, c->sto6, 1);
     double* qa = dalloc(2 * (size_t)nz);
 #pragma omp parallel for schedule(static)
     for (int iz = 0; iz < nz; ++iz) {
@@ -1655,7 +1769,7 @@ void oracle_energy(octx* c, double* out) {
     double sk = 0, se = 0;
     for (int iz = 0; iz < nz; ++iz) { sk += qa[2 * iz]; se += qa[2 * iz + 1]; }
     free(qa);
-    ekinx = c->h2o2mimp * sk * dv;
+    ekinx = c->qi[0].h2o2mimp * sk * dv;
     eimpu = se * dv;
     etot = etot4 + ekinx + eimpu;
   } else {
@@ -1753,8 +1867,8 @@ void oracle_statics_init(octx* c, double n4, double* mu4) {
   if (c->p.core_otc) oracle_term_alfa(c);
   if (c->quantum) {
     /* BCN4HeDFT.f90:781-791: psi1x = psi2x = psix, fdenx = F(denx) */
-    memcpy(c->psi1x, c->psix, n * sizeof(double));
-    memcpy(c->psi2x, c->psix, n * sizeof(double));
+    memcpy(c->qi[0].psi1x, c->qi[0].psix, n * sizeof(double));
+    memcpy(c->qi[0].psi2x, c->qi[0].psix, n * sizeof(double));
     statics_fdenx(c);
   }
   oracle_poten(c);
@@ -1767,8 +1881,8 @@ void oracle_statics_init(octx* c, double n4, double* mu4) {
 /* denx = psix^2, fdenx = F(denx) */
 static void statics_fdenx(octx* c) {
 #pragma omp parallel for schedule(static)
-  for (long i = 0; i < (long)c->n; ++i) c->denx[i] = c->psix[i] * c->psix[i];
-  fftfw(c, c->denx, c->fdenx);
+  for (long i = 0; i < (long)c->n; ++i) c->qi[0].denx[i] = c->qi[0].psix[i] * c->qi[0].psix[i];
+  fftfw(c, c->qi[0].denx, c->fdenx);
 }
 
 /* ironing (He_statics/ironing.f90:4-33): f = (sum of the six neighbours)/12 +
@@ -1823,20 +1937,20 @@ void oracle_evolo(octx* c, double deltat, double vdt1, double vdt2, int iron, do
  * eigenvalue, out: <psix|H|psix> */
 void oracle_evolox(octx* c, double deltatx, double vdt1, double vdt2, int ironx, double* epsx, double* epsxerr) {
   size_t n = c->n;
-  deriv(c, 2, 1, c->psix, c->sto1, 1);
-  deriv(c, 2, 2, c->psix, c->sto2, 1);
-  deriv(c, 2, 3, c->psix, c->sto3, 1);
+  deriv(c, 2, 1, c->qi[0].psix, c->sto1, 1);
+  deriv(c, 2, 2, c->qi[0].psix, c->sto2, 1);
+  deriv(c, 2, 3, c->qi[0].psix, c->sto3, 1);
   size_t plane = (size_t)c->nx * c->ny;
   double* acc = dalloc(c->nz);
-  double hx = c->h2o2mimp;
+  double hx = c->qi[0].h2o2mimp;
 #pragma omp parallel for schedule(static)
   for (int iz = 0; iz < c->nz; ++iz) {
     double t = 0;
     for (size_t j = 0; j < plane; ++j) {
       size_t i = iz * plane + j;
       double s4 = (c->sto1[i] + c->sto2[i] + c->sto3[i]) * hx;
-      c->hpsix[i] = -s4 + c->upotx[i] * c->psix[i];
-      t += c->psix[i] * c->hpsix[i];
+      c->qi[0].hpsix[i] = -s4 + c->qi[0].upotx[i] * c->qi[0].psix[i];
+      t += c->qi[0].psix[i] * c->qi[0].hpsix[i];
     }
     acc[iz] = t;
   }
@@ -1844,24 +1958,24 @@ void oracle_evolox(octx* c, double deltatx, double vdt1, double vdt2, int ironx,
   free(acc);
   double eold = *epsx;
 #pragma omp parallel for schedule(static)
-  for (long i = 0; i < (long)n; ++i) c->hpsix[i] = c->hpsix[i] - eold * c->psix[i];
+  for (long i = 0; i < (long)n; ++i) c->qi[0].hpsix[i] = c->qi[0].hpsix[i] - eold * c->qi[0].psix[i];
   *epsxerr = fabs(1.0 - eold / hmean);
   *epsx = hmean;
-  for (int k = 0; k < ironx; ++k) oracle_ironing(c->nx, c->ny, c->nz, c->hpsix);
+  for (int k = 0; k < ironx; ++k) oracle_ironing(c->nx, c->ny, c->nz, c->qi[0].hpsix);
   double pmin = c->p.psimin;
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)n; ++i) {
-    double p = c->psix[i], p1 = c->psi1x[i], p2 = c->psi2x[i];
-    double nw = p - deltatx * c->hpsix[i] + vdt1 * (p - p1) + vdt2 * (p1 - p2);
-    c->psi2x[i] = p1;
-    c->psi1x[i] = p;
-    c->psix[i] = (nw > pmin) ? nw : pmin;
+    double p = c->qi[0].psix[i], p1 = c->qi[0].psi1x[i], p2 = c->qi[0].psi2x[i];
+    double nw = p - deltatx * c->qi[0].hpsix[i] + vdt1 * (p - p1) + vdt2 * (p1 - p2);
+    c->qi[0].psi2x[i] = p1;
+    c->qi[0].psi1x[i] = p;
+    c->qi[0].psix[i] = (nw > pmin) ? nw : pmin;
   }
-  double cnorm = sqrt(1.0 / (sumsq_plane(c, c->psix) * c->dxyz));
+  double cnorm = sqrt(1.0 / (sumsq_plane(c, c->qi[0].psix) * c->dxyz));
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)n; ++i) {
-    c->psix[i] = c->psix[i] * cnorm;
-    c->denx[i] = c->psix[i] * c->psix[i];
+    c->qi[0].psix[i] = c->qi[0].psix[i] * cnorm;
+    c->qi[0].denx[i] = c->qi[0].psix[i] * c->qi[0].psix[i];
   }
 }
 
@@ -1916,14 +2030,16 @@ void oracle_tddft_init(octx* c) {
   memset(c->qr, 0, sizeof c->qr); memset(c->qv, 0, sizeof c->qv);
   memset(c->rimpold, 0, sizeof c->rimpold); memset(c->vimpold, 0, sizeof c->vimpold);
   memset(c->aimpold, 0, sizeof c->aimpold);
-  if (c->quantum) memset(c->qx, 0, sizeof(double) * 2 * n); /* DFT4He3d.f90:617 (quantum driver) */
+  for (int k = 0; k < c->quantum; ++k) memset(c->qi[k].qx, 0, sizeof(double) * 2 * n); /* DFT4He3d.f90:617 (quantum driver) */
 }
 
 void oracle_tddft_iteration(octx* c, double deltat, int iter, int nrk, double* errHe, double* errimp,
                             double* errvimp) {
   if (iter <= nrk) {
     oracle_steprk(c, deltat);
-    if (errHe) *errHe = 0; if (errimp) *errimp = 0; if (errvimp) *errvimp = 0;
+    if (errHe) *errHe = 0;
+    if (errimp) *errimp = 0;
+    if (errvimp) *errvimp = 0;
   } else {
     oracle_steppc(c, deltat, errHe, errimp, errvimp);
   }

@@ -126,6 +126,8 @@ def lib():
         L.oracle_ironing.argtypes = [ip, ip, ip, vp]
         L.oracle_tddft_iteration.argtypes = [vp, C.c_double, ip, ip, dp, dp, dp]
         L.oracle_set_quantum_impurity.argtypes = [vp, vp, C.c_double, C.c_double]
+        L.oracle_set_quantum_impurities.argtypes = [vp, ip, vp, vp, vp, vp, C.c_double]
+        L.oracle_select_quantum.argtypes = [vp, ip]
         L.oracle_quantum_centroid.argtypes = [vp, dp, vp]
         L.oracle_observables.argtypes = [vp, ip, C.c_double, vp]
         L.oracle_num_threads.restype = ip
@@ -458,6 +460,30 @@ class Oracle:
     def set_quantum_impurity(self, uimp_k, m_imp_u, mu4=0.0):
         k = np.ascontiguousarray(np.broadcast_to(np.asarray(uimp_k, dtype=np.float64), self.hshape))
         lib().oracle_set_quantum_impurity(self._c, _ptr(k), float(m_imp_u), float(mu4))
+
+    def set_quantum_impurities(self, uimp_ks, masses_u, mu4=0.0, uimp_kk=None):
+        """N_imp quantum impurities: uimp_ks[i] = uimp_k(:,:,:,i) (a real half
+        spectrum each), masses_u[i] in u, uimp_kk = {(i, j): uimp_k_k(:,:,:,i,j)}
+        for the pairs whose Selec_gs_k_k is not 'null' (i != j; row i is the
+        impurity that feels the potential)."""
+        nq = len(masses_u)
+        uk = np.ascontiguousarray(np.stack([np.broadcast_to(np.asarray(u, dtype=np.float64), self.hshape)
+                                            for u in uimp_ks]))
+        ukk = np.zeros((nq, nq) + tuple(self.hshape))
+        pair = np.zeros(nq * nq, dtype=np.int32)
+        for (i, j), u in (uimp_kk or {}).items():
+            if i == j:
+                raise ValueError("uimp_k_k(i,i) is not used")
+            ukk[i, j] = np.broadcast_to(np.asarray(u, dtype=np.float64), self.hshape)
+            pair[i * nq + j] = 1
+        m = np.ascontiguousarray(masses_u, dtype=np.float64)
+        lib().oracle_set_quantum_impurities(self._c, nq, _ptr(uk), _ptr(ukk), pair.ctypes.data_as(C.c_void_p),
+                                            _ptr(m), float(mu4))
+
+    def select_quantum(self, k):
+        """The impurity the PSIX* / UPOTX / UIMPK fields and quantum_centroid
+        refer to."""
+        lib().oracle_select_quantum(self._c, int(k))
 
     def quantum_centroid(self):
         nrm = C.c_double(0)

@@ -11,7 +11,13 @@ and of the package's uimp_k transform, on the CPU.
   a wrong sign of i or a wrong mass factor misses by orders of magnitude);
   coupled real-time evolution conserves both norms;
 * energy: ekinx of a plane wave, eHeX of a constant kernel; centroid of a
-  symmetric packet.
+  symmetric packet;
+* several impurities: constant X-He and X-X kernels give potx4 and each upotx
+  in closed form (and fix which impurity of a pair feels uimp_k_k(i,j)); a
+  brute-force convolution fixes the pair term; two free plane waves of
+  different masses rotate independently; ekinx, eHeX and the pair energy of
+  plane waves in closed form; swapping two identical impurities swaps the
+  trajectory; each impurity keeps its norm.
 """
 import numpy as np
 import pytest
@@ -190,3 +196,126 @@ def test_centroid_of_symmetric_packet():
     nrm, rc = o.quantum_centroid()
     assert abs(nrm - 1.0) < 1e-12
     assert np.abs(rc - np.array(r0)).max() < 1e-10
+
+
+# ---- several quantum impurities (potenimp.f90:13-39, steprk.f90:84-106,
+# steppc.f90:64-139, energy.f90:102-132) ------------------------------------------
+def _two_oracle(n, xmax, uks, masses, psixs, uimp_kk=None, den=None, **kw):
+    o = O.Oracle(n, n, n, xmax, xmax, xmax, O.default_params(core_otc=0, **kw))
+    o.set("DEN", den if den is not None else I.woods_saxon(n, n, n, xmax, xmax, xmax, 20.0, seed=1))
+    o.set_quantum_impurities(uks, masses, uimp_kk=uimp_kk)
+    for k, p in enumerate(psixs):
+        o.select_quantum(k)
+        o.set("PSIX", p)
+    return o
+
+
+def _fields(o, k, *names):
+    o.select_quantum(k)
+    return [o.field(n) for n in names]
+
+
+def test_two_impurities_potenimp_constant_kernels():
+    """Constant kernels turn every convolution into a product: potx4 = c0|psix0|^2
+    + c1|psix1|^2, upotx_i = c_i den + a_ij |psix_j|^2 with row i the impurity that
+    feels uimp_k_k(i,j) (a swapped pair index gives the other constant)."""
+    n, xmax = 16, 3.2
+    p0 = I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(0.5, 0.0, 0.0), width=1.0)
+    p1 = I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(-0.8, 0.3, 0.0), width=1.3, k=(0.0, 0.4, 0.0))
+    c0, c1, a01, a10 = 37.5, -12.0, 5.25, 80.0
+    o = _two_oracle(n, xmax, [c0, c1], [4.002603, 20.0], [p0, p1], uimp_kk={(0, 1): a01, (1, 0): a10})
+    o.potenimp()
+    d0, d1, den = np.abs(p0) ** 2, np.abs(p1) ** 2, o.field("DEN")
+    assert relerr(o.field("POTX4"), c0 * d0 + c1 * d1) < 1e-13
+    assert relerr(_fields(o, 0, "UPOTX")[0], c0 * den + a01 * d1) < 1e-13
+    assert relerr(_fields(o, 1, "UPOTX")[0], c1 * den + a10 * d0) < 1e-13
+    # a 'null' pair (no table) couples nothing
+    o2 = _two_oracle(n, xmax, [c0, c1], [4.002603, 20.0], [p0, p1], uimp_kk={(0, 1): a01})
+    o2.potenimp()
+    assert relerr(_fields(o2, 1, "UPOTX")[0], c1 * den) < 1e-13
+
+
+def test_pair_term_brute_force_periodic_convolution():
+    n, xmax = 8, 2.0
+    rng = np.random.default_rng(7)
+    k = rng.standard_normal((n, n, n))
+    k = 0.5 * (k + np.roll(k[::-1, ::-1, ::-1], 1, axis=(0, 1, 2)))
+    uk = np.fft.rfftn(k).real
+    p0 = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    p1 = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    o = _two_oracle(n, xmax, [0.0, 0.0], [4.0, 4.0], [p0, p1], uimp_kk={(0, 1): uk}, den=rng.random((n, n, n)))
+    o.potenimp()
+    d1 = np.abs(p1) ** 2
+    brute = np.zeros_like(d1)
+    for s in np.ndindex(n, n, n):
+        brute += d1[s] * np.roll(k, s, axis=(0, 1, 2))
+    assert relerr(_fields(o, 0, "UPOTX")[0], brute) < 1e-12
+
+
+def test_two_free_plane_waves_rotate_independently():
+    """No coupling: each impurity rotates by exp(-i h2o2mimp_k kappa_k t) with its
+    own mass and wave vector (one slot rotation shared by both histories)."""
+    n, xmax = 16, 3.2
+    h = 2 * xmax / n
+    ms, waves = (2 * 4.002603, 7.0), (_plane_wave(n, xmax, 4), _plane_wave(n, xmax, 2) * 0.5)
+    kap = [-(O.derivnd(2, w, h, 1, 8) / w).real.mean() for w in waves]
+    o = _two_oracle(n, xmax, [0.0, 0.0], ms, waves, den=np.full((n, n, n), 0.02), icon=8)
+    o.tddft_init()
+    dt = 5e-4 / I.PSTOK
+    for it in range(1, 13):
+        o.tddft_iteration(dt, it, 3)
+    for k in range(2):
+        w = H2O2M4 * 4.002603 / ms[k] * kap[k]
+        assert relerr(_fields(o, k, "PSIX")[0], waves[k] * np.exp(-1j * w * 12 * dt)) < 1e-11
+
+
+def test_two_impurity_energies_closed_forms():
+    n, xmax = 16, 3.2
+    h = 2 * xmax / n
+    w0, w1 = _plane_wave(n, xmax, 3) * 0.7, _plane_wave(n, xmax, 1) * 1.1
+    ms, c0, c1, a01, a10 = (3.0, 9.0), 11.0, -4.0, 2.5, 6.0
+    o = _two_oracle(n, xmax, [c0, c1], ms, [w0, w1], uimp_kk={(0, 1): a01, (1, 0): a10}, icon=8)
+    o.tddft_init()
+    e = o.energy()
+    dv = h ** 3
+    den = o.field("DEN")
+    ekx = 0.0
+    for w, m in ((w0, ms[0]), (w1, ms[1])):
+        k1 = (O.derivnd(1, w, h, 1, 8) / w).mean()
+        ekx += H2O2M4 * 4.002603 / m * abs(k1) ** 2 * (np.abs(w) ** 2).sum()
+    assert abs(e["ekinx"] - ekx * dv) < 1e-11 * e["ekinx"]
+    ehx = ((c0 * np.abs(w0) ** 2 + c1 * np.abs(w1) ** 2) * den).sum() * dv
+    assert abs(e["eHeX"] - ehx) < 1e-11 * abs(ehx)
+    # pairs k < m with uimp_k_k(k,m) only (energy.f90:122-129)
+    eii = a01 * (np.abs(w0) ** 2 * np.abs(w1) ** 2).sum() * dv
+    assert abs(e["eimpu_impu"] - eii) < 1e-12 * abs(eii)
+    assert abs(e["eimpu"] - (e["ekinx"] + e["eHeX"] + e["eimpu_impu"])) < 1e-12 * abs(e["eimpu"])
+
+
+def test_swapping_identical_impurities_swaps_the_trajectory():
+    """Two impurities of one species (same mass, X-He kernel and a symmetric X-X
+    kernel): starting from (A, B) and from (B, A) gives the same helium and the
+    impurity wave functions swapped."""
+    n, xmax = 16, 3.2
+    v = Q.lennard_jones(10.22, 2.556)
+    uk = Q.uimp_k_table(n, n, n, xmax, xmax, xmax, v, 2.0, float(v(2.0)))
+    vxx = Q.lennard_jones(20.0, 3.0)
+    ukk = Q.uimp_k_table(n, n, n, xmax, xmax, xmax, vxx, 2.5, float(vxx(2.5)))
+    A = I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(1.0, 0.0, 0.5), width=1.0, k=(0.5, 0.0, 0.0))
+    B = I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(-1.2, 0.4, 0.0), width=1.1, k=(0.0, -0.3, 0.2))
+    runs = []
+    for pair in ((A, B), (B, A)):
+        o = _two_oracle(n, xmax, [uk, uk], [4.0, 4.0], pair, uimp_kk={(0, 1): ukk, (1, 0): ukk})
+        o.tddft_init()
+        dt = 2e-4 / I.PSTOK
+        for it in range(1, 9):
+            o.tddft_iteration(dt, it, 3)
+        runs.append(o)
+    a, b = runs
+    assert relerr(a.field("PSI"), b.field("PSI")) < 1e-12
+    assert relerr(_fields(a, 0, "PSIX")[0], _fields(b, 1, "PSIX")[0]) < 1e-12
+    assert relerr(_fields(a, 1, "PSIX")[0], _fields(b, 0, "PSIX")[0]) < 1e-12
+    assert np.abs(_fields(a, 0, "PSIX")[0] - A).max() > 1e-4  # it moved
+    dv = (2 * xmax / n) ** 3
+    for k, p in enumerate((A, B)):  # each impurity keeps its norm
+        assert abs((np.abs(_fields(a, k, "PSIX")[0]) ** 2).sum() * dv - (np.abs(p) ** 2).sum() * dv) < 1e-7
