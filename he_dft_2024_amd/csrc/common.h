@@ -366,16 +366,23 @@ struct hedft_ctx {
   int zconv_fused;     // 1: one workgroup applies all kernels of a convolution
   int prefer_eager;    // 1: hedft_tddft_run_pc / batches skip graph replay
   int lap_late;        // where the next step's x+y Laplacians run (hedft_tuning.lap_late)
-  // quantum impurity (Dynamics_quantum_impurities): one impurity wave function
-  // psix with its own PC history, coupled through the real half-spectrum
-  // kernel uimpk: potx4 = F^-1[F(|psix|^2) uimpk] enters pot4, upotx =
-  // F^-1[F(den) uimpk] drives psix
-  int quantum;
-  double h2o2mimp, mu4, m_imp_u;
-  void *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *lapx;  // complex N
-  double *potx4, *upotx;                                     // real N (R storage)
-  double* uimpk;                                             // real half spectrum (R storage)
-  double2* specq[2];                                         // half spectra: |psix|^2, den x uimpk
+  // quantum impurities (Dynamics_quantum_impurities): quantum = N_imp wave
+  // functions psix, each with its own PC history, coupled to the helium through
+  // its real half-spectrum kernel uimpk (potx4 = sum_k F^-1[F(|psix_k|^2)
+  // uimpk_k] enters pot4, upotx_k = F^-1[F(den) uimpk_k] drives psix_k) and to
+  // each other through uimpkk (upotx_i += F^-1[F(|psix_j|^2) uimpkk[i][j]]);
+  // qsel picks the impurity the PSIX* fields and the centroid refer to
+  int quantum, qsel;
+  struct QImp {
+    double h2o2mimp, m_imp_u;
+    void *psix, *psixold[3], *hpsixold[2], *pcx, *qx, *lapx;  // complex N
+    double* upotx;                                             // real N (R storage)
+    double* uimpk;                                             // real half spectrum (R storage)
+  } qi[HEDFT_MAX_QIMP];
+  double* uimpkk[HEDFT_MAX_QIMP][HEDFT_MAX_QIMP];  // null: a 'null' pair, or i = j
+  double mu4;
+  double* potx4;      // real N (R storage)
+  double2* specq[2];  // half spectra: |psix|^2, den x uimpk
   // statics (He_statics with limp): psix is real N, psixold[0..1] hold psi1x,
   // psi2x of evolox, specq[0] holds fdenx = F(psix^2) as the driver leaves it
 };
@@ -396,7 +403,7 @@ int fftbk(hedft_ctx* c, double2* in, double* out);
 
 // x-pass prologue (R2C input) and epilogue (C2R output) operations
 enum XInOp { XIN_COPY = 0, XIN_CORR = 1, XIN_FGRAD = 2, XIN_DOT3 = 3, XIN_ABS2 = 4 };
-enum XOutOp { XOUT_STORE = 0, XOUT_POT4 = 1 };
+enum XOutOp { XOUT_STORE = 0, XOUT_POT4 = 1, XOUT_ACC = 2 };  // ACC: out += value (real-space sums)
 // R: storage type of the real fields (double: FP64 path, float: mixed path);
 // the pointwise arithmetic of prologues and epilogues is done in double.
 template <typename R>

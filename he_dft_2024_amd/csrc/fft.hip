@@ -400,6 +400,8 @@ __device__ __forceinline__ void xout_apply(const XOutT<R>& o, size_t i, double v
       p += o.csolid * (1.0 + th + o.beta * dn * (1.0 - th * th));
     }
     o.out[i] = (R)p;
+  } else if (o.op == XOUT_ACC) {
+    o.out[i] = (R)((double)o.out[i] + v);
   } else {
     o.out[i] = (R)v;
   }

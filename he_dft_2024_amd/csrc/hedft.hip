@@ -326,11 +326,19 @@ int hedft_destroy(hedft_ctx* c) {
                   c->timec_r, c->hpsi_r, c->psi1s, c->psi2s, c->psil, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
                   c->hpsiold[1], c->pc, c->q, c->lap, c->workc1, c->fden, c->spec[0], c->spec[1], c->spec[2],
                   c->spec[3], c->workk1, c->fvlj4, c->wcgk, c->kalfs, c->tw, c->twf, c->lamtab, c->partials, c->scal,
-                  c->imp, c->lstop, c->potion, c->psix, c->psixold[0], c->psixold[1], c->psixold[2],
-                  c->hpsixold[0], c->hpsixold[1], c->pcx, c->qx, c->lapx, c->potx4, c->upotx, c->uimpk,
-                  c->specq[0], c->specq[1], c->specd, c->coef2_dev, c->coef1_dev};
+                  c->imp, c->lstop, c->potion, c->potx4, c->specq[0], c->specq[1], c->specd, c->coef2_dev,
+                  c->coef1_dev};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  for (int k = 0; k < HEDFT_MAX_QIMP; ++k) {
+    hedft_ctx::QImp& x = c->qi[k];
+    void* qp[] = {x.psix, x.psixold[0], x.psixold[1], x.psixold[2], x.hpsixold[0], x.hpsixold[1], x.pcx, x.qx,
+                  x.lapx, x.upotx, x.uimpk};
+    for (void* q : qp)
+      if (q) hipFree(q);
+    for (int m = 0; m < HEDFT_MAX_QIMP; ++m)
+      if (c->uimpkk[k][m]) hipFree(c->uimpkk[k][m]);
+  }
   clear_pairs(c);
   if (c->host_scal) hipHostFree(c->host_scal);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -393,27 +401,30 @@ static int field_ptr(hedft_ctx* c, int f, void** ptr, size_t* bytes, int* comput
     case HEDFT_PSIXOLD1: case HEDFT_PSIXOLD2: case HEDFT_PSIXOLD3: case HEDFT_HPSIXOLD1:
     case HEDFT_HPSIXOLD2: case HEDFT_PCX: case HEDFT_QX:
       if (!c->quantum) return HEDFT_ESTATE;
-      if (!td) {  // statics: real psix, its potentials and kernel
-        switch (f) {
-          case HEDFT_PSIX: *ptr = c->psix; *bytes = rN; break;
-          case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
-          case HEDFT_UPOTX: *ptr = c->upotx; *bytes = rN; break;
-          case HEDFT_DENX: *ptr = c->workc1; *bytes = rN; *computed = 1; break;
-          case HEDFT_UIMPK: *ptr = c->uimpk; *bytes = kN; break;
-          default: return HEDFT_EINVAL;
+      {
+        hedft_ctx::QImp& q = c->qi[c->qsel];
+        if (!td) {  // statics: real psix, its potentials and kernel
+          switch (f) {
+            case HEDFT_PSIX: *ptr = q.psix; *bytes = rN; break;
+            case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
+            case HEDFT_UPOTX: *ptr = q.upotx; *bytes = rN; break;
+            case HEDFT_DENX: *ptr = c->workc1; *bytes = rN; *computed = 1; break;
+            case HEDFT_UIMPK: *ptr = q.uimpk; *bytes = kN; break;
+            default: return HEDFT_EINVAL;
+          }
+          break;
         }
-        break;
-      }
-      switch (f) {
-        case HEDFT_PSIX: *ptr = c->psix; *bytes = cN; break;
-        case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
-        case HEDFT_UPOTX: *ptr = c->upotx; *bytes = rN; break;
-        case HEDFT_DENX: *ptr = c->workc1; *bytes = rN; *computed = 1; break;
-        case HEDFT_UIMPK: *ptr = c->uimpk; *bytes = kN; break;
-        case HEDFT_PCX: *ptr = c->pcx; *bytes = cN; break;
-        case HEDFT_QX: *ptr = c->qx; *bytes = cN; break;
-        case HEDFT_HPSIXOLD1: case HEDFT_HPSIXOLD2: *ptr = c->hpsixold[f - HEDFT_HPSIXOLD1]; *bytes = cN; break;
-        default: *ptr = c->psixold[f - HEDFT_PSIXOLD1]; *bytes = cN; break;
+        switch (f) {
+          case HEDFT_PSIX: *ptr = q.psix; *bytes = cN; break;
+          case HEDFT_POTX4: *ptr = c->potx4; *bytes = rN; break;
+          case HEDFT_UPOTX: *ptr = q.upotx; *bytes = rN; break;
+          case HEDFT_DENX: *ptr = c->workc1; *bytes = rN; *computed = 1; break;
+          case HEDFT_UIMPK: *ptr = q.uimpk; *bytes = kN; break;
+          case HEDFT_PCX: *ptr = q.pcx; *bytes = cN; break;
+          case HEDFT_QX: *ptr = q.qx; *bytes = cN; break;
+          case HEDFT_HPSIXOLD1: case HEDFT_HPSIXOLD2: *ptr = q.hpsixold[f - HEDFT_HPSIXOLD1]; *bytes = cN; break;
+          default: *ptr = q.psixold[f - HEDFT_PSIXOLD1]; *bytes = cN; break;
+        }
       }
       break;
     case HEDFT_PSIL:
@@ -474,9 +485,9 @@ int hedft_download(hedft_ctx* c, int32_t field, void* host, size_t bytes) {
     double* w = (double*)c->workc1;
     if (field == HEDFT_UIMP) hipLaunchKernelGGL(k_eval_uimp, gr, bl, 0, c->stream, c->d, make_impdev(c), w);
     else if (field == HEDFT_DENX && c->mode == HEDFT_MODE_STATICS)
-      hipLaunchKernelGGL(k_sq<double>, gr, bl, 0, c->stream, c->d.n, rz(c->psix), w);
+      hipLaunchKernelGGL(k_sq<double>, gr, bl, 0, c->stream, c->d.n, rz(c->qi[c->qsel].psix), w);
     else if (field == HEDFT_DENX)
-      hipLaunchKernelGGL(k_abs2, gr, bl, 0, c->stream, c->d.n, (const double2*)c->psix, w);
+      hipLaunchKernelGGL(k_abs2, gr, bl, 0, c->stream, c->d.n, (const double2*)c->qi[c->qsel].psix, w);
     else if (field == HEDFT_FALFS && c->mixed)
       hipLaunchKernelGGL(k_eval_falfs<float>, gr, bl, 0, c->stream, c->d.n, (const float*)c->denalf,
                          c->p.den0s, w);
@@ -570,10 +581,14 @@ int hedft_steprk(hedft_ctx* c, double deltat) {
 
 static int read_pc_errors(hedft_ctx* c, double* errHe, double* errimp, double* errvimp) {
   if (!errHe && !errimp && !errvimp) return HEDFT_OK;
-  double s[8];
+  double s[8], sx[HEDFT_MAX_QIMP] = {0.0};
   HEDFT_TRY(read_scal(c, 0, 8, s));
+  if (c->quantum) HEDFT_TRY(read_scal(c, S_ERRPCX, c->quantum, sx));
+  // errPsix = sum_k c9 sum|pcx_k| / nxyz (Dynamics_quantum_impurities/steppc.f90:115-139)
+  double ex = 0.0;
+  for (int k = 0; k < c->quantum; ++k) ex += (9.0 / 121.0) * sx[k];
   if (errHe) *errHe = (9.0 / 121.0) * s[S_ERRPC] / (double)c->d.n;
-  if (errimp) *errimp = c->quantum ? (9.0 / 121.0) * s[S_ERRPCX] / (double)c->d.n : c->nimp ? s[S_ERRIMP] : 0.0;
+  if (errimp) *errimp = c->quantum ? ex / (double)c->d.n : c->nimp ? s[S_ERRIMP] : 0.0;
   if (errvimp) *errvimp = c->nimp ? s[S_ERRVIMP] : 0.0;
   return HEDFT_OK;
 }
@@ -673,55 +688,94 @@ static int set_quantum_statics(hedft_ctx* c, const double* vq, size_t bytes, dou
   if (c->p.lexternal) return HEDFT_EINVAL;  // one external field in the pot4 assembly
   const size_t rN = sizeof(double) * c->d.n, kN = sizeof(double) * c->d.nh, hN = sizeof(double2) * c->d.nh;
   int rc = HEDFT_OK;
-  void** reals[] = {&c->psix, &c->psixold[0], &c->psixold[1], (void**)&c->potx4, (void**)&c->upotx};
+  void** reals[] = {&c->qi[0].psix, &c->qi[0].psixold[0], &c->qi[0].psixold[1], (void**)&c->potx4, (void**)&c->qi[0].upotx};
   for (void** f : reals)
     if (!*f) rc = rc ? rc : alloc(c, f, rN);
-  if (!c->uimpk) rc = rc ? rc : alloc(c, (void**)&c->uimpk, kN);
+  if (!c->qi[0].uimpk) rc = rc ? rc : alloc(c, (void**)&c->qi[0].uimpk, kN);
   if (!c->specq[0]) rc = rc ? rc : alloc(c, (void**)&c->specq[0], hN);
   if (rc != HEDFT_OK) return rc;
-  HEDFT_CHECK(hipMemcpyAsync(c->uimpk, vq, bytes, hipMemcpyHostToDevice, c->stream));
+  HEDFT_CHECK(hipMemcpyAsync(c->qi[0].uimpk, vq, bytes, hipMemcpyHostToDevice, c->stream));
   HEDFT_CHECK(hipStreamSynchronize(c->stream));
-  c->m_imp_u = m_imp_u;
-  c->h2o2mimp = c->p.h2o2m4 * (4.002602 / m_imp_u);
+  c->qi[0].m_imp_u = m_imp_u;
+  c->qi[0].h2o2mimp = c->p.h2o2m4 * (4.002602 / m_imp_u);
   c->mu4 = 0.0;
   c->quantum = 1;
   return done(c);
 }
 
-int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u, double mu4) {
-  if (!c || !uimp_k || c->nimp > 0 || !(m_imp_u > 0.0)) return HEDFT_EINVAL;
+// a real half-spectrum kernel into device storage R (FP64 table, converted on the mixed path)
+static int upload_kernel(hedft_ctx* c, double* dev, const double* host, size_t bytes) {
+  if (c->mixed) {
+    HEDFT_CHECK(hipMemcpyAsync(c->workc1, host, bytes, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, c->stream, (long long)c->d.nh,
+                       (const double*)c->workc1, (float*)dev);
+  } else {
+    HEDFT_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  return HEDFT_OK;
+}
+
+// potenimpini (Dynamics_quantum_impurities/potenimpini.f90:19-44): uimp_k(:,:,:,i)
+// and uimp_k_k(:,:,:,i,j) of nq impurities, and their masses (steprk.f90:86)
+int hedft_set_quantum_impurities(hedft_ctx* c, int32_t nq, const double* uimp_k, const double* uimp_kk,
+                                 const int32_t* pair, size_t bytes, const double* m_imp_u, double mu4) {
+  if (!c || !uimp_k || !m_imp_u || c->nimp > 0 || nq < 1 || nq > HEDFT_MAX_QIMP) return HEDFT_EINVAL;
   if (bytes != sizeof(double) * (size_t)c->d.nh) return HEDFT_EINVAL;
+  for (int k = 0; k < nq; ++k)
+    if (!(m_imp_u[k] > 0.0) || (pair && pair[k * nq + k])) return HEDFT_EINVAL;  // uimp_k_k(i,i) is no pair
+  if (pair && !uimp_kk) return HEDFT_EINVAL;
   hipSetDevice(c->device);
-  HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old kernel
-  if (c->mode == HEDFT_MODE_STATICS) return set_quantum_statics(c, uimp_k, bytes, m_imp_u);
+  HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old kernels
+  if (c->mode == HEDFT_MODE_STATICS) {
+    if (nq != 1) return HEDFT_EINVAL;  // He_statics carries one impurity wave function
+    return set_quantum_statics(c, uimp_k, bytes, m_imp_u[0]);
+  }
   const size_t fw = c->mixed ? sizeof(float) : sizeof(double);
   const size_t cN = sizeof(double2) * c->d.n, rN = fw * c->d.n, kN = fw * c->d.nh, hN = 2 * fw * c->d.nh;
+  const size_t nh = (size_t)c->d.nh;
   int rc = HEDFT_OK;
-  void** cplx[] = {&c->psix, &c->psixold[0], &c->psixold[1], &c->psixold[2], &c->hpsixold[0], &c->hpsixold[1],
-                   &c->pcx, &c->qx, &c->lapx};
-  for (void** f : cplx)
-    if (!*f) rc = rc ? rc : alloc(c, f, cN);
   if (!c->potx4) rc = rc ? rc : alloc(c, (void**)&c->potx4, rN);
-  if (!c->upotx) rc = rc ? rc : alloc(c, (void**)&c->upotx, rN);
-  if (!c->uimpk) rc = rc ? rc : alloc(c, (void**)&c->uimpk, kN);
   for (int k = 0; k < 2; ++k)
     if (!c->specq[k]) rc = rc ? rc : alloc(c, (void**)&c->specq[k], hN);
-  if (rc != HEDFT_OK) return rc;
-  if (c->mixed) {  // FP64 table -> double scratch -> float kernel
-    HEDFT_CHECK(hipMemcpyAsync(c->workc1, uimp_k, bytes, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, c->stream, (long long)c->d.nh,
-                       (const double*)c->workc1, (float*)c->uimpk);
-  } else {
-    HEDFT_CHECK(hipMemcpyAsync(c->uimpk, uimp_k, bytes, hipMemcpyHostToDevice, c->stream));
+  for (int k = 0; k < nq && rc == HEDFT_OK; ++k) {
+    hedft_ctx::QImp& q = c->qi[k];
+    void** cplx[] = {&q.psix, &q.psixold[0], &q.psixold[1], &q.psixold[2], &q.hpsixold[0], &q.hpsixold[1],
+                     &q.pcx, &q.qx, &q.lapx};
+    for (void** f : cplx)
+      if (!*f) rc = rc ? rc : alloc(c, f, cN);
+    if (!q.upotx) rc = rc ? rc : alloc(c, (void**)&q.upotx, rN);
+    if (!q.uimpk) rc = rc ? rc : alloc(c, (void**)&q.uimpk, kN);
+    if (rc == HEDFT_OK) rc = upload_kernel(c, q.uimpk, uimp_k + k * nh, bytes);
+    for (int m = 0; m < HEDFT_MAX_QIMP && rc == HEDFT_OK; ++m) {
+      const bool on = m < nq && m != k && uimp_kk && pair && pair[k * nq + m];
+      if (on && !c->uimpkk[k][m]) rc = alloc(c, (void**)&c->uimpkk[k][m], kN);
+      if (on && rc == HEDFT_OK) rc = upload_kernel(c, c->uimpkk[k][m], uimp_kk + ((size_t)k * nq + m) * nh, bytes);
+      if (!on && c->uimpkk[k][m]) {
+        hipFree(c->uimpkk[k][m]);
+        c->uimpkk[k][m] = nullptr;
+      }
+    }
+    q.m_imp_u = m_imp_u[k];
+    q.h2o2mimp = c->p.h2o2m4 * 4.002603 / m_imp_u[k];
   }
+  if (rc != HEDFT_OK) return rc;
   HEDFT_CHECK(hipStreamSynchronize(c->stream));
-  c->m_imp_u = m_imp_u;
-  c->h2o2mimp = c->p.h2o2m4 * 4.002603 / m_imp_u;
   c->mu4 = mu4;
-  c->quantum = 1;
+  c->quantum = nq;
+  c->qsel = 0;
   c->graph_valid = 0;
   c->lap_fresh = 0;
   return done(c);
+}
+
+int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u, double mu4) {
+  return hedft_set_quantum_impurities(c, 1, uimp_k, nullptr, nullptr, bytes, &m_imp_u, mu4);
+}
+
+int hedft_select_quantum_impurity(hedft_ctx* c, int32_t k) {
+  if (!c || k < 0 || k >= (c->quantum > 0 ? c->quantum : 1)) return HEDFT_EINVAL;
+  c->qsel = k;
+  return HEDFT_OK;
 }
 
 int hedft_tddft_init(hedft_ctx* c) {
@@ -738,7 +792,8 @@ int hedft_tddft_init(hedft_ctx* c) {
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_QR, 0, sizeof(double) * 24, c->stream));
   HEDFT_CHECK(hipMemsetAsync(c->imp + IMP_ROLD, 0, sizeof(double) * (IMP_MASS - IMP_ROLD), c->stream));
   // qx = 0 (Dynamics_quantum_impurities/DFT4He3d.f90:617)
-  if (c->quantum) HEDFT_CHECK(hipMemsetAsync(c->qx, 0, sizeof(double2) * c->d.n, c->stream));
+  for (int k = 0; k < c->quantum; ++k)
+    HEDFT_CHECK(hipMemsetAsync(c->qi[k].qx, 0, sizeof(double2) * c->d.n, c->stream));
   HEDFT_TRY(hedft_potenimp(c));
   HEDFT_TRY(tddft_after_step(c));
   c->rk_done = 0;

@@ -18,11 +18,11 @@ enum {
   S_MUOLD = 4,
   S_ERRIMP = 5,
   S_ERRVIMP = 6,
-  S_ERRPCX = 7,  // sum|pcx| of a quantum impurity
+  S_ERRPCX = 32,   // sum|pcx_k| of quantum impurity k (HEDFT_MAX_QIMP slots)
   S_ENERGY = 8,  // 8 partial energy sums
-  S_ENERGYQ = 16,  // 2 quantum-impurity energy sums
+  S_ENERGYQ = 16,  // quantum-impurity energy sums: 2 per impurity, then one per pair (14)
   S_OBS = 16,      // 13 observable sums (used alone)
-  NSCAL = 32
+  NSCAL = 40
 };
 
 static const double RK_A[4] = {0.5, 1.0 - 0.70710678118654752440, 1.0 + 0.70710678118654752440, 1.0 / 6.0};
@@ -32,7 +32,8 @@ static const double C112 = 112.0 / 121.0, C9 = 9.0 / 121.0, C4O3 = 4.0 / 3.0;
 
 static inline size_t partials_region(hedft_ctx* c) { return c->partials_cap / 8; }
 static inline double* gen_partials(hedft_ctx* c) { return c->partials; }
-static inline double* x_partials(hedft_ctx* c) { return c->partials + 2 * partials_region(c); }  // psix corrector
+// psix corrector of quantum impurity k: regions 2 .. 2 + HEDFT_MAX_QIMP - 1 of the 8
+static inline double* x_partials(hedft_ctx* c, int k = 0) { return c->partials + (2 + k) * partials_region(c); }
 
 // ---- reductions -------------------------------------------------------------------
 template <int NR>
@@ -105,6 +106,14 @@ static XOutT<R> xo_store(double* out) {
 }
 
 template <typename R>
+static XOutT<R> xo_acc(double* out) {
+  XOutT<R> o{};
+  o.op = XOUT_ACC;
+  o.out = fld<R>(out);
+  return o;
+}
+
+template <typename R>
 static XOutT<R> xo_pot4(hedft_ctx* c) {
   XOutT<R> o{};
   o.op = XOUT_POT4;
@@ -149,7 +158,8 @@ int poten_statics(hedft_ctx* c);
 int statics_refresh(hedft_ctx* c);
 int alpha_term_run(hedft_ctx* c);    // term_alfa on its own (stages B and C of alpha_s)
 int potenimp_quantum(hedft_ctx* c);  // upotx and potx4 of a quantum impurity
-int potx4_conv_run(hedft_ctx* c, double* out);  // F^-1[F(|psix|^2) uimp_k] into out
+// F^-1[F(|psix_k|^2) kern] into out (added to out with acc != 0)
+int qconv_run(hedft_ctx* c, double* out, int k, const double* kern, int acc);
 // steppers.hip
 void flush_side_errors(hedft_ctx* c);
 void flush_side_impurity(hedft_ctx* c);

@@ -319,21 +319,62 @@ static int alpha_term(hedft_ctx* c) {
 
 // the serial kernels a step left for the side branch (impurity Newton PC,
 // convolution and one inverse of the impurity density
+// F^-1[F(|psix_k|^2) kern] into out (stored, or added with acc)
 template <typename R>
-static int potx4_conv(hedft_ctx* c, double* out) {
+static int qconv(hedft_ctx* c, double* out, int k, const double* kern, bool acc) {
   XInT<R> xa{};
   xa.op = XIN_ABS2;
-  xa.z = (const double2*)c->psix;
+  xa.z = (const double2*)c->qi[k].psix;
   cplx_t<R>* sx = spc<R>(c->specq[0]);
   HEDFT_TRY(fwd_xy<R>(c, 1, &xa, &sx));
-  const R* ku = fld<R>(c->uimpk);
+  const R* ku = fld<R>(const_cast<double*>(kern));
   HEDFT_TRY(zconv<R>(c, 1, &sx, 1, &ku, &sx, 1, nullptr));
-  XOutT<R> xo = xo_store<R>(out);
+  XOutT<R> xo = acc ? xo_acc<R>(out) : xo_store<R>(out);
   return inv_yx<R>(c, 1, &sx, &xo);
 }
+template <typename R>
+static int potx4_conv(hedft_ctx* c, double* out, int k = 0, bool acc = false) {
+  return qconv<R>(c, out, k, c->qi[k].uimpk, acc);
+}
 
-// potenimp of Dynamics_quantum_impurities (potenimp.f90:20-30) on its own:
-// upotx = F^-1[F(den) uimp_k], potx4
+// Several quantum impurities (potenimp.f90:13-39), from the x+y spectrum of
+// den in sden: for each impurity i, potx4 += F^-1[F(|psix_i|^2) uimp_k_i],
+// upotx_i = F^-1[fden uimp_k_i] + sum_{j != i} F^-1[F(|psix_j|^2)
+// uimp_k_k(i,j)], each term convolved and transformed back on its own and
+// summed in real space in the reference's order (specq[0] holds |psix_j|^2's
+// x+y spectrum, specq[1] a term's spectrum)
+template <typename R>
+static int potenimp_multi(hedft_ctx* c, cplx_t<R>* sden) {
+  cplx_t<R>* sx = spc<R>(c->specq[0]);
+  cplx_t<R>* st = spc<R>(c->specq[1]);
+  auto fwd_denx = [&](int j) {
+    XInT<R> xa{};
+    xa.op = XIN_ABS2;
+    xa.z = (const double2*)c->qi[j].psix;
+    return fwd_xy<R>(c, 1, &xa, &sx);
+  };
+  auto term = [&](cplx_t<R>* sxy, const double* kern, double* out, bool acc) {
+    const R* ku = fld<R>(const_cast<double*>(kern));
+    HEDFT_TRY(zconv<R>(c, 1, &sxy, 1, &ku, &st, 1, nullptr));
+    XOutT<R> xo = acc ? xo_acc<R>(out) : xo_store<R>(out);
+    return inv_yx<R>(c, 1, &st, &xo);
+  };
+  for (int i = 0; i < c->quantum; ++i) {
+    HEDFT_TRY(fwd_denx(i));
+    HEDFT_TRY(term(sx, c->qi[i].uimpk, c->potx4, i > 0));
+    HEDFT_TRY(term(sden, c->qi[i].uimpk, c->qi[i].upotx, false));
+    for (int j = 0; j < c->quantum; ++j) {
+      if (j == i || !c->uimpkk[i][j]) continue;
+      HEDFT_TRY(fwd_denx(j));
+      HEDFT_TRY(term(sx, c->uimpkk[i][j], c->qi[i].upotx, true));
+    }
+  }
+  return HEDFT_OK;
+}
+
+// potenimp of Dynamics_quantum_impurities (potenimp.f90:13-39) on its own:
+// upotx = F^-1[F(den) uimp_k], potx4 (one impurity), or every impurity's
+// terms
 template <typename R>
 static int potenimp_q(hedft_ctx* c) {
   XInT<R> xi{};
@@ -341,10 +382,11 @@ static int potenimp_q(hedft_ctx* c) {
   xi.a[0] = fld<R>(c->den);
   cplx_t<R>* s0 = spc<R>(c->spec[0]);
   HEDFT_TRY(fwd_xy<R>(c, 1, &xi, &s0));
-  const R* ku = fld<R>(c->uimpk);
+  if (c->quantum > 1) return potenimp_multi<R>(c, s0);
+  const R* ku = fld<R>(c->qi[0].uimpk);
   cplx_t<R>* s1 = spc<R>(c->specq[1]);
   HEDFT_TRY(zconv<R>(c, 1, &s0, 1, &ku, &s1, 1, nullptr));
-  XOutT<R> xo = xo_store<R>(c->upotx);
+  XOutT<R> xo = xo_store<R>(c->qi[0].upotx);
   HEDFT_TRY(inv_yx<R>(c, 1, &s1, &xo));
   return potx4_conv<R>(c, c->potx4);
 }
@@ -356,7 +398,9 @@ static int potenimp_q(hedft_ctx* c) {
 // With a quantum impurity this is potenimp + poten of
 // Dynamics_quantum_impurities (potenimp.f90:20-30, poten.f90:26-138): upotx =
 // F^-1[F(den) uimp_k] is one more output of the den convolution, potx4 is
-// convolved on the side branch and added in the pot4 assembly.
+// convolved on the side branch and added in the pot4 assembly.  With several
+// quantum impurities their terms (potenimp_multi) follow the den convolutions
+// on the main stream, while den's x+y spectrum is still in spec[0].
 template <typename R>
 static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   // where the next step's x+y Laplacians run: beside the den convolutions
@@ -391,7 +435,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     // the quantum-impurity poten calls derden only for the OTC core
     // (Dynamics_quantum_impurities/poten.f90:48): nothing else reads grad(den) there
     if (derden) HEDFT_TRY(derden_force(c, want_force, want_force, !mid));
-    if (c->quantum) HEDFT_TRY(potx4_conv<R>(c, c->potx4));
+    if (c->quantum == 1) HEDFT_TRY(potx4_conv<R>(c, c->potx4));
     if (mid) {
       HEDFT_CHECK(hipEventRecord(c->ev_mid, c->side_stream));
       if (derden && want_force) HEDFT_TRY(derden_force_finalize(c, want_force));
@@ -399,7 +443,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     }
     if (with_lap && !late && !on_main) {
       HEDFT_TRY(lap_xy(c, c->psi, 1));
-      if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+      for (int k = 0; k < c->quantum; ++k) HEDFT_TRY(lap_xy(c, c->qi[k].psix, 1, c->qi[k].lapx));
     }
   }
   // den convolutions: dencg, delj4 [, denalf] [, upotx].  With the OTC core,
@@ -416,10 +460,10 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     so[nk] = spc<R>(c->spec[3]);
     xo[nk++] = xo_store<R>(c->denalf);
   }
-  if (c->quantum) {
-    kk[nk] = fld<R>(c->uimpk);
+  if (c->quantum == 1) {
+    kk[nk] = fld<R>(c->qi[0].uimpk);
     so[nk] = spc<R>(c->specq[1]);
-    xo[nk++] = xo_store<R>(c->upotx);
+    xo[nk++] = xo_store<R>(c->qi[0].upotx);
   }
   if (c->zconv_fused) {
     // one workgroup per column tile applies up to three kernels
@@ -438,7 +482,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     XOutT<R> xi[4];
     int ni = 0;
     for (int k = 0; k < nk; ++k) {
-      if (xo[k].out == fld<R>(c->delj4) || (c->quantum && xo[k].out == fld<R>(c->upotx))) {
+      if (xo[k].out == fld<R>(c->delj4) || (c->quantum == 1 && xo[k].out == fld<R>(c->qi[0].upotx))) {
         c->defer_spec[c->ndefer] = reinterpret_cast<double2*>(so[k]);
         c->defer_out[c->ndefer++] = reinterpret_cast<double*>(xo[k].out);
       } else {
@@ -450,9 +494,10 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
   } else {
     HEDFT_TRY(inv_yx<R>(c, nk, so, xo));
   }
+  if (c->quantum > 1) HEDFT_TRY(potenimp_multi<R>(c, s0));
   if (on_main) {
     HEDFT_TRY(lap_xy(c, c->psi, 1));
-    if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+    for (int k = 0; k < c->quantum; ++k) HEDFT_TRY(lap_xy(c, c->qi[k].psix, 1, c->qi[k].lapx));
   }
   if (mid) HEDFT_CHECK(hipStreamWaitEvent(c->main_stream, c->ev_mid, 0));
   else HEDFT_TRY(join(c));
@@ -464,7 +509,7 @@ static int poten_tddft_t(hedft_ctx* c, int want_force, int with_lap) {
     {
       SideBranch sb(c);
       HEDFT_TRY(lap_xy(c, c->psi, 1));
-      if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+      for (int k = 0; k < c->quantum; ++k) HEDFT_TRY(lap_xy(c, c->qi[k].psix, 1, c->qi[k].lapx));
     }
   }
   if (with_lap) c->lap_fresh = 1;
@@ -490,10 +535,10 @@ int poten_statics(hedft_ctx* c) {
   HEDFT_TRY(inv_yx(c, 1, &s2, &xo));
   if (c->quantum) {
     double2* in[2] = {c->fden, c->specq[0]};
-    const double* kq[2] = {c->uimpk, c->uimpk};
+    const double* kq[2] = {c->qi[0].uimpk, c->qi[0].uimpk};
     double2* so[2] = {c->spec[0], c->spec[1]};
     HEDFT_TRY(zconv(c, 2, in, 1, kq, so, 0, nullptr));
-    XOut xq[2] = {xo_store<double>(c->upotx), xo_store<double>(c->potx4)};
+    XOut xq[2] = {xo_store<double>(c->qi[0].upotx), xo_store<double>(c->potx4)};
     HEDFT_TRY(inv_yx(c, 2, so, xq));
   }
   HEDFT_TRY(stage_b<double>(c));
@@ -503,7 +548,7 @@ int poten_statics(hedft_ctx* c) {
 
 // fdenx = F(psix^2) of the statics impurity (BCN4HeDFT.f90:788,1130)
 static int statics_fdenx(hedft_ctx* c) {
-  hipLaunchKernelGGL(k_sq<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, (const double*)c->psix, c->work2);
+  hipLaunchKernelGGL(k_sq<double>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, (const double*)c->qi[0].psix, c->work2);
   return fftfw(c, c->work2, c->specq[0]);
 }
 
@@ -544,8 +589,8 @@ int poten(hedft_ctx* c, int want_force) {
 
 int alpha_term_run(hedft_ctx* c) { return c->mixed ? alpha_term<float>(c) : alpha_term<double>(c); }
 int potenimp_quantum(hedft_ctx* c) { return c->mixed ? potenimp_q<float>(c) : potenimp_q<double>(c); }
-int potx4_conv_run(hedft_ctx* c, double* out) {
-  return c->mixed ? potx4_conv<float>(c, out) : potx4_conv<double>(c, out);
+int qconv_run(hedft_ctx* c, double* out, int k, const double* kern, int acc) {
+  return c->mixed ? qconv<float>(c, out, k, kern, acc != 0) : qconv<double>(c, out, k, kern, acc != 0);
 }
 
 }  // namespace hedft

@@ -122,34 +122,71 @@ static int energy_sums(hedft_ctx* c, double* s) {
 }
 
 // quantum impurity terms (Dynamics_quantum_impurities/energy.f90:102-132):
-// sum |grad psix|^2 and sum F^-1[F(|psix|^2) uimp_k] den
+// sum |grad psix_k|^2, and (with_conv) sum conv den with conv = sum_k
+// F^-1[F(|psix_k|^2) uimp_k]
 template <typename R>
 struct OutEnergyX {
   static constexpr int NR = 2;
   double* partials;
   const double* acc;
   const R *conv, *den;
+  int with_conv;
   __device__ __forceinline__ void store(size_t i, int, int, int, double2 v, double* red) const {
     red[0] += acc[i] + vsq<double2>(v);
-    red[1] += (double)conv[i] * (double)den[i];
+    if (with_conv) red[1] += (double)conv[i] * (double)den[i];
   }
 };
 
+// per-block sums of a[i] |z_i|^2 over a fixed grid
 template <typename R>
-static int energy_sums_x(hedft_ctx* c, double* s) {
-  HEDFT_TRY(potx4_conv_run(c, c->work1));
-  const double2* px = (const double2*)c->psix;
-  launch_pencil_xy<double2, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<double2>{px},
-                                  OutSqXY<double2>{nullptr, (double*)c->workc1});
-  OutEnergyX<R> o;
-  o.partials = gen_partials(c);
-  o.acc = (const double*)c->workc1;
-  o.conv = fld<R>(c->work1);
-  o.den = fld<R>(c->den);
-  launch_pencil<double2, 2>(c->stream, c->d, c->coef[0][2], InPlain<double2>{px}, o);
-  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
-                     (int)pencil_geom<double2>(c->d, 2).nblocks, c->scal + S_ENERGYQ, 1.0);
-  return read_scal(c, S_ENERGYQ, 2, s);
+__global__ void __launch_bounds__(256) k_dot_abs2(long long n, const R* a, const double2* z, double* partials) {
+  double red[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double2 v = z[i];
+    red[0] += (double)a[i] * (v.x * v.x + v.y * v.y);
+  }
+  block_reduce_store<1>(red, partials);
+}
+
+// sk[k] = sum |grad psix_k|^2, *sh = sum conv den, eij[p] = sum
+// F^-1[F(|psix_k|^2) uimp_k_k(k,m)] |psix_m|^2 for the p-th pair k < m with a
+// kernel (energy.f90:102-129; the X-He potential summed in real space term by
+// term instead of in the spectrum, equal to rounding); returns the pair count
+template <typename R>
+static int energy_sums_x(hedft_ctx* c, double* sk, double* sh, double* eij, int* npairs) {
+  const int nq = c->quantum;
+  for (int k = 0; k < nq; ++k) HEDFT_TRY(qconv_run(c, c->work1, k, c->qi[k].uimpk, k > 0));
+  for (int k = 0; k < nq; ++k) {
+    const double2* px = (const double2*)c->qi[k].psix;
+    launch_pencil_xy<double2, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<double2>{px},
+                                    OutSqXY<double2>{nullptr, (double*)c->workc1});
+    OutEnergyX<R> o;
+    o.partials = gen_partials(c);
+    o.acc = (const double*)c->workc1;
+    o.conv = fld<R>(c->work1);
+    o.den = fld<R>(c->den);
+    o.with_conv = (k == 0);
+    launch_pencil<double2, 2>(c->stream, c->d, c->coef[0][2], InPlain<double2>{px}, o);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(256), 0, c->stream, gen_partials(c),
+                       (int)pencil_geom<double2>(c->d, 2).nblocks, c->scal + S_ENERGYQ + 2 * k, 1.0);
+  }
+  int np = 0;
+  for (int k = 0; k < nq; ++k)
+    for (int m = k + 1; m < nq; ++m) {
+      if (!c->uimpkk[k][m]) continue;
+      HEDFT_TRY(qconv_run(c, c->work2, k, c->uimpkk[k][m], 0));
+      hipLaunchKernelGGL(k_dot_abs2<R>, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, fld<R>(c->work2),
+                         (const double2*)c->qi[m].psix, gen_partials(c));
+      finalize<1>(c, gen_partials(c), EW_BLOCKS, S_ENERGYQ + 2 * HEDFT_MAX_QIMP + np, 1.0);
+      ++np;
+    }
+  double s[2 * HEDFT_MAX_QIMP + HEDFT_MAX_QIMP * (HEDFT_MAX_QIMP - 1) / 2];
+  HEDFT_TRY(read_scal(c, S_ENERGYQ, 2 * HEDFT_MAX_QIMP + np, s));
+  for (int k = 0; k < nq; ++k) sk[k] = s[2 * k];
+  *sh = s[1];
+  for (int p = 0; p < np; ++p) eij[p] = s[2 * HEDFT_MAX_QIMP + p];
+  *npairs = np;
+  return HEDFT_OK;
 }
 
 // statics impurity wave function: sum |grad psix|^2 (He_statics/energy.f90:81-90)
@@ -162,7 +199,7 @@ struct OutSumSq {
   }
 };
 static int energy_sums_xs(hedft_ctx* c, double* s) {
-  const double* px = rz(c->psix);
+  const double* px = rz(c->qi[0].psix);
   launch_pencil_xy<double, true>(c->stream, c->d, c->coef[0][0], c->coef[0][1], InPlain<double>{px},
                                  OutSqXY<double>{nullptr, (double*)c->workc1});
   OutSumSq o;
@@ -289,18 +326,23 @@ int hedft_energy(hedft_ctx* c, hedft_energies* e) {
       e->eimpu_impu = eij;
     }
     if (c->quantum) {
-      double sx[2];
-      if (c->mixed) HEDFT_TRY(energy_sums_x<float>(c, sx));
-      else HEDFT_TRY(energy_sums_x<double>(c, sx));
-      e->ekinx = c->h2o2mimp * sx[0] * dv;
-      e->eHeX = sx[1] * dv;
+      double sk[HEDFT_MAX_QIMP], sh, eij[HEDFT_MAX_QIMP * (HEDFT_MAX_QIMP - 1) / 2];
+      int np = 0;
+      if (c->mixed) HEDFT_TRY(energy_sums_x<float>(c, sk, &sh, eij, &np));
+      else HEDFT_TRY(energy_sums_x<double>(c, sk, &sh, eij, &np));
+      double ek = 0.0, ei = 0.0;
+      for (int k = 0; k < c->quantum; ++k) ek += c->qi[k].h2o2mimp * sk[k];
+      for (int q = 0; q < np; ++q) ei += eij[q];
+      e->ekinx = ek * dv;
+      e->eHeX = sh * dv;
+      e->eimpu_impu = ei * dv;
     }
     e->eimpu = e->ekinx + e->eHeX + e->eimpu_impu;
   } else if (c->quantum) {
     // He_statics/energy.f90:76-93,137-149: etot = etot4 + ekinx + eimpu
     double sk;
     HEDFT_TRY(energy_sums_xs(c, &sk));
-    e->ekinx = c->h2o2mimp * sk * dv;
+    e->ekinx = c->qi[0].h2o2mimp * sk * dv;
     e->eimpu = s[5] * dv;
     e->etot = e->etot4 + e->ekinx + e->eimpu;
     e->norm = s[7] * dv;
@@ -337,7 +379,7 @@ int hedft_compute_observables(hedft_ctx* c, int32_t n4, double mhe, hedft_observ
 
 int hedft_quantum_centroid(hedft_ctx* c, double* norm, double* rcm) {
   if (!c || !c->quantum) return HEDFT_EINVAL;
-  hipLaunchKernelGGL(k_centroid, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, (const double2*)c->psix,
+  hipLaunchKernelGGL(k_centroid, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, (const double2*)c->qi[c->qsel].psix,
                      -c->g.xmax, -c->g.ymax, -c->g.zmax, c->hx, c->hy, c->hz, gen_partials(c));
   hipLaunchKernelGGL(k_finalize<4>, dim3(1), dim3(256), 0, c->stream, gen_partials(c), EW_BLOCKS,
                      c->scal + S_ENERGYQ, 1.0);

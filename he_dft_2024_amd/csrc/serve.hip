@@ -253,7 +253,7 @@ int admit(const hedft_job& j, int device, hedft_ctx** out) {
     const std::vector<double> uk = rc == HEDFT_OK ? uimp_k_lj(j) : std::vector<double>();
     if (rc == HEDFT_OK) rc = hedft_set_quantum_impurity(c, uk.data(), uk.size() * sizeof(double), j.mass_amu, 0.0);
     if (rc == HEDFT_OK) {
-      const std::vector<double2> px = wave_packet(j, c->h2o2mimp);
+      const std::vector<double2> px = wave_packet(j, c->qi[0].h2o2mimp);
       rc = hedft_upload(c, HEDFT_PSIX, px.data(), px.size() * sizeof(double2));
     }
   } else {

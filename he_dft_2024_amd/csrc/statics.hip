@@ -548,8 +548,8 @@ int hedft_statics_init(hedft_ctx* c, double n4, double* mu4) {
                      n4, c->dxyz);
   if (c->quantum) {  // psi1x = psi2x = psix (BCN4HeDFT.f90:781-785)
     const size_t b = sizeof(double) * c->d.n;
-    HEDFT_CHECK(hipMemcpyAsync(c->psixold[0], c->psix, b, hipMemcpyDeviceToDevice, c->stream));
-    HEDFT_CHECK(hipMemcpyAsync(c->psixold[1], c->psix, b, hipMemcpyDeviceToDevice, c->stream));
+    HEDFT_CHECK(hipMemcpyAsync(c->qi[0].psixold[0], c->qi[0].psix, b, hipMemcpyDeviceToDevice, c->stream));
+    HEDFT_CHECK(hipMemcpyAsync(c->qi[0].psixold[1], c->qi[0].psix, b, hipMemcpyDeviceToDevice, c->stream));
   }
   HEDFT_TRY(statics_refresh(c));
   HEDFT_TRY(poten_statics(c));
@@ -591,28 +591,28 @@ int hedft_evolox(hedft_ctx* c, double deltatx, double vdt1, double vdt2, int32_t
                  double* epsxerr) {
   if (!c || c->mode != HEDFT_MODE_STATICS || !c->quantum || ironx < 0 || !epsx) return HEDFT_EINVAL;
   // hpsix = -(lap psix) h2o2mx + upotx psix, sum(psix hpsix) dxyz (evolox.f90:39-48)
-  HEDFT_TRY(lap_xy(c, c->psix, 0));
+  HEDFT_TRY(lap_xy(c, c->qi[0].psix, 0));
   OutHpsiR o;
   o.partials = gen_partials(c);
   o.lapxy = rz(c->lap);
-  o.psi = rz(c->psix);
-  o.pot4 = c->upotx;
+  o.psi = rz(c->qi[0].psix);
+  o.pot4 = c->qi[0].upotx;
   o.hpsi = c->hpsi_r;
-  o.h2o2m4 = c->h2o2mimp;
-  launch_pencil<double, 2>(c->stream, c->d, c->coef[1][2], InPlain<double>{rz(c->psix)}, o);
+  o.h2o2m4 = c->qi[0].h2o2mimp;
+  launch_pencil<double, 2>(c->stream, c->d, c->coef[1][2], InPlain<double>{rz(c->qi[0].psix)}, o);
   finalize<1>(c, gen_partials(c), pencil_geom<double>(c->d, 2).nblocks, S_MU, c->dxyz);
   const double eold = *epsx;
-  hipLaunchKernelGGL(k_evolox_g, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->hpsi_r, rz(c->psix), eold);
+  hipLaunchKernelGGL(k_evolox_g, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, c->hpsi_r, rz(c->qi[0].psix), eold);
   double* g = c->hpsi_r;
   for (int k = 0; k < ironx; ++k) {  // ping-pong through work1 / work2
     double* dst = (g == c->work1) ? c->work2 : c->work1;
     hipLaunchKernelGGL(k_ironing, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d, g, dst);
     g = dst;
   }
-  hipLaunchKernelGGL(k_evolox_new, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psix),
-                     rz(c->psixold[0]), rz(c->psixold[1]), g, deltatx, vdt1, vdt2, c->p.psimin, gen_partials(c));
+  hipLaunchKernelGGL(k_evolox_new, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->qi[0].psix),
+                     rz(c->qi[0].psixold[0]), rz(c->qi[0].psixold[1]), g, deltatx, vdt1, vdt2, c->p.psimin, gen_partials(c));
   finalize<1>(c, gen_partials(c), EW_BLOCKS, S_SUMSQ, 1.0);
-  hipLaunchKernelGGL(k_norm1, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->psix), c->scal, c->dxyz);
+  hipLaunchKernelGGL(k_norm1, dim3(EW_BLOCKS), dim3(256), 0, c->stream, c->d.n, rz(c->qi[0].psix), c->scal, c->dxyz);
   double hmean;
   HEDFT_TRY(read_scal(c, S_MU, 1, &hmean));
   if (epsxerr) *epsxerr = std::fabs(1.0 - eold / hmean);

@@ -312,7 +312,7 @@ void flush_side_errors(hedft_ctx* c) {
   if (c->side_pre & 4)
     finalize<1>(c, gen_partials(c), c->side_nblocks, S_ERRPC, 1.0);
   if (c->side_pre & 8)
-    finalize<1>(c, x_partials(c), c->side_nblocks, S_ERRPCX, 1.0);
+    for (int k = 0; k < c->quantum; ++k) finalize<1>(c, x_partials(c, k), c->side_nblocks, S_ERRPCX + k, 1.0);
   c->side_pre &= ~(4 | 8);
 }
 void flush_side_impurity(hedft_ctx* c) {
@@ -355,12 +355,12 @@ static HpsiC<R> make_hpsic(hedft_ctx* c) {
 }
 
 template <typename R>
-static HpsiX<R> make_hpsix(hedft_ctx* c) {
+static HpsiX<R> make_hpsix(hedft_ctx* c, int k) {
   HpsiX<R> h;
-  h.lapxy = c->lap25_tz > 0 ? nullptr : (const double2*)c->lapx;
-  h.upot = fld<R>(c->upotx);
+  h.lapxy = c->lap25_tz > 0 ? nullptr : (const double2*)c->qi[k].lapx;
+  h.upot = fld<R>(c->qi[k].upotx);
   h.lam = make_lam(c);
-  h.h2o2m = c->h2o2mimp;
+  h.h2o2m = c->qi[k].h2o2mimp;
   return h;
 }
 
@@ -368,7 +368,7 @@ static HpsiX<R> make_hpsix(hedft_ctx* c) {
 int prepare_lap(hedft_ctx* c) {
   if (c->lap_fresh || c->lap25_tz > 0) return HEDFT_OK;
   HEDFT_TRY(lap_xy(c, c->psi, 1));
-  if (c->quantum) HEDFT_TRY(lap_xy(c, c->psix, 1, c->lapx));
+  for (int k = 0; k < c->quantum; ++k) HEDFT_TRY(lap_xy(c, c->qi[k].psix, 1, c->qi[k].lapx));
   return HEDFT_OK;
 }
 
@@ -399,30 +399,31 @@ static void kt_mark(hedft_ctx* c, int slot, int end) {
   if (end) c->kt_n[slot]++;
 }
 
-// RK-Gill stage of a quantum impurity (Dynamics_quantum_impurities/steprk.f90:84-106)
+// RK-Gill stage of quantum impurity k (Dynamics_quantum_impurities/steprk.f90:84-106)
 template <typename R>
-static void steprk_x(hedft_ctx* c, double dt, int jrun) {
+static void steprk_x(hedft_ctx* c, double dt, int jrun, int k) {
+  hedft_ctx::QImp& x = c->qi[k];
   OutRK<R, HpsiX<R>> o;
   o.partials = nullptr;
-  o.h = make_hpsix<R>(c);
-  o.psi = cz(c->psix);
-  o.q = cz(c->qx);
-  o.po0 = cz(c->psixold[0]);
-  o.po1 = cz(c->psixold[1]);
-  o.po2 = cz(c->psixold[2]);
-  o.ho0 = cz(c->hpsixold[0]);
-  o.ho1 = cz(c->hpsixold[1]);
+  o.h = make_hpsix<R>(c, k);
+  o.psi = cz(x.psix);
+  o.q = cz(x.qx);
+  o.po0 = cz(x.psixold[0]);
+  o.po1 = cz(x.psixold[1]);
+  o.po2 = cz(x.psixold[2]);
+  o.ho0 = cz(x.hpsixold[0]);
+  o.ho1 = cz(x.hpsixold[1]);
   o.den = nullptr;
   o.a = RK_A[jrun];
   o.b = RK_B[jrun];
   o.cc = RK_C[jrun];
   o.dt = dt;
   o.first = (jrun == 0);
-  launch_step(c, c->psix, c->lapx, o);
+  launch_step(c, x.psix, x.lapx, o);
 }
 
-// STEPRK (Dynamics_TDDFT/steprk.f90:1-144; with a quantum impurity
-// Dynamics_quantum_impurities/steprk.f90:53-124: psix takes the same stage)
+// STEPRK (Dynamics_TDDFT/steprk.f90:1-144; with quantum impurities
+// Dynamics_quantum_impurities/steprk.f90:53-124: every psix takes the same stage)
 template <typename R>
 static int steprk_t(hedft_ctx* c, double dt) {
   for (int jrun = 0; jrun < 4; ++jrun) {
@@ -444,7 +445,7 @@ static int steprk_t(hedft_ctx* c, double dt) {
     o.dt = dt;
     o.first = (jrun == 0);
     launch_step(c, c->psi, c->lap, o);
-    if (c->quantum) steprk_x<R>(c, dt, jrun);
+    for (int k = 0; k < c->quantum; ++k) steprk_x<R>(c, dt, jrun, k);
     c->lap_fresh = 0;
     if (c->nimp > 0)
       hipLaunchKernelGGL(k_imp_rk, dim3(1), dim3(64), 0, c->stream, c->imp, c->nimp, (int)(jrun == 0), RK_A[jrun],
@@ -458,9 +459,10 @@ static int steprk_t(hedft_ctx* c, double dt) {
 }
 
 // STEPPC (Dynamics_TDDFT/steppc.f90:1-180); errHe sum into scal[S_ERRPC].
-// A quantum impurity takes the same predictor and corrector
-// (Dynamics_quantum_impurities/steppc.f90:64-95,116-139) with the same slot
-// indices (ioldpx, ioldhx rotate with ioldp, ioldh); sum|pcx| into scal[S_ERRPCX].
+// Quantum impurities take the same predictor and corrector
+// (Dynamics_quantum_impurities/steppc.f90:64-95,115-139) with the same slot
+// indices (ioldpx, ioldhx rotate with ioldp, ioldh for every impurity);
+// sum|pcx_k| into scal[S_ERRPCX + k].
 template <typename R>
 static int steppc_t(hedft_ctx* c, double dt) {
   HEDFT_TRY(prepare_lap(c));
@@ -478,18 +480,19 @@ static int steppc_t(hedft_ctx* c, double dt) {
     kt_mark(c, 0, 0);
     launch_step(c, c->psi, c->lap, o);
     kt_mark(c, 0, 1);
-    if (c->quantum) {
+    for (int k = 0; k < c->quantum; ++k) {
+      hedft_ctx::QImp& x = c->qi[k];
       OutPred<R, HpsiX<R>> ox;
       ox.partials = nullptr;
-      ox.h = make_hpsix<R>(c);
-      ox.psi = cz(c->psix);
-      ox.pc = cz(c->pcx);
-      ox.po3 = cz(c->psixold[c->ioldp[2]]);
-      ox.ho1 = cz(c->hpsixold[c->ioldh[0]]);
-      ox.ho2 = cz(c->hpsixold[c->ioldh[1]]);
+      ox.h = make_hpsix<R>(c, k);
+      ox.psi = cz(x.psix);
+      ox.pc = cz(x.pcx);
+      ox.po3 = cz(x.psixold[c->ioldp[2]]);
+      ox.ho1 = cz(x.hpsixold[c->ioldh[0]]);
+      ox.ho2 = cz(x.hpsixold[c->ioldh[1]]);
       ox.den = nullptr;
       ox.dt = dt;
-      launch_step(c, c->psix, c->lapx, ox);
+      launch_step(c, x.psix, x.lapx, ox);
     }
     c->lap_fresh = 0;
     std::swap(c->ioldh[0], c->ioldh[1]);
@@ -518,20 +521,21 @@ static int steppc_t(hedft_ctx* c, double dt) {
     kt_mark(c, 1, 0);
     launch_step(c, c->psi, c->lap, o);
     kt_mark(c, 1, 1);
-    if (c->quantum) {
+    for (int k = 0; k < c->quantum; ++k) {
+      hedft_ctx::QImp& x = c->qi[k];
       OutCorr<R, HpsiX<R>> ox;
-      ox.partials = x_partials(c);
-      ox.h = make_hpsix<R>(c);
-      ox.psi = cz(c->psix);
-      ox.pc = cz(c->pcx);
-      ox.po3 = cz(c->psixold[c->ioldp[2]]);
-      ox.po2 = cz(c->psixold[c->ioldp[1]]);
-      ox.ho1 = cz(c->hpsixold[c->ioldh[0]]);
-      ox.ho2 = cz(c->hpsixold[c->ioldh[1]]);
+      ox.partials = x_partials(c, k);
+      ox.h = make_hpsix<R>(c, k);
+      ox.psi = cz(x.psix);
+      ox.pc = cz(x.pcx);
+      ox.po3 = cz(x.psixold[c->ioldp[2]]);
+      ox.po2 = cz(x.psixold[c->ioldp[1]]);
+      ox.ho1 = cz(x.hpsixold[c->ioldh[0]]);
+      ox.ho2 = cz(x.hpsixold[c->ioldh[1]]);
       ox.den = nullptr;
       ox.dt = dt;
-      launch_step(c, c->psix, c->lapx, ox);
-      c->side_pre |= 8;  // errpsix reduction, with errHe's
+      launch_step(c, x.psix, x.lapx, ox);
+      c->side_pre |= 8;  // errpsix reductions, with errHe's
     }
     c->lap_fresh = 0;
     c->side_pre |= 4;  // errHe reduction, at the start of the next side branch

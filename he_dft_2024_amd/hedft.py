@@ -170,6 +170,8 @@ def lib():
                                                C.c_char_p]
         L.hedft_compute_observables.argtypes = [vp, ip, d, C.POINTER(Observables)]
         L.hedft_set_quantum_impurity.argtypes = [vp, vp, C.c_size_t, d, d]
+        L.hedft_set_quantum_impurities.argtypes = [vp, ip, vp, vp, vp, C.c_size_t, vp, d]
+        L.hedft_select_quantum_impurity.argtypes = [vp, ip]
         L.hedft_quantum_centroid.argtypes = [vp, dp, dp]
         L.hedft_job_bytes.restype = C.c_size_t
         L.hedft_job_bytes.argtypes = [C.POINTER(Job)]
@@ -345,6 +347,30 @@ class Droplet:
         k = np.ascontiguousarray(np.broadcast_to(np.asarray(uimp_k, dtype=np.float64), self.hshape))
         check(lib().hedft_set_quantum_impurity(self._c, _ptr(k), k.nbytes, float(m_imp_u), float(mu4)),
               "set_quantum_impurity")
+
+    def set_quantum_impurities(self, uimp_ks, masses_u, mu4=0.0, uimp_kk=None):
+        """N_imp quantum impurities: uimp_ks[i] = uimp_k(:,:,:,i) on the half
+        spectrum, masses_u[i] in u, uimp_kk = {(i, j): uimp_k_k(:,:,:,i,j)} for
+        the pairs whose Selec_gs_k_k is not 'null' (row i is the impurity that
+        feels the potential).  Selects impurity 0."""
+        nq = len(masses_u)
+        hs = self.hshape
+        uk = np.ascontiguousarray(np.stack([np.broadcast_to(np.asarray(u, dtype=np.float64), hs) for u in uimp_ks]))
+        ukk = np.zeros((nq, nq) + hs)
+        pair = np.zeros(nq * nq, dtype=np.int32)
+        for (i, j), u in (uimp_kk or {}).items():
+            if i == j:
+                raise ValueError("uimp_k_k(i,i) is not used")
+            ukk[i, j] = np.broadcast_to(np.asarray(u, dtype=np.float64), hs)
+            pair[i * nq + j] = 1
+        m = np.ascontiguousarray(masses_u, dtype=np.float64)
+        check(lib().hedft_set_quantum_impurities(self._c, nq, _ptr(uk), _ptr(ukk), pair.ctypes.data_as(C.c_void_p),
+                                                 uk[0].nbytes, _ptr(m), float(mu4)), "set_quantum_impurities")
+
+    def select_quantum(self, k):
+        """The quantum impurity the PSIX*, UPOTX, UIMPK fields and
+        quantum_centroid refer to."""
+        check(lib().hedft_select_quantum_impurity(self._c, int(k)), "select_quantum")
 
     def quantum_centroid(self):
         """(norm, centre) of |psix|^2."""

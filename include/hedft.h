@@ -43,6 +43,7 @@ extern "C" {
 #define HEDFT_PREC_MIXED 1   /* FP32 mean field/stencils, FP64 psi + sums     */
 
 #define HEDFT_MAX_IMP 4
+#define HEDFT_MAX_QIMP 4 /* quantum impurities (wave functions) per context */
 
 /* ---- field identifiers (module arrays of the reference) ------------------ */
 enum hedft_field {
@@ -301,6 +302,31 @@ int hedft_forceimp_noalloc(const hedft_grid* g, int32_t device, void* stream, co
  * Upload psix (HEDFT_PSIX) before hedft_tddft_init. */
 int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes, double m_imp_u,
                                double mu4);
+/* N_imp = nq quantum impurities on a TDDFT context (nq <= HEDFT_MAX_QIMP; a
+ * statics context takes nq = 1 only), replacing potenimpini.f90:19-44 and the
+ * N_imp loops of Dynamics_quantum_impurities: uimp_k holds nq half spectra
+ * (uimp_k(:,:,:,i), bytes each), uimp_kk nq*nq half spectra in the order
+ * (i, j) (uimp_k_k(:,:,:,i,j): the potential impurity j exerts on impurity i)
+ * and pair[i*nq+j] != 0 marks the pairs whose Selec_gs_k_k is not 'null'
+ * (uimp_kk and pair may be NULL: no pair coupling); m_imp_u[i] in u.  Each
+ * impurity has its own psix and PC history:
+ *   potx4   = sum_i F^-1[F(|psix_i|^2) uimp_k_i]
+ *   upotx_i = F^-1[fden uimp_k_i] + sum_{j != i} F^-1[F(|psix_j|^2) uimp_k_k(i,j)]
+ * every term transformed back on its own and summed in real space in that
+ * order (potenimp.f90:13-39); the steppers advance every psix with the same
+ * slot rotation; steppc's errimp is sum_i c9 sum|pcx_i| / nxyz
+ * (steppc.f90:115-139); hedft_energy's ekinx sums the impurities, eHeX uses
+ * the spectral sum and eimpu_impu = sum_{k<m} sum(F^-1[F(|psix_k|^2)
+ * uimp_k_k(k,m)] |psix_m|^2) dxyz (energy.f90:102-132).  The PSIX*, UPOTX and
+ * UIMPK fields and hedft_quantum_centroid refer to the impurity selected by
+ * hedft_select_quantum_impurity (0 after this call).  Upload every psix
+ * before hedft_tddft_init. */
+int hedft_set_quantum_impurities(hedft_ctx* c, int32_t nq, const double* uimp_k, const double* uimp_kk,
+                                 const int32_t* pair, size_t bytes, const double* m_imp_u, double mu4);
+/* The quantum impurity (0-based) the PSIX*, UPOTX, UIMPK fields and
+ * hedft_quantum_centroid refer to (no counterpart in the reference, whose
+ * arrays carry the impurity as a last index). */
+int hedft_select_quantum_impurity(hedft_ctx* c, int32_t k);
 /* Norm and centre of the impurity density, sum(|psix|^2)*dxyz and
  * sum(|psix|^2 r)*dxyz/norm (DFT4He3d.f90:682-700, the rimp_k file rows).
  * Synchronises. */

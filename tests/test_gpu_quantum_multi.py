@@ -1,0 +1,196 @@
+"""GPU parity of several quantum impurities (N_imp > 1, potenimp.f90:13-39,
+steprk.f90:84-106, steppc.f90:64-139, energy.f90:102-132) against the FP64
+oracle, through the C-ABI: two and three Lennard-Jones impurity wave packets in
+a seeded droplet, X-X pair kernels uimp_k_k on some pairs and 'null' on the
+rest; the single-impurity entry point is the N_imp = 1 case of the same code."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import relerr
+from he_dft_2024_amd import inputs as I
+from he_dft_2024_amd import quantum as Q
+
+pytestmark = pytest.mark.gpu
+
+RTOL_OP = 1e-12
+RTOL_TRAJ = 1e-9
+N, XMAX, N4 = 32, 6.4, 60.0
+DT = 5e-4 / I.PSTOK
+PACKETS = (dict(r0=(0.4, -0.8, 1.2), width=1.3, k=(0.6, 0.0, -0.3)),
+           dict(r0=(-1.5, 0.9, -0.6), width=1.1, k=(0.0, -0.4, 0.5)),
+           dict(r0=(1.8, 1.2, -1.4), width=1.5, k=(-0.3, 0.2, 0.0)))
+MASSES = (2 * 4.002603, 20.18, 6.94)
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+_K = {}
+
+
+def _kernels():
+    if not _K:
+        v = Q.lennard_jones(10.22, 2.556)
+        _K["x"] = Q.uimp_k_table(N, N, N, XMAX, XMAX, XMAX, v, 2.0, float(v(2.0)))
+        vxx = Q.lennard_jones(20.0, 3.0)
+        _K["xx"] = Q.uimp_k_table(N, N, N, XMAX, XMAX, XMAX, vxx, 2.5, float(vxx(2.5)))
+        vxy = Q.lennard_jones(12.0, 2.8)
+        _K["xy"] = Q.uimp_k_table(N, N, N, XMAX, XMAX, XMAX, vxy, 2.2, float(vxy(2.2)))
+    return _K
+
+
+def _setup(x, nq, pairs):
+    k = _kernels()
+    x.set("DEN", I.woods_saxon(N, N, N, XMAX, XMAX, XMAX, N4, seed=3))
+    uks = [k["x"], 0.7 * k["x"], k["xy"]][:nq]
+    ukk = {p: (k["xx"] if p[0] < p[1] else k["xy"]) for p in pairs}
+    x.set_quantum_impurities(uks, MASSES[:nq], mu4=-1.5, uimp_kk=ukk)
+    for q in range(nq):
+        x.select_quantum(q)
+        x.set("PSIX", I.gaussian_packet(N, N, N, XMAX, XMAX, XMAX, **PACKETS[q]))
+    x.select_quantum(0)
+
+
+def _both(gpu, nq, pairs, otc=1, precision=0):
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, gpu.default_params(core_otc=otc, precision=precision),
+                    mode=gpu.MODE_TDDFT)
+    o = O.Oracle(N, N, N, XMAX, XMAX, XMAX, O.default_params(core_otc=otc), mode=O.MODE_TDDFT)
+    for x in (d, o):
+        _setup(x, nq, pairs)
+    return d, o
+
+
+def _per_impurity(x, nq, *names):
+    out = []
+    for q in range(nq):
+        x.select_quantum(q)
+        out.append([x.field(n) for n in names])
+    x.select_quantum(0)
+    return out
+
+
+@pytest.mark.parametrize("otc,pairs", [(1, ((0, 1), (1, 0))), (0, ((1, 0),)), (1, ())])
+def test_two_impurities_potenimp_poten_vs_oracle(gpu, otc, pairs):
+    _say("2 quantum impurities potenimp+poten otc", otc, "pairs", pairs)
+    d, o = _both(gpu, 2, pairs, otc)
+    for x in (d, o):
+        x.potenimp()
+    assert relerr(d.field("POTX4"), o.field("POTX4")) < RTOL_OP
+    for q, (a, b) in enumerate(zip(_per_impurity(d, 2, "UPOTX"), _per_impurity(o, 2, "UPOTX"))):
+        assert relerr(a[0], b[0]) < RTOL_OP, q
+    for x in (d, o):
+        x.poten()
+    for name in ("POTX4", "DENCG", "POT4") + (("DENALF", "UALPHAS") if otc else ()):
+        assert relerr(d.field(name), o.field(name)) < RTOL_OP, name
+    for q, (a, b) in enumerate(zip(_per_impurity(d, 2, "UPOTX", "DENX"), _per_impurity(o, 2, "UPOTX", "PSIX"))):
+        assert relerr(a[0], b[0]) < RTOL_OP, q
+        assert relerr(a[1], np.abs(b[1]) ** 2) < 1e-15, q
+    d.close()
+
+
+@pytest.mark.parametrize("nq,pairs,otc", [(2, ((0, 1), (1, 0)), 1), (3, ((0, 1), (2, 0), (1, 2)), 0)])
+def test_impurities_trajectory_vs_oracle(gpu, nq, pairs, otc):
+    _say(nq, "quantum impurities 3 RK + 9 PC otc", otc, "pairs", pairs)
+    d, o = _both(gpu, nq, pairs, otc)
+    errs = []
+    for x in (d, o):
+        x.tddft_init()
+        errs.append(np.array([x.tddft_iteration(DT, it, 3) for it in range(1, 13)]))
+    assert relerr(errs[0][3:, :2], errs[1][3:, :2]) < RTOL_TRAJ
+    assert np.all(errs[0][3:, 1] > 0)
+    for name in ("PSI", "DEN", "POT4", "POTX4"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    names = ("PSIX", "UPOTX", "PSIXOLD1", "PSIXOLD2", "PSIXOLD3", "HPSIXOLD1", "HPSIXOLD2")
+    for q, (a, b) in enumerate(zip(_per_impurity(d, nq, *names), _per_impurity(o, nq, *names))):
+        for name, x, y in zip(names, a, b):
+            assert relerr(x, y) < RTOL_TRAJ, (q, name)
+    eg, eo = d.energy(), o.energy()
+    assert abs(eo["eimpu_impu"]) > 0
+    for k in ("ekin4", "elj4", "ecor4", "ekinx", "eHeX", "eimpu_impu", "eimpu", "etot", "norm"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
+    for q in range(nq):
+        d.select_quantum(q)
+        o.select_quantum(q)
+        (ng, rg), (no, ro) = d.quantum_centroid(), o.quantum_centroid()
+        assert abs(ng - no) < 1e-12 and np.abs(rg - ro).max() < 1e-10, q
+    d.close()
+
+
+def test_one_impurity_through_both_entry_points_is_bit_identical(gpu):
+    """hedft_set_quantum_impurity is hedft_set_quantum_impurities with N_imp = 1."""
+    _say("one impurity: single vs multi entry point")
+    k = _kernels()["x"]
+    res = []
+    for multi in (False, True):
+        d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+        d.set("DEN", I.woods_saxon(N, N, N, XMAX, XMAX, XMAX, N4, seed=3))
+        if multi:
+            d.set_quantum_impurities([k], MASSES[:1], mu4=-1.5)
+        else:
+            d.set_quantum_impurity(k, MASSES[0], -1.5)
+        d.set("PSIX", I.gaussian_packet(N, N, N, XMAX, XMAX, XMAX, **PACKETS[0]))
+        d.tddft_init()
+        e = [d.tddft_iteration(DT, it, 3) for it in range(1, 7)]
+        d.run_pc(DT, 6, True)
+        res.append((e, [d.field(n) for n in ("PSI", "PSIX", "POTX4", "UPOTX", "PCX")], d.energy()))
+        d.close()
+    assert res[0][0] == res[1][0]
+    for a, b in zip(res[0][1], res[1][1]):
+        assert np.array_equal(a, b)
+    assert res[0][2] == res[1][2]
+
+
+def test_impurities_graph_replay_and_mixed(gpu):
+    _say("2 quantum impurities: graph vs eager, mixed vs fp64")
+    pairs = ((0, 1), (1, 0))
+    outs = {}
+    for prec, use_graph in ((0, False), (0, True), (1, True)):
+        d, o = _both(gpu, 2, pairs, 1, precision=prec)
+        o.close()
+        d.tddft_init()
+        for it in range(1, 4):
+            d.tddft_iteration(DT, it, 3)
+        d.run_pc(DT, 13, use_graph)
+        d.run_pc(DT, 5, use_graph)
+        outs[(prec, use_graph)] = ([d.field("PSI")] + [f[0] for f in _per_impurity(d, 2, "PSIX")], d.energy())
+        d.close()
+    for a, b in zip(outs[(0, True)][0], outs[(0, False)][0]):
+        assert relerr(a, b) < 1e-13
+    for a, b in zip(outs[(1, True)][0], outs[(0, False)][0]):
+        assert relerr(a, b) < 2e-5
+    for k in ("etot", "ekinx", "eHeX", "eimpu_impu"):
+        e1, e0 = outs[(1, True)][1][k], outs[(0, False)][1][k]
+        assert abs(e1 - e0) < 1e-4 * abs(e0), k
+
+
+def test_impurities_abi_errors(gpu):
+    _say("quantum impurities abi errors")
+    L = gpu.lib()
+    hs = (N, N, N // 2 + 1)
+    k = np.zeros((4,) + hs)
+    kk = np.zeros((4, 4) + hs)
+    m = np.full(5, 4.0)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    nb = k[0].nbytes
+    d = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_TDDFT)
+    for nq in (0, 5):
+        assert L.hedft_set_quantum_impurities(d._c, nq, vp(k), None, None, nb, vp(m), 0.0) == -1
+    diag = np.zeros(4, dtype=np.int32)
+    diag[0] = 1   # uimp_k_k(1,1) is not a pair
+    assert L.hedft_set_quantum_impurities(d._c, 2, vp(k), vp(kk), vp(diag), nb, vp(m), 0.0) == -1
+    bad = m.copy()
+    bad[1] = 0.0
+    assert L.hedft_set_quantum_impurities(d._c, 2, vp(k), None, None, nb, vp(bad), 0.0) == -1
+    assert L.hedft_select_quantum_impurity(d._c, 1) == -1   # no impurity yet
+    assert L.hedft_set_quantum_impurities(d._c, 3, vp(k), None, None, nb, vp(m), 0.0) == 0
+    assert L.hedft_select_quantum_impurity(d._c, 2) == 0
+    assert L.hedft_select_quantum_impurity(d._c, 3) == -1
+    assert L.hedft_select_quantum_impurity(d._c, -1) == -1
+    d.close()
+    s = gpu.Droplet(N, N, N, XMAX, XMAX, XMAX, mode=gpu.MODE_STATICS)
+    assert L.hedft_set_quantum_impurities(s._c, 2, vp(k), None, None, nb, vp(m), 0.0) == -1   # statics: one
+    s.close()
