@@ -747,9 +747,9 @@ int hedft_set_quantum_impurities(hedft_ctx* c, int32_t nq, const double* uimp_k,
     if (!q.uimpk) rc = rc ? rc : alloc(c, (void**)&q.uimpk, kN);
     if (rc == HEDFT_OK) rc = upload_kernel(c, q.uimpk, uimp_k + k * nh, bytes);
     for (int m = 0; m < HEDFT_MAX_QIMP && rc == HEDFT_OK; ++m) {
-      const bool on = m < nq && m != k && uimp_kk && pair && pair[k * nq + m];
+      const bool on = m < nq && m != k && uimp_kk && pair && pair[k + m * nq];
       if (on && !c->uimpkk[k][m]) rc = alloc(c, (void**)&c->uimpkk[k][m], kN);
-      if (on && rc == HEDFT_OK) rc = upload_kernel(c, c->uimpkk[k][m], uimp_kk + ((size_t)k * nq + m) * nh, bytes);
+      if (on && rc == HEDFT_OK) rc = upload_kernel(c, c->uimpkk[k][m], uimp_kk + ((size_t)m * nq + k) * nh, bytes);
       if (!on && c->uimpkk[k][m]) {
         hipFree(c->uimpkk[k][m]);
         c->uimpkk[k][m] = nullptr;

@@ -1,8 +1,9 @@
 !> Drop-in replacements for the time-step subroutines of the quantum-impurity
 !> program of BCN-4He-DFT (Dynamics_quantum_impurities): STEPRK(deltat),
 !> STEPPC(deltat,errHe,errpsix), poten, potenimp and energy keep their names and
-!> signatures and run on the GPU through module hedft_gpu, for one impurity
-!> (N_imp = 1).  The module arrays (den, psi, psix, pot4, potx4, upotx, ...) stay
+!> signatures and run on the GPU through module hedft_gpu, for N_imp <= 4
+!> impurities with their uimp_k_k pair kernels.  The module arrays (den, psi,
+!> psix, pot4, potx4, upotx, ...) stay
 !> on the device between calls.  Two calls are added to the driver:
 !>   call hedft_dropin_start(Lambdah, txmean, tymean, tzmean, txsurf, tysurf, tzsurf)
 !> after timec has been set (before the time loop, DFT4He3d.f90:661), and
@@ -29,14 +30,16 @@ subroutine hedft_dropin_start(Lambdah, txmean, tymean, tzmean, txsurf, tysurf, t
   use deriva, only: npd, icon
   use rho, only: den, psi
   use impur, only: psix, mu4
-  use quantal_imp, only: N_imp, uimp_k, m_imp_u
+  use quantal_imp, only: N_imp, uimp_k, uimp_k_k, m_imp_u, selec_gs_k_k
   implicit none
   real(8), intent(in) :: Lambdah, txmean, tymean, tzmean, txsurf, tysurf, tzsurf
   type(hedft_grid_t) :: g
   type(hedft_params_t) :: p
   integer(c_size_t) :: n, nh
+  integer(c_int32_t) :: pair(N_imp, N_imp)
+  integer :: i, j
 
-  if (N_imp /= 1) stop 'hedft_dropin_start: one quantum impurity (N_imp = 1)'
+  if (N_imp < 1 .or. N_imp > HEDFT_MAX_QIMP) stop 'hedft_dropin_start: 1 <= N_imp <= 4'
   g%nx = nx; g%ny = ny; g%nz = nz; g%pad_ = 0
   g%xmax = xmax; g%ymax = ymax; g%zmax = zmax
   call hedft_default_params(p)
@@ -52,11 +55,22 @@ subroutine hedft_dropin_start(Lambdah, txmean, tymean, tzmean, txsurf, tysurf, t
   call hedft_check(hedft_create(g, p, HEDFT_MODE_TDDFT, 0_c_int32_t, hctx), 'create')
   n = int(nx, c_size_t) * ny * nz
   nh = int(nx / 2 + 1, c_size_t) * ny * nz
-  call hedft_check(hedft_set_quantum_impurity(hctx, uimp_k(:, :, :, 1), 8_c_size_t * nh, m_imp_u(1), mu4), &
-                   'set_quantum_impurity')
+  ! the pairs potenimp.f90:32 and energy.f90:123 skip: i = j and 'null' selectors
+  do j = 1, N_imp
+    do i = 1, N_imp
+      pair(i, j) = 0
+      if (i /= j .and. trim(adjustl(selec_gs_k_k(i, j))) /= 'null') pair(i, j) = 1
+    end do
+  end do
+  call hedft_check(hedft_set_quantum_impurities(hctx, int(N_imp, c_int32_t), uimp_k, uimp_k_k, pair, &
+                                                8_c_size_t * nh, m_imp_u, mu4), 'set_quantum_impurities')
   call hedft_check(hedft_upload(hctx, HEDFT_DEN, den, 8_c_size_t * n), 'upload den')
   call hedft_check(hedft_upload(hctx, HEDFT_PSI, psi, 16_c_size_t * n), 'upload psi')
-  call hedft_check(hedft_upload(hctx, HEDFT_PSIX, psix(:, :, :, 1), 16_c_size_t * n), 'upload psix')
+  do i = 1, N_imp
+    call hedft_check(hedft_select_quantum_impurity(hctx, int(i - 1, c_int32_t)), 'select')
+    call hedft_check(hedft_upload(hctx, HEDFT_PSIX, psix(:, :, :, i), 16_c_size_t * n), 'upload psix')
+  end do
+  call hedft_check(hedft_select_quantum_impurity(hctx, 0_c_int32_t), 'select')
 end subroutine hedft_dropin_start
 
 subroutine hedft_dropin_pull()
@@ -67,15 +81,21 @@ subroutine hedft_dropin_pull()
   use rho, only: den, psi
   use field, only: pot4
   use impur, only: psix, potx4, upotx
+  use quantal_imp, only: N_imp
   implicit none
   integer(c_size_t) :: n
+  integer :: i
   n = int(nx, c_size_t) * ny * nz
   call hedft_check(hedft_download(hctx, HEDFT_DEN, den, 8_c_size_t * n), 'download den')
   call hedft_check(hedft_download(hctx, HEDFT_PSI, psi, 16_c_size_t * n), 'download psi')
   call hedft_check(hedft_download(hctx, HEDFT_POT4, pot4, 8_c_size_t * n), 'download pot4')
-  call hedft_check(hedft_download(hctx, HEDFT_PSIX, psix(:, :, :, 1), 16_c_size_t * n), 'download psix')
   call hedft_check(hedft_download(hctx, HEDFT_POTX4, potx4, 8_c_size_t * n), 'download potx4')
-  call hedft_check(hedft_download(hctx, HEDFT_UPOTX, upotx(:, :, :, 1), 8_c_size_t * n), 'download upotx')
+  do i = 1, N_imp
+    call hedft_check(hedft_select_quantum_impurity(hctx, int(i - 1, c_int32_t)), 'select')
+    call hedft_check(hedft_download(hctx, HEDFT_PSIX, psix(:, :, :, i), 16_c_size_t * n), 'download psix')
+    call hedft_check(hedft_download(hctx, HEDFT_UPOTX, upotx(:, :, :, i), 8_c_size_t * n), 'download upotx')
+  end do
+  call hedft_check(hedft_select_quantum_impurity(hctx, 0_c_int32_t), 'select')
 end subroutine hedft_dropin_pull
 
 SUBROUTINE STEPRK(deltat)

@@ -13,7 +13,7 @@ module hedft_gpu
        HEDFT_EHIP = -3, HEDFT_ENOMEM = -4, HEDFT_ESTATE = -5, HEDFT_EBOUNDS = -6
   integer(c_int32_t), parameter :: HEDFT_MODE_STATICS = 0, HEDFT_MODE_TDDFT = 1
   integer(c_int32_t), parameter :: HEDFT_PREC_FP64 = 0, HEDFT_PREC_MIXED = 1
-  integer(c_int32_t), parameter :: HEDFT_MAX_IMP = 4
+  integer(c_int32_t), parameter :: HEDFT_MAX_IMP = 4, HEDFT_MAX_QIMP = 4
 
   ! field identifiers (enum hedft_field)
   integer(c_int32_t), parameter :: HEDFT_DEN = 0, HEDFT_PSI = 1, HEDFT_POT4 = 2, &
@@ -227,6 +227,22 @@ module hedft_gpu
       type(*), intent(in) :: uimp_k(*)
       integer(c_size_t), value :: nbytes
       real(c_double), value :: m_imp_u, mu4
+    end function
+    integer(c_int) function hedft_set_quantum_impurities(ctx, nq, uimp_k, uimp_kk, pair, nbytes, m_imp_u, mu4) &
+        bind(C, name="hedft_set_quantum_impurities")
+      import :: c_int, c_int32_t, c_ptr, c_double, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nq
+      type(*), intent(in) :: uimp_k(*), uimp_kk(*)
+      integer(c_int32_t), intent(in) :: pair(*)
+      integer(c_size_t), value :: nbytes
+      real(c_double), intent(in) :: m_imp_u(*)
+      real(c_double), value :: mu4
+    end function
+    integer(c_int) function hedft_select_quantum_impurity(ctx, k) bind(C, name="hedft_select_quantum_impurity")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: k
     end function
     integer(c_int) function hedft_quantum_centroid(ctx, norm, rcm) bind(C, name="hedft_quantum_centroid")
       import :: c_int, c_ptr, c_double

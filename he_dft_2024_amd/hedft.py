@@ -361,8 +361,8 @@ class Droplet:
         for (i, j), u in (uimp_kk or {}).items():
             if i == j:
                 raise ValueError("uimp_k_k(i,i) is not used")
-            ukk[i, j] = np.broadcast_to(np.asarray(u, dtype=np.float64), hs)
-            pair[i * nq + j] = 1
+            ukk[j, i] = np.broadcast_to(np.asarray(u, dtype=np.float64), hs)
+            pair[i + j * nq] = 1  # uimp_k_k(:,:,:,i,j) column-major
         m = np.ascontiguousarray(masses_u, dtype=np.float64)
         check(lib().hedft_set_quantum_impurities(self._c, nq, _ptr(uk), _ptr(ukk), pair.ctypes.data_as(C.c_void_p),
                                                  uk[0].nbytes, _ptr(m), float(mu4)), "set_quantum_impurities")

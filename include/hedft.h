@@ -305,10 +305,12 @@ int hedft_set_quantum_impurity(hedft_ctx* c, const double* uimp_k, size_t bytes,
 /* N_imp = nq quantum impurities on a TDDFT context (nq <= HEDFT_MAX_QIMP; a
  * statics context takes nq = 1 only), replacing potenimpini.f90:19-44 and the
  * N_imp loops of Dynamics_quantum_impurities: uimp_k holds nq half spectra
- * (uimp_k(:,:,:,i), bytes each), uimp_kk nq*nq half spectra in the order
- * (i, j) (uimp_k_k(:,:,:,i,j): the potential impurity j exerts on impurity i)
- * and pair[i*nq+j] != 0 marks the pairs whose Selec_gs_k_k is not 'null'
- * (uimp_kk and pair may be NULL: no pair coupling); m_imp_u[i] in u.  Each
+ * (uimp_k(:,:,:,i), bytes each), uimp_kk nq*nq half spectra laid out as the
+ * reference's uimp_k_k(nx/2+1,ny,nz,N_imp,N_imp) (column-major: spectrum
+ * (i, j), the potential impurity j exerts on impurity i, at offset
+ * (i + j*nq) * bytes), and pair[i + j*nq] != 0 marks the pairs whose
+ * Selec_gs_k_k(i,j) is not 'null' (i != j; uimp_kk and pair may be NULL: no
+ * pair coupling); m_imp_u[i] in u.  Each
  * impurity has its own psix and PC history:
  *   potx4   = sum_i F^-1[F(|psix_i|^2) uimp_k_i]
  *   upotx_i = F^-1[fden uimp_k_i] + sum_{j != i} F^-1[F(|psix_j|^2) uimp_k_k(i,j)]

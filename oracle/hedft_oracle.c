@@ -1222,9 +1222,10 @@ void oracle_update_aimp(octx* c) {
 /* ======================================================================= */
 /* potenimpini.f90:19-44 leaves uimp_k(:,:,:,i) and uimp_k_k(:,:,:,i,j) (zero
  * for i = j and for 'null' pairs); h2o2mimp(i) as steprk.f90:86.  uimpkk
- * holds nq*nq half spectra in the order (i, j) (row i = the impurity that
- * feels the potential), pair[i*nq+j] = 0 marks a 'null' pair; both may be
- * NULL for a single impurity. */
+ * holds nq*nq half spectra laid out as the Fortran array uimp_k_k(:,:,:,i,j)
+ * (column-major: spectrum (i, j) at offset (i + j*nq)*nh; row i = the impurity
+ * that feels the potential), pair[i + j*nq] = 0 marks a 'null' pair; both may
+ * be NULL for a single impurity. */
 void oracle_set_quantum_impurities(octx* c, int nq, const double* uimpk, const double* uimpkk, const int* pair,
                                    const double* m_imp_u, double mu4) {
   size_t n = c->n, nh = c->nh;
@@ -1257,9 +1258,9 @@ void oracle_set_quantum_impurities(octx* c, int nq, const double* uimpk, const d
     for (int m = 0; m < MAXQ; ++m) {
       free(c->uimpkk[k][m]);
       c->uimpkk[k][m] = NULL;
-      if (m < nq && m != k && uimpkk && pair && pair[k * nq + m]) {
+      if (m < nq && m != k && uimpkk && pair && pair[k + m * nq]) {
         c->uimpkk[k][m] = dalloc(nh);
-        memcpy(c->uimpkk[k][m], uimpkk + ((size_t)k * nq + m) * nh, sizeof(double) * nh);
+        memcpy(c->uimpkk[k][m], uimpkk + ((size_t)m * nq + k) * nh, sizeof(double) * nh);
       }
     }
   }

@@ -241,3 +241,85 @@ def test_dropin_statics_reference_names_match_python_sequence(gpu, tmp_path):
     assert mu == mus_f[-1]
     assert np.array_equal(psi_f, s.field("PSI"))
     s.close()
+
+
+QDEXE = os.path.join(FDIR, "_build", "hedft_dropin_quantum_run")
+
+
+@pytest.mark.gpu
+def test_dropin_quantum_reference_names_match_python_sequence(gpu, tmp_path):
+    """The quantum-impurity drop-in (hedft_dropin_quantum.o + the reference's own
+    Dynamics_quantum_impurities/modules.f90) with N_imp = 2: uimp_k_k(:,:,:,1,2)
+    set, Selec_gs_k_k(2,1) = 'null'.  STEPRK / STEPPC / potenimp / poten by
+    their reference names in the order of DFT4He3d.f90:667-677 are bit-identical
+    to the same entry points from Python."""
+    if not os.path.exists(QDEXE):
+        pytest.skip("drop-in driver program not built (reference tree or amdflang absent at build time)")
+    from he_dft_2024_amd import quantum as Q
+    n, xmax, n4, niter, nq, mu4 = 32, 6.4, 60.0, 6, 2, -1.5
+    dt = 5e-4 / I.PSTOK
+    print("[case] fortran quantum drop-in by reference names, 2 impurities, 32^3 3 RK + 3 PC", flush=True)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=4)
+    v, vxx = Q.lennard_jones(10.22, 2.556), Q.lennard_jones(20.0, 3.0)
+    uk = Q.uimp_k_table(n, n, n, xmax, xmax, xmax, v, 2.0, float(v(2.0)))
+    ukk = Q.uimp_k_table(n, n, n, xmax, xmax, xmax, vxx, 2.5, float(vxx(2.5)))
+    masses = np.array([2 * 4.002603, 20.18])
+    psix = [I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(0.4, -0.8, 1.2), width=1.3, k=(0.6, 0.0, -0.3)),
+            I.gaussian_packet(n, n, n, xmax, xmax, xmax, r0=(-1.5, 0.9, -0.6), width=1.1, k=(0.0, -0.4, 0.5))]
+    pair = np.zeros((nq, nq), dtype=np.int32)       # pair[j, i] = Fortran pair(i, j)
+    pair[1, 0] = 1                                    # uimp_k_k(:,:,:,1,2): impurity 2 acts on impurity 1
+    ukk_f = np.zeros((nq, nq) + uk.shape)             # [j, i] = uimp_k_k(:,:,:,i,j)
+    ukk_f[1, 0] = ukk
+    fin, fout = tmp_path / "qin.bin", tmp_path / "qout.bin"
+    with open(fin, "wb") as f:
+        np.array([n, niter, nq], dtype=np.int32).tofile(f)
+        np.array([xmax, dt, mu4], dtype=np.float64).tofile(f)
+        masses.tofile(f)
+        pair.tofile(f)
+        np.stack([uk, 0.7 * uk]).tofile(f)
+        ukk_f.tofile(f)
+        np.ascontiguousarray(den, dtype=np.float64).tofile(f)
+        np.stack(psix).astype(np.complex128).tofile(f)
+    r = subprocess.run([QDEXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    scal = raw[:6]
+    off, nn = 6, n ** 3
+    psi_f = raw[off:off + 2 * nn].view(np.complex128).reshape(n, n, n)
+    off += 2 * nn
+    psix_f = raw[off:off + 2 * nq * nn].view(np.complex128).reshape(nq, n, n, n)
+    off += 2 * nq * nn
+    upotx_f = raw[off:off + nq * nn].reshape(nq, n, n, n)
+    potx4_f = raw[off + nq * nn:].reshape(n, n, n)
+
+    psi0 = np.sqrt(den).astype(np.complex128)
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+    d.set_quantum_impurities([uk, 0.7 * uk], masses, mu4=mu4, uimp_kk={(0, 1): ukk})
+    d.set("DEN", np.abs(psi0) ** 2)
+    d.set("PSI", psi0)
+    for k in range(nq):
+        d.select_quantum(k)
+        d.set("PSIX", psix[k])
+    d.select_quantum(0)
+    d.potenimp()
+    d.poten()
+    err = (0.0, 0.0, 0.0)
+    for it in range(1, niter + 1):
+        if it <= 3:
+            d.steprk(dt)
+        else:
+            err = d.steppc(dt)
+        d.potenimp()
+        d.poten()
+    en = d.energy()
+    assert scal[0] == err[0] and scal[1] == err[1]
+    assert abs(en["eimpu_impu"]) > 0
+    for i, k in enumerate(("etot", "ekinx", "eHeX", "eimpu_impu")):
+        assert scal[2 + i] == en[k], k
+    assert np.array_equal(psi_f, d.field("PSI"))
+    assert np.array_equal(potx4_f, d.field("POTX4"))
+    for k in range(nq):
+        d.select_quantum(k)
+        assert np.array_equal(psix_f[k], d.field("PSIX")), k
+        assert np.array_equal(upotx_f[k], d.field("UPOTX")), k
+    d.close()
