@@ -638,6 +638,12 @@ __global__ void __launch_bounds__(TX * TY) k_lap25(const double2* __restrict__ P
   // no stencil)
   constexpr int NHALO = 12 * TX + 12 * TY, HU = (NHALO + NT - 1) / NT;
   __shared__ double2 buf[2][TH * TW];
+  // halo ring: the halos of planes z+1..z+6 at step z, plane p in slot p % 6.
+  // A plane's halo is read from global memory when the plane enters the z
+  // window (the step its neighbouring tiles read the same plane into theirs,
+  // so it comes from the L2) and parked here until the plane is staged; each
+  // thread reads back only the entries it wrote
+  __shared__ double2 ring[6 * NHALO];
   constexpr int NR = Out::NR;
   double red[NR > 0 ? NR : 1];
 #pragma unroll
@@ -762,22 +768,44 @@ __global__ void __launch_bounds__(TX * TY) k_lap25(const double2* __restrict__ P
     for (int u = 0; u < HU; ++u)
       if (hoff[u] >= 0) buf[0][hoff[u]] = vmul(hsg[u], hv[u]);
   }
+  // the halos of planes z0+1..z0+6 into the ring
+  {
+    double2 hp[6][HU];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const size_t pj = (size_t)min(z0 + 1 + j, z1 - 1) * plane;
+#pragma unroll
+      for (int u = 0; u < HU; ++u) hp[j][u] = P[pj + hsrc[u]];
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double2* R = ring + ((z0 + 1 + j) % 6) * NHALO;
+#pragma unroll
+      for (int u = 0; u < HU; ++u)
+        if (hoff[u] >= 0) R[threadIdx.x + u * NT] = vmul(hsg[u], hp[j][u]);
+    }
+  }
   // the epilogue's operands at the point, loaded one step ahead
   typename Out::Ops ops = o.load((size_t)z0 * plane + col, gx, gy, z0);
-  // Per step: the next plane's halo and queue value are loaded before the
-  // barrier and stored into the other LDS buffer after the stencils, the
-  // epilogue then runs on operands loaded in the previous step, and the next
-  // step's operands are loaded last.  Every load is issued unconditionally
-  // (from a clamped, valid address when its value is not needed), so each
-  // wait covers only loads issued well before it.
+  // Per step: the halo of plane z+7 (for the ring) and the next window value
+  // are loaded before the barrier; after the stencils the next plane is
+  // staged into the other LDS buffer from the window and the ring; the
+  // epilogue runs on operands loaded in the previous step; the next step's
+  // operands are loaded, and the halo of plane z+7 goes into the ring last,
+  // in the slot plane z+1 has just left.  Every load is issued
+  // unconditionally (from a clamped, valid address when its value is not
+  // needed), so each wait covers only loads issued well before it.
   for (int z = z0; z < z1; ++z) {
     const int zn = z + 1;
     const bool more = zn < z1;
     const int bn = more ? qbase(zn) : b;
-    const size_t pn = (size_t)min(zn, nz - 1) * plane;
+    // (past the chunk: the chunk's last plane again, an L2 hit)
+    const int zl = min(zn, z1 - 1);
+    const size_t pn = (size_t)zl * plane;
+    const size_t p7 = (size_t)min(z + 7, z1 - 1) * plane;
     double2 hv[HU];
 #pragma unroll
-    for (int u = 0; u < HU; ++u) hv[u] = P[pn + hsrc[u]];
+    for (int u = 0; u < HU; ++u) hv[u] = P[p7 + hsrc[u]];
     double nqf;
     const int nqz = zwrap(bn + 12, nqf);
     const double2 nqr = P[(size_t)nqz * plane + col];
@@ -796,16 +824,21 @@ __global__ void __launch_bounds__(TX * TY) k_lap25(const double2* __restrict__ P
 #pragma unroll
       for (int j = 0; j < 13; ++j) dz = vfma(cfz.rows[r][j], q[j], dz);
     }
+    double2* R = ring + (zn % 6) * NHALO;
     if (more) {
       double2* Tn = buf[(zn - z0) & 1];
       Tn[cpos] = centre(zn - b);
 #pragma unroll
       for (int u = 0; u < HU; ++u)
-        if (hoff[u] >= 0) Tn[hoff[u]] = vmul(hsg[u], hv[u]);
+        if (hoff[u] >= 0) Tn[hoff[u]] = R[threadIdx.x + u * NT];
     }
     const double2 sxy = make_double2(dx.x + dy.x, dx.y + dy.y);
     o.apply((size_t)z * plane + col, gx, gy, z, make_double2(sxy.x + dz.x, sxy.y + dz.y), cen, ops, red);
-    ops = o.load(pn + col, gx, gy, min(zn, nz - 1));
+    ops = o.load(pn + col, gx, gy, zl);
+    // plane z+7 takes the slot of plane z+1 (z+7 = z+1 mod 6)
+#pragma unroll
+    for (int u = 0; u < HU; ++u)
+      if (hoff[u] >= 0) R[threadIdx.x + u * NT] = vmul(hsg[u], hv[u]);
     if (bn != b) {
 #pragma unroll
       for (int j = 0; j < 12; ++j) q[j] = q[j + 1];
