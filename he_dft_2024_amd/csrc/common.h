@@ -351,6 +351,15 @@ struct hedft_ctx {
   double graph_dt;
   int graph_valid;
   int graph_slots[13];  // PC slot indices the graph was captured from
+  int graph_epoch;      // incremented by every capture of the period graph
+  // graph of the first rem_n (< period) PC iterations from one slot phase,
+  // captured after the period graph of epoch rem_epoch; rem_pre / rem_post:
+  // the host-side step state it starts from and leaves (HostStep, hedft.hip)
+  hipGraph_t rem_graph;
+  hipGraphExec_t rem_exec;
+  double rem_dt;
+  int rem_n, rem_epoch;
+  unsigned char rem_pre[512], rem_post[512];
   // tuning: x+y FFT launches with at most this many arrays use separate row and
   // column passes instead of one whole-plane workgroup per z-plane
   int xy_split_nb[2];  // [forward, inverse]

@@ -321,6 +321,8 @@ int hedft_destroy(hedft_ctx* c) {
   if (c->main_stream) hipStreamSynchronize(c->main_stream);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   if (c->graph) hipGraphDestroy(c->graph);
+  if (c->rem_exec) hipGraphExecDestroy(c->rem_exec);
+  if (c->rem_graph) hipGraphDestroy(c->rem_graph);
   void* ptrs[] = {c->den, c->pot4, c->dencg, c->delj4, c->denalf, c->intx, c->inty, c->intz, c->ualphas,
                   c->dxden, c->dyden, c->dzden, c->uext, c->sto1, c->ualph_conv, c->work1, c->work2,
                   c->timec_r, c->hpsi_r, c->psi1s, c->psi2s, c->psil, c->psi, c->psiold[0], c->psiold[1], c->psiold[2], c->hpsiold[0],
@@ -834,6 +836,64 @@ static void slot_state(const hedft_ctx* c, int* s) {
   std::memcpy(s + 11, c->iolda, 2 * sizeof(int));
 }
 
+// The host-side state a PC iteration advances: the slot indices, the psi /
+// spare pointers the steppers swap, and the side-branch bookkeeping.  A full
+// period returns all of it to where it started.
+struct HostStep {
+  int slots[13];
+  void *psi, *lap;
+  void *psix[HEDFT_MAX_QIMP], *lapx[HEDFT_MAX_QIMP];
+  int side_pre, side_args[9], lap_fresh, ndefer;
+  double side_dt;
+  size_t side_nblocks;
+  double2* defer_spec[2];
+  double* defer_out[2];
+};
+static_assert(sizeof(HostStep) <= sizeof(((hedft_ctx*)nullptr)->rem_pre), "rem_pre holds a HostStep");
+static void host_save(const hedft_ctx* c, HostStep& h) {
+  std::memset(&h, 0, sizeof h);
+  slot_state(c, h.slots);
+  h.psi = c->psi;
+  h.lap = c->lap;
+  for (int k = 0; k < HEDFT_MAX_QIMP; ++k) {
+    h.psix[k] = c->qi[k].psix;
+    h.lapx[k] = c->qi[k].lapx;
+  }
+  h.side_pre = c->side_pre;
+  std::memcpy(h.side_args, c->side_args, sizeof h.side_args);
+  h.lap_fresh = c->lap_fresh;
+  h.ndefer = c->ndefer;
+  h.side_dt = c->side_dt;
+  h.side_nblocks = c->side_nblocks;
+  for (int k = 0; k < 2; ++k) {
+    h.defer_spec[k] = c->defer_spec[k];
+    h.defer_out[k] = c->defer_out[k];
+  }
+}
+static void host_restore(hedft_ctx* c, const HostStep& h) {
+  std::memcpy(c->ioldp, h.slots, 3 * sizeof(int));
+  std::memcpy(c->ioldh, h.slots + 3, 2 * sizeof(int));
+  std::memcpy(c->ioldr, h.slots + 5, 3 * sizeof(int));
+  std::memcpy(c->ioldv, h.slots + 8, 3 * sizeof(int));
+  std::memcpy(c->iolda, h.slots + 11, 2 * sizeof(int));
+  c->psi = h.psi;
+  c->lap = h.lap;
+  for (int k = 0; k < HEDFT_MAX_QIMP; ++k) {
+    c->qi[k].psix = h.psix[k];
+    c->qi[k].lapx = h.lapx[k];
+  }
+  c->side_pre = h.side_pre;
+  std::memcpy(c->side_args, h.side_args, sizeof h.side_args);
+  c->lap_fresh = h.lap_fresh;
+  c->ndefer = h.ndefer;
+  c->side_dt = h.side_dt;
+  c->side_nblocks = h.side_nblocks;
+  for (int k = 0; k < 2; ++k) {
+    c->defer_spec[k] = h.defer_spec[k];
+    c->defer_out[k] = h.defer_out[k];
+  }
+}
+
 // Prepares a context for graph replay: x+y Laplacian of psi current, and a
 // graph of PC_PERIOD iterations captured from the current slot state and dt.
 static int ensure_graph(hedft_ctx* c, double deltat) {
@@ -865,11 +925,65 @@ static int ensure_graph(hedft_ctx* c, double deltat) {
   if (e != hipSuccess) return HEDFT_EHIP;
   c->graph = gr;
   HEDFT_CHECK(hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0));
+  // the device-side set-up of the executable graph is done here rather than at
+  // its first launch
+  HEDFT_CHECK(hipGraphUpload(c->graph_exec, c->stream));
   // capture advanced the host-side slot indices by one full period, i.e. back
   // to where they started
   c->graph_valid = 1;
   c->graph_dt = deltat;
+  c->graph_epoch++;
   std::memcpy(c->graph_slots, slots, sizeof slots);
+  return HEDFT_OK;
+}
+
+// the remainder graph replays the next nrem iterations from the current state
+static bool rem_ready(const hedft_ctx* c, double deltat, int nrem) {
+  if (!c->rem_exec || !c->graph_valid || c->rem_epoch != c->graph_epoch || c->rem_dt != deltat ||
+      c->rem_n != nrem)
+    return false;
+  HostStep h;
+  host_save(c, h);
+  return std::memcmp(&h, c->rem_pre, sizeof h) == 0;
+}
+
+// Captures the first nrem (< PC_PERIOD) iterations from the current state
+// into the remainder graph, after the period graph (ensure_graph), so that a
+// run of q periods + nrem iterations replays graphs only.  The capture
+// advances the host-side state by nrem iterations; it is put back, and the
+// advanced state is kept for the replay to apply.
+static int ensure_rem_graph(hedft_ctx* c, double deltat, int nrem) {
+  if (nrem <= 0 || nrem >= PC_PERIOD || !c->graph_valid) return HEDFT_OK;
+  if (rem_ready(c, deltat, nrem)) return HEDFT_OK;
+  if (c->rem_exec) hipGraphExecDestroy(c->rem_exec);
+  if (c->rem_graph) hipGraphDestroy(c->rem_graph);
+  c->rem_exec = nullptr;
+  c->rem_graph = nullptr;
+  HostStep pre, post;
+  host_save(c, pre);
+  HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = HEDFT_OK;
+  for (int s = 0; s < nrem && rc == HEDFT_OK; ++s) {
+    rc = steppc_impl(c, deltat);
+    if (rc == HEDFT_OK) rc = tddft_after_step(c);
+  }
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  host_save(c, post);
+  host_restore(c, pre);
+  if (rc != HEDFT_OK) {
+    if (gr) hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return HEDFT_EHIP;
+  c->rem_graph = gr;
+  HEDFT_CHECK(hipGraphInstantiate(&c->rem_exec, gr, nullptr, nullptr, 0));
+  HEDFT_CHECK(hipGraphUpload(c->rem_exec, c->stream));
+  c->rem_dt = deltat;
+  c->rem_n = nrem;
+  c->rem_epoch = c->graph_epoch;
+  std::memcpy(c->rem_pre, &pre, sizeof pre);
+  std::memcpy(c->rem_post, &post, sizeof post);
   return HEDFT_OK;
 }
 
@@ -881,6 +995,13 @@ static int enqueue_pc(hedft_ctx* c, double deltat, int nsteps, int use_graph) {
     HEDFT_TRY(ensure_graph(c, deltat));
     for (; done_steps + PC_PERIOD <= nsteps; done_steps += PC_PERIOD)
       HEDFT_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  }
+  if (use_graph && done_steps < nsteps && rem_ready(c, deltat, nsteps - done_steps)) {
+    HEDFT_CHECK(hipGraphLaunch(c->rem_exec, c->stream));
+    HostStep post;
+    std::memcpy(&post, c->rem_post, sizeof post);
+    host_restore(c, post);
+    done_steps = nsteps;
   }
   for (; done_steps < nsteps; ++done_steps) {
     HEDFT_TRY(steppc_impl(c, deltat));
@@ -914,7 +1035,10 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
   HEDFT_TRY(imp_ready(c));
   if (!c->rk_done) return HEDFT_ESTATE;
   hipSetDevice(c->device);
-  if (use_graph && nsteps >= PC_PERIOD) HEDFT_TRY(ensure_graph(c, deltat));  // capture outside the timing
+  if (use_graph && nsteps >= PC_PERIOD && !c->prefer_eager) {  // captures outside the timing
+    HEDFT_TRY(ensure_graph(c, deltat));
+    HEDFT_TRY(ensure_rem_graph(c, deltat, nsteps % PC_PERIOD));
+  }
   HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
   HEDFT_TRY(enqueue_pc(c, deltat, nsteps, use_graph));
   HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
@@ -924,6 +1048,36 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
   c->last_ms = ms;
   if (elapsed_ms) *elapsed_ms = ms;
   return done(c);
+}
+
+int hedft_tddft_prepare_pc(hedft_ctx* c, double deltat, int32_t skip, int32_t nsteps) {
+  if (!c || c->mode != HEDFT_MODE_TDDFT || skip < 0 || nsteps < 0) return HEDFT_EINVAL;
+  HEDFT_TRY(imp_ready(c));
+  if (!c->rk_done) return HEDFT_ESTATE;
+  if (nsteps < PC_PERIOD || c->prefer_eager) return HEDFT_OK;
+  hipSetDevice(c->device);
+  HostStep pre;
+  host_save(c, pre);
+  int rc = HEDFT_OK;
+  // the host-side state repeats every PC_PERIOD iterations: bring it to the
+  // phase of the run by capturing skip % PC_PERIOD iterations into a graph
+  // that is thrown away (capture launches nothing)
+  const int sk = skip % PC_PERIOD;
+  if (sk > 0) {
+    HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < sk && rc == HEDFT_OK; ++s) {
+      rc = steppc_impl(c, deltat);
+      if (rc == HEDFT_OK) rc = tddft_after_step(c);
+    }
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (gr) hipGraphDestroy(gr);
+    if (rc == HEDFT_OK && e != hipSuccess) rc = HEDFT_EHIP;
+  }
+  if (rc == HEDFT_OK) rc = ensure_graph(c, deltat);
+  if (rc == HEDFT_OK) rc = ensure_rem_graph(c, deltat, nsteps % PC_PERIOD);
+  host_restore(c, pre);
+  return rc;
 }
 
 int hedft_batch_run_pc(hedft_ctx* const* ctxs, int32_t nctx, const double* deltat, int32_t nsteps,

@@ -154,6 +154,7 @@ def lib():
         L.hedft_statics_refresh.argtypes = [vp]
         L.hedft_tddft_iteration.argtypes = [vp, d, ip, ip, dp, dp, dp]
         L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
+        L.hedft_tddft_prepare_pc.argtypes = [vp, d, ip, ip]
         L.hedft_last_elapsed.argtypes = [vp, dp]
         L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
         L.hedft_batch_run_pc.argtypes = [C.POINTER(vp), ip, dp, ip, ip, dp]
@@ -452,6 +453,11 @@ class Droplet:
         ms = C.c_double(0)
         check(lib().hedft_tddft_run_pc(self._c, dt, nsteps, int(use_graph), C.byref(ms)), "run_pc")
         return ms.value
+
+    def prepare_pc(self, dt, skip, nsteps):
+        """Capture now (host only) the graphs run_pc(dt, nsteps) will replay
+        after `skip` more PC iterations; nothing runs on the GPU."""
+        check(lib().hedft_tddft_prepare_pc(self._c, dt, int(skip), int(nsteps)), "prepare_pc")
 
     def autotune(self, dt, trial_steps=12):
         """Time the launch-structure choices on this droplet (it advances) and

@@ -380,9 +380,18 @@ int hedft_tddft_iteration(hedft_ctx* c, double deltat, int32_t iter, int32_t nrk
                           double* errHe, double* errimp, double* errvimp);
 /* nsteps PC iterations (after the RK start) with the per-step work captured
  * once in a HIP graph and replayed; returns the elapsed device time of the
- * replays measured with HIP events on the context's stream (ms). */
+ * replays measured with HIP events on the context's stream (ms).  The graph
+ * holds one period of the slot rotation (6 iterations); a remainder of
+ * nsteps % 6 iterations replays a second graph captured from the same slot
+ * phase, so no iteration of a graph run is launched one kernel at a time. */
 int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_graph,
                        double* elapsed_ms);
+/* No counterpart in the reference (its time loop has no graphs): captures, on
+ * the host only, the graphs that hedft_tddft_run_pc(c, deltat, nsteps, 1)
+ * will replay once skip more PC iterations have run (by any entry point), so
+ * that the capture does not happen between those iterations and the run.
+ * Nothing is launched and the context's state is unchanged. */
+int hedft_tddft_prepare_pc(hedft_ctx* c, double deltat, int32_t skip, int32_t nsteps);
 
 /* Batched ensemble (no counterpart in the reference; the reference runs one
  * droplet per process): advances nctx independent TDDFT contexts (any grid,

@@ -297,6 +297,40 @@ def test_graph_replay_matches_eager(gpu):
     assert relerr(outs[1][1], outs[0][1]) < 1e-13
 
 
+def test_graph_remainder_replays_match_eager(gpu):
+    """Runs of q periods + r < 6 iterations replay the period graph q times and
+    a graph of the r remaining iterations captured from the same slot phase
+    (no eager steps inside the timed region); every field matches eager steps
+    bit for bit, also as the phase moves between runs and a captured
+    remainder graph is reused or recaptured, and when the graphs of a run were
+    captured ahead of the iterations before it (hedft_tddft_prepare_pc)."""
+    n, xmax, n4 = 32, 6.4, 60.0
+    _say("graph remainder vs eager")
+    import he_dft_2024_amd as H
+    p = H.default_params(core_otc=1)
+    dt = 5e-4 / I.PSTOK
+    names = ("PSI", "DEN", "POT4", "PC", "PSIOLD1", "PSIOLD2", "PSIOLD3", "HPSIOLD1", "HPSIOLD2")
+    outs = []
+    for use_graph, prepare in ((False, False), (True, False), (True, True)):
+        d = H.Droplet(n, n, n, xmax, xmax, xmax, p)
+        I.tddft_setup(d, n, xmax, n4, seed=3, r0=(0.0, 0.0, -4.0))
+        d.tddft_init()
+        for it in range(1, 4):
+            d.tddft_iteration(dt, it, 3)
+        if prepare:
+            d.prepare_pc(dt, 3, 20)
+        snaps = []
+        for k in (3, 20, 20, 8, 14, 5, 11):
+            d.run_pc(dt, k, use_graph)
+            snaps.append([d.field(x) for x in names] + [d.impurity_state()["rimp"]])
+        outs.append(snaps)
+        d.close()
+    for v in (1, 2):
+        for i, (a, b) in enumerate(zip(outs[v], outs[0])):
+            for name, x, y in zip(names + ("rimp",), a, b):
+                assert np.array_equal(x, y), (v, i, name)
+
+
 def test_uimp_on_grid_line_vs_oracle(gpu):
     """An impurity on a grid line puts grid points at integral r/DelInter
     (r = 4 A exactly here), where the reference's int()/mod() interpolation

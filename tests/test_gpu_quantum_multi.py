@@ -167,6 +167,24 @@ def test_impurities_graph_replay_and_mixed(gpu):
         assert abs(e1 - e0) < 1e-4 * abs(e0), k
 
 
+def test_impurities_graph_remainder_matches_eager(gpu):
+    _say("2 quantum impurities: period + remainder graphs vs eager")
+    outs = []
+    for use_graph in (False, True):
+        d, o = _both(gpu, 2, ((0, 1), (1, 0)), 1)
+        o.close()
+        d.tddft_init()
+        for it in range(1, 4):
+            d.tddft_iteration(DT, it, 3)
+        d.run_pc(DT, 16, use_graph)
+        d.run_pc(DT, 9, use_graph)
+        outs.append([d.field("PSI"), d.field("POTX4")] + [f for fs in _per_impurity(d, 2, "PSIX", "UPOTX", "PCX")
+                                                         for f in fs])
+        d.close()
+    for a, b in zip(outs[1], outs[0]):
+        assert np.array_equal(a, b)
+
+
 def test_impurities_abi_errors(gpu):
     _say("quantum impurities abi errors")
     L = gpu.lib()
