@@ -250,15 +250,28 @@ int admit(const hedft_job& j, int device, hedft_ctx** out) {
   if (rc == HEDFT_OK) rc = hedft_upload(c, HEDFT_DEN, rho.data(), rho.size() * sizeof(double));
   if (j.quantum) {
     if (!(j.width > 0.0) || !(j.mass_amu > 0.0)) rc = HEDFT_EINVAL;
-    const std::vector<double> uk = rc == HEDFT_OK ? uimp_k_lj(j) : std::vector<double>();
-    if (rc == HEDFT_OK) rc = hedft_set_quantum_impurity(c, uk.data(), uk.size() * sizeof(double), j.mass_amu, 0.0);
+    const size_t nh = (size_t)(j.n / 2 + 1) * j.n * j.n;
+    std::vector<double> uk;
+    if (rc == HEDFT_OK && !j.uimp_k) uk = uimp_k_lj(j);
+    const double* ukp = j.uimp_k ? j.uimp_k : uk.data();
+    if (rc == HEDFT_OK) rc = hedft_set_quantum_impurity(c, ukp, nh * sizeof(double), j.mass_amu, 0.0);
     if (rc == HEDFT_OK) {
       const std::vector<double2> px = wave_packet(j, c->qi[0].h2o2mimp);
       rc = hedft_upload(c, HEDFT_PSIX, px.data(), px.size() * sizeof(double2));
     }
   } else {
     double delinter = 0.0, rmax = 0.0;
-    const std::vector<double> tab = lj_table(j, &delinter, &rmax);
+    std::vector<double> tab;
+    if (j.potion) {
+      if (j.npot < 2 || !(j.delinter > 0.0)) rc = HEDFT_EINVAL;
+      else {
+        tab.assign(j.potion, j.potion + j.npot);
+        delinter = j.delinter;
+        rmax = j.delinter * (j.npot - 1);
+      }
+    } else {
+      tab = lj_table(j, &delinter, &rmax);
+    }
     const double mass = j.mass_amu * kMpU;
     const double r0[3] = {j.r0[0], j.r0[1], j.r0[2]};
     const double v0[3] = {j.v0[0] * kPstok, j.v0[1] * kPstok, j.v0[2] * kPstok};

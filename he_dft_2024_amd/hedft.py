@@ -69,7 +69,9 @@ class Job(C.Structure):
                 ("nsteps", C.c_int32), ("nrk", C.c_int32), ("h", C.c_double), ("n4", C.c_double),
                 ("dt", C.c_double), ("r0", C.c_double * 3), ("v0", C.c_double * 3),
                 ("mass_amu", C.c_double), ("eps", C.c_double), ("sigma", C.c_double),
-                ("seed", C.c_uint64), ("quantum", C.c_int32), ("pad_", C.c_int32), ("width", C.c_double)]
+                ("seed", C.c_uint64), ("quantum", C.c_int32), ("pad_", C.c_int32), ("width", C.c_double),
+                ("potion", C.c_void_p), ("npot", C.c_int32), ("pad2_", C.c_int32), ("delinter", C.c_double),
+                ("uimp_k", C.c_void_p)]
 
 
 class JobResult(C.Structure):
@@ -504,10 +506,23 @@ def batch_run_pc(droplets, dt, nsteps, chunk=6):
 
 def make_job(arrival_s=0.0, n=32, h=0.4, n4=60.0, nsteps=30, nrk=3, dt=5e-4 / 7.63822291, precision=PREC_FP64,
              r0=(0.0, 0.0, -4.0), v0=(0.0, 0.0, 3.0), mass_amu=108.0, eps=20.0, sigma=4.0, seed=0, quantum=False,
-             width=1.5):
-    """hedft_job: classical impurity, or with quantum=True a wave packet of the given width."""
-    return Job(arrival_s, n, precision, nsteps, nrk, h, n4, dt, (C.c_double * 3)(*r0), (C.c_double * 3)(*v0),
-               mass_amu, eps, sigma, seed, int(bool(quantum)), 0, width)
+             width=1.5, potion=None, delinter=0.0, uimp_k=None):
+    """hedft_job: classical impurity, or with quantum=True a wave packet of the given width.
+    The He-X potential is LJ(eps, sigma) unless potion (classical: V on r_i = i delinter)
+    or uimp_k (quantum: the half-spectrum kernel) is given."""
+    j = Job(arrival_s, n, precision, nsteps, nrk, h, n4, dt, (C.c_double * 3)(*r0), (C.c_double * 3)(*v0),
+            mass_amu, eps, sigma, seed, int(bool(quantum)), 0, width)
+    keep = []  # the arrays the job points to live as long as the job object
+    if potion is not None:
+        t = np.ascontiguousarray(potion, dtype=np.float64)
+        keep.append(t)
+        j.potion, j.npot, j.delinter = t.ctypes.data, t.size, float(delinter)
+    if uimp_k is not None:
+        u = np.ascontiguousarray(np.broadcast_to(np.asarray(uimp_k, dtype=np.float64), (n, n, n // 2 + 1)))
+        keep.append(u)
+        j.uimp_k = u.ctypes.data
+    j._keep = keep
+    return j
 
 
 def job_bytes(job) -> int:

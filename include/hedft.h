@@ -456,6 +456,17 @@ typedef struct hedft_job {
   int32_t quantum;           /* 0: classical impurity, 1: quantum impurity (wave packet) */
   int32_t pad_;
   double width;              /* quantum: Gaussian width of the packet (A) */
+  /* Caller-supplied He-X potential in place of the LJ eps/sigma above (any
+   * potential, e.g. one of the reference's V_impur.f90 catalogue evaluated on
+   * the host, as its potenimpini does); NULL: LJ.  The arrays are read while
+   * the job is admitted and must stay valid until hedft_serve returns. */
+  const double* potion;      /* classical: npot values V(i delinter), i < npot, rmax = (npot-1) delinter
+                                (potenimpini.f90:31-59, the table hedft_set_impurities takes) */
+  int32_t npot;
+  int32_t pad2_;
+  double delinter;
+  const double* uimp_k;      /* quantum: uimp_k on the half spectrum, (n/2+1) n n values
+                                (potenimpini.f90:19-29, as hedft_set_quantum_impurity takes) */
 } hedft_job;
 
 typedef struct hedft_job_result {

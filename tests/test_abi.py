@@ -70,3 +70,35 @@ def test_algorithmic_bytes_match_the_counting_convention():
     N, Nh = n ** 3, (n // 2 + 1) * n * n
     assert bench.algorithmic_bytes_per_pc_step(n) == 832 * N + 1472 * Nh
     assert abs(bench.algorithmic_bytes_per_pc_step(n) - 3.3125e9) < 0.001e9
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every public struct of include/hedft.h has the size and field offsets
+    its ctypes mirror in he_dft_2024_amd.hedft assumes (gcc on the header)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    from he_dft_2024_amd import hedft as H
+    if not shutil.which("gcc"):
+        pytest.skip("gcc absent")
+    pairs = {"hedft_grid": H.Grid, "hedft_params": H.Params, "hedft_energies": H.Energies,
+             "hedft_observables": H.Observables, "hedft_denfile": H.DenFile, "hedft_job": H.Job,
+             "hedft_job_result": H.JobResult, "hedft_serve_opts": H.ServeOpts, "hedft_serve_stats": H.ServeStats,
+             "hedft_tuning": H.Tuning}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hedft.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (l.split() for l in out if l)}
+    for cname, py in pairs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -121,3 +121,67 @@ def test_serve_quantum_job_matches_python_setup(gpu):
     for key in ("etot", "etot4", "ekin4", "norm"):
         assert abs(r[key] - e[key]) <= 1e-9 * abs(e[key]), (key, r[key], e[key])
     assert np.abs(np.array(r["rimp"]) - rc).max() < 1e-9
+
+
+def _serve_density(n, h, n4, seed):
+    xm = 0.5 * n * h
+    x = -xm + h * np.arange(n)
+    rr = np.sqrt(x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2)
+    rho = 0.0218 / (1.0 + np.exp((rr - 2.22 * n4 ** (1.0 / 3.0)) / 0.7))
+    rho = rho * (1.0 + 1e-3 * _splitmix_uniform(seed, n ** 3).reshape(n, n, n))
+    return rho * (n4 / (rho.sum() * h ** 3))
+
+
+def test_serve_caller_supplied_potentials(gpu):
+    """Jobs that bring their own He-X potential (hedft_job.potion for a classical
+    impurity, hedft_job.uimp_k for a quantum one) in place of the LJ parameters
+    equal the same set-up through the Python binding."""
+    from he_dft_2024_amd import inputs as I
+    from he_dft_2024_amd import quantum as Q
+    print("[case] serve jobs with caller-supplied potentials", flush=True)
+    n, h, n4, nsteps, nrk = 32, 0.4, 70.0, 12, 3
+    xm = 0.5 * n * h
+    rho = _serve_density(n, h, n4, 78)
+    # classical: a capped Morse potential on the potenimpini grid
+    npot, rmax = 100 * n + 1, 4.0 * xm
+    dl = rmax / (npot - 1)
+    r = dl * np.arange(npot)
+    morse = 15.0 * (np.exp(-2.0 * 1.1 * (r - 4.2)) - 2.0 * np.exp(-1.1 * (r - 4.2)))
+    morse = np.minimum(morse, 300.0)
+    jc = gpu.make_job(n=n, h=h, n4=n4, nsteps=nsteps, nrk=nrk, r0=(0.2, 0.1, -3.0), v0=(0.0, 0.5, 3.0), seed=78,
+                      mass_amu=84.0, potion=morse, delinter=dl)
+    # quantum: a kernel from the Python transform of a potential that is not LJ
+    v = (lambda rr: -8.0 * np.exp(-((rr - 3.5) / 1.2) ** 2) + 40.0 * np.exp(-rr / 0.8))
+    uk = Q.uimp_k_table(n, n, n, xm, xm, xm, v, 1.0, float(v(1.0)))
+    jq = gpu.make_job(n=n, h=h, n4=n4, nsteps=nsteps, nrk=nrk, dt=5e-5 / I.PSTOK, r0=(0.3, -0.2, -3.0),
+                      v0=(0.0, 1.0, 4.0), seed=78, quantum=True, mass_amu=4.0, width=1.5, uimp_k=uk)
+    res, _ = gpu.serve([jc, jq], hbm_budget=4 * (gpu.job_bytes(jc) + gpu.job_bytes(jq)), chunk=6)
+    assert res[0]["status"] == 0 and res[1]["status"] == 0
+    # the same two set-ups through the binding
+    d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1))
+    d.set("DEN", rho)
+    d.set_impurities(morse[None, :], dl, rmax, [84.0 * I.MP_U])
+    d.set_impurity_state(np.array([jc.r0[:]]), np.array([jc.v0[:]]) * I.PSTOK, np.zeros((1, 3)))
+    d.tddft_init()
+    for it in range(1, nrk + 1):
+        d.tddft_iteration(jc.dt, it, nrk)
+    d.run_pc(jc.dt, nsteps)
+    ec, rc = d.energy(), d.impurity_state()["rimp"][0]
+    d.close()
+    h2o2mimp = 6.05969638298 * 4.002603 / 4.0
+    k = np.array(jq.v0[:]) * I.PSTOK / (2.0 * h2o2mimp)
+    d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1))
+    d.set("DEN", rho)
+    d.set_quantum_impurity(uk, 4.0, 0.0)
+    d.set("PSIX", I.gaussian_packet(n, n, n, xm, xm, xm, r0=tuple(jq.r0[:]), width=1.5, k=tuple(k)))
+    d.tddft_init()
+    for it in range(1, nrk + 1):
+        d.tddft_iteration(jq.dt, it, nrk)
+    d.run_pc(jq.dt, nsteps)
+    eq = d.energy()
+    _, rq = d.quantum_centroid()
+    d.close()
+    for got, e, rim in ((res[0], ec, rc), (res[1], eq, rq)):
+        for key in ("etot", "etot4", "ekin4", "norm"):
+            assert abs(got[key] - e[key]) <= 1e-12 * abs(e[key]), (key, got[key], e[key])
+        assert np.abs(np.array(got["rimp"]) - rim).max() < 1e-11
