@@ -352,6 +352,7 @@ struct hedft_ctx {
   int graph_valid;
   int graph_slots[13];  // PC slot indices the graph was captured from
   int graph_epoch;      // incremented by every capture of the period graph
+  double graph_n4;      // statics: the atom number the graph was captured with
   // graph of the first rem_n (< period) PC iterations from one slot phase,
   // captured after the period graph of epoch rem_epoch; rem_pre / rem_post:
   // the host-side step state it starts from and leaves (HostStep, hedft.hip)

@@ -730,6 +730,7 @@ int hedft_set_quantum_impurities(hedft_ctx* c, int32_t nq, const double* uimp_k,
   HEDFT_CHECK(hipStreamSynchronize(c->stream));  // queued work may still read the old kernels
   if (c->mode == HEDFT_MODE_STATICS) {
     if (nq != 1) return HEDFT_EINVAL;  // He_statics carries one impurity wave function
+    c->graph_valid = 0;
     return set_quantum_statics(c, uimp_k, bytes, m_imp_u[0]);
   }
   const size_t fw = c->mixed ? sizeof(float) : sizeof(double);
@@ -937,6 +938,41 @@ static int ensure_graph(hedft_ctx* c, double deltat) {
   return HEDFT_OK;
 }
 
+// statics: a graph of PC_PERIOD iterations of the driver loop (STEPPCR,
+// refresh, poten) from the current slot state, dt and atom number
+__global__ void k_copy_scal_mf(double* s, int from, int to) { s[to] = s[from]; }
+static int ensure_statics_graph(hedft_ctx* c, double deltat, double n4) {
+  int slots[13];
+  slot_state(c, slots);
+  if (c->graph_valid && c->graph_dt == deltat && c->graph_n4 == n4 &&
+      std::memcmp(slots, c->graph_slots, sizeof slots) == 0)
+    return HEDFT_OK;
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = 0;
+  HEDFT_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = HEDFT_OK;
+  for (int s = 0; s < PC_PERIOD && rc == HEDFT_OK; ++s) rc = statics_pc_iter(c, deltat, n4);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  if (rc != HEDFT_OK) {
+    if (gr) hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return HEDFT_EHIP;
+  c->graph = gr;
+  HEDFT_CHECK(hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0));
+  HEDFT_CHECK(hipGraphUpload(c->graph_exec, c->stream));
+  c->graph_valid = 1;
+  c->graph_dt = deltat;
+  c->graph_n4 = n4;
+  c->graph_epoch++;
+  std::memcpy(c->graph_slots, slots, sizeof slots);
+  return HEDFT_OK;
+}
+
 // the remainder graph replays the next nrem iterations from the current state
 static bool rem_ready(const hedft_ctx* c, double deltat, int nrem) {
   if (!c->rem_exec || !c->graph_valid || c->rem_epoch != c->graph_epoch || c->rem_dt != deltat ||
@@ -1047,6 +1083,47 @@ int hedft_tddft_run_pc(hedft_ctx* c, double deltat, int32_t nsteps, int32_t use_
   HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   if (elapsed_ms) *elapsed_ms = ms;
+  return done(c);
+}
+
+int hedft_statics_run(hedft_ctx* c, double deltat, int32_t iter0, int32_t niter, double n4, int32_t use_graph,
+                      double* mu4, double* mu4err, double* errpc, double* elapsed_ms) {
+  if (!c || c->mode != HEDFT_MODE_STATICS || iter0 < 1 || niter < 1 || !(n4 > 0.0)) return HEDFT_EINVAL;
+  if (iter0 > 4 && !c->rk_done) return HEDFT_ESTATE;
+  hipSetDevice(c->device);
+  const int it1 = iter0 + niter;  // one past the last iteration
+  const int first_pc = std::max(iter0, 5);
+  // PC iterations before the last one replay the graph in whole periods
+  const int npc = std::max(0, it1 - first_pc);
+  const int q = (use_graph && !c->prefer_eager) ? (npc - 1) / PC_PERIOD : 0;
+  if (q > 0 && iter0 > 4) HEDFT_TRY(ensure_statics_graph(c, deltat, n4));  // capture outside the timing
+  HEDFT_CHECK(hipEventRecord(c->ev0, c->stream));
+  int it = iter0;
+  for (; it < it1 && it <= 4; ++it) {
+    if (it == it1 - 1) hipLaunchKernelGGL(k_copy_scal_mf, dim3(1), dim3(1), 0, c->stream, c->scal, S_HMEAN, S_MUOLD);
+    HEDFT_TRY(statics_rk_iter(c, deltat, n4));
+  }
+  if (q > 0) {
+    HEDFT_TRY(ensure_statics_graph(c, deltat, n4));  // a no-op unless the RK start just ended
+    for (int k = 0; k < q; ++k) HEDFT_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+    it += q * PC_PERIOD;
+  }
+  for (; it < it1; ++it) {
+    if (it == it1 - 1) hipLaunchKernelGGL(k_copy_scal_mf, dim3(1), dim3(1), 0, c->stream, c->scal, S_HMEAN, S_MUOLD);
+    HEDFT_TRY(statics_pc_iter(c, deltat, n4));
+  }
+  HEDFT_CHECK(hipEventRecord(c->ev1, c->stream));
+  HEDFT_CHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HEDFT_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if (elapsed_ms) *elapsed_ms = ms;
+  double sc[5];
+  HEDFT_TRY(read_scal(c, 0, 5, sc));
+  // mu4err = |1 - mu4/hmean| with mu4 the previous iteration's hmean (steprkr.f90:93, steppcr.f90:104)
+  if (mu4err) *mu4err = std::fabs(1.0 - sc[S_MUOLD] / sc[S_HMEAN]);
+  if (mu4) *mu4 = sc[S_HMEAN];
+  if (errpc) *errpc = (it1 - 1 > 4) ? (9.0 / 121.0) * sc[S_ERRPC] / (double)c->d.n : 0.0;
   return done(c);
 }
 

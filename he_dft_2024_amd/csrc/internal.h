@@ -156,6 +156,8 @@ int poten(hedft_ctx* c, int want_force);
 int poten_tddft(hedft_ctx* c, int want_force, int with_lap = 0);
 int poten_statics(hedft_ctx* c);
 int statics_refresh(hedft_ctx* c);
+int statics_rk_iter(hedft_ctx* c, double dt, double n4);
+int statics_pc_iter(hedft_ctx* c, double dt, double n4);
 int alpha_term_run(hedft_ctx* c);    // term_alfa on its own (stages B and C of alpha_s)
 int potenimp_quantum(hedft_ctx* c);  // upotx and potx4 of a quantum impurity
 // F^-1[F(|psix_k|^2) kern] into out (added to out with acc != 0)

@@ -419,6 +419,19 @@ static int steppcr_impl(hedft_ctx* c, double dt, double n4) {
   return hipGetLastError() == hipSuccess ? HEDFT_OK : HEDFT_EHIP;
 }
 
+// one iteration of the statics driver loop (BCN4HeDFT.f90:1113-1160): the
+// stepper, then the driver's refresh (derden, fden, dencg, denalf) and poten
+int statics_rk_iter(hedft_ctx* c, double dt, double n4) {
+  HEDFT_TRY(steprkr_impl(c, dt, n4, nullptr));
+  HEDFT_TRY(statics_refresh(c));
+  return poten_statics(c);
+}
+int statics_pc_iter(hedft_ctx* c, double dt, double n4) {
+  HEDFT_TRY(steppcr_impl(c, dt, n4));
+  HEDFT_TRY(statics_refresh(c));
+  return poten_statics(c);
+}
+
 // H_l psil (steprkrl.f90:44-57): six derivative passes of psil with one-sided
 // rows, accumulated in work1, then hpsi and sum(den*hpsi)*dxyz/n4 into S_MU
 static int hpsi_log(hedft_ctx* c, double n4) {

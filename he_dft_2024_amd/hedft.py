@@ -157,6 +157,7 @@ def lib():
         L.hedft_tddft_iteration.argtypes = [vp, d, ip, ip, dp, dp, dp]
         L.hedft_tddft_run_pc.argtypes = [vp, d, ip, ip, dp]
         L.hedft_tddft_prepare_pc.argtypes = [vp, d, ip, ip]
+        L.hedft_statics_run.argtypes = [vp, d, ip, ip, d, ip, dp, dp, dp, dp]
         L.hedft_last_elapsed.argtypes = [vp, dp]
         L.hedft_tddft_probe_pc.argtypes = [vp, d, ip, dp, dp]
         L.hedft_batch_run_pc.argtypes = [C.POINTER(vp), ip, dp, ip, ip, dp]
@@ -406,6 +407,14 @@ class Droplet:
         check(lib().hedft_statics_iteration(self._c, dt, iterrk, n4, C.byref(m), C.byref(e)),
               "statics_iteration")
         return m.value, e.value
+
+    def statics_run(self, dt, iter0, niter, n4, use_graph=True):
+        """niter statics iterations from iterrk = iter0 without a host sync in
+        between; returns (mu4, mu4err, errpc, ms) of the last one."""
+        m, e, pc, ms = C.c_double(0), C.c_double(0), C.c_double(0), C.c_double(0)
+        check(lib().hedft_statics_run(self._c, dt, int(iter0), int(niter), n4, int(use_graph), C.byref(m),
+                                      C.byref(e), C.byref(pc), C.byref(ms)), "statics_run")
+        return m.value, e.value, pc.value, ms.value
 
     def evolox(self, dtx, epsx, vdt=(0.6, 0.1), ironx=1):
         e, r = C.c_double(epsx), C.c_double(0)

@@ -342,6 +342,17 @@ int hedft_statics_init(hedft_ctx* c, double n4, double* mu4);
  * fftfw(den), dencg/denalf, poten (BCN4HeDFT.f90:1113-1160). */
 int hedft_statics_iteration(hedft_ctx* c, double deltat, int32_t iterrk, double n4,
                             double* mu4, double* mu4err);
+/* No counterpart in the reference (its driver reads mu4 back every iteration
+ * for the convergence test): niter iterations iterrk = iter0 .. of the same
+ * loop as hedft_statics_iteration, queued without a host synchronisation in
+ * between; with use_graph the PC iterations before the last replay a captured
+ * graph of six (the slot rotation's period).  *mu4, *mu4err, *errpc: those of
+ * the last iteration (mu4err against the iteration before it; errpc 0 when the
+ * last iteration was an RK one); *elapsed_ms: device time of the iterations
+ * (HIP events; any pointer may be NULL).  A caller that tests convergence
+ * every k iterations calls this with niter = k. */
+int hedft_statics_run(hedft_ctx* c, double deltat, int32_t iter0, int32_t niter, double n4, int32_t use_graph,
+                      double* mu4, double* mu4err, double* errpc, double* elapsed_ms);
 /* evolo(deltat,mu4,mu4err,n4), He_statics/evolo.f90:5-96: steepest descent
  * psi <- |psi - g + vdt1 (psi - psi1) + vdt2 (psi1 - psi2)| renormalised, with
  * g = deltat (H - mu) psi smoothed iron times by ironing.f90:4-33 (the module

@@ -145,6 +145,46 @@ def test_statics_trajectory_vs_oracle(gpu):
     d.close()
 
 
+def test_statics_run_equals_iterations(gpu):
+    """hedft_statics_run (iterations queued without host syncs, PC iterations
+    replayed from a captured graph) leaves every field, mu4 and its error equal
+    bit for bit to hedft_statics_iteration called once per iteration, across
+    runs that start in the RK start and at other slot phases."""
+    import he_dft_2024_amd as H
+    n, xmax, n4 = 32, 6.4, 50.0
+    _say("statics run vs iterations")
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=0)
+    dt = 0.01 * 0.4 ** 2 / (4 * 6.05969638298)
+    names = ("PSI", "DEN", "POT4", "PC", "PSIOLD1", "PSIOLD2", "PSIOLD3", "HPSIOLD1", "HPSIOLD2", "DENCG",
+             "DENALF", "TIMEC")
+    out = []
+    for mode in ("iterations", "graph", "eager"):
+        d = H.Droplet(n, n, n, xmax, xmax, xmax, H.default_params(core_otc=1), mode=H.MODE_STATICS)
+        d.set("DEN", den)
+        mu = d.statics_init(n4)
+        snaps = []
+        it = 1
+        for k in (3, 17, 1, 13, 8):
+            if mode == "iterations":
+                for j in range(it, it + k):
+                    prev = mu
+                    mu, err = d.statics_iteration(dt, j, n4, mu)
+                snaps.append([mu, err])
+            else:
+                m, e, pc, ms = d.statics_run(dt, it, k, n4, use_graph=(mode == "graph"))
+                assert ms > 0
+                snaps.append([m, e])
+            snaps[-1] += [d.field(x) for x in names]
+            it += k
+        out.append(snaps)
+        d.close()
+    for v in (1, 2):
+        for i, (a, b) in enumerate(zip(out[v], out[0])):
+            assert a[0] == b[0] and a[1] == b[1], (v, i)
+            for name, x, y in zip(names, a[2:], b[2:]):
+                assert np.array_equal(x, y), (v, i, name)
+
+
 # the absorbing layer of the reference's TDDFT example (Lambdah = 2, t*mean =
 # box half-width - 2 A, t*surf = 2 A; Dynamics_TDDFT/input_files/
 # xe-he1000-hoc-200mps.input:2-4), here on all three axes
