@@ -284,6 +284,47 @@ module hedft_gpu
       integer(c_int32_t), value :: nsteps, use_graph
       real(c_double), intent(out) :: elapsed_ms
     end function
+    integer(c_int) function hedft_statics_iteration_log(ctx, deltat, iterrk, n4, mu4, mu4err, errpc) &
+         bind(C, name="hedft_statics_iteration_log")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      integer(c_int32_t), value :: iterrk
+      real(c_double), intent(inout) :: mu4
+      real(c_double), intent(out) :: mu4err, errpc
+    end function
+    integer(c_int) function hedft_get_tuning(ctx, t) bind(C, name="hedft_get_tuning")
+      import :: c_int, c_ptr, hedft_tuning_t
+      type(c_ptr), value :: ctx
+      type(hedft_tuning_t), intent(out) :: t
+    end function
+    integer(c_int) function hedft_last_elapsed(ctx, ms) bind(C, name="hedft_last_elapsed")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms
+    end function
+    integer(c_int) function hedft_batch_run_pc(ctxs, nctx, deltat, nsteps, chunk, elapsed_ms) &
+         bind(C, name="hedft_batch_run_pc")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int32_t), value :: nctx, nsteps, chunk
+      real(c_double), intent(in) :: deltat(*)
+      real(c_double), intent(out) :: elapsed_ms
+    end function
+    integer(c_int) function hedft_tddft_prepare_pc(ctx, deltat, skip, nsteps) bind(C, name="hedft_tddft_prepare_pc")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+      integer(c_int32_t), value :: skip, nsteps
+    end function
+    integer(c_int) function hedft_statics_run(ctx, deltat, iter0, niter, n4, use_graph, mu4, mu4err, errpc, &
+                                              elapsed_ms) bind(C, name="hedft_statics_run")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, n4
+      integer(c_int32_t), value :: iter0, niter, use_graph
+      real(c_double), intent(out) :: mu4, mu4err, errpc, elapsed_ms
+    end function
     integer(c_int) function hedft_evolo(ctx, deltat, vdt1, vdt2, iron, n4, mu4, mu4err) bind(C, name="hedft_evolo")
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -18,7 +18,8 @@ QEXE = os.path.join(FDIR, "_build", "hedft_fortran_qcheck")
 
 
 def test_fortran_sources_present():
-    for f in ("hedft_gpu.f90", "hedft_fortran_check.f90", "hedft_fortran_qcheck.f90", "hedft_dropin_tddft.f90",
+    for f in ("hedft_gpu.f90", "hedft_fortran_check.f90", "hedft_fortran_qcheck.f90", "hedft_fortran_scheck.f90",
+              "hedft_dropin_quantum_run.f90", "hedft_dropin_tddft.f90",
               "hedft_dropin_quantum.f90", "hedft_dropin_statics.f90", "hedft_dropin_tddft_run.f90",
               "hedft_dropin_statics_run.f90", "Makefile"):
         assert os.path.exists(os.path.join(FDIR, f)), f
@@ -322,4 +323,40 @@ def test_dropin_quantum_reference_names_match_python_sequence(gpu, tmp_path):
         d.select_quantum(k)
         assert np.array_equal(psix_f[k], d.field("PSIX")), k
         assert np.array_equal(upotx_f[k], d.field("UPOTX")), k
+    d.close()
+
+
+SEXE2 = os.path.join(FDIR, "_build", "hedft_fortran_scheck")
+
+
+@pytest.mark.gpu
+def test_fortran_statics_runs_match_python_binding(gpu, tmp_path):
+    """Module hedft_gpu's hedft_statics_run / hedft_get_tuning /
+    hedft_last_elapsed from Fortran: batches of k statics iterations give the
+    same mu4, mu4err, errpc and psi as Droplet.statics_run, bit for bit."""
+    if not os.path.exists(SEXE2):
+        pytest.skip("Fortran check program not built (amdflang absent at build time)")
+    n, xmax, n4, nbatch, k = 32, 6.4, 50.0, 4, 7
+    dt = 0.01 * 0.4 ** 2 / (4 * 6.05969638298)
+    print("[case] fortran statics runs, 32^3 4 x 7 iterations", flush=True)
+    den = I.woods_saxon(n, n, n, xmax, xmax, xmax, n4, seed=0)
+    fin, fout = tmp_path / "sin.bin", tmp_path / "sout.bin"
+    with open(fin, "wb") as f:
+        np.array([n, nbatch, k], dtype=np.int32).tofile(f)
+        np.array([xmax, dt, n4], dtype=np.float64).tofile(f)
+        np.ascontiguousarray(den, dtype=np.float64).tofile(f)
+    r = subprocess.run([SEXE2, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    hist_f, ok, psi_f = raw[:3 * nbatch].reshape(nbatch, 3), raw[3 * nbatch], raw[3 * nbatch + 1:]
+    assert ok == 1.0
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_STATICS)
+    d.set("DEN", den)
+    d.statics_init(n4)
+    hist = []
+    for b in range(nbatch):
+        m, e, pc, _ = d.statics_run(dt, 1 + b * k, k, n4)
+        hist.append((m, e, pc))
+    assert np.array_equal(hist_f, np.array(hist))
+    assert np.array_equal(psi_f, d.field("PSI").ravel())
     d.close()
