@@ -93,3 +93,39 @@ def test_mixed_energy_per_atom_after_1000_steps_vs_fp64_oracle(gpu):
     dm, do = m.field("DEN"), o.field("DEN")
     assert np.linalg.norm(dm - do) / np.linalg.norm(do) < 1e-5
     m.close()
+
+
+def test_c2_256_trajectory_vs_oracle(gpu):
+    """The largest configuration at its size: N=5000 on a 256^3 grid (h = 0.4 A),
+    OTC, the 108 amu LJ impurity at 5 A/ps (500 m/s), 4 RK-Gill + 2 PC
+    iterations; FP64 against the oracle (planes of 256 take the row/column FFT
+    passes, z pencils and stepper chunks of 256), and the mixed path's per-term
+    energy error and density L2 error against the same oracle."""
+    n, xmax, n4 = 256, 51.2, 5000.0
+    dt = 5e-4 / I.PSTOK
+    _say("256^3 N=5000 OTC, 4 RK + 2 PC; FP64 and mixed vs oracle")
+    o = O.Oracle(n, n, n, xmax, xmax, xmax, O.default_params(core_otc=1), O.MODE_TDDFT)
+    d = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+    m = gpu.Droplet(n, n, n, xmax, xmax, xmax, gpu.default_params(core_otc=1, precision=gpu.PREC_MIXED),
+                    mode=gpu.MODE_TDDFT)
+    errs = []
+    for x in (d, m, o):
+        I.tddft_setup(x, n, xmax, n4, seed=2, mass_amu=108.0, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 5.0))
+        x.tddft_init()
+        errs.append(np.array([x.tddft_iteration(dt, it, 4) for it in range(1, 7)]))
+    assert relerr(errs[0][4:, 0], errs[2][4:, 0]) < RTOL_TRAJ
+    for name in ("PSI", "DEN", "POT4", "DELJ4", "DENCG", "UALPHAS", "PC", "PSIOLD1", "HPSIOLD1"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    sg, so = d.impurity_state(), o.impurity_state()
+    for k in ("rimp", "vimp", "aimp", "force"):
+        assert relerr(sg[k], so[k]) < RTOL_TRAJ, k
+    eg, eo, em = d.energy(), o.energy(), m.energy()
+    for k in ETERMS:
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
+        assert abs(em[k] - eo[k]) <= 2e-6 * abs(eo[k]) + 1e-9, (k, em[k], eo[k])
+    print("[c2 mixed per-term rel err]", {k: float(abs(em[k] - eo[k]) / abs(eo[k])) for k in ETERMS if eo[k]},
+          flush=True)
+    dm, do = m.field("DEN"), o.field("DEN")
+    assert np.linalg.norm(dm - do) / np.linalg.norm(do) < 1e-6
+    d.close()
+    m.close()
