@@ -148,3 +148,36 @@ def test_quantum_abi_errors(gpu):
     rc = L.hedft_set_impurities(d._c, 1, 10, 1.0, 10.0, pot.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p))
     assert rc == -1  # one impurity model per context
     d.close()
+
+
+def test_quantum_256_reference_example_vs_oracle(gpu):
+    """The size of the reference's own quantum example
+    (Dynamics_quantum_impurities/input_files/input.0: 256^3, +-32 A, N=1000,
+    Lsolid so the OT core with the solid penalty, icon 24, absorbing layer
+    from t*mean = 30 A, dt = 5e-5 ps, mass 4.002603 u) with a moving packet:
+    3 RK + 2 PC iterations against the oracle."""
+    n, xm, n4 = 256, 32.0, 1000.0
+    dt = 5e-5 / I.PSTOK
+    _say("quantum 256^3 reference-example size, 3 RK + 2 PC vs oracle")
+    kw = dict(core_otc=0, lsolid=1, icon=24, lambdah=2.0, tmean=(30.0, 30.0, 30.0), tsurf=(2.0, 2.0, 2.0))
+    v = Q.lennard_jones(1.0, 5.3)
+    uk = Q.uimp_k_table(n, n, n, xm, xm, xm, v, 3.17502, float(v(3.17502)))
+    d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(**kw), mode=gpu.MODE_TDDFT)
+    o = O.Oracle(n, n, n, xm, xm, xm, O.default_params(**kw), mode=O.MODE_TDDFT)
+    errs = []
+    for x in (d, o):
+        I.quantum_setup(x, n, xm, n4, uk, seed=11, m_imp_u=4.002603, r0=(2.0, -1.0, 8.0), width=2.0,
+                        k=(0.0, 0.3, -0.8))
+        x.tddft_init()
+        errs.append(np.array([x.tddft_iteration(dt, it, 3) for it in range(1, 6)]))
+    # errHe and errpsix sum |predictor - corrector|, which at this example's small step is a
+    # tiny fraction of psi: the fields' last-bit differences show there amplified (1e-8)
+    assert relerr(errs[0][3:, :2], errs[1][3:, :2]) < 1e-6
+    for name in ("PSI", "DEN", "POT4", "PSIX", "POTX4", "UPOTX", "PSIXOLD1", "HPSIXOLD1", "TIMEC"):
+        assert relerr(d.field(name), o.field(name)) < RTOL_TRAJ, name
+    eg, eo = d.energy(), o.energy()
+    for k in ("ekin4", "elj4", "ecor4", "esolid", "ekinx", "eHeX", "eimpu", "etot", "norm"):
+        assert abs(eg[k] - eo[k]) <= 1e-10 * abs(eo[k]) + 1e-12, k
+    (ng, rg), (no, ro) = d.quantum_centroid(), o.quantum_centroid()
+    assert abs(ng - no) < 1e-12 and np.abs(rg - ro).max() < 1e-10
+    d.close()
