@@ -129,3 +129,37 @@ def test_c2_256_trajectory_vs_oracle(gpu):
     assert np.linalg.norm(dm - do) / np.linalg.norm(do) < 1e-6
     d.close()
     m.close()
+
+
+def test_headline_long_run_conserves_norm_and_energy(gpu):
+    """Size-independent invariants of real-time TDDFT on the headline grid over
+    the graph-replayed path the bench times: N=1000 on 128^3, OTC, no impurity
+    and no absorbing layer (tmean beyond the box), 4 RK + 1200 PC iterations.
+    The evolution is unitary: the norm stays at N to 1e-12 (FP64) / 1e-9
+    (mixed).  The energy of the unrelaxed start settles within the first few
+    hundred iterations (2.5e-4 relative) and then stays constant: it moves by
+    less than 2e-5 relative between iterations 600 and 1200.  Mixed and FP64
+    agree on E/N to 1e-6 after the 1204 iterations."""
+    n, xm, n4 = 128, 25.6, 1000.0
+    dt = 5e-4 / I.PSTOK
+    _say("128^3 N=1000, 4 RK + 1200 PC graph replay: norm and energy (FP64, mixed)")
+    out = {}
+    for prec in (gpu.PREC_FP64, gpu.PREC_MIXED):
+        d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1, precision=prec), mode=gpu.MODE_TDDFT)
+        d.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=1))
+        d.tddft_init()
+        for it in range(1, 5):
+            d.tddft_iteration(dt, it, 4)
+        e0 = d.energy()
+        d.run_pc(dt, 600, True)
+        e1 = d.energy()
+        d.run_pc(dt, 600, True)
+        e2 = d.energy()
+        d.close()
+        out[prec] = (e0, e1, e2)
+        tol = 1e-12 if prec == gpu.PREC_FP64 else 1e-9
+        for e in (e1, e2):
+            assert abs(e["norm"] - n4) <= tol * n4, (prec, e["norm"])
+        assert abs(e2["etot"] - e1["etot"]) <= 2e-5 * abs(e1["etot"]), (prec, e1["etot"], e2["etot"])
+    ef, em = out[gpu.PREC_FP64][2], out[gpu.PREC_MIXED][2]
+    assert abs(em["etot"] - ef["etot"]) <= 1e-6 * abs(ef["etot"])
