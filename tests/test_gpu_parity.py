@@ -516,3 +516,28 @@ def test_tddft_three_impurities_with_pair_terms_vs_oracle(gpu):
         x.forceimp()
     assert relerr(d.impurity_state()["force"], o.impurity_state()["force"]) < RTOL_OP
     d.close()
+
+
+def test_run_entry_points_reject_bad_arguments(gpu):
+    """hedft_statics_run and hedft_tddft_prepare_pc: wrong mode, counts out of
+    range and PC iterations before the RK start are errors, not launches."""
+    import ctypes as C
+    import he_dft_2024_amd as H
+    _say("statics_run / prepare_pc argument checks")
+    L = H.lib()
+    d0 = C.c_double(0)
+    s = H.Droplet(32, 32, 32, 6.4, 6.4, 6.4, H.default_params(core_otc=1), mode=H.MODE_STATICS)
+    s.set("DEN", I.woods_saxon(32, 32, 32, 6.4, 6.4, 6.4, 50.0, seed=0))
+    s.statics_init(50.0)
+    args = lambda it0, nit, n4: (s._c, 1e-5, it0, nit, n4, 1, C.byref(d0), C.byref(d0), C.byref(d0),  # noqa: E731
+                                 C.byref(d0))
+    assert L.hedft_statics_run(*args(0, 5, 50.0)) == -1      # iterations count from 1
+    assert L.hedft_statics_run(*args(1, 0, 50.0)) == -1      # nothing to run
+    assert L.hedft_statics_run(*args(1, 5, 0.0)) == -1       # no atoms
+    assert L.hedft_statics_run(*args(5, 5, 50.0)) != 0       # PC before the RK start
+    assert L.hedft_tddft_prepare_pc(s._c, 1e-5, 0, 12) == -1  # a statics context
+    s.close()
+    t = H.Droplet(32, 32, 32, 6.4, 6.4, 6.4, H.default_params(core_otc=1), mode=H.MODE_TDDFT)
+    assert L.hedft_statics_run(t._c, 1e-5, 1, 5, 50.0, 1, None, None, None, None) == -1  # a TDDFT context
+    assert L.hedft_tddft_prepare_pc(t._c, 1e-5, -1, 12) == -1  # negative skip
+    t.close()
