@@ -47,6 +47,9 @@ subroutine hedft_dropin_statics_start(n4, mu4)
   p%h2o2m4 = h2o2m4; p%cp4 = cp4; p%cpp4 = cpp4; p%alphas = alphas; p%l = l; p%den0s = den0s
   p%csolid = C; p%beta_solid = beta; p%den_m = den_m
   p%h4 = h4; p%eps4 = eps4; p%sigma4 = sigma4
+  ! 'OTE' (alpha_s in the energy only, from the start-up term_alfa) has no
+  ! counterpart in the library: stop rather than run it as 'OT '
+  if (core4 == 'OTE') stop 'hedft drop-in: core4 = OTE is not supported (OT, OP or OTC)'
   p%core_otc = merge(1, 0, core4 == 'OTC')
   p%lsolid = merge(1, 0, lsolid)
   p%npd = npd; p%icon = icon
