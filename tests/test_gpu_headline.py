@@ -163,3 +163,31 @@ def test_headline_long_run_conserves_norm_and_energy(gpu):
         assert abs(e2["etot"] - e1["etot"]) <= 2e-5 * abs(e1["etot"]), (prec, e1["etot"], e2["etot"])
     ef, em = out[gpu.PREC_FP64][2], out[gpu.PREC_MIXED][2]
     assert abs(em["etot"] - ef["etot"]) <= 1e-6 * abs(ef["etot"])
+
+
+def test_headline_run_is_deterministic(gpu):
+    """The headline workload twice through graph replay and once through eager
+    launches (two streams, the 2.5-D stepper's LDS halo ring, deterministic
+    block reductions): every field, the impurity state and the energies agree
+    bit for bit, so no launch order or race leaks into the results."""
+    n, xm, n4 = 128, 25.6, 1000.0
+    dt = 5e-4 / I.PSTOK
+    _say("128^3 determinism: graph twice, eager once, 4 RK + 60 PC")
+    names = ("PSI", "DEN", "POT4", "PC", "PSIOLD1", "PSIOLD3", "HPSIOLD2", "UALPHAS", "DELJ4")
+    runs = []
+    for use_graph in (True, True, False):
+        d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+        I.tddft_setup(d, n, xm, n4, seed=1, mass_amu=108.0, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 3.0))
+        d.tddft_init()
+        for it in range(1, 5):
+            d.tddft_iteration(dt, it, 4)
+        d.run_pc(dt, 60, use_graph)
+        st = d.impurity_state()
+        runs.append(([d.field(x) for x in names], [st[k] for k in ("rimp", "vimp", "force")], d.energy()))
+        d.close()
+    for r in runs[1:]:
+        for name, a, b in zip(names, r[0], runs[0][0]):
+            assert np.array_equal(a, b), name
+        for a, b in zip(r[1], runs[0][1]):
+            assert np.array_equal(a, b)
+        assert r[2] == runs[0][2]
