@@ -30,11 +30,15 @@
  * definition, kernels equal their defining integrals / closed forms, the mean
  * field satisfies the uniform-density invariants and dE/drho = pot4, the
  * steppers show their order on exactly solvable problems, the impurity force
- * is -dE/dR.  Parity unpinned (restated only): the statics stale-state
- * sequencing and renormalisation order (steprkr.f90:77-83,
- * BCN4HeDFT.f90:1131-1160), the impurity PC velocity-slot quirk
- * (Dynamics_TDDFT/steppc.f90:92) with the step-start aimp, and the absorbing
- * layer profile (DFT4He3d.f90:592-601).
+ * is -dE/dR (tests/test_oracle_integrators.py): the impurity PC slots and the
+ * step-start aimp (Dynamics_TDDFT/steppc.f90:80-97) by exactness on a constant
+ * acceleration and first-order convergence of a pair oscillator, the absorbing
+ * layer (DFT4He3d.f90:592-601) by a decaying plane wave and its REAL(4)
+ * profile, the statics stale state (He_statics/poten.f90:27-52,
+ * steprkr.f90:77-83, BCN4HeDFT.f90:1113-1160) by equivalences with external
+ * potentials, affinity of poten in den and the driver's fresh field after each
+ * iteration.  The FFT arithmetic of the reference (MKL) is pinned only by the
+ * DFT definition.
  */
 #include <complex.h>
 #include <math.h>

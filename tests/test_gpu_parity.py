@@ -119,6 +119,98 @@ def test_poten_statics_stale_state_vs_oracle(gpu):
     d.close()
 
 
+@pytest.mark.parametrize("log_psi", [False, True])
+def test_statics_steppers_read_the_drivers_fden(gpu, log_psi):
+    """Direct STEPRKR/STEPPCR calls with the Lennard-Jones term alone: their inner
+    poten rebuilds delj4 from the driver's fden (He_statics/poten.f90:27-30,
+    refreshed only by the driver, BCN4HeDFT.f90:1128-1131), so pot4 = delj4 + uext
+    stays one field and the run equals the run with eps4 = 0 and uext + delj4 as
+    external potential, bit for bit, and the oracle's run."""
+    n, xm, n4 = 32, 6.4, 60.0
+    _say("statics stale fden: LJ alone vs functional off + uext, log psi", log_psi)
+    h = 2 * xm / n
+    dt = 0.02 * h * h / (4 * 6.05969638298)
+    x = I.grid_axes(n, xm)
+    uext = 0.4 * (x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2) - 3.0
+    den = I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=4, noise=0.0 if log_psi else 1e-3)
+    names = ("PSI", "DEN", "POT4", "PSIOLD1", "PSIOLD3", "HPSIOLD2") + (("PSIL",) if log_psi else ("TIMEC",))
+
+    def run(mod, eps4_off, ue):
+        p = mod.default_params(cp4=0.0, cpp4=0.0, core_otc=0, lsolid=0, lexternal=1,
+                               **({"eps4": 0.0} if eps4_off else {}))
+        s = (gpu.Droplet if mod is gpu else O.Oracle)(n, n, n, xm, xm, xm, p, mode=mod.MODE_STATICS)
+        s.set("UEXT", ue)
+        s.set("DEN", den)
+        mu = s.statics_init(n4)
+        dlj = np.array(s.field("DELJ4"))
+        rk, pc = (s.steprkrl, s.steppcrl) if log_psi else (s.steprkr, s.steppcr)
+        if log_psi:
+            s.statics_log_start()
+        log = []
+        for it in range(1, 11):
+            r = rk(dt, mu, n4) if it <= 4 else pc(dt, mu, n4)
+            mu = r[0]
+            log.append(r)
+        out = [np.array(s.field(k)) for k in names]
+        s.close()
+        return dlj, log, out
+
+    dlj, log_a, a = run(gpu, False, uext)
+    _, log_b, b = run(gpu, True, uext + dlj)
+    _, log_o, o = run(O, False, uext)
+    assert np.abs(dlj).max() > 1.0
+    assert log_a == log_b
+    for k, u, v, w in zip(names, a, b, o):
+        assert np.array_equal(u, v), k
+        if k in ("PSI", "DEN", "POT4", "PSIL", "TIMEC"):
+            assert relerr(u, w) < RTOL_TRAJ, k
+    assert relerr(np.array([r[:2] for r in log_a]), np.array([r[:2] for r in log_o])) < RTOL_TRAJ
+
+
+def test_statics_alpha_s_field_and_affine_poten(gpu):
+    """term_alfa reads only the driver's denalf and grad(den) (He_statics/term_alfa.f90):
+    direct stepper calls with the OTC core equal the 'OT ' run with uext + ualphas as
+    external potential; and the statics poten, with the driver's arrays held, is affine
+    in den (its only read of the current den is the correlation convolution,
+    He_statics/poten.f90:42-52), which a refresh from the current den breaks."""
+    n, xm, n4 = 32, 6.4, 60.0
+    _say("statics stale denalf/grad and affine poten")
+    h = 2 * xm / n
+    dt = 0.02 * h * h / (4 * 6.05969638298)
+    x = I.grid_axes(n, xm)
+    uext = 0.4 * (x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2) - 3.0
+    den = I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=4)
+    res = []
+    for otc in (1, 0):
+        d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=otc, lexternal=1), mode=gpu.MODE_STATICS)
+        d.set("UEXT", uext if otc else uext + ua)
+        d.set("DEN", den)
+        mu = d.statics_init(n4)
+        if otc:
+            ua = np.array(d.field("UALPHAS"))
+        for it in range(1, 11):
+            mu = (d.steprkr(dt, mu, n4) if it <= 4 else d.steppcr(dt, mu, n4))[0]
+        res.append((mu, [np.array(d.field(k)) for k in ("PSI", "POT4", "PSIOLD2", "HPSIOLD1")]))
+        d.close()
+    assert np.abs(ua).max() > 0.1
+    assert abs(res[0][0] - res[1][0]) < 1e-13 * abs(res[0][0])
+    for u, v in zip(res[0][1], res[1][1]):
+        assert np.abs(u - v).max() < 1e-12 * np.abs(v).max()
+    d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1, lexternal=1), mode=gpu.MODE_STATICS)
+    d.set("UEXT", uext)
+    d.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=6))
+    d.statics_init(n4)
+    d1 = I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=7)
+    d2 = 0.6 * I.woods_saxon(n, n, n, xm, xm, xm, 0.7 * n4, seed=8)
+    pots = []
+    for dd in (d1, d2, 0.5 * (d1 + d2)):
+        d.set("DEN", dd)
+        d.poten()
+        pots.append(np.array(d.field("POT4")))
+    d.close()
+    assert np.abs(pots[2] - 0.5 * (pots[0] + pots[1])).max() < 1e-13 * np.abs(pots[2]).max()
+
+
 def test_statics_trajectory_vs_oracle(gpu):
     n, xmax, n4 = 32, 6.4, 50.0
     _say("statics 4 RK + 8 PC")

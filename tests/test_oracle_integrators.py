@@ -20,8 +20,15 @@ and the absorbing layer, by what the mathematics fixes (CPU only).
   13-point stencil's symbol; RK + PC reach it at fourth order.
 * timec's real part is the REAL(4) rounding of Lambda (DFT4He3d.f90:595 builds
   it with cmplx(...,1.d0) and no KIND).
+* The statics stale state (He_statics/poten.f90, term_alfa.f90, steprkr.f90,
+  steppcr.f90, BCN4HeDFT.f90:1113-1160): inside the steppers the mean field
+  reads the driver's fden, dencg, denalf and grad(den) and only the correlation
+  convolution reads the current den, shown by equivalences with external
+  potentials and by affinity in den; after a driver iteration the field is the
+  fresh one.
 """
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 from he_dft_2024_amd import inputs as I
@@ -262,3 +269,149 @@ def test_evolox_converges_to_discrete_ground_state():
     assert np.abs(o.field("PSIX") - v3).max() < 1e-9 * np.abs(v3).max()
     # the discrete eigenvalue approaches the trap's 3 h2o2mx / sigma^2
     assert abs(eps - 3 * hx / sigma ** 2) < 1e-6 * eps
+
+
+# ---------------------------------------------------------- statics stale state
+# Inside STEPRKR/STEPPCR the statics poten rebuilds delj4 from the fden the
+# driver last computed (He_statics/poten.f90:27-30); only the driver refreshes
+# fden, after a whole step (BCN4HeDFT.f90:1128-1131).  With the Lennard-Jones
+# term alone (cp4 = cpp4 = 0, core 'OT ', no solid term) pot4 = delj4 + uext is
+# then one fixed field through any run of direct stepper calls, so that run is
+# bit for bit the run with the functional off (eps4 = 0, fvlj4 = 0) and
+# uext + delj4 as its external potential.  A poten that transformed the current
+# den inside the stepper would follow the changing density and miss; the run
+# with the driver's refresh between the steps shows by how much.
+def _stale_statics(eps4_off, uext, den, n, xm, n4, dt, refresh=False, log_psi=False):
+    p = O.default_params(cp4=0.0, cpp4=0.0, core_otc=0, lsolid=0, lexternal=1,
+                         **({"eps4": 0.0} if eps4_off else {}))
+    o = O.Oracle(n, n, n, xm, xm, xm, p, O.MODE_STATICS)
+    o.set("UEXT", uext)
+    o.set("DEN", den)
+    mu = o.statics_init(n4)
+    start = (o.field("DELJ4").copy(), o.field("POT4").copy(), mu)   # field() is a view
+    rk, pc = (o.steprkrl, o.steppcrl) if log_psi else (o.steprkr, o.steppcr)
+    if log_psi:
+        o.statics_log_start()
+    log = []
+    for it in range(1, 11):
+        r = rk(dt, mu, n4) if it <= 4 else pc(dt, mu, n4)
+        mu = r[0]
+        log.append(r)
+        if refresh:
+            o.statics_refresh_poten()
+    out = {k: o.field(k).copy() for k in ("PSI", "DEN", "POT4", "PSIOLD1", "PSIOLD2", "PSIOLD3", "HPSIOLD1",
+                                            "HPSIOLD2", "TIMEC")}
+    o.close()
+    return start, log, out
+
+
+@pytest.mark.parametrize("log_psi", [False, True])
+def test_statics_steppers_read_the_drivers_fden(log_psi):
+    n, xm, n4 = 32, 6.4, 60.0
+    h = 2 * xm / n
+    dt = 0.02 * h * h / (4 * H2O2M4)
+    x = I.grid_axes(n, xm)
+    R2 = x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2
+    uext = 0.4 * R2 - 3.0
+    # the log-psi scheme amplifies grid-scale noise through |grad psil|^2: a smooth start
+    den = I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=4, noise=0.0 if log_psi else 1e-3)
+    (dlj, pot_a, mu_a), log_a, a = _stale_statics(False, uext, den, n, xm, n4, dt, log_psi=log_psi)
+    assert np.abs(dlj).max() > 1.0                  # the LJ field is on
+    (dlj_b, pot_b, mu_b), log_b, b = _stale_statics(True, uext + dlj, den, n, xm, n4, dt, log_psi=log_psi)
+    assert not dlj_b.any()
+    assert np.array_equal(pot_a, pot_b) and mu_a == mu_b
+    assert log_a == log_b
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    moved = np.abs(a["DEN"] - den / den.sum() * a["DEN"].sum()).max() / den.max()
+    assert moved > 1e-2, moved
+    _, log_r, r = _stale_statics(False, uext, den, n, xm, n4, dt, refresh=True, log_psi=log_psi)
+    diff = np.abs(r["PSI"] - a["PSI"]).max() / np.abs(a["PSI"]).max()
+    assert diff > 1e-5, diff
+
+
+# term_alfa reads only the driver's denalf and grad(den) (He_statics/term_alfa.f90),
+# so with the OTC core ualphas is one fixed field through direct stepper calls
+# too: the run equals the 'OT ' run with uext + ualphas as external potential
+# (to rounding: pot4 adds the same terms in another order).  And the only term
+# of the statics poten that reads the current den is the correlation convolution
+# W*(den dencg (cp4 + cpp4 dencg)) with the driver's dencg: poten is affine in
+# den while the driver's arrays stay, which a dencg, denalf or grad(den) rebuilt
+# from the current den (the driver's refresh) breaks at O(1).
+def _statics_case(n=32, xm=6.4, n4=60.0):
+    x = I.grid_axes(n, xm)
+    uext = 0.4 * (x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2) - 3.0
+    return n, xm, n4, uext
+
+
+def test_statics_steppers_read_the_drivers_alpha_s_field():
+    n, xm, n4, uext = _statics_case()
+    h = 2 * xm / n
+    dt = 0.02 * h * h / (4 * H2O2M4)
+    den = I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=4)
+    res = []
+    for otc in (1, 0):
+        o = O.Oracle(n, n, n, xm, xm, xm, O.default_params(core_otc=otc, lexternal=1), O.MODE_STATICS)
+        o.set("UEXT", uext if otc else uext + ua)
+        o.set("DEN", den)
+        mu = o.statics_init(n4)
+        if otc:
+            ua = o.field("UALPHAS").copy()
+        for it in range(1, 11):
+            mu = (o.steprkr(dt, mu, n4) if it <= 4 else o.steppcr(dt, mu, n4))[0]
+        res.append((mu, [o.field(k).copy() for k in ("PSI", "POT4", "PSIOLD2", "HPSIOLD1")]))
+        o.close()
+    assert np.abs(ua).max() > 0.1
+    assert abs(res[0][0] - res[1][0]) < 1e-13 * abs(res[0][0])
+    for u, v in zip(res[0][1], res[1][1]):
+        assert np.abs(u - v).max() < 1e-12 * np.abs(v).max()
+
+
+def test_statics_poten_is_affine_in_den():
+    n, xm, n4, uext = _statics_case()
+    o = O.Oracle(n, n, n, xm, xm, xm, O.default_params(core_otc=1, lexternal=1), O.MODE_STATICS)
+    o.set("UEXT", uext)
+    o.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=6))
+    o.statics_init(n4)
+    d1 = I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=7)
+    d2 = 0.6 * I.woods_saxon(n, n, n, xm, xm, xm, 0.7 * n4, seed=8)
+    out = {}
+    for fresh in (False, True):
+        pots = []
+        for d in (d1, d2, 0.5 * (d1 + d2)):
+            o.set("DEN", d)
+            if fresh:
+                o.statics_refresh_poten()
+            else:
+                o.poten()
+            pots.append(o.field("POT4").copy())
+        out[fresh] = np.abs(pots[2] - 0.5 * (pots[0] + pots[1])).max() / np.abs(pots[2]).max()
+    o.close()
+    assert out[False] < 1e-13, out
+    assert out[True] > 1e-3, out
+
+
+# The driver refreshes grad(den), fden, dencg and denalf after a whole step and
+# then calls poten (BCN4HeDFT.f90:1125-1160, ppot = 1): after each driver
+# iteration the mean field is the fresh one of the current density, the one a
+# new context builds from it.  Skipping the refresh, or calling poten before it,
+# leaves the previous step's field.
+def test_statics_driver_iteration_ends_with_the_fresh_mean_field():
+    n, xm, n4, uext = _statics_case()
+    h = 2 * xm / n
+    dt = 0.02 * h * h / (4 * H2O2M4)
+    a = O.Oracle(n, n, n, xm, xm, xm, O.default_params(core_otc=1, lexternal=1), O.MODE_STATICS)
+    a.set("UEXT", uext)
+    a.set("DEN", I.woods_saxon(n, n, n, xm, xm, xm, n4, seed=4))
+    mu = a.statics_init(n4)
+    for it in range(1, 7):
+        mu = a.statics_iteration(dt, it, n4, mu)[0]
+    b = O.Oracle(n, n, n, xm, xm, xm, O.default_params(core_otc=1, lexternal=1), O.MODE_STATICS)
+    b.set("UEXT", uext)
+    b.set("DEN", a.field("DEN"))
+    b.statics_init(n4)
+    for k in ("POT4", "DENCG", "DENALF", "DXDEN", "DZDEN", "UALPHAS", "DELJ4"):
+        u, v = a.field(k), b.field(k)
+        assert np.abs(u - v).max() < 1e-12 * np.abs(v).max(), k
+    a.close()
+    b.close()
