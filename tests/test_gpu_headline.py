@@ -165,18 +165,21 @@ def test_headline_long_run_conserves_norm_and_energy(gpu):
     assert abs(em["etot"] - ef["etot"]) <= 1e-6 * abs(ef["etot"])
 
 
-def test_headline_run_is_deterministic(gpu):
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_headline_run_is_deterministic(gpu, prec):
     """The headline workload twice through graph replay and once through eager
     launches (two streams, the 2.5-D stepper's LDS halo ring, deterministic
     block reductions): every field, the impurity state and the energies agree
     bit for bit, so no launch order or race leaks into the results."""
     n, xm, n4 = 128, 25.6, 1000.0
     dt = 5e-4 / I.PSTOK
-    _say("128^3 determinism: graph twice, eager once, 4 RK + 60 PC")
+    precision = gpu.PREC_FP64 if prec == "fp64" else gpu.PREC_MIXED
+    _say("128^3 determinism: graph twice, eager once, 4 RK + 60 PC,", prec)
     names = ("PSI", "DEN", "POT4", "PC", "PSIOLD1", "PSIOLD3", "HPSIOLD2", "UALPHAS", "DELJ4")
     runs = []
     for use_graph in (True, True, False):
-        d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1), mode=gpu.MODE_TDDFT)
+        d = gpu.Droplet(n, n, n, xm, xm, xm, gpu.default_params(core_otc=1, precision=precision),
+                        mode=gpu.MODE_TDDFT)
         I.tddft_setup(d, n, xm, n4, seed=1, mass_amu=108.0, r0=(0.0, 0.0, -20.0), v_aps=(0.0, 0.0, 3.0))
         d.tddft_init()
         for it in range(1, 5):
